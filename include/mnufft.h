@@ -1,0 +1,183 @@
+/*
+ * mnufft: device-resident NUFFT plans on AMD Instinct MI355X (gfx950).
+ *
+ * This is the device-pointer face of the engine: the counterpart of the
+ * reference's GPU plan objects, whose methods record work into a caller's
+ * command encoder on caller-owned GPU buffers.  Here every call enqueues
+ * work on a caller's HIP stream (NULL: the default stream) on caller-owned
+ * device pointers, and returns once the work is enqueued.
+ *
+ * Conventions (as the reference's plan surface):
+ *  - coordinates are point-major, points[j * dim + axis], radians in
+ *    [-3 pi, 3 pi] for types 1 and 2 (periodic modulo 2 pi);
+ *  - complex values are interleaved (re, im): float2 for F32 plans, double2
+ *    otherwise;
+ *  - modes are axis-0 fastest, centered (index k holds frequency
+ *    k - floor(N/2)) or FFT order;
+ *  - batches are transform-major: vector 0 whole, then vector 1, ...
+ *
+ * Every function returns 0 or one of the MNUFFT_ERROR_* codes and then
+ * mnufft_last_error() describes the failure (thread-local, valid until the
+ * next call on the thread).
+ */
+#ifndef MNUFFT_H
+#define MNUFFT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MNUFFT_SUCCESS 0
+#define MNUFFT_ERROR_INVALID_ARGUMENT 1
+#define MNUFFT_ERROR_PLAN 2
+#define MNUFFT_ERROR_GPU_UNAVAILABLE 3
+#define MNUFFT_ERROR_GPU 4
+#define MNUFFT_ERROR_INTERNAL 5
+
+/* Arithmetic of a plan (same values as WGPU_NUFFT_PRECISION_*); DF64 runs
+ * native f64 on MI355X. */
+#define MNUFFT_PRECISION_F64 1
+#define MNUFFT_PRECISION_DF64 2
+#define MNUFFT_PRECISION_F32 3
+
+/* Stages reported by mnufft_stage_times, in this order. */
+#define MNUFFT_STAGE_SORT 0
+#define MNUFFT_STAGE_PREPARE 1
+#define MNUFFT_STAGE_SPREAD 2
+#define MNUFFT_STAGE_FFT 3
+#define MNUFFT_STAGE_DECONV 4
+#define MNUFFT_STAGE_INTERP 5
+#define MNUFFT_STAGE_TYPE3 6
+#define MNUFFT_STAGE_COUNT 7
+
+typedef struct mnufft_plan_s *mnufft_plan;
+
+/*
+ * A type-1 or type-2 plan for `batch` vectors over one point set.
+ * Replaces NufftConfig (src/config.rs:64-230) with NufftPlan::type1_gpu /
+ * NufftPlan::type2_gpu (src/plan.rs:457-463, 606-651): n_modes per axis
+ * (axis 0 first), isign >= 0 positive, eps the tolerance, precision one of
+ * MNUFFT_PRECISION_*, mode_order 0 centered / 1 FFT, sigma the upsampling
+ * factor (0: 2).
+ */
+int mnufft_makeplan(int32_t type, int32_t dim, const int64_t *n_modes, int32_t isign,
+                    int64_t batch, double eps, int32_t precision, int32_t mode_order, double sigma,
+                    mnufft_plan *plan);
+
+/*
+ * A type-3 plan: NufftType3Config + NufftType3Plan::new (src/type3.rs:90-344).
+ * source_bounds / target_bounds hold (lower, upper) per axis; every point
+ * and target must lie inside them (a caller guarantee, src/type3.rs:346-353).
+ */
+int mnufft_makeplan_type3(int32_t dim, int32_t isign, int64_t batch, double eps, int32_t precision,
+                          double sigma, const double *source_bounds, const double *target_bounds,
+                          mnufft_plan *plan);
+
+/* Releases a plan and its device scratch; NULL is ignored. */
+void mnufft_destroy(mnufft_plan plan);
+
+/*
+ * Bins M points for later executions: NufftPlan::set_points_gpu
+ * (src/plan.rs:874-925).  The plan keeps the device pointer; the points must
+ * stay unchanged while executions use them.
+ */
+int mnufft_set_points(mnufft_plan plan, int64_t M, const void *d_points, void *stream);
+
+/*
+ * Runs active_batch (1..batch) transforms on the points last set:
+ * NufftPlan::execute_type1_gpu_batch / execute_type2_gpu_batch
+ * (src/plan.rs:931-977).  Type 1 reads strengths (M x active_batch) and
+ * writes modes; type 2 reads modes and writes values at the points.  Zero
+ * points give an all-zero type-1 output.
+ */
+int mnufft_execute(mnufft_plan plan, int64_t active_batch, const void *d_in, void *d_out,
+                   void *stream);
+
+/* set_points + execute: NufftPlan::encode_type{1,2}_gpu_batch
+ * (src/plan.rs:808-872, 1090-1177). */
+int mnufft_encode(mnufft_plan plan, int64_t active_batch, int64_t M, const void *d_points,
+                  const void *d_in, void *d_out, void *stream);
+
+/*
+ * Type 3: NufftType3Plan::encode_gpu_batch (src/type3.rs:473-539).  Reads M
+ * sources and their strengths (M x active_batch), N targets, writes the N x
+ * active_batch outputs.
+ */
+int mnufft_execute_type3(mnufft_plan plan, int64_t active_batch, int64_t M, const void *d_sources,
+                         const void *d_strengths, int64_t N, const void *d_targets, void *d_out,
+                         void *stream);
+
+/*
+ * The reference's stable bin permutation of M points for a type-1 plan:
+ * the count / scan / scatter / large-bin sort sequence of
+ * NdPointBins::encode with NdBinOrder::Stable (src/gpu_nd_bins.rs:336-471,
+ * src/gpu_bin_sort.rs:1-56).  d_sorted receives M point indices (ascending
+ * index inside each bin), d_offsets (may be NULL) the nbins + 1 bin offsets;
+ * *nbins receives the bin count of the geometry chosen for M.
+ */
+int mnufft_bin_sort(mnufft_plan plan, int64_t M, const void *d_points, uint32_t *d_sorted,
+                    uint32_t *d_offsets, int64_t *nbins, void *stream);
+
+/* Bin count of the geometry a type-1 plan chooses for M points (sizes the
+ * offsets buffer of mnufft_bin_sort). */
+int mnufft_bin_count(mnufft_plan plan, int64_t M, int64_t *nbins);
+
+/*
+ * Plan facts, for tests and benchmarks: info[0] kernel width, [1..3] fine
+ * lengths, [4..6] bin sides of the geometry last chosen (type 1) or used
+ * (type 2), [7] its bin count, [8] geometry index, [9..11] spread block edge,
+ * [12] spread run, [13..15] type-3 outer grid lengths.  Writes min(n, 16).
+ */
+int mnufft_plan_info(mnufft_plan plan, int64_t *info, int32_t n);
+
+/* Kernel beta of the plan. */
+double mnufft_plan_beta(mnufft_plan plan);
+
+/*
+ * Milliseconds per stage of the plan's last set_points / execute, measured
+ * with HIP events on the stream of the call (-1 for stages that did not
+ * run); waits for those events.  The stage indices are MNUFFT_STAGE_*.
+ */
+int mnufft_stage_times(mnufft_plan plan, double *ms, int32_t n);
+
+/*
+ * FP64 direct sums on the GPU: reference_type1_f64 / reference_type2_f64
+ * (src/direct.rs:9-118) and reference_type3_f64 (src/type3.rs:609-673).
+ * kind 1: out[o] = sum_j in[j] exp(i sign k_o.x_j) over the linear mode
+ * indices d_sel (all modes if NULL); kind 2: out[o] = sum_k in[k]
+ * exp(i sign k.x_o) at the points d_sel (all if NULL); kind 3: out[o] =
+ * sum_j in[j] exp(i sign s_o.x_j) at the targets d_sel (all if NULL).
+ */
+int mnufft_direct_f64(int32_t kind, int32_t dim, const int64_t *n_modes, int32_t isign,
+                      int32_t mode_order, int64_t M, const double *d_x, const double *d_in,
+                      int64_t nout, const int64_t *d_sel, const double *d_s, double *d_out,
+                      void *stream);
+
+/*
+ * Host-side planning facts (no GPU needed):
+ *  - mnufft_kernel_params: ES width and beta of a tolerance
+ *    (EsKernel::for_tolerance, src/kernel.rs:21-57);
+ *  - mnufft_fine_grid_size: fine-grid length of an axis
+ *    (select_fine_grid_size, src/kernel.rs:278-333), -1 if none;
+ *  - mnufft_layout_query: the reference spread layout and the bin geometry
+ *    it chooses for point_count points (NdSpreadLayout::for_grid and
+ *    choose_geometry, src/gpu_nd_spread.rs:194-459).  out (64 entries):
+ *    [0] run, [1] workgroup size, [2] row batch, [3] geometry count,
+ *    [4] chosen geometry, [5..7] block, [8..10] blocks, [16 + 3g + a] the
+ *    bin side of geometry g along axis a.
+ */
+int mnufft_kernel_params(double eps, double sigma, int32_t *width, double *beta);
+int64_t mnufft_fine_grid_size(int64_t n_modes, double sigma, int32_t width);
+int mnufft_layout_query(int32_t dim, const int64_t *fine, int32_t width, int32_t precision,
+                        int64_t point_count, int64_t *out);
+
+/* The message of the last error on this thread. */
+const char *mnufft_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MNUFFT_H */

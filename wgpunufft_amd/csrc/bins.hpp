@@ -1,0 +1,53 @@
+// Point binning (count, scan, scatter, in-bin order, prepare).
+#pragma once
+
+#include "nufft_internal.hpp"
+
+namespace mnufft {
+
+// Device buffers of one plan's binning, grown on demand and reused by every
+// execution in stream order.
+class BinSorter {
+public:
+    ~BinSorter();
+    void release();
+
+    // Bins the M points (point-major coordinates on the device) and orders
+    // each bin; afterwards sorted() and offsets() hold the permutation and
+    // the nbins + 1 bin offsets (the last one is M).
+    template <int D, typename T>
+    bool sort(int64_t M, const T *coords, const GridParams &g, BinOrder order,
+              hipStream_t stream);
+
+    // Prepared records in final slot order: rec_start[slot] = (wrapped start
+    // per axis, original index), rec_off[4 * slot + axis] = start - position.
+    // out_sorted (optional) receives the final permutation.
+    template <int D, typename T>
+    bool prepare(int64_t M, const T *coords, const GridParams &g, BinOrder order,
+                 int4 *rec_start, T *rec_off, uint32_t *out_sorted, hipStream_t stream);
+
+    const uint32_t *sorted() const { return sorted_; }
+    const uint32_t *offsets() const { return offsets_; }
+    int64_t nbins() const { return nbins_; }
+
+private:
+    bool ensure(int64_t nbins, int64_t M, bool need_sort_scratch);
+
+    uint32_t *counts_ = nullptr;
+    uint32_t *offsets_ = nullptr;
+    uint32_t *keys_ = nullptr;
+    uint32_t *ranks_ = nullptr;
+    uint32_t *sorted_ = nullptr;
+    uint32_t *large_list_ = nullptr;
+    uint32_t *tile_sums_ = nullptr;
+    void *scratch_a_ = nullptr;
+    void *scratch_b_ = nullptr;
+    int64_t bins_cap_ = 0, points_cap_ = 0, sort_scratch_cap_ = 0;
+    int64_t nbins_ = 0, M_ = 0;
+};
+
+template <typename T>
+bool gather_strengths(int64_t M, int64_t vectors, const int4 *rec_start, const T *c, T *c_sorted,
+                      hipStream_t stream);
+
+}  // namespace mnufft

@@ -1,0 +1,1070 @@
+// C ABI of the engine: the device-pointer API (include/mnufft.h) and the
+// host-array API compatible with the reference's C interface
+// (include/wgpu_nufft.h; reference capi/src/lib.rs:326-934,
+// capi/src/plan.rs:156-893, capi/src/error.rs:27-91).
+//
+// No C++ exception or panic crosses the boundary: every entry point catches
+// and maps failures to the documented codes, with a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mnufft.h"
+#include "../../include/wgpu_nufft.h"
+#include "modes.hpp"
+#include "plan.hpp"
+
+using mnufft::Plan;
+using mnufft::Precision;
+using mnufft::set_error;
+
+namespace {
+
+constexpr double kMaxCoordinate = 3.0 * M_PI;
+
+
+bool gpu_available(std::string *why)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        if (why) *why = "no HIP device is available";
+        return false;
+    }
+    return true;
+}
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return MNUFFT_ERROR_INTERNAL;
+    } catch (const std::exception &e) {
+        set_error(std::string("internal error: ") + e.what());
+        return MNUFFT_ERROR_INTERNAL;
+    } catch (...) {
+        set_error("internal error");
+        return MNUFFT_ERROR_INTERNAL;
+    }
+}
+
+int make_plan(int type, int dim, const int64_t *n_modes, int isign, int64_t batch, double eps,
+              int precision, int mode_order, double sigma, std::unique_ptr<Plan> *out)
+{
+    if (type < 1 || type > 2) {
+        set_error("plan type must be 1 or 2 here (type 3 has its own constructor)");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (dim < 1 || dim > 3) {
+        set_error("GPU plans take 1 to 3 dimensions, got " + std::to_string(dim));
+        return MNUFFT_ERROR_PLAN;
+    }
+    if (!n_modes) {
+        set_error("n_modes is null");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (batch < 1) {
+        set_error("batch must be at least 1");
+        return MNUFFT_ERROR_PLAN;
+    }
+    if (mode_order != 0 && mode_order != 1) {
+        set_error("mode order must be 0 (centered) or 1 (FFT)");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    std::string why;
+    if (!gpu_available(&why)) {
+        set_error(why);
+        return MNUFFT_ERROR_GPU_UNAVAILABLE;
+    }
+    auto p = std::make_unique<Plan>();
+    p->type = type;
+    p->dim = dim;
+    for (int a = 0; a < dim; ++a) {
+        if (n_modes[a] < 1) {
+            set_error("mode count along axis " + std::to_string(a) + " must be positive");
+            return MNUFFT_ERROR_PLAN;
+        }
+        p->n_modes[a] = n_modes[a];
+    }
+    p->isign = isign >= 0 ? 1 : -1;
+    p->batch = batch;
+    p->eps = eps;
+    p->sigma = sigma == 0.0 ? 2.0 : sigma;
+    p->precision = precision == MNUFFT_PRECISION_F32 ? Precision::F32
+                   : precision == MNUFFT_PRECISION_DF64 ? Precision::Df64
+                                                        : Precision::F64;
+    p->mode_order = mode_order == 1 ? mnufft::ModeOrder::Fft : mnufft::ModeOrder::Centered;
+    if (!p->init_common()) return MNUFFT_ERROR_PLAN;
+    *out = std::move(p);
+    return MNUFFT_SUCCESS;
+}
+
+}  // namespace
+
+// ======================================================================
+// Device-pointer API.
+// ======================================================================
+struct mnufft_plan_s {
+    std::unique_ptr<Plan> plan;
+};
+
+extern "C" {
+
+const char *mnufft_last_error(void) { return mnufft::last_error(); }
+
+int mnufft_makeplan(int32_t type, int32_t dim, const int64_t *n_modes, int32_t isign,
+                    int64_t batch, double eps, int32_t precision, int32_t mode_order, double sigma,
+                    mnufft_plan *plan)
+{
+    return guarded([&] {
+        if (!plan) {
+            set_error("plan output pointer is null");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        *plan = nullptr;
+        std::unique_ptr<Plan> p;
+        int rc = make_plan(type, dim, n_modes, isign, batch, eps, precision, mode_order, sigma, &p);
+        if (rc) return rc;
+        *plan = new mnufft_plan_s{std::move(p)};
+        return MNUFFT_SUCCESS;
+    });
+}
+
+int mnufft_makeplan_type3(int32_t dim, int32_t isign, int64_t batch, double eps, int32_t precision,
+                          double sigma, const double *source_bounds, const double *target_bounds,
+                          mnufft_plan *plan)
+{
+    return guarded([&] {
+        if (!plan || !source_bounds || !target_bounds) {
+            set_error("null argument");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        *plan = nullptr;
+        if (dim < 1 || dim > 3) {
+            set_error("GPU plans take 1 to 3 dimensions");
+            return MNUFFT_ERROR_PLAN;
+        }
+        if (batch < 1) {
+            set_error("batch must be at least 1");
+            return MNUFFT_ERROR_PLAN;
+        }
+        std::string why;
+        if (!gpu_available(&why)) {
+            set_error(why);
+            return MNUFFT_ERROR_GPU_UNAVAILABLE;
+        }
+        auto p = std::make_unique<Plan>();
+        p->type = 3;
+        p->dim = dim;
+        p->isign = isign >= 0 ? 1 : -1;
+        p->batch = batch;
+        p->eps = eps;
+        p->sigma = sigma == 0.0 ? 2.0 : sigma;
+        p->precision = precision == MNUFFT_PRECISION_F32 ? Precision::F32 : Precision::F64;
+        if (!p->init_type3(source_bounds, target_bounds)) return MNUFFT_ERROR_PLAN;
+        *plan = new mnufft_plan_s{std::move(p)};
+        return MNUFFT_SUCCESS;
+    });
+}
+
+void mnufft_destroy(mnufft_plan plan)
+{
+    if (!plan) return;
+    try {
+        (void)hipDeviceSynchronize();
+        delete plan;
+    } catch (...) {
+    }
+}
+
+int mnufft_set_points(mnufft_plan plan, int64_t M, const void *d_points, void *stream)
+{
+    return guarded([&] {
+        if (!plan || (M > 0 && !d_points) || M < 0) {
+            set_error("invalid plan, point count or point pointer");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (plan->plan->type == 3) {
+            set_error("type-3 plans take their points in mnufft_execute_type3");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (M > (int64_t)UINT32_MAX - 1) {
+            set_error("point count exceeds the 32-bit point index space");
+            return MNUFFT_ERROR_PLAN;
+        }
+        if (!plan->plan->set_points(M, d_points, (hipStream_t)stream)) return MNUFFT_ERROR_GPU;
+        return MNUFFT_SUCCESS;
+    });
+}
+
+int mnufft_execute(mnufft_plan plan, int64_t active_batch, const void *d_in, void *d_out,
+                   void *stream)
+{
+    return guarded([&] {
+        if (!plan || !d_out || (!d_in && plan->plan->point_count > 0)) {
+            set_error("invalid plan or null buffer");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        Plan &p = *plan->plan;
+        if (p.type == 3) {
+            set_error("type-3 plans execute with mnufft_execute_type3");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (!p.points_set) {
+            set_error("points must be set before executing");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (active_batch < 1 || active_batch > p.batch) {
+            set_error("active batch " + std::to_string(active_batch) + " outside 1.." +
+                      std::to_string(p.batch));
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (!p.execute(active_batch, d_in, d_out, (hipStream_t)stream)) return MNUFFT_ERROR_GPU;
+        return MNUFFT_SUCCESS;
+    });
+}
+
+int mnufft_encode(mnufft_plan plan, int64_t active_batch, int64_t M, const void *d_points,
+                  const void *d_in, void *d_out, void *stream)
+{
+    int rc = mnufft_set_points(plan, M, d_points, stream);
+    if (rc) return rc;
+    return mnufft_execute(plan, active_batch, d_in, d_out, stream);
+}
+
+int mnufft_execute_type3(mnufft_plan plan, int64_t active_batch, int64_t M, const void *d_sources,
+                         const void *d_strengths, int64_t N, const void *d_targets, void *d_out,
+                         void *stream)
+{
+    return guarded([&] {
+        if (!plan || plan->plan->type != 3) {
+            set_error("not a type-3 plan");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        Plan &p = *plan->plan;
+        if (active_batch < 1 || active_batch > p.batch) {
+            set_error("active batch outside 1..batch");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (M < 0 || N < 0 || (M > 0 && (!d_sources || !d_strengths)) ||
+            (N > 0 && (!d_targets || !d_out))) {
+            set_error("invalid counts or null buffers");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (!p.execute_type3(active_batch, M, d_sources, d_strengths, N, d_targets, d_out,
+                             (hipStream_t)stream))
+            return MNUFFT_ERROR_GPU;
+        return MNUFFT_SUCCESS;
+    });
+}
+
+int mnufft_bin_count(mnufft_plan plan, int64_t M, int64_t *nbins)
+{
+    return guarded([&] {
+        if (!plan || !nbins || plan->plan->type != 1) {
+            set_error("bin counts are defined for type-1 plans");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        *nbins = plan->plan->type1_geometry(M, nullptr).nbins;
+        return MNUFFT_SUCCESS;
+    });
+}
+
+int mnufft_bin_sort(mnufft_plan plan, int64_t M, const void *d_points, uint32_t *d_sorted,
+                    uint32_t *d_offsets, int64_t *nbins, void *stream)
+{
+    return guarded([&] {
+        if (!plan || plan->plan->type != 1 || (M > 0 && (!d_points || !d_sorted))) {
+            set_error("bin sorting needs a type-1 plan and device buffers");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (nbins) *nbins = plan->plan->type1_geometry(M, nullptr).nbins;
+        if (!plan->plan->bin_sort(M, d_points, d_sorted, d_offsets, (hipStream_t)stream))
+            return MNUFFT_ERROR_GPU;
+        return MNUFFT_SUCCESS;
+    });
+}
+
+int mnufft_plan_info(mnufft_plan plan, int64_t *info, int32_t n)
+{
+    if (!plan || !info) {
+        set_error("null argument");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    const Plan &p = *plan->plan;
+    const Plan &q = p.type == 3 && p.outer ? *p.outer : p;
+    int64_t v[16] = {0};
+    v[0] = q.kernel.width;
+    for (int a = 0; a < 3; ++a) {
+        v[1 + a] = a < q.dim ? q.fine[a] : 1;
+        v[4 + a] = a < q.dim ? ((int64_t)1 << q.gp.ax[a].shift) : 1;
+        v[9 + a] = a < q.dim ? q.sp.block[a] : 1;
+        v[13 + a] = a < p.dim ? p.type3_outer[a] : 1;
+    }
+    v[7] = q.gp.nbins;
+    v[8] = q.geometry_index;
+    v[12] = q.sp.run;
+    for (int i = 0; i < n && i < 16; ++i) info[i] = v[i];
+    return MNUFFT_SUCCESS;
+}
+
+double mnufft_plan_beta(mnufft_plan plan) { return plan ? plan->plan->kernel.beta : 0.0; }
+
+int mnufft_stage_times(mnufft_plan plan, double *ms, int32_t n)
+{
+    return guarded([&] {
+        if (!plan || !ms) {
+            set_error("null argument");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        return plan->plan->stage_times(ms, n) ? MNUFFT_SUCCESS : MNUFFT_ERROR_GPU;
+    });
+}
+
+int mnufft_direct_f64(int32_t kind, int32_t dim, const int64_t *n_modes, int32_t isign,
+                      int32_t mode_order, int64_t M, const double *d_x, const double *d_in,
+                      int64_t nout, const int64_t *d_sel, const double *d_s, double *d_out,
+                      void *stream)
+{
+    return guarded([&] {
+        if (kind < 1 || kind > 3 || dim < 1 || dim > 3 || !d_out || (kind != 3 && !n_modes)) {
+            set_error("invalid direct-sum arguments");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        mnufft::DirectParams dp{};
+        dp.mode_total = 1;
+        for (int a = 0; a < 3; ++a) {
+            dp.n_modes[a] = (kind != 3 && a < dim) ? n_modes[a] : 1;
+            dp.mode_total *= dp.n_modes[a];
+        }
+        dp.M = M;
+        dp.nout = nout;
+        dp.centered = mode_order == 0;
+        dp.sign = isign >= 0 ? 1.0 : -1.0;
+        if (nout == 0) return MNUFFT_SUCCESS;
+        return mnufft::direct_sum(kind, dim, dp, d_x, d_in, d_sel, d_s, d_out,
+                                  (hipStream_t)stream)
+                   ? MNUFFT_SUCCESS
+                   : MNUFFT_ERROR_GPU;
+    });
+}
+
+int mnufft_kernel_params(double eps, double sigma, int32_t *width, double *beta)
+{
+    mnufft::EsKernel k;
+    if (!width || !beta) {
+        set_error("null argument");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (!mnufft::es_kernel_for_tolerance(eps, sigma == 0.0 ? 2.0 : sigma, &k))
+        return MNUFFT_ERROR_PLAN;
+    *width = k.width;
+    *beta = k.beta;
+    return MNUFFT_SUCCESS;
+}
+
+int64_t mnufft_fine_grid_size(int64_t n_modes, double sigma, int32_t width)
+{
+    return mnufft::select_fine_grid_size(n_modes, sigma == 0.0 ? 2.0 : sigma, width);
+}
+
+int mnufft_layout_query(int32_t dim, const int64_t *fine, int32_t width, int32_t precision,
+                        int64_t point_count, int64_t *out)
+{
+    return guarded([&] {
+        if (dim < 1 || dim > 3 || !fine || !out) {
+            set_error("invalid layout query");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        mnufft::SpreadLayout L = mnufft::build_spread_layout(
+            dim, fine, width, precision == MNUFFT_PRECISION_F32 ? Precision::F32 : Precision::F64);
+        for (int i = 0; i < 64; ++i) out[i] = 0;
+        out[0] = L.run;
+        out[1] = L.workgroup_size;
+        out[2] = L.row_batch;
+        out[3] = (int64_t)L.geometries.size();
+        out[4] = L.choose_geometry(point_count);
+        for (int a = 0; a < 3; ++a) {
+            out[5 + a] = a < dim ? L.block[a] : 1;
+            out[8 + a] = a < dim ? L.blocks[a] : 1;
+        }
+        for (size_t g = 0; g < L.geometries.size() && g < 16; ++g)
+            for (int a = 0; a < 3; ++a) out[16 + 3 * g + a] = a < dim ? L.geometries[g].side[a] : 1;
+        return MNUFFT_SUCCESS;
+    });
+}
+
+}  // extern "C"
+
+// ======================================================================
+// Host-array API (reference C ABI).
+// ======================================================================
+namespace {
+
+struct HostPlan {
+    int type = 1;
+    int dim = 1;
+    int64_t n_modes[3] = {1, 1, 1};
+    int isign = 1;
+    int64_t ntrans = 1;
+    double eps = 1e-6;
+    double sigma = 2.0;
+    int mode_order = 0;
+    Precision precision = Precision::F64;
+    bool float_api = false;
+    std::unique_ptr<Plan> plan;
+    // device copies
+    void *d_points = nullptr, *d_targets = nullptr, *d_in = nullptr, *d_out = nullptr;
+    size_t points_bytes = 0, targets_bytes = 0, in_bytes = 0, out_bytes = 0;
+    int64_t M = 0, N = 0;
+    bool points_set = false;
+    ~HostPlan()
+    {
+        (void)hipDeviceSynchronize();
+        for (void *p : {d_points, d_targets, d_in, d_out})
+            if (p) (void)hipFree(p);
+    }
+    size_t real() const { return precision == Precision::F32 ? 4 : 8; }
+};
+
+bool grow(void **p, size_t *cap, size_t need)
+{
+    if (need <= *cap && *p) return true;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (!mnufft::hip_ok(hipMalloc(p, need ? need : 16), "device buffer")) return false;
+    *cap = need;
+    return true;
+}
+
+int hp_makeplan(bool float_api, int32_t type, int32_t dim, const int64_t *n_modes, int32_t isign,
+                int64_t ntrans, double eps, HostPlan **out, const wgpu_nufft_opts *opts)
+{
+    if (!out) {
+        set_error("plan output pointer is null");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    wgpu_nufft_opts o;
+    std::memset(&o, 0, sizeof(o));
+    if (opts) o = *opts;
+    if (o.backend == WGPU_NUFFT_BACKEND_CPU) {
+        set_error("this engine has no CPU backend; use BACKEND_AUTO or BACKEND_GPU");
+        return WGPU_NUFFT_ERROR_GPU_UNAVAILABLE;
+    }
+    if (o.backend < 0 || o.backend > 2 || o.precision < 0 || o.precision > 3 ||
+        (o.mode_order != 0 && o.mode_order != 1) || o.threads < 0 || !(o.sigma >= 0.0)) {
+        set_error("invalid options");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (o.adapter_index > 1 || (o.adapter_index < 0)) {
+        set_error("adapter selection matches no HIP device");
+        return WGPU_NUFFT_ERROR_GPU_UNAVAILABLE;
+    }
+    if (type < 1 || type > 3) {
+        set_error("type must be 1, 2 or 3");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (dim < 1 || dim > 3) {
+        set_error("dim must be 1, 2 or 3");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (ntrans < 1) {
+        set_error("ntrans must be at least 1");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    std::string why;
+    if (!gpu_available(&why)) {
+        set_error(why);
+        return WGPU_NUFFT_ERROR_GPU_UNAVAILABLE;
+    }
+    auto hp = std::make_unique<HostPlan>();
+    hp->type = type;
+    hp->dim = dim;
+    hp->isign = isign >= 0 ? 1 : -1;
+    hp->ntrans = ntrans;
+    hp->eps = eps;
+    hp->sigma = o.sigma == 0.0 ? 2.0 : o.sigma;
+    hp->mode_order = o.mode_order;
+    hp->float_api = float_api;
+    hp->precision = (float_api || o.precision == WGPU_NUFFT_PRECISION_F32) ? Precision::F32
+                                                                          : Precision::F64;
+    if (type != 3) {
+        if (!n_modes) {
+            set_error("n_modes is null");
+            return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        for (int a = 0; a < dim; ++a) hp->n_modes[a] = n_modes[a];
+        int prec = hp->precision == Precision::F32 ? MNUFFT_PRECISION_F32 : MNUFFT_PRECISION_F64;
+        int rc = make_plan(type, dim, n_modes, isign, ntrans, eps, prec, o.mode_order, hp->sigma,
+                           &hp->plan);
+        if (rc) return rc;
+    } else {
+        mnufft::EsKernel k;
+        if (!mnufft::es_kernel_for_tolerance(eps, hp->sigma, &k)) return WGPU_NUFFT_ERROR_PLAN;
+    }
+    *out = hp.release();
+    return WGPU_NUFFT_SUCCESS;
+}
+
+// Interleaves per-axis host arrays (or takes point-major ones) into the
+// plan's arithmetic type, validating the coordinate range of types 1 and 2.
+template <typename S>
+bool pack_points(const HostPlan &hp, int64_t M, const S *const *axes, const S *nd,
+                 std::vector<unsigned char> *out, bool check_range, const char *what)
+{
+    int d = hp.dim;
+    out->resize((size_t)M * d * hp.real());
+    for (int64_t j = 0; j < M; ++j)
+        for (int a = 0; a < d; ++a) {
+            double v = nd ? (double)nd[j * d + a] : (double)axes[a][j];
+            if (!std::isfinite(v) || (check_range && std::fabs(v) > kMaxCoordinate)) {
+                set_error(std::string(what) + " " + std::to_string(j) + " axis " +
+                          std::to_string(a) + " is " + std::to_string(v) +
+                          (check_range ? ", outside [-3 pi, 3 pi]" : ", not finite"));
+                return false;
+            }
+            if (hp.precision == Precision::F32) {
+                float f = nd ? (float)nd[j * d + a] : (float)axes[a][j];
+                std::memcpy(out->data() + ((size_t)j * d + a) * 4, &f, 4);
+            } else {
+                std::memcpy(out->data() + ((size_t)j * d + a) * 8, &v, 8);
+            }
+        }
+    return true;
+}
+
+template <typename S>
+int hp_setpts(HostPlan *hp, int64_t M, const S *const *axes, const S *nd, int64_t N,
+              const S *const *taxes, const S *tnd)
+{
+    if (!hp) {
+        set_error("plan is null");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (M < 0 || (M > 0 && !nd && !axes[0]) || (hp->dim > 1 && M > 0 && !nd && !axes[1]) ||
+        (hp->dim > 2 && M > 0 && !nd && !axes[2])) {
+        set_error("point count or coordinate arrays invalid");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    std::vector<unsigned char> pts;
+    if (!pack_points<S>(*hp, M, axes, nd, &pts, hp->type != 3, "point"))
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    if (!grow(&hp->d_points, &hp->points_bytes, pts.size())) return WGPU_NUFFT_ERROR_GPU;
+    if (!pts.empty() &&
+        !mnufft::hip_ok(hipMemcpy(hp->d_points, pts.data(), pts.size(), hipMemcpyHostToDevice),
+                        "point upload"))
+        return WGPU_NUFFT_ERROR_GPU;
+    hp->M = M;
+    if (hp->type == 3) {
+        if (N < 0 || (N > 0 && !tnd && (!taxes[0] || (hp->dim > 1 && !taxes[1]) ||
+                                         (hp->dim > 2 && !taxes[2])))) {
+            set_error("target count or target arrays invalid");
+            return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        std::vector<unsigned char> tg;
+        if (!pack_points<S>(*hp, N, taxes, tnd, &tg, false, "target"))
+            return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+        if (!grow(&hp->d_targets, &hp->targets_bytes, tg.size())) return WGPU_NUFFT_ERROR_GPU;
+        if (!tg.empty() &&
+            !mnufft::hip_ok(hipMemcpy(hp->d_targets, tg.data(), tg.size(), hipMemcpyHostToDevice),
+                            "target upload"))
+            return WGPU_NUFFT_ERROR_GPU;
+        hp->N = N;
+        // bounds from the actual points, as the plan's caller guarantee
+        double sb[6], tb[6];
+        for (int a = 0; a < hp->dim; ++a) {
+            double lo = 0, hi = 0, tlo = 0, thi = 0;
+            for (int64_t j = 0; j < M; ++j) {
+                double v = nd ? (double)nd[j * hp->dim + a] : (double)axes[a][j];
+                lo = j == 0 ? v : std::min(lo, v);
+                hi = j == 0 ? v : std::max(hi, v);
+            }
+            for (int64_t k = 0; k < N; ++k) {
+                double v = tnd ? (double)tnd[k * hp->dim + a] : (double)taxes[a][k];
+                tlo = k == 0 ? v : std::min(tlo, v);
+                thi = k == 0 ? v : std::max(thi, v);
+            }
+            sb[2 * a] = lo;
+            sb[2 * a + 1] = hi;
+            tb[2 * a] = tlo;
+            tb[2 * a + 1] = thi;
+        }
+        auto p = std::make_unique<Plan>();
+        p->type = 3;
+        p->dim = hp->dim;
+        p->isign = hp->isign;
+        p->batch = hp->ntrans;
+        p->eps = hp->eps;
+        p->sigma = hp->sigma;
+        p->precision = hp->precision;
+        if (!p->init_type3(sb, tb)) return WGPU_NUFFT_ERROR_PLAN;
+        hp->plan = std::move(p);
+    } else {
+        if (!hp->plan->set_points(M, hp->d_points, nullptr)) return WGPU_NUFFT_ERROR_GPU;
+    }
+    hp->points_set = true;
+    return WGPU_NUFFT_SUCCESS;
+}
+
+template <typename S>
+int hp_execute(HostPlan *hp, S *c, S *f)
+{
+    if (!hp) {
+        set_error("plan is null");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (!hp->points_set) {
+        set_error("setpts must be called before execute");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    int64_t modes = 1;
+    for (int a = 0; a < hp->dim; ++a) modes *= hp->n_modes[a];
+    int64_t nin, nout;
+    const S *src;
+    S *dst;
+    if (hp->type == 1) {
+        nin = hp->M * hp->ntrans;
+        nout = modes * hp->ntrans;
+        src = c;
+        dst = f;
+    } else if (hp->type == 2) {
+        nin = modes * hp->ntrans;
+        nout = hp->M * hp->ntrans;
+        src = f;
+        dst = c;
+    } else {
+        nin = hp->M * hp->ntrans;
+        nout = hp->N * hp->ntrans;
+        src = c;
+        dst = f;
+    }
+    if ((nin > 0 && !src) || (nout > 0 && !dst)) {
+        set_error("input or output array is null");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    size_t rb = hp->real();
+    // widen or narrow to the plan's arithmetic
+    std::vector<unsigned char> in((size_t)nin * 2 * rb);
+    for (int64_t i = 0; i < 2 * nin; ++i) {
+        if (!std::isfinite((double)src[i])) {
+            set_error("input value " + std::to_string(i / 2) + " is not finite");
+            return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (rb == 4) {
+            float v = (float)src[i];
+            std::memcpy(in.data() + 4 * i, &v, 4);
+        } else {
+            double v = (double)src[i];
+            std::memcpy(in.data() + 8 * i, &v, 8);
+        }
+    }
+    if (!grow(&hp->d_in, &hp->in_bytes, in.size()) ||
+        !grow(&hp->d_out, &hp->out_bytes, (size_t)nout * 2 * rb))
+        return WGPU_NUFFT_ERROR_GPU;
+    if (!in.empty() &&
+        !mnufft::hip_ok(hipMemcpy(hp->d_in, in.data(), in.size(), hipMemcpyHostToDevice),
+                        "input upload"))
+        return WGPU_NUFFT_ERROR_GPU;
+    bool ok;
+    if (hp->type == 3)
+        ok = hp->plan->execute_type3(hp->ntrans, hp->M, hp->d_points, hp->d_in, hp->N,
+                                     hp->d_targets, hp->d_out, nullptr);
+    else
+        ok = hp->plan->execute(hp->ntrans, hp->d_in, hp->d_out, nullptr);
+    if (!ok) return WGPU_NUFFT_ERROR_GPU;
+    if (hp->type == 2 && hp->M == 0) return WGPU_NUFFT_SUCCESS;
+    std::vector<unsigned char> out((size_t)nout * 2 * rb);
+    if (!out.empty() &&
+        !mnufft::hip_ok(hipMemcpy(out.data(), hp->d_out, out.size(), hipMemcpyDeviceToHost),
+                        "output download"))
+        return WGPU_NUFFT_ERROR_GPU;
+    for (int64_t i = 0; i < 2 * nout; ++i) {
+        if (rb == 4) {
+            float v;
+            std::memcpy(&v, out.data() + 4 * i, 4);
+            dst[i] = (S)v;
+        } else {
+            double v;
+            std::memcpy(&v, out.data() + 8 * i, 8);
+            dst[i] = (S)v;
+        }
+    }
+    return WGPU_NUFFT_SUCCESS;
+}
+
+std::mutex g_name_mutex;
+std::string g_gpu_name;
+
+template <typename S>
+int one_call(bool float_api, int type, int dim, int64_t M, const S *x, const S *y, const S *z,
+             S *c, int isign, double eps, const int64_t *nm, int64_t N, const S *s, const S *t,
+             const S *u, S *f, const wgpu_nufft_opts *opts)
+{
+    HostPlan *hp = nullptr;
+    int rc = hp_makeplan(float_api, type, dim, nm, isign, 1, eps, &hp, opts);
+    if (rc) return rc;
+    std::unique_ptr<HostPlan> owner(hp);
+    const S *axes[3] = {x, y, z};
+    const S *taxes[3] = {s, t, u};
+    rc = hp_setpts<S>(hp, M, axes, nullptr, N, taxes, nullptr);
+    if (rc) return rc;
+    return hp_execute<S>(hp, c, f);
+}
+
+}  // namespace
+
+struct wgpu_nufft_plan_s {
+    std::unique_ptr<HostPlan> hp;
+};
+struct wgpu_nufftf_plan_s {
+    std::unique_ptr<HostPlan> hp;
+};
+
+extern "C" {
+
+void wgpu_nufft_default_opts(wgpu_nufft_opts *opts)
+{
+    if (opts) std::memset(opts, 0, sizeof(*opts));
+}
+
+int wgpu_nufft_list_adapters(wgpu_nufft_adapter *adapters, int32_t capacity, int32_t *count)
+{
+    return guarded([&] {
+        if (!count || (capacity > 0 && !adapters)) {
+            set_error("null argument");
+            return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+        for (int i = 0; i < n && i < capacity; ++i) {
+            hipDeviceProp_t prop;
+            std::memset(&adapters[i], 0, sizeof(wgpu_nufft_adapter));
+            if (hipGetDeviceProperties(&prop, i) == hipSuccess) {
+                std::snprintf(adapters[i].name, sizeof(adapters[i].name), "%s", prop.name);
+                std::snprintf(adapters[i].pci_bus_id, sizeof(adapters[i].pci_bus_id),
+                              "%04x:%02x:%02x.0", prop.pciDomainID, prop.pciBusID,
+                              prop.pciDeviceID);
+            }
+            std::snprintf(adapters[i].backend, sizeof(adapters[i].backend), "HIP");
+            std::snprintf(adapters[i].device_type, sizeof(adapters[i].device_type), "DiscreteGpu");
+            adapters[i].is_default = i == 0;
+        }
+        return WGPU_NUFFT_SUCCESS;
+    });
+}
+
+const char *wgpu_nufft_last_error(void) { return mnufft::last_error(); }
+
+const char *wgpu_nufft_version(void) { return "0.3.2"; }
+
+const char *wgpu_nufft_gpu_name(void)
+{
+    std::lock_guard<std::mutex> lock(g_name_mutex);
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        g_gpu_name.clear();
+        return g_gpu_name.c_str();
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess)
+        g_gpu_name = std::string(prop.name) + " (HIP)";
+    return g_gpu_name.c_str();
+}
+
+void wgpu_nufft_shutdown(void) {}
+
+int wgpu_nufft_makeplan(int32_t type, int32_t dim, const int64_t *n_modes, int32_t isign,
+                        int64_t ntrans, double eps, wgpu_nufft_plan *plan,
+                        const wgpu_nufft_opts *opts)
+{
+    return guarded([&] {
+        HostPlan *hp = nullptr;
+        int rc = hp_makeplan(false, type, dim, n_modes, isign, ntrans, eps, &hp, opts);
+        if (plan) *plan = nullptr;
+        if (rc) return rc;
+        *plan = new wgpu_nufft_plan_s{std::unique_ptr<HostPlan>(hp)};
+        return WGPU_NUFFT_SUCCESS;
+    });
+}
+
+int wgpu_nufft_setpts(wgpu_nufft_plan plan, int64_t M, const double *x, const double *y,
+                      const double *z, int64_t N, const double *s, const double *t,
+                      const double *u)
+{
+    return guarded([&] {
+        const double *axes[3] = {x, y, z}, *taxes[3] = {s, t, u};
+        return hp_setpts<double>(plan ? plan->hp.get() : nullptr, M, axes, nullptr, N, taxes, nullptr);
+    });
+}
+
+int wgpu_nufft_setpts_nd(wgpu_nufft_plan plan, int64_t M, const double *points, int64_t N,
+                         const double *targets)
+{
+    return guarded([&] {
+        const double *axes[3] = {points, points, points}, *taxes[3] = {targets, targets, targets};
+        if (M > 0 && !points) {
+            set_error("points array is null");
+            return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        return hp_setpts<double>(plan ? plan->hp.get() : nullptr, M, axes, points, N, taxes, targets);
+    });
+}
+
+int wgpu_nufft_execute(wgpu_nufft_plan plan, double *c, double *f)
+{
+    return guarded([&] { return hp_execute<double>(plan ? plan->hp.get() : nullptr, c, f); });
+}
+
+void wgpu_nufft_destroy(wgpu_nufft_plan plan)
+{
+    try {
+        delete plan;
+    } catch (...) {
+    }
+}
+
+int wgpu_nufft_plan_backend(wgpu_nufft_plan plan) { return plan ? WGPU_NUFFT_BACKEND_GPU : -1; }
+
+int wgpu_nufft_plan_precision(wgpu_nufft_plan plan)
+{
+    if (!plan) return -1;
+    return plan->hp->precision == Precision::F32 ? WGPU_NUFFT_PRECISION_F32
+                                                : WGPU_NUFFT_PRECISION_F64;
+}
+
+int wgpu_nufftf_makeplan(int32_t type, int32_t dim, const int64_t *n_modes, int32_t isign,
+                         int64_t ntrans, double eps, wgpu_nufftf_plan *plan,
+                         const wgpu_nufft_opts *opts)
+{
+    return guarded([&] {
+        HostPlan *hp = nullptr;
+        int rc = hp_makeplan(true, type, dim, n_modes, isign, ntrans, eps, &hp, opts);
+        if (plan) *plan = nullptr;
+        if (rc) return rc;
+        *plan = new wgpu_nufftf_plan_s{std::unique_ptr<HostPlan>(hp)};
+        return WGPU_NUFFT_SUCCESS;
+    });
+}
+
+int wgpu_nufftf_setpts(wgpu_nufftf_plan plan, int64_t M, const float *x, const float *y,
+                       const float *z, int64_t N, const float *s, const float *t, const float *u)
+{
+    return guarded([&] {
+        const float *axes[3] = {x, y, z}, *taxes[3] = {s, t, u};
+        return hp_setpts<float>(plan ? plan->hp.get() : nullptr, M, axes, nullptr, N, taxes, nullptr);
+    });
+}
+
+int wgpu_nufftf_setpts_nd(wgpu_nufftf_plan plan, int64_t M, const float *points, int64_t N,
+                          const float *targets)
+{
+    return guarded([&] {
+        const float *axes[3] = {points, points, points}, *taxes[3] = {targets, targets, targets};
+        if (M > 0 && !points) {
+            set_error("points array is null");
+            return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        return hp_setpts<float>(plan ? plan->hp.get() : nullptr, M, axes, points, N, taxes, targets);
+    });
+}
+
+int wgpu_nufftf_execute(wgpu_nufftf_plan plan, float *c, float *f)
+{
+    return guarded([&] { return hp_execute<float>(plan ? plan->hp.get() : nullptr, c, f); });
+}
+
+void wgpu_nufftf_destroy(wgpu_nufftf_plan plan)
+{
+    try {
+        delete plan;
+    } catch (...) {
+    }
+}
+
+int wgpu_nufftf_plan_backend(wgpu_nufftf_plan plan) { return plan ? WGPU_NUFFT_BACKEND_GPU : -1; }
+
+int wgpu_nufftf_plan_precision(wgpu_nufftf_plan plan)
+{
+    return plan ? WGPU_NUFFT_PRECISION_F32 : -1;
+}
+
+int wgpu_nufft1d1(int64_t M, const double *x, const double *c, int32_t isign, double eps,
+                  int64_t ms, double *f, const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, 1, 1};
+    return guarded([&] {
+        return one_call<double>(false, 1, 1, M, x, nullptr, nullptr, const_cast<double *>(c),
+                                isign, eps, nm, 0, nullptr, nullptr, nullptr, f, opts);
+    });
+}
+int wgpu_nufft1d2(int64_t M, const double *x, double *c, int32_t isign, double eps, int64_t ms,
+                  const double *f, const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, 1, 1};
+    return guarded([&] {
+        return one_call<double>(false, 2, 1, M, x, nullptr, nullptr, c, isign, eps, nm, 0,
+                                nullptr, nullptr, nullptr, const_cast<double *>(f), opts);
+    });
+}
+int wgpu_nufft1d3(int64_t M, const double *x, const double *c, int32_t isign, double eps,
+                  int64_t N, const double *s, double *f, const wgpu_nufft_opts *opts)
+{
+    return guarded([&] {
+        return one_call<double>(false, 3, 1, M, x, nullptr, nullptr, const_cast<double *>(c),
+                                isign, eps, nullptr, N, s, nullptr, nullptr, f, opts);
+    });
+}
+int wgpu_nufft2d1(int64_t M, const double *x, const double *y, const double *c, int32_t isign,
+                  double eps, int64_t ms, int64_t mt, double *f, const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, mt, 1};
+    return guarded([&] {
+        return one_call<double>(false, 1, 2, M, x, y, nullptr, const_cast<double *>(c), isign,
+                                eps, nm, 0, nullptr, nullptr, nullptr, f, opts);
+    });
+}
+int wgpu_nufft2d2(int64_t M, const double *x, const double *y, double *c, int32_t isign,
+                  double eps, int64_t ms, int64_t mt, const double *f,
+                  const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, mt, 1};
+    return guarded([&] {
+        return one_call<double>(false, 2, 2, M, x, y, nullptr, c, isign, eps, nm, 0, nullptr,
+                                nullptr, nullptr, const_cast<double *>(f), opts);
+    });
+}
+int wgpu_nufft2d3(int64_t M, const double *x, const double *y, const double *c, int32_t isign,
+                  double eps, int64_t N, const double *s, const double *t, double *f,
+                  const wgpu_nufft_opts *opts)
+{
+    return guarded([&] {
+        return one_call<double>(false, 3, 2, M, x, y, nullptr, const_cast<double *>(c), isign,
+                                eps, nullptr, N, s, t, nullptr, f, opts);
+    });
+}
+int wgpu_nufft3d1(int64_t M, const double *x, const double *y, const double *z, const double *c,
+                  int32_t isign, double eps, int64_t ms, int64_t mt, int64_t mu, double *f,
+                  const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, mt, mu};
+    return guarded([&] {
+        return one_call<double>(false, 1, 3, M, x, y, z, const_cast<double *>(c), isign, eps, nm,
+                                0, nullptr, nullptr, nullptr, f, opts);
+    });
+}
+int wgpu_nufft3d2(int64_t M, const double *x, const double *y, const double *z, double *c,
+                  int32_t isign, double eps, int64_t ms, int64_t mt, int64_t mu,
+                  const double *f, const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, mt, mu};
+    return guarded([&] {
+        return one_call<double>(false, 2, 3, M, x, y, z, c, isign, eps, nm, 0, nullptr, nullptr,
+                                nullptr, const_cast<double *>(f), opts);
+    });
+}
+int wgpu_nufft3d3(int64_t M, const double *x, const double *y, const double *z, const double *c,
+                  int32_t isign, double eps, int64_t N, const double *s, const double *t,
+                  const double *u, double *f, const wgpu_nufft_opts *opts)
+{
+    return guarded([&] {
+        return one_call<double>(false, 3, 3, M, x, y, z, const_cast<double *>(c), isign, eps,
+                                nullptr, N, s, t, u, f, opts);
+    });
+}
+
+int wgpu_nufftf1d1(int64_t M, const float *x, const float *c, int32_t isign, double eps,
+                   int64_t ms, float *f, const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, 1, 1};
+    return guarded([&] {
+        return one_call<float>(true, 1, 1, M, x, nullptr, nullptr, const_cast<float *>(c), isign,
+                               eps, nm, 0, nullptr, nullptr, nullptr, f, opts);
+    });
+}
+int wgpu_nufftf1d2(int64_t M, const float *x, float *c, int32_t isign, double eps, int64_t ms,
+                   const float *f, const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, 1, 1};
+    return guarded([&] {
+        return one_call<float>(true, 2, 1, M, x, nullptr, nullptr, c, isign, eps, nm, 0, nullptr,
+                               nullptr, nullptr, const_cast<float *>(f), opts);
+    });
+}
+int wgpu_nufftf1d3(int64_t M, const float *x, const float *c, int32_t isign, double eps,
+                   int64_t N, const float *s, float *f, const wgpu_nufft_opts *opts)
+{
+    return guarded([&] {
+        return one_call<float>(true, 3, 1, M, x, nullptr, nullptr, const_cast<float *>(c), isign,
+                               eps, nullptr, N, s, nullptr, nullptr, f, opts);
+    });
+}
+int wgpu_nufftf2d1(int64_t M, const float *x, const float *y, const float *c, int32_t isign,
+                   double eps, int64_t ms, int64_t mt, float *f, const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, mt, 1};
+    return guarded([&] {
+        return one_call<float>(true, 1, 2, M, x, y, nullptr, const_cast<float *>(c), isign, eps,
+                               nm, 0, nullptr, nullptr, nullptr, f, opts);
+    });
+}
+int wgpu_nufftf2d2(int64_t M, const float *x, const float *y, float *c, int32_t isign,
+                   double eps, int64_t ms, int64_t mt, const float *f,
+                   const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, mt, 1};
+    return guarded([&] {
+        return one_call<float>(true, 2, 2, M, x, y, nullptr, c, isign, eps, nm, 0, nullptr,
+                               nullptr, nullptr, const_cast<float *>(f), opts);
+    });
+}
+int wgpu_nufftf2d3(int64_t M, const float *x, const float *y, const float *c, int32_t isign,
+                   double eps, int64_t N, const float *s, const float *t, float *f,
+                   const wgpu_nufft_opts *opts)
+{
+    return guarded([&] {
+        return one_call<float>(true, 3, 2, M, x, y, nullptr, const_cast<float *>(c), isign, eps,
+                               nullptr, N, s, t, nullptr, f, opts);
+    });
+}
+int wgpu_nufftf3d1(int64_t M, const float *x, const float *y, const float *z, const float *c,
+                   int32_t isign, double eps, int64_t ms, int64_t mt, int64_t mu, float *f,
+                   const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, mt, mu};
+    return guarded([&] {
+        return one_call<float>(true, 1, 3, M, x, y, z, const_cast<float *>(c), isign, eps, nm, 0,
+                               nullptr, nullptr, nullptr, f, opts);
+    });
+}
+int wgpu_nufftf3d2(int64_t M, const float *x, const float *y, const float *z, float *c,
+                   int32_t isign, double eps, int64_t ms, int64_t mt, int64_t mu,
+                   const float *f, const wgpu_nufft_opts *opts)
+{
+    int64_t nm[3] = {ms, mt, mu};
+    return guarded([&] {
+        return one_call<float>(true, 2, 3, M, x, y, z, c, isign, eps, nm, 0, nullptr, nullptr,
+                               nullptr, const_cast<float *>(f), opts);
+    });
+}
+int wgpu_nufftf3d3(int64_t M, const float *x, const float *y, const float *z, const float *c,
+                   int32_t isign, double eps, int64_t N, const float *s, const float *t,
+                   const float *u, float *f, const wgpu_nufft_opts *opts)
+{
+    return guarded([&] {
+        return one_call<float>(true, 3, 3, M, x, y, z, const_cast<float *>(c), isign, eps,
+                               nullptr, N, s, t, u, f, opts);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,14 @@
+// Type-2 interpolation from the fine grid at the points.
+#pragma once
+
+#include "nufft_internal.hpp"
+
+namespace mnufft {
+
+// out[v * M + j] = sum over the support of point j of grid[v] * weights, for
+// the `vectors` grids; points are visited in the order of `sorted`.
+template <int D, typename T>
+bool interpolate(int64_t M, int64_t vectors, const uint32_t *sorted, const T *coords,
+                 const GridParams &g, const T *grid, T *out, hipStream_t stream);
+
+}  // namespace mnufft

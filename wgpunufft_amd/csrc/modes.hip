@@ -1,0 +1,345 @@
+// Mode-space stages: type-1 deconvolution / truncation, type-2
+// predeconvolution onto the fine grid, type-3 rescale and phase stages, and
+// FP64 direct sums.
+//
+// Reference behaviour:
+//  * deconvolution (src/gpu_type1_nd.rs:617-687): modes[v,k] =
+//    fine[v, fold(k)] * amp[k], fold centered: k < N/2 ? n - (N/2 - k) :
+//    k - N/2; FFT order: k < ceil(N/2) ? k : n - (N - k);
+//  * predeconvolution (src/gpu_3d.rs:902-970, src/gpu_nd.rs:618-697): the
+//    inverse mapping onto a cleared grid;
+//  * amplitudes (src/gpu_nd.rs:910-1014): amp[k] = prod_a 1/phi_hat_a(|k_a|),
+//    formed in f64;
+//  * type 3 (src/gpu_type3.rs:741-1297): x' = (x - C)/gamma and
+//    c' = c exp(i isign D.x); s' = h gamma (s - D) and
+//    factor = exp(i isign C.(s - D)) / prod_a phi_hat(s'_a); out = interp * factor.
+//
+// MI355X design: every stage is one HBM-streaming pass, one lane per output
+// element.  Predeconvolution writes every fine cell once (the mode or zero),
+// which fuses the reference's clear pass into the scatter and keeps the
+// sequence free of memset nodes.
+#include <hip/hip_runtime.h>
+
+#include "modes.hpp"
+
+namespace mnufft {
+
+namespace {
+
+inline unsigned grid_for(int64_t n, int threads)
+{
+    int64_t b = (n + threads - 1) / threads;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_deconv(ModeParams mp, const double *__restrict__ inv_phihat,
+                                                const T *__restrict__ grid, T *__restrict__ modes,
+                                                int64_t vectors)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= mp.mode_total * vectors) return;
+    int64_t v = t / mp.mode_total;
+    int64_t k = t - v * mp.mode_total;
+    int64_t rest = k, fine_lin = 0, stride = 1;
+    double amp = 1.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        int64_t N = mp.n_modes[a], n = mp.fine[a];
+        int64_t ka = rest % N;
+        rest /= N;
+        int64_t m = mp.centered ? ka - N / 2 : (ka < (N + 1) / 2 ? ka : ka - N);
+        int64_t f = m < 0 ? m + n : m;
+        fine_lin += f * stride;
+        stride *= n;
+        amp *= inv_phihat[mp.phihat_off[a] + (m < 0 ? -m : m)];
+    }
+    const T *src = grid + 2 * (v * mp.fine_total + fine_lin);
+    T a = (T)amp;
+    modes[2 * t] = src[0] * a;
+    modes[2 * t + 1] = src[1] * a;
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_predeconv(ModeParams mp,
+                                                   const double *__restrict__ inv_phihat,
+                                                   const T *__restrict__ modes,
+                                                   T *__restrict__ grid, int64_t vectors)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= mp.fine_total * vectors) return;
+    int64_t v = t / mp.fine_total;
+    int64_t f = t - v * mp.fine_total;
+    int64_t rest = f, mode_lin = 0, stride = 1;
+    bool has = true;
+    double amp = 1.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        int64_t N = mp.n_modes[a], n = mp.fine[a];
+        int64_t fa = rest % n;
+        rest /= n;
+        int64_t hi = N - 1 - N / 2;  // largest frequency
+        int64_t m;
+        if (fa <= hi)
+            m = fa;
+        else if (fa >= n - N / 2)
+            m = fa - n;
+        else {
+            has = false;
+            m = 0;
+        }
+        int64_t ka = mp.centered ? m + N / 2 : (m >= 0 ? m : m + N);
+        mode_lin += ka * stride;
+        stride *= N;
+        amp *= inv_phihat[mp.phihat_off[a] + (m < 0 ? -m : m)];
+    }
+    T re = (T)0, im = (T)0;
+    if (has) {
+        const T *src = modes + 2 * (v * mp.mode_total + mode_lin);
+        T a = (T)amp;
+        re = src[0] * a;
+        im = src[1] * a;
+    }
+    grid[2 * t] = re;
+    grid[2 * t + 1] = im;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T *__restrict__ p, int64_t n)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = (T)0;
+}
+
+// ---------------------------------------------------------------------
+// Type 3.
+// ---------------------------------------------------------------------
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_type3_source(Type3Params tp, int64_t M, int64_t vectors,
+                                                      const T *__restrict__ x,
+                                                      const T *__restrict__ c,
+                                                      T *__restrict__ xr, T *__restrict__ cp)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M * vectors) return;
+    int64_t j = t % M;
+    double phase = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        double xa = (double)x[j * D + a];
+        if (t < M) xr[j * D + a] = (T)((xa - tp.src_center[a]) * tp.inv_gamma[a]);
+        phase += xa * tp.tgt_center[a];
+    }
+    double sn, cs;
+    sincos(phase * tp.sign, &sn, &cs);
+    double re = (double)c[2 * t], im = (double)c[2 * t + 1];
+    cp[2 * t] = (T)(re * cs - im * sn);
+    cp[2 * t + 1] = (T)(re * sn + im * cs);
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_type3_target(Type3Params tp, int64_t N,
+                                                      const T *__restrict__ s,
+                                                      T *__restrict__ sr,
+                                                      double *__restrict__ factor)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    double phase = 0.0, phi = 1.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        double cen = (double)s[k * D + a] - tp.tgt_center[a];
+        double resc = cen * tp.tgt_scale[a];
+        sr[k * D + a] = (T)resc;
+        phase += cen * tp.src_center[a];
+        double ft = 0.0;
+        for (int q = 0; q < tp.nquad; ++q) ft += tp.quad_w[q] * cos(resc * tp.quad_x[q]);
+        phi *= ft;
+    }
+    double sn, cs;
+    sincos(phase * tp.sign, &sn, &cs);
+    factor[2 * k] = cs / phi;
+    factor[2 * k + 1] = sn / phi;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_type3_final(int64_t N, int64_t vectors,
+                                                     const T *__restrict__ interp,
+                                                     const double *__restrict__ factor,
+                                                     T *__restrict__ out)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * vectors) return;
+    int64_t k = t % N;
+    double fr = factor[2 * k], fi = factor[2 * k + 1];
+    double re = (double)interp[2 * t], im = (double)interp[2 * t + 1];
+    out[2 * t] = (T)(re * fr - im * fi);
+    out[2 * t + 1] = (T)(re * fi + im * fr);
+}
+
+// ---------------------------------------------------------------------
+// FP64 direct sums (reference src/direct.rs:9-131, src/type3.rs:609-673):
+// one lane per output, the inputs streamed through LDS in tiles.
+// ---------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void k_direct(int kind, DirectParams dp,
+                                                const double *__restrict__ x,
+                                                const double *__restrict__ in,
+                                                const int64_t *__restrict__ sel,
+                                                const double *__restrict__ s,
+                                                double *__restrict__ out)
+{
+    __shared__ double tx[256 * 3];
+    __shared__ double tc[256 * 2];
+    int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool has = o < dp.nout;
+    int64_t target = has ? (sel ? sel[o] : o) : 0;
+    // what this lane holds fixed: a mode (type 1), a point (type 2) or a
+    // target frequency (type 3)
+    double fixed[3] = {0.0, 0.0, 0.0};
+    if (kind == 1) {
+        int64_t rest = target;
+        for (int a = 0; a < D; ++a) {
+            int64_t N = dp.n_modes[a];
+            int64_t ka = rest % N;
+            rest /= N;
+            fixed[a] = (double)(dp.centered ? ka - N / 2 : (ka < (N + 1) / 2 ? ka : ka - N));
+        }
+    } else if (kind == 2) {
+        for (int a = 0; a < D; ++a) fixed[a] = has ? x[target * D + a] : 0.0;
+    } else {
+        for (int a = 0; a < D; ++a) fixed[a] = has ? s[target * D + a] : 0.0;
+    }
+    double re = 0.0, im = 0.0;
+    int64_t count = kind == 2 ? dp.mode_total : dp.M;
+    for (int64_t base = 0; base < count; base += 256) {
+        int64_t i = base + threadIdx.x;
+        if (i < count) {
+            if (kind == 2) {
+                int64_t rest = i;
+                for (int a = 0; a < D; ++a) {
+                    int64_t N = dp.n_modes[a];
+                    int64_t ka = rest % N;
+                    rest /= N;
+                    tx[threadIdx.x * 3 + a] =
+                        (double)(dp.centered ? ka - N / 2 : (ka < (N + 1) / 2 ? ka : ka - N));
+                }
+            } else {
+                for (int a = 0; a < D; ++a) tx[threadIdx.x * 3 + a] = x[i * D + a];
+            }
+            tc[2 * threadIdx.x] = in[2 * i];
+            tc[2 * threadIdx.x + 1] = in[2 * i + 1];
+        }
+        __syncthreads();
+        int64_t m = count - base < 256 ? count - base : 256;
+        for (int q = 0; q < m; ++q) {
+            double ph = 0.0;
+            for (int a = 0; a < D; ++a) ph += fixed[a] * tx[q * 3 + a];
+            double sn, cs;
+            sincos(dp.sign * ph, &sn, &cs);
+            double cr = tc[2 * q], ci = tc[2 * q + 1];
+            re += cr * cs - ci * sn;
+            im += cr * sn + ci * cs;
+        }
+        __syncthreads();
+    }
+    if (has) {
+        out[2 * o] = re;
+        out[2 * o + 1] = im;
+    }
+}
+
+}  // namespace
+
+template <int D, typename T>
+bool deconvolve(const ModeParams &mp, const double *inv_phihat, const T *grid, T *modes,
+                int64_t vectors, hipStream_t stream)
+{
+    int64_t n = mp.mode_total * vectors;
+    k_deconv<D, T><<<grid_for(n, 256), 256, 0, stream>>>(mp, inv_phihat, grid, modes, vectors);
+    return hip_ok(hipGetLastError(), "deconvolution launch");
+}
+
+template <int D, typename T>
+bool predeconvolve(const ModeParams &mp, const double *inv_phihat, const T *modes, T *grid,
+                   int64_t vectors, hipStream_t stream)
+{
+    int64_t n = mp.fine_total * vectors;
+    k_predeconv<D, T><<<grid_for(n, 256), 256, 0, stream>>>(mp, inv_phihat, modes, grid, vectors);
+    return hip_ok(hipGetLastError(), "predeconvolution launch");
+}
+
+template <typename T>
+bool fill_zero(T *p, int64_t n, hipStream_t stream)
+{
+    if (n <= 0) return true;
+    k_fill<T><<<grid_for(n, 256), 256, 0, stream>>>(p, n);
+    return hip_ok(hipGetLastError(), "fill launch");
+}
+
+template <int D, typename T>
+bool type3_source(const Type3Params &tp, int64_t M, int64_t vectors, const T *x, const T *c, T *xr,
+                  T *cp, hipStream_t stream)
+{
+    if (M == 0) return true;
+    k_type3_source<D, T><<<grid_for(M * vectors, 256), 256, 0, stream>>>(tp, M, vectors, x, c, xr,
+                                                                          cp);
+    return hip_ok(hipGetLastError(), "type-3 source launch");
+}
+
+template <int D, typename T>
+bool type3_target(const Type3Params &tp, int64_t N, const T *s, T *sr, double *factor,
+                  hipStream_t stream)
+{
+    if (N == 0) return true;
+    k_type3_target<D, T><<<grid_for(N, 256), 256, 0, stream>>>(tp, N, s, sr, factor);
+    return hip_ok(hipGetLastError(), "type-3 target launch");
+}
+
+template <typename T>
+bool type3_final(int64_t N, int64_t vectors, const T *interp, const double *factor, T *out,
+                 hipStream_t stream)
+{
+    if (N == 0) return true;
+    k_type3_final<T><<<grid_for(N * vectors, 256), 256, 0, stream>>>(N, vectors, interp, factor,
+                                                                     out);
+    return hip_ok(hipGetLastError(), "type-3 final launch");
+}
+
+bool direct_sum(int kind, int d, const DirectParams &dp, const double *x, const double *in,
+                const int64_t *sel, const double *s, double *out, hipStream_t stream)
+{
+    unsigned g = grid_for(dp.nout, 256);
+    switch (d) {
+    case 1: k_direct<1><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
+    case 2: k_direct<2><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
+    case 3: k_direct<3><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
+    default: set_error("direct sums take 1 to 3 dimensions"); return false;
+    }
+    return hip_ok(hipGetLastError(), "direct-sum launch");
+}
+
+#define MNUFFT_MODES_INSTANTIATE(D, T)                                                            \
+    template bool deconvolve<D, T>(const ModeParams &, const double *, const T *, T *, int64_t,    \
+                                   hipStream_t);                                                  \
+    template bool predeconvolve<D, T>(const ModeParams &, const double *, const T *, T *, int64_t,  \
+                                      hipStream_t);                                               \
+    template bool type3_source<D, T>(const Type3Params &, int64_t, int64_t, const T *, const T *,  \
+                                      T *, T *, hipStream_t);                                     \
+    template bool type3_target<D, T>(const Type3Params &, int64_t, const T *, T *, double *,       \
+                                      hipStream_t);
+MNUFFT_MODES_INSTANTIATE(1, float)
+MNUFFT_MODES_INSTANTIATE(2, float)
+MNUFFT_MODES_INSTANTIATE(3, float)
+MNUFFT_MODES_INSTANTIATE(1, double)
+MNUFFT_MODES_INSTANTIATE(2, double)
+MNUFFT_MODES_INSTANTIATE(3, double)
+template bool fill_zero<float>(float *, int64_t, hipStream_t);
+template bool fill_zero<double>(double *, int64_t, hipStream_t);
+template bool fill_zero<uint32_t>(uint32_t *, int64_t, hipStream_t);
+template bool type3_final<float>(int64_t, int64_t, const float *, const double *, float *,
+                                 hipStream_t);
+template bool type3_final<double>(int64_t, int64_t, const double *, const double *, double *,
+                                  hipStream_t);
+
+}  // namespace mnufft

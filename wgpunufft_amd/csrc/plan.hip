@@ -1,0 +1,590 @@
+// Device-resident NUFFT plans: the MI355X counterpart of the reference's
+// NufftPlan (src/plan.rs:403-1229) and NufftType3Plan (src/type3.rs:346-539).
+//
+// A plan owns its scratch (bins, prepared records, fine grid, rocFFT work
+// area) and enqueues every stage on the caller's HIP stream; executions of
+// one plan must stay in stream order (src/plan.rs:452-456).
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+
+#include "bins.hpp"
+#include "interp.hpp"
+#include "modes.hpp"
+#include "plan.hpp"
+#include "spread.hpp"
+
+namespace mnufft {
+
+namespace {
+
+std::once_flag g_rocfft_once;
+bool g_rocfft_ok = false;
+
+bool rocfft_ready()
+{
+    std::call_once(g_rocfft_once, [] { g_rocfft_ok = rocfft_setup() == rocfft_status_success; });
+    if (!g_rocfft_ok) set_error("rocFFT setup failed");
+    return g_rocfft_ok;
+}
+
+int64_t pow2_ceil(int64_t v)
+{
+    int64_t p = 1;
+    while (p < v) p *= 2;
+    return p;
+}
+
+int log2i(int64_t v)
+{
+    int s = 0;
+    while (((int64_t)1 << s) < v) ++s;
+    return s;
+}
+
+}  // namespace
+
+SpreadParams make_spread_params(int d, const int64_t *fine, int width)
+{
+    SpreadParams sp{};
+    // cells per lane along axis 0: 8 in 3D, 4 in 2D, 4 in 1D; never more
+    // than n0 + 1 - w, so a run sees every support offset at most once
+    int run = d == 3 ? 8 : 4;
+    while (run > 2 && run > fine[0] + 1 - width) run /= 2;
+    int64_t lanes_other = 1;
+    int64_t block[kMaxDims] = {1, 1, 1};
+    if (d == 3) {
+        block[1] = 8;
+        block[2] = 8;
+    } else if (d == 2) {
+        block[1] = 16;
+    }
+    for (int a = 1; a < d; ++a) lanes_other *= block[a];
+    int groups0 = (int)(64 / lanes_other);
+    block[0] = (int64_t)run * groups0;
+    sp.run = run;
+    sp.groups0 = groups0;
+    sp.block_count = 1;
+    for (int a = 0; a < kMaxDims; ++a) {
+        sp.block[a] = a < d ? block[a] : 1;
+        sp.nblocks[a] = a < d ? (fine[a] + block[a] - 1) / block[a] : 1;
+        sp.block_count *= sp.nblocks[a];
+    }
+    return sp;
+}
+
+FftPlan::~FftPlan()
+{
+    if (info) rocfft_execution_info_destroy((rocfft_execution_info)info);
+    if (plan) rocfft_plan_destroy((rocfft_plan)plan);
+    if (work) (void)hipFree(work);
+}
+
+Plan::~Plan()
+{
+    for (void *p : {(void *)d_inv_phihat, grid, (void *)rec_start, rec_off, c_sorted, t3_xr,
+                    t3_cp, t3_sr, t3_factor, t3_interp})
+        if (p) (void)hipFree(p);
+    for (auto &e : events)
+        if (e) (void)hipEventDestroy(e);
+}
+
+// ----------------------------------------------------------------------
+// Plan creation.
+// ----------------------------------------------------------------------
+bool Plan::init_common()
+{
+    if (!es_kernel_for_tolerance(eps, sigma, &kernel)) return false;
+    mode_total = 1;
+    fine_total = 1;
+    for (int a = 0; a < dim; ++a) {
+        if (!fine_given) {
+            fine[a] = select_fine_grid_size(n_modes[a], sigma, kernel.width);
+            if (fine[a] < 0) {
+                set_error("no supported fine-grid length for axis " + std::to_string(a));
+                return false;
+            }
+        }
+        mode_total *= n_modes[a];
+        fine_total *= fine[a];
+    }
+    std::vector<std::vector<double>> coeffs;
+    std::vector<double> table;
+    for (int a = 0; a < dim; ++a) {
+        coeffs.push_back(centered_fourier_coefficients(kernel, fine[a]));
+        phihat_off[a] = (int64_t)table.size();
+        for (double v : coeffs.back()) table.push_back(1.0 / v);
+    }
+    if (!fine_given &&
+        !validate_f32_rounding(std::vector<int64_t>(n_modes, n_modes + dim), eps, sigma, precision,
+                               coeffs))
+        return false;
+    if (!hip_ok(hipMalloc(&d_inv_phihat, sizeof(double) * table.size()), "amplitude table"))
+        return false;
+    if (!hip_ok(hipMemcpy(d_inv_phihat, table.data(), sizeof(double) * table.size(),
+                          hipMemcpyHostToDevice),
+                "amplitude upload"))
+        return false;
+    layout = build_spread_layout(dim, fine, kernel.width,
+                                 precision == Precision::F32 ? Precision::F32 : Precision::F64);
+    sp = make_spread_params(dim, fine, kernel.width);
+    for (auto &e : events)
+        if (!hip_ok(hipEventCreate(&e), "event")) return false;
+    return true;
+}
+
+GridParams Plan::grid_params(const int64_t *side) const
+{
+    GridParams g{};
+    g.dims = dim;
+    g.width = kernel.width;
+    g.half_width = kernel.half_width();
+    g.beta = kernel.beta;
+    g.nbins = 1;
+    g.fine_total = fine_total;
+    for (int a = 0; a < kMaxDims; ++a) {
+        AxisParams &x = g.ax[a];
+        if (a >= dim) {
+            x.n = 1;
+            x.bins = 1;
+            x.stride = g.nbins;
+            continue;
+        }
+        x.n = fine[a];
+        x.shift = log2i(side[a]);
+        x.bins = (fine[a] + side[a] - 1) / side[a];
+        x.stride = g.nbins;
+        g.nbins *= x.bins;
+        double s = (double)fine[a] / 6.283185307179586;
+        x.scale_hi = (float)s;
+        x.scale_lo = (float)(s - (double)x.scale_hi);
+        x.scale = s;
+        x.origin = (double)(fine[a] / 2);
+    }
+    return g;
+}
+
+GridParams Plan::type1_geometry(int64_t M, int *chosen) const
+{
+    int gi = layout.choose_geometry(M);
+    if (chosen) *chosen = gi;
+    return grid_params(layout.geometries[gi].side);
+}
+
+GridParams Plan::type2_geometry() const
+{
+    int64_t side[kMaxDims] = {1, 1, 1};
+    for (int a = 0; a < dim; ++a) {
+        int64_t s = dim == 3 ? 4 : dim == 2 ? 8 : 32;
+        side[a] = std::min<int64_t>(s, pow2_ceil(fine[a]));
+    }
+    return grid_params(side);
+}
+
+ModeParams Plan::mode_params() const
+{
+    ModeParams mp{};
+    for (int a = 0; a < kMaxDims; ++a) {
+        mp.n_modes[a] = a < dim ? n_modes[a] : 1;
+        mp.fine[a] = a < dim ? fine[a] : 1;
+        mp.phihat_off[a] = a < dim ? phihat_off[a] : 0;
+    }
+    mp.mode_total = mode_total;
+    mp.fine_total = fine_total;
+    mp.centered = mode_order == ModeOrder::Centered;
+    return mp;
+}
+
+size_t Plan::real_bytes() const { return precision == Precision::F32 ? 4 : 8; }
+
+bool Plan::ensure_grid(int64_t vectors)
+{
+    size_t need = (size_t)vectors * (size_t)fine_total * 2 * real_bytes();
+    if (need <= grid_bytes) return true;
+    if (grid) (void)hipFree(grid);
+    grid = nullptr;
+    grid_bytes = 0;
+    if (!hip_ok(hipMalloc(&grid, need), "fine grid")) return false;
+    grid_bytes = need;
+    return true;
+}
+
+bool Plan::ensure_records(int64_t M)
+{
+    if (M <= rec_cap) return true;
+    if (rec_start) (void)hipFree(rec_start);
+    if (rec_off) (void)hipFree(rec_off);
+    rec_start = nullptr;
+    rec_off = nullptr;
+    int64_t cap = M + M / 4 + 64;
+    if (!hip_ok(hipMalloc(&rec_start, sizeof(int4) * cap), "prepared starts")) return false;
+    if (!hip_ok(hipMalloc(&rec_off, 4 * real_bytes() * cap), "prepared offsets")) return false;
+    rec_cap = cap;
+    return true;
+}
+
+bool Plan::ensure_strengths(int64_t M, int64_t vectors)
+{
+    size_t need = (size_t)std::max<int64_t>(M, 1) * vectors * 2 * real_bytes();
+    if (need <= c_sorted_bytes) return true;
+    if (c_sorted) (void)hipFree(c_sorted);
+    c_sorted = nullptr;
+    if (!hip_ok(hipMalloc(&c_sorted, need), "sorted strengths")) return false;
+    c_sorted_bytes = need;
+    return true;
+}
+
+FftPlan *Plan::fft_for(int64_t vectors)
+{
+    auto it = ffts.find(vectors);
+    if (it != ffts.end()) return it->second.get();
+    if (!rocfft_ready()) return nullptr;
+    auto fp = std::make_unique<FftPlan>();
+    size_t lengths[kMaxDims];
+    for (int a = 0; a < dim; ++a) lengths[a] = (size_t)fine[a];  // fastest first
+    rocfft_transform_type tt = isign >= 0 ? rocfft_transform_type_complex_inverse
+                                          : rocfft_transform_type_complex_forward;
+    rocfft_precision pr =
+        precision == Precision::F32 ? rocfft_precision_single : rocfft_precision_double;
+    rocfft_plan p = nullptr;
+    if (rocfft_plan_create(&p, rocfft_placement_inplace, tt, pr, (size_t)dim, lengths,
+                           (size_t)vectors, nullptr) != rocfft_status_success) {
+        set_error("rocFFT plan creation failed");
+        return nullptr;
+    }
+    fp->plan = p;
+    size_t ws = 0;
+    rocfft_plan_get_work_buffer_size(p, &ws);
+    rocfft_execution_info info = nullptr;
+    rocfft_execution_info_create(&info);
+    fp->info = info;
+    if (ws) {
+        if (!hip_ok(hipMalloc(&fp->work, ws), "rocFFT work buffer")) return nullptr;
+        rocfft_execution_info_set_work_buffer(info, fp->work, ws);
+    }
+    FftPlan *raw = fp.get();
+    ffts[vectors] = std::move(fp);
+    return raw;
+}
+
+bool Plan::fft(int64_t vectors, void *data, hipStream_t stream)
+{
+    FftPlan *fp = fft_for(vectors);
+    if (!fp) return false;
+    rocfft_execution_info info = (rocfft_execution_info)fp->info;
+    rocfft_execution_info_set_stream(info, stream);
+    void *buf[1] = {data};
+    if (rocfft_execute((rocfft_plan)fp->plan, buf, nullptr, info) != rocfft_status_success) {
+        set_error("rocFFT execution failed");
+        return false;
+    }
+    return true;
+}
+
+void Plan::mark(int stage, bool end, hipStream_t stream)
+{
+    if (!timing) return;
+    (void)hipEventRecord(events[2 * stage + (end ? 1 : 0)], stream);
+    stage_used[stage] = true;
+}
+
+void Plan::reset_marks()
+{
+    for (auto &u : stage_used) u = false;
+}
+
+// ----------------------------------------------------------------------
+// Type 1 and type 2.
+// ----------------------------------------------------------------------
+template <int D, typename T>
+bool Plan::set_points_t(int64_t M, const T *pts, hipStream_t stream)
+{
+    reset_marks();
+    points = pts;
+    point_count = M;
+    points_set = false;
+    if (type == 1) {
+        gp = type1_geometry(M, &geometry_index);
+        mark(kStageSort, false, stream);
+        if (!sorter.sort<D, T>(M, pts, gp, BinOrder::Canonical, stream)) return false;
+        mark(kStageSort, true, stream);
+        if (!ensure_records(M)) return false;
+        mark(kStagePrepare, false, stream);
+        if (!sorter.prepare<D, T>(M, pts, gp, BinOrder::Canonical, rec_start, (T *)rec_off,
+                                  nullptr, stream))
+            return false;
+        mark(kStagePrepare, true, stream);
+    } else {
+        gp = type2_geometry();
+        mark(kStageSort, false, stream);
+        if (!sorter.sort<D, T>(M, pts, gp, BinOrder::Grouped, stream)) return false;
+        mark(kStageSort, true, stream);
+    }
+    points_set = true;
+    return true;
+}
+
+template <int D, typename T>
+bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bool fresh_marks)
+{
+    if (fresh_marks) reset_marks();
+    const int64_t M = point_count;
+    if (!ensure_grid(active)) return false;
+    ModeParams mp = mode_params();
+    if (type == 1) {
+        if (M == 0) return fill_zero<T>(out, 2 * mode_total * active, stream);
+        if (!ensure_strengths(M, active)) return false;
+        mark(kStageSpread, false, stream);
+        if (!gather_strengths<T>(M, active, rec_start, in, (T *)c_sorted, stream)) return false;
+        if (!spreader.spread<D, T>(gp, sp, sorter.offsets(), rec_start, (const T *)rec_off,
+                                   (const T *)c_sorted, M, active, (T *)grid, stream))
+            return false;
+        mark(kStageSpread, true, stream);
+        if (spread_only) return true;
+        mark(kStageFft, false, stream);
+        if (!fft(active, grid, stream)) return false;
+        mark(kStageFft, true, stream);
+        mark(kStageDeconv, false, stream);
+        if (!deconvolve<D, T>(mp, d_inv_phihat, (const T *)grid, out, active, stream))
+            return false;
+        mark(kStageDeconv, true, stream);
+    } else {
+        if (M == 0) return true;
+        mark(kStageDeconv, false, stream);
+        if (!predeconvolve<D, T>(mp, d_inv_phihat, in, (T *)grid, active, stream)) return false;
+        mark(kStageDeconv, true, stream);
+        mark(kStageFft, false, stream);
+        if (!fft(active, grid, stream)) return false;
+        mark(kStageFft, true, stream);
+        mark(kStageInterp, false, stream);
+        if (!interpolate<D, T>(M, active, sorter.sorted(), (const T *)points, gp,
+                               (const T *)grid, out, stream))
+            return false;
+        mark(kStageInterp, true, stream);
+    }
+    return true;
+}
+
+template <int D, typename T>
+bool Plan::bin_sort_t(int64_t M, const T *pts, uint32_t *sorted, uint32_t *offsets,
+                      hipStream_t stream)
+{
+    GridParams g = type1_geometry(M, nullptr);
+    reset_marks();
+    mark(kStageSort, false, stream);
+    if (!sorter.sort<D, T>(M, pts, g, BinOrder::Index, stream)) return false;
+    if (!ensure_records(M)) return false;
+    if (!sorter.prepare<D, T>(M, pts, g, BinOrder::Index, rec_start, (T *)rec_off, sorted, stream))
+        return false;
+    mark(kStageSort, true, stream);
+    if (offsets &&
+        !hip_ok(hipMemcpyAsync(offsets, sorter.offsets(), sizeof(uint32_t) * (g.nbins + 1),
+                               hipMemcpyDeviceToDevice, stream),
+                "offsets copy"))
+        return false;
+    return true;
+}
+
+// ----------------------------------------------------------------------
+// Type 3: rescale + pre-phase, raw spread onto the outer grid, target
+// rescale + correction, inner type 2 on the outer grid, final multiply
+// (src/gpu_type3.rs:267-430).
+// ----------------------------------------------------------------------
+bool Plan::init_type3(const double *src_bounds, const double *tgt_bounds)
+{
+    if (!es_kernel_for_tolerance(eps, sigma, &kernel)) return false;
+    std::memset(&t3, 0, sizeof(t3));
+    t3.sign = isign >= 0 ? 1.0 : -1.0;
+    int64_t outer_n[kMaxDims] = {1, 1, 1};
+    for (int a = 0; a < dim; ++a) {
+        double slo = src_bounds[2 * a], shi = src_bounds[2 * a + 1];
+        double tlo = tgt_bounds[2 * a], thi = tgt_bounds[2 * a + 1];
+        if (!(std::isfinite(slo) && std::isfinite(shi) && std::isfinite(tlo) &&
+              std::isfinite(thi)) ||
+            slo > shi || tlo > thi) {
+            set_error("type-3 bounds must be finite with lower <= upper");
+            return false;
+        }
+        double X = (shi - slo) * 0.5, C = (shi + slo) * 0.5;
+        if (std::fabs(C) < 0.1 * X) {
+            X += std::fabs(C);
+            C = 0.0;
+        }
+        double S = (thi - tlo) * 0.5, Dc = (thi + tlo) * 0.5;
+        if (std::fabs(Dc) < 0.1 * S) {
+            S += std::fabs(Dc);
+            Dc = 0.0;
+        }
+        double x = X, s = S;
+        if (x == 0.0) {
+            if (s == 0.0) {
+                x = 1.0;
+                s = 1.0;
+            } else {
+                x = 1.0 / s;
+            }
+        } else {
+            s = std::max(s, 1.0 / x);
+        }
+        double requested = 2.0 * sigma * s * x / M_PI + (double)(kernel.width + 1);
+        int64_t minimum = std::max<int64_t>((int64_t)requested, 2 * (int64_t)kernel.width);
+        int64_t nf = next_even_235(minimum);
+        double h = 2.0 * M_PI / (double)nf;
+        double gamma = (double)nf / (2.0 * sigma * s);
+        outer_n[a] = nf;
+        t3.src_center[a] = C;
+        t3.inv_gamma[a] = 1.0 / gamma;
+        t3.tgt_center[a] = Dc;
+        t3.tgt_scale[a] = h * gamma;
+    }
+    int pn = kernel.width + 2;
+    std::vector<double> nodes(2 * pn), weights(2 * pn);
+    gauss_legendre(2 * pn, nodes.data(), weights.data());
+    double hw = kernel.half_width();
+    t3.nquad = pn;
+    for (int i = 0; i < pn; ++i) {
+        double d = hw * nodes[pn + i];
+        t3.quad_x[i] = d;
+        t3.quad_w[i] = 2.0 * hw * weights[pn + i] * kernel.evaluate(d);
+    }
+    // Raw spread onto the outer grid: a type-1 plan whose fine grid is the
+    // outer grid itself.
+    outer = std::make_unique<Plan>();
+    outer->type = 1;
+    outer->dim = dim;
+    outer->isign = isign;
+    outer->batch = batch;
+    outer->eps = eps;
+    outer->sigma = sigma;
+    outer->precision = precision;
+    outer->mode_order = ModeOrder::Centered;
+    outer->fine_given = true;
+    outer->spread_only = true;
+    outer->timing = false;
+    for (int a = 0; a < dim; ++a) {
+        outer->n_modes[a] = outer_n[a] / 2;
+        outer->fine[a] = outer_n[a];
+    }
+    if (!outer->init_common()) return false;
+    inner = std::make_unique<Plan>();
+    inner->type = 2;
+    inner->dim = dim;
+    inner->isign = isign;
+    inner->batch = batch;
+    inner->eps = eps;
+    inner->sigma = sigma;
+    inner->precision = precision;
+    inner->mode_order = ModeOrder::Centered;
+    inner->timing = false;
+    for (int a = 0; a < dim; ++a) inner->n_modes[a] = outer_n[a];
+    if (!inner->init_common()) return false;
+    for (auto &e : events)
+        if (!hip_ok(hipEventCreate(&e), "event")) return false;
+    for (int a = 0; a < dim; ++a) type3_outer[a] = outer_n[a];
+    return true;
+}
+
+template <int D, typename T>
+bool Plan::execute_type3_t(int64_t active, int64_t M, const T *x, const T *c, int64_t N,
+                           const T *s, T *out, hipStream_t stream)
+{
+    reset_marks();
+    if (N == 0) return true;
+    size_t rb = real_bytes();
+    auto grow = [&](void **p, size_t *cap, size_t need, const char *what) {
+        if (need <= *cap) return true;
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+        if (!hip_ok(hipMalloc(p, need), what)) return false;
+        *cap = need;
+        return true;
+    };
+    if (!grow(&t3_xr, &t3_xr_bytes, std::max<int64_t>(M, 1) * D * rb, "type-3 sources") ||
+        !grow(&t3_cp, &t3_cp_bytes, std::max<int64_t>(M, 1) * active * 2 * rb, "type-3 strengths") ||
+        !grow(&t3_sr, &t3_sr_bytes, N * D * rb, "type-3 targets") ||
+        !grow(&t3_factor, &t3_factor_bytes, N * 2 * sizeof(double), "type-3 factors") ||
+        !grow(&t3_interp, &t3_interp_bytes, N * active * 2 * rb, "type-3 interpolated"))
+        return false;
+    mark(kStageType3, false, stream);
+    if (!type3_source<D, T>(t3, M, active, x, c, (T *)t3_xr, (T *)t3_cp, stream)) return false;
+    if (!outer->set_points_t<D, T>(M, (const T *)t3_xr, stream)) return false;
+    if (M == 0) {
+        if (!outer->ensure_grid(active)) return false;
+        if (!fill_zero<T>((T *)outer->grid, 2 * outer->fine_total * active, stream)) return false;
+    } else if (!outer->execute_t<D, T>(active, (const T *)t3_cp, nullptr, stream, false)) {
+        return false;
+    }
+    if (!type3_target<D, T>(t3, N, s, (T *)t3_sr, (double *)t3_factor, stream)) return false;
+    if (!inner->set_points_t<D, T>(N, (const T *)t3_sr, stream)) return false;
+    if (!inner->execute_t<D, T>(active, (const T *)outer->grid, (T *)t3_interp, stream, false))
+        return false;
+    if (!type3_final<T>(N, active, (const T *)t3_interp, (const double *)t3_factor, out, stream))
+        return false;
+    mark(kStageType3, true, stream);
+    return true;
+}
+
+// ----------------------------------------------------------------------
+// Dispatch on rank and precision.
+// ----------------------------------------------------------------------
+#define MNUFFT_DISPATCH(PLAN, CALL_F, CALL_D)                                                     \
+    do {                                                                                          \
+        bool f32 = (PLAN).precision == Precision::F32;                                            \
+        switch ((PLAN).dim) {                                                                     \
+        case 1: return f32 ? CALL_F(1, float) : CALL_D(1, double);                                \
+        case 2: return f32 ? CALL_F(2, float) : CALL_D(2, double);                                \
+        case 3: return f32 ? CALL_F(3, float) : CALL_D(3, double);                                \
+        default: set_error("GPU plans take 1 to 3 dimensions"); return false;                     \
+        }                                                                                         \
+    } while (0)
+
+bool Plan::set_points(int64_t M, const void *pts, hipStream_t stream)
+{
+#define SP(D, T) set_points_t<D, T>(M, (const T *)pts, stream)
+    MNUFFT_DISPATCH(*this, SP, SP);
+#undef SP
+}
+
+bool Plan::execute(int64_t active, const void *in, void *out, hipStream_t stream)
+{
+#define EX(D, T) execute_t<D, T>(active, (const T *)in, (T *)out, stream, true)
+    MNUFFT_DISPATCH(*this, EX, EX);
+#undef EX
+}
+
+bool Plan::bin_sort(int64_t M, const void *pts, uint32_t *sorted, uint32_t *offsets,
+                    hipStream_t stream)
+{
+#define BS(D, T) bin_sort_t<D, T>(M, (const T *)pts, sorted, offsets, stream)
+    MNUFFT_DISPATCH(*this, BS, BS);
+#undef BS
+}
+
+bool Plan::execute_type3(int64_t active, int64_t M, const void *x, const void *c, int64_t N,
+                         const void *s, void *out, hipStream_t stream)
+{
+#define T3(D, T) execute_type3_t<D, T>(active, M, (const T *)x, (const T *)c, N, (const T *)s, \
+                                       (T *)out, stream)
+    MNUFFT_DISPATCH(*this, T3, T3);
+#undef T3
+}
+
+bool Plan::stage_times(double *ms, int n)
+{
+    for (int i = 0; i < n; ++i) ms[i] = -1.0;
+    for (int st = 0; st < kStageCount && st < n; ++st) {
+        if (!stage_used[st]) continue;
+        if (!hip_ok(hipEventSynchronize(events[2 * st + 1]), "stage event")) return false;
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, events[2 * st], events[2 * st + 1]) == hipSuccess) ms[st] = t;
+    }
+    return true;
+}
+
+}  // namespace mnufft

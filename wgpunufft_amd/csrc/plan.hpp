@@ -1,0 +1,114 @@
+// Device-resident plan objects behind the C ABI.
+#pragma once
+
+#include <map>
+#include <memory>
+
+#include "bins.hpp"
+#include "modes.hpp"
+#include "nufft_internal.hpp"
+#include "spread.hpp"
+
+namespace mnufft {
+
+struct FftPlan {
+    ~FftPlan();
+    void *plan = nullptr;  // rocfft_plan
+    void *info = nullptr;  // rocfft_execution_info
+    void *work = nullptr;
+};
+
+struct Plan {
+    ~Plan();
+
+    // configuration (NufftConfig, src/config.rs:64-74)
+    int type = 1;
+    int dim = 1;
+    int64_t n_modes[kMaxDims] = {1, 1, 1};
+    int isign = 1;
+    int64_t batch = 1;
+    double eps = 1e-6;
+    double sigma = 2.0;
+    Precision precision = Precision::F32;
+    ModeOrder mode_order = ModeOrder::Centered;
+    bool fine_given = false;   // fine[] set by the caller (type-3 outer grid)
+    bool spread_only = false;  // stop after spreading (type-3 outer grid)
+    bool timing = true;
+
+    // planned quantities
+    EsKernel kernel;
+    int64_t fine[kMaxDims] = {1, 1, 1};
+    int64_t fine_total = 1, mode_total = 1;
+    int64_t phihat_off[kMaxDims] = {0, 0, 0};
+    double *d_inv_phihat = nullptr;
+    SpreadLayout layout;
+    SpreadParams sp{};
+
+    // points
+    const void *points = nullptr;
+    int64_t point_count = 0;
+    bool points_set = false;
+    GridParams gp{};
+    int geometry_index = 0;
+
+    // scratch
+    BinSorter sorter;
+    Spreader spreader;
+    void *grid = nullptr;
+    size_t grid_bytes = 0;
+    int4 *rec_start = nullptr;
+    void *rec_off = nullptr;
+    int64_t rec_cap = 0;
+    void *c_sorted = nullptr;
+    size_t c_sorted_bytes = 0;
+    std::map<int64_t, std::unique_ptr<FftPlan>> ffts;
+
+    // type 3
+    Type3Params t3{};
+    int64_t type3_outer[kMaxDims] = {1, 1, 1};
+    std::unique_ptr<Plan> outer, inner;
+    void *t3_xr = nullptr, *t3_cp = nullptr, *t3_sr = nullptr, *t3_factor = nullptr,
+         *t3_interp = nullptr;
+    size_t t3_xr_bytes = 0, t3_cp_bytes = 0, t3_sr_bytes = 0, t3_factor_bytes = 0,
+           t3_interp_bytes = 0;
+
+    // stage timing
+    hipEvent_t events[2 * kStageCount] = {};
+    bool stage_used[kStageCount] = {};
+
+    bool init_common();
+    bool init_type3(const double *src_bounds, const double *tgt_bounds);
+    GridParams grid_params(const int64_t *side) const;
+    GridParams type1_geometry(int64_t M, int *chosen) const;
+    GridParams type2_geometry() const;
+    ModeParams mode_params() const;
+    size_t real_bytes() const;
+    bool ensure_grid(int64_t vectors);
+    bool ensure_records(int64_t M);
+    bool ensure_strengths(int64_t M, int64_t vectors);
+    FftPlan *fft_for(int64_t vectors);
+    bool fft(int64_t vectors, void *data, hipStream_t stream);
+    void mark(int stage, bool end, hipStream_t stream);
+    void reset_marks();
+
+    bool set_points(int64_t M, const void *pts, hipStream_t stream);
+    bool execute(int64_t active, const void *in, void *out, hipStream_t stream);
+    bool bin_sort(int64_t M, const void *pts, uint32_t *sorted, uint32_t *offsets,
+                  hipStream_t stream);
+    bool execute_type3(int64_t active, int64_t M, const void *x, const void *c, int64_t N,
+                       const void *s, void *out, hipStream_t stream);
+    bool stage_times(double *ms, int n);
+
+    template <int D, typename T>
+    bool set_points_t(int64_t M, const T *pts, hipStream_t stream);
+    template <int D, typename T>
+    bool execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bool fresh_marks);
+    template <int D, typename T>
+    bool bin_sort_t(int64_t M, const T *pts, uint32_t *sorted, uint32_t *offsets,
+                    hipStream_t stream);
+    template <int D, typename T>
+    bool execute_type3_t(int64_t active, int64_t M, const T *x, const T *c, int64_t N,
+                         const T *s, T *out, hipStream_t stream);
+};
+
+}  // namespace mnufft

@@ -1,0 +1,632 @@
+// Deterministic output-stationary type-1 spreading on gfx950.
+//
+// Reference behaviour (src/gpu_nd_spread.rs:1-36, shader 1336-1695,
+// classify 1704-1771, plan 1781-1905, reduce 1914-2056): every block of
+// fine-grid cells is owned by one workgroup whose lanes keep their cells'
+// sums in registers; the workgroup walks the bins that can reach the block in
+// a fixed order and streams their records through LDS, so every cell sums its
+// contributions in a fixed order and is written exactly once -- no float
+// atomics.  Blocks whose reach holds more records than a light limit are
+// split into parts of consecutive records; partial sums land in a scratch
+// and are added in part order.
+//
+// MI355X design: a block is one 64-lane wavefront (lane = one run of `R`
+// cells along axis 0 and one cell along every other axis); records are
+// staged 64 at a time, compacted with a ballot so the accumulate loop only
+// sees records whose support touches the block, with each record's kernel
+// weights computed once by the lane that staged it.  Light blocks and heavy
+// parts form one flat list of work items consumed by a grid-stride loop,
+// which keeps dense clusters spread over all 256 CUs.
+#include <hip/hip_runtime.h>
+
+#include "fold.hpp"
+#include "spread.hpp"
+
+namespace mnufft {
+
+namespace {
+
+constexpr int kLanes = 64;
+constexpr int kKw = 16;  // weights per axis staged per record
+
+template <typename T>
+__device__ inline T es_weight(T d, int w, T beta)
+{
+    T s = (T)2 * d / (T)w;
+    T q = (T)1 - s * s;
+    if (!(q > (T)0)) return (T)0;
+    if constexpr (sizeof(T) == 4)
+        return expf(beta * (sqrtf(q) - 1.0f));
+    else
+        return exp(beta * (sqrt(q) - 1.0));
+}
+
+// Bin intervals of one axis that hold the support starts reaching a block
+// (starts b0-w+1 .. b0+B-1, periodic).  A reach within one bin side of the
+// whole axis takes every bin once.
+struct AxisReach {
+    int64_t lo0, len0, lo1, len1;
+    __device__ int64_t count() const { return len0 + len1; }
+    __device__ int64_t bin(int64_t k) const { return k < len0 ? lo0 + k : lo1 + (k - len0); }
+};
+
+__device__ inline AxisReach axis_reach(int64_t b0, int64_t B, int w, const AxisParams &a)
+{
+    AxisReach r{0, 0, 0, 0};
+    int64_t n = a.n;
+    int64_t L = B + w - 1 < n ? B + w - 1 : n;
+    int64_t side = (int64_t)1 << a.shift;
+    if (L + side > n) {
+        r.lo0 = 0;
+        r.len0 = a.bins;
+        return r;
+    }
+    int64_t r0 = b0 - (w - 1);
+    if (r0 < 0) r0 += n;
+    if (r0 + L <= n) {
+        r.lo0 = r0 >> a.shift;
+        r.len0 = ((r0 + L - 1) >> a.shift) - r.lo0 + 1;
+    } else {
+        r.lo0 = r0 >> a.shift;
+        r.len0 = a.bins - r.lo0;
+        r.lo1 = 0;
+        r.len1 = ((r0 + L - n - 1) >> a.shift) + 1;
+    }
+    return r;
+}
+
+template <int D>
+struct BlockReach {
+    AxisReach ax[3];
+    int64_t b0[3];
+    int64_t nseg;
+    // segment t -> record slots [first, last)
+    __device__ void segment(int64_t t, const GridParams &g, const uint32_t *offsets,
+                            int64_t *first, int64_t *last) const
+    {
+        int64_t nxi = ax[0].len1 > 0 ? 2 : 1;
+        int64_t ix = t % nxi;
+        int64_t rest = t / nxi;
+        int64_t row = 0, stride = g.ax[0].bins;
+#pragma unroll
+        for (int a = 1; a < D; ++a) {
+            int64_t cnt = ax[a].count();
+            int64_t k = rest % cnt;
+            rest /= cnt;
+            row += ax[a].bin(k) * stride;
+            stride *= g.ax[a].bins;
+        }
+        int64_t lo = ix == 0 ? ax[0].lo0 : ax[0].lo1;
+        int64_t len = ix == 0 ? ax[0].len0 : ax[0].len1;
+        *first = offsets[row + lo];
+        *last = offsets[row + lo + len];
+    }
+};
+
+template <int D>
+__device__ inline BlockReach<D> block_reach(int64_t block, const GridParams &g,
+                                            const SpreadParams &sp)
+{
+    BlockReach<D> r;
+    int64_t rest = block;
+    r.nseg = 1;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        int64_t bi = rest % sp.nblocks[a];
+        rest /= sp.nblocks[a];
+        r.b0[a] = bi * sp.block[a];
+        r.ax[a] = axis_reach(r.b0[a], sp.block[a], g.width, g.ax[a]);
+        if (a == 0)
+            r.nseg *= r.ax[0].len1 > 0 ? 2 : 1;
+        else
+            r.nseg *= r.ax[a].count();
+    }
+    return r;
+}
+
+// Records in reach of every block.
+template <int D>
+__global__ __launch_bounds__(256) void k_classify(GridParams g, SpreadParams sp,
+                                                  const uint32_t *__restrict__ offsets,
+                                                  uint32_t *__restrict__ totals)
+{
+    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= sp.block_count) return;
+    BlockReach<D> r = block_reach<D>(b, g, sp);
+    uint64_t total = 0;
+    for (int64_t t = 0; t < r.nseg; ++t) {
+        int64_t f, l;
+        r.segment(t, g, offsets, &f, &l);
+        total += (uint64_t)(l - f);
+    }
+    totals[b] = (uint32_t)(total > 0xffffffffull ? 0xffffffffull : total);
+}
+
+// One workgroup plans the work items: a light block is one item, a heavy
+// block as many parts as its records need.  ctrl[0] = items, ctrl[1] = heavy
+// parts, ctrl[2] = heavy blocks, ctrl[3] = records per part.
+__global__ __launch_bounds__(1024) void k_plan(int64_t nblocks, const uint32_t *__restrict__ totals,
+                                               uint32_t light_limit, uint32_t min_part,
+                                               int64_t part_capacity,
+                                               uint32_t *__restrict__ parts,
+                                               uint32_t *__restrict__ item_off,
+                                               uint32_t *__restrict__ heavy_base,
+                                               uint32_t *__restrict__ heavy_list,
+                                               uint32_t *__restrict__ ctrl)
+{
+    __shared__ uint64_t s_heavy_records;
+    __shared__ uint32_t s_heavy_blocks;
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_wsum[16];
+    __shared__ uint32_t s_carry[2];
+    if (threadIdx.x == 0) {
+        s_heavy_records = 0;
+        s_heavy_blocks = 0;
+        s_carry[0] = s_carry[1] = 0;
+    }
+    __syncthreads();
+    uint64_t mine = 0;
+    uint32_t mine_blocks = 0;
+    for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        uint32_t t = totals[b];
+        if (t > light_limit) {
+            mine += t;
+            ++mine_blocks;
+        }
+    }
+    atomicAdd((unsigned long long *)&s_heavy_records, (unsigned long long)mine);
+    atomicAdd(&s_heavy_blocks, mine_blocks);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t spare = part_capacity > (int64_t)s_heavy_blocks
+                             ? (uint64_t)(part_capacity - s_heavy_blocks)
+                             : 1ull;
+        uint64_t part = (s_heavy_records + spare - 1) / spare;
+        if (part < min_part) part = min_part;
+        s_part = (uint32_t)(part > 0xffffffffull ? 0xffffffffull : part);
+    }
+    __syncthreads();
+    uint32_t part = s_part;
+    // Ordered exclusive scans of items and heavy parts, 1024 blocks a round.
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int64_t base = 0; base < nblocks; base += blockDim.x) {
+        int64_t b = base + threadIdx.x;
+        uint32_t np = 0, hp = 0, hb = 0;
+        if (b < nblocks) {
+            uint32_t t = totals[b];
+            if (t > light_limit) {
+                np = (uint32_t)((t + (uint64_t)part - 1) / part);
+                hp = np;
+                hb = 1;
+            } else {
+                np = 1;
+            }
+            parts[b] = np;
+        }
+        // pack (items, heavy parts) into one 64-bit scan via two passes
+        for (int pass = 0; pass < 2; ++pass) {
+            uint32_t v = pass == 0 ? np : hp;
+            uint32_t x = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (lane == 63) s_wsum[wid] = x;
+            __syncthreads();
+            uint32_t pre = 0, all = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+                if (w < wid) pre += s_wsum[w];
+                all += s_wsum[w];
+            }
+            uint32_t ex = s_carry[pass] + pre + x - v;
+            if (b < nblocks) {
+                if (pass == 0)
+                    item_off[b] = ex;
+                else
+                    heavy_base[b] = ex;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) s_carry[pass] += all;
+            __syncthreads();
+        }
+        if (hb) {
+            uint32_t slot = atomicAdd(&ctrl[2], 1u);  // ctrl[2] cleared by the host
+            heavy_list[slot] = (uint32_t)b;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ctrl[0] = s_carry[0];
+        ctrl[1] = s_carry[1];
+        ctrl[3] = part;
+    }
+}
+
+template <typename T, int R>
+struct Staged {
+    int s[3];
+    T cre, cim;
+    T kxp[2 * R + kKw];
+    T ky[kKw];
+    T kz[kKw];
+};
+
+// One work item: records [rlo, rhi) of block b's reach, for vector v.
+// Writes the block's cells (light) or a partial slot (heavy part).
+template <int D, typename T, int R>
+__global__ __launch_bounds__(kLanes) void k_spread(GridParams g, SpreadParams sp,
+                                                   const uint32_t *__restrict__ offsets,
+                                                   const int4 *__restrict__ rec_start,
+                                                   const T *__restrict__ rec_off,
+                                                   const T *__restrict__ c_sorted, int64_t M,
+                                                   const uint32_t *__restrict__ totals,
+                                                   const uint32_t *__restrict__ parts,
+                                                   const uint32_t *__restrict__ item_off,
+                                                   const uint32_t *__restrict__ heavy_base,
+                                                   const uint32_t *__restrict__ ctrl,
+                                                   T *__restrict__ grid, T *__restrict__ partial,
+                                                   int64_t block_cells)
+{
+    __shared__ Staged<T, R> st[kLanes];
+    __shared__ int64_t seg_first[kLanes];
+    __shared__ int64_t seg_excl[kLanes + 1];
+
+    const int lane = threadIdx.x;
+    const int64_t v = blockIdx.y;
+    const int w = g.width;
+    const T beta = (T)g.beta;
+    const int64_t n_items = ctrl[0];
+    const uint32_t part_len_cap = ctrl[3];
+    const T *cv = c_sorted + 2 * v * M;
+
+    // lane -> cells: run of R cells along axis 0 and one cell per other axis
+    int lane_x = lane % sp.groups0;
+    int rest_lane = lane / sp.groups0;
+    int lane_c[3] = {0, 0, 0};
+    lane_c[0] = lane_x * R;
+#pragma unroll
+    for (int a = 1; a < D; ++a) {
+        lane_c[a] = rest_lane % (int)sp.block[a];
+        rest_lane /= (int)sp.block[a];
+    }
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        // block of this item: last b with item_off[b] <= item
+        int64_t lo = 0, hi = sp.block_count - 1;
+        while (lo < hi) {
+            int64_t mid = (lo + hi + 1) >> 1;
+            if ((int64_t)item_off[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int64_t b = lo;
+        const int64_t p = item - item_off[b];
+        const uint32_t np = parts[b];
+        const int64_t total = totals[b];
+        int64_t rlo = 0, rhi = total;
+        if (np > 1) {
+            int64_t len = part_len_cap;
+            rlo = p * len;
+            rhi = rlo + len < total ? rlo + len : total;
+        }
+        BlockReach<D> r = block_reach<D>(b, g, sp);
+        int64_t cell[3];
+#pragma unroll
+        for (int a = 0; a < D; ++a) cell[a] = r.b0[a] + lane_c[a];
+
+        T acc_re[R], acc_im[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) acc_re[i] = acc_im[i] = (T)0;
+
+        int64_t cursor = 0;
+        for (int64_t sb = 0; sb < r.nseg && cursor < rhi; sb += kLanes) {
+            int64_t t = sb + lane;
+            int64_t f = 0, l = 0;
+            if (t < r.nseg) r.segment(t, g, offsets, &f, &l);
+            int64_t cnt = l - f;
+            // inclusive wave scan of segment lengths
+            int64_t x = cnt;
+#pragma unroll
+            for (int o = 1; o < kLanes; o <<= 1) {
+                int64_t y = __shfl_up(x, o, kLanes);
+                if (lane >= o) x += y;
+            }
+            int64_t batch_total = __shfl(x, kLanes - 1, kLanes);
+            seg_first[lane] = f;
+            seg_excl[lane] = x - cnt;
+            if (lane == 0) seg_excl[kLanes] = batch_total;
+            __syncthreads();
+            int64_t klo = rlo > cursor ? rlo - cursor : 0;
+            int64_t khi = rhi - cursor < batch_total ? rhi - cursor : batch_total;
+            for (int64_t k0 = klo; k0 < khi; k0 += kLanes) {
+                int64_t k = k0 + lane;
+                bool touch = false;
+                int64_t slot = 0;
+                if (k < khi) {
+                    int s_lo = 0, s_hi = kLanes - 1;
+                    while (s_lo < s_hi) {
+                        int mid = (s_lo + s_hi + 1) >> 1;
+                        if (seg_excl[mid] <= k)
+                            s_lo = mid;
+                        else
+                            s_hi = mid - 1;
+                    }
+                    slot = seg_first[s_lo] + (k - seg_excl[s_lo]);
+                    int4 s4 = rec_start[slot];
+                    int ss[3] = {s4.x, s4.y, s4.z};
+                    touch = true;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        int64_t n = g.ax[a].n;
+                        int64_t e = r.b0[a] - ss[a];
+                        if (e < 0) e += n;
+                        touch = touch && (e < w || e > n - sp.block[a]);
+                    }
+                }
+                // compact the touching records into LDS
+                uint64_t mask = __ballot(touch);
+                int pos = __popcll(mask & ((1ull << lane) - 1ull));
+                int nstaged = __popcll(mask);
+                if (touch) {
+                    Staged<T, R> &S = st[pos];
+                    int4 s4 = rec_start[slot];
+                    S.s[0] = s4.x;
+                    S.s[1] = s4.y;
+                    S.s[2] = s4.z;
+                    S.cre = cv[2 * slot];
+                    S.cim = cv[2 * slot + 1];
+                    T ox = rec_off[4 * slot + 0];
+#pragma unroll
+                    for (int j = 0; j < 2 * R + kKw; ++j) {
+                        int tt = j - (R - 1);
+                        S.kxp[j] = (tt >= 0 && tt < w) ? es_weight<T>((T)tt + ox, w, beta) : (T)0;
+                    }
+                    if (D > 1) {
+                        T oy = rec_off[4 * slot + 1];
+                        for (int j = 0; j < w; ++j) S.ky[j] = es_weight<T>((T)j + oy, w, beta);
+                    }
+                    if (D > 2) {
+                        T oz = rec_off[4 * slot + 2];
+                        for (int j = 0; j < w; ++j) S.kz[j] = es_weight<T>((T)j + oz, w, beta);
+                    }
+                }
+                __syncthreads();
+                for (int q = 0; q < nstaged; ++q) {
+                    const Staged<T, R> &S = st[q];
+                    T wyz = (T)1;
+                    bool on = true;
+                    if (D > 1) {
+                        int64_t dy = cell[1] - S.s[1];
+                        if (dy < 0) dy += g.ax[1].n;
+                        on = dy < w;
+                        wyz = on ? S.ky[dy] : (T)0;
+                    }
+                    if (D > 2) {
+                        int64_t dz = cell[2] - S.s[2];
+                        if (dz < 0) dz += g.ax[2].n;
+                        on = on && dz < w;
+                        wyz = on ? wyz * S.kz[dz] : (T)0;
+                    }
+                    if (!on) continue;
+                    int64_t nx = g.ax[0].n;
+                    int64_t e = cell[0] - S.s[0];
+                    if (e < 0) e += nx;
+                    int base;
+                    if (e < w)
+                        base = (int)e + R - 1;
+                    else if (e > nx - R)
+                        base = (int)(e - nx) + R - 1;
+                    else
+                        continue;
+                    T vre = S.cre * wyz, vim = S.cim * wyz;
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        T wx = S.kxp[base + i];
+                        acc_re[i] += wx * vre;
+                        acc_im[i] += wx * vim;
+                    }
+                }
+                __syncthreads();
+            }
+            cursor += batch_total;
+            __syncthreads();
+        }
+
+        // write the lane's cells
+        if (np == 1) {
+            bool inside = true;
+            int64_t lin = 0, stride = 1;
+#pragma unroll
+            for (int a = 1; a < D; ++a) {
+                inside = inside && cell[a] < g.ax[a].n;
+                stride *= g.ax[a - 1].n;
+                lin += cell[a] * stride;
+            }
+            if (inside) {
+                T *out = grid + 2 * (v * g.fine_total + lin);
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    int64_t xc = cell[0] + i;
+                    if (xc < g.ax[0].n) {
+                        out[2 * xc] = acc_re[i];
+                        out[2 * xc + 1] = acc_im[i];
+                    }
+                }
+            }
+        } else {
+            int64_t slot = (int64_t)heavy_base[b] + p;
+            T *out = partial + 2 * ((slot * gridDim.y + v) * block_cells + (int64_t)lane * R);
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                out[2 * i] = acc_re[i];
+                out[2 * i + 1] = acc_im[i];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Heavy blocks: each cell adds its partial sums in part order.
+template <int D, typename T, int R>
+__global__ __launch_bounds__(256) void k_reduce(GridParams g, SpreadParams sp,
+                                                const uint32_t *__restrict__ parts,
+                                                const uint32_t *__restrict__ heavy_base,
+                                                const uint32_t *__restrict__ heavy_list,
+                                                const uint32_t *__restrict__ ctrl,
+                                                const T *__restrict__ partial, T *__restrict__ grid,
+                                                int64_t block_cells)
+{
+    const int64_t v = blockIdx.y;
+    const int64_t vectors = gridDim.y;
+    const int64_t nheavy = ctrl[2];
+    const int64_t work = nheavy * block_cells;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < work;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t hb = t / block_cells;
+        int64_t c = t - hb * block_cells;  // lane * R + i
+        int64_t b = heavy_list[hb];
+        int64_t np = parts[b];
+        int64_t base = heavy_base[b];
+        T sre = (T)0, sim = (T)0;
+        for (int64_t p = 0; p < np; ++p) {
+            const T *src = partial + 2 * (((base + p) * vectors + v) * block_cells + c);
+            sre += src[0];
+            sim += src[1];
+        }
+        // cell coordinates
+        int lane = (int)(c / R), i = (int)(c % R);
+        int lane_x = lane % sp.groups0;
+        int rl = lane / sp.groups0;
+        int64_t rest = b;
+        int64_t cell[3] = {0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            int64_t bi = rest % sp.nblocks[a];
+            rest /= sp.nblocks[a];
+            cell[a] = bi * sp.block[a];
+        }
+        cell[0] += lane_x * R + i;
+#pragma unroll
+        for (int a = 1; a < D; ++a) {
+            cell[a] += rl % (int)sp.block[a];
+            rl /= (int)sp.block[a];
+        }
+        bool inside = true;
+        int64_t lin = 0, stride = 1;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            inside = inside && cell[a] < g.ax[a].n;
+            lin += cell[a] * stride;
+            stride *= g.ax[a].n;
+        }
+        if (inside) {
+            T *out = grid + 2 * (v * g.fine_total + lin);
+            out[0] = sre;
+            out[1] = sim;
+        }
+    }
+}
+
+inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+__global__ void k_fill_ctrl(uint32_t *ctrl)
+{
+    if (threadIdx.x < 4) ctrl[threadIdx.x] = 0;
+}
+
+template <int D, typename T, int R>
+bool launch_spread(Spreader &s, const GridParams &g, const SpreadParams &sp,
+                   const uint32_t *offsets, const int4 *rec_start, const T *rec_off,
+                   const T *c_sorted, int64_t M, int64_t vectors, T *grid, hipStream_t stream)
+{
+    int64_t nb = sp.block_count;
+    int64_t cells = (int64_t)kLanes * R;
+    k_classify<D><<<grid_for(nb, 256), 256, 0, stream>>>(g, sp, offsets, s.totals);
+    k_fill_ctrl<<<1, 64, 0, stream>>>(s.ctrl);
+    k_plan<<<1, 1024, 0, stream>>>(nb, s.totals, s.light_limit, s.min_part, s.part_capacity,
+                                   s.parts, s.item_off, s.heavy_base, s.heavy_list, s.ctrl);
+    // Enough workgroups to fill the GPU several times over; each loops over
+    // items with a stride.
+    unsigned items_grid = (unsigned)std::min<int64_t>(nb + 16384, 65536);
+    dim3 grid3(items_grid, (unsigned)vectors);
+    k_spread<D, T, R><<<grid3, kLanes, 0, stream>>>(g, sp, offsets, rec_start, rec_off, c_sorted,
+                                                    M, s.totals, s.parts, s.item_off,
+                                                    s.heavy_base, s.ctrl, grid,
+                                                    (T *)s.partial, cells);
+    dim3 rgrid(2048, (unsigned)vectors);
+    k_reduce<D, T, R><<<rgrid, 256, 0, stream>>>(g, sp, s.parts, s.heavy_base, s.heavy_list,
+                                                 s.ctrl, (const T *)s.partial, grid, cells);
+    return hip_ok(hipGetLastError(), "spread launch");
+}
+
+}  // namespace
+
+Spreader::~Spreader() { release(); }
+
+void Spreader::release()
+{
+    for (void *p : {(void *)totals, (void *)parts, (void *)item_off, (void *)heavy_base,
+                    (void *)heavy_list, (void *)ctrl, partial})
+        if (p) (void)hipFree(p);
+    totals = parts = item_off = heavy_base = heavy_list = ctrl = nullptr;
+    partial = nullptr;
+    blocks_cap = 0;
+    partial_bytes = 0;
+}
+
+bool Spreader::ensure(int64_t nblocks, int64_t partial_need)
+{
+    if (nblocks > blocks_cap) {
+        for (uint32_t **p : {&totals, &parts, &item_off, &heavy_base, &heavy_list}) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+            if (!hip_ok(hipMalloc(p, sizeof(uint32_t) * (nblocks + 1)), "spread block arrays"))
+                return false;
+        }
+        if (!ctrl && !hip_ok(hipMalloc(&ctrl, sizeof(uint32_t) * 64), "spread control"))
+            return false;
+        blocks_cap = nblocks;
+    }
+    if (partial_need > partial_bytes) {
+        if (partial) (void)hipFree(partial);
+        partial = nullptr;
+        if (!hip_ok(hipMalloc(&partial, partial_need), "spread partial sums")) return false;
+        partial_bytes = partial_need;
+    }
+    return true;
+}
+
+template <int D, typename T>
+bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
+                      const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
+                      int64_t vectors, T *grid, hipStream_t stream)
+{
+    int64_t cells = (int64_t)kLanes * sp.run;
+    int64_t per_part = cells * vectors * 2 * (int64_t)sizeof(T);
+    // part slots: the scratch budget, at least one per block
+    part_capacity = std::max<int64_t>(budget_bytes / per_part, sp.block_count + 1);
+    if (!ensure(sp.block_count, part_capacity * per_part)) return false;
+    switch (sp.run) {
+    case 2: return launch_spread<D, T, 2>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M,
+                                          vectors, grid, stream);
+    case 4: return launch_spread<D, T, 4>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M,
+                                          vectors, grid, stream);
+    case 8: return launch_spread<D, T, 8>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M,
+                                          vectors, grid, stream);
+    default: set_error("unsupported spread run"); return false;
+    }
+}
+
+#define MNUFFT_SPREAD_INSTANTIATE(D, T)                                                           \
+    template bool Spreader::spread<D, T>(const GridParams &, const SpreadParams &,                 \
+                                         const uint32_t *, const int4 *, const T *, const T *,     \
+                                         int64_t, int64_t, T *, hipStream_t);
+MNUFFT_SPREAD_INSTANTIATE(1, float)
+MNUFFT_SPREAD_INSTANTIATE(2, float)
+MNUFFT_SPREAD_INSTANTIATE(3, float)
+MNUFFT_SPREAD_INSTANTIATE(1, double)
+MNUFFT_SPREAD_INSTANTIATE(2, double)
+MNUFFT_SPREAD_INSTANTIATE(3, double)
+
+}  // namespace mnufft

@@ -1,0 +1,45 @@
+// Deterministic output-stationary type-1 spreader.
+#pragma once
+
+#include "nufft_internal.hpp"
+
+namespace mnufft {
+
+// Device buffers of the spreader's work plan, reused across executions.
+struct Spreader {
+    ~Spreader();
+    void release();
+    bool ensure(int64_t nblocks, int64_t partial_need);
+
+    // Spreads the prepared records (in bin order) of `vectors` strength
+    // vectors onto `grid` (vectors * fine_total complex values).  Every cell
+    // is written exactly once.
+    template <int D, typename T>
+    bool spread(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
+                const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
+                int64_t vectors, T *grid, hipStream_t stream);
+
+    uint32_t *totals = nullptr;      // records in reach per block
+    uint32_t *parts = nullptr;       // work items per block
+    uint32_t *item_off = nullptr;    // first work item per block
+    uint32_t *heavy_base = nullptr;  // first partial slot per heavy block
+    uint32_t *heavy_list = nullptr;  // heavy blocks
+    uint32_t *ctrl = nullptr;        // items, heavy parts, heavy blocks, part size
+    void *partial = nullptr;
+    int64_t blocks_cap = 0;
+    int64_t partial_bytes = 0;
+    int64_t part_capacity = 0;
+    // Records in reach above which a block is split into parts, and the
+    // smallest part.
+    uint32_t light_limit = 4096;
+    uint32_t min_part = 1024;
+    // Scratch for partial sums of heavy parts.
+    int64_t budget_bytes = (int64_t)256 << 20;
+};
+
+// Output-stationary blocks for a fine grid: 64 lanes, `run` cells per lane
+// along axis 0 (run + width - 1 <= n0 keeps every cell's support offset
+// unique within a run).
+SpreadParams make_spread_params(int d, const int64_t *fine, int width);
+
+}  // namespace mnufft
