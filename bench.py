@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark of the hot path on one MI355X.
+
+Workload (one step): a 3D type-1 and a 3D type-2 F32 transform at tol 1e-6
+over 128^3 modes (fine grid 256^3) and M = 10^7 points, half uniform in
+[-pi, pi)^3 and half a Gaussian cluster (sigma 0.05) around (0.3, -1.1, 2.0)
+wrapped into the box, strengths / coefficients complex N(0,1).  Each
+transform includes its own point binning (encode = set_points + execute),
+the rocFFT fine-grid FFT and the (pre)deconvolution.  Inputs are resident
+in HBM before timing starts.
+
+value = M / (seconds per step): points per second through both transforms.
+The roofline object reports the dominant kernel of the path, from HIP events
+around that kernel over the timed region; cpu_baseline times the CPU port of
+the reference's CpuNufftPlan on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "NU points/s for 3D type-1/2 at tol 1e-6, MFMA vs plain GPU; % HBM/MFMA roofline"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--points", type=int, default=10_000_000)
+    ap.add_argument("--modes", type=int, default=128)
+    ap.add_argument("--eps", type=float, default=1e-6)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
+    ap.add_argument("--cpu-points", type=int, default=3_000_000,
+                    help="points of the CPU baseline sample")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the transforms would shard)", file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    import wgpunufft_amd as W
+    from wgpunufft_amd import synthetic as S
+
+    dev = torch.device("cuda:0")
+    M = args.points
+    N = args.modes
+    n_modes = [N, N, N]
+    t0 = time.time()
+    x = S.config2_points(M)
+    c = S.complex_normal(S.config_seed(2, 3), M)
+    fm = S.complex_normal(S.config_seed(2, 4), N ** 3)
+    xt = torch.from_numpy(x).to(dev)
+    ct = torch.from_numpy(c).to(dev)
+    ft = torch.from_numpy(fm).to(dev)
+    f_out = torch.empty(N ** 3, dtype=torch.complex64, device=dev)
+    c_out = torch.empty(M, dtype=torch.complex64, device=dev)
+    cfg = W.NufftConfig(n_modes, args.eps)
+    p1 = W.NufftPlan.type1_gpu(cfg)
+    p2 = W.NufftPlan.type2_gpu(cfg)
+    print(f"# data ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    def step():
+        p1.encode_gpu(xt, ct, out=f_out)
+        p2.encode_gpu(xt, ft, out=c_out)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    p1.timing_history(True)
+    p2.timing_history(True)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t_end = time.perf_counter()
+    st1 = p1.stage_times()
+    st2 = p2.stage_times()
+    p1.timing_history(False)
+    p2.timing_history(False)
+    K = args.steps
+    ms_step = (t_end - t_start) * 1e3 / K
+    avg = lambda v: None if v is None else v / K  # noqa: E731
+
+    info1 = p1.info()
+    w = info1["width"]
+    nf = int(np.prod(info1["fine"]))
+    # algorithmic work per launch (one vector)
+    spread_flops = 4.0 * w ** 3 * M
+    interp_bytes = M * (4 * 3 + 8) + 8 * nf
+    t_spread_ms = avg(st1["spread_kernel"])
+    t_interp_ms = avg(st2["interp"])
+    spread_tflops = spread_flops / (t_spread_ms * 1e-3) / 1e12
+    interp_gbs = interp_bytes / (t_interp_ms * 1e-3) / 1e9
+    roof_spread = {"kernel": "k_spread (type-1 output-stationary spread)", "bound": "mfma",
+                   "achieved": round(spread_tflops, 3), "peak": FP32_PEAK_TFLOPS,
+                   "unit": "TFLOP/s", "frac": round(spread_tflops / FP32_PEAK_TFLOPS, 4),
+                   "traffic": None, "avg_ms": round(t_spread_ms, 4),
+                   "algorithmic": f"4*w^3*M = {spread_flops:.4g} flop (w={w}, M={M})"}
+    roof_interp = {"kernel": "k_interp (type-2 interpolation)", "bound": "hbm",
+                   "achieved": round(interp_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(interp_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                   "avg_ms": round(t_interp_ms, 4),
+                   "algorithmic": f"M*(4d+8)+8*N_f = {interp_bytes:.4g} B"}
+    dominant, other = (roof_spread, roof_interp) if t_spread_ms >= t_interp_ms else \
+        (roof_interp, roof_spread)
+
+    stages = {"type1": {k: (None if v is None else round(v / K, 4)) for k, v in st1.items()},
+              "type2": {k: (None if v is None else round(v / K, 4)) for k, v in st2.items()}}
+    t1_ms = sum(v for k, v in stages["type1"].items() if v is not None and k != "spread_kernel")
+    t2_ms = sum(v for v in stages["type2"].values() if v is not None)
+
+    result = {
+        "metric": METRIC,
+        "value": M / (ms_step * 1e-3),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64)",
+        "config": {"workload": f"3D type-1 + type-2 F32, {N}^3 modes, sigma 2, tol {args.eps:g}, "
+                               f"M={M} (half uniform, half Gaussian sigma 0.05 cluster)",
+                   "n_modes": n_modes, "fine": info1["fine"], "width": w,
+                   "points": M, "bin_geometry_t1": info1["bin_side"],
+                   "spreader": "plain FP32 output-stationary"},
+        "roofline": dominant,
+        "roofline_other": other,
+        "stages_ms": stages,
+        "type1_points_per_s": M / (t1_ms * 1e-3) if t1_ms else None,
+        "type2_points_per_s": M / (t2_ms * 1e-3) if t2_ms else None,
+    }
+
+    if not args.no_check:
+        result["error"] = accuracy(W, S, np, torch, dev, x, c, fm, n_modes, f_out, c_out)
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(args, np, S, n_modes)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def accuracy(W, S, np, torch, dev, x, c, fm, n_modes, f_out, c_out):
+    """Relative l2 error against FP64 direct sums on 2,000 seeded modes
+    (type 1) and 2,000 seeded points (type 2), on the GPU direct kernel."""
+    nm = int(np.prod(n_modes))
+    sel_m = np.sort(S.splitmix64(S.config_seed(2, 7), 2000) % np.uint64(nm)).astype(np.int64)
+    sel_p = np.sort(S.splitmix64(S.config_seed(2, 8), 2000) % np.uint64(len(x))).astype(np.int64)
+    xd = torch.from_numpy(x.astype(np.float64)).to(dev)
+    cd = torch.from_numpy(c.astype(np.complex128)).to(dev)
+    fd = torch.from_numpy(fm.astype(np.complex128)).to(dev)
+    r1 = W.reference_type1_f64(n_modes, xd, cd, modes=torch.from_numpy(sel_m).to(dev))
+    r2 = W.reference_type2_f64(n_modes, xd, fd, points=torch.from_numpy(sel_p).to(dev))
+    torch.cuda.synchronize()
+    g1 = f_out.cpu().numpy()[sel_m].astype(np.complex128)
+    g2 = c_out.cpu().numpy()[sel_p].astype(np.complex128)
+    e1 = float(np.linalg.norm(g1 - r1.cpu().numpy()) / np.linalg.norm(r1.cpu().numpy()))
+    e2 = float(np.linalg.norm(g2 - r2.cpu().numpy()) / np.linalg.norm(r2.cpu().numpy()))
+    return {"type1_rel_l2": e1, "type2_rel_l2": e2, "sample": "2000 modes / 2000 points"}
+
+
+def cpu_baseline(args, np, S, n_modes):
+    """CPU port of the reference's CpuNufftPlan (oracle/cpu_nufft.c), timed
+    on this host on the first cpu_points points of the same workload."""
+    try:
+        from oracle import cpu_nufft
+    except Exception as e:  # the port is optional for the GPU numbers
+        return {"value": None, "unit": "points/s", "error": str(e)}
+    m = args.cpu_points
+    full = S.config2_points(args.points)
+    half = args.points // 2
+    # keep the workload's uniform / clustered mix
+    x = np.concatenate([full[: m // 2], full[half: half + m - m // 2]])
+    c = S.complex_normal(S.config_seed(2, 3), m)
+    fm = S.complex_normal(S.config_seed(2, 4), int(np.prod(n_modes)))
+    threads = cpu_nufft.threads()
+    t1 = cpu_nufft.time_type1(x, c, n_modes, args.eps, threads)
+    t2 = cpu_nufft.time_type2(x, fm, n_modes, args.eps, threads)
+    return {"value": m / (t1 + t2), "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": f"first {m // 2} uniform + {m - m // 2} clustered points of the workload, "
+                      f"same 128^3 grid; median of 3 executes each of type 1 and type 2",
+            "type1_s": t1, "type2_s": t2}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

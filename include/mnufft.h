@@ -50,7 +50,8 @@ extern "C" {
 #define MNUFFT_STAGE_DECONV 4
 #define MNUFFT_STAGE_INTERP 5
 #define MNUFFT_STAGE_TYPE3 6
-#define MNUFFT_STAGE_COUNT 7
+#define MNUFFT_STAGE_SPREAD_KERNEL 7 /* the spread kernel alone, inside SPREAD */
+#define MNUFFT_STAGE_COUNT 8
 
 typedef struct mnufft_plan_s *mnufft_plan;
 
@@ -141,6 +142,15 @@ double mnufft_plan_beta(mnufft_plan plan);
  * run); waits for those events.  The stage indices are MNUFFT_STAGE_*.
  */
 int mnufft_stage_times(mnufft_plan plan, double *ms, int32_t n);
+
+/*
+ * With on != 0, every later call of the plan records its stage events into a
+ * fresh slot, and mnufft_stage_times returns the sums over all calls since
+ * the previous read (then starts a new history): a caller can time many
+ * executions without synchronizing between them.  on = 0 restores
+ * last-call timing.
+ */
+int mnufft_timing_history(mnufft_plan plan, int32_t on);
 
 /*
  * FP64 direct sums on the GPU: reference_type1_f64 / reference_type2_f64

@@ -202,3 +202,28 @@ def test_type3_axis_metadata():
     m = O.type3_axis(1.0, 3.0, 0, 1, 2.0, 7)
     assert m["C"] == 2.0
     assert O.next_even_235(1287) == 1296
+
+
+def test_cpu_port_matches_direct_sums():
+    """The CPU baseline (port of CpuNufftPlan) meets the reference's CPU
+    tolerance against the direct sums, in 2D and 3D, both types."""
+    from oracle import cpu_nufft as C
+    from wgpunufft_amd import synthetic as S
+    for nm in ([16, 12, 10], [30, 20]):
+        d = len(nm)
+        x = S.uniform_coords(S.config_seed(1, 900 + d), 2000, d)
+        c = S.complex_normal(S.config_seed(1, 901), 2000)
+        fm = S.complex_normal(S.config_seed(1, 902), int(np.prod(nm)))
+        for isign in (1, -1):
+            f = C.type1(x, c, nm, 1e-6, isign)
+            ref = O.direct_type1(x.astype(np.float64), c.astype(np.complex128), nm, isign)
+            assert rel_l2(f, ref) <= 10e-6
+            v = C.type2(x, fm, nm, 1e-6, isign)
+            ref2 = O.direct_type2(x.astype(np.float64), fm.astype(np.complex128), nm, isign)
+            assert rel_l2(v, ref2) <= 10e-6
+    # thread-count independence, bitwise (tests/cpu_nufft.rs:314)
+    x = S.uniform_coords(S.config_seed(1, 903), 5000, 3)
+    c = S.complex_normal(S.config_seed(1, 904), 5000)
+    a = C.type1(x, c, [20, 20, 20], 1e-6, 1, 1)
+    b = C.type1(x, c, [20, 20, 20], 1e-6, 1, 4)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

@@ -23,7 +23,7 @@ __all__ = [
 F64 = 1
 DF64 = 2
 F32 = 3
-STAGES = ["sort", "prepare", "spread", "fft", "deconv", "interp", "type3"]
+STAGES = ["sort", "prepare", "spread", "fft", "deconv", "interp", "type3", "spread_kernel"]
 
 
 def _torch():
@@ -116,11 +116,16 @@ class _PlanBase:
     def beta(self):
         return lib().mnufft_plan_beta(self._handle)
 
+    def timing_history(self, on=True):
+        """Sum stage times over every call until the next stage_times()."""
+        check(lib().mnufft_timing_history(self._handle, 1 if on else 0))
+
     def stage_times(self):
-        """Milliseconds per stage of the last call (None where a stage did
-        not run); waits for the stage events."""
-        ms = (ctypes.c_double * 7)()
-        check(lib().mnufft_stage_times(self._handle, ms, 7))
+        """Milliseconds per stage of the last call, or summed over the calls
+        since timing_history(True) (None where a stage did not run); waits
+        for the stage events."""
+        ms = (ctypes.c_double * len(STAGES))()
+        check(lib().mnufft_stage_times(self._handle, ms, len(STAGES)))
         return {name: (ms[i] if ms[i] >= 0 else None) for i, name in enumerate(STAGES)}
 
 

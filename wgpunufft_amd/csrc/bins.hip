@@ -33,23 +33,72 @@ constexpr int kSortRun = 2048;
 // Count: bin key and atomic rank of every point; bins reaching
 // kSmallBin + 1 points are listed once for the large-bin sort.
 // ---------------------------------------------------------------------
+// Each workgroup counts kCountPoints consecutive points: their keys are
+// first aggregated in an LDS hash table (LDS atomics give every point its
+// rank among the workgroup's points of the same bin), then every distinct
+// bin takes one global atomic.  A dense cluster, whose points crowd a few
+// bins, then costs a few global atomics per workgroup instead of one per
+// point on the same address.
+constexpr int kCountThreads = 256;
+constexpr int kCountPerThread = 4;
+constexpr int kCountPoints = kCountThreads * kCountPerThread;
+constexpr int kHashSize = 2 * kCountPoints;
+constexpr uint32_t kEmpty = 0xffffffffu;
+
 template <int D, typename T>
-__global__ __launch_bounds__(256) void k_count(int64_t M, const T *__restrict__ coords,
-                                               GridParams g, uint32_t *__restrict__ counts,
-                                               uint32_t *__restrict__ keys,
-                                               uint32_t *__restrict__ ranks,
-                                               uint32_t *__restrict__ large_list,
-                                               uint32_t *__restrict__ large_count)
+__global__ __launch_bounds__(kCountThreads) void k_count(int64_t M, const T *__restrict__ coords,
+                                                         GridParams g,
+                                                         uint32_t *__restrict__ counts,
+                                                         uint32_t *__restrict__ keys,
+                                                         uint32_t *__restrict__ ranks,
+                                                         uint32_t *__restrict__ large_list,
+                                                         uint32_t *__restrict__ large_count)
 {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= M) return;
-    uint32_t key = (uint32_t)point_bin_key<D, T>(coords, j, g);
-    uint32_t rank = atomicAdd(&counts[key], 1u);
-    keys[j] = key;
-    ranks[j] = rank;
-    if (rank == kSmallBin) {
-        uint32_t slot = atomicAdd(large_count, 1u);
-        large_list[slot] = key;
+    __shared__ uint32_t ht_key[kHashSize];
+    __shared__ uint32_t ht_cnt[kHashSize];
+    __shared__ uint32_t ht_base[kHashSize];
+    for (int h = threadIdx.x; h < kHashSize; h += kCountThreads) {
+        ht_key[h] = kEmpty;
+        ht_cnt[h] = 0u;
+    }
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kCountPoints;
+    uint32_t my_key[kCountPerThread], my_slot[kCountPerThread], my_rank[kCountPerThread];
+#pragma unroll
+    for (int i = 0; i < kCountPerThread; ++i) {
+        int64_t j = base + (int64_t)i * kCountThreads + threadIdx.x;
+        my_slot[i] = kEmpty;
+        if (j >= M) continue;
+        uint32_t key = (uint32_t)point_bin_key<D, T>(coords, j, g);
+        my_key[i] = key;
+        uint32_t h = (key * 2654435761u) & (kHashSize - 1);
+        for (;;) {
+            uint32_t old = atomicCAS(&ht_key[h], kEmpty, key);
+            if (old == kEmpty || old == key) break;
+            h = (h + 1) & (kHashSize - 1);
+        }
+        my_slot[i] = h;
+        my_rank[i] = atomicAdd(&ht_cnt[h], 1u);
+    }
+    __syncthreads();
+    for (int h = threadIdx.x; h < kHashSize; h += kCountThreads) {
+        uint32_t key = ht_key[h];
+        if (key == kEmpty) continue;
+        uint32_t n = ht_cnt[h];
+        uint32_t old = atomicAdd(&counts[key], n);
+        ht_base[h] = old;
+        if (old <= kSmallBin && old + n > kSmallBin) {
+            uint32_t slot = atomicAdd(large_count, 1u);
+            large_list[slot] = key;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kCountPerThread; ++i) {
+        int64_t j = base + (int64_t)i * kCountThreads + threadIdx.x;
+        if (j >= M) continue;
+        keys[j] = my_key[i];
+        ranks[j] = ht_base[my_slot[i]] + my_rank[i];
     }
 }
 
@@ -452,8 +501,8 @@ bool BinSorter::sort(int64_t M, const T *coords, const GridParams &g, BinOrder o
         return hip_ok(hipGetLastError(), "bin clear");
     }
     uint32_t *large_count = counts_ + g.nbins + 1;
-    k_count<D, T><<<grid_for(M, 256), 256, 0, stream>>>(M, coords, g, counts_, keys_, ranks_,
-                                                        large_list_, large_count);
+    k_count<D, T><<<grid_for(M, kCountPoints), kCountThreads, 0, stream>>>(
+        M, coords, g, counts_, keys_, ranks_, large_list_, large_count);
     int64_t n = g.nbins + 1;
     int64_t tiles = (n + kScanTile - 1) / kScanTile;
     k_scan_reduce<<<(unsigned)tiles, kScanThreads, 0, stream>>>(counts_, n, tile_sums_);

@@ -325,8 +325,19 @@ int mnufft_stage_times(mnufft_plan plan, double *ms, int32_t n)
             set_error("null argument");
             return MNUFFT_ERROR_INVALID_ARGUMENT;
         }
-        return plan->plan->stage_times(ms, n) ? MNUFFT_SUCCESS : MNUFFT_ERROR_GPU;
+        return plan->plan->stage_times(ms, n, nullptr) ? MNUFFT_SUCCESS : MNUFFT_ERROR_GPU;
     });
+}
+
+int mnufft_timing_history(mnufft_plan plan, int32_t on)
+{
+    if (!plan) {
+        set_error("null plan");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    plan->plan->history = on != 0;
+    plan->plan->slots_filled = 0;
+    return MNUFFT_SUCCESS;
 }
 
 int mnufft_direct_f64(int32_t kind, int32_t dim, const int64_t *n_modes, int32_t isign,

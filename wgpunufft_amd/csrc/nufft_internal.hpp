@@ -137,7 +137,8 @@ enum Stage : int {
     kStageDeconv = 4,      // deconvolution / predeconvolution
     kStageInterp = 5,      // type-2 interpolation
     kStageType3 = 6,       // type-3 source/target/final stages
-    kStageCount = 7,
+    kStageSpreadKernel = 7,  // the output-stationary spread kernel alone
+    kStageCount = 8,
 };
 
 }  // namespace mnufft

@@ -78,6 +78,24 @@ SpreadParams make_spread_params(int d, const int64_t *fine, int width)
     return sp;
 }
 
+bool make_sweep_params(int d, const int64_t *fine, int width, SpreadParams *sp)
+{
+    constexpr int64_t tile = 16;
+    if (d != 3 || width > 8) return false;
+    if (fine[0] < tile + width + 1 || fine[1] < tile + width || fine[2] < tile + width) return false;
+    SpreadParams q{};
+    q.run = 0;
+    q.groups0 = 0;
+    q.block_count = 1;
+    for (int a = 0; a < kMaxDims; ++a) {
+        q.block[a] = tile;
+        q.nblocks[a] = (fine[a] + tile - 1) / tile;
+        q.block_count *= q.nblocks[a];
+    }
+    *sp = q;
+    return true;
+}
+
 FftPlan::~FftPlan()
 {
     if (info) rocfft_execution_info_destroy((rocfft_execution_info)info);
@@ -90,8 +108,9 @@ Plan::~Plan()
     for (void *p : {(void *)d_inv_phihat, grid, (void *)rec_start, rec_off, c_sorted, t3_xr,
                     t3_cp, t3_sr, t3_factor, t3_interp})
         if (p) (void)hipFree(p);
-    for (auto &e : events)
-        if (e) (void)hipEventDestroy(e);
+    for (auto &sl : slots)
+        for (auto &e : sl.ev)
+            if (e) (void)hipEventDestroy(e);
 }
 
 // ----------------------------------------------------------------------
@@ -133,8 +152,7 @@ bool Plan::init_common()
     layout = build_spread_layout(dim, fine, kernel.width,
                                  precision == Precision::F32 ? Precision::F32 : Precision::F64);
     sp = make_spread_params(dim, fine, kernel.width);
-    for (auto &e : events)
-        if (!hip_ok(hipEventCreate(&e), "event")) return false;
+    sweep = make_sweep_params(dim, fine, kernel.width, &sp);
     return true;
 }
 
@@ -167,6 +185,19 @@ GridParams Plan::grid_params(const int64_t *side) const
         x.origin = (double)(fine[a] / 2);
     }
     return g;
+}
+
+GridParams Plan::spread_geometry(int64_t M, int *chosen) const
+{
+    if (!sweep) return type1_geometry(M, chosen);
+    // one-cell bins along axes 1 and 2; along axis 0 the smallest power of
+    // two from 2 that keeps the bin count within 2^24
+    int64_t side[kMaxDims] = {2, 1, 1};
+    while (side[0] < fine[0] &&
+           ((fine[0] + side[0] - 1) / side[0]) * fine[1] * fine[2] > ((int64_t)1 << 24))
+        side[0] *= 2;
+    if (chosen) *chosen = -1;
+    return grid_params(side);
 }
 
 GridParams Plan::type1_geometry(int64_t M, int *chosen) const
@@ -286,17 +317,28 @@ bool Plan::fft(int64_t vectors, void *data, hipStream_t stream)
     return true;
 }
 
-void Plan::mark(int stage, bool end, hipStream_t stream)
+bool Plan::begin_call()
 {
-    if (!timing) return;
-    (void)hipEventRecord(events[2 * stage + (end ? 1 : 0)], stream);
-    stage_used[stage] = true;
+    if (!timing) return true;
+    slot = history ? slots_filled++ : 0;
+    if (!history) slots_filled = 1;
+    while ((int64_t)slots.size() <= slot) {
+        slots.emplace_back();
+        for (auto &e : slots.back().ev)
+            if (!hip_ok(hipEventCreate(&e), "timing event")) return false;
+    }
+    for (auto &u : slots[slot].used) u = false;
+    return true;
 }
 
-void Plan::reset_marks()
+void Plan::mark(int stage, bool end, hipStream_t stream)
 {
-    for (auto &u : stage_used) u = false;
+    if (!timing || slot >= (int64_t)slots.size()) return;
+    (void)hipEventRecord(slots[slot].ev[2 * stage + (end ? 1 : 0)], stream);
+    slots[slot].used[stage] = true;
 }
+
+void Plan::reset_marks() { (void)begin_call(); }
 
 // ----------------------------------------------------------------------
 // Type 1 and type 2.
@@ -309,7 +351,7 @@ bool Plan::set_points_t(int64_t M, const T *pts, hipStream_t stream)
     point_count = M;
     points_set = false;
     if (type == 1) {
-        gp = type1_geometry(M, &geometry_index);
+        gp = spread_geometry(M, &geometry_index);
         mark(kStageSort, false, stream);
         if (!sorter.sort<D, T>(M, pts, gp, BinOrder::Canonical, stream)) return false;
         mark(kStageSort, true, stream);
@@ -340,6 +382,10 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         if (M == 0) return fill_zero<T>(out, 2 * mode_total * active, stream);
         if (!ensure_strengths(M, active)) return false;
         mark(kStageSpread, false, stream);
+        bool timed = timing && slot < (int64_t)slots.size();
+        spreader.kernel_start = timed ? slots[slot].ev[2 * kStageSpreadKernel] : nullptr;
+        spreader.kernel_end = timed ? slots[slot].ev[2 * kStageSpreadKernel + 1] : nullptr;
+        if (timed) slots[slot].used[kStageSpreadKernel] = true;
         if (!gather_strengths<T>(M, active, rec_start, in, (T *)c_sorted, stream)) return false;
         if (!spreader.spread<D, T>(gp, sp, sorter.offsets(), rec_start, (const T *)rec_off,
                                    (const T *)c_sorted, M, active, (T *)grid, stream))
@@ -483,8 +529,6 @@ bool Plan::init_type3(const double *src_bounds, const double *tgt_bounds)
     inner->timing = false;
     for (int a = 0; a < dim; ++a) inner->n_modes[a] = outer_n[a];
     if (!inner->init_common()) return false;
-    for (auto &e : events)
-        if (!hip_ok(hipEventCreate(&e), "event")) return false;
     for (int a = 0; a < dim; ++a) type3_outer[a] = outer_n[a];
     return true;
 }
@@ -575,15 +619,22 @@ bool Plan::execute_type3(int64_t active, int64_t M, const void *x, const void *c
 #undef T3
 }
 
-bool Plan::stage_times(double *ms, int n)
+bool Plan::stage_times(double *ms, int n, int64_t *calls)
 {
     for (int i = 0; i < n; ++i) ms[i] = -1.0;
-    for (int st = 0; st < kStageCount && st < n; ++st) {
-        if (!stage_used[st]) continue;
-        if (!hip_ok(hipEventSynchronize(events[2 * st + 1]), "stage event")) return false;
-        float t = 0.0f;
-        if (hipEventElapsedTime(&t, events[2 * st], events[2 * st + 1]) == hipSuccess) ms[st] = t;
+    int64_t filled = std::min<int64_t>(slots_filled, (int64_t)slots.size());
+    for (int64_t sl = 0; sl < filled; ++sl) {
+        for (int st = 0; st < kStageCount && st < n; ++st) {
+            if (!slots[sl].used[st]) continue;
+            if (!hip_ok(hipEventSynchronize(slots[sl].ev[2 * st + 1]), "stage event")) return false;
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, slots[sl].ev[2 * st], slots[sl].ev[2 * st + 1]) ==
+                hipSuccess)
+                ms[st] = (ms[st] < 0 ? 0.0 : ms[st]) + t;
+        }
     }
+    if (calls) *calls = filled;
+    if (history) slots_filled = 0;
     return true;
 }
 

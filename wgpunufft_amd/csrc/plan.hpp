@@ -43,6 +43,7 @@ struct Plan {
     double *d_inv_phihat = nullptr;
     SpreadLayout layout;
     SpreadParams sp{};
+    bool sweep = false;  // 3D row-sweep spreader (else output-stationary blocks)
 
     // points
     const void *points = nullptr;
@@ -72,14 +73,24 @@ struct Plan {
     size_t t3_xr_bytes = 0, t3_cp_bytes = 0, t3_sr_bytes = 0, t3_factor_bytes = 0,
            t3_interp_bytes = 0;
 
-    // stage timing
-    hipEvent_t events[2 * kStageCount] = {};
-    bool stage_used[kStageCount] = {};
+    // stage timing: one slot of events per call, or (history mode) one per
+    // call since the last read, so callers can sum stage durations over a
+    // timed region without synchronizing inside it
+    struct TimingSlot {
+        hipEvent_t ev[2 * kStageCount] = {};
+        bool used[kStageCount] = {};
+    };
+    std::vector<TimingSlot> slots;
+    int64_t slot = 0;
+    int64_t slots_filled = 0;
+    bool history = false;
+    bool begin_call();
 
     bool init_common();
     bool init_type3(const double *src_bounds, const double *tgt_bounds);
     GridParams grid_params(const int64_t *side) const;
     GridParams type1_geometry(int64_t M, int *chosen) const;
+    GridParams spread_geometry(int64_t M, int *chosen) const;
     GridParams type2_geometry() const;
     ModeParams mode_params() const;
     size_t real_bytes() const;
@@ -97,7 +108,7 @@ struct Plan {
                   hipStream_t stream);
     bool execute_type3(int64_t active, int64_t M, const void *x, const void *c, int64_t N,
                        const void *s, void *out, hipStream_t stream);
-    bool stage_times(double *ms, int n);
+    bool stage_times(double *ms, int n, int64_t *calls);
 
     template <int D, typename T>
     bool set_points_t(int64_t M, const T *pts, hipStream_t stream);

@@ -528,6 +528,335 @@ __global__ __launch_bounds__(256) void k_reduce(GridParams g, SpreadParams sp,
     }
 }
 
+// ---------------------------------------------------------------------
+// Row-sweep spreader (3D, w <= 8).
+//
+// With bins of one cell along axes 1 and 2, every record of a bin row
+// (sy, sz) reaches the same 7x7 (w x w) rows of cells.  One wavefront owns a
+// tile of TX x TY x TZ cells and walks the rows of its reach in order, z
+// outer and y inner.  Lane (ly, dz) owns support offset dz along axis 2 and,
+// along axis 1, the cell y = ly (mod w) of the current window [sy, sy + w):
+// as sy advances by one, the lane whose window cell leaves (dy = 0) is
+// complete for this sz and adds its TX accumulators into an LDS ring of w
+// cell planes, then takes the entering cell.  Every record therefore costs
+// w FMAs per lane on registers with static indices (a switch on the record's
+// x offset), with no redundant cells along x and none idle along y and z
+// except at the tile border.  A plane of the ring is complete once the sweep
+// has passed its z, and is written out exactly once.  Sums stay in a fixed
+// order: rows in order, records in their (canonical) row order, ring adds in
+// row order.
+// ---------------------------------------------------------------------
+constexpr int kSweepRows = 64;
+constexpr int kSweepTile = 16;  // tile edge of the row sweep along every axis
+
+template <typename T, int W>
+struct SweepRec {
+    int e;  // x offset of the support start from the tile's reach start
+    T cre, cim;
+    T kx[W];
+    T ky[W];
+    T kz[W];
+};
+
+template <int E, int W, int TX, typename T>
+__device__ __forceinline__ void sweep_accumulate(T (&ar)[TX], T (&ai)[TX], const T *kx, T vre,
+                                                 T vim)
+{
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        constexpr int base = E - (W - 1);
+        const int x = base + j;
+        if (x >= 0 && x < TX) {
+            T k = kx[j];
+            ar[x] += k * vre;
+            ai[x] += k * vim;
+        }
+    }
+}
+
+template <int W, int TX, typename T>
+__device__ __forceinline__ void sweep_dispatch(int e, T (&ar)[TX], T (&ai)[TX], const T *kx, T vre,
+                                               T vim)
+{
+#define MNUFFT_SWEEP_CASE(E)                                                                      \
+    case E:                                                                                       \
+        if constexpr (E < TX + W - 1) sweep_accumulate<E, W, TX, T>(ar, ai, kx, vre, vim);        \
+        break;
+    switch (e) {
+        MNUFFT_SWEEP_CASE(0) MNUFFT_SWEEP_CASE(1) MNUFFT_SWEEP_CASE(2) MNUFFT_SWEEP_CASE(3)
+        MNUFFT_SWEEP_CASE(4) MNUFFT_SWEEP_CASE(5) MNUFFT_SWEEP_CASE(6) MNUFFT_SWEEP_CASE(7)
+        MNUFFT_SWEEP_CASE(8) MNUFFT_SWEEP_CASE(9) MNUFFT_SWEEP_CASE(10) MNUFFT_SWEEP_CASE(11)
+        MNUFFT_SWEEP_CASE(12) MNUFFT_SWEEP_CASE(13) MNUFFT_SWEEP_CASE(14) MNUFFT_SWEEP_CASE(15)
+        MNUFFT_SWEEP_CASE(16) MNUFFT_SWEEP_CASE(17) MNUFFT_SWEEP_CASE(18) MNUFFT_SWEEP_CASE(19)
+        MNUFFT_SWEEP_CASE(20) MNUFFT_SWEEP_CASE(21) MNUFFT_SWEEP_CASE(22) MNUFFT_SWEEP_CASE(23)
+        MNUFFT_SWEEP_CASE(24) MNUFFT_SWEEP_CASE(25) MNUFFT_SWEEP_CASE(26) MNUFFT_SWEEP_CASE(27)
+        MNUFFT_SWEEP_CASE(28) MNUFFT_SWEEP_CASE(29) MNUFFT_SWEEP_CASE(30) MNUFFT_SWEEP_CASE(31)
+    default: break;
+    }
+#undef MNUFFT_SWEEP_CASE
+}
+
+template <typename T, int W, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kLanes) void k_sweep3(GridParams g, SpreadParams sp,
+                                                   const uint32_t *__restrict__ offsets,
+                                                   const int4 *__restrict__ rec_start,
+                                                   const T *__restrict__ rec_off,
+                                                   const T *__restrict__ c_sorted, int64_t M,
+                                                   const uint32_t *__restrict__ totals,
+                                                   const uint32_t *__restrict__ parts,
+                                                   const uint32_t *__restrict__ item_off,
+                                                   const uint32_t *__restrict__ heavy_base,
+                                                   const uint32_t *__restrict__ ctrl,
+                                                   T *__restrict__ grid, T *__restrict__ partial)
+{
+    static_assert(TX + W - 1 <= 32, "x offsets beyond the dispatch table");
+    constexpr int NRY = TY + W - 1;
+    constexpr int NRZ = TZ + W - 1;
+    constexpr int NROWS = NRY * NRZ;
+    __shared__ T ring_re[W][TY][TX];
+    __shared__ T ring_im[W][TY][TX];
+    __shared__ SweepRec<T, W> st[kLanes];
+    __shared__ int64_t row_f0[kSweepRows], row_n0[kSweepRows], row_f1[kSweepRows],
+        row_n1[kSweepRows], row_excl[kSweepRows];
+
+    const int lane = threadIdx.x;
+    const int ly = lane % W;
+    const int dz = lane / W;
+    const bool active = lane < W * W;
+    const int64_t v = blockIdx.y;
+    const T beta = (T)g.beta;
+    const int64_t n_items = ctrl[0];
+    const uint32_t part_len_cap = ctrl[3];
+    const T *cv = c_sorted + 2 * v * M;
+    const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n, n2 = g.ax[2].n;
+    const int64_t bins0 = g.ax[0].bins;
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        int64_t lo = 0, hi = sp.block_count - 1;
+        while (lo < hi) {
+            int64_t mid = (lo + hi + 1) >> 1;
+            if ((int64_t)item_off[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int64_t b = lo;
+        const int64_t p = item - item_off[b];
+        const uint32_t np = parts[b];
+        const int64_t total = totals[b];
+        int64_t rlo = 0, rhi = total;
+        if (np > 1) {
+            rlo = p * (int64_t)part_len_cap;
+            rhi = rlo + part_len_cap < total ? rlo + part_len_cap : total;
+        }
+        const int64_t bx = b % sp.nblocks[0];
+        const int64_t by = (b / sp.nblocks[0]) % sp.nblocks[1];
+        const int64_t bz = b / (sp.nblocks[0] * sp.nblocks[1]);
+        const int64_t x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
+        const AxisReach rx = axis_reach(x0, TX, W, g.ax[0]);
+        int64_t xs = x0 - (W - 1);
+        if (xs < 0) xs += n0;
+
+        for (int t = lane; t < W * TY * TX; t += kLanes) {
+            (&ring_re[0][0][0])[t] = (T)0;
+            (&ring_im[0][0][0])[t] = (T)0;
+        }
+        T ar[TX], ai[TX];
+#pragma unroll
+        for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
+        __syncthreads();
+
+        int64_t cursor = 0;
+        for (int rb = 0; rb < NROWS; rb += kSweepRows) {
+            {
+                int r = rb + lane;
+                int64_t f0 = 0, c0 = 0, f1 = 0, c1 = 0;
+                if (r < NROWS && cursor < rhi) {
+                    int ry = r % NRY, rz = r / NRY;
+                    int64_t sy = y0 - (W - 1) + ry;
+                    int64_t sz = z0 - (W - 1) + rz;
+                    if (sy < 0) sy += n1;
+                    if (sy >= n1) sy -= n1;
+                    if (sz < 0) sz += n2;
+                    if (sz >= n2) sz -= n2;
+                    int64_t row = bins0 * (sy + n1 * sz);
+                    f0 = offsets[row + rx.lo0];
+                    c0 = (int64_t)offsets[row + rx.lo0 + rx.len0] - f0;
+                    if (rx.len1 > 0) {
+                        f1 = offsets[row + rx.lo1];
+                        c1 = (int64_t)offsets[row + rx.lo1 + rx.len1] - f1;
+                    }
+                }
+                int64_t cnt = c0 + c1;
+                int64_t xsum = cnt;
+#pragma unroll
+                for (int o = 1; o < kLanes; o <<= 1) {
+                    int64_t y = __shfl_up(xsum, o, kLanes);
+                    if (lane >= o) xsum += y;
+                }
+                row_f0[lane] = f0;
+                row_n0[lane] = c0;
+                row_f1[lane] = f1;
+                row_n1[lane] = c1;
+                row_excl[lane] = xsum - cnt;
+                __syncthreads();
+            }
+            const int nq = NROWS - rb < kSweepRows ? NROWS - rb : kSweepRows;
+            for (int q = 0; q < nq; ++q) {
+                const int r = rb + q;
+                const int ry = r % NRY, rz = r / NRY;
+                if (ry == 0) {
+#pragma unroll
+                    for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
+                }
+                const int64_t sy_rel = y0 - (W - 1) + ry;
+                int dy = (int)((ly - sy_rel) % W);
+                if (dy < 0) dy += W;
+                const int64_t row_pos = cursor + row_excl[q];
+                for (int k = 0; k < 2; ++k) {
+                    const int64_t f = k == 0 ? row_f0[q] : row_f1[q];
+                    const int64_t n = k == 0 ? row_n0[q] : row_n1[q];
+                    const int64_t pos0 = row_pos + (k == 0 ? 0 : row_n0[q]);
+                    int64_t jlo = rlo > pos0 ? rlo - pos0 : 0;
+                    int64_t jhi = rhi - pos0 < n ? rhi - pos0 : n;
+                    for (int64_t j0 = jlo; j0 < jhi; j0 += kLanes) {
+                        const int64_t j = j0 + lane;
+                        bool take = false;
+                        int64_t slot = 0;
+                        int e = 0;
+                        if (j < jhi) {
+                            slot = f + j;
+                            int64_t ee = (int64_t)rec_start[slot].x - xs;
+                            if (ee < 0) ee += n0;
+                            e = (int)ee;
+                            take = ee < TX + W - 1;
+                        }
+                        uint64_t mask = __ballot(take);
+                        int pos = __popcll(mask & ((1ull << lane) - 1ull));
+                        int nst = __popcll(mask);
+                        if (take) {
+                            SweepRec<T, W> &S = st[pos];
+                            S.e = e;
+                            S.cre = cv[2 * slot];
+                            S.cim = cv[2 * slot + 1];
+                            T ox = rec_off[4 * slot + 0];
+                            T oy = rec_off[4 * slot + 1];
+                            T oz = rec_off[4 * slot + 2];
+#pragma unroll
+                            for (int jj = 0; jj < W; ++jj) {
+                                S.kx[jj] = es_weight<T>((T)jj + ox, W, beta);
+                                S.ky[jj] = es_weight<T>((T)jj + oy, W, beta);
+                                S.kz[jj] = es_weight<T>((T)jj + oz, W, beta);
+                            }
+                        }
+                        __syncthreads();
+                        if (active) {
+                            for (int s2 = 0; s2 < nst; ++s2) {
+                                const SweepRec<T, W> &S = st[s2];
+                                T kx[W];
+#pragma unroll
+                                for (int jj = 0; jj < W; ++jj) kx[jj] = S.kx[jj];
+                                T wyz = S.ky[dy] * S.kz[dz];
+                                T vre = S.cre * wyz, vim = S.cim * wyz;
+                                sweep_dispatch<W, TX, T>(S.e, ar, ai, kx, vre, vim);
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+                // the lane whose window cell leaves adds it to the ring
+                if (active && dy == 0) {
+                    const int yy = ry - (W - 1);
+                    const int zz = rz - (W - 1) + dz;
+                    if (yy >= 0 && yy < TY && zz >= 0 && zz < TZ) {
+                        const int slot_z = zz % W;
+#pragma unroll
+                        for (int i = 0; i < TX; ++i) {
+                            ring_re[slot_z][yy][i] += ar[i];
+                            ring_im[slot_z][yy][i] += ai[i];
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
+                }
+                if (ry == NRY - 1) {
+                    // the plane of this sz is complete
+                    __syncthreads();
+                    const int zz = rz - (W - 1);
+                    if (zz >= 0 && zz < TZ) {
+                        const int slot_z = zz % W;
+                        for (int t = lane; t < TY * TX; t += kLanes) {
+                            const int yy = t / TX, xx = t % TX;
+                            T re = ring_re[slot_z][yy][xx], im = ring_im[slot_z][yy][xx];
+                            ring_re[slot_z][yy][xx] = (T)0;
+                            ring_im[slot_z][yy][xx] = (T)0;
+                            if (np == 1) {
+                                const int64_t gx = x0 + xx, gy = y0 + yy, gz = z0 + zz;
+                                if (gx < n0 && gy < n1 && gz < n2) {
+                                    T *o = grid + 2 * (v * g.fine_total + gx + n0 * (gy + n1 * gz));
+                                    o[0] = re;
+                                    o[1] = im;
+                                }
+                            } else {
+                                const int64_t pslot = (int64_t)heavy_base[b] + p;
+                                T *o = partial +
+                                       2 * (((pslot * gridDim.y + v) * TZ + zz) * (TY * TX) + t);
+                                o[0] = re;
+                                o[1] = im;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            cursor += row_excl[kLanes - 1] + row_n0[kLanes - 1] + row_n1[kLanes - 1];
+            __syncthreads();
+        }
+    }
+}
+
+// Heavy tiles of the row sweep: every cell adds its partial sums in part
+// order.
+template <typename T, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_reduce_tiles(GridParams g, SpreadParams sp,
+                                                      const uint32_t *__restrict__ parts,
+                                                      const uint32_t *__restrict__ heavy_base,
+                                                      const uint32_t *__restrict__ heavy_list,
+                                                      const uint32_t *__restrict__ ctrl,
+                                                      const T *__restrict__ partial,
+                                                      T *__restrict__ grid)
+{
+    constexpr int64_t cells = (int64_t)TX * TY * TZ;
+    const int64_t v = blockIdx.y;
+    const int64_t vectors = gridDim.y;
+    const int64_t work = (int64_t)ctrl[2] * cells;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < work;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t hb = t / cells;
+        const int64_t c = t - hb * cells;
+        const int64_t b = heavy_list[hb];
+        const int64_t np = parts[b];
+        const int64_t base = heavy_base[b];
+        T sre = (T)0, sim = (T)0;
+        for (int64_t q = 0; q < np; ++q) {
+            const T *src = partial + 2 * (((base + q) * vectors + v) * cells + c);
+            sre += src[0];
+            sim += src[1];
+        }
+        const int64_t xx = c % TX, yy = (c / TX) % TY, zz = c / (TX * TY);
+        const int64_t bx = b % sp.nblocks[0];
+        const int64_t by = (b / sp.nblocks[0]) % sp.nblocks[1];
+        const int64_t bz = b / (sp.nblocks[0] * sp.nblocks[1]);
+        const int64_t gx = bx * TX + xx, gy = by * TY + yy, gz = bz * TZ + zz;
+        const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n;
+        if (gx < n0 && gy < n1 && gz < g.ax[2].n) {
+            T *o = grid + 2 * (v * g.fine_total + gx + n0 * (gy + n1 * gz));
+            o[0] = sre;
+            o[1] = sim;
+        }
+    }
+}
+
 inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 __global__ void k_fill_ctrl(uint32_t *ctrl)
@@ -550,14 +879,41 @@ bool launch_spread(Spreader &s, const GridParams &g, const SpreadParams &sp,
     // items with a stride.
     unsigned items_grid = (unsigned)std::min<int64_t>(nb + 16384, 65536);
     dim3 grid3(items_grid, (unsigned)vectors);
+    if (s.kernel_start) (void)hipEventRecord(s.kernel_start, stream);
     k_spread<D, T, R><<<grid3, kLanes, 0, stream>>>(g, sp, offsets, rec_start, rec_off, c_sorted,
                                                     M, s.totals, s.parts, s.item_off,
                                                     s.heavy_base, s.ctrl, grid,
                                                     (T *)s.partial, cells);
+    if (s.kernel_end) (void)hipEventRecord(s.kernel_end, stream);
     dim3 rgrid(2048, (unsigned)vectors);
     k_reduce<D, T, R><<<rgrid, 256, 0, stream>>>(g, sp, s.parts, s.heavy_base, s.heavy_list,
                                                  s.ctrl, (const T *)s.partial, grid, cells);
     return hip_ok(hipGetLastError(), "spread launch");
+}
+
+template <typename T, int W>
+bool launch_sweep(Spreader &s, const GridParams &g, const SpreadParams &sp,
+                  const uint32_t *offsets, const int4 *rec_start, const T *rec_off,
+                  const T *c_sorted, int64_t M, int64_t vectors, T *grid, hipStream_t stream)
+{
+    constexpr int TX = kSweepTile, TY = kSweepTile, TZ = kSweepTile;
+    int64_t nb = sp.block_count;
+    k_classify<3><<<grid_for(nb, 256), 256, 0, stream>>>(g, sp, offsets, s.totals);
+    k_fill_ctrl<<<1, 64, 0, stream>>>(s.ctrl);
+    k_plan<<<1, 1024, 0, stream>>>(nb, s.totals, s.light_limit, s.min_part, s.part_capacity,
+                                   s.parts, s.item_off, s.heavy_base, s.heavy_list, s.ctrl);
+    unsigned items_grid = (unsigned)std::min<int64_t>(nb + 16384, 65536);
+    dim3 grid3(items_grid, (unsigned)vectors);
+    if (s.kernel_start) (void)hipEventRecord(s.kernel_start, stream);
+    k_sweep3<T, W, TX, TY, TZ><<<grid3, kLanes, 0, stream>>>(
+        g, sp, offsets, rec_start, rec_off, c_sorted, M, s.totals, s.parts, s.item_off,
+        s.heavy_base, s.ctrl, grid, (T *)s.partial);
+    if (s.kernel_end) (void)hipEventRecord(s.kernel_end, stream);
+    dim3 rgrid(2048, (unsigned)vectors);
+    k_reduce_tiles<T, TX, TY, TZ><<<rgrid, 256, 0, stream>>>(g, sp, s.parts, s.heavy_base,
+                                                            s.heavy_list, s.ctrl,
+                                                            (const T *)s.partial, grid);
+    return hip_ok(hipGetLastError(), "sweep launch");
 }
 
 }  // namespace
@@ -602,11 +958,28 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
                       const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
                       int64_t vectors, T *grid, hipStream_t stream)
 {
-    int64_t cells = (int64_t)kLanes * sp.run;
+    const bool sweep = sp.run == 0;
+    int64_t cells = sweep ? (int64_t)kSweepTile * kSweepTile * kSweepTile : (int64_t)kLanes * sp.run;
     int64_t per_part = cells * vectors * 2 * (int64_t)sizeof(T);
     // part slots: the scratch budget, at least one per block
     part_capacity = std::max<int64_t>(budget_bytes / per_part, sp.block_count + 1);
     if (!ensure(sp.block_count, part_capacity * per_part)) return false;
+    if (sweep) {
+        if constexpr (D == 3) {
+            switch (g.width) {
+            case 2: return launch_sweep<T, 2>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
+            case 3: return launch_sweep<T, 3>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
+            case 4: return launch_sweep<T, 4>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
+            case 5: return launch_sweep<T, 5>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
+            case 6: return launch_sweep<T, 6>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
+            case 7: return launch_sweep<T, 7>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
+            case 8: return launch_sweep<T, 8>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
+            default: break;
+            }
+        }
+        set_error("row sweep needs a 3D plan of kernel width <= 8");
+        return false;
+    }
     switch (sp.run) {
     case 2: return launch_spread<D, T, 2>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M,
                                           vectors, grid, stream);

@@ -19,6 +19,9 @@ struct Spreader {
                 const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
                 int64_t vectors, T *grid, hipStream_t stream);
 
+    // Optional events recorded right before and after the spread kernel.
+    hipEvent_t kernel_start = nullptr, kernel_end = nullptr;
+
     uint32_t *totals = nullptr;      // records in reach per block
     uint32_t *parts = nullptr;       // work items per block
     uint32_t *item_off = nullptr;    // first work item per block
@@ -41,5 +44,9 @@ struct Spreader {
 // along axis 0 (run + width - 1 <= n0 keeps every cell's support offset
 // unique within a run).
 SpreadParams make_spread_params(int d, const int64_t *fine, int width);
+
+// Row-sweep tiles (3D, width <= 8, every axis at least tile + width long):
+// run = 0 marks them.  Returns false when the grid does not qualify.
+bool make_sweep_params(int d, const int64_t *fine, int width, SpreadParams *sp);
 
 }  // namespace mnufft
