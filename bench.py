@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
     ap.add_argument("--cpu-points", type=int, default=3_000_000,
                     help="points of the CPU baseline sample")
+    ap.add_argument("--config", type=int, default=2,
+                    help="2: the headline workload (default); 1, 3, 4, 5: the other "
+                         "configurations, reported as extra JSON lines")
     return ap.parse_args()
 
 
@@ -55,6 +58,10 @@ def main():
 
     import wgpunufft_amd as W
     from wgpunufft_amd import synthetic as S
+
+    if args.config != 2:
+        from wgpunufft_amd import bench_configs
+        return bench_configs.run(args)
 
     dev = torch.device("cuda:0")
     M = args.points
@@ -107,12 +114,18 @@ def main():
     t_interp_ms = avg(st2["interp"])
     spread_tflops = spread_flops / (t_spread_ms * 1e-3) / 1e12
     interp_gbs = interp_bytes / (t_interp_ms * 1e-3) / 1e9
-    roof_spread = {"kernel": "k_spread (type-1 output-stationary spread)", "bound": "mfma",
+    sweep = info1["run"] == 0
+    tiled = p2.info()["bin_side"] == [8, 8, 8]
+    spread_name = ("k_sweep3 (type-1 3D row-sweep spread)" if sweep
+                   else "k_spread (type-1 output-stationary spread)")
+    interp_name = ("k_interp_tiled3 (type-2 LDS-tiled interpolation)" if tiled
+                   else "k_interp (type-2 interpolation)")
+    roof_spread = {"kernel": spread_name, "bound": "mfma",
                    "achieved": round(spread_tflops, 3), "peak": FP32_PEAK_TFLOPS,
                    "unit": "TFLOP/s", "frac": round(spread_tflops / FP32_PEAK_TFLOPS, 4),
                    "traffic": None, "avg_ms": round(t_spread_ms, 4),
                    "algorithmic": f"4*w^3*M = {spread_flops:.4g} flop (w={w}, M={M})"}
-    roof_interp = {"kernel": "k_interp (type-2 interpolation)", "bound": "hbm",
+    roof_interp = {"kernel": interp_name, "bound": "hbm",
                    "achieved": round(interp_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(interp_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                    "avg_ms": round(t_interp_ms, 4),
@@ -142,7 +155,8 @@ def main():
                                f"M={M} (half uniform, half Gaussian sigma 0.05 cluster)",
                    "n_modes": n_modes, "fine": info1["fine"], "width": w,
                    "points": M, "bin_geometry_t1": info1["bin_side"],
-                   "spreader": "plain FP32 output-stationary"},
+                   "spreader": "plain FP32 row sweep" if sweep else "plain FP32 output-stationary",
+                   "interp": "LDS-tiled" if tiled else "per-point gather"},
         "roofline": dominant,
         "roofline_other": other,
         "stages_ms": stages,

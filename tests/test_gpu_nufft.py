@@ -302,3 +302,64 @@ def test_gpu_direct_sums_match_oracle():
     fm = S.complex_normal(S.config_seed(3, 3), 72, dtype=np.complex128)
     got2 = W.reference_type2_f64(n_modes, torch.from_numpy(x).cuda(), torch.from_numpy(fm).cuda())
     assert rel_l2(got2.cpu().numpy(), O.direct_type2(x, fm, n_modes)) <= 1e-13
+
+
+def test_f64_row_sweep_and_tiled_interp():
+    """F64 3D plans at a width <= 8 run the row-sweep spreader and the
+    LDS-tiled interpolation (fine grids of at least 24 cells per axis)."""
+    torch, W, O, S = _setup()
+    n_modes = [16, 14, 12]
+    M = 4000
+    x = S.clustered_coords(S.config_seed(5, 20), M, (0.3, -1.1, 2.0), 0.2).astype(np.float64)
+    c = S.complex_normal(S.config_seed(5, 21), M, dtype=np.complex128)
+    fm = S.complex_normal(S.config_seed(5, 22), int(np.prod(n_modes)), dtype=np.complex128)
+    cfg = W.NufftConfig(n_modes, 1e-6).with_precision(W.F64)
+    p1 = W.NufftPlan.type1_gpu(cfg)
+    p2 = W.NufftPlan.type2_gpu(cfg)
+    xt = torch.from_numpy(x).cuda()
+    f = p1.encode_gpu(xt, torch.from_numpy(c).cuda()).cpu().numpy()
+    vv = p2.encode_gpu(xt, torch.from_numpy(fm).cuda()).cpu().numpy()
+    assert p2.info()["bin_side"] == [8, 8, 8]
+    e1 = rel_l2(f, O.direct_type1(x, c, n_modes))
+    e2 = rel_l2(vv, O.direct_type2(x, fm, n_modes))
+    print(f"NUFFT_F64_SWEEP t1={e1:.3e} t2={e2:.3e}")
+    assert e1 <= 32e-6 and e2 <= 32e-6
+
+
+def test_bin_sort_multi_round_merges():
+    """A dense cluster puts tens of thousands of points in single bins, so
+    the large-bin sort runs several merge rounds; the permutation stays the
+    reference's, bit for bit."""
+    torch, W, O, S = _setup()
+    shape = [32, 32, 32]
+    M = 200000
+    x = S.clustered_coords(S.config_seed(2, 700), M, (0.3, -1.1, 2.0), 0.002)
+    plan = W.NufftPlan.type1_gpu(W.NufftConfig(shape, 1e-6))
+    info = plan.info()
+    fine = info["fine"]
+    w = info["width"]
+    lay = O.layout(fine, w, 0, M)
+    side = lay["geometries"][lay["chosen"]]
+    keys, _, _, amb = O.bin_keys_f32(fine, w, side, x)
+    nbins = int(np.prod([(f + s - 1) // s for f, s in zip(fine, side)]))
+    ref_sorted, ref_off = O.bin_sort(keys, nbins)
+    assert amb == 0 and np.diff(ref_off).max() > 4 * 2048
+    srt, off = plan.bin_sort(torch.from_numpy(x).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(off.cpu().numpy().astype(np.uint32), ref_off)
+    assert np.array_equal(srt.cpu().numpy().astype(np.uint32), ref_sorted)
+
+
+def test_type2_batched_tiled_interp_matches_oracle():
+    torch, W, O, S = _setup()
+    n_modes = [20, 18, 16]
+    M, B = 5000, 2
+    x = S.uniform_coords(S.config_seed(4, 30), M, 3)
+    fm = S.complex_normal(S.config_seed(4, 31), int(np.prod(n_modes)) * B)
+    plan = W.NufftPlan.type2_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
+    out = plan.encode_gpu(torch.from_numpy(x).cuda(), torch.from_numpy(fm).cuda()).cpu().numpy()
+    N = int(np.prod(n_modes))
+    for v in range(B):
+        ref = O.direct_type2(x.astype(np.float64), fm[v * N:(v + 1) * N].astype(np.complex128),
+                             n_modes)
+        assert rel_l2(out[v * M:(v + 1) * M], ref) <= 32e-6

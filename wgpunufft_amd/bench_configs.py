@@ -1,0 +1,164 @@
+"""The benchmark configurations other than the headline one (bench.py
+--config N): each prints one JSON line with points/s, the per-stage times
+and the error against FP64 direct sums on a seeded sample.
+
+1. 2D type-1 F32, 128x128 modes, tol 1e-6, M = 100,000 uniform points.
+3. 2D type-1, 1024x1024 modes, M = 5e7 uniform points: F32 at tol 1e-6 and
+   tol 1e-3.
+4. (a) 64 independent 2D type-2 F32 problems at tol 1e-5, modes per axis
+   log-uniform in [64, 512], M log-uniform in [1e4, 2e6]; (b) 32 vectors
+   over one 3D point set of 8192 radial spokes x 512 samples, 256^3 modes.
+5. 3D type-3, M = 1e6 sources in [-5,5]^3; the targets are scaled down from
+   [-200,200]^3 to [-40,40]^3 so the inner fine grid fits one GPU (the full
+   interval needs a 2592^3 inner grid, 279 GB in complex128).
+"""
+import json
+import time
+
+import numpy as np
+
+
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def _err_t1(W, S, torch, x, c, n_modes, out, sample=1000, seed=1):
+    nm = int(np.prod(n_modes))
+    sel = np.sort(S.splitmix64(seed, min(sample, nm)) % np.uint64(nm)).astype(np.int64)
+    dev = out.device
+    r = W.reference_type1_f64(n_modes, torch.from_numpy(x.astype(np.float64)).to(dev),
+                              torch.from_numpy(c.astype(np.complex128)).to(dev),
+                              modes=torch.from_numpy(sel).to(dev)).cpu().numpy()
+    g = out.cpu().numpy()[sel].astype(np.complex128)
+    return float(np.linalg.norm(g - r) / np.linalg.norm(r))
+
+
+def _line(args, metric, value, unit, ms, dtype, config, extra):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+         "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded)",
+         "config": config}
+    d.update(extra)
+    print(json.dumps(d), flush=True)
+
+
+def config1(args, W, S, torch):
+    dev = torch.device("cuda:0")
+    M, n_modes = 100_000, [128, 128]
+    x = S.uniform_coords(S.config_seed(1, 1), M, 2)
+    c = S.complex_normal(S.config_seed(1, 2), M)
+    xt, ct = torch.from_numpy(x).to(dev), torch.from_numpy(c).to(dev)
+    plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6))
+    out = torch.empty(int(np.prod(n_modes)), dtype=torch.complex64, device=dev)
+    t = _timed(lambda: plan.encode_gpu(xt, ct, out=out), args.steps, args.warmup)
+    plan.encode_gpu(xt, ct, out=out)
+    st = plan.stage_times()
+    err = _err_t1(W, S, torch, x, c, n_modes, out)
+    _line(args, "NU points/s, 2D type-1 F32 128^2 modes tol 1e-6", M / t, "points/s", t * 1e3,
+          "f32", {"workload": "config 1: 2D type-1, 128x128 modes, M=1e5 uniform"},
+          {"stages_ms": st, "error": {"type1_rel_l2": err, "sample": "1000 modes"}})
+
+
+def config3(args, W, S, torch):
+    dev = torch.device("cuda:0")
+    M, n_modes = 50_000_000, [1024, 1024]
+    x = S.uniform_coords(S.config_seed(3, 1), M, 2)
+    c = S.complex_normal(S.config_seed(3, 2), M)
+    xt, ct = torch.from_numpy(x).to(dev), torch.from_numpy(c).to(dev)
+    out = torch.empty(int(np.prod(n_modes)), dtype=torch.complex64, device=dev)
+    for eps in (1e-6, 1e-3):
+        plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, eps))
+        t = _timed(lambda: plan.encode_gpu(xt, ct, out=out), args.steps, args.warmup)
+        plan.encode_gpu(xt, ct, out=out)
+        st = plan.stage_times()
+        err = _err_t1(W, S, torch, x, c, n_modes, out, sample=2000, seed=S.config_seed(3, 7))
+        _line(args, f"NU points/s, 2D type-1 1024^2 modes tol {eps:g} (plain FP32 spreader)",
+              M / t, "points/s", t * 1e3, "f32",
+              {"workload": f"config 3: 2D type-1, 1024x1024 modes, M=5e7 uniform, tol {eps:g}"},
+              {"stages_ms": st, "error": {"type1_rel_l2": err, "sample": "2000 modes"}})
+
+
+def config4(args, W, S, torch):
+    dev = torch.device("cuda:0")
+    # (a) 64 independent 2D type-2 problems
+    rng_m = S.uniform01(S.config_seed(4, 1), 64 * 3).reshape(64, 3)
+    problems = []
+    total_pts = 0
+    for i in range(64):
+        n0 = int(round(np.exp(np.log(64) + rng_m[i, 0] * (np.log(512) - np.log(64)))))
+        n1 = int(round(np.exp(np.log(64) + rng_m[i, 1] * (np.log(512) - np.log(64)))))
+        M = int(round(np.exp(np.log(1e4) + rng_m[i, 2] * (np.log(2e6) - np.log(1e4)))))
+        x = torch.from_numpy(S.uniform_coords(S.config_seed(4, 100 + i), M, 2)).to(dev)
+        f = torch.from_numpy(S.complex_normal(S.config_seed(4, 200 + i), n0 * n1)).to(dev)
+        plan = W.NufftPlan.type2_gpu(W.NufftConfig([n0, n1], 1e-5))
+        out = torch.empty(M, dtype=torch.complex64, device=dev)
+        problems.append((plan, x, f, out))
+        total_pts += M
+
+    def run_all():
+        for plan, x, f, out in problems:
+            plan.encode_gpu(x, f, out=out)
+
+    t = _timed(run_all, args.steps, args.warmup)
+    _line(args, "NU points/s, 64 independent 2D type-2 problems tol 1e-5", total_pts / t,
+          "points/s", t * 1e3, "f32",
+          {"workload": f"config 4a: 64 2D type-2 plans, modes log-uniform [64,512], "
+                       f"M log-uniform [1e4,2e6] (total {total_pts})"},
+          {"launch": "one plan per problem, back to back on one stream"})
+    # (b) 32 vectors over a radial 3D point set
+    x = S.radial_spokes()
+    M = x.shape[0]
+    B = 32
+    n_modes = [256, 256, 256]
+    c = S.complex_normal(S.config_seed(4, 5), M * B)
+    xt, ct = torch.from_numpy(x).to(dev), torch.from_numpy(c).to(dev)
+    plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
+    out = torch.empty(int(np.prod(n_modes)) * B, dtype=torch.complex64, device=dev)
+    t = _timed(lambda: plan.encode_gpu(xt, ct, out=out), args.steps, args.warmup)
+    plan.encode_gpu(xt, ct, out=out)
+    st = plan.stage_times()
+    err = _err_t1(W, S, torch, x, c[:M], n_modes, out[: int(np.prod(n_modes))], sample=500,
+                  seed=S.config_seed(4, 9))
+    _line(args, "NU points/s, 3D type-1 256^3 modes, 32 vectors, radial", M * B / t, "points/s",
+          t * 1e3, "f32",
+          {"workload": "config 4b: 3D type-1 256^3 modes tol 1e-6, 8192 spokes x 512, B=32"},
+          {"stages_ms": st, "error": {"type1_rel_l2_vector0": err, "sample": "500 modes"}})
+
+
+def config5(args, W, S, torch):
+    dev = torch.device("cuda:0")
+    M = N = 1_000_000
+    x = S.uniform_coords(S.config_seed(5, 1), M, 3, lo=-5.0, hi=5.0, dtype=np.float64)
+    s = S.uniform_coords(S.config_seed(5, 2), N, 3, lo=-40.0, hi=40.0, dtype=np.float64)
+    c = S.complex_normal(S.config_seed(5, 3), M, dtype=np.complex128)
+    xt, st_, ct = (torch.from_numpy(a).to(dev) for a in (x, s, c))
+    sel = np.sort(S.splitmix64(S.config_seed(5, 4), 1000) % np.uint64(N)).astype(np.int64)
+    ref = W.reference_type3_f64(xt, ct, st_, targets=torch.from_numpy(sel).to(dev)).cpu().numpy()
+    for eps in (1e-12, 1e-10, 1e-6):
+        plan = W.NufftType3Plan(3, [(-5, 5)] * 3, [(-40, 40)] * 3, eps=eps, precision=W.F64)
+        out = torch.empty(N, dtype=torch.complex128, device=dev)
+        t = _timed(lambda: plan.encode_gpu(xt, ct, st_, out=out), args.steps, args.warmup)
+        g = out.cpu().numpy()[sel]
+        err = float(np.linalg.norm(g - ref) / np.linalg.norm(ref))
+        _line(args, f"NU points/s, 3D type-3 F64 tol {eps:g}", (M + N) / t, "points/s", t * 1e3,
+              "f64",
+              {"workload": "config 5 (targets scaled to [-40,40]^3): 1e6 sources in [-5,5]^3, "
+                           f"1e6 targets, tol {eps:g}", "outer_grid": plan.info()["outer"]},
+              {"error": {"rel_l2": err, "sample": "1000 targets"}})
+
+
+def run(args):
+    import torch
+
+    import wgpunufft_amd as W
+    from wgpunufft_amd import synthetic as S
+    {1: config1, 3: config3, 4: config4, 5: config5}[args.config](args, W, S, torch)
+    return 0

@@ -9,11 +9,12 @@
 // reference permutation); canonical order sorts by coordinate bits so the
 // type-1 sums do not depend on the input order of the points.
 //
-// MI355X design: one 64-lane wavefront per 64 points for the count and
-// scatter passes (atomics land in the 256 MB Infinity Cache for bin arrays of
-// a few tens of MB); a three-kernel reduce-then-scan over 4096-element tiles;
-// bins above 64 points get a workgroup each that sorts 2048-element runs in
-// LDS and merges them along merge paths.
+// MI355X design: the count pass aggregates each workgroup's 1024 points in an
+// LDS hash table before one global atomic per distinct bin (the bin arrays,
+// a few tens of MB, stay in the 256 MB Infinity Cache); a three-kernel
+// reduce-then-scan over 4096-element tiles; bins above 64 points are sorted
+// by a tiled merge sort over all workgroups (LDS bitonic runs, merge-path
+// rounds).
 #include <hip/hip_runtime.h>
 
 #include "bins.hpp"
@@ -247,9 +248,16 @@ __device__ inline SortElem make_elem(const T *coords, uint32_t idx, int order)
 }
 
 // ---------------------------------------------------------------------
-// Large bins: one workgroup per listed bin.  Runs of kSortRun elements are
-// bitonic-sorted in LDS; longer bins are then merged pairwise through a
-// global scratch along merge paths.
+// Large bins (more than kSmallBin points): a merge sort spread over every
+// workgroup.  The listed bins are cut into tiles of kSortRun elements;
+//  1. k_large_tiles scans the tile counts of the listed bins;
+//  2. k_large_runs bitonic-sorts every tile in LDS (a bin of one tile is
+//     then done and written back);
+//  3. k_large_merge, one launch per round of run length kSortRun << r,
+//     gives every output tile of a bin still being merged its two input
+//     ranges by a merge-path search, stages them in LDS and merges there;
+//  4. k_large_final copies the merged indices of multi-tile bins back.
+// Element keys are unique (the index breaks ties), so every merge is exact.
 // ---------------------------------------------------------------------
 __device__ inline void lds_bitonic(SortElem *s, int n2)
 {
@@ -271,7 +279,8 @@ __device__ inline void lds_bitonic(SortElem *s, int n2)
     }
 }
 
-// Index of the merge-path split on diagonal `diag` of runs a[0..na), b[0..nb).
+// Elements of a taken from the first `diag` outputs of merging runs
+// a[0..na) and b[0..nb) (ties cannot occur: keys are unique).
 __device__ inline int64_t merge_split(const SortElem *a, int64_t na, const SortElem *b,
                                       int64_t nb, int64_t diag)
 {
@@ -279,7 +288,6 @@ __device__ inline int64_t merge_split(const SortElem *a, int64_t na, const SortE
     int64_t hi = diag < na ? diag : na;
     while (lo < hi) {
         int64_t mid = (lo + hi) >> 1;
-        // take a[mid] before b[diag-1-mid] iff a[mid] < b[diag-1-mid]
         if (elem_less(a[mid], b[diag - 1 - mid]))
             lo = mid + 1;
         else
@@ -288,72 +296,170 @@ __device__ inline int64_t merge_split(const SortElem *a, int64_t na, const SortE
     return lo;
 }
 
+__device__ inline int merge_rounds(int64_t n)
+{
+    int r = 0;
+    for (int64_t len = kSortRun; len < n; len <<= 1) ++r;
+    return r;
+}
+
+// tile_off[i] = first tile of listed bin i; tile_off[nlarge] = all tiles.
+__global__ __launch_bounds__(1024) void k_large_tiles(const uint32_t *__restrict__ large_list,
+                                                      const uint32_t *__restrict__ large_count,
+                                                      const uint32_t *__restrict__ offsets,
+                                                      uint32_t *__restrict__ tile_off)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    const uint32_t nlarge = *large_count;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (uint32_t base = 0; base < nlarge; base += blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t t = 0;
+        if (i < nlarge) {
+            uint32_t bin = large_list[i];
+            uint32_t n = offsets[bin + 1] - offsets[bin];
+            t = (n + kSortRun - 1) / kSortRun;
+        }
+        uint32_t x = t;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        uint32_t pre = 0, all = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            if (w < wid) pre += wsum[w];
+            all += wsum[w];
+        }
+        if (i < nlarge) tile_off[i] = carry + pre + x - t;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += all;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tile_off[nlarge] = carry;
+}
+
+__device__ inline uint32_t tile_bin(const uint32_t *tile_off, uint32_t nlarge, uint32_t t)
+{
+    uint32_t lo = 0, hi = nlarge - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (tile_off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 template <int D, typename T>
-__global__ __launch_bounds__(kSortThreads) void k_large_sort(
+__global__ __launch_bounds__(kSortThreads) void k_large_runs(
     const uint32_t *__restrict__ large_list, const uint32_t *__restrict__ large_count,
-    const uint32_t *__restrict__ offsets, const T *__restrict__ coords, int order,
-    uint32_t *__restrict__ sorted, SortElem *__restrict__ scratch_a,
-    SortElem *__restrict__ scratch_b)
+    const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ offsets,
+    const T *__restrict__ coords, int order, uint32_t *__restrict__ sorted,
+    SortElem *__restrict__ scratch_a)
 {
     __shared__ SortElem s[kSortRun];
-    uint32_t nlarge = *large_count;
-    for (uint32_t lb = blockIdx.x; lb < nlarge; lb += gridDim.x) {
-        uint32_t bin = large_list[lb];
-        int64_t lo = offsets[bin], hi = offsets[bin + 1];
-        int64_t n = hi - lo;
-        SortElem *A = scratch_a + lo;
-        SortElem *B = scratch_b + lo;
-        // Sorted runs of kSortRun elements.
-        for (int64_t r0 = 0; r0 < n; r0 += kSortRun) {
-            int m = (int)(n - r0 < kSortRun ? n - r0 : kSortRun);
-            int n2 = 1;
-            while (n2 < m) n2 <<= 1;
-            for (int i = threadIdx.x; i < n2; i += kSortThreads) {
-                if (i < m)
-                    s[i] = make_elem<D, T>(coords, sorted[lo + r0 + i], order);
-                else
-                    s[i] = SortElem{0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
-            }
-            __syncthreads();
-            lds_bitonic(s, n2);
-            for (int i = threadIdx.x; i < m; i += kSortThreads) {
-                if (n <= kSortRun)
-                    sorted[lo + i] = s[i].idx;
-                else
-                    A[r0 + i] = s[i];
-            }
-            __syncthreads();
-        }
-        if (n <= kSortRun) continue;
-        // Pairwise merges of runs of length `len`.
-        for (int64_t len = kSortRun; len < n; len <<= 1) {
-            for (int64_t p0 = 0; p0 < n; p0 += 2 * len) {
-                int64_t na = n - p0 < len ? n - p0 : len;
-                int64_t nb = n - p0 - na;
-                if (nb < 0) nb = 0;
-                if (nb > len) nb = len;
-                const SortElem *a = A + p0;
-                const SortElem *b = a + na;
-                int64_t total = na + nb;
-                int64_t per = (total + kSortThreads - 1) / kSortThreads;
-                int64_t d0 = per * threadIdx.x;
-                if (d0 < total) {
-                    int64_t d1 = d0 + per < total ? d0 + per : total;
-                    int64_t ia = merge_split(a, na, b, nb, d0);
-                    int64_t ib = d0 - ia;
-                    for (int64_t o = d0; o < d1; ++o) {
-                        bool take_a = ib >= nb || (ia < na && !elem_less(b[ib], a[ia]));
-                        B[p0 + o] = take_a ? a[ia++] : b[ib++];
-                    }
-                }
-            }
-            __syncthreads();
-            SortElem *t = A;
-            A = B;
-            B = t;
-        }
-        for (int64_t i = threadIdx.x; i < n; i += kSortThreads) sorted[lo + i] = A[i].idx;
+    const uint32_t nlarge = *large_count;
+    if (nlarge == 0) return;
+    const uint32_t ntiles = tile_off[nlarge];
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t i = tile_bin(tile_off, nlarge, t);
+        const uint32_t bin = large_list[i];
+        const int64_t lo = offsets[bin];
+        const int64_t n = (int64_t)offsets[bin + 1] - lo;
+        const int64_t r0 = (int64_t)(t - tile_off[i]) * kSortRun;
+        const int m = (int)(n - r0 < kSortRun ? n - r0 : kSortRun);
+        int n2 = 1;
+        while (n2 < m) n2 <<= 1;
+        for (int k = threadIdx.x; k < n2; k += kSortThreads)
+            s[k] = k < m ? make_elem<D, T>(coords, sorted[lo + r0 + k], order)
+                         : SortElem{0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
         __syncthreads();
+        lds_bitonic(s, n2);
+        for (int k = threadIdx.x; k < m; k += kSortThreads) {
+            if (n <= kSortRun)
+                sorted[lo + k] = s[k].idx;
+            else
+                scratch_a[lo + r0 + k] = s[k];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_large_merge(
+    const uint32_t *__restrict__ large_list, const uint32_t *__restrict__ large_count,
+    const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ offsets, int round,
+    const SortElem *__restrict__ src, SortElem *__restrict__ dst)
+{
+    __shared__ SortElem s[kSortRun];
+    __shared__ int64_t split[2];
+    const uint32_t nlarge = *large_count;
+    if (nlarge == 0) return;
+    const uint32_t ntiles = tile_off[nlarge];
+    const int64_t len = (int64_t)kSortRun << round;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t i = tile_bin(tile_off, nlarge, t);
+        const uint32_t bin = large_list[i];
+        const int64_t lo = offsets[bin];
+        const int64_t n = (int64_t)offsets[bin + 1] - lo;
+        if (len >= n) continue;  // merged in an earlier round (uniform per tile)
+        const int64_t out0 = (int64_t)(t - tile_off[i]) * kSortRun;
+        const int64_t out1 = out0 + kSortRun < n ? out0 + kSortRun : n;
+        const int64_t ps = (out0 / (2 * len)) * (2 * len);
+        const int64_t na = n - ps < len ? n - ps : len;
+        int64_t nb = n - ps - na;
+        nb = nb < 0 ? 0 : (nb > len ? len : nb);
+        const SortElem *a = src + lo + ps;
+        const SortElem *b = a + na;
+        const int64_t d0 = out0 - ps, d1 = out1 - ps;
+        if (threadIdx.x < 2) split[threadIdx.x] = merge_split(a, na, b, nb, threadIdx.x ? d1 : d0);
+        __syncthreads();
+        const int64_t ia0 = split[0], ia1 = split[1];
+        const int la = (int)(ia1 - ia0), lb = (int)((d1 - ia1) - (d0 - ia0));
+        for (int k = threadIdx.x; k < la + lb; k += kSortThreads)
+            s[k] = k < la ? a[ia0 + k] : b[(d0 - ia0) + (k - la)];
+        __syncthreads();
+        constexpr int per = kSortRun / kSortThreads;
+        const int o0 = threadIdx.x * per;
+        const int total = la + lb;
+        if (o0 < total) {
+            const int o1 = o0 + per < total ? o0 + per : total;
+            int ia = (int)merge_split(s, la, s + la, lb, o0);
+            int ib = o0 - ia;
+            for (int o = o0; o < o1; ++o) {
+                bool take_a = ib >= lb || (ia < la && elem_less(s[ia], s[la + ib]));
+                dst[lo + out0 + o] = take_a ? s[ia++] : s[la + ib++];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_large_final(
+    const uint32_t *__restrict__ large_list, const uint32_t *__restrict__ large_count,
+    const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ offsets,
+    const SortElem *__restrict__ scratch_a, const SortElem *__restrict__ scratch_b,
+    uint32_t *__restrict__ sorted)
+{
+    const uint32_t nlarge = *large_count;
+    if (nlarge == 0) return;
+    const uint32_t ntiles = tile_off[nlarge];
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t i = tile_bin(tile_off, nlarge, t);
+        const uint32_t bin = large_list[i];
+        const int64_t lo = offsets[bin];
+        const int64_t n = (int64_t)offsets[bin + 1] - lo;
+        if (n <= kSortRun) continue;
+        const SortElem *src = (merge_rounds(n) & 1) ? scratch_b : scratch_a;
+        const int64_t out0 = (int64_t)(t - tile_off[i]) * kSortRun;
+        const int64_t out1 = out0 + kSortRun < n ? out0 + kSortRun : n;
+        for (int64_t o = out0 + threadIdx.x; o < out1; o += kSortThreads)
+            sorted[lo + o] = src[lo + o].idx;
     }
 }
 
@@ -433,10 +539,10 @@ BinSorter::~BinSorter() { release(); }
 void BinSorter::release()
 {
     for (void *p : {(void *)counts_, (void *)offsets_, (void *)keys_, (void *)ranks_,
-                    (void *)sorted_, (void *)large_list_, (void *)tile_sums_,
+                    (void *)sorted_, (void *)large_list_, (void *)tile_off_, (void *)tile_sums_,
                     (void *)scratch_a_, (void *)scratch_b_})
         if (p) (void)hipFree(p);
-    counts_ = offsets_ = keys_ = ranks_ = sorted_ = large_list_ = tile_sums_ = nullptr;
+    counts_ = offsets_ = keys_ = ranks_ = sorted_ = large_list_ = tile_off_ = tile_sums_ = nullptr;
     scratch_a_ = scratch_b_ = nullptr;
     bins_cap_ = points_cap_ = sort_scratch_cap_ = 0;
 }
@@ -456,7 +562,7 @@ bool BinSorter::ensure(int64_t nbins, int64_t M, bool need_sort_scratch)
         bins_cap_ = nbins;
     }
     if (M > points_cap_) {
-        for (uint32_t **p : {&keys_, &ranks_, &sorted_, &large_list_}) {
+        for (uint32_t **p : {&keys_, &ranks_, &sorted_, &large_list_, &tile_off_}) {
             if (*p) (void)hipFree(*p);
             *p = nullptr;
         }
@@ -466,6 +572,9 @@ bool BinSorter::ensure(int64_t nbins, int64_t M, bool need_sort_scratch)
         if (!hip_ok(hipMalloc(&sorted_, sizeof(uint32_t) * cap), "sorted points")) return false;
         if (!hip_ok(hipMalloc(&large_list_, sizeof(uint32_t) * (cap / (kSmallBin + 1) + 2)),
                     "large-bin list"))
+            return false;
+        if (!hip_ok(hipMalloc(&tile_off_, sizeof(uint32_t) * (cap / (kSmallBin + 1) + 3)),
+                    "large-bin tiles"))
             return false;
         points_cap_ = cap;
     }
@@ -510,10 +619,22 @@ bool BinSorter::sort(int64_t M, const T *coords, const GridParams &g, BinOrder o
     k_scan_down<<<(unsigned)tiles, kScanThreads, 0, stream>>>(counts_, n, tile_sums_, offsets_);
     k_scatter<<<grid_for(M, 256), 256, 0, stream>>>(M, keys_, ranks_, offsets_, sorted_);
     if (order != BinOrder::Grouped) {
-        unsigned grid = (unsigned)std::min<int64_t>(1024, M / (kSmallBin + 1) + 1);
-        k_large_sort<D, T><<<grid, kSortThreads, 0, stream>>>(
-            large_list_, large_count, offsets_, coords, (int)order, sorted_,
-            (SortElem *)scratch_a_, (SortElem *)scratch_b_);
+        unsigned grid = (unsigned)std::min<int64_t>(2048, M / kSortRun + 64);
+        SortElem *A = (SortElem *)scratch_a_, *B = (SortElem *)scratch_b_;
+        k_large_tiles<<<1, 1024, 0, stream>>>(large_list_, large_count, offsets_, tile_off_);
+        k_large_runs<D, T><<<grid, kSortThreads, 0, stream>>>(large_list_, large_count, tile_off_,
+                                                              offsets_, coords, (int)order,
+                                                              sorted_, A);
+        // rounds for the largest possible bin; bins done earlier skip
+        int rounds = 0;
+        for (int64_t len = kSortRun; len < M; len <<= 1) ++rounds;
+        for (int r = 0; r < rounds; ++r)
+            k_large_merge<<<grid, kSortThreads, 0, stream>>>(large_list_, large_count, tile_off_,
+                                                             offsets_, r, (r & 1) ? B : A,
+                                                             (r & 1) ? A : B);
+        if (rounds > 0)
+            k_large_final<<<grid, kSortThreads, 0, stream>>>(large_list_, large_count, tile_off_,
+                                                             offsets_, A, B, sorted_);
     }
     return hip_ok(hipGetLastError(), "bin sort launch");
 }

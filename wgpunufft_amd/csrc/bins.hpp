@@ -39,6 +39,7 @@ private:
     uint32_t *ranks_ = nullptr;
     uint32_t *sorted_ = nullptr;
     uint32_t *large_list_ = nullptr;
+    uint32_t *tile_off_ = nullptr;
     uint32_t *tile_sums_ = nullptr;
     void *scratch_a_ = nullptr;
     void *scratch_b_ = nullptr;

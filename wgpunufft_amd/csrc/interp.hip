@@ -122,6 +122,155 @@ __global__ __launch_bounds__(256) void k_interp(int64_t M, int64_t vectors,
     }
 }
 
+// ---------------------------------------------------------------------
+// LDS-tiled 3D interpolation.  Type-2 bins here are tiles of TB^3 cells of
+// support starts; a work item is one tile and up to kTilePoints of its
+// points.  The workgroup stages the tile plus its (w-1)-cell halo (periodic)
+// in LDS once, then every lane evaluates whole points from LDS: w^3 LDS reads
+// and 2 w^3 FMAs per point, with the fine grid read from HBM about
+// (TB+w-1)^3/TB^3 times per item instead of w^3 times per point.  Each
+// output is one lane's fixed-order sum.
+// ---------------------------------------------------------------------
+constexpr int kTileEdge = 8;
+constexpr int kTilePoints = 1024;
+constexpr int kTileThreads = 256;
+
+// item_off[b] = first item of bin b (exclusive scan of ceil(count / P)),
+// item_off[nbins] = items.
+__global__ __launch_bounds__(1024) void k_tile_items(int64_t nbins, const uint32_t *__restrict__ offsets,
+                                                     uint32_t *__restrict__ item_off)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int64_t base = 0; base < nbins; base += blockDim.x) {
+        int64_t b = base + threadIdx.x;
+        uint32_t t = 0;
+        if (b < nbins) {
+            uint32_t n = offsets[b + 1] - offsets[b];
+            t = (n + kTilePoints - 1) / kTilePoints;
+        }
+        uint32_t x = t;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        uint32_t pre = 0, all = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            if (w < wid) pre += wsum[w];
+            all += wsum[w];
+        }
+        if (b < nbins) item_off[b] = carry + pre + x - t;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += all;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) item_off[nbins] = carry;
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(kTileThreads) void k_interp_tiled3(
+    int64_t M, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
+    const uint32_t *__restrict__ item_off, const T *__restrict__ coords, GridParams g,
+    const T *__restrict__ grid, T *__restrict__ out)
+{
+    constexpr int E = kTileEdge + W - 1;
+    __shared__ T tre[E * E * E];
+    __shared__ T tim[E * E * E];
+    const int64_t nbins = g.nbins;
+    const int64_t n_items = item_off[nbins];
+    const int64_t v = blockIdx.y;
+    const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n, n2 = g.ax[2].n;
+    const T beta = (T)g.beta;
+    const T *gv = grid + 2 * v * g.fine_total;
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        int64_t lo = 0, hi = nbins - 1;
+        while (lo < hi) {
+            int64_t mid = (lo + hi + 1) >> 1;
+            if ((int64_t)item_off[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int64_t b = lo;
+        const int64_t k = item - item_off[b];
+        const int64_t bx = b % g.ax[0].bins, by = (b / g.ax[0].bins) % g.ax[1].bins,
+                      bz = b / (g.ax[0].bins * g.ax[1].bins);
+        const int64_t x0 = bx * kTileEdge, y0 = by * kTileEdge, z0 = bz * kTileEdge;
+        for (int t = threadIdx.x; t < E * E * E; t += kTileThreads) {
+            int lx = t % E, ly = (t / E) % E, lz = t / (E * E);
+            int64_t gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+            gx = gx >= n0 ? gx - n0 : gx;
+            gy = gy >= n1 ? gy - n1 : gy;
+            gz = gz >= n2 ? gz - n2 : gz;
+            const T *src = gv + 2 * (gx + n0 * (gy + n1 * gz));
+            tre[t] = src[0];
+            tim[t] = src[1];
+        }
+        __syncthreads();
+        const int64_t first = (int64_t)offsets[b] + k * kTilePoints;
+        int64_t last = first + kTilePoints;
+        if (last > (int64_t)offsets[b + 1]) last = offsets[b + 1];
+        for (int64_t s = first + threadIdx.x; s < last; s += kTileThreads) {
+            const uint32_t idx = sorted[s];
+            const bool reach = point_in_reach<3, T>(coords, idx);
+            int l[3];
+            T wt[3][W];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                int64_t start;
+                double off;
+                support_start<T>(coords[(int64_t)idx * 3 + a], reach, g, a, &start, &off);
+                int64_t ws = wrapped_start(start, g.ax[a].n);
+                int64_t o = a == 0 ? x0 : (a == 1 ? y0 : z0);
+                l[a] = (int)(ws - o);
+#pragma unroll
+                for (int j = 0; j < W; ++j) wt[a][j] = es_w<T>((T)j + (T)off, W, beta);
+            }
+            T sre = (T)0, sim = (T)0;
+#pragma unroll
+            for (int dz = 0; dz < W; ++dz) {
+                T pre = (T)0, pim = (T)0;
+#pragma unroll
+                for (int dy = 0; dy < W; ++dy) {
+                    const int rowb = ((l[2] + dz) * E + (l[1] + dy)) * E + l[0];
+                    T rre = (T)0, rim = (T)0;
+#pragma unroll
+                    for (int dx = 0; dx < W; ++dx) {
+                        rre += wt[0][dx] * tre[rowb + dx];
+                        rim += wt[0][dx] * tim[rowb + dx];
+                    }
+                    pre += wt[1][dy] * rre;
+                    pim += wt[1][dy] * rim;
+                }
+                sre += wt[2][dz] * pre;
+                sim += wt[2][dz] * pim;
+            }
+            T *dst = out + 2 * (v * M + (int64_t)idx);
+            dst[0] = sre;
+            dst[1] = sim;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int W>
+bool launch_tiled(int64_t M, int64_t vectors, const uint32_t *sorted, const uint32_t *offsets,
+                  uint32_t *item_off, const T *coords, const GridParams &g, const T *grid, T *out,
+                  hipStream_t stream)
+{
+    k_tile_items<<<1, 1024, 0, stream>>>(g.nbins, offsets, item_off);
+    unsigned items = (unsigned)std::min<int64_t>(g.nbins + M / kTilePoints + 64, 16384);
+    dim3 grid3(items, (unsigned)vectors);
+    k_interp_tiled3<T, W><<<grid3, kTileThreads, 0, stream>>>(M, sorted, offsets, item_off, coords,
+                                                              g, grid, out);
+    return hip_ok(hipGetLastError(), "tiled interpolation launch");
+}
+
 template <int D, typename T, int G>
 bool launch(int64_t M, int64_t vectors, const uint32_t *sorted, const T *coords,
             const GridParams &g, const T *grid, T *out, hipStream_t stream)
@@ -144,6 +293,30 @@ bool interpolate(int64_t M, int64_t vectors, const uint32_t *sorted, const T *co
     if (g.width <= 8) return launch<D, T, 8>(M, vectors, sorted, coords, g, grid, out, stream);
     return launch<D, T, 16>(M, vectors, sorted, coords, g, grid, out, stream);
 }
+
+template <typename T>
+bool interpolate_tiled3(int64_t M, int64_t vectors, const uint32_t *sorted, const uint32_t *offsets,
+                        uint32_t *item_off, const T *coords, const GridParams &g, const T *grid,
+                        T *out, hipStream_t stream)
+{
+    if (M == 0) return true;
+    switch (g.width) {
+    case 2: return launch_tiled<T, 2>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 3: return launch_tiled<T, 3>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 4: return launch_tiled<T, 4>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 5: return launch_tiled<T, 5>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 6: return launch_tiled<T, 6>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 7: return launch_tiled<T, 7>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 8: return launch_tiled<T, 8>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    default: set_error("tiled interpolation needs a kernel width <= 8"); return false;
+    }
+}
+template bool interpolate_tiled3<float>(int64_t, int64_t, const uint32_t *, const uint32_t *,
+                                        uint32_t *, const float *, const GridParams &,
+                                        const float *, float *, hipStream_t);
+template bool interpolate_tiled3<double>(int64_t, int64_t, const uint32_t *, const uint32_t *,
+                                         uint32_t *, const double *, const GridParams &,
+                                         const double *, double *, hipStream_t);
 
 #define MNUFFT_INTERP_INSTANTIATE(D, T)                                                           \
     template bool interpolate<D, T>(int64_t, int64_t, const uint32_t *, const T *,                 \

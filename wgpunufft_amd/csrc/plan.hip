@@ -106,7 +106,7 @@ FftPlan::~FftPlan()
 Plan::~Plan()
 {
     for (void *p : {(void *)d_inv_phihat, grid, (void *)rec_start, rec_off, c_sorted, t3_xr,
-                    t3_cp, t3_sr, t3_factor, t3_interp})
+                    t3_cp, t3_sr, t3_factor, t3_interp, (void *)item_off})
         if (p) (void)hipFree(p);
     for (auto &sl : slots)
         for (auto &e : sl.ev)
@@ -207,11 +207,19 @@ GridParams Plan::type1_geometry(int64_t M, int *chosen) const
     return grid_params(layout.geometries[gi].side);
 }
 
+bool Plan::tiled_interp() const
+{
+    if (dim != 3 || kernel.width > 8) return false;
+    for (int a = 0; a < 3; ++a)
+        if (fine[a] < 8 + kernel.width - 1) return false;
+    return true;
+}
+
 GridParams Plan::type2_geometry() const
 {
     int64_t side[kMaxDims] = {1, 1, 1};
     for (int a = 0; a < dim; ++a) {
-        int64_t s = dim == 3 ? 4 : dim == 2 ? 8 : 32;
+        int64_t s = dim == 3 ? (tiled_interp() ? 8 : 4) : dim == 2 ? 8 : 32;
         side[a] = std::min<int64_t>(s, pow2_ceil(fine[a]));
     }
     return grid_params(side);
@@ -256,6 +264,16 @@ bool Plan::ensure_records(int64_t M)
     if (!hip_ok(hipMalloc(&rec_start, sizeof(int4) * cap), "prepared starts")) return false;
     if (!hip_ok(hipMalloc(&rec_off, 4 * real_bytes() * cap), "prepared offsets")) return false;
     rec_cap = cap;
+    return true;
+}
+
+bool Plan::ensure_items(int64_t n)
+{
+    if (n <= item_cap) return true;
+    if (item_off) (void)hipFree(item_off);
+    item_off = nullptr;
+    if (!hip_ok(hipMalloc(&item_off, sizeof(uint32_t) * n), "interpolation items")) return false;
+    item_cap = n;
     return true;
 }
 
@@ -408,9 +426,15 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         if (!fft(active, grid, stream)) return false;
         mark(kStageFft, true, stream);
         mark(kStageInterp, false, stream);
-        if (!interpolate<D, T>(M, active, sorter.sorted(), (const T *)points, gp,
-                               (const T *)grid, out, stream))
+        if (D == 3 && tiled_interp()) {
+            if (!ensure_items(gp.nbins + 1)) return false;
+            if (!interpolate_tiled3<T>(M, active, sorter.sorted(), sorter.offsets(), item_off,
+                                       (const T *)points, gp, (const T *)grid, out, stream))
+                return false;
+        } else if (!interpolate<D, T>(M, active, sorter.sorted(), (const T *)points, gp,
+                                      (const T *)grid, out, stream)) {
             return false;
+        }
         mark(kStageInterp, true, stream);
     }
     return true;

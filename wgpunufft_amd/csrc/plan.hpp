@@ -62,6 +62,8 @@ struct Plan {
     int64_t rec_cap = 0;
     void *c_sorted = nullptr;
     size_t c_sorted_bytes = 0;
+    uint32_t *item_off = nullptr;  // work items of the tiled interpolation
+    int64_t item_cap = 0;
     std::map<int64_t, std::unique_ptr<FftPlan>> ffts;
 
     // type 3
@@ -97,6 +99,8 @@ struct Plan {
     bool ensure_grid(int64_t vectors);
     bool ensure_records(int64_t M);
     bool ensure_strengths(int64_t M, int64_t vectors);
+    bool ensure_items(int64_t n);
+    bool tiled_interp() const;
     FftPlan *fft_for(int64_t vectors);
     bool fft(int64_t vectors, void *data, hipStream_t stream);
     void mark(int stage, bool end, hipStream_t stream);

@@ -165,18 +165,36 @@ __global__ __launch_bounds__(1024) void k_plan(int64_t nblocks, const uint32_t *
         s_carry[0] = s_carry[1] = 0;
     }
     __syncthreads();
-    uint64_t mine = 0;
-    uint32_t mine_blocks = 0;
-    for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
-        uint32_t t = totals[b];
-        if (t > light_limit) {
-            mine += t;
-            ++mine_blocks;
-        }
-    }
-    atomicAdd((unsigned long long *)&s_heavy_records, (unsigned long long)mine);
-    atomicAdd(&s_heavy_blocks, mine_blocks);
+    // Raise the light limit until the heavy blocks fill at most half of the
+    // partial-sum slots, so every heavy block keeps room for several parts.
+    __shared__ uint32_t s_limit;
+    if (threadIdx.x == 0) s_limit = light_limit;
     __syncthreads();
+    for (;;) {
+        const uint32_t limit = s_limit;
+        uint64_t mine = 0;
+        uint32_t mine_blocks = 0;
+        for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
+            uint32_t t = totals[b];
+            if (t > limit) {
+                mine += t;
+                ++mine_blocks;
+            }
+        }
+        atomicAdd((unsigned long long *)&s_heavy_records, (unsigned long long)mine);
+        atomicAdd(&s_heavy_blocks, mine_blocks);
+        __syncthreads();
+        const bool fits = (int64_t)s_heavy_blocks * 2 <= part_capacity || limit >= 0x7fffffffu;
+        __syncthreads();
+        if (fits) break;
+        if (threadIdx.x == 0) {
+            s_limit = limit * 2;
+            s_heavy_records = 0;
+            s_heavy_blocks = 0;
+        }
+        __syncthreads();
+    }
+    light_limit = s_limit;
     if (threadIdx.x == 0) {
         uint64_t spare = part_capacity > (int64_t)s_heavy_blocks
                              ? (uint64_t)(part_capacity - s_heavy_blocks)
@@ -551,7 +569,8 @@ constexpr int kSweepTile = 16;  // tile edge of the row sweep along every axis
 
 template <typename T, int W>
 struct SweepRec {
-    int e;  // x offset of the support start from the tile's reach start
+    int e;    // x offset of the support start from the tile's reach start
+    int row;  // row of the tile's reach the record belongs to
     T cre, cim;
     T kx[W];
     T ky[W];
@@ -617,7 +636,7 @@ __global__ __launch_bounds__(kLanes) void k_sweep3(GridParams g, SpreadParams sp
     __shared__ T ring_im[W][TY][TX];
     __shared__ SweepRec<T, W> st[kLanes];
     __shared__ int64_t row_f0[kSweepRows], row_n0[kSweepRows], row_f1[kSweepRows],
-        row_n1[kSweepRows], row_excl[kSweepRows];
+        row_excl[kSweepRows];
 
     const int lane = threadIdx.x;
     const int ly = lane % W;
@@ -666,12 +685,78 @@ __global__ __launch_bounds__(kLanes) void k_sweep3(GridParams g, SpreadParams sp
         for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
         __syncthreads();
 
+        // Rows are walked in order; `cur` is the row being accumulated and
+        // `last_rec` the row of the last record added (both wave-uniform).
+        int cur = 0;
+        int last_rec = -(1 << 20);
+        int dy = 0;
+        auto start_row = [&](int r) {
+            const int ry = r % NRY;
+            if (ry == 0) {
+#pragma unroll
+                for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
+            }
+            const int64_t sy_rel = y0 - (W - 1) + ry;
+            int d = (int)((ly - sy_rel) % W);
+            dy = d < 0 ? d + W : d;
+        };
+        auto finish_row = [&](int r) {
+            const int ry = r % NRY, rz = r / NRY;
+            // the lane whose window cell leaves adds it to the ring (only
+            // when a record landed in the window)
+            if (r - last_rec < W && active && dy == 0) {
+                const int yy = ry - (W - 1);
+                const int zz = rz - (W - 1) + dz;
+                if (yy >= 0 && yy < TY && zz >= 0 && zz < TZ) {
+                    const int slot_z = zz % W;
+#pragma unroll
+                    for (int i = 0; i < TX; ++i) {
+                        ring_re[slot_z][yy][i] += ar[i];
+                        ring_im[slot_z][yy][i] += ai[i];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
+            }
+            if (ry == NRY - 1) {
+                // the plane of this sz is complete
+                __syncthreads();
+                const int zz = rz - (W - 1);
+                if (zz >= 0 && zz < TZ) {
+                    const int slot_z = zz % W;
+                    for (int t = lane; t < TY * TX; t += kLanes) {
+                        const int yy = t / TX, xx = t % TX;
+                        T re = ring_re[slot_z][yy][xx], im = ring_im[slot_z][yy][xx];
+                        ring_re[slot_z][yy][xx] = (T)0;
+                        ring_im[slot_z][yy][xx] = (T)0;
+                        if (np == 1) {
+                            const int64_t gx = x0 + xx, gy = y0 + yy, gz = z0 + zz;
+                            if (gx < n0 && gy < n1 && gz < n2) {
+                                T *o = grid + 2 * (v * g.fine_total + gx + n0 * (gy + n1 * gz));
+                                o[0] = re;
+                                o[1] = im;
+                            }
+                        } else {
+                            const int64_t pslot = (int64_t)heavy_base[b] + p;
+                            T *o = partial +
+                                   2 * (((pslot * gridDim.y + v) * TZ + zz) * (TY * TX) + t);
+                            o[0] = re;
+                            o[1] = im;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        };
+        start_row(0);
+
         int64_t cursor = 0;
-        for (int rb = 0; rb < NROWS; rb += kSweepRows) {
+        for (int rb = 0; rb < NROWS && cursor < rhi; rb += kSweepRows) {
+            int64_t batch_total;
             {
                 int r = rb + lane;
                 int64_t f0 = 0, c0 = 0, f1 = 0, c1 = 0;
-                if (r < NROWS && cursor < rhi) {
+                if (r < NROWS) {
                     int ry = r % NRY, rz = r / NRY;
                     int64_t sy = y0 - (W - 1) + ry;
                     int64_t sz = z0 - (W - 1) + rz;
@@ -694,123 +779,85 @@ __global__ __launch_bounds__(kLanes) void k_sweep3(GridParams g, SpreadParams sp
                     int64_t y = __shfl_up(xsum, o, kLanes);
                     if (lane >= o) xsum += y;
                 }
+                batch_total = __shfl(xsum, kLanes - 1, kLanes);
                 row_f0[lane] = f0;
                 row_n0[lane] = c0;
                 row_f1[lane] = f1;
-                row_n1[lane] = c1;
                 row_excl[lane] = xsum - cnt;
                 __syncthreads();
             }
-            const int nq = NROWS - rb < kSweepRows ? NROWS - rb : kSweepRows;
-            for (int q = 0; q < nq; ++q) {
-                const int r = rb + q;
-                const int ry = r % NRY, rz = r / NRY;
-                if (ry == 0) {
-#pragma unroll
-                    for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
+            const int64_t klo = rlo > cursor ? rlo - cursor : 0;
+            const int64_t khi = rhi - cursor < batch_total ? rhi - cursor : batch_total;
+            for (int64_t k0 = klo; k0 < khi; k0 += kLanes) {
+                const int64_t k = k0 + lane;
+                bool take = false;
+                int64_t slot = 0;
+                int e = 0, q = 0;
+                if (k < khi) {
+                    int s_lo = 0, s_hi = kSweepRows - 1;
+                    while (s_lo < s_hi) {
+                        int mid = (s_lo + s_hi + 1) >> 1;
+                        if (row_excl[mid] <= k)
+                            s_lo = mid;
+                        else
+                            s_hi = mid - 1;
+                    }
+                    q = s_lo;
+                    const int64_t j = k - row_excl[q];
+                    slot = j < row_n0[q] ? row_f0[q] + j : row_f1[q] + (j - row_n0[q]);
+                    int64_t ee = (int64_t)rec_start[slot].x - xs;
+                    if (ee < 0) ee += n0;
+                    e = (int)ee;
+                    take = ee < TX + W - 1;
                 }
-                const int64_t sy_rel = y0 - (W - 1) + ry;
-                int dy = (int)((ly - sy_rel) % W);
-                if (dy < 0) dy += W;
-                const int64_t row_pos = cursor + row_excl[q];
-                for (int k = 0; k < 2; ++k) {
-                    const int64_t f = k == 0 ? row_f0[q] : row_f1[q];
-                    const int64_t n = k == 0 ? row_n0[q] : row_n1[q];
-                    const int64_t pos0 = row_pos + (k == 0 ? 0 : row_n0[q]);
-                    int64_t jlo = rlo > pos0 ? rlo - pos0 : 0;
-                    int64_t jhi = rhi - pos0 < n ? rhi - pos0 : n;
-                    for (int64_t j0 = jlo; j0 < jhi; j0 += kLanes) {
-                        const int64_t j = j0 + lane;
-                        bool take = false;
-                        int64_t slot = 0;
-                        int e = 0;
-                        if (j < jhi) {
-                            slot = f + j;
-                            int64_t ee = (int64_t)rec_start[slot].x - xs;
-                            if (ee < 0) ee += n0;
-                            e = (int)ee;
-                            take = ee < TX + W - 1;
-                        }
-                        uint64_t mask = __ballot(take);
-                        int pos = __popcll(mask & ((1ull << lane) - 1ull));
-                        int nst = __popcll(mask);
-                        if (take) {
-                            SweepRec<T, W> &S = st[pos];
-                            S.e = e;
-                            S.cre = cv[2 * slot];
-                            S.cim = cv[2 * slot + 1];
-                            T ox = rec_off[4 * slot + 0];
-                            T oy = rec_off[4 * slot + 1];
-                            T oz = rec_off[4 * slot + 2];
+                const uint64_t mask = __ballot(take);
+                const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+                const int nst = __popcll(mask);
+                if (take) {
+                    SweepRec<T, W> &S = st[pos];
+                    S.e = e;
+                    S.row = rb + q;
+                    S.cre = cv[2 * slot];
+                    S.cim = cv[2 * slot + 1];
+                    const T ox = rec_off[4 * slot + 0];
+                    const T oy = rec_off[4 * slot + 1];
+                    const T oz = rec_off[4 * slot + 2];
 #pragma unroll
-                            for (int jj = 0; jj < W; ++jj) {
-                                S.kx[jj] = es_weight<T>((T)jj + ox, W, beta);
-                                S.ky[jj] = es_weight<T>((T)jj + oy, W, beta);
-                                S.kz[jj] = es_weight<T>((T)jj + oz, W, beta);
-                            }
-                        }
-                        __syncthreads();
-                        if (active) {
-                            for (int s2 = 0; s2 < nst; ++s2) {
-                                const SweepRec<T, W> &S = st[s2];
-                                T kx[W];
-#pragma unroll
-                                for (int jj = 0; jj < W; ++jj) kx[jj] = S.kx[jj];
-                                T wyz = S.ky[dy] * S.kz[dz];
-                                T vre = S.cre * wyz, vim = S.cim * wyz;
-                                sweep_dispatch<W, TX, T>(S.e, ar, ai, kx, vre, vim);
-                            }
-                        }
-                        __syncthreads();
+                    for (int jj = 0; jj < W; ++jj) {
+                        S.kx[jj] = es_weight<T>((T)jj + ox, W, beta);
+                        S.ky[jj] = es_weight<T>((T)jj + oy, W, beta);
+                        S.kz[jj] = es_weight<T>((T)jj + oz, W, beta);
                     }
                 }
-                // the lane whose window cell leaves adds it to the ring
-                if (active && dy == 0) {
-                    const int yy = ry - (W - 1);
-                    const int zz = rz - (W - 1) + dz;
-                    if (yy >= 0 && yy < TY && zz >= 0 && zz < TZ) {
-                        const int slot_z = zz % W;
-#pragma unroll
-                        for (int i = 0; i < TX; ++i) {
-                            ring_re[slot_z][yy][i] += ar[i];
-                            ring_im[slot_z][yy][i] += ai[i];
-                        }
+                __syncthreads();
+                for (int s2 = 0; s2 < nst; ++s2) {
+                    const SweepRec<T, W> &S = st[s2];
+                    const int rr = __builtin_amdgcn_readfirstlane(S.row);
+                    while (cur < rr) {
+                        finish_row(cur);
+                        ++cur;
+                        start_row(cur);
                     }
+                    last_rec = rr;
+                    if (active) {
+                        T kx[W];
 #pragma unroll
-                    for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
-                }
-                if (ry == NRY - 1) {
-                    // the plane of this sz is complete
-                    __syncthreads();
-                    const int zz = rz - (W - 1);
-                    if (zz >= 0 && zz < TZ) {
-                        const int slot_z = zz % W;
-                        for (int t = lane; t < TY * TX; t += kLanes) {
-                            const int yy = t / TX, xx = t % TX;
-                            T re = ring_re[slot_z][yy][xx], im = ring_im[slot_z][yy][xx];
-                            ring_re[slot_z][yy][xx] = (T)0;
-                            ring_im[slot_z][yy][xx] = (T)0;
-                            if (np == 1) {
-                                const int64_t gx = x0 + xx, gy = y0 + yy, gz = z0 + zz;
-                                if (gx < n0 && gy < n1 && gz < n2) {
-                                    T *o = grid + 2 * (v * g.fine_total + gx + n0 * (gy + n1 * gz));
-                                    o[0] = re;
-                                    o[1] = im;
-                                }
-                            } else {
-                                const int64_t pslot = (int64_t)heavy_base[b] + p;
-                                T *o = partial +
-                                       2 * (((pslot * gridDim.y + v) * TZ + zz) * (TY * TX) + t);
-                                o[0] = re;
-                                o[1] = im;
-                            }
-                        }
+                        for (int jj = 0; jj < W; ++jj) kx[jj] = S.kx[jj];
+                        const T wyz = S.ky[dy] * S.kz[dz];
+                        const T vre = S.cre * wyz, vim = S.cim * wyz;
+                        const int eu = __builtin_amdgcn_readfirstlane(S.e);
+                        sweep_dispatch<W, TX, T>(eu, ar, ai, kx, vre, vim);
                     }
-                    __syncthreads();
                 }
+                __syncthreads();
             }
-            cursor += row_excl[kLanes - 1] + row_n0[kLanes - 1] + row_n1[kLanes - 1];
+            cursor += batch_total;
             __syncthreads();
+        }
+        while (cur < NROWS) {
+            finish_row(cur);
+            ++cur;
+            if (cur < NROWS) start_row(cur);
         }
     }
 }
@@ -961,8 +1008,9 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
     const bool sweep = sp.run == 0;
     int64_t cells = sweep ? (int64_t)kSweepTile * kSweepTile * kSweepTile : (int64_t)kLanes * sp.run;
     int64_t per_part = cells * vectors * 2 * (int64_t)sizeof(T);
-    // part slots: the scratch budget, at least one per block
-    part_capacity = std::max<int64_t>(budget_bytes / per_part, sp.block_count + 1);
+    // part slots: the scratch budget (the plan pass raises the light limit
+    // until the heavy blocks fit)
+    part_capacity = std::max<int64_t>(budget_bytes / per_part, 64);
     if (!ensure(sp.block_count, part_capacity * per_part)) return false;
     if (sweep) {
         if constexpr (D == 3) {
