@@ -9,8 +9,11 @@ transform includes its own point binning (encode = set_points + execute),
 the rocFFT fine-grid FFT and the (pre)deconvolution.  Inputs are resident
 in HBM before timing starts.
 
-value = M / (seconds per step): points per second through both transforms.
-The roofline object reports the dominant kernel of the path, from HIP events
+value = M / (seconds per step): points per second through both transforms,
+with the type-1 spread on the matrix cores (FP16 hi/lo split products,
+FP32 accumulation); the same step with the plain FP32 spreader is timed
+beside it ("plain").  The roofline object reports the dominant kernel of the
+path, from HIP events
 around that kernel over the timed region; cpu_baseline times the CPU port of
 the reference's CpuNufftPlan on a bounded sample of the same workload.
 """
@@ -27,6 +30,7 @@ if ROOT not in sys.path:
 METRIC = "NU points/s for 3D type-1/2 at tol 1e-6, MFMA vs plain GPU; % HBM/MFMA roofline"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+FP16_MFMA_PEAK_TFLOPS = 2500.0
 
 
 def parse():
@@ -41,6 +45,10 @@ def parse():
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
     ap.add_argument("--cpu-points", type=int, default=3_000_000,
                     help="points of the CPU baseline sample")
+    ap.add_argument("--spread", default="mfma_f16x3",
+                    choices=["mfma_f16x3", "mfma_bf16x3", "mfma_f16"],
+                    help="matrix-core spread of the timed type-1 (the plain FP32 spread is "
+                         "timed beside it)")
     ap.add_argument("--config", type=int, default=2,
                     help="2: the headline workload (default); 1, 3, 4, 5: the other "
                          "configurations, reported as extra JSON lines")
@@ -75,77 +83,91 @@ def main():
     ct = torch.from_numpy(c).to(dev)
     ft = torch.from_numpy(fm).to(dev)
     f_out = torch.empty(N ** 3, dtype=torch.complex64, device=dev)
+    f_plain = torch.empty(N ** 3, dtype=torch.complex64, device=dev)
     c_out = torch.empty(M, dtype=torch.complex64, device=dev)
     cfg = W.NufftConfig(n_modes, args.eps)
     p1 = W.NufftPlan.type1_gpu(cfg)
+    p1m = W.NufftPlan.type1_gpu(cfg).set_spread_method(args.spread)
     p2 = W.NufftPlan.type2_gpu(cfg)
     print(f"# data ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
-    def step():
-        p1.encode_gpu(xt, ct, out=f_out)
-        p2.encode_gpu(xt, ft, out=c_out)
+    def leg(t1_plan, f_dst):
+        """Times K steps (type 1 through t1_plan, then type 2); returns ms per
+        step and the per-stage milliseconds averaged over the K steps."""
+        def step():
+            t1_plan.encode_gpu(xt, ct, out=f_dst)
+            p2.encode_gpu(xt, ft, out=c_out)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    p1.timing_history(True)
-    p2.timing_history(True)
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t_end = time.perf_counter()
-    st1 = p1.stage_times()
-    st2 = p2.stage_times()
-    p1.timing_history(False)
-    p2.timing_history(False)
-    K = args.steps
-    ms_step = (t_end - t_start) * 1e3 / K
-    avg = lambda v: None if v is None else v / K  # noqa: E731
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t1_plan.timing_history(True)
+        p2.timing_history(True)
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        t_end = time.perf_counter()
+        st1, st2 = t1_plan.stage_times(), p2.stage_times()
+        t1_plan.timing_history(False)
+        p2.timing_history(False)
+        K = args.steps
+        stages = {"type1": {k: (None if v is None else round(v / K, 4)) for k, v in st1.items()},
+                  "type2": {k: (None if v is None else round(v / K, 4)) for k, v in st2.items()}}
+        return (t_end - t_start) * 1e3 / K, stages
+
+    ms_plain, stages_plain = leg(p1, f_plain)
+    print(f"# plain leg {ms_plain:.3f} ms/step", file=sys.stderr, flush=True)
+    ms_mfma, stages_mfma = leg(p1m, f_out)
+    print(f"# {args.spread} leg {ms_mfma:.3f} ms/step", file=sys.stderr, flush=True)
 
     info1 = p1.info()
     w = info1["width"]
     nf = int(np.prod(info1["fine"]))
+    sweep = info1["run"] == 0
+    tiled = p2.info()["bin_side"] == [8, 8, 8]
     # algorithmic work per launch (one vector)
     spread_flops = 4.0 * w ** 3 * M
     interp_bytes = M * (4 * 3 + 8) + 8 * nf
-    t_spread_ms = avg(st1["spread_kernel"])
-    t_interp_ms = avg(st2["interp"])
-    spread_tflops = spread_flops / (t_spread_ms * 1e-3) / 1e12
-    interp_gbs = interp_bytes / (t_interp_ms * 1e-3) / 1e9
-    sweep = info1["run"] == 0
-    tiled = p2.info()["bin_side"] == [8, 8, 8]
-    spread_name = ("k_sweep3 (type-1 3D row-sweep spread)" if sweep
-                   else "k_spread (type-1 output-stationary spread)")
-    interp_name = ("k_interp_tiled3 (type-2 LDS-tiled interpolation)" if tiled
-                   else "k_interp (type-2 interpolation)")
-    roof_spread = {"kernel": spread_name, "bound": "mfma",
-                   "achieved": round(spread_tflops, 3), "peak": FP32_PEAK_TFLOPS,
-                   "unit": "TFLOP/s", "frac": round(spread_tflops / FP32_PEAK_TFLOPS, 4),
-                   "traffic": None, "avg_ms": round(t_spread_ms, 4),
-                   "algorithmic": f"4*w^3*M = {spread_flops:.4g} flop (w={w}, M={M})"}
-    roof_interp = {"kernel": interp_name, "bound": "hbm",
+
+    def spread_roof(t_ms, name, peak, unit_note):
+        tf = spread_flops / (t_ms * 1e-3) / 1e12
+        return {"kernel": name, "bound": "mfma", "achieved": round(tf, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(tf / peak, 5), "traffic": None,
+                "avg_ms": round(t_ms, 4),
+                "algorithmic": f"4*w^3*M = {spread_flops:.4g} flop (w={w}, M={M}); {unit_note}"}
+
+    t_mfma = stages_mfma["type1"]["spread_kernel"]
+    t_plain = stages_plain["type1"]["spread_kernel"]
+    t_interp = stages_mfma["type2"]["interp"]
+    roof_mfma = spread_roof(t_mfma, f"k_spread_mfma3 (type-1 16^3-tile contraction, {args.spread})",
+                            FP16_MFMA_PEAK_TFLOPS, "peak: dense FP16 matrix cores")
+    roof_plain = spread_roof(t_plain,
+                             "k_sweep3 (type-1 3D row-sweep spread)" if sweep
+                             else "k_spread (type-1 output-stationary spread)",
+                             FP32_PEAK_TFLOPS, "peak: FP32")
+    interp_gbs = interp_bytes / (t_interp * 1e-3) / 1e9
+    roof_interp = {"kernel": "k_interp_tiled3 (type-2 LDS-tiled interpolation)" if tiled
+                   else "k_interp (type-2 interpolation)", "bound": "hbm",
                    "achieved": round(interp_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(interp_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                   "avg_ms": round(t_interp_ms, 4),
+                   "avg_ms": round(t_interp, 4),
                    "algorithmic": f"M*(4d+8)+8*N_f = {interp_bytes:.4g} B"}
-    dominant, other = (roof_spread, roof_interp) if t_spread_ms >= t_interp_ms else \
-        (roof_interp, roof_spread)
+    dominant, other = (roof_mfma, roof_interp) if t_mfma >= t_interp else (roof_interp, roof_mfma)
 
-    stages = {"type1": {k: (None if v is None else round(v / K, 4)) for k, v in st1.items()},
-              "type2": {k: (None if v is None else round(v / K, 4)) for k, v in st2.items()}}
-    t1_ms = sum(v for k, v in stages["type1"].items() if v is not None and k != "spread_kernel")
-    t2_ms = sum(v for v in stages["type2"].values() if v is not None)
+    def pts(stages, key):
+        ms = sum(v for k, v in stages[key].items() if v is not None and k != "spread_kernel")
+        return M / (ms * 1e-3) if ms else None
 
     result = {
         "metric": METRIC,
-        "value": M / (ms_step * 1e-3),
+        "value": M / (ms_mfma * 1e-3),
         "unit": "points/s",
         "n_gpus": 1,
-        "steps": K,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(ms_mfma, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -155,17 +177,26 @@ def main():
                                f"M={M} (half uniform, half Gaussian sigma 0.05 cluster)",
                    "n_modes": n_modes, "fine": info1["fine"], "width": w,
                    "points": M, "bin_geometry_t1": info1["bin_side"],
-                   "spreader": "plain FP32 row sweep" if sweep else "plain FP32 output-stationary",
+                   "spreader": f"{args.spread} matrix-core tiles (type 1)",
                    "interp": "LDS-tiled" if tiled else "per-point gather"},
         "roofline": dominant,
-        "roofline_other": other,
-        "stages_ms": stages,
-        "type1_points_per_s": M / (t1_ms * 1e-3) if t1_ms else None,
-        "type2_points_per_s": M / (t2_ms * 1e-3) if t2_ms else None,
+        "roofline_other": [other, roof_plain],
+        "stages_ms": stages_mfma,
+        "type1_points_per_s": pts(stages_mfma, "type1"),
+        "type2_points_per_s": pts(stages_mfma, "type2"),
+        "plain": {"value": M / (ms_plain * 1e-3), "ms_per_step": round(ms_plain, 4),
+                  "spreader": "plain FP32 row sweep" if sweep else "plain FP32 output-stationary",
+                  "stages_ms": stages_plain,
+                  "type1_points_per_s": pts(stages_plain, "type1")},
+        "mfma_vs_plain": {"spread_kernel": round(t_plain / t_mfma, 3),
+                          "type1": round(pts(stages_mfma, "type1") / pts(stages_plain, "type1"), 3),
+                          "step": round(ms_plain / ms_mfma, 3)},
     }
 
     if not args.no_check:
         result["error"] = accuracy(W, S, np, torch, dev, x, c, fm, n_modes, f_out, c_out)
+        result["plain"]["error"] = accuracy(W, S, np, torch, dev, x, c, fm, n_modes, f_plain,
+                                            c_out)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, np, S, n_modes)
     print(json.dumps(result), flush=True)

@@ -153,6 +153,23 @@ int mnufft_stage_times(mnufft_plan plan, double *ms, int32_t n);
 int mnufft_timing_history(mnufft_plan plan, int32_t on);
 
 /*
+ * Arithmetic of the plan's type-1 spread (no counterpart in the reference,
+ * whose spread is plain FP32/FP64 arithmetic): MNUFFT_SPREAD_PLAIN (the
+ * default), or the 16^3-tile contraction on the matrix cores with FP16
+ * operands (MNUFFT_SPREAD_MFMA_F16, about 1e-3 relative), FP16 hi/lo splits
+ * (MNUFFT_SPREAD_MFMA_F16X3, about 1e-6) or bfloat16 hi/lo splits
+ * (MNUFFT_SPREAD_MFMA_BF16X3, about 1e-5), accumulated in FP32.  The matrix
+ * methods need an FP32 3D plan whose fine grid has at least 16 + width cells
+ * per axis and a width <= 8; otherwise MNUFFT_ERROR_INVALID_ARGUMENT.  A
+ * type-3 plan applies the method to its outer (source) grid.
+ */
+#define MNUFFT_SPREAD_PLAIN 0
+#define MNUFFT_SPREAD_MFMA_F16 1
+#define MNUFFT_SPREAD_MFMA_F16X3 2
+#define MNUFFT_SPREAD_MFMA_BF16X3 3
+int mnufft_set_spread_method(mnufft_plan plan, int32_t method);
+
+/*
  * FP64 direct sums on the GPU: reference_type1_f64 / reference_type2_f64
  * (src/direct.rs:9-118) and reference_type3_f64 (src/type3.rs:609-673).
  * kind 1: out[o] = sum_j in[j] exp(i sign k_o.x_j) over the linear mode

@@ -88,6 +88,9 @@ class NufftConfig:
         return n
 
 
+SPREAD_METHODS = {"plain": 0, "mfma_f16": 1, "mfma_f16x3": 2, "mfma_bf16x3": 3}
+
+
 class _PlanBase:
     _handle = None
 
@@ -115,6 +118,15 @@ class _PlanBase:
 
     def beta(self):
         return lib().mnufft_plan_beta(self._handle)
+
+    def set_spread_method(self, method):
+        """Type-1 spread arithmetic: "plain" (default), or the 16^3-tile
+        matrix-core contraction "mfma_f16" (FP16 operands, ~1e-3),
+        "mfma_f16x3" (FP16 hi/lo splits, ~1e-6) or "mfma_bf16x3" (bfloat16
+        hi/lo splits, ~1e-5); FP32 3D plans only."""
+        code = SPREAD_METHODS[method] if isinstance(method, str) else int(method)
+        check(lib().mnufft_set_spread_method(self._handle, code))
+        return self
 
     def timing_history(self, on=True):
         """Sum stage times over every call until the next stage_times()."""

@@ -48,6 +48,7 @@ SIGNATURES = [
     ("mnufft_plan_beta", ctypes.c_double, [_vp]),
     ("mnufft_stage_times", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
     ("mnufft_timing_history", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("mnufft_set_spread_method", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("mnufft_kernel_params", ctypes.c_int, [ctypes.c_double, ctypes.c_double,
                                             ctypes.POINTER(ctypes.c_int32),
                                             ctypes.POINTER(ctypes.c_double)]),

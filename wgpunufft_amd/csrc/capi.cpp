@@ -340,6 +340,28 @@ int mnufft_timing_history(mnufft_plan plan, int32_t on)
     return MNUFFT_SUCCESS;
 }
 
+int mnufft_set_spread_method(mnufft_plan plan, int32_t method)
+{
+    if (!plan) {
+        set_error("null plan");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    Plan &p = *plan->plan;
+    Plan &q = p.type == 3 && p.outer ? *p.outer : p;
+    if (method < MNUFFT_SPREAD_PLAIN || method > MNUFFT_SPREAD_MFMA_BF16X3) {
+        set_error("unknown spread method");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (method != MNUFFT_SPREAD_PLAIN &&
+        (q.type != 1 || q.dim != 3 || q.precision != Precision::F32 || !q.sweep)) {
+        set_error("matrix-core spreading needs an FP32 3D plan with at least 16 + width fine "
+                  "cells per axis and a width <= 8");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    q.spread_method = (mnufft::SpreadMethod)method;
+    return MNUFFT_SUCCESS;
+}
+
 int mnufft_direct_f64(int32_t kind, int32_t dim, const int64_t *n_modes, int32_t isign,
                       int32_t mode_order, int64_t M, const double *d_x, const double *d_in,
                       int64_t nout, const int64_t *d_sel, const double *d_s, double *d_out,

@@ -131,6 +131,17 @@ __global__ __launch_bounds__(256) void k_interp(int64_t M, int64_t vectors,
 // (TB+w-1)^3/TB^3 times per item instead of w^3 times per point.  Each
 // output is one lane's fixed-order sum.
 // ---------------------------------------------------------------------
+template <typename T>
+struct Complex2;
+template <>
+struct Complex2<float> {
+    using type = float2;
+};
+template <>
+struct Complex2<double> {
+    using type = double2;
+};
+
 constexpr int kTileEdge = 8;
 constexpr int kTilePoints = 1024;
 constexpr int kTileThreads = 256;
@@ -179,8 +190,8 @@ __global__ __launch_bounds__(kTileThreads) void k_interp_tiled3(
     const T *__restrict__ grid, T *__restrict__ out)
 {
     constexpr int E = kTileEdge + W - 1;
-    __shared__ T tre[E * E * E];
-    __shared__ T tim[E * E * E];
+    using T2 = typename Complex2<T>::type;
+    __shared__ T2 tile[E * E * E];
     const int64_t nbins = g.nbins;
     const int64_t n_items = item_off[nbins];
     const int64_t v = blockIdx.y;
@@ -207,9 +218,7 @@ __global__ __launch_bounds__(kTileThreads) void k_interp_tiled3(
             gx = gx >= n0 ? gx - n0 : gx;
             gy = gy >= n1 ? gy - n1 : gy;
             gz = gz >= n2 ? gz - n2 : gz;
-            const T *src = gv + 2 * (gx + n0 * (gy + n1 * gz));
-            tre[t] = src[0];
-            tim[t] = src[1];
+            tile[t] = *(const T2 *)(gv + 2 * (gx + n0 * (gy + n1 * gz)));
         }
         __syncthreads();
         const int64_t first = (int64_t)offsets[b] + k * kTilePoints;
@@ -241,8 +250,9 @@ __global__ __launch_bounds__(kTileThreads) void k_interp_tiled3(
                     T rre = (T)0, rim = (T)0;
 #pragma unroll
                     for (int dx = 0; dx < W; ++dx) {
-                        rre += wt[0][dx] * tre[rowb + dx];
-                        rim += wt[0][dx] * tim[rowb + dx];
+                        const T2 cv = tile[rowb + dx];
+                        rre += wt[0][dx] * cv.x;
+                        rim += wt[0][dx] * cv.y;
                     }
                     pre += wt[1][dy] * rre;
                     pim += wt[1][dy] * rim;

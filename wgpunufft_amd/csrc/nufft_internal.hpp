@@ -26,6 +26,11 @@ constexpr uint32_t kSmallBin = 64;
 
 enum class Precision : int { F64 = 1, Df64 = 2, F32 = 3 };
 enum class ModeOrder : int { Centered = 0, Fft = 1 };
+// Arithmetic of the type-1 spread: plain FP32/FP64 FMAs, or the per-tile
+// contraction on the matrix cores with FP16 inputs (one product), FP16
+// hi/lo splits (three products) or bfloat16 hi/lo splits (three products),
+// all accumulated in FP32.
+enum class SpreadMethod : int { Plain = 0, MfmaF16 = 1, MfmaF16x3 = 2, MfmaBf16x3 = 3 };
 // In-bin order of the sorted points.
 enum class BinOrder : int {
     Grouped = 0,    // whatever the atomic scatter left (type 2)

@@ -406,7 +406,8 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         if (timed) slots[slot].used[kStageSpreadKernel] = true;
         if (!gather_strengths<T>(M, active, rec_start, in, (T *)c_sorted, stream)) return false;
         if (!spreader.spread<D, T>(gp, sp, sorter.offsets(), rec_start, (const T *)rec_off,
-                                   (const T *)c_sorted, M, active, (T *)grid, stream))
+                                   (const T *)c_sorted, M, active, (T *)grid, stream,
+                                   spread_method))
             return false;
         mark(kStageSpread, true, stream);
         if (spread_only) return true;

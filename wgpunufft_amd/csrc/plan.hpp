@@ -44,6 +44,7 @@ struct Plan {
     SpreadLayout layout;
     SpreadParams sp{};
     bool sweep = false;  // 3D row-sweep spreader (else output-stationary blocks)
+    SpreadMethod spread_method = SpreadMethod::Plain;
 
     // points
     const void *points = nullptr;

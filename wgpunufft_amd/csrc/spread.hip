@@ -21,6 +21,7 @@
 
 #include "fold.hpp"
 #include "spread.hpp"
+#include "spread_device.hpp"
 
 namespace mnufft {
 
@@ -28,52 +29,6 @@ namespace {
 
 constexpr int kLanes = 64;
 constexpr int kKw = 16;  // weights per axis staged per record
-
-template <typename T>
-__device__ inline T es_weight(T d, int w, T beta)
-{
-    T s = (T)2 * d / (T)w;
-    T q = (T)1 - s * s;
-    if (!(q > (T)0)) return (T)0;
-    if constexpr (sizeof(T) == 4)
-        return expf(beta * (sqrtf(q) - 1.0f));
-    else
-        return exp(beta * (sqrt(q) - 1.0));
-}
-
-// Bin intervals of one axis that hold the support starts reaching a block
-// (starts b0-w+1 .. b0+B-1, periodic).  A reach within one bin side of the
-// whole axis takes every bin once.
-struct AxisReach {
-    int64_t lo0, len0, lo1, len1;
-    __device__ int64_t count() const { return len0 + len1; }
-    __device__ int64_t bin(int64_t k) const { return k < len0 ? lo0 + k : lo1 + (k - len0); }
-};
-
-__device__ inline AxisReach axis_reach(int64_t b0, int64_t B, int w, const AxisParams &a)
-{
-    AxisReach r{0, 0, 0, 0};
-    int64_t n = a.n;
-    int64_t L = B + w - 1 < n ? B + w - 1 : n;
-    int64_t side = (int64_t)1 << a.shift;
-    if (L + side > n) {
-        r.lo0 = 0;
-        r.len0 = a.bins;
-        return r;
-    }
-    int64_t r0 = b0 - (w - 1);
-    if (r0 < 0) r0 += n;
-    if (r0 + L <= n) {
-        r.lo0 = r0 >> a.shift;
-        r.len0 = ((r0 + L - 1) >> a.shift) - r.lo0 + 1;
-    } else {
-        r.lo0 = r0 >> a.shift;
-        r.len0 = a.bins - r.lo0;
-        r.lo1 = 0;
-        r.len1 = ((r0 + L - n - 1) >> a.shift) + 1;
-    }
-    return r;
-}
 
 template <int D>
 struct BlockReach {
@@ -124,22 +79,26 @@ __device__ inline BlockReach<D> block_reach(int64_t block, const GridParams &g,
     return r;
 }
 
-// Records in reach of every block.
+// Records in reach of every block: one wavefront per block, lanes over the
+// block's bin segments.
 template <int D>
 __global__ __launch_bounds__(256) void k_classify(GridParams g, SpreadParams sp,
                                                   const uint32_t *__restrict__ offsets,
                                                   uint32_t *__restrict__ totals)
 {
-    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (b >= sp.block_count) return;
     BlockReach<D> r = block_reach<D>(b, g, sp);
     uint64_t total = 0;
-    for (int64_t t = 0; t < r.nseg; ++t) {
+    for (int64_t t = lane; t < r.nseg; t += 64) {
         int64_t f, l;
         r.segment(t, g, offsets, &f, &l);
         total += (uint64_t)(l - f);
     }
-    totals[b] = (uint32_t)(total > 0xffffffffull ? 0xffffffffull : total);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o, 64);
+    if (lane == 0) totals[b] = (uint32_t)(total > 0xffffffffull ? 0xffffffffull : total);
 }
 
 // One workgroup plans the work items: a light block is one item, a heavy
@@ -485,7 +444,51 @@ __global__ __launch_bounds__(kLanes) void k_spread(GridParams g, SpreadParams sp
     }
 }
 
-// Heavy blocks: each cell adds its partial sums in part order.
+// Sums of the partial slots of heavy blocks.  Sixteen lanes share a cell,
+// lane l adding parts l, l + 16, ...; the sixteen lane sums are then added in
+// a fixed pairwise order, so a cell's sum does not depend on timing.  A
+// workgroup of 256 takes 16 consecutive (heavy block, cell) units.
+constexpr int kPartLanes = 16;
+
+template <typename T>
+struct PartSums {
+    T re[kPartLanes][kPartLanes + 1];
+    T im[kPartLanes][kPartLanes + 1];
+};
+
+template <typename T>
+__device__ inline void part_lane_sum(const T *__restrict__ partial, int64_t base, int64_t np,
+                                     int64_t vectors, int64_t v, int64_t cells, int64_t c, int pl,
+                                     T &re, T &im)
+{
+    for (int64_t q = pl; q < np; q += kPartLanes) {
+        const T *src = partial + 2 * (((base + q) * vectors + v) * cells + c);
+        re += src[0];
+        im += src[1];
+    }
+}
+
+template <typename T>
+__device__ inline void combine_lanes(const PartSums<T> &s, int u, T &re, T &im)
+{
+    T r[kPartLanes], i[kPartLanes];
+#pragma unroll
+    for (int k = 0; k < kPartLanes; ++k) {
+        r[k] = s.re[k][u];
+        i[k] = s.im[k][u];
+    }
+#pragma unroll
+    for (int w = kPartLanes / 2; w >= 1; w >>= 1) {
+#pragma unroll
+        for (int k = 0; k < w; ++k) {
+            r[k] += r[k + w];
+            i[k] += i[k + w];
+        }
+    }
+    re = r[0];
+    im = i[0];
+}
+
 template <int D, typename T, int R>
 __global__ __launch_bounds__(256) void k_reduce(GridParams g, SpreadParams sp,
                                                 const uint32_t *__restrict__ parts,
@@ -495,54 +498,62 @@ __global__ __launch_bounds__(256) void k_reduce(GridParams g, SpreadParams sp,
                                                 const T *__restrict__ partial, T *__restrict__ grid,
                                                 int64_t block_cells)
 {
+    __shared__ PartSums<T> ps;
     const int64_t v = blockIdx.y;
     const int64_t vectors = gridDim.y;
     const int64_t nheavy = ctrl[2];
     const int64_t work = nheavy * block_cells;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < work;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        int64_t hb = t / block_cells;
-        int64_t c = t - hb * block_cells;  // lane * R + i
-        int64_t b = heavy_list[hb];
-        int64_t np = parts[b];
-        int64_t base = heavy_base[b];
+    const int u = threadIdx.x % kPartLanes, pl = threadIdx.x / kPartLanes;
+    for (int64_t t0 = (int64_t)blockIdx.x * kPartLanes; t0 < work;
+         t0 += (int64_t)gridDim.x * kPartLanes) {
+        const int64_t t = t0 + u;
+        int64_t hb = 0, c = 0, b = 0;
         T sre = (T)0, sim = (T)0;
-        for (int64_t p = 0; p < np; ++p) {
-            const T *src = partial + 2 * (((base + p) * vectors + v) * block_cells + c);
-            sre += src[0];
-            sim += src[1];
+        if (t < work) {
+            hb = t / block_cells;
+            c = t - hb * block_cells;  // lane * R + i
+            b = heavy_list[hb];
+            part_lane_sum(partial, (int64_t)heavy_base[b], (int64_t)parts[b], vectors, v,
+                          block_cells, c, pl, sre, sim);
         }
-        // cell coordinates
-        int lane = (int)(c / R), i = (int)(c % R);
-        int lane_x = lane % sp.groups0;
-        int rl = lane / sp.groups0;
-        int64_t rest = b;
-        int64_t cell[3] = {0, 0, 0};
+        ps.re[pl][u] = sre;
+        ps.im[pl][u] = sim;
+        __syncthreads();
+        if (pl == 0 && t < work) {
+            combine_lanes(ps, u, sre, sim);
+            // cell coordinates
+            int lane = (int)(c / R), i = (int)(c % R);
+            int lane_x = lane % sp.groups0;
+            int rl = lane / sp.groups0;
+            int64_t rest = b;
+            int64_t cell[3] = {0, 0, 0};
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            int64_t bi = rest % sp.nblocks[a];
-            rest /= sp.nblocks[a];
-            cell[a] = bi * sp.block[a];
-        }
-        cell[0] += lane_x * R + i;
+            for (int a = 0; a < D; ++a) {
+                int64_t bi = rest % sp.nblocks[a];
+                rest /= sp.nblocks[a];
+                cell[a] = bi * sp.block[a];
+            }
+            cell[0] += lane_x * R + i;
 #pragma unroll
-        for (int a = 1; a < D; ++a) {
-            cell[a] += rl % (int)sp.block[a];
-            rl /= (int)sp.block[a];
-        }
-        bool inside = true;
-        int64_t lin = 0, stride = 1;
+            for (int a = 1; a < D; ++a) {
+                cell[a] += rl % (int)sp.block[a];
+                rl /= (int)sp.block[a];
+            }
+            bool inside = true;
+            int64_t lin = 0, stride = 1;
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            inside = inside && cell[a] < g.ax[a].n;
-            lin += cell[a] * stride;
-            stride *= g.ax[a].n;
+            for (int a = 0; a < D; ++a) {
+                inside = inside && cell[a] < g.ax[a].n;
+                lin += cell[a] * stride;
+                stride *= g.ax[a].n;
+            }
+            if (inside) {
+                T *out = grid + 2 * (v * g.fine_total + lin);
+                out[0] = sre;
+                out[1] = sim;
+            }
         }
-        if (inside) {
-            T *out = grid + 2 * (v * g.fine_total + lin);
-            out[0] = sre;
-            out[1] = sim;
-        }
+        __syncthreads();
     }
 }
 
@@ -862,8 +873,8 @@ __global__ __launch_bounds__(kLanes) void k_sweep3(GridParams g, SpreadParams sp
     }
 }
 
-// Heavy tiles of the row sweep: every cell adds its partial sums in part
-// order.
+// Heavy tiles of the row sweep and the matrix-core tiles: partial sums as in
+// k_reduce.
 template <typename T, int TX, int TY, int TZ>
 __global__ __launch_bounds__(256) void k_reduce_tiles(GridParams g, SpreadParams sp,
                                                       const uint32_t *__restrict__ parts,
@@ -874,33 +885,41 @@ __global__ __launch_bounds__(256) void k_reduce_tiles(GridParams g, SpreadParams
                                                       T *__restrict__ grid)
 {
     constexpr int64_t cells = (int64_t)TX * TY * TZ;
+    __shared__ PartSums<T> ps;
     const int64_t v = blockIdx.y;
     const int64_t vectors = gridDim.y;
     const int64_t work = (int64_t)ctrl[2] * cells;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < work;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t hb = t / cells;
-        const int64_t c = t - hb * cells;
-        const int64_t b = heavy_list[hb];
-        const int64_t np = parts[b];
-        const int64_t base = heavy_base[b];
+    const int u = threadIdx.x % kPartLanes, pl = threadIdx.x / kPartLanes;
+    for (int64_t t0 = (int64_t)blockIdx.x * kPartLanes; t0 < work;
+         t0 += (int64_t)gridDim.x * kPartLanes) {
+        const int64_t t = t0 + u;
+        int64_t hb = 0, c = 0, b = 0;
         T sre = (T)0, sim = (T)0;
-        for (int64_t q = 0; q < np; ++q) {
-            const T *src = partial + 2 * (((base + q) * vectors + v) * cells + c);
-            sre += src[0];
-            sim += src[1];
+        if (t < work) {
+            hb = t / cells;
+            c = t - hb * cells;
+            b = heavy_list[hb];
+            part_lane_sum(partial, (int64_t)heavy_base[b], (int64_t)parts[b], vectors, v, cells,
+                          c, pl, sre, sim);
         }
-        const int64_t xx = c % TX, yy = (c / TX) % TY, zz = c / (TX * TY);
-        const int64_t bx = b % sp.nblocks[0];
-        const int64_t by = (b / sp.nblocks[0]) % sp.nblocks[1];
-        const int64_t bz = b / (sp.nblocks[0] * sp.nblocks[1]);
-        const int64_t gx = bx * TX + xx, gy = by * TY + yy, gz = bz * TZ + zz;
-        const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n;
-        if (gx < n0 && gy < n1 && gz < g.ax[2].n) {
-            T *o = grid + 2 * (v * g.fine_total + gx + n0 * (gy + n1 * gz));
-            o[0] = sre;
-            o[1] = sim;
+        ps.re[pl][u] = sre;
+        ps.im[pl][u] = sim;
+        __syncthreads();
+        if (pl == 0 && t < work) {
+            combine_lanes(ps, u, sre, sim);
+            const int64_t xx = c % TX, yy = (c / TX) % TY, zz = c / (TX * TY);
+            const int64_t bx = b % sp.nblocks[0];
+            const int64_t by = (b / sp.nblocks[0]) % sp.nblocks[1];
+            const int64_t bz = b / (sp.nblocks[0] * sp.nblocks[1]);
+            const int64_t gx = bx * TX + xx, gy = by * TY + yy, gz = bz * TZ + zz;
+            const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n;
+            if (gx < n0 && gy < n1 && gz < g.ax[2].n) {
+                T *o = grid + 2 * (v * g.fine_total + gx + n0 * (gy + n1 * gz));
+                o[0] = sre;
+                o[1] = sim;
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -908,7 +927,7 @@ inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + thread
 
 __global__ void k_fill_ctrl(uint32_t *ctrl)
 {
-    if (threadIdx.x < 4) ctrl[threadIdx.x] = 0;
+    if (threadIdx.x < 8) ctrl[threadIdx.x] = 0;
 }
 
 template <int D, typename T, int R>
@@ -918,7 +937,7 @@ bool launch_spread(Spreader &s, const GridParams &g, const SpreadParams &sp,
 {
     int64_t nb = sp.block_count;
     int64_t cells = (int64_t)kLanes * R;
-    k_classify<D><<<grid_for(nb, 256), 256, 0, stream>>>(g, sp, offsets, s.totals);
+    k_classify<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(g, sp, offsets, s.totals);
     k_fill_ctrl<<<1, 64, 0, stream>>>(s.ctrl);
     k_plan<<<1, 1024, 0, stream>>>(nb, s.totals, s.light_limit, s.min_part, s.part_capacity,
                                    s.parts, s.item_off, s.heavy_base, s.heavy_list, s.ctrl);
@@ -945,7 +964,7 @@ bool launch_sweep(Spreader &s, const GridParams &g, const SpreadParams &sp,
 {
     constexpr int TX = kSweepTile, TY = kSweepTile, TZ = kSweepTile;
     int64_t nb = sp.block_count;
-    k_classify<3><<<grid_for(nb, 256), 256, 0, stream>>>(g, sp, offsets, s.totals);
+    k_classify<3><<<grid_for(nb * 64, 256), 256, 0, stream>>>(g, sp, offsets, s.totals);
     k_fill_ctrl<<<1, 64, 0, stream>>>(s.ctrl);
     k_plan<<<1, 1024, 0, stream>>>(nb, s.totals, s.light_limit, s.min_part, s.part_capacity,
                                    s.parts, s.item_off, s.heavy_base, s.heavy_list, s.ctrl);
@@ -961,6 +980,31 @@ bool launch_sweep(Spreader &s, const GridParams &g, const SpreadParams &sp,
                                                             s.heavy_list, s.ctrl,
                                                             (const T *)s.partial, grid);
     return hip_ok(hipGetLastError(), "sweep launch");
+}
+
+bool launch_mfma(Spreader &s, SpreadMethod method, const GridParams &g, const SpreadParams &sp,
+                 const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
+                 const float *c_sorted, int64_t M, int64_t vectors, float *grid,
+                 hipStream_t stream)
+{
+    constexpr int TX = kSweepTile, TY = kSweepTile, TZ = kSweepTile;
+    int64_t nb = sp.block_count;
+    k_classify<3><<<grid_for(nb * 64, 256), 256, 0, stream>>>(g, sp, offsets, s.totals);
+    k_fill_ctrl<<<1, 64, 0, stream>>>(s.ctrl);
+    if (!launch_absmax(c_sorted, 2 * M * vectors, s.ctrl + 4, stream)) return false;
+    k_plan<<<1, 1024, 0, stream>>>(nb, s.totals, s.light_limit, s.min_part, s.part_capacity,
+                                   s.parts, s.item_off, s.heavy_base, s.heavy_list, s.ctrl);
+    TileWork w{s.totals, s.parts, s.item_off, s.heavy_base, s.ctrl, s.ctrl + 4,
+               (unsigned)std::min<int64_t>(nb + 16384, 65536)};
+    if (s.kernel_start) (void)hipEventRecord(s.kernel_start, stream);
+    if (!launch_mfma_tiles(method, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, w,
+                           grid, (float *)s.partial, stream))
+        return false;
+    if (s.kernel_end) (void)hipEventRecord(s.kernel_end, stream);
+    dim3 rgrid(2048, (unsigned)vectors);
+    k_reduce_tiles<float, TX, TY, TZ><<<rgrid, 256, 0, stream>>>(
+        g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const float *)s.partial, grid);
+    return hip_ok(hipGetLastError(), "matrix-core spread launch");
 }
 
 }  // namespace
@@ -1003,7 +1047,7 @@ bool Spreader::ensure(int64_t nblocks, int64_t partial_need)
 template <int D, typename T>
 bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
                       const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
-                      int64_t vectors, T *grid, hipStream_t stream)
+                      int64_t vectors, T *grid, hipStream_t stream, SpreadMethod method)
 {
     const bool sweep = sp.run == 0;
     int64_t cells = sweep ? (int64_t)kSweepTile * kSweepTile * kSweepTile : (int64_t)kLanes * sp.run;
@@ -1012,6 +1056,15 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
     // until the heavy blocks fit)
     part_capacity = std::max<int64_t>(budget_bytes / per_part, 64);
     if (!ensure(sp.block_count, part_capacity * per_part)) return false;
+    if (method != SpreadMethod::Plain) {
+        if constexpr (D == 3 && sizeof(T) == 4) {
+            if (sweep)
+                return launch_mfma(*this, method, g, sp, offsets, rec_start, rec_off, c_sorted, M,
+                                   vectors, grid, stream);
+        }
+        set_error("matrix-core spread needs an FP32 3D plan on row-sweep tiles");
+        return false;
+    }
     if (sweep) {
         if constexpr (D == 3) {
             switch (g.width) {
@@ -1042,7 +1095,7 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
 #define MNUFFT_SPREAD_INSTANTIATE(D, T)                                                           \
     template bool Spreader::spread<D, T>(const GridParams &, const SpreadParams &,                 \
                                          const uint32_t *, const int4 *, const T *, const T *,     \
-                                         int64_t, int64_t, T *, hipStream_t);
+                                         int64_t, int64_t, T *, hipStream_t, SpreadMethod);
 MNUFFT_SPREAD_INSTANTIATE(1, float)
 MNUFFT_SPREAD_INSTANTIATE(2, float)
 MNUFFT_SPREAD_INSTANTIATE(3, float)

@@ -14,10 +14,12 @@ struct Spreader {
     // Spreads the prepared records (in bin order) of `vectors` strength
     // vectors onto `grid` (vectors * fine_total complex values).  Every cell
     // is written exactly once.
+    // A matrix-core method needs FP32 and row-sweep tiles (run == 0).
     template <int D, typename T>
     bool spread(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
                 const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
-                int64_t vectors, T *grid, hipStream_t stream);
+                int64_t vectors, T *grid, hipStream_t stream,
+                SpreadMethod method = SpreadMethod::Plain);
 
     // Optional events recorded right before and after the spread kernel.
     hipEvent_t kernel_start = nullptr, kernel_end = nullptr;
@@ -27,7 +29,8 @@ struct Spreader {
     uint32_t *item_off = nullptr;    // first work item per block
     uint32_t *heavy_base = nullptr;  // first partial slot per heavy block
     uint32_t *heavy_list = nullptr;  // heavy blocks
-    uint32_t *ctrl = nullptr;        // items, heavy parts, heavy blocks, part size
+    uint32_t *ctrl = nullptr;        // items, heavy parts, heavy blocks, part size,
+                                     // largest strength magnitude (float bits)
     void *partial = nullptr;
     int64_t blocks_cap = 0;
     int64_t partial_bytes = 0;
@@ -39,6 +42,23 @@ struct Spreader {
     // Scratch for partial sums of heavy parts.
     int64_t budget_bytes = (int64_t)256 << 20;
 };
+
+// Work plan of the 16^3 tiles handed to the matrix-core tile kernel.
+struct TileWork {
+    const uint32_t *totals, *parts, *item_off, *heavy_base, *ctrl, *cmax;
+    unsigned items_grid;
+};
+
+// Tile kernel of the matrix-core spread (spread_mfma.hip); partial sums of
+// heavy parts go to `partial` in the layout of the row-sweep tiles.
+bool launch_mfma_tiles(SpreadMethod method, const GridParams &g, const SpreadParams &sp,
+                       const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
+                       const float *c_sorted, int64_t M, int64_t vectors, const TileWork &w,
+                       float *grid, float *partial, hipStream_t stream);
+
+// out (cleared by the caller) receives the largest |re| or |im| of the n
+// floats at c as float bits.
+bool launch_absmax(const float *c, int64_t n, uint32_t *out, hipStream_t stream);
 
 // Output-stationary blocks for a fine grid: 64 lanes, `run` cells per lane
 // along axis 0 (run + width - 1 <= n0 keeps every cell's support offset

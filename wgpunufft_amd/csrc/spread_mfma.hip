@@ -1,0 +1,481 @@
+// Type-1 spreading of 3D tiles as dense contractions on the matrix cores.
+//
+// Same contract as the plain spreaders (src/gpu_nd_spread.rs:1-36): every
+// fine-grid cell of a 16^3 tile is owned by one workgroup, which sums the
+// records whose support reaches the tile in a fixed order and writes the cell
+// once; tiles with too many records are split into parts whose partial sums
+// are added in part order.  Results are bitwise repeatable and do not depend
+// on the input order of the points (the records are visited in canonical
+// order).
+//
+// The tile's sum is recast as a matrix product over its records j:
+//
+//   G[(y,z), (x,re|im)] += sum_j  Ky_j[y] Kz_j[z]  *  Kx_j[x] c_j
+//                          \_____ A (256 x K) ____/  \__ B (K x 32) __/
+//
+// with Kx_j, Ky_j, Kz_j the record's separable kernel weights laid out over
+// the tile's 16 cells of each axis (zero off the support).  The 256 (y,z)
+// rows form eight 32-row blocks (16 y x 2 z), each held in registers as the
+// accumulator of a v_mfma_f32_32x32x16 product; B's 32 columns are the 16 x
+// cells, real and imaginary parts interleaved, so the accumulator rows are
+// written out as contiguous 128-byte rows of the interleaved complex grid.
+// Records arrive in row order (z start, then y start), so the 16 records of
+// a k-block touch a narrow band of z and the row blocks outside that band
+// are skipped.
+//
+// Precision: A and B are formed in FP32 and rounded to FP16 (one product,
+// relative error about 2^-11), or split into hi + lo halves in FP16 or
+// bfloat16 with the three products hi*hi + hi*lo + lo*hi accumulated in FP32
+// (relative error about 2^-21 for FP16, 2^-16 for bfloat16).  Strengths are
+// scaled by a power of two so that the largest magnitude is below one, which
+// keeps the FP16 operands in range; the scale is undone exactly on output.
+#include <hip/hip_runtime.h>
+
+#include "spread.hpp"
+#include "spread_device.hpp"
+
+namespace mnufft {
+
+namespace {
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kTile = 16;
+constexpr int kRows = 64;   // reach rows staged per batch
+constexpr int kPitch = 20;  // LDS pitch of the weight tables, in floats
+constexpr int kBlocks = 8;  // 32-row blocks of the (y,z) plane
+
+struct MfmaRec {
+    int sx, sy, sz;  // support start relative to the tile origin
+    float ox, oy, oz;
+    float cre, cim;  // scaled strength
+};
+
+template <int MODE>
+struct Operand;
+
+template <>
+struct Operand<(int)SpreadMethod::MfmaF16> {
+    using V = f16x8;
+    __device__ static void split(const f32x8 &a, V &hi, V &lo)
+    {
+        hi = __builtin_convertvector(a, V);
+        (void)lo;
+    }
+    __device__ static f32x16 mma(const V &a, const V &b, const f32x16 &c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Operand<(int)SpreadMethod::MfmaF16x3> {
+    using V = f16x8;
+    __device__ static void split(const f32x8 &a, V &hi, V &lo)
+    {
+        hi = __builtin_convertvector(a, V);
+        lo = __builtin_convertvector(a - __builtin_convertvector(hi, f32x8), V);
+    }
+    __device__ static f32x16 mma(const V &a, const V &b, const f32x16 &c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Operand<(int)SpreadMethod::MfmaBf16x3> {
+    using V = bf16x8;
+    __device__ static void split(const f32x8 &a, V &hi, V &lo)
+    {
+        hi = __builtin_convertvector(a, V);
+        lo = __builtin_convertvector(a - __builtin_convertvector(hi, f32x8), V);
+    }
+    __device__ static f32x16 mma(const V &a, const V &b, const f32x16 &c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ inline f32x8 load8(const float *p)
+{
+    const float4 a = *(const float4 *)p;
+    const float4 b = *(const float4 *)(p + 4);
+    return f32x8{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+
+// Reach rows of a tile staged in batches of kRows: lane r holds bin row
+// rb + r of the reach (y start, then z start) as up to two slot ranges along
+// x (the second one when the reach wraps), and the exclusive scan of the
+// batch's record counts.
+struct RowBatch {
+    int64_t f0[kRows], n0[kRows], f1[kRows], excl[kRows];
+};
+
+template <int D, int TILE, int W>
+__device__ inline int64_t stage_rows(RowBatch &rows, int rb, int lane, int64_t y0, int64_t z0,
+                                     const AxisReach &rx, const GridParams &g,
+                                     const uint32_t *__restrict__ offsets)
+{
+    constexpr int NRY = TILE + W - 1;
+    constexpr int NROWS = D == 3 ? NRY * NRY : NRY;
+    const int64_t n1 = g.ax[1].n, n2 = g.ax[2].n;
+    const int r = rb + lane;
+    int64_t f0 = 0, c0 = 0, f1 = 0, c1 = 0;
+    if (r < NROWS) {
+        int64_t sy = y0 - (W - 1) + r % NRY;
+        if (sy < 0) sy += n1;
+        if (sy >= n1) sy -= n1;
+        int64_t line = sy;
+        if constexpr (D == 3) {
+            int64_t sz = z0 - (W - 1) + r / NRY;
+            if (sz < 0) sz += n2;
+            if (sz >= n2) sz -= n2;
+            line += n1 * sz;
+        }
+        const int64_t row = g.ax[0].bins * line;
+        f0 = offsets[row + rx.lo0];
+        c0 = (int64_t)offsets[row + rx.lo0 + rx.len0] - f0;
+        if (rx.len1 > 0) {
+            f1 = offsets[row + rx.lo1];
+            c1 = (int64_t)offsets[row + rx.lo1 + rx.len1] - f1;
+        }
+    }
+    const int64_t cnt = c0 + c1;
+    int64_t xsum = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(xsum, o, 64);
+        if (lane >= o) xsum += y;
+    }
+    rows.f0[lane] = f0;
+    rows.n0[lane] = c0;
+    rows.f1[lane] = f1;
+    rows.excl[lane] = xsum - cnt;
+    __syncthreads();
+    return __shfl(xsum, 63, 64);
+}
+
+// Slot of record k of the staged batch; *q = its row within the batch.
+__device__ inline int64_t batch_slot(const RowBatch &rows, int64_t k, int *q)
+{
+    int s_lo = 0, s_hi = kRows - 1;
+    while (s_lo < s_hi) {
+        const int mid = (s_lo + s_hi + 1) >> 1;
+        if (rows.excl[mid] <= k)
+            s_lo = mid;
+        else
+            s_hi = mid - 1;
+    }
+    *q = s_lo;
+    const int64_t j = k - rows.excl[s_lo];
+    return j < rows.n0[s_lo] ? rows.f0[s_lo] + j : rows.f1[s_lo] + (j - rows.n0[s_lo]);
+}
+
+// Power-of-two scale that brings the largest strength magnitude below one.
+__device__ inline void strength_scale(const uint32_t *cmax, float *scale, float *unscale)
+{
+    *scale = 1.0f;
+    *unscale = 1.0f;
+    const float m = __uint_as_float(cmax[0]);
+    if (m > 0.0f && m <= 3.4028234663852886e38f) {
+        int e = 0;
+        (void)frexpf(m, &e);
+        *scale = ldexpf(1.0f, -e);
+        *unscale = ldexpf(1.0f, e);
+    }
+}
+
+// Work item -> (tile, part, record range) of the tile plan.
+struct TileItem {
+    int64_t b, p, rlo, rhi;
+    uint32_t np;
+};
+
+__device__ inline TileItem tile_item(int64_t item, int64_t block_count,
+                                     const uint32_t *__restrict__ item_off,
+                                     const uint32_t *__restrict__ parts,
+                                     const uint32_t *__restrict__ totals, uint32_t part_len_cap)
+{
+    int64_t lo = 0, hi = block_count - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if ((int64_t)item_off[mid] <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    TileItem t;
+    t.b = lo;
+    t.p = item - item_off[lo];
+    t.np = parts[lo];
+    const int64_t total = totals[lo];
+    t.rlo = 0;
+    t.rhi = total;
+    if (t.np > 1) {
+        t.rlo = t.p * (int64_t)part_len_cap;
+        t.rhi = t.rlo + part_len_cap < total ? t.rlo + part_len_cap : total;
+    }
+    return t;
+}
+
+// Largest |re|, |im| over the strengths, as float bits (NaN above Inf).
+__global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ c, int64_t n,
+                                                uint32_t *__restrict__ out)
+{
+    uint32_t m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = __float_as_uint(c[i]) & 0x7fffffffu;
+        m = u > m ? u : m;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t y = __shfl_xor(m, o, 64);
+        m = y > m ? y : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
+template <int W, int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_spread_mfma3(
+    GridParams g, SpreadParams sp, const uint32_t *__restrict__ offsets,
+    const int4 *__restrict__ rec_start, const float *__restrict__ rec_off,
+    const float *__restrict__ c_sorted, int64_t M, const uint32_t *__restrict__ totals,
+    const uint32_t *__restrict__ parts, const uint32_t *__restrict__ item_off,
+    const uint32_t *__restrict__ heavy_base, const uint32_t *__restrict__ ctrl,
+    const uint32_t *__restrict__ cmax, float *__restrict__ grid, float *__restrict__ partial)
+{
+    using Op = Operand<MODE>;
+    using V = typename Op::V;
+    constexpr bool kSplit = MODE != (int)SpreadMethod::MfmaF16;
+    constexpr int NRY = kTile + W - 1;
+    constexpr int NROWS = NRY * NRY;
+    __shared__ __align__(16) float KX[kTile * kPitch];
+    __shared__ __align__(16) float KY[kTile * kPitch];
+    __shared__ __align__(16) float KZ[kTile * kPitch];
+    __shared__ __align__(16) float CC[2 * kPitch];
+    __shared__ MfmaRec st[64];
+    __shared__ RowBatch rows;
+
+    const int lane = threadIdx.x;
+    const int r32 = lane & 31;
+    const int half = lane >> 5;
+    const int j0 = 8 * half;   // first of the lane's 8 records in a k-block
+    const int ay = r32 & 15;   // A row: y cell, and z offset within the pair
+    const int az = r32 >> 4;
+    const int bxc = r32 >> 1;  // B column: x cell, real (0) or imaginary (1)
+    const int bpart = r32 & 1;
+
+    float scale, unscale;
+    strength_scale(cmax, &scale, &unscale);
+
+    const int64_t v = blockIdx.y;
+    const float beta = (float)g.beta;
+    const int64_t n_items = ctrl[0];
+    const uint32_t part_len_cap = ctrl[3];
+    const float *cv = c_sorted + 2 * v * M;
+    const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n, n2 = g.ax[2].n;
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const TileItem ti = tile_item(item, sp.block_count, item_off, parts, totals, part_len_cap);
+        const int64_t b = ti.b, p = ti.p, rlo = ti.rlo, rhi = ti.rhi;
+        const uint32_t np = ti.np;
+        const int64_t tbx = b % sp.nblocks[0];
+        const int64_t tby = (b / sp.nblocks[0]) % sp.nblocks[1];
+        const int64_t tbz = b / (sp.nblocks[0] * sp.nblocks[1]);
+        const int64_t x0 = tbx * kTile, y0 = tby * kTile, z0 = tbz * kTile;
+        const AxisReach rx = axis_reach(x0, kTile, W, g.ax[0]);
+        int64_t xs = x0 - (W - 1);
+        if (xs < 0) xs += n0;
+
+        f32x16 acc[kBlocks];
+#pragma unroll
+        for (int m = 0; m < kBlocks; ++m) acc[m] = f32x16{};
+
+        int64_t cursor = 0;
+        for (int rb = 0; rb < NROWS && cursor < rhi; rb += kRows) {
+            const int64_t batch_total =
+                stage_rows<3, kTile, W>(rows, rb, lane, y0, z0, rx, g, offsets);
+            const int64_t klo = rlo > cursor ? rlo - cursor : 0;
+            const int64_t khi = rhi - cursor < batch_total ? rhi - cursor : batch_total;
+            for (int64_t k0 = klo; k0 < khi; k0 += 64) {
+                const int64_t k = k0 + lane;
+                bool take = false;
+                MfmaRec R{};
+                if (k < khi) {
+                    int q;
+                    const int64_t slot = batch_slot(rows, k, &q);
+                    int64_t ee = (int64_t)rec_start[slot].x - xs;
+                    if (ee < 0) ee += n0;
+                    take = ee < kTile + W - 1;
+                    if (take) {
+                        const int r = rb + q;
+                        R.sx = (int)ee - (W - 1);
+                        R.sy = r % NRY - (W - 1);
+                        R.sz = r / NRY - (W - 1);
+                        R.ox = rec_off[4 * slot + 0];
+                        R.oy = rec_off[4 * slot + 1];
+                        R.oz = rec_off[4 * slot + 2];
+                        R.cre = cv[2 * slot] * scale;
+                        R.cim = cv[2 * slot + 1] * scale;
+                    }
+                }
+                const uint64_t mask = __ballot(take);
+                const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+                const int nst = __popcll(mask);
+                if (take) st[pos] = R;
+                __syncthreads();
+                for (int kb = 0; kb < nst; kb += 16) {
+                    const int kn = nst - kb < 16 ? nst - kb : 16;
+                    // clear the weight tables, then drop each record's
+                    // in-tile weights into its column
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        float *tb = t == 0 ? KX : (t == 1 ? KY : KZ);
+                        *(float4 *)(tb + (lane >> 2) * kPitch + 4 * (lane & 3)) =
+                            float4{0.0f, 0.0f, 0.0f, 0.0f};
+                    }
+                    __syncthreads();
+                    if (lane < 48) {
+                        const int j = lane & 15, axis = lane >> 4;
+                        if (j < kn) {
+                            const MfmaRec &S = st[kb + j];
+                            const int s = axis == 0 ? S.sx : (axis == 1 ? S.sy : S.sz);
+                            const float o = axis == 0 ? S.ox : (axis == 1 ? S.oy : S.oz);
+                            float *tb = axis == 0 ? KX : (axis == 1 ? KY : KZ);
+#pragma unroll
+                            for (int t = 0; t < W; ++t) {
+                                const int c = s + t;
+                                if (c >= 0 && c < kTile)
+                                    tb[c * kPitch + j] = es_weight<float>((float)t + o, W, beta);
+                            }
+                        }
+                    } else {
+                        const int j = lane - 48;
+                        CC[j] = j < kn ? st[kb + j].cre : 0.0f;
+                        CC[kPitch + j] = j < kn ? st[kb + j].cim : 0.0f;
+                    }
+                    __syncthreads();
+                    // band of z cells the k-block touches (records in row order)
+                    int zlo = __builtin_amdgcn_readfirstlane(st[kb].sz);
+                    int zhi = __builtin_amdgcn_readfirstlane(st[kb + kn - 1].sz) + W - 1;
+                    zlo = zlo < 0 ? 0 : zlo;
+                    zhi = zhi > kTile - 1 ? kTile - 1 : zhi;
+                    const int mlo = zlo >> 1, mhi = zhi >> 1;
+
+                    V bh, bl;
+                    {
+                        const f32x8 kx = load8(KX + bxc * kPitch + j0);
+                        const f32x8 cc = load8(CC + bpart * kPitch + j0);
+                        Op::split(kx * cc, bh, bl);
+                    }
+                    const f32x8 ky = load8(KY + ay * kPitch + j0);
+#pragma unroll
+                    for (int m = 0; m < kBlocks; ++m) {
+                        if (m < mlo || m > mhi) continue;
+                        const f32x8 kz = load8(KZ + (2 * m + az) * kPitch + j0);
+                        V ah, al;
+                        Op::split(ky * kz, ah, al);
+                        acc[m] = Op::mma(ah, bh, acc[m]);
+                        if constexpr (kSplit) {
+                            acc[m] = Op::mma(ah, bl, acc[m]);
+                            acc[m] = Op::mma(al, bh, acc[m]);
+                        }
+                    }
+                    __syncthreads();
+                }
+                __syncthreads();
+            }
+            cursor += batch_total;
+            __syncthreads();
+        }
+
+        // accumulator element i of block m: row 8*(i/4) + 4*half + i%4 of
+        // the block (y = row % 16, z = 2m + row / 16), column r32
+        const int64_t pslot = np > 1 ? (int64_t)heavy_base[b] + p : 0;
+#pragma unroll
+        for (int m = 0; m < kBlocks; ++m) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int row = 8 * (i >> 2) + 4 * half + (i & 3);
+                const int yy = row & 15, zz = 2 * m + (row >> 4);
+                const float val = acc[m][i] * unscale;
+                if (np == 1) {
+                    const int64_t gx = x0 + bxc, gy = y0 + yy, gz = z0 + zz;
+                    if (gx < n0 && gy < n1 && gz < n2)
+                        grid[2 * (v * g.fine_total + gx + n0 * (gy + n1 * gz)) + bpart] = val;
+                } else {
+                    partial[2 * (((pslot * gridDim.y + v) * kTile + zz) * (kTile * kTile) +
+                                 yy * kTile + bxc) +
+                            bpart] = val;
+                }
+            }
+        }
+    }
+}
+
+template <int W>
+bool launch_width(SpreadMethod method, dim3 grid3, const GridParams &g, const SpreadParams &sp,
+                  const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
+                  const float *c_sorted, int64_t M, const TileWork &w, float *grid,
+                  float *partial, hipStream_t stream)
+{
+    switch (method) {
+    case SpreadMethod::MfmaF16:
+        k_spread_mfma3<W, (int)SpreadMethod::MfmaF16><<<grid3, 64, 0, stream>>>(
+            g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
+            w.heavy_base, w.ctrl, w.cmax, grid, partial);
+        return true;
+    case SpreadMethod::MfmaF16x3:
+        k_spread_mfma3<W, (int)SpreadMethod::MfmaF16x3><<<grid3, 64, 0, stream>>>(
+            g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
+            w.heavy_base, w.ctrl, w.cmax, grid, partial);
+        return true;
+    case SpreadMethod::MfmaBf16x3:
+        k_spread_mfma3<W, (int)SpreadMethod::MfmaBf16x3><<<grid3, 64, 0, stream>>>(
+            g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
+            w.heavy_base, w.ctrl, w.cmax, grid, partial);
+        return true;
+    default:
+        return false;
+    }
+}
+
+}  // namespace
+
+bool launch_absmax(const float *c, int64_t n, uint32_t *out, hipStream_t stream)
+{
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    k_absmax<<<(unsigned)blocks, 256, 0, stream>>>(c, n, out);
+    return hip_ok(hipGetLastError(), "strength range launch");
+}
+
+bool launch_mfma_tiles(SpreadMethod method, const GridParams &g, const SpreadParams &sp,
+                       const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
+                       const float *c_sorted, int64_t M, int64_t vectors, const TileWork &w,
+                       float *grid, float *partial, hipStream_t stream)
+{
+    dim3 grid3(w.items_grid, (unsigned)vectors);
+    bool ok = false;
+    switch (g.width) {
+    case 2: ok = launch_width<2>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 3: ok = launch_width<3>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 4: ok = launch_width<4>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 5: ok = launch_width<5>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 6: ok = launch_width<6>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 7: ok = launch_width<7>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 8: ok = launch_width<8>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    default: break;
+    }
+    if (!ok) {
+        set_error("matrix-core spread needs a width <= 8 and an MFMA spread method");
+        return false;
+    }
+    return hip_ok(hipGetLastError(), "matrix-core spread launch");
+}
+
+}  // namespace mnufft
