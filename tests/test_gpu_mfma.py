@@ -30,11 +30,11 @@ def _setup():
     return torch, W, O, S
 
 
-def _pts(S, kind, M, seed):
+def _pts(S, kind, M, seed, d=3):
     if kind == "uniform":
-        return S.uniform_coords(seed, M, 3)
-    x = S.clustered_coords(seed, M, (0.3, -1.1, 2.0), 0.05)
-    x[: M // 3] = S.uniform_coords(seed + 1, M // 3, 3)
+        return S.uniform_coords(seed, M, d)
+    x = S.clustered_coords(seed, M, (0.3, -1.1, 2.0)[:d], 0.05)
+    x[: M // 3] = S.uniform_coords(seed + 1, M // 3, d)
     return x
 
 
@@ -45,6 +45,10 @@ CASES = [
     ("mfma_bf16x3", [16, 14, 12], 1, 0, "clustered", 4000),
     ("mfma_f16", [16, 14, 12], -1, 0, "uniform", 4000),
     ("mfma_f16", [24, 20, 18], 1, 1, "clustered", 5000),
+    ("mfma_f16x3", [40, 36], 1, 0, "uniform", 6000),
+    ("mfma_f16x3", [70, 44], -1, 1, "clustered", 8000),
+    ("mfma_bf16x3", [48, 40], 1, 0, "uniform", 6000),
+    ("mfma_f16", [64, 50], -1, 0, "clustered", 8000),
 ]
 
 
@@ -52,7 +56,7 @@ CASES = [
 def test_mfma_type1_matches_direct_oracle(method, n_modes, isign, modeord, kind, M):
     torch, W, O, S = _setup()
     eps, bound = METHODS[method]
-    x = _pts(S, kind, M, S.config_seed(2, 900 + M))
+    x = _pts(S, kind, M, S.config_seed(2, 900 + M), len(n_modes))
     c = S.complex_normal(S.config_seed(2, 901 + M), M)
     cfg = W.NufftConfig(n_modes, eps).with_isign(isign).with_mode_order(modeord)
     plain = W.NufftPlan.type1_gpu(cfg)
@@ -66,8 +70,37 @@ def test_mfma_type1_matches_direct_oracle(method, n_modes, isign, modeord, kind,
     e_p = rel_l2(fp, exact)
     d = rel_l2(fm, fp)
     print(f"MFMA {method} {n_modes} {kind}: err={e_m:.3e} plain={e_p:.3e} vs_plain={d:.3e}")
-    assert e_m <= 32 * eps
+    assert e_m <= (32 if len(n_modes) == 3 else 20) * eps
     assert d <= bound
+
+
+def test_mfma_2d_bitwise_batch_shuffle_and_rebinning():
+    """2D tiles: bitwise repeat, shuffle invariance and batch == loop; a
+    method switch after set_points re-bins the kept points."""
+    torch, W, O, S = _setup()
+    n_modes = [96, 80]
+    M, B = 30000, 2
+    x = _pts(S, "clustered", M, S.config_seed(1, 940), 2)
+    c = S.complex_normal(S.config_seed(1, 941), M * B)
+    xt = torch.from_numpy(x).cuda()
+    plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
+    plan.set_points_gpu(xt)
+    plain = plan.execute_gpu(torch.from_numpy(c).cuda()).cpu().numpy()
+    plan.set_spread_method("mfma_f16x3")
+    fb = plan.execute_gpu(torch.from_numpy(c).cuda()).cpu().numpy()
+    assert rel_l2(fb, plain) <= 2e-6
+    N = int(np.prod(n_modes))
+    for v in range(B):
+        fs = plan.encode_gpu(xt, torch.from_numpy(c[v * M:(v + 1) * M].copy()).cuda(),
+                             active_batch=1).cpu().numpy()
+        assert np.array_equal(fs.view(np.uint32), fb[v * N:(v + 1) * N].view(np.uint32))
+    p = S.shuffle_perm(S.config_seed(1, 942), M)
+    cp = np.concatenate([c[v * M:(v + 1) * M][p] for v in range(B)])
+    f2 = plan.encode_gpu(torch.from_numpy(x[p].copy()).cuda(), torch.from_numpy(cp).cuda())
+    assert np.array_equal(f2.cpu().numpy().view(np.uint32), fb.view(np.uint32))
+    sel = np.arange(0, N, 37)
+    ref = O.direct_type1(x.astype(np.float64), c[:M].astype(np.complex128), n_modes, modes=sel)
+    assert rel_l2(fb[:N][sel], ref) <= 20e-6
 
 
 def test_mfma_bitwise_repeat_and_shuffle_invariance():
@@ -133,7 +166,9 @@ def test_set_spread_method_rejects_unsupported_plans():
         W.NufftPlan.type1_gpu(W.NufftConfig([16, 14, 12], 1e-6).with_precision(W.F64)) \
             .set_spread_method("mfma_f16x3")
     with pytest.raises(Exception):
-        W.NufftPlan.type1_gpu(W.NufftConfig([40, 36], 1e-6)).set_spread_method("mfma_f16x3")
+        W.NufftPlan.type1_gpu(W.NufftConfig([12, 10], 1e-6)).set_spread_method("mfma_f16x3")
+    with pytest.raises(Exception):
+        W.NufftPlan.type1_gpu(W.NufftConfig([200], 1e-6)).set_spread_method("mfma_f16x3")
     with pytest.raises(Exception):
         W.NufftPlan.type2_gpu(W.NufftConfig([16, 14, 12], 1e-6)).set_spread_method("mfma_f16x3")
     with pytest.raises(Exception):

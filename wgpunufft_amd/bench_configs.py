@@ -3,12 +3,14 @@
 and the error against FP64 direct sums on a seeded sample.
 
 1. 2D type-1 F32, 128x128 modes, tol 1e-6, M = 100,000 uniform points.
-3. 2D type-1, 1024x1024 modes, M = 5e7 uniform points: F32 at tol 1e-6 and
-   tol 1e-3.
+3. 2D type-1, 1024x1024 modes, M = 5e7 uniform points: F32 at tol 1e-6 with
+   the plain, FP16-split and bfloat16-split spreaders, and at tol 1e-3 with
+   the plain and FP16 spreaders.
 4. (a) 64 independent 2D type-2 F32 problems at tol 1e-5, modes per axis
    log-uniform in [64, 512], M log-uniform in [1e4, 2e6]; (b) 32 vectors
    over one 3D point set of 8192 radial spokes x 512 samples, 256^3 modes.
-5. 3D type-3, M = 1e6 sources in [-5,5]^3; the targets are scaled down from
+5. 3D type-3, M = 1e6 sources in [-5,5]^3 (F64 at tol 1e-12 / 1e-10 / 1e-6,
+   F32 at 1e-6 with the plain and FP16-split spreaders); the targets are scaled down from
    [-200,200]^3 to [-40,40]^3 so the inner fine grid fits one GPU (the full
    interval needs a 2592^3 inner grid, 279 GB in complex128).
 """
@@ -74,16 +76,26 @@ def config3(args, W, S, torch):
     c = S.complex_normal(S.config_seed(3, 2), M)
     xt, ct = torch.from_numpy(x).to(dev), torch.from_numpy(c).to(dev)
     out = torch.empty(int(np.prod(n_modes)), dtype=torch.complex64, device=dev)
-    for eps in (1e-6, 1e-3):
-        plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, eps))
+    legs = [(1e-6, "plain"), (1e-6, "mfma_f16x3"), (1e-6, "mfma_bf16x3"),
+            (1e-3, "plain"), (1e-3, "mfma_f16")]
+    plain_ms = {}
+    for eps, method in legs:
+        plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, eps)).set_spread_method(method)
+        plan.set_points_gpu(xt)
         t = _timed(lambda: plan.encode_gpu(xt, ct, out=out), args.steps, args.warmup)
         plan.encode_gpu(xt, ct, out=out)
         st = plan.stage_times()
         err = _err_t1(W, S, torch, x, c, n_modes, out, sample=2000, seed=S.config_seed(3, 7))
-        _line(args, f"NU points/s, 2D type-1 1024^2 modes tol {eps:g} (plain FP32 spreader)",
+        extra = {"stages_ms": st, "error": {"type1_rel_l2": err, "sample": "2000 modes"}}
+        if method == "plain":
+            plain_ms[eps] = st["spread_kernel"]
+        elif eps in plain_ms and st["spread_kernel"]:
+            extra["spread_kernel_vs_plain"] = plain_ms[eps] / st["spread_kernel"]
+        _line(args, f"NU points/s, 2D type-1 1024^2 modes tol {eps:g} ({method} spreader)",
               M / t, "points/s", t * 1e3, "f32",
-              {"workload": f"config 3: 2D type-1, 1024x1024 modes, M=5e7 uniform, tol {eps:g}"},
-              {"stages_ms": st, "error": {"type1_rel_l2": err, "sample": "2000 modes"}})
+              {"workload": f"config 3: 2D type-1, 1024x1024 modes, M=5e7 uniform, tol {eps:g}",
+               "spreader": method}, extra)
+        del plan
 
 
 def config4(args, W, S, torch):
@@ -120,17 +132,24 @@ def config4(args, W, S, torch):
     n_modes = [256, 256, 256]
     c = S.complex_normal(S.config_seed(4, 5), M * B)
     xt, ct = torch.from_numpy(x).to(dev), torch.from_numpy(c).to(dev)
-    plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
     out = torch.empty(int(np.prod(n_modes)) * B, dtype=torch.complex64, device=dev)
-    t = _timed(lambda: plan.encode_gpu(xt, ct, out=out), args.steps, args.warmup)
-    plan.encode_gpu(xt, ct, out=out)
-    st = plan.stage_times()
-    err = _err_t1(W, S, torch, x, c[:M], n_modes, out[: int(np.prod(n_modes))], sample=500,
-                  seed=S.config_seed(4, 9))
-    _line(args, "NU points/s, 3D type-1 256^3 modes, 32 vectors, radial", M * B / t, "points/s",
-          t * 1e3, "f32",
-          {"workload": "config 4b: 3D type-1 256^3 modes tol 1e-6, 8192 spokes x 512, B=32"},
-          {"stages_ms": st, "error": {"type1_rel_l2_vector0": err, "sample": "500 modes"}})
+    N = int(np.prod(n_modes))
+    for method in ("plain", "mfma_f16x3"):
+        plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
+        plan.set_spread_method(method)
+        t = _timed(lambda: plan.encode_gpu(xt, ct, out=out), args.steps, args.warmup)
+        plan.encode_gpu(xt, ct, out=out)
+        st = plan.stage_times()
+        errs = [_err_t1(W, S, torch, x, c[v * M:(v + 1) * M], n_modes, out[v * N:(v + 1) * N],
+                        sample=500, seed=S.config_seed(4, 9)) for v in (0, B - 1)]
+        _line(args, f"NU points/s, 3D type-1 256^3 modes, 32 vectors, radial ({method})",
+              M * B / t, "points/s", t * 1e3, "f32",
+              {"workload": "config 4b: 3D type-1 256^3 modes tol 1e-6, 8192 spokes x 512, B=32",
+               "spreader": method},
+              {"stages_ms": st, "error": {"type1_rel_l2_first_vector": errs[0],
+                                          "type1_rel_l2_last_vector": errs[1],
+                                          "sample": "500 modes"}})
+        del plan
 
 
 def config5(args, W, S, torch):
@@ -142,17 +161,27 @@ def config5(args, W, S, torch):
     xt, st_, ct = (torch.from_numpy(a).to(dev) for a in (x, s, c))
     sel = np.sort(S.splitmix64(S.config_seed(5, 4), 1000) % np.uint64(N)).astype(np.int64)
     ref = W.reference_type3_f64(xt, ct, st_, targets=torch.from_numpy(sel).to(dev)).cpu().numpy()
-    for eps in (1e-12, 1e-10, 1e-6):
-        plan = W.NufftType3Plan(3, [(-5, 5)] * 3, [(-40, 40)] * 3, eps=eps, precision=W.F64)
-        out = torch.empty(N, dtype=torch.complex128, device=dev)
-        t = _timed(lambda: plan.encode_gpu(xt, ct, st_, out=out), args.steps, args.warmup)
-        g = out.cpu().numpy()[sel]
+    legs = [(1e-12, W.F64, "plain"), (1e-10, W.F64, "plain"), (1e-6, W.F64, "plain"),
+            (1e-6, W.F32, "plain"), (1e-6, W.F32, "mfma_f16x3")]
+    x32, s32 = xt.to(torch.float32), st_.to(torch.float32)
+    c32 = ct.to(torch.complex64)
+    for eps, prec, method in legs:
+        plan = W.NufftType3Plan(3, [(-5, 5)] * 3, [(-40, 40)] * 3, eps=eps, precision=prec)
+        plan.set_spread_method(method)
+        f64 = prec == W.F64
+        xa, sa, ca = (xt, st_, ct) if f64 else (x32, s32, c32)
+        out = torch.empty(N, dtype=torch.complex128 if f64 else torch.complex64, device=dev)
+        t = _timed(lambda: plan.encode_gpu(xa, ca, sa, out=out), args.steps, args.warmup)
+        g = out.cpu().numpy()[sel].astype(np.complex128)
         err = float(np.linalg.norm(g - ref) / np.linalg.norm(ref))
-        _line(args, f"NU points/s, 3D type-3 F64 tol {eps:g}", (M + N) / t, "points/s", t * 1e3,
-              "f64",
+        name = "F64" if f64 else "F32"
+        _line(args, f"NU points/s, 3D type-3 {name} tol {eps:g} ({method} spreader)",
+              (M + N) / t, "points/s", t * 1e3, "f64" if f64 else "f32",
               {"workload": "config 5 (targets scaled to [-40,40]^3): 1e6 sources in [-5,5]^3, "
-                           f"1e6 targets, tol {eps:g}", "outer_grid": plan.info()["outer"]},
+                           f"1e6 targets, tol {eps:g}", "outer_grid": plan.info()["outer"],
+               "spreader": method},
               {"error": {"rel_l2": err, "sample": "1000 targets"}})
+        del plan
 
 
 def run(args):

@@ -353,9 +353,10 @@ int mnufft_set_spread_method(mnufft_plan plan, int32_t method)
         return MNUFFT_ERROR_INVALID_ARGUMENT;
     }
     if (method != MNUFFT_SPREAD_PLAIN &&
-        (q.type != 1 || q.dim != 3 || q.precision != Precision::F32 || !q.sweep)) {
-        set_error("matrix-core spreading needs an FP32 3D plan with at least 16 + width fine "
-                  "cells per axis and a width <= 8");
+        (q.type != 1 || q.precision != Precision::F32 || !((q.dim == 3 && q.sweep) ||
+                                                           (q.dim == 2 && q.tiles2)))) {
+        set_error("matrix-core spreading needs an FP32 type-1 plan, 3D with at least 16 + width "
+                  "fine cells per axis or 2D with at least 32 + width, and a width <= 8");
         return MNUFFT_ERROR_INVALID_ARGUMENT;
     }
     q.spread_method = (mnufft::SpreadMethod)method;

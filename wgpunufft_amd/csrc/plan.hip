@@ -96,6 +96,24 @@ bool make_sweep_params(int d, const int64_t *fine, int width, SpreadParams *sp)
     return true;
 }
 
+bool make_tile2_params(int d, const int64_t *fine, int width, SpreadParams *sp)
+{
+    constexpr int64_t tile = 32;
+    if (d != 2 || width > 8) return false;
+    if (fine[0] < tile + width + 1 || fine[1] < tile + width) return false;
+    SpreadParams q{};
+    q.run = 0;
+    q.groups0 = 0;
+    q.block_count = 1;
+    for (int a = 0; a < kMaxDims; ++a) {
+        q.block[a] = a < d ? tile : 1;
+        q.nblocks[a] = a < d ? (fine[a] + tile - 1) / tile : 1;
+        q.block_count *= q.nblocks[a];
+    }
+    *sp = q;
+    return true;
+}
+
 FftPlan::~FftPlan()
 {
     if (info) rocfft_execution_info_destroy((rocfft_execution_info)info);
@@ -153,6 +171,7 @@ bool Plan::init_common()
                                  precision == Precision::F32 ? Precision::F32 : Precision::F64);
     sp = make_spread_params(dim, fine, kernel.width);
     sweep = make_sweep_params(dim, fine, kernel.width, &sp);
+    tiles2 = make_tile2_params(dim, fine, kernel.width, &tile2_sp);
     return true;
 }
 
@@ -189,12 +208,13 @@ GridParams Plan::grid_params(const int64_t *side) const
 
 GridParams Plan::spread_geometry(int64_t M, int *chosen) const
 {
-    if (!sweep) return type1_geometry(M, chosen);
-    // one-cell bins along axes 1 and 2; along axis 0 the smallest power of
+    if (!rows_geometry()) return type1_geometry(M, chosen);
+    // one-cell bins along the other axes; along axis 0 the smallest power of
     // two from 2 that keeps the bin count within 2^24
     int64_t side[kMaxDims] = {2, 1, 1};
-    while (side[0] < fine[0] &&
-           ((fine[0] + side[0] - 1) / side[0]) * fine[1] * fine[2] > ((int64_t)1 << 24))
+    int64_t lines = 1;
+    for (int a = 1; a < dim; ++a) lines *= fine[a];
+    while (side[0] < fine[0] && ((fine[0] + side[0] - 1) / side[0]) * lines > ((int64_t)1 << 24))
         side[0] *= 2;
     if (chosen) *chosen = -1;
     return grid_params(side);
@@ -370,6 +390,7 @@ bool Plan::set_points_t(int64_t M, const T *pts, hipStream_t stream)
     points_set = false;
     if (type == 1) {
         gp = spread_geometry(M, &geometry_index);
+        gp_rows = rows_geometry();
         mark(kStageSort, false, stream);
         if (!sorter.sort<D, T>(M, pts, gp, BinOrder::Canonical, stream)) return false;
         mark(kStageSort, true, stream);
@@ -398,6 +419,9 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
     ModeParams mp = mode_params();
     if (type == 1) {
         if (M == 0) return fill_zero<T>(out, 2 * mode_total * active, stream);
+        // a spread method that needs other bins re-bins the kept points
+        if (gp_rows != rows_geometry() && !set_points_t<D, T>(M, (const T *)points, stream))
+            return false;
         if (!ensure_strengths(M, active)) return false;
         mark(kStageSpread, false, stream);
         bool timed = timing && slot < (int64_t)slots.size();
@@ -405,7 +429,8 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         spreader.kernel_end = timed ? slots[slot].ev[2 * kStageSpreadKernel + 1] : nullptr;
         if (timed) slots[slot].used[kStageSpreadKernel] = true;
         if (!gather_strengths<T>(M, active, rec_start, in, (T *)c_sorted, stream)) return false;
-        if (!spreader.spread<D, T>(gp, sp, sorter.offsets(), rec_start, (const T *)rec_off,
+        if (!spreader.spread<D, T>(gp, spread_params(), sorter.offsets(), rec_start,
+                                   (const T *)rec_off,
                                    (const T *)c_sorted, M, active, (T *)grid, stream,
                                    spread_method))
             return false;

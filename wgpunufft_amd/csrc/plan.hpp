@@ -44,7 +44,19 @@ struct Plan {
     SpreadLayout layout;
     SpreadParams sp{};
     bool sweep = false;  // 3D row-sweep spreader (else output-stationary blocks)
+    SpreadParams tile2_sp{};
+    bool tiles2 = false;  // 2D grid qualifies for the matrix-core 32 x 32 tiles
     SpreadMethod spread_method = SpreadMethod::Plain;
+    bool gp_rows = false;  // points binned in one-cell rows (row tiles)
+    // Type-1 spreads on row tiles: the 3D row sweep, or a 2D matrix-core method.
+    bool rows_geometry() const
+    {
+        return sweep || (tiles2 && spread_method != SpreadMethod::Plain);
+    }
+    const SpreadParams &spread_params() const
+    {
+        return dim == 2 && spread_method != SpreadMethod::Plain ? tile2_sp : sp;
+    }
 
     // points
     const void *points = nullptr;

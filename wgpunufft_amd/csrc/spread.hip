@@ -930,81 +930,69 @@ __global__ void k_fill_ctrl(uint32_t *ctrl)
     if (threadIdx.x < 8) ctrl[threadIdx.x] = 0;
 }
 
-template <int D, typename T, int R>
-bool launch_spread(Spreader &s, const GridParams &g, const SpreadParams &sp,
-                   const uint32_t *offsets, const int4 *rec_start, const T *rec_off,
-                   const T *c_sorted, int64_t M, int64_t vectors, T *grid, hipStream_t stream)
+// Tile kernels and heavy-part reductions for one chunk of vectors (gridDim.y
+// = chunk; c_sorted and grid point at the chunk's first vector).  The work
+// plan (totals, parts, item offsets) is shared by every chunk.
+inline dim3 items_grid(int64_t nb, int64_t vectors)
 {
-    int64_t nb = sp.block_count;
-    int64_t cells = (int64_t)kLanes * R;
-    k_classify<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(g, sp, offsets, s.totals);
-    k_fill_ctrl<<<1, 64, 0, stream>>>(s.ctrl);
-    k_plan<<<1, 1024, 0, stream>>>(nb, s.totals, s.light_limit, s.min_part, s.part_capacity,
-                                   s.parts, s.item_off, s.heavy_base, s.heavy_list, s.ctrl);
-    // Enough workgroups to fill the GPU several times over; each loops over
-    // items with a stride.
-    unsigned items_grid = (unsigned)std::min<int64_t>(nb + 16384, 65536);
-    dim3 grid3(items_grid, (unsigned)vectors);
-    if (s.kernel_start) (void)hipEventRecord(s.kernel_start, stream);
-    k_spread<D, T, R><<<grid3, kLanes, 0, stream>>>(g, sp, offsets, rec_start, rec_off, c_sorted,
-                                                    M, s.totals, s.parts, s.item_off,
-                                                    s.heavy_base, s.ctrl, grid,
-                                                    (T *)s.partial, cells);
+    // enough workgroups to fill the GPU several times over; each loops over
+    // items with a stride
+    return dim3((unsigned)std::min<int64_t>(nb + 16384, 65536), (unsigned)vectors);
+}
+
+template <int D, typename T, int R>
+bool chunk_spread(Spreader &s, const GridParams &g, const SpreadParams &sp,
+                  const uint32_t *offsets, const int4 *rec_start, const T *rec_off,
+                  const T *c_sorted, int64_t M, int64_t vectors, T *grid, hipStream_t stream)
+{
+    const int64_t cells = (int64_t)kLanes * R;
+    k_spread<D, T, R><<<items_grid(sp.block_count, vectors), kLanes, 0, stream>>>(
+        g, sp, offsets, rec_start, rec_off, c_sorted, M, s.totals, s.parts, s.item_off,
+        s.heavy_base, s.ctrl, grid, (T *)s.partial, cells);
     if (s.kernel_end) (void)hipEventRecord(s.kernel_end, stream);
-    dim3 rgrid(2048, (unsigned)vectors);
-    k_reduce<D, T, R><<<rgrid, 256, 0, stream>>>(g, sp, s.parts, s.heavy_base, s.heavy_list,
-                                                 s.ctrl, (const T *)s.partial, grid, cells);
+    k_reduce<D, T, R><<<dim3(2048, (unsigned)vectors), 256, 0, stream>>>(
+        g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const T *)s.partial, grid, cells);
     return hip_ok(hipGetLastError(), "spread launch");
 }
 
 template <typename T, int W>
-bool launch_sweep(Spreader &s, const GridParams &g, const SpreadParams &sp,
-                  const uint32_t *offsets, const int4 *rec_start, const T *rec_off,
-                  const T *c_sorted, int64_t M, int64_t vectors, T *grid, hipStream_t stream)
+bool chunk_sweep(Spreader &s, const GridParams &g, const SpreadParams &sp,
+                 const uint32_t *offsets, const int4 *rec_start, const T *rec_off,
+                 const T *c_sorted, int64_t M, int64_t vectors, T *grid, hipStream_t stream)
 {
     constexpr int TX = kSweepTile, TY = kSweepTile, TZ = kSweepTile;
-    int64_t nb = sp.block_count;
-    k_classify<3><<<grid_for(nb * 64, 256), 256, 0, stream>>>(g, sp, offsets, s.totals);
-    k_fill_ctrl<<<1, 64, 0, stream>>>(s.ctrl);
-    k_plan<<<1, 1024, 0, stream>>>(nb, s.totals, s.light_limit, s.min_part, s.part_capacity,
-                                   s.parts, s.item_off, s.heavy_base, s.heavy_list, s.ctrl);
-    unsigned items_grid = (unsigned)std::min<int64_t>(nb + 16384, 65536);
-    dim3 grid3(items_grid, (unsigned)vectors);
-    if (s.kernel_start) (void)hipEventRecord(s.kernel_start, stream);
-    k_sweep3<T, W, TX, TY, TZ><<<grid3, kLanes, 0, stream>>>(
+    k_sweep3<T, W, TX, TY, TZ><<<items_grid(sp.block_count, vectors), kLanes, 0, stream>>>(
         g, sp, offsets, rec_start, rec_off, c_sorted, M, s.totals, s.parts, s.item_off,
         s.heavy_base, s.ctrl, grid, (T *)s.partial);
     if (s.kernel_end) (void)hipEventRecord(s.kernel_end, stream);
-    dim3 rgrid(2048, (unsigned)vectors);
-    k_reduce_tiles<T, TX, TY, TZ><<<rgrid, 256, 0, stream>>>(g, sp, s.parts, s.heavy_base,
-                                                            s.heavy_list, s.ctrl,
-                                                            (const T *)s.partial, grid);
+    k_reduce_tiles<T, TX, TY, TZ><<<dim3(2048, (unsigned)vectors), 256, 0, stream>>>(
+        g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const T *)s.partial, grid);
     return hip_ok(hipGetLastError(), "sweep launch");
 }
 
-bool launch_mfma(Spreader &s, SpreadMethod method, const GridParams &g, const SpreadParams &sp,
-                 const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
-                 const float *c_sorted, int64_t M, int64_t vectors, float *grid,
-                 hipStream_t stream)
+template <int D>
+bool chunk_mfma(Spreader &s, SpreadMethod method, const GridParams &g, const SpreadParams &sp,
+                const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
+                const float *c_sorted, int64_t M, int64_t vectors, const uint32_t *vmax,
+                float *grid, hipStream_t stream)
 {
-    constexpr int TX = kSweepTile, TY = kSweepTile, TZ = kSweepTile;
-    int64_t nb = sp.block_count;
-    k_classify<3><<<grid_for(nb * 64, 256), 256, 0, stream>>>(g, sp, offsets, s.totals);
-    k_fill_ctrl<<<1, 64, 0, stream>>>(s.ctrl);
-    if (!launch_absmax(c_sorted, 2 * M * vectors, s.ctrl + 4, stream)) return false;
-    k_plan<<<1, 1024, 0, stream>>>(nb, s.totals, s.light_limit, s.min_part, s.part_capacity,
-                                   s.parts, s.item_off, s.heavy_base, s.heavy_list, s.ctrl);
-    TileWork w{s.totals, s.parts, s.item_off, s.heavy_base, s.ctrl, s.ctrl + 4,
-               (unsigned)std::min<int64_t>(nb + 16384, 65536)};
-    if (s.kernel_start) (void)hipEventRecord(s.kernel_start, stream);
-    if (!launch_mfma_tiles(method, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, w,
+    constexpr int TE = D == 3 ? kSweepTile : 32;  // tile edge
+    const dim3 ig = items_grid(sp.block_count, vectors);
+    TileWork w{s.totals, s.parts, s.item_off, s.heavy_base, s.ctrl, vmax, ig.x};
+    if (!launch_mfma_tiles(method, D, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, w,
                            grid, (float *)s.partial, stream))
         return false;
     if (s.kernel_end) (void)hipEventRecord(s.kernel_end, stream);
-    dim3 rgrid(2048, (unsigned)vectors);
-    k_reduce_tiles<float, TX, TY, TZ><<<rgrid, 256, 0, stream>>>(
+    k_reduce_tiles<float, TE, TE, D == 3 ? TE : 1><<<dim3(2048, (unsigned)vectors), 256, 0,
+                                                     stream>>>(
         g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const float *)s.partial, grid);
     return hip_ok(hipGetLastError(), "matrix-core spread launch");
+}
+
+__global__ void k_fill_u32s(uint32_t *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 }  // namespace
@@ -1014,16 +1002,23 @@ Spreader::~Spreader() { release(); }
 void Spreader::release()
 {
     for (void *p : {(void *)totals, (void *)parts, (void *)item_off, (void *)heavy_base,
-                    (void *)heavy_list, (void *)ctrl, partial})
+                    (void *)heavy_list, (void *)ctrl, (void *)vmax, partial})
         if (p) (void)hipFree(p);
-    totals = parts = item_off = heavy_base = heavy_list = ctrl = nullptr;
+    totals = parts = item_off = heavy_base = heavy_list = ctrl = vmax = nullptr;
     partial = nullptr;
     blocks_cap = 0;
     partial_bytes = 0;
+    vmax_cap = 0;
 }
 
-bool Spreader::ensure(int64_t nblocks, int64_t partial_need)
+bool Spreader::ensure(int64_t nblocks, int64_t partial_need, int64_t vectors)
 {
+    if (vectors > vmax_cap) {
+        if (vmax) (void)hipFree(vmax);
+        vmax = nullptr;
+        if (!hip_ok(hipMalloc(&vmax, sizeof(uint32_t) * vectors), "strength ranges")) return false;
+        vmax_cap = vectors;
+    }
     if (nblocks > blocks_cap) {
         for (uint32_t **p : {&totals, &parts, &item_off, &heavy_base, &heavy_list}) {
             if (*p) (void)hipFree(*p);
@@ -1049,47 +1044,86 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
                       const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
                       int64_t vectors, T *grid, hipStream_t stream, SpreadMethod method)
 {
-    const bool sweep = sp.run == 0;
-    int64_t cells = sweep ? (int64_t)kSweepTile * kSweepTile * kSweepTile : (int64_t)kLanes * sp.run;
-    int64_t per_part = cells * vectors * 2 * (int64_t)sizeof(T);
-    // part slots: the scratch budget (the plan pass raises the light limit
-    // until the heavy blocks fit)
-    part_capacity = std::max<int64_t>(budget_bytes / per_part, 64);
-    if (!ensure(sp.block_count, part_capacity * per_part)) return false;
-    if (method != SpreadMethod::Plain) {
-        if constexpr (D == 3 && sizeof(T) == 4) {
-            if (sweep)
-                return launch_mfma(*this, method, g, sp, offsets, rec_start, rec_off, c_sorted, M,
-                                   vectors, grid, stream);
-        }
-        set_error("matrix-core spread needs an FP32 3D plan on row-sweep tiles");
+    const bool tiles = sp.run == 0;
+    const bool mfma = method != SpreadMethod::Plain;
+    if (vectors > 65535) {
+        set_error("at most 65535 vectors per execution");
         return false;
     }
-    if (sweep) {
-        if constexpr (D == 3) {
-            switch (g.width) {
-            case 2: return launch_sweep<T, 2>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
-            case 3: return launch_sweep<T, 3>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
-            case 4: return launch_sweep<T, 4>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
-            case 5: return launch_sweep<T, 5>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
-            case 6: return launch_sweep<T, 6>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
-            case 7: return launch_sweep<T, 7>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
-            case 8: return launch_sweep<T, 8>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, grid, stream);
-            default: break;
-            }
-        }
+    if (mfma && !((D == 2 || D == 3) && sizeof(T) == 4 && tiles)) {
+        set_error("matrix-core spread needs an FP32 2D or 3D plan on row tiles");
+        return false;
+    }
+    if (!mfma && tiles && D != 3) {
+        set_error("row tiles of a 2D plan need a matrix-core spread method");
+        return false;
+    }
+    if (!mfma && tiles && (D != 3 || g.width < 2 || g.width > 8)) {
         set_error("row sweep needs a 3D plan of kernel width <= 8");
         return false;
     }
-    switch (sp.run) {
-    case 2: return launch_spread<D, T, 2>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M,
-                                          vectors, grid, stream);
-    case 4: return launch_spread<D, T, 4>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M,
-                                          vectors, grid, stream);
-    case 8: return launch_spread<D, T, 8>(*this, g, sp, offsets, rec_start, rec_off, c_sorted, M,
-                                          vectors, grid, stream);
-    default: set_error("unsupported spread run"); return false;
+    if (!mfma && !tiles && sp.run != 2 && sp.run != 4 && sp.run != 8) {
+        set_error("unsupported spread run");
+        return false;
     }
+    // The work plan depends on the records only, never on the number of
+    // vectors: part slots are sized for one vector, and the vectors are
+    // spread in chunks that share the plan, so a batch sums every vector
+    // exactly as a single-vector call does.
+    const int64_t cells =
+        tiles ? sp.block[0] * sp.block[1] * sp.block[2] : (int64_t)kLanes * sp.run;
+    const int64_t per_part = cells * 2 * (int64_t)sizeof(T);
+    part_capacity = std::max<int64_t>(budget_bytes / per_part, 64);
+    const int64_t chunk =
+        std::max<int64_t>(1, std::min<int64_t>(vectors, chunk_budget_bytes /
+                                                            (part_capacity * per_part)));
+    if (!ensure(sp.block_count, part_capacity * per_part * chunk, vectors)) return false;
+
+    const int64_t nb = sp.block_count;
+    k_classify<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(g, sp, offsets, totals);
+    k_fill_ctrl<<<1, 64, 0, stream>>>(ctrl);
+    k_plan<<<1, 1024, 0, stream>>>(nb, totals, light_limit, min_part, part_capacity, parts,
+                                   item_off, heavy_base, heavy_list, ctrl);
+    if constexpr (sizeof(T) == 4) {
+        if (mfma) {
+            k_fill_u32s<<<grid_for(vectors, 256), 256, 0, stream>>>(vmax, vectors);
+            if (!launch_absmax(c_sorted, 2 * M, vectors, vmax, stream)) return false;
+        }
+    }
+    if (kernel_start) (void)hipEventRecord(kernel_start, stream);
+    for (int64_t v0 = 0; v0 < vectors; v0 += chunk) {
+        const int64_t vc = std::min<int64_t>(chunk, vectors - v0);
+        const T *cs = c_sorted + 2 * v0 * M;
+        T *gr = grid + 2 * v0 * g.fine_total;
+        bool ok = false;
+        if (mfma) {
+            if constexpr ((D == 2 || D == 3) && sizeof(T) == 4)
+                ok = chunk_mfma<D>(*this, method, g, sp, offsets, rec_start, rec_off, cs, M, vc,
+                                   vmax + v0, gr, stream);
+        } else if (tiles) {
+            if constexpr (D == 3) {
+                switch (g.width) {
+                case 2: ok = chunk_sweep<T, 2>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
+                case 3: ok = chunk_sweep<T, 3>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
+                case 4: ok = chunk_sweep<T, 4>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
+                case 5: ok = chunk_sweep<T, 5>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
+                case 6: ok = chunk_sweep<T, 6>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
+                case 7: ok = chunk_sweep<T, 7>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
+                case 8: ok = chunk_sweep<T, 8>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
+                default: break;
+                }
+            }
+        } else {
+            switch (sp.run) {
+            case 2: ok = chunk_spread<D, T, 2>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
+            case 4: ok = chunk_spread<D, T, 4>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
+            case 8: ok = chunk_spread<D, T, 8>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
+            default: break;
+            }
+        }
+        if (!ok) return false;
+    }
+    return true;
 }
 
 #define MNUFFT_SPREAD_INSTANTIATE(D, T)                                                           \

@@ -9,7 +9,7 @@ namespace mnufft {
 struct Spreader {
     ~Spreader();
     void release();
-    bool ensure(int64_t nblocks, int64_t partial_need);
+    bool ensure(int64_t nblocks, int64_t partial_need, int64_t vectors);
 
     // Spreads the prepared records (in bin order) of `vectors` strength
     // vectors onto `grid` (vectors * fine_total complex values).  Every cell
@@ -21,7 +21,8 @@ struct Spreader {
                 int64_t vectors, T *grid, hipStream_t stream,
                 SpreadMethod method = SpreadMethod::Plain);
 
-    // Optional events recorded right before and after the spread kernel.
+    // Optional events recorded right before the first and after the last
+    // spread kernel launch.
     hipEvent_t kernel_start = nullptr, kernel_end = nullptr;
 
     uint32_t *totals = nullptr;      // records in reach per block
@@ -29,21 +30,25 @@ struct Spreader {
     uint32_t *item_off = nullptr;    // first work item per block
     uint32_t *heavy_base = nullptr;  // first partial slot per heavy block
     uint32_t *heavy_list = nullptr;  // heavy blocks
-    uint32_t *ctrl = nullptr;        // items, heavy parts, heavy blocks, part size,
-                                     // largest strength magnitude (float bits)
+    uint32_t *ctrl = nullptr;        // items, heavy parts, heavy blocks, part size
+    uint32_t *vmax = nullptr;        // largest strength magnitude per vector (float bits)
     void *partial = nullptr;
     int64_t blocks_cap = 0;
     int64_t partial_bytes = 0;
+    int64_t vmax_cap = 0;
     int64_t part_capacity = 0;
     // Records in reach above which a block is split into parts, and the
     // smallest part.
     uint32_t light_limit = 4096;
     uint32_t min_part = 1024;
-    // Scratch for partial sums of heavy parts.
+    // Partial-sum slots of one vector, and the scratch a chunk of vectors
+    // may take (the chunk is at least one vector).
     int64_t budget_bytes = (int64_t)256 << 20;
+    int64_t chunk_budget_bytes = (int64_t)1 << 30;
 };
 
-// Work plan of the 16^3 tiles handed to the matrix-core tile kernel.
+// Work plan of the tiles (16^3 in 3D, 32^2 in 2D) handed to the matrix-core
+// tile kernel.
 struct TileWork {
     const uint32_t *totals, *parts, *item_off, *heavy_base, *ctrl, *cmax;
     unsigned items_grid;
@@ -51,14 +56,15 @@ struct TileWork {
 
 // Tile kernel of the matrix-core spread (spread_mfma.hip); partial sums of
 // heavy parts go to `partial` in the layout of the row-sweep tiles.
-bool launch_mfma_tiles(SpreadMethod method, const GridParams &g, const SpreadParams &sp,
+bool launch_mfma_tiles(SpreadMethod method, int dim, const GridParams &g, const SpreadParams &sp,
                        const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
                        const float *c_sorted, int64_t M, int64_t vectors, const TileWork &w,
                        float *grid, float *partial, hipStream_t stream);
 
-// out (cleared by the caller) receives the largest |re| or |im| of the n
-// floats at c as float bits.
-bool launch_absmax(const float *c, int64_t n, uint32_t *out, hipStream_t stream);
+// out[v] (cleared by the caller) receives the largest |re| or |im| of the n
+// floats at c + v * n as float bits, for v < vectors.
+bool launch_absmax(const float *c, int64_t n, int64_t vectors, uint32_t *out,
+                   hipStream_t stream);
 
 // Output-stationary blocks for a fine grid: 64 lanes, `run` cells per lane
 // along axis 0 (run + width - 1 <= n0 keeps every cell's support offset
@@ -68,5 +74,9 @@ SpreadParams make_spread_params(int d, const int64_t *fine, int width);
 // Row-sweep tiles (3D, width <= 8, every axis at least tile + width long):
 // run = 0 marks them.  Returns false when the grid does not qualify.
 bool make_sweep_params(int d, const int64_t *fine, int width, SpreadParams *sp);
+
+// 32 x 32 tiles of the 2D matrix-core spread (width <= 8, every axis at least
+// 32 + width long), run = 0.  Returns false when the grid does not qualify.
+bool make_tile2_params(int d, const int64_t *fine, int width, SpreadParams *sp);
 
 }  // namespace mnufft

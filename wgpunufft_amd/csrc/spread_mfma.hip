@@ -221,21 +221,23 @@ __device__ inline TileItem tile_item(int64_t item, int64_t block_count,
     return t;
 }
 
-// Largest |re|, |im| over the strengths, as float bits (NaN above Inf).
+// Largest |re|, |im| of each vector's strengths (blockIdx.y), as float bits
+// (NaN above Inf).
 __global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ c, int64_t n,
                                                 uint32_t *__restrict__ out)
 {
+    const float *cv = c + (int64_t)blockIdx.y * n;
     uint32_t m = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t u = __float_as_uint(c[i]) & 0x7fffffffu;
+        const uint32_t u = __float_as_uint(cv[i]) & 0x7fffffffu;
         m = u > m ? u : m;
     }
     for (int o = 32; o > 0; o >>= 1) {
         const uint32_t y = __shfl_xor(m, o, 64);
         m = y > m ? y : m;
     }
-    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out + blockIdx.y, m);
 }
 
 template <int W, int MODE>
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const int bpart = r32 & 1;
 
     float scale, unscale;
-    strength_scale(cmax, &scale, &unscale);
+    strength_scale(cmax + blockIdx.y, &scale, &unscale);
 
     const int64_t v = blockIdx.y;
     const float beta = (float)g.beta;
@@ -416,7 +418,177 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
 }
 
-template <int W>
+// 2D tiles of 32 x 32 cells: G[y, (x, re|im)] += sum_j Ky_j[y] * Kx_j[x] c_j,
+// one 32-row block (the tile's y cells) against two 32-column blocks (x cells
+// 0..15 and 16..31, real and imaginary parts interleaved).  The 16 records of
+// a k-block share one A operand (their y weights) across both column blocks.
+constexpr int kTile2 = 32;
+
+template <int W, int MODE>
+__global__ __launch_bounds__(64) void k_spread_mfma2(
+    GridParams g, SpreadParams sp, const uint32_t *__restrict__ offsets,
+    const int4 *__restrict__ rec_start, const float *__restrict__ rec_off,
+    const float *__restrict__ c_sorted, int64_t M, const uint32_t *__restrict__ totals,
+    const uint32_t *__restrict__ parts, const uint32_t *__restrict__ item_off,
+    const uint32_t *__restrict__ heavy_base, const uint32_t *__restrict__ ctrl,
+    const uint32_t *__restrict__ cmax, float *__restrict__ grid, float *__restrict__ partial)
+{
+    using Op = Operand<MODE>;
+    using V = typename Op::V;
+    constexpr bool kSplit = MODE != (int)SpreadMethod::MfmaF16;
+    constexpr int NROWS = kTile2 + W - 1;
+    __shared__ __align__(16) float KX[kTile2 * kPitch];
+    __shared__ __align__(16) float KY[kTile2 * kPitch];
+    __shared__ __align__(16) float CC[2 * kPitch];
+    __shared__ MfmaRec st[64];
+    __shared__ RowBatch rows;
+
+    const int lane = threadIdx.x;
+    const int r32 = lane & 31;
+    const int half = lane >> 5;
+    const int j0 = 8 * half;
+    const int bxc = r32 >> 1;  // B column: x cell within the column block
+    const int bpart = r32 & 1;
+
+    float scale, unscale;
+    strength_scale(cmax + blockIdx.y, &scale, &unscale);
+    const int64_t v = blockIdx.y;
+    const float beta = (float)g.beta;
+    const int64_t n_items = ctrl[0];
+    const uint32_t part_len_cap = ctrl[3];
+    const float *cv = c_sorted + 2 * v * M;
+    const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n;
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const TileItem ti = tile_item(item, sp.block_count, item_off, parts, totals, part_len_cap);
+        const int64_t b = ti.b;
+        const int64_t x0 = (b % sp.nblocks[0]) * kTile2, y0 = (b / sp.nblocks[0]) * kTile2;
+        const AxisReach rx = axis_reach(x0, kTile2, W, g.ax[0]);
+        int64_t xs = x0 - (W - 1);
+        if (xs < 0) xs += n0;
+
+        f32x16 acc[2] = {f32x16{}, f32x16{}};
+        int64_t cursor = 0;
+        for (int rb = 0; rb < NROWS && cursor < ti.rhi; rb += kRows) {
+            const int64_t batch_total =
+                stage_rows<2, kTile2, W>(rows, rb, lane, y0, 0, rx, g, offsets);
+            const int64_t klo = ti.rlo > cursor ? ti.rlo - cursor : 0;
+            const int64_t khi = ti.rhi - cursor < batch_total ? ti.rhi - cursor : batch_total;
+            for (int64_t k0 = klo; k0 < khi; k0 += 64) {
+                const int64_t k = k0 + lane;
+                bool take = false;
+                MfmaRec R{};
+                if (k < khi) {
+                    int q;
+                    const int64_t slot = batch_slot(rows, k, &q);
+                    int64_t ee = (int64_t)rec_start[slot].x - xs;
+                    if (ee < 0) ee += n0;
+                    take = ee < kTile2 + W - 1;
+                    if (take) {
+                        R.sx = (int)ee - (W - 1);
+                        R.sy = rb + q - (W - 1);
+                        R.ox = rec_off[4 * slot + 0];
+                        R.oy = rec_off[4 * slot + 1];
+                        R.cre = cv[2 * slot] * scale;
+                        R.cim = cv[2 * slot + 1] * scale;
+                    }
+                }
+                const uint64_t mask = __ballot(take);
+                const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+                const int nst = __popcll(mask);
+                if (take) st[pos] = R;
+                __syncthreads();
+                for (int kb = 0; kb < nst; kb += 16) {
+                    const int kn = nst - kb < 16 ? nst - kb : 16;
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        float *tb = t == 0 ? KX : KY;
+                        *(float4 *)(tb + (lane >> 1) * kPitch + 8 * (lane & 1)) =
+                            float4{0.0f, 0.0f, 0.0f, 0.0f};
+                        *(float4 *)(tb + (lane >> 1) * kPitch + 8 * (lane & 1) + 4) =
+                            float4{0.0f, 0.0f, 0.0f, 0.0f};
+                    }
+                    __syncthreads();
+                    if (lane < 32) {
+                        const int j = lane & 15, axis = lane >> 4;
+                        if (j < kn) {
+                            const MfmaRec &S = st[kb + j];
+                            const int s0 = axis == 0 ? S.sx : S.sy;
+                            const float o = axis == 0 ? S.ox : S.oy;
+                            float *tb = axis == 0 ? KX : KY;
+#pragma unroll
+                            for (int t = 0; t < W; ++t) {
+                                const int c = s0 + t;
+                                if (c >= 0 && c < kTile2)
+                                    tb[c * kPitch + j] = es_weight<float>((float)t + o, W, beta);
+                            }
+                        }
+                    } else if (lane < 48) {
+                        const int j = lane - 32;
+                        CC[j] = j < kn ? st[kb + j].cre : 0.0f;
+                        CC[kPitch + j] = j < kn ? st[kb + j].cim : 0.0f;
+                    }
+                    __syncthreads();
+                    V ah, al;
+                    Op::split(load8(KY + r32 * kPitch + j0), ah, al);
+                    const f32x8 cc = load8(CC + bpart * kPitch + j0);
+#pragma unroll
+                    for (int nb = 0; nb < 2; ++nb) {
+                        V bh, bl;
+                        Op::split(load8(KX + (16 * nb + bxc) * kPitch + j0) * cc, bh, bl);
+                        acc[nb] = Op::mma(ah, bh, acc[nb]);
+                        if constexpr (kSplit) {
+                            acc[nb] = Op::mma(ah, bl, acc[nb]);
+                            acc[nb] = Op::mma(al, bh, acc[nb]);
+                        }
+                    }
+                    __syncthreads();
+                }
+                __syncthreads();
+            }
+            cursor += batch_total;
+            __syncthreads();
+        }
+
+        // accumulator element i: row (y) 8*(i/4) + 4*half + i%4, column r32
+        const int64_t pslot = ti.np > 1 ? (int64_t)heavy_base[b] + ti.p : 0;
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int yy = 8 * (i >> 2) + 4 * half + (i & 3);
+                const int xx = 16 * nb + bxc;
+                const float val = acc[nb][i] * unscale;
+                if (ti.np == 1) {
+                    const int64_t gx = x0 + xx, gy = y0 + yy;
+                    if (gx < n0 && gy < n1)
+                        grid[2 * (v * g.fine_total + gx + n0 * gy) + bpart] = val;
+                } else {
+                    partial[2 * ((pslot * gridDim.y + v) * (kTile2 * kTile2) + yy * kTile2 + xx) +
+                            bpart] = val;
+                }
+            }
+        }
+    }
+}
+
+template <int D, int W, int MODE>
+void launch_kernel(dim3 grid3, const GridParams &g, const SpreadParams &sp,
+                   const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
+                   const float *c_sorted, int64_t M, const TileWork &w, float *grid,
+                   float *partial, hipStream_t stream)
+{
+    if constexpr (D == 3)
+        k_spread_mfma3<W, MODE><<<grid3, 64, 0, stream>>>(
+            g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
+            w.heavy_base, w.ctrl, w.cmax, grid, partial);
+    else
+        k_spread_mfma2<W, MODE><<<grid3, 64, 0, stream>>>(
+            g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
+            w.heavy_base, w.ctrl, w.cmax, grid, partial);
+}
+
+template <int D, int W>
 bool launch_width(SpreadMethod method, dim3 grid3, const GridParams &g, const SpreadParams &sp,
                   const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
                   const float *c_sorted, int64_t M, const TileWork &w, float *grid,
@@ -424,55 +596,75 @@ bool launch_width(SpreadMethod method, dim3 grid3, const GridParams &g, const Sp
 {
     switch (method) {
     case SpreadMethod::MfmaF16:
-        k_spread_mfma3<W, (int)SpreadMethod::MfmaF16><<<grid3, 64, 0, stream>>>(
-            g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
-            w.heavy_base, w.ctrl, w.cmax, grid, partial);
+        launch_kernel<D, W, (int)SpreadMethod::MfmaF16>(grid3, g, sp, offsets, rec_start, rec_off,
+                                                        c_sorted, M, w, grid, partial, stream);
         return true;
     case SpreadMethod::MfmaF16x3:
-        k_spread_mfma3<W, (int)SpreadMethod::MfmaF16x3><<<grid3, 64, 0, stream>>>(
-            g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
-            w.heavy_base, w.ctrl, w.cmax, grid, partial);
+        launch_kernel<D, W, (int)SpreadMethod::MfmaF16x3>(grid3, g, sp, offsets, rec_start,
+                                                          rec_off, c_sorted, M, w, grid, partial,
+                                                          stream);
         return true;
     case SpreadMethod::MfmaBf16x3:
-        k_spread_mfma3<W, (int)SpreadMethod::MfmaBf16x3><<<grid3, 64, 0, stream>>>(
-            g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
-            w.heavy_base, w.ctrl, w.cmax, grid, partial);
+        launch_kernel<D, W, (int)SpreadMethod::MfmaBf16x3>(grid3, g, sp, offsets, rec_start,
+                                                           rec_off, c_sorted, M, w, grid, partial,
+                                                           stream);
         return true;
     default:
         return false;
     }
 }
 
+template <int D>
+bool launch_dim(SpreadMethod method, dim3 grid3, const GridParams &g, const SpreadParams &sp,
+                const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
+                const float *c_sorted, int64_t M, const TileWork &w, float *grid, float *partial,
+                hipStream_t stream)
+{
+#define MNUFFT_MFMA_WIDTH(Wd)                                                                     \
+    case Wd:                                                                                      \
+        return launch_width<D, Wd>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted,  \
+                                   M, w, grid, partial, stream);
+    switch (g.width) {
+        MNUFFT_MFMA_WIDTH(2)
+        MNUFFT_MFMA_WIDTH(3)
+        MNUFFT_MFMA_WIDTH(4)
+        MNUFFT_MFMA_WIDTH(5)
+        MNUFFT_MFMA_WIDTH(6)
+        MNUFFT_MFMA_WIDTH(7)
+        MNUFFT_MFMA_WIDTH(8)
+    default:
+        return false;
+    }
+#undef MNUFFT_MFMA_WIDTH
+}
+
 }  // namespace
 
-bool launch_absmax(const float *c, int64_t n, uint32_t *out, hipStream_t stream)
+bool launch_absmax(const float *c, int64_t n, int64_t vectors, uint32_t *out,
+                   hipStream_t stream)
 {
     int64_t blocks = (n + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
+    if (blocks > 1024) blocks = 1024;
     if (blocks < 1) blocks = 1;
-    k_absmax<<<(unsigned)blocks, 256, 0, stream>>>(c, n, out);
+    k_absmax<<<dim3((unsigned)blocks, (unsigned)vectors), 256, 0, stream>>>(c, n, out);
     return hip_ok(hipGetLastError(), "strength range launch");
 }
 
-bool launch_mfma_tiles(SpreadMethod method, const GridParams &g, const SpreadParams &sp,
+bool launch_mfma_tiles(SpreadMethod method, int dim, const GridParams &g, const SpreadParams &sp,
                        const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
                        const float *c_sorted, int64_t M, int64_t vectors, const TileWork &w,
                        float *grid, float *partial, hipStream_t stream)
 {
     dim3 grid3(w.items_grid, (unsigned)vectors);
     bool ok = false;
-    switch (g.width) {
-    case 2: ok = launch_width<2>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
-    case 3: ok = launch_width<3>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
-    case 4: ok = launch_width<4>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
-    case 5: ok = launch_width<5>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
-    case 6: ok = launch_width<6>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
-    case 7: ok = launch_width<7>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
-    case 8: ok = launch_width<8>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
-    default: break;
-    }
+    if (dim == 3)
+        ok = launch_dim<3>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid,
+                           partial, stream);
+    else if (dim == 2)
+        ok = launch_dim<2>(method, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid,
+                           partial, stream);
     if (!ok) {
-        set_error("matrix-core spread needs a width <= 8 and an MFMA spread method");
+        set_error("matrix-core spread needs a 2D or 3D plan, a width <= 8 and an MFMA method");
         return false;
     }
     return hip_ok(hipGetLastError(), "matrix-core spread launch");
