@@ -170,6 +170,18 @@ int mnufft_timing_history(mnufft_plan plan, int32_t on);
 int mnufft_set_spread_method(mnufft_plan plan, int32_t method);
 
 /*
+ * Work-plan knobs of the type-1 spread (no counterpart in the reference):
+ * a tile whose reach holds more than light_limit records is split into
+ * parts of at least min_part records, with partial sums in slots of at most
+ * part_budget_bytes per vector (the light limit is raised until the heavy
+ * tiles fit half the slots).  Zero keeps a knob's current value (defaults
+ * 4096, 1024, 256 MiB).  Results stay bitwise repeatable for fixed knobs;
+ * other knobs change the summation order of heavy tiles only.
+ */
+int mnufft_set_spread_tuning(mnufft_plan plan, int64_t light_limit, int64_t min_part,
+                             int64_t part_budget_bytes);
+
+/*
  * FP64 direct sums on the GPU: reference_type1_f64 / reference_type2_f64
  * (src/direct.rs:9-118) and reference_type3_f64 (src/type3.rs:609-673).
  * kind 1: out[o] = sum_j in[j] exp(i sign k_o.x_j) over the linear mode

@@ -363,3 +363,61 @@ def test_type2_batched_tiled_interp_matches_oracle():
         ref = O.direct_type2(x.astype(np.float64), fm[v * N:(v + 1) * N].astype(np.complex128),
                              n_modes)
         assert rel_l2(out[v * M:(v + 1) * M], ref) <= 32e-6
+
+
+def test_spread_tuning_small_budget_matches_default():
+    """A tiny partial-sum budget forces the light limit up and makes parts
+    so large that some heavy tiles fit one part (written directly); the
+    transform stays the same to rounding, for the plain and matrix-core
+    spreads."""
+    torch, W, O, S = _setup()
+    n_modes = [32, 32, 32]
+    M = 200000
+    x = S.clustered_coords(S.config_seed(2, 1100), M, (0.3, -1.1, 2.0), 0.01)
+    x[: M // 4] = S.uniform_coords(S.config_seed(2, 1101), M // 4, 3)
+    c = S.complex_normal(S.config_seed(2, 1102), M)
+    xt, ct = torch.from_numpy(x).cuda(), torch.from_numpy(c).cuda()
+    sel = np.arange(0, int(np.prod(n_modes)), 101)
+    ref = O.direct_type1(x.astype(np.float64), c.astype(np.complex128), n_modes, modes=sel)
+    for method in ("plain", "mfma_f16x3"):
+        base = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6)).set_spread_method(method)
+        tuned = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6)).set_spread_method(method)
+        tuned.set_spread_tuning(light_limit=256, min_part=64, part_budget_bytes=1)
+        a = base.encode_gpu(xt, ct).cpu().numpy()
+        b = tuned.encode_gpu(xt, ct).cpu().numpy()
+        b2 = tuned.encode_gpu(xt, ct).cpu().numpy()
+        assert np.array_equal(b.view(np.uint32), b2.view(np.uint32))
+        assert rel_l2(b, a) <= 1e-6
+        assert rel_l2(b[sel], ref) <= 32e-6
+
+
+def test_batch_beyond_2_32_values():
+    """32 vectors over a 512^3 fine grid hold 2^32 complex values: every
+    vector, the last included, is the single-vector transform (the rocFFT
+    executions and elementwise launches are split below 2^32 lanes /
+    values); identical inputs give bitwise identical vectors."""
+    torch, W, O, S = _setup()
+    n_modes = [256, 256, 256]
+    N = int(np.prod(n_modes))
+    M, B = 20000, 32
+    x = torch.from_numpy(S.uniform_coords(S.config_seed(4, 1200), M, 3)).cuda()
+    c = torch.from_numpy(S.complex_normal(S.config_seed(4, 1201), M)).cuda()
+    single = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6))
+    f1 = single.encode_gpu(x, c)
+    plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
+    fb = plan.encode_gpu(x, c.repeat(B)).view(B, N)
+    for v in (B // 2 - 1, B // 2, B - 1):
+        assert torch.equal(fb[v], fb[0]), v
+    assert rel_l2(fb[B - 1].cpu().numpy(), f1.cpu().numpy()) <= 1e-6
+    del plan, fb
+    fm = torch.from_numpy(S.complex_normal(S.config_seed(4, 1202), N)).cuda()
+    single2 = W.NufftPlan.type2_gpu(W.NufftConfig(n_modes, 1e-6))
+    c1 = single2.encode_gpu(x, fm)
+    plan2 = W.NufftPlan.type2_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
+    cb = plan2.encode_gpu(x, fm.repeat(B)).view(B, M)
+    for v in (B // 2 - 1, B // 2, B - 1):
+        assert torch.equal(cb[v], cb[0]), v
+    assert rel_l2(cb[B - 1].cpu().numpy(), c1.cpu().numpy()) <= 1e-6
+    sel = torch.arange(0, M, 97, device="cuda")
+    r2 = W.reference_type2_f64(n_modes, x.double(), fm.to(torch.complex128), points=sel)
+    assert rel_l2(cb[B - 1][sel].cpu().numpy(), r2.cpu().numpy()) <= 32e-6

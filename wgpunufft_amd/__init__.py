@@ -128,6 +128,14 @@ class _PlanBase:
         check(lib().mnufft_set_spread_method(self._handle, code))
         return self
 
+    def set_spread_tuning(self, light_limit=0, min_part=0, part_budget_bytes=0):
+        """Work-plan knobs of the type-1 spread (0 keeps the current value):
+        records above which a tile is split into parts, the smallest part,
+        and the partial-sum budget per vector in bytes."""
+        check(lib().mnufft_set_spread_tuning(self._handle, int(light_limit), int(min_part),
+                                             int(part_budget_bytes)))
+        return self
+
     def timing_history(self, on=True):
         """Sum stage times over every call until the next stage_times()."""
         check(lib().mnufft_timing_history(self._handle, 1 if on else 0))

@@ -363,6 +363,22 @@ int mnufft_set_spread_method(mnufft_plan plan, int32_t method)
     return MNUFFT_SUCCESS;
 }
 
+int mnufft_set_spread_tuning(mnufft_plan plan, int64_t light_limit, int64_t min_part,
+                             int64_t part_budget_bytes)
+{
+    if (!plan || light_limit < 0 || min_part < 0 || part_budget_bytes < 0 ||
+        light_limit > 0x7fffffff || min_part > 0x7fffffff) {
+        set_error("invalid spread tuning");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    Plan &p = *plan->plan;
+    Plan &q = p.type == 3 && p.outer ? *p.outer : p;
+    if (light_limit) q.spreader.light_limit = (uint32_t)light_limit;
+    if (min_part) q.spreader.min_part = (uint32_t)min_part;
+    if (part_budget_bytes) q.spreader.budget_bytes = part_budget_bytes;
+    return MNUFFT_SUCCESS;
+}
+
 int mnufft_direct_f64(int32_t kind, int32_t dim, const int64_t *n_modes, int32_t isign,
                       int32_t mode_order, int64_t M, const double *d_x, const double *d_in,
                       int64_t nout, const int64_t *d_sel, const double *d_s, double *d_out,

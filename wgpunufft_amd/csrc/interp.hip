@@ -15,21 +15,24 @@
 
 #include "fold.hpp"
 #include "interp.hpp"
+#include "spread_device.hpp"
 
 namespace mnufft {
 
 namespace {
 
+// FP32 weights from the hardware sqrt/exp2 (es_weight_fast), FP64 exact.
 template <typename T>
 __device__ inline T es_w(T d, int w, T beta)
 {
-    T s = (T)2 * d / (T)w;
-    T q = (T)1 - s * s;
-    if (!(q > (T)0)) return (T)0;
-    if constexpr (sizeof(T) == 4)
-        return expf(beta * (sqrtf(q) - 1.0f));
-    else
+    if constexpr (sizeof(T) == 4) {
+        return es_weight_fast(d, 2.0f / (float)w, beta * 1.44269504088896341f);
+    } else {
+        T s = (T)2 * d / (T)w;
+        T q = (T)1 - s * s;
+        if (!(q > (T)0)) return (T)0;
         return exp(beta * (sqrt(q) - 1.0));
+    }
 }
 
 __device__ inline int64_t wrap_idx(int64_t i, int64_t n)
@@ -47,15 +50,13 @@ __global__ __launch_bounds__(256) void k_interp(int64_t M, int64_t vectors,
 {
     constexpr int P = 256 / G;
     const int j = threadIdx.x % G;
-    const int64_t item = (int64_t)blockIdx.x * P + threadIdx.x / G;
-    const bool has = item < M * vectors;
+    // blockIdx.y = vector, so each launch dimension stays within 2^32 lanes
+    const int64_t v = blockIdx.y;
+    const int64_t s0 = (int64_t)blockIdx.x * P + threadIdx.x / G;
+    const bool has = s0 < M;
+    const int64_t s = has ? s0 : 0;
     const int w = g.width;
     const T beta = (T)g.beta;
-    int64_t v = 0, s = 0;
-    if (has) {
-        v = item / M;
-        s = item - v * M;
-    }
     uint32_t idx = has ? sorted[s] : 0u;
     bool reach = has && point_in_reach<D, T>(coords, idx);
     int64_t st[3] = {0, 0, 0};
@@ -286,8 +287,7 @@ bool launch(int64_t M, int64_t vectors, const uint32_t *sorted, const T *coords,
             const GridParams &g, const T *grid, T *out, hipStream_t stream)
 {
     constexpr int P = 256 / G;
-    int64_t items = M * vectors;
-    unsigned blocks = (unsigned)((items + P - 1) / P);
+    dim3 blocks((unsigned)((M + P - 1) / P), (unsigned)vectors);
     k_interp<D, T, G><<<blocks, 256, 0, stream>>>(M, vectors, sorted, coords, g, grid, out);
     return hip_ok(hipGetLastError(), "interpolation launch");
 }

@@ -28,7 +28,10 @@ namespace {
 
 inline unsigned grid_for(int64_t n, int threads)
 {
+    // the elementwise kernels loop with a grid stride: at most 2^20
+    // workgroups keeps blocks x threads far below 2^32 work-items
     int64_t b = (n + threads - 1) / threads;
+    if (b > ((int64_t)1 << 20)) b = (int64_t)1 << 20;
     return (unsigned)(b < 1 ? 1 : b);
 }
 
@@ -37,27 +40,29 @@ __global__ __launch_bounds__(256) void k_deconv(ModeParams mp, const double *__r
                                                 const T *__restrict__ grid, T *__restrict__ modes,
                                                 int64_t vectors)
 {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= mp.mode_total * vectors) return;
-    int64_t v = t / mp.mode_total;
-    int64_t k = t - v * mp.mode_total;
-    int64_t rest = k, fine_lin = 0, stride = 1;
-    double amp = 1.0;
+    const int64_t n_total = mp.mode_total * vectors;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t v = t / mp.mode_total;
+        int64_t k = t - v * mp.mode_total;
+        int64_t rest = k, fine_lin = 0, stride = 1;
+        double amp = 1.0;
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
-        int64_t N = mp.n_modes[a], n = mp.fine[a];
-        int64_t ka = rest % N;
-        rest /= N;
-        int64_t m = mp.centered ? ka - N / 2 : (ka < (N + 1) / 2 ? ka : ka - N);
-        int64_t f = m < 0 ? m + n : m;
-        fine_lin += f * stride;
-        stride *= n;
-        amp *= inv_phihat[mp.phihat_off[a] + (m < 0 ? -m : m)];
+        for (int a = 0; a < D; ++a) {
+            int64_t N = mp.n_modes[a], n = mp.fine[a];
+            int64_t ka = rest % N;
+            rest /= N;
+            int64_t m = mp.centered ? ka - N / 2 : (ka < (N + 1) / 2 ? ka : ka - N);
+            int64_t f = m < 0 ? m + n : m;
+            fine_lin += f * stride;
+            stride *= n;
+            amp *= inv_phihat[mp.phihat_off[a] + (m < 0 ? -m : m)];
+        }
+        const T *src = grid + 2 * (v * mp.fine_total + fine_lin);
+        T a = (T)amp;
+        modes[2 * t] = src[0] * a;
+        modes[2 * t + 1] = src[1] * a;
     }
-    const T *src = grid + 2 * (v * mp.fine_total + fine_lin);
-    T a = (T)amp;
-    modes[2 * t] = src[0] * a;
-    modes[2 * t + 1] = src[1] * a;
 }
 
 template <int D, typename T>
@@ -66,49 +71,52 @@ __global__ __launch_bounds__(256) void k_predeconv(ModeParams mp,
                                                    const T *__restrict__ modes,
                                                    T *__restrict__ grid, int64_t vectors)
 {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= mp.fine_total * vectors) return;
-    int64_t v = t / mp.fine_total;
-    int64_t f = t - v * mp.fine_total;
-    int64_t rest = f, mode_lin = 0, stride = 1;
-    bool has = true;
-    double amp = 1.0;
+    const int64_t n_total = mp.fine_total * vectors;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t v = t / mp.fine_total;
+        int64_t f = t - v * mp.fine_total;
+        int64_t rest = f, mode_lin = 0, stride = 1;
+        bool has = true;
+        double amp = 1.0;
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
-        int64_t N = mp.n_modes[a], n = mp.fine[a];
-        int64_t fa = rest % n;
-        rest /= n;
-        int64_t hi = N - 1 - N / 2;  // largest frequency
-        int64_t m;
-        if (fa <= hi)
-            m = fa;
-        else if (fa >= n - N / 2)
-            m = fa - n;
-        else {
-            has = false;
-            m = 0;
+        for (int a = 0; a < D; ++a) {
+            int64_t N = mp.n_modes[a], n = mp.fine[a];
+            int64_t fa = rest % n;
+            rest /= n;
+            int64_t hi = N - 1 - N / 2;  // largest frequency
+            int64_t m;
+            if (fa <= hi)
+                m = fa;
+            else if (fa >= n - N / 2)
+                m = fa - n;
+            else {
+                has = false;
+                m = 0;
+            }
+            int64_t ka = mp.centered ? m + N / 2 : (m >= 0 ? m : m + N);
+            mode_lin += ka * stride;
+            stride *= N;
+            amp *= inv_phihat[mp.phihat_off[a] + (m < 0 ? -m : m)];
         }
-        int64_t ka = mp.centered ? m + N / 2 : (m >= 0 ? m : m + N);
-        mode_lin += ka * stride;
-        stride *= N;
-        amp *= inv_phihat[mp.phihat_off[a] + (m < 0 ? -m : m)];
+        T re = (T)0, im = (T)0;
+        if (has) {
+            const T *src = modes + 2 * (v * mp.mode_total + mode_lin);
+            T a = (T)amp;
+            re = src[0] * a;
+            im = src[1] * a;
+        }
+        grid[2 * t] = re;
+        grid[2 * t + 1] = im;
     }
-    T re = (T)0, im = (T)0;
-    if (has) {
-        const T *src = modes + 2 * (v * mp.mode_total + mode_lin);
-        T a = (T)amp;
-        re = src[0] * a;
-        im = src[1] * a;
-    }
-    grid[2 * t] = re;
-    grid[2 * t + 1] = im;
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_fill(T *__restrict__ p, int64_t n)
 {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) p[t] = (T)0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = (T)0;
 }
 
 // ---------------------------------------------------------------------
@@ -120,21 +128,23 @@ __global__ __launch_bounds__(256) void k_type3_source(Type3Params tp, int64_t M,
                                                       const T *__restrict__ c,
                                                       T *__restrict__ xr, T *__restrict__ cp)
 {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= M * vectors) return;
-    int64_t j = t % M;
-    double phase = 0.0;
+    const int64_t n_total = M * vectors;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t j = t % M;
+        double phase = 0.0;
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
-        double xa = (double)x[j * D + a];
-        if (t < M) xr[j * D + a] = (T)((xa - tp.src_center[a]) * tp.inv_gamma[a]);
-        phase += xa * tp.tgt_center[a];
+        for (int a = 0; a < D; ++a) {
+            double xa = (double)x[j * D + a];
+            if (t < M) xr[j * D + a] = (T)((xa - tp.src_center[a]) * tp.inv_gamma[a]);
+            phase += xa * tp.tgt_center[a];
+        }
+        double sn, cs;
+        sincos(phase * tp.sign, &sn, &cs);
+        double re = (double)c[2 * t], im = (double)c[2 * t + 1];
+        cp[2 * t] = (T)(re * cs - im * sn);
+        cp[2 * t + 1] = (T)(re * sn + im * cs);
     }
-    double sn, cs;
-    sincos(phase * tp.sign, &sn, &cs);
-    double re = (double)c[2 * t], im = (double)c[2 * t + 1];
-    cp[2 * t] = (T)(re * cs - im * sn);
-    cp[2 * t + 1] = (T)(re * sn + im * cs);
 }
 
 template <int D, typename T>
@@ -143,23 +153,25 @@ __global__ __launch_bounds__(256) void k_type3_target(Type3Params tp, int64_t N,
                                                       T *__restrict__ sr,
                                                       double *__restrict__ factor)
 {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    double phase = 0.0, phi = 1.0;
+    const int64_t n_total = N;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_total;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        double phase = 0.0, phi = 1.0;
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
-        double cen = (double)s[k * D + a] - tp.tgt_center[a];
-        double resc = cen * tp.tgt_scale[a];
-        sr[k * D + a] = (T)resc;
-        phase += cen * tp.src_center[a];
-        double ft = 0.0;
-        for (int q = 0; q < tp.nquad; ++q) ft += tp.quad_w[q] * cos(resc * tp.quad_x[q]);
-        phi *= ft;
+        for (int a = 0; a < D; ++a) {
+            double cen = (double)s[k * D + a] - tp.tgt_center[a];
+            double resc = cen * tp.tgt_scale[a];
+            sr[k * D + a] = (T)resc;
+            phase += cen * tp.src_center[a];
+            double ft = 0.0;
+            for (int q = 0; q < tp.nquad; ++q) ft += tp.quad_w[q] * cos(resc * tp.quad_x[q]);
+            phi *= ft;
+        }
+        double sn, cs;
+        sincos(phase * tp.sign, &sn, &cs);
+        factor[2 * k] = cs / phi;
+        factor[2 * k + 1] = sn / phi;
     }
-    double sn, cs;
-    sincos(phase * tp.sign, &sn, &cs);
-    factor[2 * k] = cs / phi;
-    factor[2 * k + 1] = sn / phi;
 }
 
 template <typename T>
@@ -168,13 +180,15 @@ __global__ __launch_bounds__(256) void k_type3_final(int64_t N, int64_t vectors,
                                                      const double *__restrict__ factor,
                                                      T *__restrict__ out)
 {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N * vectors) return;
-    int64_t k = t % N;
-    double fr = factor[2 * k], fi = factor[2 * k + 1];
-    double re = (double)interp[2 * t], im = (double)interp[2 * t + 1];
-    out[2 * t] = (T)(re * fr - im * fi);
-    out[2 * t + 1] = (T)(re * fi + im * fr);
+    const int64_t n_total = N * vectors;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t k = t % N;
+        double fr = factor[2 * k], fi = factor[2 * k + 1];
+        double re = (double)interp[2 * t], im = (double)interp[2 * t + 1];
+        out[2 * t] = (T)(re * fr - im * fi);
+        out[2 * t + 1] = (T)(re * fi + im * fr);
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -309,7 +323,12 @@ bool type3_final(int64_t N, int64_t vectors, const T *interp, const double *fact
 bool direct_sum(int kind, int d, const DirectParams &dp, const double *x, const double *in,
                 const int64_t *sel, const double *s, double *out, hipStream_t stream)
 {
-    unsigned g = grid_for(dp.nout, 256);
+    // one lane per output, no grid stride
+    if (dp.nout > ((int64_t)1 << 31)) {
+        set_error("direct sums take at most 2^31 outputs per call");
+        return false;
+    }
+    const unsigned g = (unsigned)std::max<int64_t>(1, (dp.nout + 255) / 256);
     switch (d) {
     case 1: k_direct<1><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
     case 2: k_direct<2><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;

@@ -343,14 +343,21 @@ FftPlan *Plan::fft_for(int64_t vectors)
 
 bool Plan::fft(int64_t vectors, void *data, hipStream_t stream)
 {
-    FftPlan *fp = fft_for(vectors);
-    if (!fp) return false;
-    rocfft_execution_info info = (rocfft_execution_info)fp->info;
-    rocfft_execution_info_set_stream(info, stream);
-    void *buf[1] = {data};
-    if (rocfft_execute((rocfft_plan)fp->plan, buf, nullptr, info) != rocfft_status_success) {
-        set_error("rocFFT execution failed");
-        return false;
+    // Batches of at most 2^31 complex values per rocFFT execution: larger
+    // batched executions were seen to corrupt their last transform.
+    const int64_t per_exec = std::max<int64_t>(1, ((int64_t)1 << 31) / fine_total);
+    const size_t vbytes = (size_t)fine_total * 2 * real_bytes();
+    for (int64_t v0 = 0; v0 < vectors; v0 += per_exec) {
+        const int64_t vc = std::min<int64_t>(per_exec, vectors - v0);
+        FftPlan *fp = fft_for(vc);
+        if (!fp) return false;
+        rocfft_execution_info info = (rocfft_execution_info)fp->info;
+        rocfft_execution_info_set_stream(info, stream);
+        void *buf[1] = {(char *)data + (size_t)v0 * vbytes};
+        if (rocfft_execute((rocfft_plan)fp->plan, buf, nullptr, info) != rocfft_status_success) {
+            set_error("rocFFT execution failed");
+            return false;
+        }
     }
     return true;
 }

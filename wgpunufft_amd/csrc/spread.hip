@@ -171,12 +171,17 @@ __global__ __launch_bounds__(1024) void k_plan(int64_t nblocks, const uint32_t *
         uint32_t np = 0, hp = 0, hb = 0;
         if (b < nblocks) {
             uint32_t t = totals[b];
+            np = 1;
             if (t > light_limit) {
                 np = (uint32_t)((t + (uint64_t)part - 1) / part);
-                hp = np;
-                hb = 1;
-            } else {
-                np = 1;
+                // a block that fits one part is written directly (np == 1
+                // means light to the tile kernels and the reduction)
+                if (np > 1) {
+                    hp = np;
+                    hb = 1;
+                } else {
+                    np = 1;
+                }
             }
             parts[b] = np;
         }

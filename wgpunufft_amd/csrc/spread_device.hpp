@@ -21,6 +21,20 @@ __device__ inline T es_weight(T d, int w, T beta)
         return exp(beta * (sqrt(q) - 1.0));
 }
 
+// The FP32 weight from the hardware square root and base-2 exponential
+// (v_sqrt_f32, v_exp_f32): a few ulp where the weight is near one and an
+// absolute error below 1e-12 at the support edges (weight about e^-beta),
+// against the correctly rounded sqrtf/expf sequence above.  Used where the
+// weights feed FP32 sums of many terms (interpolation) or operands rounded
+// to FP16 (matrix-core spread).
+__device__ inline float es_weight_fast(float d, float two_over_w, float beta_log2e)
+{
+    const float s = d * two_over_w;
+    const float q = 1.0f - s * s;
+    if (!(q > 0.0f)) return 0.0f;
+    return __builtin_amdgcn_exp2f(beta_log2e * (__builtin_amdgcn_sqrtf(q) - 1.0f));
+}
+
 // Bin intervals of one axis that hold the support starts reaching a block
 // (starts b0-w+1 .. b0+B-1, periodic).  A reach within one bin side of the
 // whole axis takes every bin once.

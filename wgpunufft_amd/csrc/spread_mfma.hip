@@ -274,7 +274,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     strength_scale(cmax + blockIdx.y, &scale, &unscale);
 
     const int64_t v = blockIdx.y;
-    const float beta = (float)g.beta;
+    const float two_over_w = 2.0f / (float)W;
+    const float beta_l2e = (float)(g.beta * 1.4426950408889634);
     const int64_t n_items = ctrl[0];
     const uint32_t part_len_cap = ctrl[3];
     const float *cv = c_sorted + 2 * v * M;
@@ -351,7 +352,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                             for (int t = 0; t < W; ++t) {
                                 const int c = s + t;
                                 if (c >= 0 && c < kTile)
-                                    tb[c * kPitch + j] = es_weight<float>((float)t + o, W, beta);
+                                    tb[c * kPitch + j] = es_weight_fast((float)t + o, two_over_w, beta_l2e);
                             }
                         }
                     } else {
@@ -453,7 +454,8 @@ __global__ __launch_bounds__(64) void k_spread_mfma2(
     float scale, unscale;
     strength_scale(cmax + blockIdx.y, &scale, &unscale);
     const int64_t v = blockIdx.y;
-    const float beta = (float)g.beta;
+    const float two_over_w = 2.0f / (float)W;
+    const float beta_l2e = (float)(g.beta * 1.4426950408889634);
     const int64_t n_items = ctrl[0];
     const uint32_t part_len_cap = ctrl[3];
     const float *cv = c_sorted + 2 * v * M;
@@ -520,7 +522,7 @@ __global__ __launch_bounds__(64) void k_spread_mfma2(
                             for (int t = 0; t < W; ++t) {
                                 const int c = s0 + t;
                                 if (c >= 0 && c < kTile2)
-                                    tb[c * kPitch + j] = es_weight<float>((float)t + o, W, beta);
+                                    tb[c * kPitch + j] = es_weight_fast((float)t + o, two_over_w, beta_l2e);
                             }
                         }
                     } else if (lane < 48) {
