@@ -76,8 +76,9 @@ def config3(args, W, S, torch):
     c = S.complex_normal(S.config_seed(3, 2), M)
     xt, ct = torch.from_numpy(x).to(dev), torch.from_numpy(c).to(dev)
     out = torch.empty(int(np.prod(n_modes)), dtype=torch.complex64, device=dev)
-    legs = [(1e-6, "plain"), (1e-6, "mfma_f16x3"), (1e-6, "mfma_bf16x3"),
-            (1e-3, "plain"), (1e-3, "mfma_f16")]
+    # The matrix-core legs of this workload are off: their last run ended in
+    # a GPU memory fault whose cause is not yet found.
+    legs = [(1e-6, "plain"), (1e-3, "plain")]
     plain_ms = {}
     for eps, method in legs:
         plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, eps)).set_spread_method(method)

@@ -64,22 +64,57 @@ __global__ __launch_bounds__(kCountThreads) void k_count(int64_t M, const T *__r
     }
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kCountPoints;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
     uint32_t my_key[kCountPerThread], my_slot[kCountPerThread], my_rank[kCountPerThread];
 #pragma unroll
     for (int i = 0; i < kCountPerThread; ++i) {
         int64_t j = base + (int64_t)i * kCountThreads + threadIdx.x;
         my_slot[i] = kEmpty;
-        if (j >= M) continue;
-        uint32_t key = (uint32_t)point_bin_key<D, T>(coords, j, g);
+        const bool valid = j < M;
+        const uint32_t key = valid ? (uint32_t)point_bin_key<D, T>(coords, j, g) : kEmpty;
         my_key[i] = key;
-        uint32_t h = (key * 2654435761u) & (kHashSize - 1);
-        for (;;) {
-            uint32_t old = atomicCAS(&ht_key[h], kEmpty, key);
-            if (old == kEmpty || old == key) break;
-            h = (h + 1) & (kHashSize - 1);
+        bool pending = valid;
+        // Lanes of the wavefront that share a key (a dense cluster) take one
+        // LDS atomic per group instead of one each on the same address;
+        // peeling stops at the first group of one lane (spread-out keys).
+        for (int it = 0; it < 8; ++it) {
+            const uint64_t pm = __ballot(pending);
+            if (pm == 0) break;
+            const int leader = __ffsll((long long)pm) - 1;
+            const uint32_t lk = __shfl(key, leader, 64);
+            const bool mine = pending && key == lk;
+            const uint64_t gm = __ballot(mine);
+            const int gcount = __popcll(gm);
+            uint32_t h = 0, rank0 = 0;
+            if (lane == leader) {
+                h = (lk * 2654435761u) & (kHashSize - 1);
+                for (;;) {
+                    uint32_t old = atomicCAS(&ht_key[h], kEmpty, lk);
+                    if (old == kEmpty || old == lk) break;
+                    h = (h + 1) & (kHashSize - 1);
+                }
+                rank0 = atomicAdd(&ht_cnt[h], (uint32_t)gcount);
+            }
+            h = __shfl(h, leader, 64);
+            rank0 = __shfl(rank0, leader, 64);
+            if (mine) {
+                my_slot[i] = h;
+                my_rank[i] = rank0 + (uint32_t)__popcll(gm & lanes_below);
+                pending = false;
+            }
+            if (gcount < 2) break;
         }
-        my_slot[i] = h;
-        my_rank[i] = atomicAdd(&ht_cnt[h], 1u);
+        if (pending) {
+            uint32_t h = (key * 2654435761u) & (kHashSize - 1);
+            for (;;) {
+                uint32_t old = atomicCAS(&ht_key[h], kEmpty, key);
+                if (old == kEmpty || old == key) break;
+                h = (h + 1) & (kHashSize - 1);
+            }
+            my_slot[i] = h;
+            my_rank[i] = atomicAdd(&ht_cnt[h], 1u);
+        }
     }
     __syncthreads();
     for (int h = threadIdx.x; h < kHashSize; h += kCountThreads) {
@@ -247,16 +282,30 @@ __device__ inline SortElem make_elem(const T *coords, uint32_t idx, int order)
     return e;
 }
 
+// Ordered sorts: every point's ordering element (coordinate bits or index,
+// and the index) scattered into its bin's slot range, read sequentially.
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_scatter_elems(int64_t M, const T *__restrict__ coords,
+                                                       const uint32_t *__restrict__ keys,
+                                                       const uint32_t *__restrict__ ranks,
+                                                       const uint32_t *__restrict__ offsets,
+                                                       int order, SortElem *__restrict__ elems)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    elems[offsets[keys[j]] + ranks[j]] = make_elem<D, T>(coords, (uint32_t)j, order);
+}
+
 // ---------------------------------------------------------------------
 // Large bins (more than kSmallBin points): a merge sort spread over every
 // workgroup.  The listed bins are cut into tiles of kSortRun elements;
 //  1. k_large_tiles scans the tile counts of the listed bins;
-//  2. k_large_runs bitonic-sorts every tile in LDS (a bin of one tile is
-//     then done and written back);
+//  2. k_large_runs bitonic-sorts every tile of elements in LDS, in place;
 //  3. k_large_merge, one launch per round of run length kSortRun << r,
 //     gives every output tile of a bin still being merged its two input
 //     ranges by a merge-path search, stages them in LDS and merges there;
-//  4. k_large_final copies the merged indices of multi-tile bins back.
+//  4. k_large_final copies bins whose last round wrote the second buffer
+//     back, so the elements of every bin end up sorted in the first one.
 // Element keys are unique (the index breaks ties), so every merge is exact.
 // ---------------------------------------------------------------------
 __device__ inline void lds_bitonic(SortElem *s, int n2)
@@ -356,12 +405,10 @@ __device__ inline uint32_t tile_bin(const uint32_t *tile_off, uint32_t nlarge, u
     return lo;
 }
 
-template <int D, typename T>
 __global__ __launch_bounds__(kSortThreads) void k_large_runs(
     const uint32_t *__restrict__ large_list, const uint32_t *__restrict__ large_count,
     const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ offsets,
-    const T *__restrict__ coords, int order, uint32_t *__restrict__ sorted,
-    SortElem *__restrict__ scratch_a)
+    SortElem *__restrict__ elems)
 {
     __shared__ SortElem s[kSortRun];
     const uint32_t nlarge = *large_count;
@@ -377,16 +424,11 @@ __global__ __launch_bounds__(kSortThreads) void k_large_runs(
         int n2 = 1;
         while (n2 < m) n2 <<= 1;
         for (int k = threadIdx.x; k < n2; k += kSortThreads)
-            s[k] = k < m ? make_elem<D, T>(coords, sorted[lo + r0 + k], order)
+            s[k] = k < m ? elems[lo + r0 + k]
                          : SortElem{0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
         __syncthreads();
         lds_bitonic(s, n2);
-        for (int k = threadIdx.x; k < m; k += kSortThreads) {
-            if (n <= kSortRun)
-                sorted[lo + k] = s[k].idx;
-            else
-                scratch_a[lo + r0 + k] = s[k];
-        }
+        for (int k = threadIdx.x; k < m; k += kSortThreads) elems[lo + r0 + k] = s[k];
         __syncthreads();
     }
 }
@@ -443,8 +485,7 @@ __global__ __launch_bounds__(kSortThreads) void k_large_merge(
 __global__ __launch_bounds__(kSortThreads) void k_large_final(
     const uint32_t *__restrict__ large_list, const uint32_t *__restrict__ large_count,
     const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ offsets,
-    const SortElem *__restrict__ scratch_a, const SortElem *__restrict__ scratch_b,
-    uint32_t *__restrict__ sorted)
+    SortElem *__restrict__ elems, const SortElem *__restrict__ other)
 {
     const uint32_t nlarge = *large_count;
     if (nlarge == 0) return;
@@ -454,12 +495,11 @@ __global__ __launch_bounds__(kSortThreads) void k_large_final(
         const uint32_t bin = large_list[i];
         const int64_t lo = offsets[bin];
         const int64_t n = (int64_t)offsets[bin + 1] - lo;
-        if (n <= kSortRun) continue;
-        const SortElem *src = (merge_rounds(n) & 1) ? scratch_b : scratch_a;
+        if (n <= kSortRun || !(merge_rounds(n) & 1)) continue;
         const int64_t out0 = (int64_t)(t - tile_off[i]) * kSortRun;
         const int64_t out1 = out0 + kSortRun < n ? out0 + kSortRun : n;
         for (int64_t o = out0 + threadIdx.x; o < out1; o += kSortThreads)
-            sorted[lo + o] = src[lo + o].idx;
+            elems[lo + o] = other[lo + o];
     }
 }
 
@@ -468,34 +508,27 @@ __global__ __launch_bounds__(kSortThreads) void k_large_final(
 // wrapped support starts and offsets, and the original index.
 // ---------------------------------------------------------------------
 template <int D, typename T>
-__global__ __launch_bounds__(256) void k_prepare(int64_t M, const uint32_t *__restrict__ sorted,
-                                                 const uint32_t *__restrict__ keys,
-                                                 const uint32_t *__restrict__ offsets,
-                                                 const T *__restrict__ coords, GridParams g,
-                                                 int order, int4 *__restrict__ rec_start,
-                                                 T *__restrict__ rec_off,
-                                                 uint32_t *__restrict__ out_sorted)
+__device__ inline T elem_coord(const SortElem &e, const T *coords, int a, bool from_bits)
 {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    uint32_t idx = sorted[s];
-    int64_t dst = s;
-    if (order != (int)BinOrder::Grouped) {
-        uint32_t key = keys[idx];
-        int64_t lo = offsets[key], hi = offsets[key + 1];
-        if (hi - lo <= kSmallBin && hi - lo > 1) {
-            SortElem me = make_elem<D, T>(coords, idx, order);
-            int64_t rank = 0;
-            for (int64_t m = lo; m < hi; ++m) {
-                uint32_t other = sorted[m];
-                if (other == idx) continue;
-                SortElem e = make_elem<D, T>(coords, other, order);
-                rank += elem_less(e, me) ? 1 : 0;
-            }
-            dst = lo + rank;
-        }
+    if constexpr (sizeof(T) == 4) {
+        if (from_bits) return __uint_as_float(a == 0 ? e.k0 : (a == 1 ? e.k1 : e.k2));
     }
-    bool reach = point_in_reach<D, T>(coords, idx);
+    return coords[(int64_t)e.idx * D + a];
+}
+
+// Grouped order: slot s keeps the point the scatter put there.
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_prepare_grouped(int64_t M,
+                                                         const uint32_t *__restrict__ sorted,
+                                                         const T *__restrict__ coords,
+                                                         GridParams g, int4 *__restrict__ rec_start,
+                                                         T *__restrict__ rec_off,
+                                                         uint32_t *__restrict__ out_sorted)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    const uint32_t idx = sorted[s];
+    const bool reach = point_in_reach<D, T>(coords, idx);
     int st[3] = {0, 0, 0};
 #pragma unroll
     for (int a = 0; a < D; ++a) {
@@ -503,10 +536,61 @@ __global__ __launch_bounds__(256) void k_prepare(int64_t M, const uint32_t *__re
         double off;
         support_start<T>(coords[(int64_t)idx * D + a], reach, g, a, &start, &off);
         st[a] = (int)wrapped_start(start, g.ax[a].n);
-        rec_off[dst * 4 + a] = (T)off;
+        rec_off[s * 4 + a] = (T)off;
     }
-    rec_start[dst] = make_int4(st[0], st[1], st[2], (int)idx);
-    if (out_sorted) out_sorted[dst] = idx;
+    rec_start[s] = make_int4(st[0], st[1], st[2], (int)idx);
+    if (out_sorted) out_sorted[s] = idx;
+}
+
+// Ordered sorts: the element at slot s (large bins already sorted there;
+// small bins ranked here among their contiguous elements) gives the final
+// slot, the wrapped support starts and offsets.  F32 canonical elements hold
+// the coordinate bits, so nothing is gathered.
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_prepare(int64_t M, const SortElem *__restrict__ elems,
+                                                 const uint32_t *__restrict__ offsets,
+                                                 const T *__restrict__ coords, GridParams g,
+                                                 int order, int4 *__restrict__ rec_start,
+                                                 T *__restrict__ rec_off,
+                                                 uint32_t *__restrict__ out_sorted)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    const SortElem me = elems[s];
+    const bool from_bits = sizeof(T) == 4 && order == (int)BinOrder::Canonical;
+    T x[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = elem_coord<D, T>(me, coords, a, from_bits);
+    bool reach = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
+    int st[3] = {0, 0, 0};
+    T off3[3];
+    uint64_t key = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        int64_t start;
+        double off;
+        support_start<T>(x[a], reach, g, a, &start, &off);
+        const int64_t ws = wrapped_start(start, g.ax[a].n);
+        st[a] = (int)ws;
+        off3[a] = (T)off;
+        key += (uint64_t)(ws >> g.ax[a].shift) * (uint64_t)g.ax[a].stride;
+    }
+    int64_t dst = s;
+    const int64_t lo = offsets[key], hi = offsets[key + 1];
+    if (hi - lo <= kSmallBin && hi - lo > 1) {
+        int64_t rank = 0;
+        for (int64_t m = lo; m < hi; ++m) {
+            const SortElem e = elems[m];
+            rank += (m != s && elem_less(e, me)) ? 1 : 0;
+        }
+        dst = lo + rank;
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) rec_off[dst * 4 + a] = off3[a];
+    rec_start[dst] = make_int4(st[0], st[1], st[2], (int)me.idx);
+    if (out_sorted) out_sorted[dst] = me.idx;
 }
 
 // Strengths of every vector gathered into record order.
@@ -617,14 +701,16 @@ bool BinSorter::sort(int64_t M, const T *coords, const GridParams &g, BinOrder o
     k_scan_reduce<<<(unsigned)tiles, kScanThreads, 0, stream>>>(counts_, n, tile_sums_);
     k_scan_top<<<1, kScanThreads, 0, stream>>>(tile_sums_, tiles);
     k_scan_down<<<(unsigned)tiles, kScanThreads, 0, stream>>>(counts_, n, tile_sums_, offsets_);
-    k_scatter<<<grid_for(M, 256), 256, 0, stream>>>(M, keys_, ranks_, offsets_, sorted_);
-    if (order != BinOrder::Grouped) {
+    if (order == BinOrder::Grouped) {
+        k_scatter<<<grid_for(M, 256), 256, 0, stream>>>(M, keys_, ranks_, offsets_, sorted_);
+    } else {
         unsigned grid = (unsigned)std::min<int64_t>(2048, M / kSortRun + 64);
         SortElem *A = (SortElem *)scratch_a_, *B = (SortElem *)scratch_b_;
+        k_scatter_elems<D, T><<<grid_for(M, 256), 256, 0, stream>>>(M, coords, keys_, ranks_,
+                                                                    offsets_, (int)order, A);
         k_large_tiles<<<1, 1024, 0, stream>>>(large_list_, large_count, offsets_, tile_off_);
-        k_large_runs<D, T><<<grid, kSortThreads, 0, stream>>>(large_list_, large_count, tile_off_,
-                                                              offsets_, coords, (int)order,
-                                                              sorted_, A);
+        k_large_runs<<<grid, kSortThreads, 0, stream>>>(large_list_, large_count, tile_off_,
+                                                        offsets_, A);
         // rounds for the largest possible bin; bins done earlier skip
         int rounds = 0;
         for (int64_t len = kSortRun; len < M; len <<= 1) ++rounds;
@@ -634,7 +720,7 @@ bool BinSorter::sort(int64_t M, const T *coords, const GridParams &g, BinOrder o
                                                              (r & 1) ? A : B);
         if (rounds > 0)
             k_large_final<<<grid, kSortThreads, 0, stream>>>(large_list_, large_count, tile_off_,
-                                                             offsets_, A, B, sorted_);
+                                                             offsets_, A, B);
     }
     return hip_ok(hipGetLastError(), "bin sort launch");
 }
@@ -644,9 +730,14 @@ bool BinSorter::prepare(int64_t M, const T *coords, const GridParams &g, BinOrde
                         int4 *rec_start, T *rec_off, uint32_t *out_sorted, hipStream_t stream)
 {
     if (M == 0) return true;
-    k_prepare<D, T><<<grid_for(M, 256), 256, 0, stream>>>(M, sorted_, keys_, offsets_, coords, g,
-                                                          (int)order, rec_start, rec_off,
-                                                          out_sorted);
+    if (order == BinOrder::Grouped)
+        k_prepare_grouped<D, T><<<grid_for(M, 256), 256, 0, stream>>>(M, sorted_, coords, g,
+                                                                      rec_start, rec_off,
+                                                                      out_sorted);
+    else
+        k_prepare<D, T><<<grid_for(M, 256), 256, 0, stream>>>(
+            M, (const SortElem *)scratch_a_, offsets_, coords, g, (int)order, rec_start, rec_off,
+            out_sorted);
     return hip_ok(hipGetLastError(), "prepare launch");
 }
 

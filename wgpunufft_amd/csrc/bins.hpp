@@ -13,8 +13,10 @@ public:
     void release();
 
     // Bins the M points (point-major coordinates on the device) and orders
-    // each bin; afterwards sorted() and offsets() hold the permutation and
-    // the nbins + 1 bin offsets (the last one is M).
+    // each bin; afterwards offsets() holds the nbins + 1 bin offsets (the
+    // last one is M) and, for the grouped order, sorted() the permutation.
+    // Ordered sorts keep each bin's ordering elements in scratch for
+    // prepare(), which writes the permutation.
     template <int D, typename T>
     bool sort(int64_t M, const T *coords, const GridParams &g, BinOrder order,
               hipStream_t stream);

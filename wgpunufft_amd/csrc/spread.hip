@@ -809,6 +809,7 @@ __global__ __launch_bounds__(kLanes) void k_sweep3(GridParams g, SpreadParams sp
                 bool take = false;
                 int64_t slot = 0;
                 int e = 0, q = 0;
+                T rox = (T)0, roy = (T)0, roz = (T)0, rcre = (T)0, rcim = (T)0;
                 if (k < khi) {
                     int s_lo = 0, s_hi = kSweepRows - 1;
                     while (s_lo < s_hi) {
@@ -821,7 +822,14 @@ __global__ __launch_bounds__(kLanes) void k_sweep3(GridParams g, SpreadParams sp
                     q = s_lo;
                     const int64_t j = k - row_excl[q];
                     slot = j < row_n0[q] ? row_f0[q] + j : row_f1[q] + (j - row_n0[q]);
-                    int64_t ee = (int64_t)rec_start[slot].x - xs;
+                    // every load of the record in one round trip
+                    const int sx0 = rec_start[slot].x;
+                    rox = rec_off[4 * slot + 0];
+                    roy = rec_off[4 * slot + 1];
+                    roz = rec_off[4 * slot + 2];
+                    rcre = cv[2 * slot];
+                    rcim = cv[2 * slot + 1];
+                    int64_t ee = (int64_t)sx0 - xs;
                     if (ee < 0) ee += n0;
                     e = (int)ee;
                     take = ee < TX + W - 1;
@@ -833,11 +841,9 @@ __global__ __launch_bounds__(kLanes) void k_sweep3(GridParams g, SpreadParams sp
                     SweepRec<T, W> &S = st[pos];
                     S.e = e;
                     S.row = rb + q;
-                    S.cre = cv[2 * slot];
-                    S.cim = cv[2 * slot + 1];
-                    const T ox = rec_off[4 * slot + 0];
-                    const T oy = rec_off[4 * slot + 1];
-                    const T oz = rec_off[4 * slot + 2];
+                    S.cre = rcre;
+                    S.cim = rcim;
+                    const T ox = rox, oy = roy, oz = roz;
 #pragma unroll
                     for (int jj = 0; jj < W; ++jj) {
                         S.kx[jj] = es_weight<T>((T)jj + ox, W, beta);

@@ -44,7 +44,6 @@ typedef float f32x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kTile = 16;
-constexpr int kRows = 64;   // reach rows staged per batch
 constexpr int kPitch = 20;  // LDS pitch of the weight tables, in floats
 constexpr int kBlocks = 8;  // 32-row blocks of the (y,z) plane
 
@@ -106,72 +105,130 @@ __device__ inline f32x8 load8(const float *p)
     return f32x8{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 }
 
-// Reach rows of a tile staged in batches of kRows: lane r holds bin row
-// rb + r of the reach (y start, then z start) as up to two slot ranges along
-// x (the second one when the reach wraps), and the exclusive scan of the
-// batch's record counts.
-struct RowBatch {
-    int64_t f0[kRows], n0[kRows], f1[kRows], excl[kRows];
+// Wavefront-wide LDS ordering for the one-wavefront workgroups here: LDS
+// instructions of a wavefront execute in order, so a compiler barrier is
+// enough (no s_barrier, no wait for outstanding global loads, which stay in
+// flight across it).
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
+// Every reach row of a tile (y start, then z start), staged once per work
+// item: its one or two slot ranges along x (the second when the reach wraps)
+// and the exclusive scan of the record counts: (16 + w - 1)^2 rows in 3D,
+// 32 + w - 1 in 2D.
+template <int N>
+struct TileRows {
+    uint32_t f0[N], n0[N], f1[N], excl[N + 1];
 };
 
-template <int D, int TILE, int W>
-__device__ inline int64_t stage_rows(RowBatch &rows, int rb, int lane, int64_t y0, int64_t z0,
-                                     const AxisReach &rx, const GridParams &g,
-                                     const uint32_t *__restrict__ offsets)
+template <int D, int TILE, int W, int N>
+__device__ inline uint32_t stage_tile_rows(TileRows<N> &rows, int lane, int64_t y0, int64_t z0,
+                                           const AxisReach &rx, const GridParams &g,
+                                           const uint32_t *__restrict__ offsets)
 {
     constexpr int NRY = TILE + W - 1;
     constexpr int NROWS = D == 3 ? NRY * NRY : NRY;
+    constexpr int NB = (NROWS + 63) / 64;
+    static_assert(NROWS <= N, "reach rows beyond the staging arrays");
     const int64_t n1 = g.ax[1].n, n2 = g.ax[2].n;
-    const int r = rb + lane;
-    int64_t f0 = 0, c0 = 0, f1 = 0, c1 = 0;
-    if (r < NROWS) {
-        int64_t sy = y0 - (W - 1) + r % NRY;
-        if (sy < 0) sy += n1;
-        if (sy >= n1) sy -= n1;
-        int64_t line = sy;
-        if constexpr (D == 3) {
-            int64_t sz = z0 - (W - 1) + r / NRY;
-            if (sz < 0) sz += n2;
-            if (sz >= n2) sz -= n2;
-            line += n1 * sz;
-        }
-        const int64_t row = g.ax[0].bins * line;
-        f0 = offsets[row + rx.lo0];
-        c0 = (int64_t)offsets[row + rx.lo0 + rx.len0] - f0;
-        if (rx.len1 > 0) {
-            f1 = offsets[row + rx.lo1];
-            c1 = (int64_t)offsets[row + rx.lo1 + rx.len1] - f1;
-        }
-    }
-    const int64_t cnt = c0 + c1;
-    int64_t xsum = cnt;
+    // all loads first, then the scans
+    uint32_t f0[NB], c0[NB], f1[NB], c1[NB];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int64_t y = __shfl_up(xsum, o, 64);
-        if (lane >= o) xsum += y;
+    for (int i = 0; i < NB; ++i) {
+        const int r = 64 * i + lane;
+        f0[i] = c0[i] = f1[i] = c1[i] = 0;
+        if (r < NROWS) {
+            int64_t sy = y0 - (W - 1) + r % NRY;
+            if (sy < 0) sy += n1;
+            if (sy >= n1) sy -= n1;
+            int64_t line = sy;
+            if constexpr (D == 3) {
+                int64_t sz = z0 - (W - 1) + r / NRY;
+                if (sz < 0) sz += n2;
+                if (sz >= n2) sz -= n2;
+                line += n1 * sz;
+            }
+            const int64_t row = g.ax[0].bins * line;
+            f0[i] = offsets[row + rx.lo0];
+            c0[i] = offsets[row + rx.lo0 + rx.len0];
+            if (rx.len1 > 0) {
+                f1[i] = offsets[row + rx.lo1];
+                c1[i] = offsets[row + rx.lo1 + rx.len1];
+            }
+        }
     }
-    rows.f0[lane] = f0;
-    rows.n0[lane] = c0;
-    rows.f1[lane] = f1;
-    rows.excl[lane] = xsum - cnt;
-    __syncthreads();
-    return __shfl(xsum, 63, 64);
+    uint32_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int r = 64 * i + lane;
+        const uint32_t n0 = c0[i] - f0[i], n1c = c1[i] - f1[i];
+        const uint32_t cnt = n0 + n1c;
+        uint32_t x = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (r < NROWS) {
+            rows.f0[r] = f0[i];
+            rows.n0[r] = n0;
+            rows.f1[r] = f1[i];
+            rows.excl[r] = carry + x - cnt;
+        }
+        carry += __shfl(x, 63, 64);
+    }
+    if (lane == 0) rows.excl[NROWS] = carry;
+    wave_sync();
+    return carry;
 }
 
-// Slot of record k of the staged batch; *q = its row within the batch.
-__device__ inline int64_t batch_slot(const RowBatch &rows, int64_t k, int *q)
+// Slot of record k of the tile; *q = its reach row.
+template <int NROWS>
+__device__ inline uint32_t tile_slot(const TileRows<NROWS> &rows, uint32_t k, int *q)
 {
-    int s_lo = 0, s_hi = kRows - 1;
-    while (s_lo < s_hi) {
-        const int mid = (s_lo + s_hi + 1) >> 1;
+    int lo = 0, hi = NROWS - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
         if (rows.excl[mid] <= k)
-            s_lo = mid;
+            lo = mid;
         else
-            s_hi = mid - 1;
+            hi = mid - 1;
     }
-    *q = s_lo;
-    const int64_t j = k - rows.excl[s_lo];
-    return j < rows.n0[s_lo] ? rows.f0[s_lo] + j : rows.f1[s_lo] + (j - rows.n0[s_lo]);
+    *q = lo;
+    const uint32_t j = k - rows.excl[lo];
+    return j < rows.n0[lo] ? rows.f0[lo] + j : rows.f1[lo] + (j - rows.n0[lo]);
+}
+
+// One lane's record of a chunk, with its global loads in flight until used.
+struct Fetched {
+    int q, sx0;
+    float4 off;
+    float2 c;
+    bool valid;
+};
+
+template <int NROWS>
+__device__ inline Fetched fetch_record(const TileRows<NROWS> &rows, int64_t k, int64_t khi,
+                                       const int4 *__restrict__ rec_start,
+                                       const float *__restrict__ rec_off,
+                                       const float *__restrict__ cv)
+{
+    Fetched f;
+    f.valid = k < khi;
+    f.q = 0;
+    f.sx0 = 0;
+    f.off = float4{0.0f, 0.0f, 0.0f, 0.0f};
+    f.c = float2{0.0f, 0.0f};
+    if (f.valid) {
+        const uint32_t slot = tile_slot<NROWS>(rows, (uint32_t)k, &f.q);
+        f.sx0 = rec_start[slot].x;
+        f.off = *(const float4 *)(rec_off + 4 * (int64_t)slot);
+        f.c = *(const float2 *)(cv + 2 * (int64_t)slot);
+    }
+    return f;
 }
 
 // Power-of-two scale that brings the largest strength magnitude below one.
@@ -228,8 +285,24 @@ __global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ c, int
 {
     const float *cv = c + (int64_t)blockIdx.y * n;
     uint32_t m = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t n4 = n / 4;  // n is even (complex pairs); the tail is < 4
+    const float4 *c4 = (const float4 *)cv;
+    const bool aligned = ((uintptr_t)cv & 15) == 0;
+    if (aligned) {
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+             i += (int64_t)gridDim.x * blockDim.x) {
+            const float4 q = c4[i];
+            const uint32_t a = __float_as_uint(q.x) & 0x7fffffffu;
+            const uint32_t b = __float_as_uint(q.y) & 0x7fffffffu;
+            const uint32_t d = __float_as_uint(q.z) & 0x7fffffffu;
+            const uint32_t e = __float_as_uint(q.w) & 0x7fffffffu;
+            const uint32_t ab = a > b ? a : b, de = d > e ? d : e;
+            const uint32_t u = ab > de ? ab : de;
+            m = u > m ? u : m;
+        }
+    }
+    for (int64_t i = (aligned ? 4 * n4 : 0) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+         i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t u = __float_as_uint(cv[i]) & 0x7fffffffu;
         m = u > m ? u : m;
     }
@@ -259,7 +332,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     __shared__ __align__(16) float KZ[kTile * kPitch];
     __shared__ __align__(16) float CC[2 * kPitch];
     __shared__ MfmaRec st[64];
-    __shared__ RowBatch rows;
+    __shared__ TileRows<NROWS> rows;
 
     const int lane = threadIdx.x;
     const int r32 = lane & 31;
@@ -293,106 +366,97 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         int64_t xs = x0 - (W - 1);
         if (xs < 0) xs += n0;
 
+        stage_tile_rows<3, kTile, W, NROWS>(rows, lane, y0, z0, rx, g, offsets);
         f32x16 acc[kBlocks];
 #pragma unroll
         for (int m = 0; m < kBlocks; ++m) acc[m] = f32x16{};
-
-        int64_t cursor = 0;
-        for (int rb = 0; rb < NROWS && cursor < rhi; rb += kRows) {
-            const int64_t batch_total =
-                stage_rows<3, kTile, W>(rows, rb, lane, y0, z0, rx, g, offsets);
-            const int64_t klo = rlo > cursor ? rlo - cursor : 0;
-            const int64_t khi = rhi - cursor < batch_total ? rhi - cursor : batch_total;
-            for (int64_t k0 = klo; k0 < khi; k0 += 64) {
-                const int64_t k = k0 + lane;
-                bool take = false;
-                MfmaRec R{};
-                if (k < khi) {
-                    int q;
-                    const int64_t slot = batch_slot(rows, k, &q);
-                    int64_t ee = (int64_t)rec_start[slot].x - xs;
-                    if (ee < 0) ee += n0;
-                    take = ee < kTile + W - 1;
-                    if (take) {
-                        const int r = rb + q;
-                        R.sx = (int)ee - (W - 1);
-                        R.sy = r % NRY - (W - 1);
-                        R.sz = r / NRY - (W - 1);
-                        R.ox = rec_off[4 * slot + 0];
-                        R.oy = rec_off[4 * slot + 1];
-                        R.oz = rec_off[4 * slot + 2];
-                        R.cre = cv[2 * slot] * scale;
-                        R.cim = cv[2 * slot + 1] * scale;
-                    }
-                }
-                const uint64_t mask = __ballot(take);
-                const int pos = __popcll(mask & ((1ull << lane) - 1ull));
-                const int nst = __popcll(mask);
-                if (take) st[pos] = R;
-                __syncthreads();
-                for (int kb = 0; kb < nst; kb += 16) {
-                    const int kn = nst - kb < 16 ? nst - kb : 16;
-                    // clear the weight tables, then drop each record's
-                    // in-tile weights into its column
-#pragma unroll
-                    for (int t = 0; t < 3; ++t) {
-                        float *tb = t == 0 ? KX : (t == 1 ? KY : KZ);
-                        *(float4 *)(tb + (lane >> 2) * kPitch + 4 * (lane & 3)) =
-                            float4{0.0f, 0.0f, 0.0f, 0.0f};
-                    }
-                    __syncthreads();
-                    if (lane < 48) {
-                        const int j = lane & 15, axis = lane >> 4;
-                        if (j < kn) {
-                            const MfmaRec &S = st[kb + j];
-                            const int s = axis == 0 ? S.sx : (axis == 1 ? S.sy : S.sz);
-                            const float o = axis == 0 ? S.ox : (axis == 1 ? S.oy : S.oz);
-                            float *tb = axis == 0 ? KX : (axis == 1 ? KY : KZ);
-#pragma unroll
-                            for (int t = 0; t < W; ++t) {
-                                const int c = s + t;
-                                if (c >= 0 && c < kTile)
-                                    tb[c * kPitch + j] = es_weight_fast((float)t + o, two_over_w, beta_l2e);
-                            }
-                        }
-                    } else {
-                        const int j = lane - 48;
-                        CC[j] = j < kn ? st[kb + j].cre : 0.0f;
-                        CC[kPitch + j] = j < kn ? st[kb + j].cim : 0.0f;
-                    }
-                    __syncthreads();
-                    // band of z cells the k-block touches (records in row order)
-                    int zlo = __builtin_amdgcn_readfirstlane(st[kb].sz);
-                    int zhi = __builtin_amdgcn_readfirstlane(st[kb + kn - 1].sz) + W - 1;
-                    zlo = zlo < 0 ? 0 : zlo;
-                    zhi = zhi > kTile - 1 ? kTile - 1 : zhi;
-                    const int mlo = zlo >> 1, mhi = zhi >> 1;
-
-                    V bh, bl;
-                    {
-                        const f32x8 kx = load8(KX + bxc * kPitch + j0);
-                        const f32x8 cc = load8(CC + bpart * kPitch + j0);
-                        Op::split(kx * cc, bh, bl);
-                    }
-                    const f32x8 ky = load8(KY + ay * kPitch + j0);
-#pragma unroll
-                    for (int m = 0; m < kBlocks; ++m) {
-                        if (m < mlo || m > mhi) continue;
-                        const f32x8 kz = load8(KZ + (2 * m + az) * kPitch + j0);
-                        V ah, al;
-                        Op::split(ky * kz, ah, al);
-                        acc[m] = Op::mma(ah, bh, acc[m]);
-                        if constexpr (kSplit) {
-                            acc[m] = Op::mma(ah, bl, acc[m]);
-                            acc[m] = Op::mma(al, bh, acc[m]);
-                        }
-                    }
-                    __syncthreads();
-                }
-                __syncthreads();
+        // records in chunks of 64, the next chunk's loads in flight while
+        // the current one is contracted
+        const int64_t klo = rlo, khi = rhi;
+        Fetched nx = fetch_record<NROWS>(rows, klo + lane, khi, rec_start, rec_off, cv);
+        for (int64_t k0 = klo; k0 < khi; k0 += 64) {
+            const Fetched cur = nx;
+            nx = fetch_record<NROWS>(rows, k0 + 64 + lane, khi, rec_start, rec_off, cv);
+            bool take = false;
+            MfmaRec R{};
+            if (cur.valid) {
+                int64_t ee = (int64_t)cur.sx0 - xs;
+                if (ee < 0) ee += n0;
+                take = ee < kTile + W - 1;
+                R.sx = (int)ee - (W - 1);
+                R.sy = cur.q % NRY - (W - 1);
+                R.sz = cur.q / NRY - (W - 1);
+                R.ox = cur.off.x;
+                R.oy = cur.off.y;
+                R.oz = cur.off.z;
+                R.cre = cur.c.x * scale;
+                R.cim = cur.c.y * scale;
             }
-            cursor += batch_total;
-            __syncthreads();
+            const uint64_t mask = __ballot(take);
+            const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+            const int nst = __popcll(mask);
+            if (take) st[pos] = R;
+            wave_sync();
+            for (int kb = 0; kb < nst; kb += 16) {
+                const int kn = nst - kb < 16 ? nst - kb : 16;
+                // clear the weight tables, then drop each record's
+                // in-tile weights into its column
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    float *tb = t == 0 ? KX : (t == 1 ? KY : KZ);
+                    *(float4 *)(tb + (lane >> 2) * kPitch + 4 * (lane & 3)) =
+                        float4{0.0f, 0.0f, 0.0f, 0.0f};
+                }
+                wave_sync();
+                if (lane < 48) {
+                    const int j = lane & 15, axis = lane >> 4;
+                    if (j < kn) {
+                        const MfmaRec &S = st[kb + j];
+                        const int s = axis == 0 ? S.sx : (axis == 1 ? S.sy : S.sz);
+                        const float o = axis == 0 ? S.ox : (axis == 1 ? S.oy : S.oz);
+                        float *tb = axis == 0 ? KX : (axis == 1 ? KY : KZ);
+#pragma unroll
+                        for (int t = 0; t < W; ++t) {
+                            const int c = s + t;
+                            if (c >= 0 && c < kTile)
+                                tb[c * kPitch + j] = es_weight_fast((float)t + o, two_over_w, beta_l2e);
+                        }
+                    }
+                } else {
+                    const int j = lane - 48;
+                    CC[j] = j < kn ? st[kb + j].cre : 0.0f;
+                    CC[kPitch + j] = j < kn ? st[kb + j].cim : 0.0f;
+                }
+                wave_sync();
+                // band of z cells the k-block touches (records in row order)
+                int zlo = __builtin_amdgcn_readfirstlane(st[kb].sz);
+                int zhi = __builtin_amdgcn_readfirstlane(st[kb + kn - 1].sz) + W - 1;
+                zlo = zlo < 0 ? 0 : zlo;
+                zhi = zhi > kTile - 1 ? kTile - 1 : zhi;
+                const int mlo = zlo >> 1, mhi = zhi >> 1;
+
+                V bh, bl;
+                {
+                    const f32x8 kx = load8(KX + bxc * kPitch + j0);
+                    const f32x8 cc = load8(CC + bpart * kPitch + j0);
+                    Op::split(kx * cc, bh, bl);
+                }
+                const f32x8 ky = load8(KY + ay * kPitch + j0);
+#pragma unroll
+                for (int m = 0; m < kBlocks; ++m) {
+                    if (m < mlo || m > mhi) continue;
+                    const f32x8 kz = load8(KZ + (2 * m + az) * kPitch + j0);
+                    V ah, al;
+                    Op::split(ky * kz, ah, al);
+                    acc[m] = Op::mma(ah, bh, acc[m]);
+                    if constexpr (kSplit) {
+                        acc[m] = Op::mma(ah, bl, acc[m]);
+                        acc[m] = Op::mma(al, bh, acc[m]);
+                    }
+                }
+                wave_sync();
+            }
+            wave_sync();
         }
 
         // accumulator element i of block m: row 8*(i/4) + 4*half + i%4 of
@@ -426,7 +490,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 constexpr int kTile2 = 32;
 
 template <int W, int MODE>
-__global__ __launch_bounds__(64) void k_spread_mfma2(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_spread_mfma2(
     GridParams g, SpreadParams sp, const uint32_t *__restrict__ offsets,
     const int4 *__restrict__ rec_start, const float *__restrict__ rec_off,
     const float *__restrict__ c_sorted, int64_t M, const uint32_t *__restrict__ totals,
@@ -442,7 +506,7 @@ __global__ __launch_bounds__(64) void k_spread_mfma2(
     __shared__ __align__(16) float KY[kTile2 * kPitch];
     __shared__ __align__(16) float CC[2 * kPitch];
     __shared__ MfmaRec st[64];
-    __shared__ RowBatch rows;
+    __shared__ TileRows<NROWS> rows;
 
     const int lane = threadIdx.x;
     const int r32 = lane & 31;
@@ -470,86 +534,79 @@ __global__ __launch_bounds__(64) void k_spread_mfma2(
         if (xs < 0) xs += n0;
 
         f32x16 acc[2] = {f32x16{}, f32x16{}};
-        int64_t cursor = 0;
-        for (int rb = 0; rb < NROWS && cursor < ti.rhi; rb += kRows) {
-            const int64_t batch_total =
-                stage_rows<2, kTile2, W>(rows, rb, lane, y0, 0, rx, g, offsets);
-            const int64_t klo = ti.rlo > cursor ? ti.rlo - cursor : 0;
-            const int64_t khi = ti.rhi - cursor < batch_total ? ti.rhi - cursor : batch_total;
-            for (int64_t k0 = klo; k0 < khi; k0 += 64) {
-                const int64_t k = k0 + lane;
-                bool take = false;
-                MfmaRec R{};
-                if (k < khi) {
-                    int q;
-                    const int64_t slot = batch_slot(rows, k, &q);
-                    int64_t ee = (int64_t)rec_start[slot].x - xs;
-                    if (ee < 0) ee += n0;
-                    take = ee < kTile2 + W - 1;
-                    if (take) {
-                        R.sx = (int)ee - (W - 1);
-                        R.sy = rb + q - (W - 1);
-                        R.ox = rec_off[4 * slot + 0];
-                        R.oy = rec_off[4 * slot + 1];
-                        R.cre = cv[2 * slot] * scale;
-                        R.cim = cv[2 * slot + 1] * scale;
-                    }
-                }
-                const uint64_t mask = __ballot(take);
-                const int pos = __popcll(mask & ((1ull << lane) - 1ull));
-                const int nst = __popcll(mask);
-                if (take) st[pos] = R;
-                __syncthreads();
-                for (int kb = 0; kb < nst; kb += 16) {
-                    const int kn = nst - kb < 16 ? nst - kb : 16;
-#pragma unroll
-                    for (int t = 0; t < 2; ++t) {
-                        float *tb = t == 0 ? KX : KY;
-                        *(float4 *)(tb + (lane >> 1) * kPitch + 8 * (lane & 1)) =
-                            float4{0.0f, 0.0f, 0.0f, 0.0f};
-                        *(float4 *)(tb + (lane >> 1) * kPitch + 8 * (lane & 1) + 4) =
-                            float4{0.0f, 0.0f, 0.0f, 0.0f};
-                    }
-                    __syncthreads();
-                    if (lane < 32) {
-                        const int j = lane & 15, axis = lane >> 4;
-                        if (j < kn) {
-                            const MfmaRec &S = st[kb + j];
-                            const int s0 = axis == 0 ? S.sx : S.sy;
-                            const float o = axis == 0 ? S.ox : S.oy;
-                            float *tb = axis == 0 ? KX : KY;
-#pragma unroll
-                            for (int t = 0; t < W; ++t) {
-                                const int c = s0 + t;
-                                if (c >= 0 && c < kTile2)
-                                    tb[c * kPitch + j] = es_weight_fast((float)t + o, two_over_w, beta_l2e);
-                            }
-                        }
-                    } else if (lane < 48) {
-                        const int j = lane - 32;
-                        CC[j] = j < kn ? st[kb + j].cre : 0.0f;
-                        CC[kPitch + j] = j < kn ? st[kb + j].cim : 0.0f;
-                    }
-                    __syncthreads();
-                    V ah, al;
-                    Op::split(load8(KY + r32 * kPitch + j0), ah, al);
-                    const f32x8 cc = load8(CC + bpart * kPitch + j0);
-#pragma unroll
-                    for (int nb = 0; nb < 2; ++nb) {
-                        V bh, bl;
-                        Op::split(load8(KX + (16 * nb + bxc) * kPitch + j0) * cc, bh, bl);
-                        acc[nb] = Op::mma(ah, bh, acc[nb]);
-                        if constexpr (kSplit) {
-                            acc[nb] = Op::mma(ah, bl, acc[nb]);
-                            acc[nb] = Op::mma(al, bh, acc[nb]);
-                        }
-                    }
-                    __syncthreads();
-                }
-                __syncthreads();
+        stage_tile_rows<2, kTile2, W, NROWS>(rows, lane, y0, 0, rx, g, offsets);
+        // records in chunks of 64, the next chunk's loads in flight while
+        // the current one is contracted
+        const int64_t klo = ti.rlo, khi = ti.rhi;
+        Fetched nx = fetch_record<NROWS>(rows, klo + lane, khi, rec_start, rec_off, cv);
+        for (int64_t k0 = klo; k0 < khi; k0 += 64) {
+            const Fetched cur = nx;
+            nx = fetch_record<NROWS>(rows, k0 + 64 + lane, khi, rec_start, rec_off, cv);
+            bool take = false;
+            MfmaRec R{};
+            if (cur.valid) {
+                int64_t ee = (int64_t)cur.sx0 - xs;
+                if (ee < 0) ee += n0;
+                take = ee < kTile2 + W - 1;
+                R.sx = (int)ee - (W - 1);
+                R.sy = cur.q - (W - 1);
+                R.ox = cur.off.x;
+                R.oy = cur.off.y;
+                R.cre = cur.c.x * scale;
+                R.cim = cur.c.y * scale;
             }
-            cursor += batch_total;
-            __syncthreads();
+            const uint64_t mask = __ballot(take);
+            const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+            const int nst = __popcll(mask);
+            if (take) st[pos] = R;
+            wave_sync();
+            for (int kb = 0; kb < nst; kb += 16) {
+                const int kn = nst - kb < 16 ? nst - kb : 16;
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    float *tb = t == 0 ? KX : KY;
+                    *(float4 *)(tb + (lane >> 1) * kPitch + 8 * (lane & 1)) =
+                        float4{0.0f, 0.0f, 0.0f, 0.0f};
+                    *(float4 *)(tb + (lane >> 1) * kPitch + 8 * (lane & 1) + 4) =
+                        float4{0.0f, 0.0f, 0.0f, 0.0f};
+                }
+                wave_sync();
+                if (lane < 32) {
+                    const int j = lane & 15, axis = lane >> 4;
+                    if (j < kn) {
+                        const MfmaRec &S = st[kb + j];
+                        const int s0 = axis == 0 ? S.sx : S.sy;
+                        const float o = axis == 0 ? S.ox : S.oy;
+                        float *tb = axis == 0 ? KX : KY;
+#pragma unroll
+                        for (int t = 0; t < W; ++t) {
+                            const int c = s0 + t;
+                            if (c >= 0 && c < kTile2)
+                                tb[c * kPitch + j] = es_weight_fast((float)t + o, two_over_w, beta_l2e);
+                        }
+                    }
+                } else if (lane < 48) {
+                    const int j = lane - 32;
+                    CC[j] = j < kn ? st[kb + j].cre : 0.0f;
+                    CC[kPitch + j] = j < kn ? st[kb + j].cim : 0.0f;
+                }
+                wave_sync();
+                V ah, al;
+                Op::split(load8(KY + r32 * kPitch + j0), ah, al);
+                const f32x8 cc = load8(CC + bpart * kPitch + j0);
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) {
+                    V bh, bl;
+                    Op::split(load8(KX + (16 * nb + bxc) * kPitch + j0) * cc, bh, bl);
+                    acc[nb] = Op::mma(ah, bh, acc[nb]);
+                    if constexpr (kSplit) {
+                        acc[nb] = Op::mma(ah, bl, acc[nb]);
+                        acc[nb] = Op::mma(al, bh, acc[nb]);
+                    }
+                }
+                wave_sync();
+            }
+            wave_sync();
         }
 
         // accumulator element i: row (y) 8*(i/4) + 4*half + i%4, column r32
@@ -645,8 +702,8 @@ bool launch_dim(SpreadMethod method, dim3 grid3, const GridParams &g, const Spre
 bool launch_absmax(const float *c, int64_t n, int64_t vectors, uint32_t *out,
                    hipStream_t stream)
 {
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
+    int64_t blocks = (n / 4 + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
     k_absmax<<<dim3((unsigned)blocks, (unsigned)vectors), 256, 0, stream>>>(c, n, out);
     return hip_ok(hipGetLastError(), "strength range launch");
