@@ -31,6 +31,15 @@ METRIC = "NU points/s for 3D type-1/2 at tol 1e-6, MFMA vs plain GPU; % HBM/MFMA
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP16_MFMA_PEAK_TFLOPS = 2500.0
+# HBM bytes per launch of the dominant kernels on the default workload,
+# from the FETCH_SIZE / WRITE_SIZE counter passes over `bench.py
+# --profile-step` (scaled by the same counters over a 1 GiB copy);
+# profiles/traffic_3d_t1t2_128modes_1e7pts.md holds the table.  Reported
+# only when the run uses the default workload.
+MEASURED_TRAFFIC = {
+    "k_spread_mfma3": 2.3202e9,
+    "k_interp_tiled3": 2.0450e9,
+}
 
 
 def parse():
@@ -49,6 +58,9 @@ def parse():
                     choices=["mfma_f16x3", "mfma_bf16x3", "mfma_f16"],
                     help="matrix-core spread of the timed type-1 (the plain FP32 spread is "
                          "timed beside it)")
+    ap.add_argument("--profile-step", action="store_true",
+                    help="for counter passes: a 1 GiB device copy (the byte-counter "
+                         "calibration) then two untimed steps of the headline workload")
     ap.add_argument("--config", type=int, default=2,
                     help="2: the headline workload (default); 1, 3, 4, 5: the other "
                          "configurations, reported as extra JSON lines")
@@ -70,6 +82,8 @@ def main():
     if args.config != 2:
         from wgpunufft_amd import bench_configs
         return bench_configs.run(args)
+    if args.profile_step:
+        return profile_step(args, W, S, np, torch)
 
     dev = torch.device("cuda:0")
     M = args.points
@@ -154,6 +168,12 @@ def main():
                    "frac": round(interp_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                    "avg_ms": round(t_interp, 4),
                    "algorithmic": f"M*(4d+8)+8*N_f = {interp_bytes:.4g} B"}
+    default_workload = (M, N, args.eps) == (10_000_000, 128, 1e-6)
+    for roof in (roof_mfma, roof_interp):
+        key = roof["kernel"].split(" ")[0]
+        if default_workload and key in MEASURED_TRAFFIC and args.spread == "mfma_f16x3":
+            roof["traffic"] = MEASURED_TRAFFIC[key]
+            roof["traffic_unit"] = "B per launch (FETCH_SIZE+WRITE_SIZE, copy-calibrated)"
     dominant, other = (roof_mfma, roof_interp) if t_mfma >= t_interp else (roof_interp, roof_mfma)
 
     def pts(stages, key):
@@ -200,6 +220,32 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, np, S, n_modes)
     print(json.dumps(result), flush=True)
+    return 0
+
+
+def profile_step(args, W, S, np, torch):
+    """A 1 GiB device-to-device copy, whose FETCH_SIZE / WRITE_SIZE give the
+    counters' byte scale, then two steps (matrix-core type 1 and type 2) of
+    the headline workload; nothing is timed."""
+    dev = torch.device("cuda:0")
+    src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    M, N = args.points, args.modes
+    x = torch.from_numpy(S.config2_points(M)).to(dev)
+    c = torch.from_numpy(S.complex_normal(S.config_seed(2, 3), M)).to(dev)
+    fm = torch.from_numpy(S.complex_normal(S.config_seed(2, 4), N ** 3)).to(dev)
+    cfg = W.NufftConfig([N] * 3, args.eps)
+    p1 = W.NufftPlan.type1_gpu(cfg).set_spread_method(args.spread)
+    p2 = W.NufftPlan.type2_gpu(cfg)
+    for _ in range(2):
+        p1.encode_gpu(x, c)
+        p2.encode_gpu(x, fm)
+    torch.cuda.synchronize()
+    print("profile step done", flush=True)
     return 0
 
 

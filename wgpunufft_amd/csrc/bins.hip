@@ -28,6 +28,7 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;  // 4096
 constexpr int kSortThreads = 256;
+constexpr int kRunThreads = 1024;  // LDS bitonic runs: one thread per pair
 constexpr int kSortRun = 2048;
 
 // ---------------------------------------------------------------------
@@ -117,15 +118,25 @@ __global__ __launch_bounds__(kCountThreads) void k_count(int64_t M, const T *__r
         }
     }
     __syncthreads();
-    for (int h = threadIdx.x; h < kHashSize; h += kCountThreads) {
-        uint32_t key = ht_key[h];
-        if (key == kEmpty) continue;
-        uint32_t n = ht_cnt[h];
-        uint32_t old = atomicAdd(&counts[key], n);
-        ht_base[h] = old;
-        if (old <= kSmallBin && old + n > kSmallBin) {
+    // every slot's global atomic issued before any result is awaited
+    constexpr int kSlotsPerThread = kHashSize / kCountThreads;
+    uint32_t hk[kSlotsPerThread], hn[kSlotsPerThread], ho[kSlotsPerThread];
+#pragma unroll
+    for (int q = 0; q < kSlotsPerThread; ++q) {
+        const int h = threadIdx.x + q * kCountThreads;
+        hk[q] = ht_key[h];
+        hn[q] = ht_cnt[h];
+    }
+#pragma unroll
+    for (int q = 0; q < kSlotsPerThread; ++q)
+        ho[q] = hk[q] != kEmpty ? atomicAdd(&counts[hk[q]], hn[q]) : 0u;
+#pragma unroll
+    for (int q = 0; q < kSlotsPerThread; ++q) {
+        if (hk[q] == kEmpty) continue;
+        ht_base[threadIdx.x + q * kCountThreads] = ho[q];
+        if (ho[q] <= kSmallBin && ho[q] + hn[q] > kSmallBin) {
             uint32_t slot = atomicAdd(large_count, 1u);
-            large_list[slot] = key;
+            large_list[slot] = hk[q];
         }
     }
     __syncthreads();
@@ -308,19 +319,22 @@ __global__ __launch_bounds__(256) void k_scatter_elems(int64_t M, const T *__res
 //     back, so the elements of every bin end up sorted in the first one.
 // Element keys are unique (the index breaks ties), so every merge is exact.
 // ---------------------------------------------------------------------
+// Bitonic sort of s[0..n2) (n2 a power of two): every thread takes one
+// compare-exchange pair per step (lower index i with bit j clear, partner
+// i + j), so no lane idles.
 __device__ inline void lds_bitonic(SortElem *s, int n2)
 {
+    const int half = n2 >> 1;
     for (int k = 2; k <= n2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < n2; i += kSortThreads) {
-                int p = i ^ j;
-                if (p > i) {
-                    bool up = (i & k) == 0;
-                    SortElem a = s[i], b = s[p];
-                    if (elem_less(b, a) == up) {
-                        s[i] = b;
-                        s[p] = a;
-                    }
+            for (int t = threadIdx.x; t < half; t += blockDim.x) {
+                const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                const int p = i + j;
+                const bool up = (i & k) == 0;
+                const SortElem a = s[i], b = s[p];
+                if (elem_less(b, a) == up) {
+                    s[i] = b;
+                    s[p] = a;
                 }
             }
             __syncthreads();
@@ -356,7 +370,8 @@ __device__ inline int merge_rounds(int64_t n)
 __global__ __launch_bounds__(1024) void k_large_tiles(const uint32_t *__restrict__ large_list,
                                                       const uint32_t *__restrict__ large_count,
                                                       const uint32_t *__restrict__ offsets,
-                                                      uint32_t *__restrict__ tile_off)
+                                                      uint32_t *__restrict__ tile_off,
+                                                      uint32_t *__restrict__ large_max)
 {
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry;
@@ -390,6 +405,18 @@ __global__ __launch_bounds__(1024) void k_large_tiles(const uint32_t *__restrict
         __syncthreads();
     }
     if (threadIdx.x == 0) tile_off[nlarge] = carry;
+    // the largest listed bin, for the merge rounds to skip themselves
+    uint32_t mx = 0;
+    for (uint32_t i = threadIdx.x; i < nlarge; i += blockDim.x) {
+        const uint32_t bin = large_list[i];
+        const uint32_t n = offsets[bin + 1] - offsets[bin];
+        mx = n > mx ? n : mx;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t y = __shfl_xor(mx, o, 64);
+        mx = y > mx ? y : mx;
+    }
+    if (lane == 0 && mx) atomicMax(large_max, mx);
 }
 
 __device__ inline uint32_t tile_bin(const uint32_t *tile_off, uint32_t nlarge, uint32_t t)
@@ -405,7 +432,7 @@ __device__ inline uint32_t tile_bin(const uint32_t *tile_off, uint32_t nlarge, u
     return lo;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_large_runs(
+__global__ __launch_bounds__(kRunThreads) void k_large_runs(
     const uint32_t *__restrict__ large_list, const uint32_t *__restrict__ large_count,
     const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ offsets,
     SortElem *__restrict__ elems)
@@ -423,27 +450,28 @@ __global__ __launch_bounds__(kSortThreads) void k_large_runs(
         const int m = (int)(n - r0 < kSortRun ? n - r0 : kSortRun);
         int n2 = 1;
         while (n2 < m) n2 <<= 1;
-        for (int k = threadIdx.x; k < n2; k += kSortThreads)
+        for (int k = threadIdx.x; k < n2; k += kRunThreads)
             s[k] = k < m ? elems[lo + r0 + k]
                          : SortElem{0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
         __syncthreads();
         lds_bitonic(s, n2);
-        for (int k = threadIdx.x; k < m; k += kSortThreads) elems[lo + r0 + k] = s[k];
+        for (int k = threadIdx.x; k < m; k += kRunThreads) elems[lo + r0 + k] = s[k];
         __syncthreads();
     }
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_large_merge(
     const uint32_t *__restrict__ large_list, const uint32_t *__restrict__ large_count,
-    const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ offsets, int round,
+    const uint32_t *__restrict__ large_max, const uint32_t *__restrict__ tile_off,
+    const uint32_t *__restrict__ offsets, int round,
     const SortElem *__restrict__ src, SortElem *__restrict__ dst)
 {
     __shared__ SortElem s[kSortRun];
     __shared__ int64_t split[2];
     const uint32_t nlarge = *large_count;
-    if (nlarge == 0) return;
-    const uint32_t ntiles = tile_off[nlarge];
     const int64_t len = (int64_t)kSortRun << round;
+    if (nlarge == 0 || len >= (int64_t)*large_max) return;  // every bin merged already
+    const uint32_t ntiles = tile_off[nlarge];
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t i = tile_bin(tile_off, nlarge, t);
         const uint32_t bin = large_list[i];
@@ -638,8 +666,9 @@ bool BinSorter::ensure(int64_t nbins, int64_t M, bool need_sort_scratch)
         if (offsets_) (void)hipFree(offsets_);
         if (tile_sums_) (void)hipFree(tile_sums_);
         counts_ = offsets_ = tile_sums_ = nullptr;
-        // counts: nbins + trailing zero, then the large-bin count word
-        if (!hip_ok(hipMalloc(&counts_, sizeof(uint32_t) * (nbins + 2)), "bin counts")) return false;
+        // counts: nbins + trailing zero, then the large-bin count and the
+        // largest large bin
+        if (!hip_ok(hipMalloc(&counts_, sizeof(uint32_t) * (nbins + 3)), "bin counts")) return false;
         if (!hip_ok(hipMalloc(&offsets_, sizeof(uint32_t) * (nbins + 1)), "bin offsets")) return false;
         int64_t tiles = (nbins + 1 + kScanTile - 1) / kScanTile;
         if (!hip_ok(hipMalloc(&tile_sums_, sizeof(uint32_t) * (tiles + 1)), "scan tiles")) return false;
@@ -688,12 +717,13 @@ bool BinSorter::sort(int64_t M, const T *coords, const GridParams &g, BinOrder o
     if (!ensure(g.nbins, M, order != BinOrder::Grouped)) return false;
     // Clear counts, trailing zero and the large-bin count with a fill kernel
     // (no memset nodes, so callers may capture the sequence in a graph).
-    k_fill_u32<<<grid_for(g.nbins + 2, 256), 256, 0, stream>>>(counts_, g.nbins + 2, 0u);
+    k_fill_u32<<<grid_for(g.nbins + 3, 256), 256, 0, stream>>>(counts_, g.nbins + 3, 0u);
     if (M == 0) {
         k_fill_u32<<<grid_for(g.nbins + 1, 256), 256, 0, stream>>>(offsets_, g.nbins + 1, 0u);
         return hip_ok(hipGetLastError(), "bin clear");
     }
     uint32_t *large_count = counts_ + g.nbins + 1;
+    uint32_t *large_max = counts_ + g.nbins + 2;
     k_count<D, T><<<grid_for(M, kCountPoints), kCountThreads, 0, stream>>>(
         M, coords, g, counts_, keys_, ranks_, large_list_, large_count);
     int64_t n = g.nbins + 1;
@@ -708,14 +738,16 @@ bool BinSorter::sort(int64_t M, const T *coords, const GridParams &g, BinOrder o
         SortElem *A = (SortElem *)scratch_a_, *B = (SortElem *)scratch_b_;
         k_scatter_elems<D, T><<<grid_for(M, 256), 256, 0, stream>>>(M, coords, keys_, ranks_,
                                                                     offsets_, (int)order, A);
-        k_large_tiles<<<1, 1024, 0, stream>>>(large_list_, large_count, offsets_, tile_off_);
-        k_large_runs<<<grid, kSortThreads, 0, stream>>>(large_list_, large_count, tile_off_,
-                                                        offsets_, A);
+        k_large_tiles<<<1, 1024, 0, stream>>>(large_list_, large_count, offsets_, tile_off_,
+                                              large_max);
+        k_large_runs<<<grid, kRunThreads, 0, stream>>>(large_list_, large_count, tile_off_,
+                                                       offsets_, A);
         // rounds for the largest possible bin; bins done earlier skip
         int rounds = 0;
         for (int64_t len = kSortRun; len < M; len <<= 1) ++rounds;
         for (int r = 0; r < rounds; ++r)
-            k_large_merge<<<grid, kSortThreads, 0, stream>>>(large_list_, large_count, tile_off_,
+            k_large_merge<<<grid, kSortThreads, 0, stream>>>(large_list_, large_count, large_max,
+                                                             tile_off_,
                                                              offsets_, r, (r & 1) ? B : A,
                                                              (r & 1) ? A : B);
         if (rounds > 0)

@@ -310,7 +310,15 @@ __global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ c, int
         const uint32_t y = __shfl_xor(m, o, 64);
         m = y > m ? y : m;
     }
-    if ((threadIdx.x & 63) == 0 && m) atomicMax(out + blockIdx.y, m);
+    // one atomic per workgroup: atomics on one address serialize
+    __shared__ uint32_t wmax[4];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t b = wmax[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) b = wmax[i] > b ? wmax[i] : b;
+        if (b) atomicMax(out + blockIdx.y, b);
+    }
 }
 
 template <int W, int MODE>
@@ -703,7 +711,7 @@ bool launch_absmax(const float *c, int64_t n, int64_t vectors, uint32_t *out,
                    hipStream_t stream)
 {
     int64_t blocks = (n / 4 + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
+    if (blocks > 512) blocks = 512;
     if (blocks < 1) blocks = 1;
     k_absmax<<<dim3((unsigned)blocks, (unsigned)vectors), 256, 0, stream>>>(c, n, out);
     return hip_ok(hipGetLastError(), "strength range launch");
