@@ -155,7 +155,7 @@ def main():
     t_mfma = stages_mfma["type1"]["spread_kernel"]
     t_plain = stages_plain["type1"]["spread_kernel"]
     t_interp = stages_mfma["type2"]["interp"]
-    roof_mfma = spread_roof(t_mfma, f"k_spread_mfma3 (type-1 16^3-tile contraction, {args.spread})",
+    roof_mfma = spread_roof(t_mfma, f"k_spread_mfma3 (type-1 16x16x8-tile contraction, {args.spread})",
                             FP16_MFMA_PEAK_TFLOPS, "peak: dense FP16 matrix cores")
     roof_plain = spread_roof(t_plain,
                              "k_sweep3 (type-1 3D row-sweep spread)" if sweep

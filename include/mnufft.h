@@ -155,7 +155,7 @@ int mnufft_timing_history(mnufft_plan plan, int32_t on);
 /*
  * Arithmetic of the plan's type-1 spread (no counterpart in the reference,
  * whose spread is plain FP32/FP64 arithmetic): MNUFFT_SPREAD_PLAIN (the
- * default), or the 16^3-tile contraction on the matrix cores with FP16
+ * default), or the 16 x 16 x 8-tile contraction on the matrix cores with FP16
  * operands (MNUFFT_SPREAD_MFMA_F16, about 1e-3 relative), FP16 hi/lo splits
  * (MNUFFT_SPREAD_MFMA_F16X3, about 1e-6) or bfloat16 hi/lo splits
  * (MNUFFT_SPREAD_MFMA_BF16X3, about 1e-5), accumulated in FP32.  The matrix

@@ -120,7 +120,7 @@ class _PlanBase:
         return lib().mnufft_plan_beta(self._handle)
 
     def set_spread_method(self, method):
-        """Type-1 spread arithmetic: "plain" (default), or the 16^3-tile
+        """Type-1 spread arithmetic: "plain" (default), or the 16 x 16 x 8-tile
         matrix-core contraction "mfma_f16" (FP16 operands, ~1e-3),
         "mfma_f16x3" (FP16 hi/lo splits, ~1e-6) or "mfma_bf16x3" (bfloat16
         hi/lo splits, ~1e-5); FP32 3D plans only."""

@@ -172,6 +172,13 @@ bool Plan::init_common()
     sp = make_spread_params(dim, fine, kernel.width);
     sweep = make_sweep_params(dim, fine, kernel.width, &sp);
     tiles2 = make_tile2_params(dim, fine, kernel.width, &tile2_sp);
+    if (sweep) {
+        // matrix-core tiles: the row sweep's 16 x 16 cells, 8 deep along z
+        tile3_sp = sp;
+        tile3_sp.block[2] = 8;
+        tile3_sp.nblocks[2] = (fine[2] + 7) / 8;
+        tile3_sp.block_count = tile3_sp.nblocks[0] * tile3_sp.nblocks[1] * tile3_sp.nblocks[2];
+    }
     return true;
 }
 

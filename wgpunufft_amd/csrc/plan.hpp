@@ -46,6 +46,7 @@ struct Plan {
     bool sweep = false;  // 3D row-sweep spreader (else output-stationary blocks)
     SpreadParams tile2_sp{};
     bool tiles2 = false;  // 2D grid qualifies for the matrix-core 32 x 32 tiles
+    SpreadParams tile3_sp{};  // 3D matrix-core tiles (16 x 16 x 8; row-sweep grids)
     SpreadMethod spread_method = SpreadMethod::Plain;
     bool gp_rows = false;  // points binned in one-cell rows (row tiles)
     // Type-1 spreads on row tiles: the 3D row sweep, or a 2D matrix-core method.
@@ -55,7 +56,8 @@ struct Plan {
     }
     const SpreadParams &spread_params() const
     {
-        return dim == 2 && spread_method != SpreadMethod::Plain ? tile2_sp : sp;
+        if (spread_method == SpreadMethod::Plain) return sp;
+        return dim == 2 ? tile2_sp : tile3_sp;
     }
 
     // points

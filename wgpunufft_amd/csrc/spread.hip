@@ -987,16 +987,23 @@ bool chunk_mfma(Spreader &s, SpreadMethod method, const GridParams &g, const Spr
                 const float *c_sorted, int64_t M, int64_t vectors, const uint32_t *vmax,
                 float *grid, hipStream_t stream)
 {
-    constexpr int TE = D == 3 ? kSweepTile : 32;  // tile edge
+    constexpr int TE = D == 3 ? kSweepTile : 32;  // tile edge along x and y
     const dim3 ig = items_grid(sp.block_count, vectors);
     TileWork w{s.totals, s.parts, s.item_off, s.heavy_base, s.ctrl, vmax, ig.x};
     if (!launch_mfma_tiles(method, D, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, w,
                            grid, (float *)s.partial, stream))
         return false;
     if (s.kernel_end) (void)hipEventRecord(s.kernel_end, stream);
-    k_reduce_tiles<float, TE, TE, D == 3 ? TE : 1><<<dim3(2048, (unsigned)vectors), 256, 0,
-                                                     stream>>>(
-        g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const float *)s.partial, grid);
+    const dim3 rg(2048, (unsigned)vectors);
+    if (D == 2)
+        k_reduce_tiles<float, TE, TE, 1><<<rg, 256, 0, stream>>>(
+            g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const float *)s.partial, grid);
+    else if (sp.block[2] == 8)
+        k_reduce_tiles<float, TE, TE, 8><<<rg, 256, 0, stream>>>(
+            g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const float *)s.partial, grid);
+    else
+        k_reduce_tiles<float, TE, TE, TE><<<rg, 256, 0, stream>>>(
+            g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const float *)s.partial, grid);
     return hip_ok(hipGetLastError(), "matrix-core spread launch");
 }
 

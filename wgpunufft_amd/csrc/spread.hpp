@@ -47,7 +47,7 @@ struct Spreader {
     int64_t chunk_budget_bytes = (int64_t)1 << 30;
 };
 
-// Work plan of the tiles (16^3 in 3D, 32^2 in 2D) handed to the matrix-core
+// Work plan of the tiles (16 x 16 x 8 in 3D, 32^2 in 2D) handed to the matrix-core
 // tile kernel.
 struct TileWork {
     const uint32_t *totals, *parts, *item_off, *heavy_base, *ctrl, *cmax;

@@ -1,7 +1,7 @@
 // Type-1 spreading of 3D tiles as dense contractions on the matrix cores.
 //
 // Same contract as the plain spreaders (src/gpu_nd_spread.rs:1-36): every
-// fine-grid cell of a 16^3 tile is owned by one workgroup, which sums the
+// fine-grid cell of a 16 x 16 x 8 tile (x, y, z) is owned by one workgroup, which sums the
 // records whose support reaches the tile in a fixed order and writes the cell
 // once; tiles with too many records are split into parts whose partial sums
 // are added in part order.  Results are bitwise repeatable and do not depend
@@ -11,11 +11,11 @@
 // The tile's sum is recast as a matrix product over its records j:
 //
 //   G[(y,z), (x,re|im)] += sum_j  Ky_j[y] Kz_j[z]  *  Kx_j[x] c_j
-//                          \_____ A (256 x K) ____/  \__ B (K x 32) __/
+//                          \_____ A (128 x K) ____/  \__ B (K x 32) __/
 //
 // with Kx_j, Ky_j, Kz_j the record's separable kernel weights laid out over
-// the tile's 16 cells of each axis (zero off the support).  The 256 (y,z)
-// rows form eight 32-row blocks (16 y x 2 z), each held in registers as the
+// the tile's cells of each axis (zero off the support).  The 128 (y,z)
+// rows form four 32-row blocks (16 y x 2 z), each held in registers as the
 // accumulator of a v_mfma_f32_32x32x16 product; B's 32 columns are the 16 x
 // cells, real and imaginary parts interleaved, so the accumulator rows are
 // written out as contiguous 128-byte rows of the interleaved complex grid.
@@ -45,7 +45,6 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kTile = 16;
 constexpr int kPitch = 20;  // LDS pitch of the weight tables, in floats
-constexpr int kBlocks = 8;  // 32-row blocks of the (y,z) plane
 
 struct MfmaRec {
     int sx, sy, sz;  // support start relative to the tile origin
@@ -124,13 +123,13 @@ struct TileRows {
     uint32_t f0[N], n0[N], f1[N], excl[N + 1];
 };
 
-template <int D, int TILE, int W, int N>
+template <int D, int TY, int TZ, int W, int N>
 __device__ inline uint32_t stage_tile_rows(TileRows<N> &rows, int lane, int64_t y0, int64_t z0,
                                            const AxisReach &rx, const GridParams &g,
                                            const uint32_t *__restrict__ offsets)
 {
-    constexpr int NRY = TILE + W - 1;
-    constexpr int NROWS = D == 3 ? NRY * NRY : NRY;
+    constexpr int NRY = TY + W - 1;
+    constexpr int NROWS = D == 3 ? NRY * (TZ + W - 1) : NRY;
     constexpr int NB = (NROWS + 63) / 64;
     static_assert(NROWS <= N, "reach rows beyond the staging arrays");
     const int64_t n1 = g.ax[1].n, n2 = g.ax[2].n;
@@ -321,8 +320,9 @@ __global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ c, int
     }
 }
 
-template <int W, int MODE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_spread_mfma3(
+template <int W, int MODE, int TZ>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TZ == 16 ? 2 : 3))) void
+k_spread_mfma3(
     GridParams g, SpreadParams sp, const uint32_t *__restrict__ offsets,
     const int4 *__restrict__ rec_start, const float *__restrict__ rec_off,
     const float *__restrict__ c_sorted, int64_t M, const uint32_t *__restrict__ totals,
@@ -334,7 +334,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     using V = typename Op::V;
     constexpr bool kSplit = MODE != (int)SpreadMethod::MfmaF16;
     constexpr int NRY = kTile + W - 1;
-    constexpr int NROWS = NRY * NRY;
+    constexpr int NROWS = NRY * (TZ + W - 1);
+    constexpr int kBlocks = TZ / 2;  // 32-row blocks (16 y x 2 z) of the tile
     __shared__ __align__(16) float KX[kTile * kPitch];
     __shared__ __align__(16) float KY[kTile * kPitch];
     __shared__ __align__(16) float KZ[kTile * kPitch];
@@ -369,12 +370,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const int64_t tbx = b % sp.nblocks[0];
         const int64_t tby = (b / sp.nblocks[0]) % sp.nblocks[1];
         const int64_t tbz = b / (sp.nblocks[0] * sp.nblocks[1]);
-        const int64_t x0 = tbx * kTile, y0 = tby * kTile, z0 = tbz * kTile;
+        const int64_t x0 = tbx * kTile, y0 = tby * kTile, z0 = tbz * TZ;
         const AxisReach rx = axis_reach(x0, kTile, W, g.ax[0]);
         int64_t xs = x0 - (W - 1);
         if (xs < 0) xs += n0;
 
-        stage_tile_rows<3, kTile, W, NROWS>(rows, lane, y0, z0, rx, g, offsets);
+        stage_tile_rows<3, kTile, TZ, W, NROWS>(rows, lane, y0, z0, rx, g, offsets);
         f32x16 acc[kBlocks];
 #pragma unroll
         for (int m = 0; m < kBlocks; ++m) acc[m] = f32x16{};
@@ -423,10 +424,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                         const int s = axis == 0 ? S.sx : (axis == 1 ? S.sy : S.sz);
                         const float o = axis == 0 ? S.ox : (axis == 1 ? S.oy : S.oz);
                         float *tb = axis == 0 ? KX : (axis == 1 ? KY : KZ);
+                        const int lim = axis == 2 ? TZ : kTile;
 #pragma unroll
                         for (int t = 0; t < W; ++t) {
                             const int c = s + t;
-                            if (c >= 0 && c < kTile)
+                            if (c >= 0 && c < lim)
                                 tb[c * kPitch + j] = es_weight_fast((float)t + o, two_over_w, beta_l2e);
                         }
                     }
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 int zlo = __builtin_amdgcn_readfirstlane(st[kb].sz);
                 int zhi = __builtin_amdgcn_readfirstlane(st[kb + kn - 1].sz) + W - 1;
                 zlo = zlo < 0 ? 0 : zlo;
-                zhi = zhi > kTile - 1 ? kTile - 1 : zhi;
+                zhi = zhi > TZ - 1 ? TZ - 1 : zhi;
                 const int mlo = zlo >> 1, mhi = zhi >> 1;
 
                 V bh, bl;
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                     if (gx < n0 && gy < n1 && gz < n2)
                         grid[2 * (v * g.fine_total + gx + n0 * (gy + n1 * gz)) + bpart] = val;
                 } else {
-                    partial[2 * (((pslot * gridDim.y + v) * kTile + zz) * (kTile * kTile) +
+                    partial[2 * (((pslot * gridDim.y + v) * TZ + zz) * (kTile * kTile) +
                                  yy * kTile + bxc) +
                             bpart] = val;
                 }
@@ -542,7 +544,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         if (xs < 0) xs += n0;
 
         f32x16 acc[2] = {f32x16{}, f32x16{}};
-        stage_tile_rows<2, kTile2, W, NROWS>(rows, lane, y0, 0, rx, g, offsets);
+        stage_tile_rows<2, kTile2, 1, W, NROWS>(rows, lane, y0, 0, rx, g, offsets);
         // records in chunks of 64, the next chunk's loads in flight while
         // the current one is contracted
         const int64_t klo = ti.rlo, khi = ti.rhi;
@@ -645,11 +647,16 @@ void launch_kernel(dim3 grid3, const GridParams &g, const SpreadParams &sp,
                    const float *c_sorted, int64_t M, const TileWork &w, float *grid,
                    float *partial, hipStream_t stream)
 {
-    if constexpr (D == 3)
-        k_spread_mfma3<W, MODE><<<grid3, 64, 0, stream>>>(
-            g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
-            w.heavy_base, w.ctrl, w.cmax, grid, partial);
-    else
+    if constexpr (D == 3) {
+        if (sp.block[2] == 8)
+            k_spread_mfma3<W, MODE, 8><<<grid3, 64, 0, stream>>>(
+                g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
+                w.heavy_base, w.ctrl, w.cmax, grid, partial);
+        else
+            k_spread_mfma3<W, MODE, 16><<<grid3, 64, 0, stream>>>(
+                g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
+                w.heavy_base, w.ctrl, w.cmax, grid, partial);
+    } else
         k_spread_mfma2<W, MODE><<<grid3, 64, 0, stream>>>(
             g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
             w.heavy_base, w.ctrl, w.cmax, grid, partial);
