@@ -327,9 +327,9 @@ def test_f64_row_sweep_and_tiled_interp():
 
 
 def test_bin_sort_multi_round_merges():
-    """A dense cluster puts tens of thousands of points in single bins, so
-    the large-bin sort runs several merge rounds; the permutation stays the
-    reference's, bit for bit."""
+    """A dense cluster puts tens of thousands of points in single bins (many
+    radix tiles per bin); the permutation stays the reference's, bit for
+    bit."""
     torch, W, O, S = _setup()
     shape = [32, 32, 32]
     M = 200000
@@ -348,6 +348,27 @@ def test_bin_sort_multi_round_merges():
     torch.cuda.synchronize()
     assert np.array_equal(off.cpu().numpy().astype(np.uint32), ref_off)
     assert np.array_equal(srt.cpu().numpy().astype(np.uint32), ref_sorted)
+
+
+def test_type1_dense_cluster_shuffle_invariance():
+    """Canonical in-bin order with bins of tens of thousands of points (the
+    large-bin merge sort runs several rounds): a shuffled input gives the
+    same type-1 output, bit for bit, and it matches the direct sums."""
+    torch, W, O, S = _setup()
+    n_modes = [32, 32, 32]
+    M = 200000
+    x = S.clustered_coords(S.config_seed(2, 710), M, (0.3, -1.1, 2.0), 0.002)
+    c = S.complex_normal(S.config_seed(2, 711), M)
+    plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6))
+    f0 = plan.encode_gpu(torch.from_numpy(x).cuda(), torch.from_numpy(c).cuda()).cpu().numpy()
+    p = S.shuffle_perm(S.config_seed(2, 712), M)
+    f1 = plan.encode_gpu(torch.from_numpy(x[p].copy()).cuda(),
+                         torch.from_numpy(c[p].copy()).cuda()).cpu().numpy()
+    assert np.array_equal(f0.view(np.uint32), f1.view(np.uint32))
+    N = int(np.prod(n_modes))
+    sel = np.arange(0, N, 211)
+    ref = O.direct_type1(x.astype(np.float64), c.astype(np.complex128), n_modes, modes=sel)
+    assert rel_l2(f0[sel], ref) <= 32e-6
 
 
 def test_type2_batched_tiled_interp_matches_oracle():

@@ -1,5 +1,5 @@
-// Point binning for the MI355X engine: count -> exclusive scan -> scatter ->
-// in-bin order -> prepared records.
+// Point binning for the MI355X engine: radix sort by bin key -> bin offsets
+// -> in-bin order -> prepared records.
 //
 // Reference behaviour (src/gpu_nd_bins.rs:1-33, src/gpu_bin_sort.rs:1-56):
 // points are counted into bins of support starts with integer atomics, the
@@ -9,12 +9,13 @@
 // reference permutation); canonical order sorts by coordinate bits so the
 // type-1 sums do not depend on the input order of the points.
 //
-// MI355X design: the count pass aggregates each workgroup's 1024 points in an
-// LDS hash table before one global atomic per distinct bin (the bin arrays,
-// a few tens of MB, stay in the 256 MB Infinity Cache); a three-kernel
-// reduce-then-scan over 4096-element tiles; bins above 64 points are sorted
-// by a tiled merge sort over all workgroups (LDS bitonic runs, merge-path
-// rounds).
+// MI355X design: no global atomics on the bin counts.  The points are
+// radix-sorted by bin key (least significant 8-bit digit first, per-tile
+// digit histograms, a scan of the digit-major histogram table, tile-ranked
+// scatters staged through LDS); the bin offsets then come from the sorted
+// keys, every offset written by the one position where its bin starts; bins
+// above 64 points are sorted into canonical order by a tiled merge sort over
+// all workgroups (LDS bitonic runs, merge-path rounds).
 #include <hip/hip_runtime.h>
 
 #include "bins.hpp"
@@ -30,124 +31,6 @@ constexpr int kScanTile = kScanThreads * kScanItems;  // 4096
 constexpr int kSortThreads = 256;
 constexpr int kRunThreads = 1024;  // LDS bitonic runs: one thread per pair
 constexpr int kSortRun = 2048;
-
-// ---------------------------------------------------------------------
-// Count: bin key and atomic rank of every point; bins reaching
-// kSmallBin + 1 points are listed once for the large-bin sort.
-// ---------------------------------------------------------------------
-// Each workgroup counts kCountPoints consecutive points: their keys are
-// first aggregated in an LDS hash table (LDS atomics give every point its
-// rank among the workgroup's points of the same bin), then every distinct
-// bin takes one global atomic.  A dense cluster, whose points crowd a few
-// bins, then costs a few global atomics per workgroup instead of one per
-// point on the same address.
-constexpr int kCountThreads = 256;
-constexpr int kCountPerThread = 4;
-constexpr int kCountPoints = kCountThreads * kCountPerThread;
-constexpr int kHashSize = 2 * kCountPoints;
-constexpr uint32_t kEmpty = 0xffffffffu;
-
-template <int D, typename T>
-__global__ __launch_bounds__(kCountThreads) void k_count(int64_t M, const T *__restrict__ coords,
-                                                         GridParams g,
-                                                         uint32_t *__restrict__ counts,
-                                                         uint32_t *__restrict__ keys,
-                                                         uint32_t *__restrict__ ranks,
-                                                         uint32_t *__restrict__ large_list,
-                                                         uint32_t *__restrict__ large_count)
-{
-    __shared__ uint32_t ht_key[kHashSize];
-    __shared__ uint32_t ht_cnt[kHashSize];
-    __shared__ uint32_t ht_base[kHashSize];
-    for (int h = threadIdx.x; h < kHashSize; h += kCountThreads) {
-        ht_key[h] = kEmpty;
-        ht_cnt[h] = 0u;
-    }
-    __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kCountPoints;
-    const int lane = threadIdx.x & 63;
-    const uint64_t lanes_below = (1ull << lane) - 1ull;
-    uint32_t my_key[kCountPerThread], my_slot[kCountPerThread], my_rank[kCountPerThread];
-#pragma unroll
-    for (int i = 0; i < kCountPerThread; ++i) {
-        int64_t j = base + (int64_t)i * kCountThreads + threadIdx.x;
-        my_slot[i] = kEmpty;
-        const bool valid = j < M;
-        const uint32_t key = valid ? (uint32_t)point_bin_key<D, T>(coords, j, g) : kEmpty;
-        my_key[i] = key;
-        bool pending = valid;
-        // Lanes of the wavefront that share a key (a dense cluster) take one
-        // LDS atomic per group instead of one each on the same address;
-        // peeling stops at the first group of one lane (spread-out keys).
-        for (int it = 0; it < 8; ++it) {
-            const uint64_t pm = __ballot(pending);
-            if (pm == 0) break;
-            const int leader = __ffsll((long long)pm) - 1;
-            const uint32_t lk = __shfl(key, leader, 64);
-            const bool mine = pending && key == lk;
-            const uint64_t gm = __ballot(mine);
-            const int gcount = __popcll(gm);
-            uint32_t h = 0, rank0 = 0;
-            if (lane == leader) {
-                h = (lk * 2654435761u) & (kHashSize - 1);
-                for (;;) {
-                    uint32_t old = atomicCAS(&ht_key[h], kEmpty, lk);
-                    if (old == kEmpty || old == lk) break;
-                    h = (h + 1) & (kHashSize - 1);
-                }
-                rank0 = atomicAdd(&ht_cnt[h], (uint32_t)gcount);
-            }
-            h = __shfl(h, leader, 64);
-            rank0 = __shfl(rank0, leader, 64);
-            if (mine) {
-                my_slot[i] = h;
-                my_rank[i] = rank0 + (uint32_t)__popcll(gm & lanes_below);
-                pending = false;
-            }
-            if (gcount < 2) break;
-        }
-        if (pending) {
-            uint32_t h = (key * 2654435761u) & (kHashSize - 1);
-            for (;;) {
-                uint32_t old = atomicCAS(&ht_key[h], kEmpty, key);
-                if (old == kEmpty || old == key) break;
-                h = (h + 1) & (kHashSize - 1);
-            }
-            my_slot[i] = h;
-            my_rank[i] = atomicAdd(&ht_cnt[h], 1u);
-        }
-    }
-    __syncthreads();
-    // every slot's global atomic issued before any result is awaited
-    constexpr int kSlotsPerThread = kHashSize / kCountThreads;
-    uint32_t hk[kSlotsPerThread], hn[kSlotsPerThread], ho[kSlotsPerThread];
-#pragma unroll
-    for (int q = 0; q < kSlotsPerThread; ++q) {
-        const int h = threadIdx.x + q * kCountThreads;
-        hk[q] = ht_key[h];
-        hn[q] = ht_cnt[h];
-    }
-#pragma unroll
-    for (int q = 0; q < kSlotsPerThread; ++q)
-        ho[q] = hk[q] != kEmpty ? atomicAdd(&counts[hk[q]], hn[q]) : 0u;
-#pragma unroll
-    for (int q = 0; q < kSlotsPerThread; ++q) {
-        if (hk[q] == kEmpty) continue;
-        ht_base[threadIdx.x + q * kCountThreads] = ho[q];
-        if (ho[q] <= kSmallBin && ho[q] + hn[q] > kSmallBin) {
-            uint32_t slot = atomicAdd(large_count, 1u);
-            large_list[slot] = hk[q];
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < kCountPerThread; ++i) {
-        int64_t j = base + (int64_t)i * kCountThreads + threadIdx.x;
-        if (j >= M) continue;
-        keys[j] = my_key[i];
-        ranks[j] = ht_base[my_slot[i]] + my_rank[i];
-    }
-}
 
 // ---------------------------------------------------------------------
 // Exclusive scan of n u32 counts (reduce, top-level scan, down-sweep).
@@ -176,7 +59,14 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *total)
     return base + x - v;
 }
 
+// The scanned value of entry k is in[k] - sub[k] (sub may be null).
+__device__ inline uint32_t scan_value(const uint32_t *in, const uint32_t *sub, int64_t k)
+{
+    return sub ? in[k] - sub[k] : in[k];
+}
+
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ in,
+                                                              const uint32_t *__restrict__ sub,
                                                               int64_t n,
                                                               uint32_t *__restrict__ tile_sums)
 {
@@ -185,7 +75,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         int64_t k = base + (int64_t)i * kScanThreads + threadIdx.x;
-        if (k < n) s += in[k];
+        if (k < n) s += scan_value(in, sub, k);
     }
     uint32_t total;
     block_exclusive_scan(s, &total);
@@ -206,10 +96,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(uint32_t *__restrict_
     }
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *__restrict__ in,
+// out may alias in: every entry is read and written by the same thread.
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in,
+                                                            const uint32_t *__restrict__ sub,
                                                             int64_t n,
                                                             const uint32_t *__restrict__ tile_sums,
-                                                            uint32_t *__restrict__ out)
+                                                            uint32_t *out)
 {
     int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
@@ -217,7 +109,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *__re
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         int64_t k = base + i;
-        v[i] = k < n ? in[k] : 0u;
+        v[i] = k < n ? scan_value(in, sub, k) : 0u;
         s += v[i];
     }
     uint32_t total;
@@ -231,16 +123,291 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *__re
 }
 
 // ---------------------------------------------------------------------
-// Scatter: sorted[offset[key] + rank] = point.
+// Least-significant-digit radix sort of the points by bin key, 8 bits per
+// pass, stable, with no global atomics: every pass is a per-tile digit
+// histogram, an exclusive scan of the digit-major histogram table, and a
+// scatter in which every tile ranks its keys per digit in LDS and writes
+// them out digit by digit (contiguous runs per digit).  Pass 0 computes the
+// bin keys from the coordinates; the value of a point is its index, so
+// stability leaves every bin in ascending index order (the reference
+// permutation).
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_scatter(int64_t M, const uint32_t *__restrict__ keys,
-                                                 const uint32_t *__restrict__ ranks,
-                                                 const uint32_t *__restrict__ offsets,
-                                                 uint32_t *__restrict__ sorted)
+constexpr int kRadixThreads = 256;
+constexpr int kRadixWaves = kRadixThreads / 64;
+constexpr int kRadixItems = 16;
+constexpr int kRadixTile = kRadixThreads * kRadixItems;  // 4096 keys
+constexpr int kDigits = 256;
+
+// Lanes of the wavefront whose digit equals this lane's (valid lanes only).
+__device__ inline uint64_t digit_peers(uint32_t d, bool valid)
 {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= M) return;
-    sorted[offsets[keys[j]] + ranks[j]] = (uint32_t)j;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+// Point j of a tile: item i of lane `lane` in wavefront w (wavefront-major,
+// then item, then lane), the order in which ranks are handed out.
+__device__ inline int64_t radix_pos(int64_t t0, int w, int i, int lane)
+{
+    return t0 + (int64_t)w * (kRadixItems * 64) + (int64_t)i * 64 + lane;
+}
+
+// Adds the wavefront's digits of one item to its LDS counters; returns each
+// lane's rank among the wavefront's earlier keys of the same digit.
+__device__ inline uint32_t wave_rank_digit(uint32_t d, bool valid, int lane,
+                                           uint32_t *__restrict__ wcnt)
+{
+    const uint64_t peers = digit_peers(d, valid);
+    const uint32_t below = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    // LDS instructions of a wavefront execute in order: every lane reads the
+    // counter before the digit's first lane writes it back
+    const uint32_t base = wcnt[d];
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("" ::: "memory");
+    if (valid && below == 0) wcnt[d] = base + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("" ::: "memory");
+    return base + below;
+}
+
+__device__ inline void radix_clear(uint32_t (*wcnt)[kDigits])
+{
+    for (int i = threadIdx.x; i < kRadixWaves * kDigits; i += kRadixThreads)
+        (&wcnt[0][0])[i] = 0u;
+    __syncthreads();
+}
+
+// The tile's count of every digit into the digit-major table.
+__device__ inline void radix_store_hist(uint32_t (*wcnt)[kDigits], uint32_t *__restrict__ hist,
+                                        int64_t ntiles)
+{
+    __syncthreads();
+    const int d = threadIdx.x;
+    uint32_t s = 0;
+#pragma unroll
+    for (int w = 0; w < kRadixWaves; ++w) s += wcnt[w][d];
+    hist[(int64_t)d * ntiles + blockIdx.x] = s;
+}
+
+// Pass 0 histogram: sort key of every point (written out) and its low
+// digit.  The sort key is the bin key, followed by the in-bin cell with
+// `cells`.
+template <int D, typename T>
+__global__ __launch_bounds__(kRadixThreads) void k_radix_keys(int64_t M,
+                                                              const T *__restrict__ coords,
+                                                              GridParams g, bool cells,
+                                                              uint32_t *__restrict__ keys,
+                                                              uint32_t *__restrict__ hist,
+                                                              int64_t ntiles)
+{
+    __shared__ uint32_t wcnt[kRadixWaves][kDigits];
+    radix_clear(wcnt);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
+    // coordinates of 8 items in flight at a time
+    constexpr int kGroup = 8;
+    for (int i0 = 0; i0 < kRadixItems; i0 += kGroup) {
+        T x[kGroup][D];
+#pragma unroll
+        for (int i = 0; i < kGroup; ++i) {
+            const int64_t j = radix_pos(t0, w, i0 + i, lane);
+#pragma unroll
+            for (int a = 0; a < D; ++a) x[i][a] = j < M ? coords[j * D + a] : (T)0;
+        }
+#pragma unroll
+        for (int i = 0; i < kGroup; ++i) {
+            const int64_t j = radix_pos(t0, w, i0 + i, lane);
+            const bool valid = j < M;
+            uint32_t key = 0;
+            if (valid) {
+                key = (uint32_t)bin_key_of<D, T>(x[i], g, cells);
+                keys[j] = key;
+            }
+            (void)wave_rank_digit(key & (kDigits - 1), valid, lane, wcnt[w]);
+        }
+    }
+    radix_store_hist(wcnt, hist, ntiles);
+}
+
+// Histogram of the digit at `shift` for passes after the first.
+__global__ __launch_bounds__(kRadixThreads) void k_radix_hist(int64_t M,
+                                                              const uint32_t *__restrict__ keys,
+                                                              int shift,
+                                                              uint32_t *__restrict__ hist,
+                                                              int64_t ntiles)
+{
+    __shared__ uint32_t wcnt[kRadixWaves][kDigits];
+    radix_clear(wcnt);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
+    uint32_t k[kRadixItems];
+#pragma unroll
+    for (int i = 0; i < kRadixItems; ++i) {
+        const int64_t j = radix_pos(t0, w, i, lane);
+        k[i] = j < M ? keys[j] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < kRadixItems; ++i)
+        (void)wave_rank_digit((k[i] >> shift) & (kDigits - 1), radix_pos(t0, w, i, lane) < M,
+                              lane, wcnt[w]);
+    radix_store_hist(wcnt, hist, ntiles);
+}
+
+// Scatter of one pass: hist holds the exclusive scan of the digit-major
+// table, i.e. where each (digit, tile) run starts.  vals_in == nullptr means
+// the value of point j is j.
+__global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
+    int64_t M, const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
+    int shift, const uint32_t *__restrict__ hist, int64_t ntiles, uint32_t *__restrict__ keys_out,
+    uint32_t *__restrict__ vals_out)
+{
+    __shared__ uint32_t wcnt[kRadixWaves][kDigits];
+    __shared__ uint32_t dstart[kDigits], gbase[kDigits];
+    __shared__ uint32_t skey[kRadixTile], sval[kRadixTile];
+    radix_clear(wcnt);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
+    uint32_t k[kRadixItems], v[kRadixItems], r[kRadixItems];
+#pragma unroll
+    for (int i = 0; i < kRadixItems; ++i) {
+        const int64_t j = radix_pos(t0, w, i, lane);
+        const bool valid = j < M;
+        k[i] = valid ? keys_in[j] : 0u;
+        v[i] = valid ? (vals_in ? vals_in[j] : (uint32_t)j) : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < kRadixItems; ++i)
+        r[i] = wave_rank_digit((k[i] >> shift) & (kDigits - 1), radix_pos(t0, w, i, lane) < M,
+                               lane, wcnt[w]);
+    __syncthreads();
+    {
+        // per digit: earlier wavefronts' counts, the tile's digit start and
+        // the run's global start
+        const int d = threadIdx.x;
+        uint32_t s = 0;
+#pragma unroll
+        for (int q = 0; q < kRadixWaves; ++q) {
+            const uint32_t c = wcnt[q][d];
+            wcnt[q][d] = s;
+            s += c;
+        }
+        uint32_t total;
+        dstart[d] = block_exclusive_scan(s, &total);
+        gbase[d] = hist[(int64_t)d * ntiles + blockIdx.x];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kRadixItems; ++i) {
+        if (radix_pos(t0, w, i, lane) >= M) continue;
+        const uint32_t d = (k[i] >> shift) & (kDigits - 1);
+        const uint32_t p = dstart[d] + wcnt[w][d] + r[i];
+        skey[p] = k[i];
+        sval[p] = v[i];
+    }
+    __syncthreads();
+    const int64_t n_here = M - t0 < kRadixTile ? M - t0 : kRadixTile;
+#pragma unroll
+    for (int i = 0; i < kRadixItems; ++i) {
+        const int s = i * kRadixThreads + threadIdx.x;
+        if (s >= n_here) continue;
+        const uint32_t key = skey[s];
+        const uint32_t d = (key >> shift) & (kDigits - 1);
+        const int64_t o = (int64_t)gbase[d] + (s - (int64_t)dstart[d]);
+        keys_out[o] = key;
+        vals_out[o] = sval[s];
+    }
+}
+
+// Bin offsets from the sorted keys: position j (0 <= j <= M, j = M standing
+// for a key of nbins) is the offset of every bin in (bin[j-1], bin[j]] (bin =
+// sort key >> cell_shift), which gives each of the nbins + 1 offsets exactly
+// one writer.  Gaps of up to
+// kGapInline bins are written here; longer ones are listed (kGapMid: one
+// workgroup each, longer: split over every workgroup) for k_fill_gaps.
+// With a large-bin list, bins of more than kSmallBin points are listed once
+// (at their last position).
+constexpr int64_t kGapInline = 64;
+constexpr int64_t kGapMid = 4096;
+
+__global__ __launch_bounds__(256) void k_bin_offsets(int64_t M, const uint32_t *__restrict__ keys,
+                                                     int cell_shift, int64_t nbins,
+                                                     uint32_t *__restrict__ offsets,
+                                                     uint4 *__restrict__ gaps_mid,
+                                                     uint4 *__restrict__ gaps_long,
+                                                     uint32_t *__restrict__ ctrl,
+                                                     uint32_t *__restrict__ large_list)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    int64_t lo = 0, hi = -1;  // bins [lo, hi] take offset j
+    bool large = false;
+    uint32_t k = 0;
+    if (j <= M) {
+        const int64_t kp = j > 0 ? (int64_t)(keys[j - 1] >> cell_shift) : -1;
+        const int64_t kj = j < M ? (int64_t)(keys[j] >> cell_shift) : nbins;
+        k = (uint32_t)kj;
+        if (kj != kp) {
+            lo = kp + 1;
+            hi = kj;
+        }
+        // (large bins are listed for orders without in-bin cells only)
+        if (large_list && j < M && (j == M - 1 || keys[j + 1] != k) &&
+            j >= (int64_t)kSmallBin && keys[j - kSmallBin] == k)
+            large = true;
+    }
+    const int64_t len = hi - lo + 1;
+    if (len > 0 && len <= kGapInline)
+        for (int64_t b = lo; b <= hi; ++b) offsets[b] = (uint32_t)j;
+    // one atomic per wavefront and list
+    const bool mid = len > kGapInline && len <= kGapMid;
+    const bool lng = len > kGapMid;
+    const uint64_t mm = __ballot(mid), ml = __ballot(lng), mg = __ballot(large);
+    uint32_t bm = 0, bl = 0, bg = 0;
+    if (lane == 0) {
+        if (mm) bm = atomicAdd(ctrl + 2, (uint32_t)__popcll(mm));
+        if (ml) bl = atomicAdd(ctrl + 3, (uint32_t)__popcll(ml));
+        if (mg) bg = atomicAdd(ctrl + 0, (uint32_t)__popcll(mg));
+    }
+    bm = __shfl(bm, 0, 64);
+    bl = __shfl(bl, 0, 64);
+    bg = __shfl(bg, 0, 64);
+    const uint4 gap = make_uint4((uint32_t)lo, (uint32_t)hi, (uint32_t)j, 0u);
+    if (mid) gaps_mid[bm + __popcll(mm & below)] = gap;
+    if (lng) gaps_long[bl + __popcll(ml & below)] = gap;
+    if (large) large_list[bg + __popcll(mg & below)] = k;
+}
+
+// Listed gaps: a mid gap per workgroup (grid-stride); every long gap cut in
+// chunks of kGapMid bins spread round-robin over the workgroups.
+__global__ __launch_bounds__(256) void k_fill_gaps(const uint4 *__restrict__ gaps_mid,
+                                                   const uint4 *__restrict__ gaps_long,
+                                                   const uint32_t *__restrict__ ctrl,
+                                                   uint32_t *__restrict__ offsets)
+{
+    const uint32_t n_mid = ctrl[2], n_long = ctrl[3];
+    for (uint32_t q = blockIdx.x; q < n_mid; q += gridDim.x) {
+        const uint4 gp = gaps_mid[q];
+        for (int64_t b = (int64_t)gp.x + threadIdx.x; b <= (int64_t)gp.y; b += blockDim.x)
+            offsets[b] = gp.z;
+    }
+    for (uint32_t q = 0; q < n_long; ++q) {
+        const uint4 gp = gaps_long[q];
+        const int64_t len = (int64_t)gp.y - gp.x + 1;
+        const int64_t chunks = (len + kGapMid - 1) / kGapMid;
+        for (int64_t c = (blockIdx.x + (int64_t)gridDim.x - q % gridDim.x) % gridDim.x; c < chunks;
+             c += gridDim.x) {
+            const int64_t c0 = (int64_t)gp.x + c * kGapMid;
+            const int64_t c1 = c0 + kGapMid - 1 < (int64_t)gp.y ? c0 + kGapMid - 1 : (int64_t)gp.y;
+            for (int64_t b = c0 + threadIdx.x; b <= c1; b += blockDim.x) offsets[b] = gp.z;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -293,18 +460,16 @@ __device__ inline SortElem make_elem(const T *coords, uint32_t idx, int order)
     return e;
 }
 
-// Ordered sorts: every point's ordering element (coordinate bits or index,
-// and the index) scattered into its bin's slot range, read sequentially.
+// Canonical order: the ordering element of the point at every sorted slot
+// (its coordinate bits and index), gathered from the radix-sorted indices.
 template <int D, typename T>
-__global__ __launch_bounds__(256) void k_scatter_elems(int64_t M, const T *__restrict__ coords,
-                                                       const uint32_t *__restrict__ keys,
-                                                       const uint32_t *__restrict__ ranks,
-                                                       const uint32_t *__restrict__ offsets,
-                                                       int order, SortElem *__restrict__ elems)
+__global__ __launch_bounds__(256) void k_make_elems(int64_t M, const uint32_t *__restrict__ sorted,
+                                                    const T *__restrict__ coords,
+                                                    SortElem *__restrict__ elems)
 {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= M) return;
-    elems[offsets[keys[j]] + ranks[j]] = make_elem<D, T>(coords, (uint32_t)j, order);
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    elems[s] = make_elem<D, T>(coords, sorted[s], (int)BinOrder::Canonical);
 }
 
 // ---------------------------------------------------------------------
@@ -650,45 +815,59 @@ BinSorter::~BinSorter() { release(); }
 
 void BinSorter::release()
 {
-    for (void *p : {(void *)counts_, (void *)offsets_, (void *)keys_, (void *)ranks_,
-                    (void *)sorted_, (void *)large_list_, (void *)tile_off_, (void *)tile_sums_,
-                    (void *)scratch_a_, (void *)scratch_b_})
+    for (uint32_t *p : {ctrl_, offsets_, keys_, keys_b_, vals_, sorted_, hist_, hist_sums_,
+                        large_list_, tile_off_})
         if (p) (void)hipFree(p);
-    counts_ = offsets_ = keys_ = ranks_ = sorted_ = large_list_ = tile_off_ = tile_sums_ = nullptr;
+    for (void *p : {(void *)gaps_, scratch_a_, scratch_b_})
+        if (p) (void)hipFree(p);
+    ctrl_ = offsets_ = nullptr;
+    keys_ = keys_b_ = vals_ = sorted_ = hist_ = hist_sums_ = large_list_ = tile_off_ = nullptr;
+    gaps_ = nullptr;
     scratch_a_ = scratch_b_ = nullptr;
     bins_cap_ = points_cap_ = sort_scratch_cap_ = 0;
 }
 
+namespace {
+
+bool alloc_u32(uint32_t **p, int64_t n, const char *what)
+{
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    return hip_ok(hipMalloc(p, sizeof(uint32_t) * (size_t)(n > 0 ? n : 1)), what);
+}
+
+int64_t radix_tiles(int64_t M) { return (M + kRadixTile - 1) / kRadixTile; }
+
+}  // namespace
+
 bool BinSorter::ensure(int64_t nbins, int64_t M, bool need_sort_scratch)
 {
     if (nbins > bins_cap_) {
-        if (counts_) (void)hipFree(counts_);
-        if (offsets_) (void)hipFree(offsets_);
-        if (tile_sums_) (void)hipFree(tile_sums_);
-        counts_ = offsets_ = tile_sums_ = nullptr;
-        // counts: nbins + trailing zero, then the large-bin count and the
-        // largest large bin
-        if (!hip_ok(hipMalloc(&counts_, sizeof(uint32_t) * (nbins + 3)), "bin counts")) return false;
-        if (!hip_ok(hipMalloc(&offsets_, sizeof(uint32_t) * (nbins + 1)), "bin offsets")) return false;
-        int64_t tiles = (nbins + 1 + kScanTile - 1) / kScanTile;
-        if (!hip_ok(hipMalloc(&tile_sums_, sizeof(uint32_t) * (tiles + 1)), "scan tiles")) return false;
+        if (!alloc_u32(&offsets_, nbins + 1, "bin offsets")) return false;
+        // gap lists: at most one gap of more than kGapInline (kGapMid) bins
+        // per kGapInline + 1 (kGapMid + 1) offsets
+        const int64_t n_mid = (nbins + 1) / (kGapInline + 1) + 2;
+        const int64_t n_long = (nbins + 1) / (kGapMid + 1) + 2;
+        if (gaps_) (void)hipFree(gaps_);
+        gaps_ = nullptr;
+        if (!hip_ok(hipMalloc(&gaps_, sizeof(uint4) * (n_mid + n_long)), "bin gap lists"))
+            return false;
+        gaps_mid_cap_ = n_mid;
         bins_cap_ = nbins;
     }
+    if (!ctrl_ && !alloc_u32(&ctrl_, 4, "bin sort counters")) return false;
     if (M > points_cap_) {
-        for (uint32_t **p : {&keys_, &ranks_, &sorted_, &large_list_, &tile_off_}) {
-            if (*p) (void)hipFree(*p);
-            *p = nullptr;
-        }
-        int64_t cap = M + M / 4 + 1024;
-        if (!hip_ok(hipMalloc(&keys_, sizeof(uint32_t) * cap), "point keys")) return false;
-        if (!hip_ok(hipMalloc(&ranks_, sizeof(uint32_t) * cap), "point ranks")) return false;
-        if (!hip_ok(hipMalloc(&sorted_, sizeof(uint32_t) * cap), "sorted points")) return false;
-        if (!hip_ok(hipMalloc(&large_list_, sizeof(uint32_t) * (cap / (kSmallBin + 1) + 2)),
-                    "large-bin list"))
+        const int64_t cap = M + M / 4 + 1024;
+        if (!alloc_u32(&keys_, cap, "point keys")) return false;
+        if (!alloc_u32(&keys_b_, cap, "point keys")) return false;
+        if (!alloc_u32(&vals_, cap, "point values")) return false;
+        if (!alloc_u32(&sorted_, cap, "sorted points")) return false;
+        const int64_t hist = kDigits * radix_tiles(cap);
+        if (!alloc_u32(&hist_, hist, "radix histograms")) return false;
+        if (!alloc_u32(&hist_sums_, (hist + kScanTile - 1) / kScanTile + 1, "radix scan tiles"))
             return false;
-        if (!hip_ok(hipMalloc(&tile_off_, sizeof(uint32_t) * (cap / (kSmallBin + 1) + 3)),
-                    "large-bin tiles"))
-            return false;
+        if (!alloc_u32(&large_list_, cap / (kSmallBin + 1) + 2, "large-bin list")) return false;
+        if (!alloc_u32(&tile_off_, cap / (kSmallBin + 1) + 3, "large-bin tiles")) return false;
         points_cap_ = cap;
     }
     if (need_sort_scratch && sort_scratch_cap_ < points_cap_) {
@@ -708,36 +887,79 @@ __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
     if (i < n) p[i] = v;
 }
 
+namespace {
+
+// Exclusive scan of in[k] - sub[k] (sub may be null) over n entries into out
+// (which may be in).
+void scan_u32(const uint32_t *in, const uint32_t *sub, int64_t n, uint32_t *sums, uint32_t *out,
+              hipStream_t stream)
+{
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    k_scan_reduce<<<(unsigned)tiles, kScanThreads, 0, stream>>>(in, sub, n, sums);
+    k_scan_top<<<1, kScanThreads, 0, stream>>>(sums, tiles);
+    k_scan_down<<<(unsigned)tiles, kScanThreads, 0, stream>>>(in, sub, n, sums, out);
+}
+
+}  // namespace
+
 template <int D, typename T>
 bool BinSorter::sort(int64_t M, const T *coords, const GridParams &g, BinOrder order,
                      hipStream_t stream)
 {
     nbins_ = g.nbins;
     M_ = M;
-    if (!ensure(g.nbins, M, order != BinOrder::Grouped)) return false;
-    // Clear counts, trailing zero and the large-bin count with a fill kernel
-    // (no memset nodes, so callers may capture the sequence in a graph).
-    k_fill_u32<<<grid_for(g.nbins + 3, 256), 256, 0, stream>>>(counts_, g.nbins + 3, 0u);
+    if (!ensure(g.nbins, M, order == BinOrder::Canonical)) return false;
+    // Clear the counters (large bins, largest large bin, mid and long gaps)
+    // with a fill kernel (no memset nodes, so callers may capture the
+    // sequence in a graph).
+    k_fill_u32<<<1, 64, 0, stream>>>(ctrl_, 4, 0u);
     if (M == 0) {
         k_fill_u32<<<grid_for(g.nbins + 1, 256), 256, 0, stream>>>(offsets_, g.nbins + 1, 0u);
         return hip_ok(hipGetLastError(), "bin clear");
     }
-    uint32_t *large_count = counts_ + g.nbins + 1;
-    uint32_t *large_max = counts_ + g.nbins + 2;
-    k_count<D, T><<<grid_for(M, kCountPoints), kCountThreads, 0, stream>>>(
-        M, coords, g, counts_, keys_, ranks_, large_list_, large_count);
-    int64_t n = g.nbins + 1;
-    int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    k_scan_reduce<<<(unsigned)tiles, kScanThreads, 0, stream>>>(counts_, n, tile_sums_);
-    k_scan_top<<<1, kScanThreads, 0, stream>>>(tile_sums_, tiles);
-    k_scan_down<<<(unsigned)tiles, kScanThreads, 0, stream>>>(counts_, n, tile_sums_, offsets_);
-    if (order == BinOrder::Grouped) {
-        k_scatter<<<grid_for(M, 256), 256, 0, stream>>>(M, keys_, ranks_, offsets_, sorted_);
-    } else {
+    uint32_t *large_count = ctrl_;
+    uint32_t *large_max = ctrl_ + 1;
+
+    // radix passes over the bits of the largest key, values ending in sorted_;
+    // the grouped order also sorts by the support-start cell inside the bin
+    // (neighbouring points then read neighbouring grid windows)
+    int bits = 0;
+    while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)g.nbins) ++bits;
+    const int cbits = cell_bits<D>(g);
+    const bool cells = order == BinOrder::Grouped && bits + cbits <= 32;
+    const int cell_shift = cells ? cbits : 0;
+    bits += cell_shift;
+    const int passes = bits > 8 ? (bits + 7) / 8 : 1;
+    const int64_t ntiles = radix_tiles(M);
+    const int64_t nhist = kDigits * ntiles;
+    uint32_t *K[2] = {keys_, keys_b_};
+    uint32_t *V[2];
+    V[passes & 1] = sorted_;
+    V[(passes + 1) & 1] = vals_;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        if (p == 0)
+            k_radix_keys<D, T><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
+                M, coords, g, cells, K[0], hist_, ntiles);
+        else
+            k_radix_hist<<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(M, K[p & 1], shift,
+                                                                         hist_, ntiles);
+        scan_u32(hist_, nullptr, nhist, hist_sums_, hist_, stream);
+        k_radix_scatter<<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
+            M, K[p & 1], p == 0 ? nullptr : V[p & 1], shift, hist_, ntiles, K[(p + 1) & 1],
+            V[(p + 1) & 1]);
+    }
+    const uint32_t *sorted_keys = K[passes & 1];
+    uint4 *gaps_long = gaps_ + gaps_mid_cap_;
+    k_bin_offsets<<<grid_for(M + 1, 256), 256, 0, stream>>>(
+        M, sorted_keys, cell_shift, g.nbins, offsets_, gaps_, gaps_long, ctrl_,
+        order == BinOrder::Canonical ? large_list_ : nullptr);
+    k_fill_gaps<<<1024, 256, 0, stream>>>(gaps_, gaps_long, ctrl_, offsets_);
+
+    if (order == BinOrder::Canonical) {
         unsigned grid = (unsigned)std::min<int64_t>(2048, M / kSortRun + 64);
         SortElem *A = (SortElem *)scratch_a_, *B = (SortElem *)scratch_b_;
-        k_scatter_elems<D, T><<<grid_for(M, 256), 256, 0, stream>>>(M, coords, keys_, ranks_,
-                                                                    offsets_, (int)order, A);
+        k_make_elems<D, T><<<grid_for(M, 256), 256, 0, stream>>>(M, sorted_, coords, A);
         k_large_tiles<<<1, 1024, 0, stream>>>(large_list_, large_count, offsets_, tile_off_,
                                               large_max);
         k_large_runs<<<grid, kRunThreads, 0, stream>>>(large_list_, large_count, tile_off_,
@@ -762,7 +984,9 @@ bool BinSorter::prepare(int64_t M, const T *coords, const GridParams &g, BinOrde
                         int4 *rec_start, T *rec_off, uint32_t *out_sorted, hipStream_t stream)
 {
     if (M == 0) return true;
-    if (order == BinOrder::Grouped)
+    // the radix sort leaves every bin in index order: the grouped and the
+    // index orders read the sorted indices as they are
+    if (order != BinOrder::Canonical)
         k_prepare_grouped<D, T><<<grid_for(M, 256), 256, 0, stream>>>(M, sorted_, coords, g,
                                                                       rec_start, rec_off,
                                                                       out_sorted);

@@ -35,14 +35,18 @@ public:
 private:
     bool ensure(int64_t nbins, int64_t M, bool need_sort_scratch);
 
-    uint32_t *counts_ = nullptr;
+    uint32_t *ctrl_ = nullptr;     // large bins, largest large bin, mid gaps, long gaps
     uint32_t *offsets_ = nullptr;
-    uint32_t *keys_ = nullptr;
-    uint32_t *ranks_ = nullptr;
+    uint4 *gaps_ = nullptr;        // gap lists of the offsets (mid, then long)
+    int64_t gaps_mid_cap_ = 0;
+    uint32_t *keys_ = nullptr;     // radix key buffers (ping-pong)
+    uint32_t *keys_b_ = nullptr;
+    uint32_t *vals_ = nullptr;     // radix value buffer paired with sorted_
     uint32_t *sorted_ = nullptr;
+    uint32_t *hist_ = nullptr;     // digit-major per-tile digit counts
+    uint32_t *hist_sums_ = nullptr;
     uint32_t *large_list_ = nullptr;
     uint32_t *tile_off_ = nullptr;
-    uint32_t *tile_sums_ = nullptr;
     void *scratch_a_ = nullptr;
     void *scratch_b_ = nullptr;
     int64_t bins_cap_ = 0, points_cap_ = 0, sort_scratch_cap_ = 0;

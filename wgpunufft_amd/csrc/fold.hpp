@@ -93,22 +93,49 @@ __device__ inline bool point_in_reach(const T *coords, int64_t j)
     return r;
 }
 
-// Bin key of point j: axis 0 fastest over the wrapped starts shifted by the
-// bin sides.
+// Bin key of a point from its coordinates: axis 0 fastest over the wrapped
+// starts shifted by the bin sides.  With `cell`, the start's cell inside its
+// bin follows below the bin key (axis 0 fastest, the bin sides' bits).
 template <int D, typename T>
-__device__ inline uint64_t point_bin_key(const T *coords, int64_t j, const GridParams &g)
+__device__ inline uint64_t bin_key_of(const T *x, const GridParams &g, bool cell = false)
 {
-    bool reach = point_in_reach<D, T>(coords, j);
-    uint64_t key = 0;
+    bool reach = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
+    uint64_t key = 0, sub = 0;
+    int sub_bits = 0;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         int64_t s;
         double off;
-        support_start<T>(coords[j * D + a], reach, g, a, &s, &off);
+        support_start<T>(x[a], reach, g, a, &s, &off);
         int64_t ws = wrapped_start(s, g.ax[a].n);
         key += (uint64_t)(ws >> g.ax[a].shift) * (uint64_t)g.ax[a].stride;
+        if (cell) {
+            sub |= (uint64_t)(ws & (((int64_t)1 << g.ax[a].shift) - 1)) << sub_bits;
+            sub_bits += g.ax[a].shift;
+        }
     }
-    return key;
+    return cell ? (key << sub_bits) | sub : key;
+}
+
+// Bits of the in-bin cell index that bin_key_of appends.
+template <int D>
+__device__ __host__ inline int cell_bits(const GridParams &g)
+{
+    int b = 0;
+    for (int a = 0; a < D; ++a) b += g.ax[a].shift;
+    return b;
+}
+
+// Bin key of point j of point-major coordinates.
+template <int D, typename T>
+__device__ inline uint64_t point_bin_key(const T *coords, int64_t j, const GridParams &g)
+{
+    T x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = coords[j * D + a];
+    return bin_key_of<D, T>(x, g);
 }
 
 }  // namespace mnufft

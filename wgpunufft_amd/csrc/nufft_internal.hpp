@@ -33,7 +33,7 @@ enum class ModeOrder : int { Centered = 0, Fft = 1 };
 enum class SpreadMethod : int { Plain = 0, MfmaF16 = 1, MfmaF16x3 = 2, MfmaBf16x3 = 3 };
 // In-bin order of the sorted points.
 enum class BinOrder : int {
-    Grouped = 0,    // whatever the atomic scatter left (type 2)
+    Grouped = 0,    // by support-start cell inside each bin (type 2)
     Index = 1,      // ascending original index: the reference permutation
     Canonical = 2,  // ascending coordinate bits, then index (type-1 spread)
 };
