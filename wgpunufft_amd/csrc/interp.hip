@@ -124,13 +124,17 @@ __global__ __launch_bounds__(256) void k_interp(int64_t M, int64_t vectors,
 }
 
 // ---------------------------------------------------------------------
-// LDS-tiled 3D interpolation.  Type-2 bins here are tiles of TB^3 cells of
-// support starts; a work item is one tile and up to kTilePoints of its
-// points.  The workgroup stages the tile plus its (w-1)-cell halo (periodic)
-// in LDS once, then every lane evaluates whole points from LDS: w^3 LDS reads
-// and 2 w^3 FMAs per point, with the fine grid read from HBM about
-// (TB+w-1)^3/TB^3 times per item instead of w^3 times per point.  Each
-// output is one lane's fixed-order sum.
+// LDS-tiled 3D interpolation.  Type-2 bins here are tiles of TX x TY x TZ
+// (8^3) cells of support starts; a work item is one tile and up to P of its
+// points, which the bin sort leaves ordered by support-start cell, so the
+// lanes of a wavefront read neighbouring windows.  The workgroup stages the
+// tile plus its (w-1)-cell halo (periodic) in LDS once, then every lane
+// evaluates whole points from LDS: w^2 rows of w cells (F32: read as aligned
+// pairs of cells, 16 bytes per LDS read) and 2 w^3 FMAs per point, with the
+// fine grid read from HBM about (TX+w-1)(TY+w-1)(TZ+w-1)/(TX TY TZ) times
+// per item instead of w^3 times per point.  Work items are dealt to the
+// workgroups in order, so the items of a heavy tile spread over every XCD.
+// Each output is one lane's fixed-order sum.
 // ---------------------------------------------------------------------
 template <typename T>
 struct Complex2;
@@ -143,13 +147,10 @@ struct Complex2<double> {
     using type = double2;
 };
 
-constexpr int kTileEdge = 8;
-constexpr int kTilePoints = 1024;
-constexpr int kTileThreads = 256;
-
 // item_off[b] = first item of bin b (exclusive scan of ceil(count / P)),
 // item_off[nbins] = items.
-__global__ __launch_bounds__(1024) void k_tile_items(int64_t nbins, const uint32_t *__restrict__ offsets,
+__global__ __launch_bounds__(1024) void k_tile_items(int64_t nbins, uint32_t points_per_item,
+                                                     const uint32_t *__restrict__ offsets,
                                                      uint32_t *__restrict__ item_off)
 {
     __shared__ uint32_t wsum[16];
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(1024) void k_tile_items(int64_t nbins, const uint32
         uint32_t t = 0;
         if (b < nbins) {
             uint32_t n = offsets[b + 1] - offsets[b];
-            t = (n + kTilePoints - 1) / kTilePoints;
+            t = (n + points_per_item - 1) / points_per_item;
         }
         uint32_t x = t;
         for (int o = 1; o < 64; o <<= 1) {
@@ -184,48 +185,68 @@ __global__ __launch_bounds__(1024) void k_tile_items(int64_t nbins, const uint32
     if (threadIdx.x == 0) item_off[nbins] = carry;
 }
 
-template <typename T, int W>
-__global__ __launch_bounds__(kTileThreads) void k_interp_tiled3(
+// Tile and part of work item `item`.
+struct TileItem3 {
+    int64_t b, k, x0, y0, z0;
+};
+
+__device__ inline TileItem3 tile_item3(int64_t item, const uint32_t *__restrict__ item_off,
+                                       const GridParams &g, int tx, int ty, int tz)
+{
+    int64_t lo = 0, hi = g.nbins - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if ((int64_t)item_off[mid] <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    TileItem3 t;
+    t.b = lo;
+    t.k = item - item_off[lo];
+    t.x0 = (lo % g.ax[0].bins) * tx;
+    t.y0 = ((lo / g.ax[0].bins) % g.ax[1].bins) * ty;
+    t.z0 = lo / (g.ax[0].bins * g.ax[1].bins) * tz;
+    return t;
+}
+
+template <typename T, int W, int TX, int TY, int TZ, int THREADS, int P>
+__global__ __launch_bounds__(THREADS) void k_interp_tiled3(
     int64_t M, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
     const uint32_t *__restrict__ item_off, const T *__restrict__ coords, GridParams g,
     const T *__restrict__ grid, T *__restrict__ out)
 {
-    constexpr int E = kTileEdge + W - 1;
+    // F32: every row of a point's window is read as NP aligned pairs of
+    // cells (16-byte LDS reads), starting at the even cell at or before the
+    // window, with a zero weight on the cell outside it
+    constexpr bool kPairs = sizeof(T) == 4;
+    constexpr int NP = (W + 2) / 2;
+    constexpr int EX = TX + W - 1, EY = TY + W - 1, EZ = TZ + W - 1;
+    constexpr int EXR = kPairs && TX + 2 * NP - 2 > EX ? TX + 2 * NP - 2 : EX;
+    constexpr int RX = kPairs ? (EXR + 1) / 2 * 2 : EX;  // LDS row stride (cells)
+    constexpr int CELLS = RX * EY * EZ;
     using T2 = typename Complex2<T>::type;
-    __shared__ T2 tile[E * E * E];
-    const int64_t nbins = g.nbins;
-    const int64_t n_items = item_off[nbins];
+    __shared__ __align__(16) T2 tile[CELLS];
+    const int64_t n_items = item_off[g.nbins];
     const int64_t v = blockIdx.y;
     const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n, n2 = g.ax[2].n;
     const T beta = (T)g.beta;
     const T *gv = grid + 2 * v * g.fine_total;
     for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-        int64_t lo = 0, hi = nbins - 1;
-        while (lo < hi) {
-            int64_t mid = (lo + hi + 1) >> 1;
-            if ((int64_t)item_off[mid] <= item)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const int64_t b = lo;
-        const int64_t k = item - item_off[b];
-        const int64_t bx = b % g.ax[0].bins, by = (b / g.ax[0].bins) % g.ax[1].bins,
-                      bz = b / (g.ax[0].bins * g.ax[1].bins);
-        const int64_t x0 = bx * kTileEdge, y0 = by * kTileEdge, z0 = bz * kTileEdge;
-        for (int t = threadIdx.x; t < E * E * E; t += kTileThreads) {
-            int lx = t % E, ly = (t / E) % E, lz = t / (E * E);
-            int64_t gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+        const TileItem3 ti = tile_item3(item, item_off, g, TX, TY, TZ);
+        for (int t = threadIdx.x; t < CELLS; t += THREADS) {
+            const int lx = t % RX, ly = (t / RX) % EY, lz = t / (RX * EY);
+            int64_t gx = ti.x0 + lx, gy = ti.y0 + ly, gz = ti.z0 + lz;
             gx = gx >= n0 ? gx - n0 : gx;
             gy = gy >= n1 ? gy - n1 : gy;
             gz = gz >= n2 ? gz - n2 : gz;
             tile[t] = *(const T2 *)(gv + 2 * (gx + n0 * (gy + n1 * gz)));
         }
         __syncthreads();
-        const int64_t first = (int64_t)offsets[b] + k * kTilePoints;
-        int64_t last = first + kTilePoints;
-        if (last > (int64_t)offsets[b + 1]) last = offsets[b + 1];
-        for (int64_t s = first + threadIdx.x; s < last; s += kTileThreads) {
+        const int64_t first = (int64_t)offsets[ti.b] + ti.k * P;
+        int64_t last = first + P;
+        if (last > (int64_t)offsets[ti.b + 1]) last = offsets[ti.b + 1];
+        for (int64_t s = first + threadIdx.x; s < last; s += THREADS) {
             const uint32_t idx = sorted[s];
             const bool reach = point_in_reach<3, T>(coords, idx);
             int l[3];
@@ -236,30 +257,63 @@ __global__ __launch_bounds__(kTileThreads) void k_interp_tiled3(
                 double off;
                 support_start<T>(coords[(int64_t)idx * 3 + a], reach, g, a, &start, &off);
                 int64_t ws = wrapped_start(start, g.ax[a].n);
-                int64_t o = a == 0 ? x0 : (a == 1 ? y0 : z0);
+                int64_t o = a == 0 ? ti.x0 : (a == 1 ? ti.y0 : ti.z0);
                 l[a] = (int)(ws - o);
 #pragma unroll
                 for (int j = 0; j < W; ++j) wt[a][j] = es_w<T>((T)j + (T)off, W, beta);
             }
             T sre = (T)0, sim = (T)0;
+            if constexpr (kPairs) {
+                const int sh = l[0] & 1;
+                T wx[2 * NP];
 #pragma unroll
-            for (int dz = 0; dz < W; ++dz) {
-                T pre = (T)0, pim = (T)0;
-#pragma unroll
-                for (int dy = 0; dy < W; ++dy) {
-                    const int rowb = ((l[2] + dz) * E + (l[1] + dy)) * E + l[0];
-                    T rre = (T)0, rim = (T)0;
-#pragma unroll
-                    for (int dx = 0; dx < W; ++dx) {
-                        const T2 cv = tile[rowb + dx];
-                        rre += wt[0][dx] * cv.x;
-                        rim += wt[0][dx] * cv.y;
-                    }
-                    pre += wt[1][dy] * rre;
-                    pim += wt[1][dy] * rim;
+                for (int k = 0; k < 2 * NP; ++k) {
+                    const T a0 = k < W ? wt[0][k] : (T)0;
+                    const T a1 = k >= 1 && k - 1 < W ? wt[0][k - 1] : (T)0;
+                    wx[k] = sh ? a1 : a0;
                 }
-                sre += wt[2][dz] * pre;
-                sim += wt[2][dz] * pim;
+#pragma unroll
+                for (int dz = 0; dz < W; ++dz) {
+                    T pre = (T)0, pim = (T)0;
+#pragma unroll
+                    for (int dy = 0; dy < W; ++dy) {
+                        const int rowb = ((l[2] + dz) * EY + (l[1] + dy)) * RX + l[0] - sh;
+                        const float4 *rp = (const float4 *)(tile + rowb);
+                        T rre = (T)0, rim = (T)0;
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) {
+                            const float4 c2 = rp[q];
+                            rre += wx[2 * q] * c2.x;
+                            rim += wx[2 * q] * c2.y;
+                            rre += wx[2 * q + 1] * c2.z;
+                            rim += wx[2 * q + 1] * c2.w;
+                        }
+                        pre += wt[1][dy] * rre;
+                        pim += wt[1][dy] * rim;
+                    }
+                    sre += wt[2][dz] * pre;
+                    sim += wt[2][dz] * pim;
+                }
+            } else {
+#pragma unroll
+                for (int dz = 0; dz < W; ++dz) {
+                    T pre = (T)0, pim = (T)0;
+#pragma unroll
+                    for (int dy = 0; dy < W; ++dy) {
+                        const int rowb = ((l[2] + dz) * EY + (l[1] + dy)) * RX + l[0];
+                        T rre = (T)0, rim = (T)0;
+#pragma unroll
+                        for (int dx = 0; dx < W; ++dx) {
+                            const T2 cv = tile[rowb + dx];
+                            rre += wt[0][dx] * cv.x;
+                            rim += wt[0][dx] * cv.y;
+                        }
+                        pre += wt[1][dy] * rre;
+                        pim += wt[1][dy] * rim;
+                    }
+                    sre += wt[2][dz] * pre;
+                    sim += wt[2][dz] * pim;
+                }
             }
             T *dst = out + 2 * (v * M + (int64_t)idx);
             dst[0] = sre;
@@ -269,17 +323,30 @@ __global__ __launch_bounds__(kTileThreads) void k_interp_tiled3(
     }
 }
 
-template <typename T, int W>
+template <typename T, int W, int TX, int TY, int TZ, int THREADS, int P>
 bool launch_tiled(int64_t M, int64_t vectors, const uint32_t *sorted, const uint32_t *offsets,
                   uint32_t *item_off, const T *coords, const GridParams &g, const T *grid, T *out,
                   hipStream_t stream)
 {
-    k_tile_items<<<1, 1024, 0, stream>>>(g.nbins, offsets, item_off);
-    unsigned items = (unsigned)std::min<int64_t>(g.nbins + M / kTilePoints + 64, 16384);
-    dim3 grid3(items, (unsigned)vectors);
-    k_interp_tiled3<T, W><<<grid3, kTileThreads, 0, stream>>>(M, sorted, offsets, item_off, coords,
-                                                              g, grid, out);
+    k_tile_items<<<1, 1024, 0, stream>>>(g.nbins, (uint32_t)P, offsets, item_off);
+    int64_t items = std::min<int64_t>(g.nbins + M / P + 64, 16384);
+    dim3 grid3((unsigned)items, (unsigned)vectors);
+    k_interp_tiled3<T, W, TX, TY, TZ, THREADS, P><<<grid3, THREADS, 0, stream>>>(
+        M, sorted, offsets, item_off, coords, g, grid, out);
     return hip_ok(hipGetLastError(), "tiled interpolation launch");
+}
+
+template <typename T, int W>
+bool launch_tiled_w(int64_t M, int64_t vectors, const uint32_t *sorted, const uint32_t *offsets,
+                    uint32_t *item_off, const T *coords, const GridParams &g, const T *grid, T *out,
+                    hipStream_t stream)
+{
+    // the bin sides of the type-2 geometry are the tile
+    if (g.ax[0].shift == 3 && g.ax[1].shift == 3 && g.ax[2].shift == 3)
+        return launch_tiled<T, W, 8, 8, 8, 256, 1024>(M, vectors, sorted, offsets, item_off,
+                                                      coords, g, grid, out, stream);
+    set_error("tiled interpolation: unsupported tile geometry");
+    return false;
 }
 
 template <int D, typename T, int G>
@@ -311,13 +378,13 @@ bool interpolate_tiled3(int64_t M, int64_t vectors, const uint32_t *sorted, cons
 {
     if (M == 0) return true;
     switch (g.width) {
-    case 2: return launch_tiled<T, 2>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
-    case 3: return launch_tiled<T, 3>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
-    case 4: return launch_tiled<T, 4>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
-    case 5: return launch_tiled<T, 5>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
-    case 6: return launch_tiled<T, 6>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
-    case 7: return launch_tiled<T, 7>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
-    case 8: return launch_tiled<T, 8>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 2: return launch_tiled_w<T, 2>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 3: return launch_tiled_w<T, 3>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 4: return launch_tiled_w<T, 4>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 5: return launch_tiled_w<T, 5>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 6: return launch_tiled_w<T, 6>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 7: return launch_tiled_w<T, 7>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
+    case 8: return launch_tiled_w<T, 8>(M, vectors, sorted, offsets, item_off, coords, g, grid, out, stream);
     default: set_error("tiled interpolation needs a kernel width <= 8"); return false;
     }
 }
