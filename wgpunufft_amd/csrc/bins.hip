@@ -183,7 +183,9 @@ __device__ inline void radix_clear(uint32_t (*wcnt)[kDigits])
     __syncthreads();
 }
 
-// The tile's count of every digit into the digit-major table.
+// The tile's count of every digit into the digit-major table (histogram
+// passes count with LDS atomics; the scatter ranks with wavefront matches,
+// which keeps it stable).
 __device__ inline void radix_store_hist(uint32_t (*wcnt)[kDigits], uint32_t *__restrict__ hist,
                                         int64_t ntiles)
 {
@@ -224,12 +226,11 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_keys(int64_t M,
         for (int i = 0; i < kGroup; ++i) {
             const int64_t j = radix_pos(t0, w, i0 + i, lane);
             const bool valid = j < M;
-            uint32_t key = 0;
             if (valid) {
-                key = (uint32_t)bin_key_of<D, T>(x[i], g, cells);
+                const uint32_t key = bin_key_of<D, T>(x[i], g, cells);
                 keys[j] = key;
+                atomicAdd(&wcnt[w][key & (kDigits - 1)], 1u);
             }
-            (void)wave_rank_digit(key & (kDigits - 1), valid, lane, wcnt[w]);
         }
     }
     radix_store_hist(wcnt, hist, ntiles);
@@ -254,8 +255,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(int64_t M,
     }
 #pragma unroll
     for (int i = 0; i < kRadixItems; ++i)
-        (void)wave_rank_digit((k[i] >> shift) & (kDigits - 1), radix_pos(t0, w, i, lane) < M,
-                              lane, wcnt[w]);
+        if (radix_pos(t0, w, i, lane) < M) atomicAdd(&wcnt[w][(k[i] >> shift) & (kDigits - 1)], 1u);
     radix_store_hist(wcnt, hist, ntiles);
 }
 
@@ -789,13 +789,13 @@ __global__ __launch_bounds__(256) void k_prepare(int64_t M, const SortElem *__re
 // Strengths of every vector gathered into record order.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gather_strengths(int64_t M, int64_t vectors,
-                                                          const int4 *__restrict__ rec_start,
+                                                          const uint32_t *__restrict__ perm,
                                                           const T *__restrict__ c,
                                                           T *__restrict__ c_sorted)
 {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
-    int64_t idx = (uint32_t)rec_start[s].w;
+    int64_t idx = perm[s];
     for (int64_t v = 0; v < vectors; ++v) {
         const T *src = c + 2 * (v * M + idx);
         T *dst = c_sorted + 2 * (v * M + s);
@@ -998,12 +998,11 @@ bool BinSorter::prepare(int64_t M, const T *coords, const GridParams &g, BinOrde
 }
 
 template <typename T>
-bool gather_strengths(int64_t M, int64_t vectors, const int4 *rec_start, const T *c, T *c_sorted,
+bool gather_strengths(int64_t M, int64_t vectors, const uint32_t *perm, const T *c, T *c_sorted,
                       hipStream_t stream)
 {
     if (M == 0) return true;
-    k_gather_strengths<T><<<grid_for(M, 256), 256, 0, stream>>>(M, vectors, rec_start, c,
-                                                                c_sorted);
+    k_gather_strengths<T><<<grid_for(M, 256), 256, 0, stream>>>(M, vectors, perm, c, c_sorted);
     return hip_ok(hipGetLastError(), "strength gather launch");
 }
 
@@ -1018,9 +1017,9 @@ MNUFFT_BINS_INSTANTIATE(3, float)
 MNUFFT_BINS_INSTANTIATE(1, double)
 MNUFFT_BINS_INSTANTIATE(2, double)
 MNUFFT_BINS_INSTANTIATE(3, double)
-template bool gather_strengths<float>(int64_t, int64_t, const int4 *, const float *, float *,
-                                      hipStream_t);
-template bool gather_strengths<double>(int64_t, int64_t, const int4 *, const double *, double *,
-                                       hipStream_t);
+template bool gather_strengths<float>(int64_t, int64_t, const uint32_t *, const float *,
+                                      float *, hipStream_t);
+template bool gather_strengths<double>(int64_t, int64_t, const uint32_t *, const double *,
+                                       double *, hipStream_t);
 
 }  // namespace mnufft

@@ -53,8 +53,9 @@ private:
     int64_t nbins_ = 0, M_ = 0;
 };
 
+// c_sorted[v][s] = c[v][perm[s]] for every vector v.
 template <typename T>
-bool gather_strengths(int64_t M, int64_t vectors, const int4 *rec_start, const T *c, T *c_sorted,
+bool gather_strengths(int64_t M, int64_t vectors, const uint32_t *perm, const T *c, T *c_sorted,
                       hipStream_t stream);
 
 }  // namespace mnufft

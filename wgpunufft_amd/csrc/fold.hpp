@@ -72,7 +72,9 @@ __device__ inline void support_start(T x, bool reach, const GridParams &g, int a
         return;
     }
     double pos = fold_position<T>(x, g.ax[axis]);
-    int64_t s = (int64_t)ceil(pos - g.half_width);
+    // pos is in [0, n), so the start fits 32 bits (a direct f64 -> i32
+    // conversion)
+    const int s = (int)ceil(pos - g.half_width);
     *start = s;
     *offset = (double)s - pos;
 }
@@ -96,23 +98,25 @@ __device__ inline bool point_in_reach(const T *coords, int64_t j)
 // Bin key of a point from its coordinates: axis 0 fastest over the wrapped
 // starts shifted by the bin sides.  With `cell`, the start's cell inside its
 // bin follows below the bin key (axis 0 fastest, the bin sides' bits).
+// Keys are below 2^32 (bin counts, and bin count times cells when `cell`),
+// so the arithmetic is 32-bit.
 template <int D, typename T>
-__device__ inline uint64_t bin_key_of(const T *x, const GridParams &g, bool cell = false)
+__device__ inline uint32_t bin_key_of(const T *x, const GridParams &g, bool cell = false)
 {
     bool reach = true;
 #pragma unroll
     for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
-    uint64_t key = 0, sub = 0;
+    uint32_t key = 0, sub = 0;
     int sub_bits = 0;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         int64_t s;
         double off;
         support_start<T>(x[a], reach, g, a, &s, &off);
-        int64_t ws = wrapped_start(s, g.ax[a].n);
-        key += (uint64_t)(ws >> g.ax[a].shift) * (uint64_t)g.ax[a].stride;
+        const uint32_t ws = (uint32_t)wrapped_start(s, g.ax[a].n);
+        key += (ws >> g.ax[a].shift) * (uint32_t)g.ax[a].stride;
         if (cell) {
-            sub |= (uint64_t)(ws & (((int64_t)1 << g.ax[a].shift) - 1)) << sub_bits;
+            sub |= (ws & ((1u << g.ax[a].shift) - 1u)) << sub_bits;
             sub_bits += g.ax[a].shift;
         }
     }
@@ -130,7 +134,7 @@ __device__ __host__ inline int cell_bits(const GridParams &g)
 
 // Bin key of point j of point-major coordinates.
 template <int D, typename T>
-__device__ inline uint64_t point_bin_key(const T *coords, int64_t j, const GridParams &g)
+__device__ inline uint32_t point_bin_key(const T *coords, int64_t j, const GridParams &g)
 {
     T x[D];
 #pragma unroll

@@ -123,7 +123,8 @@ FftPlan::~FftPlan()
 
 Plan::~Plan()
 {
-    for (void *p : {(void *)d_inv_phihat, grid, (void *)rec_start, rec_off, c_sorted, t3_xr,
+    for (void *p : {(void *)d_inv_phihat, grid, (void *)rec_start, rec_off, (void *)rec_perm,
+                    c_sorted, t3_xr,
                     t3_cp, t3_sr, t3_factor, t3_interp, (void *)item_off})
         if (p) (void)hipFree(p);
     for (auto &sl : slots)
@@ -285,11 +286,14 @@ bool Plan::ensure_records(int64_t M)
     if (M <= rec_cap) return true;
     if (rec_start) (void)hipFree(rec_start);
     if (rec_off) (void)hipFree(rec_off);
+    if (rec_perm) (void)hipFree(rec_perm);
     rec_start = nullptr;
     rec_off = nullptr;
+    rec_perm = nullptr;
     int64_t cap = M + M / 4 + 64;
     if (!hip_ok(hipMalloc(&rec_start, sizeof(int4) * cap), "prepared starts")) return false;
     if (!hip_ok(hipMalloc(&rec_off, 4 * real_bytes() * cap), "prepared offsets")) return false;
+    if (!hip_ok(hipMalloc(&rec_perm, sizeof(uint32_t) * cap), "record permutation")) return false;
     rec_cap = cap;
     return true;
 }
@@ -411,7 +415,7 @@ bool Plan::set_points_t(int64_t M, const T *pts, hipStream_t stream)
         if (!ensure_records(M)) return false;
         mark(kStagePrepare, false, stream);
         if (!sorter.prepare<D, T>(M, pts, gp, BinOrder::Canonical, rec_start, (T *)rec_off,
-                                  nullptr, stream))
+                                  rec_perm, stream))
             return false;
         mark(kStagePrepare, true, stream);
     } else {
@@ -442,7 +446,7 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         spreader.kernel_start = timed ? slots[slot].ev[2 * kStageSpreadKernel] : nullptr;
         spreader.kernel_end = timed ? slots[slot].ev[2 * kStageSpreadKernel + 1] : nullptr;
         if (timed) slots[slot].used[kStageSpreadKernel] = true;
-        if (!gather_strengths<T>(M, active, rec_start, in, (T *)c_sorted, stream)) return false;
+        if (!gather_strengths<T>(M, active, rec_perm, in, (T *)c_sorted, stream)) return false;
         if (!spreader.spread<D, T>(gp, spread_params(), sorter.offsets(), rec_start,
                                    (const T *)rec_off,
                                    (const T *)c_sorted, M, active, (T *)grid, stream,

@@ -74,6 +74,7 @@ struct Plan {
     size_t grid_bytes = 0;
     int4 *rec_start = nullptr;
     void *rec_off = nullptr;
+    uint32_t *rec_perm = nullptr;  // original index of every record slot (type 1)
     int64_t rec_cap = 0;
     void *c_sorted = nullptr;
     size_t c_sorted_bytes = 0;
