@@ -170,6 +170,21 @@ int mnufft_timing_history(mnufft_plan plan, int32_t on);
 int mnufft_set_spread_method(mnufft_plan plan, int32_t method);
 
 /*
+ * Order in which a type-1 plan sums the points of one bin.  The default,
+ * MNUFFT_POINT_ORDER_INDEX, is ascending original index, the reference's
+ * order (src/gpu_bin_sort.rs:1-7: consumers restore original point order):
+ * results are bitwise repeatable and a batch equals its vectors one by one.
+ * MNUFFT_POINT_ORDER_COORDINATES (no counterpart in the reference) sums in
+ * ascending coordinate bits, then index, so the output also does not depend
+ * on the order in which the points are given (bitwise), at the cost of an
+ * in-bin sort in set_points.  A change after set_points re-bins the kept
+ * points at the next execution.  Type-2 plans: MNUFFT_ERROR_INVALID_ARGUMENT.
+ */
+#define MNUFFT_POINT_ORDER_INDEX 0
+#define MNUFFT_POINT_ORDER_COORDINATES 1
+int mnufft_set_point_order(mnufft_plan plan, int32_t order);
+
+/*
  * Work-plan knobs of the type-1 spread (no counterpart in the reference):
  * a tile whose reach holds more than light_limit records is split into
  * parts of at least min_part records, with partial sums in slots of at most

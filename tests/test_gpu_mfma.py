@@ -4,8 +4,9 @@ The reference spreads in plain FP32; the matrix-core methods must give the
 same transform to the reference's 3D tolerance (relative l2 error against
 FP64 direct sums <= 32*eps, tests/gpu_nufft_3d.rs:13), stay within their
 stated operand error of the plain FP32 spread, and keep the plain path's
-bitwise repeatability, shuffle invariance and batch == loop properties
-(tests/gpu_nufft_batch.rs:96-133).
+bitwise repeatability and batch == loop properties
+(tests/gpu_nufft_batch.rs:96-133), and its shuffle invariance under the
+coordinate point order.
 """
 import numpy as np
 import pytest
@@ -84,6 +85,7 @@ def test_mfma_2d_bitwise_batch_shuffle_and_rebinning():
     c = S.complex_normal(S.config_seed(1, 941), M * B)
     xt = torch.from_numpy(x).cuda()
     plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
+    plan.set_point_order("coordinates")
     plan.set_points_gpu(xt)
     plain = plan.execute_gpu(torch.from_numpy(c).cuda()).cpu().numpy()
     plan.set_spread_method("mfma_f16x3")
@@ -110,6 +112,7 @@ def test_mfma_bitwise_repeat_and_shuffle_invariance():
     x = _pts(S, "clustered", M, S.config_seed(2, 960))
     c = S.complex_normal(S.config_seed(2, 962), M)
     plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6)).set_spread_method("mfma_f16x3")
+    plan.set_point_order("coordinates")
     runs = [plan.encode_gpu(torch.from_numpy(x).cuda(), torch.from_numpy(c).cuda()).cpu().numpy()
             for _ in range(3)]
     for r in runs[1:]:

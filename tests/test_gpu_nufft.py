@@ -126,23 +126,33 @@ def test_single_mode_closed_form():
 
 
 def test_type1_bitwise_repeat_and_shuffle_invariance():
+    """Index order (default, the reference's): bitwise repeatable.  Coordinate
+    order: also independent of the input order of the points, bit for bit; a
+    switch after set_points re-bins the kept points."""
     torch, W, O, S = _setup()
     n_modes = [24, 20, 16]
     M = 20000
     x = S.clustered_coords(S.config_seed(2, 60), M, (0.3, -1.1, 2.0), 0.05)
     x[: M // 2] = S.uniform_coords(S.config_seed(2, 61), M // 2, 3)
     c = S.complex_normal(S.config_seed(2, 62), M)
+    xt, ct = torch.from_numpy(x).cuda(), torch.from_numpy(c).cuda()
     plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6))
-    runs = []
-    for r in range(3):
-        f = plan.encode_gpu(torch.from_numpy(x).cuda(), torch.from_numpy(c).cuda())
-        runs.append(f.cpu().numpy())
+    runs = [plan.encode_gpu(xt, ct).cpu().numpy() for _ in range(3)]
     for r in runs[1:]:
         assert np.array_equal(r.view(np.uint32), runs[0].view(np.uint32))
+    plan.set_points_gpu(xt)
+    plan.set_point_order("coordinates")
+    canon = plan.execute_gpu(ct).cpu().numpy()
+    assert rel_l2(canon, runs[0]) <= 1e-6
     for s in range(2):
         p = S.shuffle_perm(S.config_seed(2, 70 + s), M)
         f = plan.encode_gpu(torch.from_numpy(x[p].copy()).cuda(), torch.from_numpy(c[p].copy()).cuda())
-        assert np.array_equal(f.cpu().numpy().view(np.uint32), runs[0].view(np.uint32))
+        assert np.array_equal(f.cpu().numpy().view(np.uint32), canon.view(np.uint32))
+    plan.set_point_order("index")
+    again = plan.encode_gpu(xt, ct).cpu().numpy()
+    assert np.array_equal(again.view(np.uint32), runs[0].view(np.uint32))
+    with pytest.raises(Exception):
+        W.NufftPlan.type2_gpu(W.NufftConfig(n_modes, 1e-6)).set_point_order("coordinates")
 
 
 def test_set_points_then_execute_equals_encode():
@@ -359,7 +369,7 @@ def test_type1_dense_cluster_shuffle_invariance():
     M = 200000
     x = S.clustered_coords(S.config_seed(2, 710), M, (0.3, -1.1, 2.0), 0.002)
     c = S.complex_normal(S.config_seed(2, 711), M)
-    plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6))
+    plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6)).set_point_order("coordinates")
     f0 = plan.encode_gpu(torch.from_numpy(x).cuda(), torch.from_numpy(c).cuda()).cpu().numpy()
     p = S.shuffle_perm(S.config_seed(2, 712), M)
     f1 = plan.encode_gpu(torch.from_numpy(x[p].copy()).cuda(),

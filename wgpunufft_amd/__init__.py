@@ -89,6 +89,7 @@ class NufftConfig:
 
 
 SPREAD_METHODS = {"plain": 0, "mfma_f16": 1, "mfma_f16x3": 2, "mfma_bf16x3": 3}
+POINT_ORDERS = {"index": 0, "coordinates": 1}
 
 
 class _PlanBase:
@@ -126,6 +127,15 @@ class _PlanBase:
         hi/lo splits, ~1e-5); FP32 3D plans only."""
         code = SPREAD_METHODS[method] if isinstance(method, str) else int(method)
         check(lib().mnufft_set_spread_method(self._handle, code))
+        return self
+
+    def set_point_order(self, order):
+        """Type-1 summation order inside a bin: "index" (default, the
+        reference's ascending original index) or "coordinates" (coordinate
+        bits, then index: the output then does not depend on the input order
+        of the points, bit for bit)."""
+        code = POINT_ORDERS[order] if isinstance(order, str) else int(order)
+        check(lib().mnufft_set_point_order(self._handle, code))
         return self
 
     def set_spread_tuning(self, light_limit=0, min_part=0, part_budget_bytes=0):

@@ -709,7 +709,9 @@ __device__ inline T elem_coord(const SortElem &e, const T *coords, int a, bool f
     return coords[(int64_t)e.idx * D + a];
 }
 
-// Grouped order: slot s keeps the point the scatter put there.
+// Grouped and index orders: slot s keeps the point the radix sort put there.
+// The coordinates are gathered once (all loads in flight before any store)
+// and each record is written with two 16-byte stores.
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_prepare_grouped(int64_t M,
                                                          const uint32_t *__restrict__ sorted,
@@ -721,17 +723,29 @@ __global__ __launch_bounds__(256) void k_prepare_grouped(int64_t M,
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     const uint32_t idx = sorted[s];
-    const bool reach = point_in_reach<D, T>(coords, idx);
+    T x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = coords[(int64_t)idx * D + a];
+    bool reach = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
     int st[3] = {0, 0, 0};
+    T off[4] = {(T)0, (T)0, (T)0, (T)0};
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         int64_t start;
-        double off;
-        support_start<T>(coords[(int64_t)idx * D + a], reach, g, a, &start, &off);
+        double o;
+        support_start<T>(x[a], reach, g, a, &start, &o);
         st[a] = (int)wrapped_start(start, g.ax[a].n);
-        rec_off[s * 4 + a] = (T)off;
+        off[a] = (T)o;
     }
     rec_start[s] = make_int4(st[0], st[1], st[2], (int)idx);
+    if constexpr (sizeof(T) == 4) {
+        *(float4 *)(rec_off + s * 4) = make_float4(off[0], off[1], off[2], off[3]);
+    } else {
+        *(double2 *)(rec_off + s * 4) = make_double2(off[0], off[1]);
+        *(double2 *)(rec_off + s * 4 + 2) = make_double2(off[2], off[3]);
+    }
     if (out_sorted) out_sorted[s] = idx;
 }
 

@@ -363,6 +363,27 @@ int mnufft_set_spread_method(mnufft_plan plan, int32_t method)
     return MNUFFT_SUCCESS;
 }
 
+int mnufft_set_point_order(mnufft_plan plan, int32_t order)
+{
+    if (!plan) {
+        set_error("null plan");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    Plan &p = *plan->plan;
+    Plan &q = p.type == 3 && p.outer ? *p.outer : p;
+    if (order != MNUFFT_POINT_ORDER_INDEX && order != MNUFFT_POINT_ORDER_COORDINATES) {
+        set_error("unknown point order");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (q.type != 1) {
+        set_error("the point order applies to type-1 and type-3 plans");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    q.point_order = order == MNUFFT_POINT_ORDER_INDEX ? mnufft::BinOrder::Index
+                                                      : mnufft::BinOrder::Canonical;
+    return MNUFFT_SUCCESS;
+}
+
 int mnufft_set_spread_tuning(mnufft_plan plan, int64_t light_limit, int64_t min_part,
                              int64_t part_budget_bytes)
 {

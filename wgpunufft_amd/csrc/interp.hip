@@ -248,14 +248,18 @@ __global__ __launch_bounds__(THREADS) void k_interp_tiled3(
         if (last > (int64_t)offsets[ti.b + 1]) last = offsets[ti.b + 1];
         for (int64_t s = first + threadIdx.x; s < last; s += THREADS) {
             const uint32_t idx = sorted[s];
-            const bool reach = point_in_reach<3, T>(coords, idx);
+            T x[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) x[a] = coords[(int64_t)idx * 3 + a];
+            const bool reach = FoldTraits<T>::in_reach(x[0]) && FoldTraits<T>::in_reach(x[1]) &&
+                               FoldTraits<T>::in_reach(x[2]);
             int l[3];
             T wt[3][W];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 int64_t start;
                 double off;
-                support_start<T>(coords[(int64_t)idx * 3 + a], reach, g, a, &start, &off);
+                support_start<T>(x[a], reach, g, a, &start, &off);
                 int64_t ws = wrapped_start(start, g.ax[a].n);
                 int64_t o = a == 0 ? ti.x0 : (a == 1 ? ti.y0 : ti.z0);
                 l[a] = (int)(ws - o);

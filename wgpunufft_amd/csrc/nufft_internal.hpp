@@ -35,7 +35,7 @@ enum class SpreadMethod : int { Plain = 0, MfmaF16 = 1, MfmaF16x3 = 2, MfmaBf16x
 enum class BinOrder : int {
     Grouped = 0,    // by support-start cell inside each bin (type 2)
     Index = 1,      // ascending original index: the reference permutation
-    Canonical = 2,  // ascending coordinate bits, then index (type-1 spread)
+    Canonical = 2,  // ascending coordinate bits, then index (type 1, opt-in)
 };
 
 // ---------------------------------------------------------------------

@@ -409,13 +409,14 @@ bool Plan::set_points_t(int64_t M, const T *pts, hipStream_t stream)
     if (type == 1) {
         gp = spread_geometry(M, &geometry_index);
         gp_rows = rows_geometry();
+        binned_order = point_order;
         mark(kStageSort, false, stream);
-        if (!sorter.sort<D, T>(M, pts, gp, BinOrder::Canonical, stream)) return false;
+        if (!sorter.sort<D, T>(M, pts, gp, point_order, stream)) return false;
         mark(kStageSort, true, stream);
         if (!ensure_records(M)) return false;
         mark(kStagePrepare, false, stream);
-        if (!sorter.prepare<D, T>(M, pts, gp, BinOrder::Canonical, rec_start, (T *)rec_off,
-                                  rec_perm, stream))
+        if (!sorter.prepare<D, T>(M, pts, gp, point_order, rec_start, (T *)rec_off, rec_perm,
+                                  stream))
             return false;
         mark(kStagePrepare, true, stream);
     } else {
@@ -437,8 +438,10 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
     ModeParams mp = mode_params();
     if (type == 1) {
         if (M == 0) return fill_zero<T>(out, 2 * mode_total * active, stream);
-        // a spread method that needs other bins re-bins the kept points
-        if (gp_rows != rows_geometry() && !set_points_t<D, T>(M, (const T *)points, stream))
+        // a spread method that needs other bins, or another in-bin order,
+        // re-bins the kept points
+        if ((gp_rows != rows_geometry() || binned_order != point_order) &&
+            !set_points_t<D, T>(M, (const T *)points, stream))
             return false;
         if (!ensure_strengths(M, active)) return false;
         mark(kStageSpread, false, stream);

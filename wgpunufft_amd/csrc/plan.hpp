@@ -48,6 +48,10 @@ struct Plan {
     bool tiles2 = false;  // 2D grid qualifies for the matrix-core 32 x 32 tiles
     SpreadParams tile3_sp{};  // 3D matrix-core tiles (16 x 16 x 8; row-sweep grids)
     SpreadMethod spread_method = SpreadMethod::Plain;
+    // Type-1 order of the points inside a bin: ascending original index (the
+    // reference's), or coordinate bits then index (shuffle invariant).
+    BinOrder point_order = BinOrder::Index;
+    BinOrder binned_order = BinOrder::Index;  // order of the current binning
     bool gp_rows = false;  // points binned in one-cell rows (row tiles)
     // Type-1 spreads on row tiles: the 3D row sweep, or a 2D matrix-core method.
     bool rows_geometry() const
