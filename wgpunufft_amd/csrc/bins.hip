@@ -442,19 +442,21 @@ __device__ inline SortElem make_elem(const T *coords, uint32_t idx, int order)
 {
     SortElem e{0u, 0u, 0u, idx};
     if (order != (int)BinOrder::Canonical) return e;
+    T x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = coords[(int64_t)idx * D + a];
     if constexpr (sizeof(T) == 4) {
-        e.k0 = __float_as_uint((float)coords[(int64_t)idx * D + 0]);
-        if (D > 1) e.k1 = __float_as_uint((float)coords[(int64_t)idx * D + (D > 1 ? 1 : 0)]);
-        if (D > 2) e.k2 = __float_as_uint((float)coords[(int64_t)idx * D + (D > 2 ? 2 : 0)]);
+        e.k0 = __float_as_uint((float)x[0]);
+        if constexpr (D > 1) e.k1 = __float_as_uint((float)x[1]);
+        if constexpr (D > 2) e.k2 = __float_as_uint((float)x[2]);
     } else {
-        uint64_t b0 = (uint64_t)__double_as_longlong((double)coords[(int64_t)idx * D + 0]);
+        uint64_t b0 = (uint64_t)__double_as_longlong((double)x[0]);
         e.k0 = (uint32_t)(b0 >> 32);
         e.k1 = (uint32_t)b0;
         uint64_t rest = 0;
-        if (D > 1) rest = (uint64_t)__double_as_longlong((double)coords[(int64_t)idx * D + (D > 1 ? 1 : 0)]);
-        if (D > 2)
-            rest = rest * 0x9e3779b97f4a7c15ull ^
-                   (uint64_t)__double_as_longlong((double)coords[(int64_t)idx * D + (D > 2 ? 2 : 0)]);
+        if constexpr (D > 1) rest = (uint64_t)__double_as_longlong((double)x[1]);
+        if constexpr (D > 2)
+            rest = rest * 0x9e3779b97f4a7c15ull ^ (uint64_t)__double_as_longlong((double)x[2]);
         e.k2 = D > 1 ? mix32(rest) : 0u;
     }
     return e;

@@ -58,30 +58,35 @@ __global__ __launch_bounds__(256) void k_interp(int64_t M, int64_t vectors,
     const int w = g.width;
     const T beta = (T)g.beta;
     uint32_t idx = has ? sorted[s] : 0u;
-    bool reach = has && point_in_reach<D, T>(coords, idx);
+    T x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = has ? coords[(int64_t)idx * D + a] : (T)0;
+    bool reach = has;
+#pragma unroll
+    for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
     int64_t st[3] = {0, 0, 0};
     T wt[3] = {(T)0, (T)0, (T)0};
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         int64_t start = 0;
         double off = kOutsideOffset;
-        if (has) support_start<T>(coords[(int64_t)idx * D + a], reach, g, a, &start, &off);
+        if (has) support_start<T>(x[a], reach, g, a, &start, &off);
         st[a] = wrapped_start(start, g.ax[a].n);
         wt[a] = j < w ? es_w<T>((T)j + (T)off, w, beta) : (T)0;
     }
     T are = (T)0, aim = (T)0;
     const T *gv = grid + 2 * v * g.fine_total;
     if (has && j < w) {
-        int64_t x = wrap_idx(st[0] + j, g.ax[0].n);
+        int64_t xc = wrap_idx(st[0] + j, g.ax[0].n);
         if (D == 1) {
-            are = gv[2 * x];
-            aim = gv[2 * x + 1];
+            are = gv[2 * xc];
+            aim = gv[2 * xc + 1];
         } else if (D == 2) {
             int64_t n0 = g.ax[0].n;
             for (int dy = 0; dy < w; ++dy) {
                 T wy = __shfl(wt[1], dy, G);
                 int64_t y = wrap_idx(st[1] + dy, g.ax[1].n);
-                const T *p = gv + 2 * (x + n0 * y);
+                const T *p = gv + 2 * (xc + n0 * y);
                 are += wy * p[0];
                 aim += wy * p[1];
             }
@@ -98,7 +103,7 @@ __global__ __launch_bounds__(256) void k_interp(int64_t M, int64_t vectors,
                 int64_t z = n01 * wrap_idx(st[2] + dz, g.ax[2].n);
                 T pre = (T)0, pim = (T)0;
                 for (int dy = 0; dy < w; ++dy) {
-                    const T *p = gv + 2 * (x + ys[dy] + z);
+                    const T *p = gv + 2 * (xc + ys[dy] + z);
                     pre += wys[dy] * p[0];
                     pim += wys[dy] * p[1];
                 }

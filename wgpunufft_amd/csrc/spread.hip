@@ -998,11 +998,8 @@ bool chunk_mfma(Spreader &s, SpreadMethod method, const GridParams &g, const Spr
     if (D == 2)
         k_reduce_tiles<float, TE, TE, 1><<<rg, 256, 0, stream>>>(
             g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const float *)s.partial, grid);
-    else if (sp.block[2] == 8)
-        k_reduce_tiles<float, TE, TE, 8><<<rg, 256, 0, stream>>>(
-            g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const float *)s.partial, grid);
     else
-        k_reduce_tiles<float, TE, TE, TE><<<rg, 256, 0, stream>>>(
+        k_reduce_tiles<float, TE, TE, 8><<<rg, 256, 0, stream>>>(
             g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const float *)s.partial, grid);
     return hip_ok(hipGetLastError(), "matrix-core spread launch");
 }

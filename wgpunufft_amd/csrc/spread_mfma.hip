@@ -321,7 +321,7 @@ __global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ c, int
 }
 
 template <int W, int MODE, int TZ>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TZ == 16 ? 2 : 3))) void
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void
 k_spread_mfma3(
     GridParams g, SpreadParams sp, const uint32_t *__restrict__ offsets,
     const int4 *__restrict__ rec_start, const float *__restrict__ rec_off,
@@ -647,16 +647,11 @@ void launch_kernel(dim3 grid3, const GridParams &g, const SpreadParams &sp,
                    const float *c_sorted, int64_t M, const TileWork &w, float *grid,
                    float *partial, hipStream_t stream)
 {
-    if constexpr (D == 3) {
-        if (sp.block[2] == 8)
-            k_spread_mfma3<W, MODE, 8><<<grid3, 64, 0, stream>>>(
-                g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
-                w.heavy_base, w.ctrl, w.cmax, grid, partial);
-        else
-            k_spread_mfma3<W, MODE, 16><<<grid3, 64, 0, stream>>>(
-                g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
-                w.heavy_base, w.ctrl, w.cmax, grid, partial);
-    } else
+    if constexpr (D == 3)
+        k_spread_mfma3<W, MODE, 8><<<grid3, 64, 0, stream>>>(
+            g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
+            w.heavy_base, w.ctrl, w.cmax, grid, partial);
+    else
         k_spread_mfma2<W, MODE><<<grid3, 64, 0, stream>>>(
             g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
             w.heavy_base, w.ctrl, w.cmax, grid, partial);
