@@ -11,8 +11,10 @@ in HBM before timing starts.
 
 value = M / (seconds per step): points per second through both transforms,
 with the type-1 spread on the matrix cores (FP16 hi/lo split products,
-FP32 accumulation); the same step with the plain FP32 spreader is timed
-beside it ("plain").  The roofline object reports the dominant kernel of the
+FP32 accumulation) and bins summed in coordinate order, so the type-1 output
+is bitwise identical for any ordering of the points (checked on one shuffle,
+"bitwise"); the same step with the plain FP32 spreader ("plain") and with
+the reference's index order ("index_order") is timed beside it.  The roofline object reports the dominant kernel of the
 path, from HIP events
 around that kernel over the timed region; cpu_baseline times the CPU port of
 the reference's CpuNufftPlan on a bounded sample of the same workload.
@@ -100,8 +102,11 @@ def main():
     f_plain = torch.empty(N ** 3, dtype=torch.complex64, device=dev)
     c_out = torch.empty(M, dtype=torch.complex64, device=dev)
     cfg = W.NufftConfig(n_modes, args.eps)
-    p1 = W.NufftPlan.type1_gpu(cfg)
-    p1m = W.NufftPlan.type1_gpu(cfg).set_spread_method(args.spread)
+    # the workload asks for bitwise-identical type-1 output across point
+    # orderings: bins are summed in coordinate order
+    p1 = W.NufftPlan.type1_gpu(cfg).set_point_order("coordinates")
+    p1m = W.NufftPlan.type1_gpu(cfg).set_spread_method(args.spread).set_point_order("coordinates")
+    p1i = W.NufftPlan.type1_gpu(cfg).set_spread_method(args.spread)
     p2 = W.NufftPlan.type2_gpu(cfg)
     print(f"# data ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
@@ -133,8 +138,20 @@ def main():
 
     ms_plain, stages_plain = leg(p1, f_plain)
     print(f"# plain leg {ms_plain:.3f} ms/step", file=sys.stderr, flush=True)
+    f_index = torch.empty_like(f_out)
+    ms_index, stages_index = leg(p1i, f_index)
+    print(f"# {args.spread} index-order leg {ms_index:.3f} ms/step", file=sys.stderr, flush=True)
     ms_mfma, stages_mfma = leg(p1m, f_out)
     print(f"# {args.spread} leg {ms_mfma:.3f} ms/step", file=sys.stderr, flush=True)
+    # bitwise repeat and point-order independence of the timed type 1
+    f_rep = p1m.encode_gpu(xt, ct)
+    perm = torch.from_numpy(S.shuffle_perm(S.config_seed(2, 9), M)).to(dev)
+    f_shuf = p1m.encode_gpu(xt[perm].contiguous(), ct[perm].contiguous())
+    torch.cuda.synchronize()
+    bits = lambda t: t.view(torch.float32).view(torch.int32)
+    bitwise = {"repeat": bool(torch.equal(bits(f_rep), bits(f_out))),
+               "shuffled_points": bool(torch.equal(bits(f_shuf), bits(f_out)))}
+    del f_rep, f_shuf, perm
 
     info1 = p1.info()
     w = info1["width"]
@@ -198,7 +215,9 @@ def main():
                    "n_modes": n_modes, "fine": info1["fine"], "width": w,
                    "points": M, "bin_geometry_t1": info1["bin_side"],
                    "spreader": f"{args.spread} matrix-core tiles (type 1)",
+                   "point_order": "coordinates (type-1 output independent of point order)",
                    "interp": "LDS-tiled" if tiled else "per-point gather"},
+        "bitwise": bitwise,
         "roofline": dominant,
         "roofline_other": [other, roof_plain],
         "stages_ms": stages_mfma,
@@ -208,6 +227,11 @@ def main():
                   "spreader": "plain FP32 row sweep" if sweep else "plain FP32 output-stationary",
                   "stages_ms": stages_plain,
                   "type1_points_per_s": pts(stages_plain, "type1")},
+        "index_order": {"value": M / (ms_index * 1e-3), "ms_per_step": round(ms_index, 4),
+                        "note": "same step with bins summed in ascending point index (the "
+                                "reference's order; bitwise repeatable, not point-order "
+                                "independent)",
+                        "stages_ms": stages_index},
         "mfma_vs_plain": {"spread_kernel": round(t_plain / t_mfma, 3),
                           "type1": round(pts(stages_mfma, "type1") / pts(stages_plain, "type1"), 3),
                           "step": round(ms_plain / ms_mfma, 3)},
@@ -239,7 +263,7 @@ def profile_step(args, W, S, np, torch):
     c = torch.from_numpy(S.complex_normal(S.config_seed(2, 3), M)).to(dev)
     fm = torch.from_numpy(S.complex_normal(S.config_seed(2, 4), N ** 3)).to(dev)
     cfg = W.NufftConfig([N] * 3, args.eps)
-    p1 = W.NufftPlan.type1_gpu(cfg).set_spread_method(args.spread)
+    p1 = W.NufftPlan.type1_gpu(cfg).set_spread_method(args.spread).set_point_order("coordinates")
     p2 = W.NufftPlan.type2_gpu(cfg)
     for _ in range(2):
         p1.encode_gpu(x, c)

@@ -137,6 +137,9 @@ constexpr int kRadixWaves = kRadixThreads / 64;
 constexpr int kRadixItems = 16;
 constexpr int kRadixTile = kRadixThreads * kRadixItems;  // 4096 keys
 constexpr int kDigits = 256;
+// Sort keys below the bin key: none, the in-bin start cell, the quantized
+// in-bin position.
+constexpr int kKeyBin = 0, kKeyCell = 1, kKeyPosition = 2;
 
 // Lanes of the wavefront whose digit equals this lane's (valid lanes only).
 __device__ inline uint64_t digit_peers(uint32_t d, bool valid)
@@ -197,14 +200,50 @@ __device__ inline void radix_store_hist(uint32_t (*wcnt)[kDigits], uint32_t *__r
     hist[(int64_t)d * ntiles + blockIdx.x] = s;
 }
 
-// Pass 0 histogram: sort key of every point (written out) and its low
-// digit.  The sort key is the bin key, followed by the in-bin cell with
-// `cells`.
+// Sort key of a point below its bin key (sub_bits per axis, axis 0 lowest):
+// grouped order, the support-start cell inside the bin (sub_bits = the bin
+// side's bits); canonical order, the position inside the bin quantized to
+// 2^sub_bits steps per axis, a function of the coordinates that splits dense
+// bins into small groups before the coordinate-bit order is applied.
 template <int D, typename T>
+__device__ inline uint64_t sort_key_of(const T *x, const GridParams &g, int mode, int sub_bits)
+{
+    bool reach = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
+    uint32_t key = 0;
+    uint64_t sub = 0;
+    int at = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        int64_t s;
+        double off;
+        support_start<T>(x[a], reach, g, a, &s, &off);
+        const uint32_t ws = (uint32_t)wrapped_start(s, g.ax[a].n);
+        key += (ws >> g.ax[a].shift) * (uint32_t)g.ax[a].stride;
+        if (mode == kKeyCell) {
+            sub |= (uint64_t)(ws & ((1u << g.ax[a].shift) - 1u)) << at;
+        } else if (mode == kKeyPosition) {
+            // position in the bin in units of the bin side, in (0, 1]
+            const double side = (double)(1u << g.ax[a].shift);
+            const double frac = 1.0 - off - g.half_width;  // position - (start - 1) - w/2
+            const double u = ((double)(ws & ((1u << g.ax[a].shift) - 1u)) + frac) / side;
+            double q = floor(u * (double)(1u << sub_bits));
+            q = q < 0.0 ? 0.0 : (q > (double)((1u << sub_bits) - 1u) ? (double)((1u << sub_bits) - 1u) : q);
+            sub |= (uint64_t)q << at;
+        }
+        at += mode == kKeyCell ? g.ax[a].shift : sub_bits;
+    }
+    return mode == kKeyBin ? (uint64_t)key : ((uint64_t)key << at) | sub;
+}
+
+// Pass 0 histogram: sort key of every point (written out) and its low
+// digit.
+template <int D, typename T, typename K>
 __global__ __launch_bounds__(kRadixThreads) void k_radix_keys(int64_t M,
                                                               const T *__restrict__ coords,
-                                                              GridParams g, bool cells,
-                                                              uint32_t *__restrict__ keys,
+                                                              GridParams g, int mode,
+                                                              int sub_bits, K *__restrict__ keys,
                                                               uint32_t *__restrict__ hist,
                                                               int64_t ntiles)
 {
@@ -227,18 +266,26 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_keys(int64_t M,
             const int64_t j = radix_pos(t0, w, i0 + i, lane);
             const bool valid = j < M;
             if (valid) {
-                const uint32_t key = bin_key_of<D, T>(x[i], g, cells);
+                const K key = (K)sort_key_of<D, T>(x[i], g, mode, sub_bits);
                 keys[j] = key;
-                atomicAdd(&wcnt[w][key & (kDigits - 1)], 1u);
+                atomicAdd(&wcnt[w][(uint32_t)key & (kDigits - 1)], 1u);
             }
         }
     }
     radix_store_hist(wcnt, hist, ntiles);
 }
 
+// The 32-bit word of a key that holds the digit at `shift`, and the digit.
+template <typename K>
+__device__ inline uint32_t key_digit(K k, int shift)
+{
+    return (uint32_t)(k >> shift) & (kDigits - 1);
+}
+
 // Histogram of the digit at `shift` for passes after the first.
+template <typename K>
 __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(int64_t M,
-                                                              const uint32_t *__restrict__ keys,
+                                                              const K *__restrict__ keys,
                                                               int shift,
                                                               uint32_t *__restrict__ hist,
                                                               int64_t ntiles)
@@ -247,44 +294,51 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(int64_t M,
     radix_clear(wcnt);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
+    // only the 32-bit word that holds the digit is read
+    const uint32_t *kw = (const uint32_t *)keys;
+    constexpr int kWords = sizeof(K) / 4;
+    const int word = shift >> 5, wshift = shift & 31;
     uint32_t k[kRadixItems];
 #pragma unroll
     for (int i = 0; i < kRadixItems; ++i) {
         const int64_t j = radix_pos(t0, w, i, lane);
-        k[i] = j < M ? keys[j] : 0u;
+        k[i] = j < M ? kw[j * kWords + word] : 0u;
     }
 #pragma unroll
     for (int i = 0; i < kRadixItems; ++i)
-        if (radix_pos(t0, w, i, lane) < M) atomicAdd(&wcnt[w][(k[i] >> shift) & (kDigits - 1)], 1u);
+        if (radix_pos(t0, w, i, lane) < M) atomicAdd(&wcnt[w][(k[i] >> wshift) & (kDigits - 1)], 1u);
     radix_store_hist(wcnt, hist, ntiles);
 }
 
 // Scatter of one pass: hist holds the exclusive scan of the digit-major
 // table, i.e. where each (digit, tile) run starts.  vals_in == nullptr means
 // the value of point j is j.
+template <typename K>
 __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
-    int64_t M, const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
-    int shift, const uint32_t *__restrict__ hist, int64_t ntiles, uint32_t *__restrict__ keys_out,
+    int64_t M, const K *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, int shift,
+    const uint32_t *__restrict__ hist, int64_t ntiles, K *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out)
 {
     __shared__ uint32_t wcnt[kRadixWaves][kDigits];
     __shared__ uint32_t dstart[kDigits], gbase[kDigits];
-    __shared__ uint32_t skey[kRadixTile], sval[kRadixTile];
+    __shared__ K skey[kRadixTile];
+    __shared__ uint32_t sval[kRadixTile];
     radix_clear(wcnt);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
-    uint32_t k[kRadixItems], v[kRadixItems], r[kRadixItems];
+    K k[kRadixItems];
+    uint32_t v[kRadixItems], r[kRadixItems];
 #pragma unroll
     for (int i = 0; i < kRadixItems; ++i) {
         const int64_t j = radix_pos(t0, w, i, lane);
         const bool valid = j < M;
-        k[i] = valid ? keys_in[j] : 0u;
+        k[i] = valid ? keys_in[j] : (K)0;
         v[i] = valid ? (vals_in ? vals_in[j] : (uint32_t)j) : 0u;
     }
 #pragma unroll
     for (int i = 0; i < kRadixItems; ++i)
-        r[i] = wave_rank_digit((k[i] >> shift) & (kDigits - 1), radix_pos(t0, w, i, lane) < M,
-                               lane, wcnt[w]);
+        r[i] = wave_rank_digit(key_digit(k[i], shift), radix_pos(t0, w, i, lane) < M, lane,
+                               wcnt[w]);
     __syncthreads();
     {
         // per digit: earlier wavefronts' counts, the tile's digit start and
@@ -305,7 +359,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
 #pragma unroll
     for (int i = 0; i < kRadixItems; ++i) {
         if (radix_pos(t0, w, i, lane) >= M) continue;
-        const uint32_t d = (k[i] >> shift) & (kDigits - 1);
+        const uint32_t d = key_digit(k[i], shift);
         const uint32_t p = dstart[d] + wcnt[w][d] + r[i];
         skey[p] = k[i];
         sval[p] = v[i];
@@ -316,8 +370,8 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
     for (int i = 0; i < kRadixItems; ++i) {
         const int s = i * kRadixThreads + threadIdx.x;
         if (s >= n_here) continue;
-        const uint32_t key = skey[s];
-        const uint32_t d = (key >> shift) & (kDigits - 1);
+        const K key = skey[s];
+        const uint32_t d = key_digit(key, shift);
         const int64_t o = (int64_t)gbase[d] + (s - (int64_t)dstart[d]);
         keys_out[o] = key;
         vals_out[o] = sval[s];
@@ -327,40 +381,51 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
 // Bin offsets from the sorted keys: position j (0 <= j <= M, j = M standing
 // for a key of nbins) is the offset of every bin in (bin[j-1], bin[j]] (bin =
 // sort key >> cell_shift), which gives each of the nbins + 1 offsets exactly
-// one writer.  Gaps of up to
-// kGapInline bins are written here; longer ones are listed (kGapMid: one
-// workgroup each, longer: split over every workgroup) for k_fill_gaps.
-// With a large-bin list, bins of more than kSmallBin points are listed once
-// (at their last position).
+// one writer.  Gaps of up to kGapInline bins are written here; longer ones
+// are listed (kGapMid: one workgroup each, longer: split over every
+// workgroup) for k_fill_gaps.  With a segment list, every group of more than
+// kSmallBin equal sort keys is listed once, by its first position, as
+// [first, end) (end found by a binary search over the sorted keys).
 constexpr int64_t kGapInline = 64;
 constexpr int64_t kGapMid = 4096;
 
-__global__ __launch_bounds__(256) void k_bin_offsets(int64_t M, const uint32_t *__restrict__ keys,
+template <typename K>
+__global__ __launch_bounds__(256) void k_bin_offsets(int64_t M, const K *__restrict__ keys,
                                                      int cell_shift, int64_t nbins,
                                                      uint32_t *__restrict__ offsets,
                                                      uint4 *__restrict__ gaps_mid,
                                                      uint4 *__restrict__ gaps_long,
                                                      uint32_t *__restrict__ ctrl,
-                                                     uint32_t *__restrict__ large_list)
+                                                     uint2 *__restrict__ segs)
 {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
     int64_t lo = 0, hi = -1;  // bins [lo, hi] take offset j
     bool large = false;
-    uint32_t k = 0;
+    uint32_t seg_end = 0;
     if (j <= M) {
         const int64_t kp = j > 0 ? (int64_t)(keys[j - 1] >> cell_shift) : -1;
         const int64_t kj = j < M ? (int64_t)(keys[j] >> cell_shift) : nbins;
-        k = (uint32_t)kj;
         if (kj != kp) {
             lo = kp + 1;
             hi = kj;
         }
-        // (large bins are listed for orders without in-bin cells only)
-        if (large_list && j < M && (j == M - 1 || keys[j + 1] != k) &&
-            j >= (int64_t)kSmallBin && keys[j - kSmallBin] == k)
-            large = true;
+        if (segs && j + (int64_t)kSmallBin < M) {
+            const K k = keys[j];
+            if ((j == 0 || keys[j - 1] != k) && keys[j + kSmallBin] == k) {
+                large = true;
+                int64_t a = j + kSmallBin, b = M;  // first key above k in (a, b]
+                while (b - a > 1) {
+                    const int64_t mid = (a + b) >> 1;
+                    if (keys[mid] == k)
+                        a = mid;
+                    else
+                        b = mid;
+                }
+                seg_end = (uint32_t)b;
+            }
+        }
     }
     const int64_t len = hi - lo + 1;
     if (len > 0 && len <= kGapInline)
@@ -381,7 +446,7 @@ __global__ __launch_bounds__(256) void k_bin_offsets(int64_t M, const uint32_t *
     const uint4 gap = make_uint4((uint32_t)lo, (uint32_t)hi, (uint32_t)j, 0u);
     if (mid) gaps_mid[bm + __popcll(mm & below)] = gap;
     if (lng) gaps_long[bl + __popcll(ml & below)] = gap;
-    if (large) large_list[bg + __popcll(mg & below)] = k;
+    if (large) segs[bg + __popcll(mg & below)] = make_uint2((uint32_t)j, seg_end);
 }
 
 // Listed gaps: a mid gap per workgroup (grid-stride); every long gap cut in
@@ -437,20 +502,17 @@ __device__ inline uint32_t mix32(uint64_t v)
     return (uint32_t)v;
 }
 
+// Ordering element of a point from its coordinates.
 template <int D, typename T>
-__device__ inline SortElem make_elem(const T *coords, uint32_t idx, int order)
+__device__ inline SortElem elem_of(const T *x, uint32_t idx)
 {
     SortElem e{0u, 0u, 0u, idx};
-    if (order != (int)BinOrder::Canonical) return e;
-    T x[D];
-#pragma unroll
-    for (int a = 0; a < D; ++a) x[a] = coords[(int64_t)idx * D + a];
     if constexpr (sizeof(T) == 4) {
         e.k0 = __float_as_uint((float)x[0]);
         if constexpr (D > 1) e.k1 = __float_as_uint((float)x[1]);
         if constexpr (D > 2) e.k2 = __float_as_uint((float)x[2]);
     } else {
-        uint64_t b0 = (uint64_t)__double_as_longlong((double)x[0]);
+        const uint64_t b0 = (uint64_t)__double_as_longlong((double)x[0]);
         e.k0 = (uint32_t)(b0 >> 32);
         e.k1 = (uint32_t)b0;
         uint64_t rest = 0;
@@ -462,28 +524,27 @@ __device__ inline SortElem make_elem(const T *coords, uint32_t idx, int order)
     return e;
 }
 
-// Canonical order: the ordering element of the point at every sorted slot
-// (its coordinate bits and index), gathered from the radix-sorted indices.
+// Ordering element of point idx of point-major coordinates.
 template <int D, typename T>
-__global__ __launch_bounds__(256) void k_make_elems(int64_t M, const uint32_t *__restrict__ sorted,
-                                                    const T *__restrict__ coords,
-                                                    SortElem *__restrict__ elems)
+__device__ inline SortElem elem_at(const T *coords, uint32_t idx)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    elems[s] = make_elem<D, T>(coords, sorted[s], (int)BinOrder::Canonical);
+    T x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = coords[(int64_t)idx * D + a];
+    return elem_of<D, T>(x, idx);
 }
 
 // ---------------------------------------------------------------------
-// Large bins (more than kSmallBin points): a merge sort spread over every
-// workgroup.  The listed bins are cut into tiles of kSortRun elements;
-//  1. k_large_tiles scans the tile counts of the listed bins;
+// Large groups (more than kSmallBin points of one sort key): a merge sort
+// spread over every workgroup.  The listed segments are cut into tiles of
+// kSortRun elements;
+//  1. k_large_tiles scans the tile counts of the listed segments;
 //  2. k_large_runs bitonic-sorts every tile of elements in LDS, in place;
 //  3. k_large_merge, one launch per round of run length kSortRun << r,
-//     gives every output tile of a bin still being merged its two input
+//     gives every output tile of a segment still being merged its two input
 //     ranges by a merge-path search, stages them in LDS and merges there;
-//  4. k_large_final copies bins whose last round wrote the second buffer
-//     back, so the elements of every bin end up sorted in the first one.
+//  4. k_large_final copies segments whose last round wrote the second
+//     buffer back, so every segment ends up sorted in the first one.
 // Element keys are unique (the index breaks ties), so every merge is exact.
 // ---------------------------------------------------------------------
 // Bitonic sort of s[0..n2) (n2 a power of two): every thread takes one
@@ -533,10 +594,10 @@ __device__ inline int merge_rounds(int64_t n)
     return r;
 }
 
-// tile_off[i] = first tile of listed bin i; tile_off[nlarge] = all tiles.
-__global__ __launch_bounds__(1024) void k_large_tiles(const uint32_t *__restrict__ large_list,
+// tile_off[i] = first tile of listed segment i; tile_off[nlarge] = all
+// tiles.
+__global__ __launch_bounds__(1024) void k_large_tiles(const uint2 *__restrict__ segs,
                                                       const uint32_t *__restrict__ large_count,
-                                                      const uint32_t *__restrict__ offsets,
                                                       uint32_t *__restrict__ tile_off,
                                                       uint32_t *__restrict__ large_max)
 {
@@ -550,9 +611,8 @@ __global__ __launch_bounds__(1024) void k_large_tiles(const uint32_t *__restrict
         uint32_t i = base + threadIdx.x;
         uint32_t t = 0;
         if (i < nlarge) {
-            uint32_t bin = large_list[i];
-            uint32_t n = offsets[bin + 1] - offsets[bin];
-            t = (n + kSortRun - 1) / kSortRun;
+            const uint2 sg = segs[i];
+            t = (sg.y - sg.x + kSortRun - 1) / kSortRun;
         }
         uint32_t x = t;
         for (int o = 1; o < 64; o <<= 1) {
@@ -572,11 +632,11 @@ __global__ __launch_bounds__(1024) void k_large_tiles(const uint32_t *__restrict
         __syncthreads();
     }
     if (threadIdx.x == 0) tile_off[nlarge] = carry;
-    // the largest listed bin, for the merge rounds to skip themselves
+    // the largest listed segment, for the merge rounds to skip themselves
     uint32_t mx = 0;
     for (uint32_t i = threadIdx.x; i < nlarge; i += blockDim.x) {
-        const uint32_t bin = large_list[i];
-        const uint32_t n = offsets[bin + 1] - offsets[bin];
+        const uint2 sg = segs[i];
+        const uint32_t n = sg.y - sg.x;
         mx = n > mx ? n : mx;
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -599,10 +659,13 @@ __device__ inline uint32_t tile_bin(const uint32_t *tile_off, uint32_t nlarge, u
     return lo;
 }
 
+// Each tile's ordering elements are built from the sorted indices and the
+// coordinates, sorted in LDS and stored at the tile's slots.
+template <int D, typename T>
 __global__ __launch_bounds__(kRunThreads) void k_large_runs(
-    const uint32_t *__restrict__ large_list, const uint32_t *__restrict__ large_count,
-    const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ offsets,
-    SortElem *__restrict__ elems)
+    const uint2 *__restrict__ segs, const uint32_t *__restrict__ large_count,
+    const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ sorted,
+    const T *__restrict__ coords, SortElem *__restrict__ elems)
 {
     __shared__ SortElem s[kSortRun];
     const uint32_t nlarge = *large_count;
@@ -610,15 +673,15 @@ __global__ __launch_bounds__(kRunThreads) void k_large_runs(
     const uint32_t ntiles = tile_off[nlarge];
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t i = tile_bin(tile_off, nlarge, t);
-        const uint32_t bin = large_list[i];
-        const int64_t lo = offsets[bin];
-        const int64_t n = (int64_t)offsets[bin + 1] - lo;
+        const uint2 sg = segs[i];
+        const int64_t lo = sg.x;
+        const int64_t n = (int64_t)sg.y - lo;
         const int64_t r0 = (int64_t)(t - tile_off[i]) * kSortRun;
         const int m = (int)(n - r0 < kSortRun ? n - r0 : kSortRun);
         int n2 = 1;
         while (n2 < m) n2 <<= 1;
         for (int k = threadIdx.x; k < n2; k += kRunThreads)
-            s[k] = k < m ? elems[lo + r0 + k]
+            s[k] = k < m ? elem_at<D, T>(coords, sorted[lo + r0 + k])
                          : SortElem{0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
         __syncthreads();
         lds_bitonic(s, n2);
@@ -628,22 +691,21 @@ __global__ __launch_bounds__(kRunThreads) void k_large_runs(
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_large_merge(
-    const uint32_t *__restrict__ large_list, const uint32_t *__restrict__ large_count,
-    const uint32_t *__restrict__ large_max, const uint32_t *__restrict__ tile_off,
-    const uint32_t *__restrict__ offsets, int round,
+    const uint2 *__restrict__ segs, const uint32_t *__restrict__ large_count,
+    const uint32_t *__restrict__ large_max, const uint32_t *__restrict__ tile_off, int round,
     const SortElem *__restrict__ src, SortElem *__restrict__ dst)
 {
     __shared__ SortElem s[kSortRun];
     __shared__ int64_t split[2];
     const uint32_t nlarge = *large_count;
     const int64_t len = (int64_t)kSortRun << round;
-    if (nlarge == 0 || len >= (int64_t)*large_max) return;  // every bin merged already
+    if (nlarge == 0 || len >= (int64_t)*large_max) return;  // every segment merged already
     const uint32_t ntiles = tile_off[nlarge];
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t i = tile_bin(tile_off, nlarge, t);
-        const uint32_t bin = large_list[i];
-        const int64_t lo = offsets[bin];
-        const int64_t n = (int64_t)offsets[bin + 1] - lo;
+        const uint2 sg = segs[i];
+        const int64_t lo = sg.x;
+        const int64_t n = (int64_t)sg.y - lo;
         if (len >= n) continue;  // merged in an earlier round (uniform per tile)
         const int64_t out0 = (int64_t)(t - tile_off[i]) * kSortRun;
         const int64_t out1 = out0 + kSortRun < n ? out0 + kSortRun : n;
@@ -678,18 +740,18 @@ __global__ __launch_bounds__(kSortThreads) void k_large_merge(
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_large_final(
-    const uint32_t *__restrict__ large_list, const uint32_t *__restrict__ large_count,
-    const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ offsets,
-    SortElem *__restrict__ elems, const SortElem *__restrict__ other)
+    const uint2 *__restrict__ segs, const uint32_t *__restrict__ large_count,
+    const uint32_t *__restrict__ tile_off, SortElem *__restrict__ elems,
+    const SortElem *__restrict__ other)
 {
     const uint32_t nlarge = *large_count;
     if (nlarge == 0) return;
     const uint32_t ntiles = tile_off[nlarge];
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t i = tile_bin(tile_off, nlarge, t);
-        const uint32_t bin = large_list[i];
-        const int64_t lo = offsets[bin];
-        const int64_t n = (int64_t)offsets[bin + 1] - lo;
+        const uint2 sg = segs[i];
+        const int64_t lo = sg.x;
+        const int64_t n = (int64_t)sg.y - lo;
         if (n <= kSortRun || !(merge_rounds(n) & 1)) continue;
         const int64_t out0 = (int64_t)(t - tile_off[i]) * kSortRun;
         const int64_t out1 = out0 + kSortRun < n ? out0 + kSortRun : n;
@@ -751,55 +813,78 @@ __global__ __launch_bounds__(256) void k_prepare_grouped(int64_t M,
     if (out_sorted) out_sorted[s] = idx;
 }
 
-// Ordered sorts: the element at slot s (large bins already sorted there;
-// small bins ranked here among their contiguous elements) gives the final
-// slot, the wrapped support starts and offsets.  F32 canonical elements hold
-// the coordinate bits, so nothing is gathered.
+
+// Canonical order: the point at slot s of the radix order (groups of equal
+// sort keys, in index order) takes the final slot lo + rank inside its group
+// (rank by coordinate bits, then index; the group found from the sorted keys).
+// Groups of more than kSmallBin points were sorted by the large-group sort:
+// their elements are read from there and keep their slots.  Then the wrapped
+// support starts and offsets of the record.
 template <int D, typename T>
-__global__ __launch_bounds__(256) void k_prepare(int64_t M, const SortElem *__restrict__ elems,
-                                                 const uint32_t *__restrict__ offsets,
+__global__ __launch_bounds__(256) void k_prepare(int64_t M, const uint32_t *__restrict__ sorted,
+                                                 const uint64_t *__restrict__ skeys,
+                                                 const SortElem *__restrict__ large_elems,
                                                  const T *__restrict__ coords, GridParams g,
-                                                 int order, int4 *__restrict__ rec_start,
+                                                 int4 *__restrict__ rec_start,
                                                  T *__restrict__ rec_off,
                                                  uint32_t *__restrict__ out_sorted)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
-    const SortElem me = elems[s];
-    const bool from_bits = sizeof(T) == 4 && order == (int)BinOrder::Canonical;
+    const uint64_t key = skeys[s];
+    // the group of equal keys around s, looked for up to kSmallBin + 1 on
+    // either side
+    int64_t lo = s, hi = s + 1;
+    while (lo > 0 && s - lo <= (int64_t)kSmallBin && skeys[lo - 1] == key) --lo;
+    while (hi < M && hi - s <= (int64_t)kSmallBin && skeys[hi] == key) ++hi;
+    const bool large = hi - lo > (int64_t)kSmallBin;
+    uint32_t idx;
     T x[3];
+    int64_t dst = s;
+    if (large) {
+        const SortElem e = large_elems[s];
+        idx = e.idx;
 #pragma unroll
-    for (int a = 0; a < D; ++a) x[a] = elem_coord<D, T>(me, coords, a, from_bits);
+        for (int a = 0; a < D; ++a) x[a] = elem_coord<D, T>(e, coords, a, sizeof(T) == 4);
+    } else {
+        idx = sorted[s];
+#pragma unroll
+        for (int a = 0; a < D; ++a) x[a] = coords[(int64_t)idx * D + a];
+        if (hi - lo > 1) {
+            const SortElem me = elem_of<D, T>(x, idx);
+            int64_t rank = 0;
+            for (int64_t m = lo; m < hi; ++m) {
+                if (m == s) continue;
+                const uint32_t im = sorted[m];
+                T y[3];
+#pragma unroll
+                for (int a = 0; a < D; ++a) y[a] = coords[(int64_t)im * D + a];
+                rank += elem_less(elem_of<D, T>(y, im), me) ? 1 : 0;
+            }
+            dst = lo + rank;
+        }
+    }
     bool reach = true;
 #pragma unroll
     for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
     int st[3] = {0, 0, 0};
-    T off3[3];
-    uint64_t key = 0;
+    T off[4] = {(T)0, (T)0, (T)0, (T)0};
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         int64_t start;
-        double off;
-        support_start<T>(x[a], reach, g, a, &start, &off);
-        const int64_t ws = wrapped_start(start, g.ax[a].n);
-        st[a] = (int)ws;
-        off3[a] = (T)off;
-        key += (uint64_t)(ws >> g.ax[a].shift) * (uint64_t)g.ax[a].stride;
+        double o;
+        support_start<T>(x[a], reach, g, a, &start, &o);
+        st[a] = (int)wrapped_start(start, g.ax[a].n);
+        off[a] = (T)o;
     }
-    int64_t dst = s;
-    const int64_t lo = offsets[key], hi = offsets[key + 1];
-    if (hi - lo <= kSmallBin && hi - lo > 1) {
-        int64_t rank = 0;
-        for (int64_t m = lo; m < hi; ++m) {
-            const SortElem e = elems[m];
-            rank += (m != s && elem_less(e, me)) ? 1 : 0;
-        }
-        dst = lo + rank;
+    rec_start[dst] = make_int4(st[0], st[1], st[2], (int)idx);
+    if constexpr (sizeof(T) == 4) {
+        *(float4 *)(rec_off + dst * 4) = make_float4(off[0], off[1], off[2], off[3]);
+    } else {
+        *(double2 *)(rec_off + dst * 4) = make_double2(off[0], off[1]);
+        *(double2 *)(rec_off + dst * 4 + 2) = make_double2(off[2], off[3]);
     }
-#pragma unroll
-    for (int a = 0; a < D; ++a) rec_off[dst * 4 + a] = off3[a];
-    rec_start[dst] = make_int4(st[0], st[1], st[2], (int)me.idx);
-    if (out_sorted) out_sorted[dst] = me.idx;
+    if (out_sorted) out_sorted[dst] = idx;
 }
 
 // Strengths of every vector gathered into record order.
@@ -874,15 +959,18 @@ bool BinSorter::ensure(int64_t nbins, int64_t M, bool need_sort_scratch)
     if (!ctrl_ && !alloc_u32(&ctrl_, 4, "bin sort counters")) return false;
     if (M > points_cap_) {
         const int64_t cap = M + M / 4 + 1024;
-        if (!alloc_u32(&keys_, cap, "point keys")) return false;
-        if (!alloc_u32(&keys_b_, cap, "point keys")) return false;
+        // 64-bit sort keys (canonical order), else 32-bit
+        if (!alloc_u32(&keys_, 2 * cap, "point keys")) return false;
+        if (!alloc_u32(&keys_b_, 2 * cap, "point keys")) return false;
         if (!alloc_u32(&vals_, cap, "point values")) return false;
         if (!alloc_u32(&sorted_, cap, "sorted points")) return false;
         const int64_t hist = kDigits * radix_tiles(cap);
         if (!alloc_u32(&hist_, hist, "radix histograms")) return false;
         if (!alloc_u32(&hist_sums_, (hist + kScanTile - 1) / kScanTile + 1, "radix scan tiles"))
             return false;
-        if (!alloc_u32(&large_list_, cap / (kSmallBin + 1) + 2, "large-bin list")) return false;
+        // (first, end) of every group above kSmallBin points
+        if (!alloc_u32(&large_list_, 2 * (cap / (kSmallBin + 1) + 2), "large-group list"))
+            return false;
         if (!alloc_u32(&tile_off_, cap / (kSmallBin + 1) + 3, "large-bin tiles")) return false;
         points_cap_ = cap;
     }
@@ -918,6 +1006,35 @@ void scan_u32(const uint32_t *in, const uint32_t *sub, int64_t n, uint32_t *sums
 
 }  // namespace
 
+// LSD radix passes over `bits` key bits: sort keys end in sorted_keys_,
+// point indices in sorted_.
+template <int D, typename T, typename K>
+void BinSorter::radix_passes(int64_t M, const T *coords, const GridParams &g, int mode,
+                             int sub_bits, int bits, hipStream_t stream)
+{
+    const int passes = bits > 8 ? (bits + 7) / 8 : 1;
+    const int64_t ntiles = radix_tiles(M);
+    const int64_t nhist = kDigits * ntiles;
+    K *Kb[2] = {(K *)keys_, (K *)keys_b_};
+    uint32_t *V[2];
+    V[passes & 1] = sorted_;
+    V[(passes + 1) & 1] = vals_;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        if (p == 0)
+            k_radix_keys<D, T, K><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
+                M, coords, g, mode, sub_bits, Kb[0], hist_, ntiles);
+        else
+            k_radix_hist<K><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(M, Kb[p & 1], shift,
+                                                                            hist_, ntiles);
+        scan_u32(hist_, nullptr, nhist, hist_sums_, hist_, stream);
+        k_radix_scatter<K><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
+            M, Kb[p & 1], p == 0 ? nullptr : V[p & 1], shift, hist_, ntiles, Kb[(p + 1) & 1],
+            V[(p + 1) & 1]);
+    }
+    sorted_keys_ = Kb[passes & 1];
+}
+
 template <int D, typename T>
 bool BinSorter::sort(int64_t M, const T *coords, const GridParams &g, BinOrder order,
                      hipStream_t stream)
@@ -936,61 +1053,58 @@ bool BinSorter::sort(int64_t M, const T *coords, const GridParams &g, BinOrder o
     uint32_t *large_count = ctrl_;
     uint32_t *large_max = ctrl_ + 1;
 
-    // radix passes over the bits of the largest key, values ending in sorted_;
-    // the grouped order also sorts by the support-start cell inside the bin
-    // (neighbouring points then read neighbouring grid windows)
+    // radix passes over the bits of the largest key, values ending in sorted_.
+    // Below the bin key, the grouped order also sorts by the support-start
+    // cell inside the bin (neighbouring points then read neighbouring grid
+    // windows) and the canonical order by the position inside the bin
+    // quantized to 2^5 steps per axis (64-bit keys; groups of equal keys then
+    // hold a few points even in dense clusters, ranked by coordinate bits in
+    // prepare); the index order sorts by bin only, so stability leaves every
+    // bin in index order.
     int bits = 0;
     while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)g.nbins) ++bits;
-    const int cbits = cell_bits<D>(g);
-    const bool cells = order == BinOrder::Grouped && bits + cbits <= 32;
-    const int cell_shift = cells ? cbits : 0;
-    bits += cell_shift;
-    const int passes = bits > 8 ? (bits + 7) / 8 : 1;
-    const int64_t ntiles = radix_tiles(M);
-    const int64_t nhist = kDigits * ntiles;
-    uint32_t *K[2] = {keys_, keys_b_};
-    uint32_t *V[2];
-    V[passes & 1] = sorted_;
-    V[(passes + 1) & 1] = vals_;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
-        if (p == 0)
-            k_radix_keys<D, T><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
-                M, coords, g, cells, K[0], hist_, ntiles);
-        else
-            k_radix_hist<<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(M, K[p & 1], shift,
-                                                                         hist_, ntiles);
-        scan_u32(hist_, nullptr, nhist, hist_sums_, hist_, stream);
-        k_radix_scatter<<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
-            M, K[p & 1], p == 0 ? nullptr : V[p & 1], shift, hist_, ntiles, K[(p + 1) & 1],
-            V[(p + 1) & 1]);
+    int mode = kKeyBin, sub_bits = 0, cell_shift = 0;
+    if (order == BinOrder::Grouped && bits + cell_bits<D>(g) <= 32) {
+        mode = kKeyCell;
+        cell_shift = cell_bits<D>(g);
+    } else if (order == BinOrder::Canonical) {
+        sub_bits = std::min(5, (64 - bits) / D);
+        mode = kKeyPosition;
+        cell_shift = D * sub_bits;
     }
-    const uint32_t *sorted_keys = K[passes & 1];
+    bits += cell_shift;
+    const bool wide = order == BinOrder::Canonical;
+    if (wide)
+        radix_passes<D, T, uint64_t>(M, coords, g, mode, sub_bits, bits, stream);
+    else
+        radix_passes<D, T, uint32_t>(M, coords, g, mode, sub_bits, bits, stream);
+    uint2 *segs = (uint2 *)large_list_;
     uint4 *gaps_long = gaps_ + gaps_mid_cap_;
-    k_bin_offsets<<<grid_for(M + 1, 256), 256, 0, stream>>>(
-        M, sorted_keys, cell_shift, g.nbins, offsets_, gaps_, gaps_long, ctrl_,
-        order == BinOrder::Canonical ? large_list_ : nullptr);
+    if (wide)
+        k_bin_offsets<uint64_t><<<grid_for(M + 1, 256), 256, 0, stream>>>(
+            M, (const uint64_t *)sorted_keys_, cell_shift, g.nbins, offsets_, gaps_, gaps_long,
+            ctrl_, segs);
+    else
+        k_bin_offsets<uint32_t><<<grid_for(M + 1, 256), 256, 0, stream>>>(
+            M, (const uint32_t *)sorted_keys_, cell_shift, g.nbins, offsets_, gaps_, gaps_long,
+            ctrl_, nullptr);
     k_fill_gaps<<<1024, 256, 0, stream>>>(gaps_, gaps_long, ctrl_, offsets_);
 
     if (order == BinOrder::Canonical) {
         unsigned grid = (unsigned)std::min<int64_t>(2048, M / kSortRun + 64);
         SortElem *A = (SortElem *)scratch_a_, *B = (SortElem *)scratch_b_;
-        k_make_elems<D, T><<<grid_for(M, 256), 256, 0, stream>>>(M, sorted_, coords, A);
-        k_large_tiles<<<1, 1024, 0, stream>>>(large_list_, large_count, offsets_, tile_off_,
-                                              large_max);
-        k_large_runs<<<grid, kRunThreads, 0, stream>>>(large_list_, large_count, tile_off_,
-                                                       offsets_, A);
-        // rounds for the largest possible bin; bins done earlier skip
+        k_large_tiles<<<1, 1024, 0, stream>>>(segs, large_count, tile_off_, large_max);
+        k_large_runs<D, T><<<grid, kRunThreads, 0, stream>>>(segs, large_count, tile_off_,
+                                                             sorted_, coords, A);
+        // rounds for the largest possible group; groups done earlier skip
         int rounds = 0;
         for (int64_t len = kSortRun; len < M; len <<= 1) ++rounds;
         for (int r = 0; r < rounds; ++r)
-            k_large_merge<<<grid, kSortThreads, 0, stream>>>(large_list_, large_count, large_max,
-                                                             tile_off_,
-                                                             offsets_, r, (r & 1) ? B : A,
+            k_large_merge<<<grid, kSortThreads, 0, stream>>>(segs, large_count, large_max,
+                                                             tile_off_, r, (r & 1) ? B : A,
                                                              (r & 1) ? A : B);
         if (rounds > 0)
-            k_large_final<<<grid, kSortThreads, 0, stream>>>(large_list_, large_count, tile_off_,
-                                                             offsets_, A, B);
+            k_large_final<<<grid, kSortThreads, 0, stream>>>(segs, large_count, tile_off_, A, B);
     }
     return hip_ok(hipGetLastError(), "bin sort launch");
 }
@@ -1008,8 +1122,8 @@ bool BinSorter::prepare(int64_t M, const T *coords, const GridParams &g, BinOrde
                                                                       out_sorted);
     else
         k_prepare<D, T><<<grid_for(M, 256), 256, 0, stream>>>(
-            M, (const SortElem *)scratch_a_, offsets_, coords, g, (int)order, rec_start, rec_off,
-            out_sorted);
+            M, sorted_, (const uint64_t *)sorted_keys_, (const SortElem *)scratch_a_, coords, g,
+            rec_start, rec_off, out_sorted);
     return hip_ok(hipGetLastError(), "prepare launch");
 }
 

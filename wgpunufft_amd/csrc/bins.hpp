@@ -34,6 +34,9 @@ public:
 
 private:
     bool ensure(int64_t nbins, int64_t M, bool need_sort_scratch);
+    template <int D, typename T, typename K>
+    void radix_passes(int64_t M, const T *coords, const GridParams &g, int mode, int sub_bits,
+                      int bits, hipStream_t stream);
 
     uint32_t *ctrl_ = nullptr;     // large bins, largest large bin, mid gaps, long gaps
     uint32_t *offsets_ = nullptr;
@@ -44,8 +47,9 @@ private:
     uint32_t *vals_ = nullptr;     // radix value buffer paired with sorted_
     uint32_t *sorted_ = nullptr;
     uint32_t *hist_ = nullptr;     // digit-major per-tile digit counts
+    const void *sorted_keys_ = nullptr;  // sort keys in slot order (keys_ or keys_b_)
     uint32_t *hist_sums_ = nullptr;
-    uint32_t *large_list_ = nullptr;
+    uint32_t *large_list_ = nullptr;  // (first, end) of the groups above kSmallBin
     uint32_t *tile_off_ = nullptr;
     void *scratch_a_ = nullptr;
     void *scratch_b_ = nullptr;

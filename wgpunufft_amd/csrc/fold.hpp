@@ -1,6 +1,6 @@
-// Device-side fold of a coordinate onto the fine grid, its support start and
-// its bin key (reference src/gpu_nd_wgsl.rs:138-188, 198-310 and the count
-// shader src/gpu_nd_bins.rs:474-554).
+// Device-side fold of a coordinate onto the fine grid and its support start
+// (reference src/gpu_nd_wgsl.rs:138-188, 198-310 and the count shader
+// src/gpu_nd_bins.rs:474-554).
 //
 // F32 coordinates fold in f64: x * (s_hi + s_lo) with s = n / (2 pi) split
 // into an f32 head and tail as the reference does; both partial products are
@@ -95,51 +95,13 @@ __device__ inline bool point_in_reach(const T *coords, int64_t j)
     return r;
 }
 
-// Bin key of a point from its coordinates: axis 0 fastest over the wrapped
-// starts shifted by the bin sides.  With `cell`, the start's cell inside its
-// bin follows below the bin key (axis 0 fastest, the bin sides' bits).
-// Keys are below 2^32 (bin counts, and bin count times cells when `cell`),
-// so the arithmetic is 32-bit.
-template <int D, typename T>
-__device__ inline uint32_t bin_key_of(const T *x, const GridParams &g, bool cell = false)
-{
-    bool reach = true;
-#pragma unroll
-    for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
-    uint32_t key = 0, sub = 0;
-    int sub_bits = 0;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-        int64_t s;
-        double off;
-        support_start<T>(x[a], reach, g, a, &s, &off);
-        const uint32_t ws = (uint32_t)wrapped_start(s, g.ax[a].n);
-        key += (ws >> g.ax[a].shift) * (uint32_t)g.ax[a].stride;
-        if (cell) {
-            sub |= (ws & ((1u << g.ax[a].shift) - 1u)) << sub_bits;
-            sub_bits += g.ax[a].shift;
-        }
-    }
-    return cell ? (key << sub_bits) | sub : key;
-}
-
-// Bits of the in-bin cell index that bin_key_of appends.
+// Bits of the in-bin start cell (the bin sides' bits).
 template <int D>
 __device__ __host__ inline int cell_bits(const GridParams &g)
 {
     int b = 0;
     for (int a = 0; a < D; ++a) b += g.ax[a].shift;
     return b;
-}
-
-// Bin key of point j of point-major coordinates.
-template <int D, typename T>
-__device__ inline uint32_t point_bin_key(const T *coords, int64_t j, const GridParams &g)
-{
-    T x[D];
-#pragma unroll
-    for (int a = 0; a < D; ++a) x[a] = coords[j * D + a];
-    return bin_key_of<D, T>(x, g);
 }
 
 }  // namespace mnufft
