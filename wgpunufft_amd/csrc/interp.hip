@@ -153,7 +153,10 @@ struct Complex2<double> {
 };
 
 // item_off[b] = first item of bin b (exclusive scan of ceil(count / P)),
-// item_off[nbins] = items.
+// item_off[nbins] = items.  One workgroup; every thread takes kItemsPerThread
+// consecutive bins per round.
+constexpr int kItemsPerThread = 8;
+
 __global__ __launch_bounds__(1024) void k_tile_items(int64_t nbins, uint32_t points_per_item,
                                                      const uint32_t *__restrict__ offsets,
                                                      uint32_t *__restrict__ item_off)
@@ -162,17 +165,25 @@ __global__ __launch_bounds__(1024) void k_tile_items(int64_t nbins, uint32_t poi
     __shared__ uint32_t carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int64_t base = 0; base < nbins; base += blockDim.x) {
-        int64_t b = base + threadIdx.x;
-        uint32_t t = 0;
-        if (b < nbins) {
-            uint32_t n = offsets[b + 1] - offsets[b];
-            t = (n + points_per_item - 1) / points_per_item;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int64_t kRound = 1024 * kItemsPerThread;
+    for (int64_t base = 0; base < nbins; base += kRound) {
+        const int64_t b0 = base + (int64_t)threadIdx.x * kItemsPerThread;
+        uint32_t t[kItemsPerThread];
+        uint32_t mine = 0;
+#pragma unroll
+        for (int i = 0; i < kItemsPerThread; ++i) {
+            const int64_t b = b0 + i;
+            t[i] = 0;
+            if (b < nbins) {
+                const uint32_t n = offsets[b + 1] - offsets[b];
+                t[i] = (n + points_per_item - 1) / points_per_item;
+            }
+            mine += t[i];
         }
-        uint32_t x = t;
+        uint32_t x = mine;
         for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(x, o, 64);
+            const uint32_t y = __shfl_up(x, o, 64);
             if (lane >= o) x += y;
         }
         if (lane == 63) wsum[wid] = x;
@@ -182,7 +193,12 @@ __global__ __launch_bounds__(1024) void k_tile_items(int64_t nbins, uint32_t poi
             if (w < wid) pre += wsum[w];
             all += wsum[w];
         }
-        if (b < nbins) item_off[b] = carry + pre + x - t;
+        uint32_t run = carry + pre + x - mine;
+#pragma unroll
+        for (int i = 0; i < kItemsPerThread; ++i) {
+            if (b0 + i < nbins) item_off[b0 + i] = run;
+            run += t[i];
+        }
         __syncthreads();
         if (threadIdx.x == 0) carry += all;
         __syncthreads();
