@@ -381,6 +381,57 @@ def test_type1_dense_cluster_shuffle_invariance():
     assert rel_l2(f0[sel], ref) <= 32e-6
 
 
+@pytest.mark.parametrize("case", ["tight_cluster", "duplicates", "f64", "2d"])
+def test_coordinate_order_large_groups_and_ranks(case):
+    """Coordinate order where equal sort keys form large groups: a cluster
+    far tighter than one position step (groups of thousands: the
+    large-group merge sort with several rounds) and points repeated 200
+    times with equal strengths (groups of 200); plus F64 and 2D plans.  A
+    shuffled input gives the same output bit for bit, and it matches the
+    direct sums."""
+    torch, W, O, S = _setup()
+    d = 2 if case == "2d" else 3
+    n_modes = [40, 36] if d == 2 else [24, 20, 16]
+    f64 = case == "f64"
+    if case == "tight_cluster":
+        M = 60000
+        x = S.clustered_coords(S.config_seed(2, 720), M, (0.3, -1.1, 2.0), 1e-4)
+        # distinct coordinates: exact duplicates keep index order (not
+        # permutation independent when their strengths differ)
+        assert np.unique(x, axis=0).shape[0] == M
+        c = S.complex_normal(S.config_seed(2, 721), M)
+    elif case == "duplicates":
+        base = S.clustered_coords(S.config_seed(2, 722), 300, (0.3, -1.1, 2.0), 0.05)
+        cb = S.complex_normal(S.config_seed(2, 723), 300)
+        x = np.repeat(base, 200, axis=0)
+        c = np.repeat(cb, 200)
+        M = x.shape[0]
+    else:
+        M = 30000
+        x = S.clustered_coords(S.config_seed(2, 724), M, (0.3, -1.1, 2.0)[:d], 0.02)
+        x[: M // 2] = S.uniform_coords(S.config_seed(2, 725), M // 2, d)
+        c = S.complex_normal(S.config_seed(2, 726), M)
+    if f64:
+        x = x.astype(np.float64)
+        c = c.astype(np.complex128)
+    cfg = W.NufftConfig(n_modes, 1e-9 if f64 else 1e-6)
+    if f64:
+        cfg = cfg.with_precision(W.F64)
+    plan = W.NufftPlan.type1_gpu(cfg).set_point_order("coordinates")
+    f0 = plan.encode_gpu(torch.from_numpy(x).cuda(), torch.from_numpy(c).cuda()).cpu().numpy()
+    p = S.shuffle_perm(S.config_seed(2, 727), M)
+    f1 = plan.encode_gpu(torch.from_numpy(x[p].copy()).cuda(),
+                         torch.from_numpy(c[p].copy()).cuda()).cpu().numpy()
+    view = np.uint64 if f64 else np.uint32
+    assert np.array_equal(f0.view(view), f1.view(view))
+    N = int(np.prod(n_modes))
+    sel = np.arange(0, N, 97)
+    ref = O.direct_type1(x.astype(np.float64), c.astype(np.complex128), n_modes, modes=sel)
+    err = rel_l2(f0[sel], ref)
+    print(f"COORD {case} err={err:.3e}")
+    assert err <= (1e-7 if f64 else 32e-6)
+
+
 def test_type2_batched_tiled_interp_matches_oracle():
     torch, W, O, S = _setup()
     n_modes = [20, 18, 16]
