@@ -72,8 +72,9 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the transforms would shard)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (type 1 would "
+              "shard points with one reduce-scatter of the fine grid, type 2 points with a "
+              "replicated grid)", file=sys.stderr)
         return 2
     import numpy as np
     import torch

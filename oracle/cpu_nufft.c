@@ -1,5 +1,5 @@
 /*
- * CPU port of the reference's CpuNufftPlan -- the CPU BASELINE of bench.py
+ * CPU port of the reference's CpuNufftPlan -- the CPU baseline of bench.py
  * (TEST INFRASTRUCTURE: only tests/ and bench.py's cpu_baseline leg load it).
  *
  * Follows src/cpu.rs and the files of src/cpu/ for F32 plans:
