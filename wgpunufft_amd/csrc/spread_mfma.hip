@@ -336,9 +336,10 @@ k_spread_mfma3(
     constexpr int NRY = kTile + W - 1;
     constexpr int NROWS = NRY * (TZ + W - 1);
     constexpr int kBlocks = TZ / 2;  // 32-row blocks (16 y x 2 z) of the tile
-    __shared__ __align__(16) float KX[kTile * kPitch];
-    __shared__ __align__(16) float KY[kTile * kPitch];
-    __shared__ __align__(16) float KZ[kTile * kPitch];
+    // one spare row per table takes the weights that fall outside the tile
+    __shared__ __align__(16) float KX[(kTile + 1) * kPitch];
+    __shared__ __align__(16) float KY[(kTile + 1) * kPitch];
+    __shared__ __align__(16) float KZ[(kTile + 1) * kPitch];
     __shared__ __align__(16) float CC[2 * kPitch];
     __shared__ MfmaRec st[64];
     __shared__ TileRows<NROWS> rows;
@@ -424,12 +425,12 @@ k_spread_mfma3(
                         const int s = axis == 0 ? S.sx : (axis == 1 ? S.sy : S.sz);
                         const float o = axis == 0 ? S.ox : (axis == 1 ? S.oy : S.oz);
                         float *tb = axis == 0 ? KX : (axis == 1 ? KY : KZ);
-                        const int lim = axis == 2 ? TZ : kTile;
+                        const unsigned lim = axis == 2 ? TZ : kTile;
 #pragma unroll
                         for (int t = 0; t < W; ++t) {
-                            const int c = s + t;
-                            if (c >= 0 && c < lim)
-                                tb[c * kPitch + j] = es_weight_fast((float)t + o, two_over_w, beta_l2e);
+                            const unsigned c = (unsigned)(s + t);
+                            const unsigned row = c < lim ? c : (unsigned)kTile;
+                            tb[row * kPitch + j] = es_weight_fast((float)t + o, two_over_w, beta_l2e);
                         }
                     }
                 } else {
@@ -512,8 +513,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     using V = typename Op::V;
     constexpr bool kSplit = MODE != (int)SpreadMethod::MfmaF16;
     constexpr int NROWS = kTile2 + W - 1;
-    __shared__ __align__(16) float KX[kTile2 * kPitch];
-    __shared__ __align__(16) float KY[kTile2 * kPitch];
+    // one spare row per table takes the weights that fall outside the tile
+    __shared__ __align__(16) float KX[(kTile2 + 1) * kPitch];
+    __shared__ __align__(16) float KY[(kTile2 + 1) * kPitch];
     __shared__ __align__(16) float CC[2 * kPitch];
     __shared__ MfmaRec st[64];
     __shared__ TileRows<NROWS> rows;
@@ -590,9 +592,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                         float *tb = axis == 0 ? KX : KY;
 #pragma unroll
                         for (int t = 0; t < W; ++t) {
-                            const int c = s0 + t;
-                            if (c >= 0 && c < kTile2)
-                                tb[c * kPitch + j] = es_weight_fast((float)t + o, two_over_w, beta_l2e);
+                            const unsigned c = (unsigned)(s0 + t);
+                            const unsigned row = c < (unsigned)kTile2 ? c : (unsigned)kTile2;
+                            tb[row * kPitch + j] = es_weight_fast((float)t + o, two_over_w, beta_l2e);
                         }
                     }
                 } else if (lane < 48) {
