@@ -253,15 +253,27 @@ __global__ __launch_bounds__(THREADS) void k_interp_tiled3(
     const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n, n2 = g.ax[2].n;
     const T beta = (T)g.beta;
     const T *gv = grid + 2 * v * g.fine_total;
+    // tile loads: every wavefront fills RPW rows of RX cells at a time, each
+    // lane keeping its x cell
+    constexpr int ROWS = EY * EZ;
+    static_assert(RX <= 64, "a tile row must fit one wavefront");
+    constexpr int RPW = 64 / RX;
+    const int lane = threadIdx.x & 63;
+    const int rsub = lane / RX, lx = lane % RX;
+    const int r_first = (int)(threadIdx.x >> 6) * RPW + rsub;
+    constexpr int R_STEP = (THREADS / 64) * RPW;
     for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         const TileItem3 ti = tile_item3(item, item_off, g, TX, TY, TZ);
-        for (int t = threadIdx.x; t < CELLS; t += THREADS) {
-            const int lx = t % RX, ly = (t / RX) % EY, lz = t / (RX * EY);
-            int64_t gx = ti.x0 + lx, gy = ti.y0 + ly, gz = ti.z0 + lz;
+        if (rsub < RPW) {
+            int64_t gx = ti.x0 + lx;
             gx = gx >= n0 ? gx - n0 : gx;
-            gy = gy >= n1 ? gy - n1 : gy;
-            gz = gz >= n2 ? gz - n2 : gz;
-            tile[t] = *(const T2 *)(gv + 2 * (gx + n0 * (gy + n1 * gz)));
+            for (int r = r_first; r < ROWS; r += R_STEP) {
+                const int ly = r % EY, lz = r / EY;
+                int64_t gy = ti.y0 + ly, gz = ti.z0 + lz;
+                gy = gy >= n1 ? gy - n1 : gy;
+                gz = gz >= n2 ? gz - n2 : gz;
+                tile[r * RX + lx] = *(const T2 *)(gv + 2 * (gx + n0 * (gy + n1 * gz)));
+            }
         }
         __syncthreads();
         const int64_t first = (int64_t)offsets[ti.b] + ti.k * P;
@@ -284,8 +296,24 @@ __global__ __launch_bounds__(THREADS) void k_interp_tiled3(
                 int64_t ws = wrapped_start(start, g.ax[a].n);
                 int64_t o = a == 0 ? ti.x0 : (a == 1 ? ti.y0 : ti.z0);
                 l[a] = (int)(ws - o);
+                if constexpr (sizeof(T) == 4) {
+                    // the kernel at distances j + off: scaled distance off * 2/w +
+                    // j * 2/w, exponent as one FMA (hardware sqrt / exp2)
+                    constexpr float two_over_w = 2.0f / (float)W;
+                    const float u0 = (float)off * two_over_w;
+                    const float bl = (float)beta * 1.44269504088896341f;
 #pragma unroll
-                for (int j = 0; j < W; ++j) wt[a][j] = es_w<T>((T)j + (T)off, W, beta);
+                    for (int j = 0; j < W; ++j) {
+                        const float sv = u0 + (float)j * two_over_w;
+                        const float q = __builtin_fmaf(-sv, sv, 1.0f);
+                        const float e = __builtin_amdgcn_exp2f(
+                            __builtin_fmaf(__builtin_amdgcn_sqrtf(q), bl, -bl));
+                        wt[a][j] = q > 0.0f ? e : 0.0f;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < W; ++j) wt[a][j] = es_w<T>((T)j + (T)off, W, beta);
+                }
             }
             T sre = (T)0, sim = (T)0;
             if constexpr (kPairs) {
