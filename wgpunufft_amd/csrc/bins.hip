@@ -59,14 +59,7 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *total)
     return base + x - v;
 }
 
-// The scanned value of entry k is in[k] - sub[k] (sub may be null).
-__device__ inline uint32_t scan_value(const uint32_t *in, const uint32_t *sub, int64_t k)
-{
-    return sub ? in[k] - sub[k] : in[k];
-}
-
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ in,
-                                                              const uint32_t *__restrict__ sub,
                                                               int64_t n,
                                                               uint32_t *__restrict__ tile_sums)
 {
@@ -75,45 +68,35 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         int64_t k = base + (int64_t)i * kScanThreads + threadIdx.x;
-        if (k < n) s += scan_value(in, sub, k);
+        if (k < n) s += in[k];
     }
     uint32_t total;
     block_exclusive_scan(s, &total);
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_top(uint32_t *__restrict__ tile_sums,
-                                                           int64_t ntiles)
-{
-    uint32_t carry = 0;
-    for (int64_t base = 0; base < ntiles; base += kScanThreads) {
-        int64_t k = base + threadIdx.x;
-        uint32_t v = k < ntiles ? tile_sums[k] : 0u;
-        uint32_t total;
-        uint32_t ex = block_exclusive_scan(v, &total);
-        if (k < ntiles) tile_sums[k] = carry + ex;
-        carry += total;
-    }
-}
-
-// out may alias in: every entry is read and written by the same thread.
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in,
-                                                            const uint32_t *__restrict__ sub,
-                                                            int64_t n,
+// Second half of the scan: every tile adds the sums of the tiles before it
+// (summed here, a few thousand at most) to its own exclusive scan.  out may
+// alias in: every entry is read and written by the same thread.
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in, int64_t n,
                                                             const uint32_t *__restrict__ tile_sums,
                                                             uint32_t *out)
 {
+    uint32_t before = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kScanThreads) before += tile_sums[t];
+    uint32_t carry;
+    block_exclusive_scan(before, &carry);
     int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
     uint32_t s = 0;
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         int64_t k = base + i;
-        v[i] = k < n ? scan_value(in, sub, k) : 0u;
+        v[i] = k < n ? in[k] : 0u;
         s += v[i];
     }
     uint32_t total;
-    uint32_t run = block_exclusive_scan(s, &total) + tile_sums[blockIdx.x];
+    uint32_t run = block_exclusive_scan(s, &total) + carry;
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         int64_t k = base + i;
@@ -993,15 +976,13 @@ __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
 
 namespace {
 
-// Exclusive scan of in[k] - sub[k] (sub may be null) over n entries into out
-// (which may be in).
-void scan_u32(const uint32_t *in, const uint32_t *sub, int64_t n, uint32_t *sums, uint32_t *out,
-              hipStream_t stream)
+// Exclusive scan of n entries into out (which may be in): tile sums, then
+// every tile's scan plus the sum of the tiles before it.
+void scan_u32(const uint32_t *in, int64_t n, uint32_t *sums, uint32_t *out, hipStream_t stream)
 {
     const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    k_scan_reduce<<<(unsigned)tiles, kScanThreads, 0, stream>>>(in, sub, n, sums);
-    k_scan_top<<<1, kScanThreads, 0, stream>>>(sums, tiles);
-    k_scan_down<<<(unsigned)tiles, kScanThreads, 0, stream>>>(in, sub, n, sums, out);
+    k_scan_reduce<<<(unsigned)tiles, kScanThreads, 0, stream>>>(in, n, sums);
+    k_scan_down<<<(unsigned)tiles, kScanThreads, 0, stream>>>(in, n, sums, out);
 }
 
 }  // namespace
@@ -1027,7 +1008,7 @@ void BinSorter::radix_passes(int64_t M, const T *coords, const GridParams &g, in
         else
             k_radix_hist<K><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(M, Kb[p & 1], shift,
                                                                             hist_, ntiles);
-        scan_u32(hist_, nullptr, nhist, hist_sums_, hist_, stream);
+        scan_u32(hist_, nhist, hist_sums_, hist_, stream);
         k_radix_scatter<K><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
             M, Kb[p & 1], p == 0 ? nullptr : V[p & 1], shift, hist_, ntiles, Kb[(p + 1) & 1],
             V[(p + 1) & 1]);
