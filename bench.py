@@ -39,7 +39,7 @@ FP16_MFMA_PEAK_TFLOPS = 2500.0
 # profiles/traffic_3d_t1t2_128modes_1e7pts.md holds the table.  Reported
 # only when the run uses the default workload.
 MEASURED_TRAFFIC = {
-    "k_spread_mfma3": 2.8481e9,
+    "mfma_spread": 2.0303e9,
     "k_interp_tiled3": 2.4198e9,
 }
 
@@ -173,7 +173,10 @@ def main():
     t_mfma = stages_mfma["type1"]["spread_kernel"]
     t_plain = stages_plain["type1"]["spread_kernel"]
     t_interp = stages_mfma["type2"]["interp"]
-    roof_mfma = spread_roof(t_mfma, f"k_spread_mfma3 (type-1 16x16x8-tile contraction, {args.spread})",
+    roof_mfma = spread_roof(t_mfma, f"mfma_spread (type-1 matrix-core spread, {args.spread}: "
+                                    "k_spread_mfma3 16x16x8 tiles of the light bins, "
+                                    "k_reduce_tiles, k_dense_windows + k_dense_reduce + "
+                                    "k_dense_add for the dense bins; one event span)",
                             FP16_MFMA_PEAK_TFLOPS, "peak: dense FP16 matrix cores")
     roof_plain = spread_roof(t_plain,
                              "k_sweep3 (type-1 3D row-sweep spread)" if sweep
@@ -189,7 +192,8 @@ def main():
     default_workload = (M, N, args.eps) == (10_000_000, 128, 1e-6)
     for roof in (roof_mfma, roof_interp):
         key = roof["kernel"].split(" ")[0]
-        if default_workload and key in MEASURED_TRAFFIC and args.spread == "mfma_f16x3":
+        if (default_workload and MEASURED_TRAFFIC.get(key) is not None
+                and args.spread == "mfma_f16x3"):
             roof["traffic"] = MEASURED_TRAFFIC[key]
             roof["traffic_unit"] = "B per launch (FETCH_SIZE+WRITE_SIZE, copy-calibrated)"
     dominant, other = (roof_mfma, roof_interp) if t_mfma >= t_interp else (roof_interp, roof_mfma)

@@ -197,6 +197,19 @@ int mnufft_set_spread_tuning(mnufft_plan plan, int64_t light_limit, int64_t min_
                              int64_t part_budget_bytes);
 
 /*
+ * Dense bins of the 3D matrix-core type-1 spread (no counterpart in the
+ * reference, whose heavy blocks cover clusters): a bin holding at least
+ * min_records points (default 32; doubled, up to seven times, while the
+ * bins' windows exceed 16384 parts of 1024 points) is summed on its own over
+ * its window of (bin side + w - 1) cells per axis, and the windows are added
+ * to the tile sums in a fixed bin order.  0 turns dense bins off.  Applies
+ * from the next set_points (an execute re-bins kept points); output stays
+ * bitwise repeatable and, in coordinate order, independent of the point
+ * order.  Grids whose bins do not fit 16 x 8 x 8-cell windows have none.
+ */
+int mnufft_set_dense_bins(mnufft_plan plan, int64_t min_records);
+
+/*
  * FP64 direct sums on the GPU: reference_type1_f64 / reference_type2_f64
  * (src/direct.rs:9-118) and reference_type3_f64 (src/type3.rs:609-673).
  * kind 1: out[o] = sum_j in[j] exp(i sign k_o.x_j) over the linear mode

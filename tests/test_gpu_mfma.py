@@ -124,17 +124,19 @@ def test_mfma_bitwise_repeat_and_shuffle_invariance():
         assert np.array_equal(f.cpu().numpy().view(np.uint32), runs[0].view(np.uint32))
 
 
-def test_mfma_heavy_tiles_batch_equals_loop():
+@pytest.mark.parametrize("dense_min", [0, 32])
+def test_mfma_heavy_tiles_batch_equals_loop(dense_min):
     """A dense cluster splits tiles into parts (partial sums added in part
-    order); a batch of two vectors equals the vectors one by one, bit for
-    bit, and matches the direct sums."""
+    order) or, with dense bins, is summed bin by bin; a batch of two vectors
+    equals the vectors one by one, bit for bit, and matches the direct
+    sums."""
     torch, W, O, S = _setup()
     n_modes = [32, 32, 32]
     M, B = 200000, 2
     x = S.clustered_coords(S.config_seed(2, 980), M, (0.3, -1.1, 2.0), 0.002)
     c = S.complex_normal(S.config_seed(2, 981), M * B)
     plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
-    plan.set_spread_method("mfma_f16x3")
+    plan.set_spread_method("mfma_f16x3").set_dense_bins(dense_min)
     xt = torch.from_numpy(x).cuda()
     fb = plan.encode_gpu(xt, torch.from_numpy(c).cuda()).cpu().numpy()
     N = int(np.prod(n_modes))

@@ -146,6 +146,14 @@ class _PlanBase:
                                              int(part_budget_bytes)))
         return self
 
+    def set_dense_bins(self, min_records):
+        """Dense bins of the 3D matrix-core type-1 spread: a bin of at least
+        min_records points (default 32) is summed once over its window and
+        the windows are added to the tile sums in a fixed bin order; 0 turns
+        them off.  Takes effect at the next set_points or execute."""
+        check(lib().mnufft_set_dense_bins(self._handle, int(min_records)))
+        return self
+
     def timing_history(self, on=True):
         """Sum stage times over every call until the next stage_times()."""
         check(lib().mnufft_timing_history(self._handle, 1 if on else 0))

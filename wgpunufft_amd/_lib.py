@@ -52,6 +52,7 @@ SIGNATURES = [
     ("mnufft_set_point_order", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("mnufft_set_spread_tuning", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64,
                                                 ctypes.c_int64]),
+    ("mnufft_set_dense_bins", ctypes.c_int, [_vp, ctypes.c_int64]),
     ("mnufft_kernel_params", ctypes.c_int, [ctypes.c_double, ctypes.c_double,
                                             ctypes.POINTER(ctypes.c_int32),
                                             ctypes.POINTER(ctypes.c_double)]),

@@ -756,6 +756,27 @@ __device__ inline T elem_coord(const SortElem &e, const T *coords, int a, bool f
     return coords[(int64_t)e.idx * D + a];
 }
 
+// Final slot of the record at bin-order slot s of bin b when dense bins are
+// split off: light records first (slot minus the dense records before the
+// bin), then the dense bins' records in bin order.
+__device__ inline int64_t dense_slot(int64_t s, int64_t b, int64_t M, int64_t nbins,
+                                     const uint32_t *__restrict__ offsets,
+                                     const uint32_t *__restrict__ dense_before)
+{
+    const uint32_t d0 = dense_before[b];
+    if (dense_before[b + 1] == d0) return s - d0;
+    return M - (int64_t)dense_before[nbins] + d0 + (s - (int64_t)offsets[b]);
+}
+
+template <int D>
+__device__ inline int64_t bin_of_starts(const int *st, const GridParams &g)
+{
+    int64_t b = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) b += (int64_t)(st[a] >> g.ax[a].shift) * g.ax[a].stride;
+    return b;
+}
+
 // Grouped and index orders: slot s keeps the point the radix sort put there.
 // The coordinates are gathered once (all loads in flight before any store)
 // and each record is written with two 16-byte stores.
@@ -765,9 +786,11 @@ __global__ __launch_bounds__(256) void k_prepare_grouped(int64_t M,
                                                          const T *__restrict__ coords,
                                                          GridParams g, int4 *__restrict__ rec_start,
                                                          T *__restrict__ rec_off,
-                                                         uint32_t *__restrict__ out_sorted)
+                                                         uint32_t *__restrict__ out_sorted,
+                                                         const uint32_t *__restrict__ offsets,
+                                                         const uint32_t *__restrict__ dense_before)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     const uint32_t idx = sorted[s];
     T x[D];
@@ -785,6 +808,10 @@ __global__ __launch_bounds__(256) void k_prepare_grouped(int64_t M,
         support_start<T>(x[a], reach, g, a, &start, &o);
         st[a] = (int)wrapped_start(start, g.ax[a].n);
         off[a] = (T)o;
+    }
+    if (dense_before) {
+        const int64_t sorted_slot = s;
+        s = dense_slot(sorted_slot, bin_of_starts<D>(st, g), M, g.nbins, offsets, dense_before);
     }
     rec_start[s] = make_int4(st[0], st[1], st[2], (int)idx);
     if constexpr (sizeof(T) == 4) {
@@ -810,7 +837,9 @@ __global__ __launch_bounds__(256) void k_prepare(int64_t M, const uint32_t *__re
                                                  const T *__restrict__ coords, GridParams g,
                                                  int4 *__restrict__ rec_start,
                                                  T *__restrict__ rec_off,
-                                                 uint32_t *__restrict__ out_sorted)
+                                                 uint32_t *__restrict__ out_sorted,
+                                                 const uint32_t *__restrict__ offsets,
+                                                 const uint32_t *__restrict__ dense_before)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
@@ -860,6 +889,8 @@ __global__ __launch_bounds__(256) void k_prepare(int64_t M, const uint32_t *__re
         st[a] = (int)wrapped_start(start, g.ax[a].n);
         off[a] = (T)o;
     }
+    if (dense_before)
+        dst = dense_slot(dst, bin_of_starts<D>(st, g), M, g.nbins, offsets, dense_before);
     rec_start[dst] = make_int4(st[0], st[1], st[2], (int)idx);
     if constexpr (sizeof(T) == 4) {
         *(float4 *)(rec_off + dst * 4) = make_float4(off[0], off[1], off[2], off[3]);
@@ -974,18 +1005,16 @@ __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
     if (i < n) p[i] = v;
 }
 
-namespace {
-
 // Exclusive scan of n entries into out (which may be in): tile sums, then
 // every tile's scan plus the sum of the tiles before it.
+int64_t scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
+
 void scan_u32(const uint32_t *in, int64_t n, uint32_t *sums, uint32_t *out, hipStream_t stream)
 {
-    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    const int64_t tiles = scan_tiles(n);
     k_scan_reduce<<<(unsigned)tiles, kScanThreads, 0, stream>>>(in, n, sums);
     k_scan_down<<<(unsigned)tiles, kScanThreads, 0, stream>>>(in, n, sums, out);
 }
-
-}  // namespace
 
 // LSD radix passes over `bits` key bits: sort keys end in sorted_keys_,
 // point indices in sorted_.
@@ -1092,7 +1121,8 @@ bool BinSorter::sort(int64_t M, const T *coords, const GridParams &g, BinOrder o
 
 template <int D, typename T>
 bool BinSorter::prepare(int64_t M, const T *coords, const GridParams &g, BinOrder order,
-                        int4 *rec_start, T *rec_off, uint32_t *out_sorted, hipStream_t stream)
+                        int4 *rec_start, T *rec_off, uint32_t *out_sorted, hipStream_t stream,
+                        const uint32_t *dense_before)
 {
     if (M == 0) return true;
     // the radix sort leaves every bin in index order: the grouped and the
@@ -1100,11 +1130,12 @@ bool BinSorter::prepare(int64_t M, const T *coords, const GridParams &g, BinOrde
     if (order != BinOrder::Canonical)
         k_prepare_grouped<D, T><<<grid_for(M, 256), 256, 0, stream>>>(M, sorted_, coords, g,
                                                                       rec_start, rec_off,
-                                                                      out_sorted);
+                                                                      out_sorted, offsets_,
+                                                                      dense_before);
     else
         k_prepare<D, T><<<grid_for(M, 256), 256, 0, stream>>>(
             M, sorted_, (const uint64_t *)sorted_keys_, (const SortElem *)scratch_a_, coords, g,
-            rec_start, rec_off, out_sorted);
+            rec_start, rec_off, out_sorted, offsets_, dense_before);
     return hip_ok(hipGetLastError(), "prepare launch");
 }
 
@@ -1121,7 +1152,8 @@ bool gather_strengths(int64_t M, int64_t vectors, const uint32_t *perm, const T 
     template bool BinSorter::sort<D, T>(int64_t, const T *, const GridParams &, BinOrder,          \
                                         hipStream_t);                                             \
     template bool BinSorter::prepare<D, T>(int64_t, const T *, const GridParams &, BinOrder,       \
-                                           int4 *, T *, uint32_t *, hipStream_t);
+                                           int4 *, T *, uint32_t *, hipStream_t,                 \
+                                           const uint32_t *);
 MNUFFT_BINS_INSTANTIATE(1, float)
 MNUFFT_BINS_INSTANTIATE(2, float)
 MNUFFT_BINS_INSTANTIATE(3, float)

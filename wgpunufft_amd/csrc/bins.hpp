@@ -23,10 +23,14 @@ public:
 
     // Prepared records in final slot order: rec_start[slot] = (wrapped start
     // per axis, original index), rec_off[4 * slot + axis] = start - position.
-    // out_sorted (optional) receives the final permutation.
+    // out_sorted (optional) receives the final permutation.  With
+    // dense_before (dense records before each bin, nbins + 1 values; a bin
+    // with any is dense) the light bins' records come first, in bin order,
+    // then the dense bins' records, in bin order.
     template <int D, typename T>
     bool prepare(int64_t M, const T *coords, const GridParams &g, BinOrder order,
-                 int4 *rec_start, T *rec_off, uint32_t *out_sorted, hipStream_t stream);
+                 int4 *rec_start, T *rec_off, uint32_t *out_sorted, hipStream_t stream,
+                 const uint32_t *dense_before = nullptr);
 
     const uint32_t *sorted() const { return sorted_; }
     const uint32_t *offsets() const { return offsets_; }
@@ -56,6 +60,11 @@ private:
     int64_t bins_cap_ = 0, points_cap_ = 0, sort_scratch_cap_ = 0;
     int64_t nbins_ = 0, M_ = 0;
 };
+
+// Exclusive scan of n u32 values (wrapping) into out (may alias in); sums
+// holds scan_tiles(n) values of scratch.
+void scan_u32(const uint32_t *in, int64_t n, uint32_t *sums, uint32_t *out, hipStream_t stream);
+int64_t scan_tiles(int64_t n);
 
 // c_sorted[v][s] = c[v][perm[s]] for every vector v.
 template <typename T>

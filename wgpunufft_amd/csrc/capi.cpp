@@ -400,6 +400,18 @@ int mnufft_set_spread_tuning(mnufft_plan plan, int64_t light_limit, int64_t min_
     return MNUFFT_SUCCESS;
 }
 
+int mnufft_set_dense_bins(mnufft_plan plan, int64_t min_records)
+{
+    if (!plan || min_records < 0 || min_records > 0x7fffffff) {
+        set_error("invalid dense-bin threshold");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    Plan &p = *plan->plan;
+    Plan &q = p.type == 3 && p.outer ? *p.outer : p;
+    q.dense.threshold = (uint32_t)min_records;
+    return MNUFFT_SUCCESS;
+}
+
 int mnufft_direct_f64(int32_t kind, int32_t dim, const int64_t *n_modes, int32_t isign,
                       int32_t mode_order, int64_t M, const double *d_x, const double *d_in,
                       int64_t nout, const int64_t *d_sel, const double *d_s, double *d_out,

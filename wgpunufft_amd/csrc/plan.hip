@@ -410,13 +410,15 @@ bool Plan::set_points_t(int64_t M, const T *pts, hipStream_t stream)
         gp = spread_geometry(M, &geometry_index);
         gp_rows = rows_geometry();
         binned_order = point_order;
+        binned_dense = sizeof(T) == 4 ? dense_threshold(gp) : 0u;
         mark(kStageSort, false, stream);
         if (!sorter.sort<D, T>(M, pts, gp, point_order, stream)) return false;
         mark(kStageSort, true, stream);
         if (!ensure_records(M)) return false;
         mark(kStagePrepare, false, stream);
+        if (binned_dense && !dense.classify(gp, sorter.offsets(), stream)) return false;
         if (!sorter.prepare<D, T>(M, pts, gp, point_order, rec_start, (T *)rec_off, rec_perm,
-                                  stream))
+                                  stream, binned_dense ? dense.before : nullptr))
             return false;
         mark(kStagePrepare, true, stream);
     } else {
@@ -440,7 +442,8 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         if (M == 0) return fill_zero<T>(out, 2 * mode_total * active, stream);
         // a spread method that needs other bins, or another in-bin order,
         // re-bins the kept points
-        if ((gp_rows != rows_geometry() || binned_order != point_order) &&
+        if ((gp_rows != rows_geometry() || binned_order != point_order ||
+             binned_dense != (sizeof(T) == 4 ? dense_threshold(gp) : 0u)) &&
             !set_points_t<D, T>(M, (const T *)points, stream))
             return false;
         if (!ensure_strengths(M, active)) return false;
@@ -450,10 +453,10 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         spreader.kernel_end = timed ? slots[slot].ev[2 * kStageSpreadKernel + 1] : nullptr;
         if (timed) slots[slot].used[kStageSpreadKernel] = true;
         if (!gather_strengths<T>(M, active, rec_perm, in, (T *)c_sorted, stream)) return false;
-        if (!spreader.spread<D, T>(gp, spread_params(), sorter.offsets(), rec_start,
-                                   (const T *)rec_off,
-                                   (const T *)c_sorted, M, active, (T *)grid, stream,
-                                   spread_method))
+        if (!spreader.spread<D, T>(gp, spread_params(),
+                                   binned_dense ? dense.light : sorter.offsets(), rec_start,
+                                   (const T *)rec_off, (const T *)c_sorted, M, active, (T *)grid,
+                                   stream, spread_method, binned_dense ? &dense : nullptr))
             return false;
         mark(kStageSpread, true, stream);
         if (spread_only) return true;

@@ -53,6 +53,18 @@ struct Plan {
     BinOrder point_order = BinOrder::Index;
     BinOrder binned_order = BinOrder::Index;  // order of the current binning
     bool gp_rows = false;  // points binned in one-cell rows (row tiles)
+    // Dense bins of the 3D matrix-core spread (records of dense bins kept
+    // apart); binned_dense is the threshold of the current binning, 0 when
+    // the binning has none.
+    DenseBins dense;
+    uint32_t binned_dense = 0;
+    uint32_t dense_threshold(const GridParams &g) const
+    {
+        if (type != 1 || dim != 3 || !sweep || spread_method == SpreadMethod::Plain ||
+            precision != Precision::F32 || dense.threshold == 0 || DenseBins::x_blocks(g) == 0)
+            return 0;
+        return dense.threshold;
+    }
     // Type-1 spreads on row tiles: the 3D row sweep, or a 2D matrix-core method.
     bool rows_geometry() const
     {

@@ -56,6 +56,27 @@ struct BlockReach {
         *first = offsets[row + lo];
         *last = offsets[row + lo + len];
     }
+    // segment t -> bins [first, last)
+    __device__ void segment_bins(int64_t t, const GridParams &g, int64_t *first,
+                                 int64_t *last) const
+    {
+        int64_t nxi = ax[0].len1 > 0 ? 2 : 1;
+        int64_t ix = t % nxi;
+        int64_t rest = t / nxi;
+        int64_t row = 0, stride = g.ax[0].bins;
+#pragma unroll
+        for (int a = 1; a < D; ++a) {
+            int64_t cnt = ax[a].count();
+            int64_t k = rest % cnt;
+            rest /= cnt;
+            row += ax[a].bin(k) * stride;
+            stride *= g.ax[a].bins;
+        }
+        int64_t lo = ix == 0 ? ax[0].lo0 : ax[0].lo1;
+        int64_t len = ix == 0 ? ax[0].len0 : ax[0].len1;
+        *first = row + lo;
+        *last = row + lo + len;
+    }
 };
 
 template <int D>
@@ -80,25 +101,36 @@ __device__ inline BlockReach<D> block_reach(int64_t block, const GridParams &g,
 }
 
 // Records in reach of every block: one wavefront per block, lanes over the
-// block's bin segments.
+// block's bin segments.  With dense bins (dense_before: dense records before
+// each bin) touched[b] also tells whether a dense bin reaches the block.
 template <int D>
 __global__ __launch_bounds__(256) void k_classify(GridParams g, SpreadParams sp,
                                                   const uint32_t *__restrict__ offsets,
-                                                  uint32_t *__restrict__ totals)
+                                                  uint32_t *__restrict__ totals,
+                                                  const uint32_t *__restrict__ dense_before,
+                                                  uint32_t *__restrict__ touched)
 {
     const int lane = threadIdx.x & 63;
     const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (b >= sp.block_count) return;
     BlockReach<D> r = block_reach<D>(b, g, sp);
     uint64_t total = 0;
+    uint32_t dense = 0;
     for (int64_t t = lane; t < r.nseg; t += 64) {
         int64_t f, l;
-        r.segment(t, g, offsets, &f, &l);
-        total += (uint64_t)(l - f);
+        r.segment_bins(t, g, &f, &l);
+        total += (uint64_t)(offsets[l] - offsets[f]);
+        if (dense_before) dense |= dense_before[l] - dense_before[f];
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o, 64);
-    if (lane == 0) totals[b] = (uint32_t)(total > 0xffffffffull ? 0xffffffffull : total);
+    for (int o = 32; o > 0; o >>= 1) {
+        total += __shfl_xor(total, o, 64);
+        dense |= __shfl_xor(dense, o, 64);
+    }
+    if (lane == 0) {
+        totals[b] = (uint32_t)(total > 0xffffffffull ? 0xffffffffull : total);
+        if (touched) touched[b] = dense ? 1u : 0u;
+    }
 }
 
 // One workgroup plans the work items: a light block is one item, a heavy
@@ -985,7 +1017,7 @@ template <int D>
 bool chunk_mfma(Spreader &s, SpreadMethod method, const GridParams &g, const SpreadParams &sp,
                 const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
                 const float *c_sorted, int64_t M, int64_t vectors, const uint32_t *vmax,
-                float *grid, hipStream_t stream)
+                float *grid, bool mark_end, hipStream_t stream)
 {
     constexpr int TE = D == 3 ? kSweepTile : 32;  // tile edge along x and y
     const dim3 ig = items_grid(sp.block_count, vectors);
@@ -993,7 +1025,7 @@ bool chunk_mfma(Spreader &s, SpreadMethod method, const GridParams &g, const Spr
     if (!launch_mfma_tiles(method, D, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, w,
                            grid, (float *)s.partial, stream))
         return false;
-    if (s.kernel_end) (void)hipEventRecord(s.kernel_end, stream);
+    if (mark_end && s.kernel_end) (void)hipEventRecord(s.kernel_end, stream);
     const dim3 rg(2048, (unsigned)vectors);
     if (D == 2)
         k_reduce_tiles<float, TE, TE, 1><<<rg, 256, 0, stream>>>(
@@ -1017,9 +1049,9 @@ Spreader::~Spreader() { release(); }
 void Spreader::release()
 {
     for (void *p : {(void *)totals, (void *)parts, (void *)item_off, (void *)heavy_base,
-                    (void *)heavy_list, (void *)ctrl, (void *)vmax, partial})
+                    (void *)heavy_list, (void *)touched, (void *)ctrl, (void *)vmax, partial})
         if (p) (void)hipFree(p);
-    totals = parts = item_off = heavy_base = heavy_list = ctrl = vmax = nullptr;
+    totals = parts = item_off = heavy_base = heavy_list = touched = ctrl = vmax = nullptr;
     partial = nullptr;
     blocks_cap = 0;
     partial_bytes = 0;
@@ -1035,7 +1067,7 @@ bool Spreader::ensure(int64_t nblocks, int64_t partial_need, int64_t vectors)
         vmax_cap = vectors;
     }
     if (nblocks > blocks_cap) {
-        for (uint32_t **p : {&totals, &parts, &item_off, &heavy_base, &heavy_list}) {
+        for (uint32_t **p : {&totals, &parts, &item_off, &heavy_base, &heavy_list, &touched}) {
             if (*p) (void)hipFree(*p);
             *p = nullptr;
             if (!hip_ok(hipMalloc(p, sizeof(uint32_t) * (nblocks + 1)), "spread block arrays"))
@@ -1057,7 +1089,8 @@ bool Spreader::ensure(int64_t nblocks, int64_t partial_need, int64_t vectors)
 template <int D, typename T>
 bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
                       const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
-                      int64_t vectors, T *grid, hipStream_t stream, SpreadMethod method)
+                      int64_t vectors, T *grid, hipStream_t stream, SpreadMethod method,
+                      DenseBins *dense)
 {
     const bool tiles = sp.run == 0;
     const bool mfma = method != SpreadMethod::Plain;
@@ -1089,13 +1122,23 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
         tiles ? sp.block[0] * sp.block[1] * sp.block[2] : (int64_t)kLanes * sp.run;
     const int64_t per_part = cells * 2 * (int64_t)sizeof(T);
     part_capacity = std::max<int64_t>(budget_bytes / per_part, 64);
-    const int64_t chunk =
+    if (dense && !(mfma && D == 3 && sizeof(T) == 4 && tiles && DenseBins::x_blocks(g))) {
+        set_error("dense bins need the 3D matrix-core spread");
+        return false;
+    }
+    int64_t chunk =
         std::max<int64_t>(1, std::min<int64_t>(vectors, chunk_budget_bytes /
                                                             (part_capacity * per_part)));
+    if (dense)
+        chunk = std::max<int64_t>(
+            1, std::min<int64_t>(chunk, chunk_budget_bytes / (dense->part_cap *
+                                                              DenseBins::window_floats(g) *
+                                                              (int64_t)sizeof(float))));
     if (!ensure(sp.block_count, part_capacity * per_part * chunk, vectors)) return false;
 
     const int64_t nb = sp.block_count;
-    k_classify<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(g, sp, offsets, totals);
+    k_classify<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(
+        g, sp, offsets, totals, dense ? dense->before : nullptr, dense ? touched : nullptr);
     k_fill_ctrl<<<1, 64, 0, stream>>>(ctrl);
     k_plan<<<1, 1024, 0, stream>>>(nb, totals, light_limit, min_part, part_capacity, parts,
                                    item_off, heavy_base, heavy_list, ctrl);
@@ -1112,9 +1155,14 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
         T *gr = grid + 2 * v0 * g.fine_total;
         bool ok = false;
         if (mfma) {
-            if constexpr ((D == 2 || D == 3) && sizeof(T) == 4)
+            if constexpr ((D == 2 || D == 3) && sizeof(T) == 4) {
                 ok = chunk_mfma<D>(*this, method, g, sp, offsets, rec_start, rec_off, cs, M, vc,
-                                   vmax + v0, gr, stream);
+                                   vmax + v0, gr, dense == nullptr, stream);
+                if (ok && dense)
+                    ok = dense->spread(method, g, sp, M, rec_start, rec_off, cs, vmax + v0, vc,
+                                       touched, gr, stream);
+                if (ok && dense && kernel_end) (void)hipEventRecord(kernel_end, stream);
+            }
         } else if (tiles) {
             if constexpr (D == 3) {
                 switch (g.width) {
@@ -1144,7 +1192,8 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
 #define MNUFFT_SPREAD_INSTANTIATE(D, T)                                                           \
     template bool Spreader::spread<D, T>(const GridParams &, const SpreadParams &,                 \
                                          const uint32_t *, const int4 *, const T *, const T *,     \
-                                         int64_t, int64_t, T *, hipStream_t, SpreadMethod);
+                                         int64_t, int64_t, T *, hipStream_t, SpreadMethod,         \
+                                         DenseBins *);
 MNUFFT_SPREAD_INSTANTIATE(1, float)
 MNUFFT_SPREAD_INSTANTIATE(2, float)
 MNUFFT_SPREAD_INSTANTIATE(3, float)

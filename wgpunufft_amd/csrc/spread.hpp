@@ -5,6 +5,40 @@
 
 namespace mnufft {
 
+// Dense bins of the 3D matrix-core spread (spread_dense.hip).  classify()
+// runs on sorted points: a bin of at least `threshold` records (the threshold
+// doubled, up to seven times, until the bins' window parts fit part_cap) is
+// dense.  Its records are then kept after the light records (prepare() with
+// `before`), the tile kernels see the light records only (offsets `light`),
+// and spread() sums each dense bin's window of (side + w - 1) cells per axis
+// in parts of 1024 records, adds the parts in order and adds the windows to
+// the cells of the touched tiles in a fixed bin order.
+struct DenseBins {
+    ~DenseBins();
+    void release();
+    // Column blocks of 8 x cells of the bins' windows (1 or 2), or 0 when
+    // the grid does not qualify (3D, bins dividing every axis, windows
+    // within 16 x 8 x 8 cells).
+    static int x_blocks(const GridParams &g);
+    static int64_t window_floats(const GridParams &g) { return 1024 * (int64_t)x_blocks(g); }
+    bool classify(const GridParams &g, const uint32_t *offsets, hipStream_t stream);
+    bool spread(SpreadMethod method, const GridParams &g, const SpreadParams &sp, int64_t M,
+                const int4 *rec_start, const float *rec_off, const float *c_sorted,
+                const uint32_t *cmax, int64_t vectors, const uint32_t *touched, float *grid,
+                hipStream_t stream);
+
+    uint32_t threshold = 32;   // records of the smallest dense bin (0: no dense bins)
+    int64_t part_cap = 16384;  // window parts (1024 records each) at most
+    uint32_t *before = nullptr;  // dense records before each bin (nbins + 1)
+    uint32_t *light = nullptr;   // light-record offsets (nbins + 1)
+    uint32_t *info = nullptr;    // 1 + first window part of a dense bin, 0 for a light bin
+    uint2 *parts = nullptr;      // (bin, part) of every window part
+    uint32_t *ctrl = nullptr;    // parts per threshold level, threshold, parts
+    uint32_t *sums = nullptr;    // scan scratch
+    float *win = nullptr;        // window parts, part_cap per vector
+    int64_t bins_cap = 0, parts_alloc = 0, win_bytes = 0;
+};
+
 // Device buffers of the spreader's work plan, reused across executions.
 struct Spreader {
     ~Spreader();
@@ -15,11 +49,13 @@ struct Spreader {
     // vectors onto `grid` (vectors * fine_total complex values).  Every cell
     // is written exactly once.
     // A matrix-core method needs FP32 and row-sweep tiles (run == 0).
+    // With `dense`, offsets are the light-record offsets and the dense bins'
+    // windows are added after the tile kernels.
     template <int D, typename T>
     bool spread(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
                 const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
                 int64_t vectors, T *grid, hipStream_t stream,
-                SpreadMethod method = SpreadMethod::Plain);
+                SpreadMethod method = SpreadMethod::Plain, DenseBins *dense = nullptr);
 
     // Optional events recorded right before the first and after the last
     // spread kernel launch.
@@ -30,6 +66,7 @@ struct Spreader {
     uint32_t *item_off = nullptr;    // first work item per block
     uint32_t *heavy_base = nullptr;  // first partial slot per heavy block
     uint32_t *heavy_list = nullptr;  // heavy blocks
+    uint32_t *touched = nullptr;     // blocks reached by a dense bin
     uint32_t *ctrl = nullptr;        // items, heavy parts, heavy blocks, part size
     uint32_t *vmax = nullptr;        // largest strength magnitude per vector (float bits)
     void *partial = nullptr;

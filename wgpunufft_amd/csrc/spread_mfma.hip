@@ -31,6 +31,7 @@
 // keeps the FP16 operands in range; the scale is undone exactly on output.
 #include <hip/hip_runtime.h>
 
+#include "mfma_ops.hpp"
 #include "spread.hpp"
 #include "spread_device.hpp"
 
@@ -38,10 +39,7 @@ namespace mnufft {
 
 namespace {
 
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
+using namespace mfma;
 
 constexpr int kTile = 16;
 constexpr int kPitch = 20;  // LDS pitch of the weight tables, in floats
@@ -51,68 +49,6 @@ struct MfmaRec {
     float ox, oy, oz;
     float cre, cim;  // scaled strength
 };
-
-template <int MODE>
-struct Operand;
-
-template <>
-struct Operand<(int)SpreadMethod::MfmaF16> {
-    using V = f16x8;
-    __device__ static void split(const f32x8 &a, V &hi, V &lo)
-    {
-        hi = __builtin_convertvector(a, V);
-        (void)lo;
-    }
-    __device__ static f32x16 mma(const V &a, const V &b, const f32x16 &c)
-    {
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
-    }
-};
-
-template <>
-struct Operand<(int)SpreadMethod::MfmaF16x3> {
-    using V = f16x8;
-    __device__ static void split(const f32x8 &a, V &hi, V &lo)
-    {
-        hi = __builtin_convertvector(a, V);
-        lo = __builtin_convertvector(a - __builtin_convertvector(hi, f32x8), V);
-    }
-    __device__ static f32x16 mma(const V &a, const V &b, const f32x16 &c)
-    {
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
-    }
-};
-
-template <>
-struct Operand<(int)SpreadMethod::MfmaBf16x3> {
-    using V = bf16x8;
-    __device__ static void split(const f32x8 &a, V &hi, V &lo)
-    {
-        hi = __builtin_convertvector(a, V);
-        lo = __builtin_convertvector(a - __builtin_convertvector(hi, f32x8), V);
-    }
-    __device__ static f32x16 mma(const V &a, const V &b, const f32x16 &c)
-    {
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
-    }
-};
-
-__device__ inline f32x8 load8(const float *p)
-{
-    const float4 a = *(const float4 *)p;
-    const float4 b = *(const float4 *)(p + 4);
-    return f32x8{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-}
-
-// Wavefront-wide LDS ordering for the one-wavefront workgroups here: LDS
-// instructions of a wavefront execute in order, so a compiler barrier is
-// enough (no s_barrier, no wait for outstanding global loads, which stay in
-// flight across it).
-__device__ inline void wave_sync()
-{
-    __builtin_amdgcn_wave_barrier();
-    __asm__ volatile("" ::: "memory");
-}
 
 // Every reach row of a tile (y start, then z start), staged once per work
 // item: its one or two slot ranges along x (the second when the reach wraps)
@@ -228,20 +164,6 @@ __device__ inline Fetched fetch_record(const TileRows<NROWS> &rows, int64_t k, i
         f.c = *(const float2 *)(cv + 2 * (int64_t)slot);
     }
     return f;
-}
-
-// Power-of-two scale that brings the largest strength magnitude below one.
-__device__ inline void strength_scale(const uint32_t *cmax, float *scale, float *unscale)
-{
-    *scale = 1.0f;
-    *unscale = 1.0f;
-    const float m = __uint_as_float(cmax[0]);
-    if (m > 0.0f && m <= 3.4028234663852886e38f) {
-        int e = 0;
-        (void)frexpf(m, &e);
-        *scale = ldexpf(1.0f, -e);
-        *unscale = ldexpf(1.0f, e);
-    }
 }
 
 // Work item -> (tile, part, record range) of the tile plan.
