@@ -901,15 +901,17 @@ __global__ __launch_bounds__(256) void k_prepare(int64_t M, const uint32_t *__re
     if (out_sorted) out_sorted[dst] = idx;
 }
 
-// Strengths of every vector gathered into record order.
+// Strengths of every vector gathered into record order (the first
+// M - *tail slots when tail is given).
 template <typename T>
 __global__ __launch_bounds__(256) void k_gather_strengths(int64_t M, int64_t vectors,
                                                           const uint32_t *__restrict__ perm,
                                                           const T *__restrict__ c,
-                                                          T *__restrict__ c_sorted)
+                                                          T *__restrict__ c_sorted,
+                                                          const uint32_t *__restrict__ tail)
 {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
+    if (s >= M - (tail ? (int64_t)*tail : 0)) return;
     int64_t idx = perm[s];
     for (int64_t v = 0; v < vectors; ++v) {
         const T *src = c + 2 * (v * M + idx);
@@ -1141,10 +1143,11 @@ bool BinSorter::prepare(int64_t M, const T *coords, const GridParams &g, BinOrde
 
 template <typename T>
 bool gather_strengths(int64_t M, int64_t vectors, const uint32_t *perm, const T *c, T *c_sorted,
-                      hipStream_t stream)
+                      hipStream_t stream, const uint32_t *tail)
 {
     if (M == 0) return true;
-    k_gather_strengths<T><<<grid_for(M, 256), 256, 0, stream>>>(M, vectors, perm, c, c_sorted);
+    k_gather_strengths<T><<<grid_for(M, 256), 256, 0, stream>>>(M, vectors, perm, c, c_sorted,
+                                                                tail);
     return hip_ok(hipGetLastError(), "strength gather launch");
 }
 
@@ -1161,8 +1164,8 @@ MNUFFT_BINS_INSTANTIATE(1, double)
 MNUFFT_BINS_INSTANTIATE(2, double)
 MNUFFT_BINS_INSTANTIATE(3, double)
 template bool gather_strengths<float>(int64_t, int64_t, const uint32_t *, const float *,
-                                      float *, hipStream_t);
+                                      float *, hipStream_t, const uint32_t *);
 template bool gather_strengths<double>(int64_t, int64_t, const uint32_t *, const double *,
-                                       double *, hipStream_t);
+                                       double *, hipStream_t, const uint32_t *);
 
 }  // namespace mnufft

@@ -66,9 +66,10 @@ private:
 void scan_u32(const uint32_t *in, int64_t n, uint32_t *sums, uint32_t *out, hipStream_t stream);
 int64_t scan_tiles(int64_t n);
 
-// c_sorted[v][s] = c[v][perm[s]] for every vector v.
+// c_sorted[v][s] = c[v][perm[s]] for every vector v, for the first M - *tail
+// slots when tail (a device value) is given, else all M.
 template <typename T>
 bool gather_strengths(int64_t M, int64_t vectors, const uint32_t *perm, const T *c, T *c_sorted,
-                      hipStream_t stream);
+                      hipStream_t stream, const uint32_t *tail = nullptr);
 
 }  // namespace mnufft

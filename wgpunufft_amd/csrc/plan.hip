@@ -452,7 +452,11 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         spreader.kernel_start = timed ? slots[slot].ev[2 * kStageSpreadKernel] : nullptr;
         spreader.kernel_end = timed ? slots[slot].ev[2 * kStageSpreadKernel + 1] : nullptr;
         if (timed) slots[slot].used[kStageSpreadKernel] = true;
-        if (!gather_strengths<T>(M, active, rec_perm, in, (T *)c_sorted, stream)) return false;
+        // dense-bin records read their strengths in point order
+        if (!gather_strengths<T>(M, active, rec_perm, in, (T *)c_sorted, stream,
+                                 binned_dense ? dense.before + gp.nbins : nullptr))
+            return false;
+        if constexpr (sizeof(T) == 4) dense.strengths = in;
         if (!spreader.spread<D, T>(gp, spread_params(),
                                    binned_dense ? dense.light : sorter.offsets(), rec_start,
                                    (const T *)rec_off, (const T *)c_sorted, M, active, (T *)grid,

@@ -1145,7 +1145,11 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
     if constexpr (sizeof(T) == 4) {
         if (mfma) {
             k_fill_u32s<<<grid_for(vectors, 256), 256, 0, stream>>>(vmax, vectors);
-            if (!launch_absmax(c_sorted, 2 * M, vectors, vmax, stream)) return false;
+            // the largest magnitude of the caller's vectors (with dense bins
+            // only the light records are gathered)
+            if (!launch_absmax(dense ? dense->strengths : c_sorted, 2 * M, vectors, vmax,
+                               stream))
+                return false;
         }
     }
     if (kernel_start) (void)hipEventRecord(kernel_start, stream);
@@ -1159,8 +1163,9 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
                 ok = chunk_mfma<D>(*this, method, g, sp, offsets, rec_start, rec_off, cs, M, vc,
                                    vmax + v0, gr, dense == nullptr, stream);
                 if (ok && dense)
-                    ok = dense->spread(method, g, sp, M, rec_start, rec_off, cs, vmax + v0, vc,
-                                       touched, gr, stream);
+                    ok = dense->spread(method, g, sp, M, rec_start, rec_off,
+                                       dense->strengths + 2 * v0 * M, vmax + v0, vc, touched,
+                                       gr, stream);
                 if (ok && dense && kernel_end) (void)hipEventRecord(kernel_end, stream);
             }
         } else if (tiles) {

@@ -22,12 +22,15 @@ struct DenseBins {
     static int x_blocks(const GridParams &g);
     static int64_t window_floats(const GridParams &g) { return 1024 * (int64_t)x_blocks(g); }
     bool classify(const GridParams &g, const uint32_t *offsets, hipStream_t stream);
+    // The dense records' strengths are read from the caller's vectors
+    // (strengths, point order) through the original index of each record.
     bool spread(SpreadMethod method, const GridParams &g, const SpreadParams &sp, int64_t M,
-                const int4 *rec_start, const float *rec_off, const float *c_sorted,
+                const int4 *rec_start, const float *rec_off, const float *c,
                 const uint32_t *cmax, int64_t vectors, const uint32_t *touched, float *grid,
                 hipStream_t stream);
 
     uint32_t threshold = 32;   // records of the smallest dense bin (0: no dense bins)
+    const float *strengths = nullptr;  // the execution's strength vectors, point order
     int64_t part_cap = 16384;  // window parts (1024 records each) at most
     uint32_t *before = nullptr;  // dense records before each bin (nbins + 1)
     uint32_t *light = nullptr;   // light-record offsets (nbins + 1)

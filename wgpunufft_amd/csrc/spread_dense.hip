@@ -131,7 +131,9 @@ __global__ __launch_bounds__(256) void k_dense_finish(const uint32_t *__restrict
     info[b] = tag;
 }
 
-// One lane's record of a k-block, its loads in flight until used.
+// One lane's record of a k-block, its loads in flight until used: the
+// record two k-blocks ahead, its strength (through the record's original
+// index) one k-block ahead.
 struct DenseRec {
     int4 st;
     float4 off;
@@ -140,7 +142,7 @@ struct DenseRec {
 };
 
 __device__ inline DenseRec dense_fetch(int64_t slot, int64_t end, const int4 *__restrict__ rec_start,
-                                       const float *__restrict__ rec_off, const float *__restrict__ cv)
+                                       const float *__restrict__ rec_off)
 {
     DenseRec r;
     r.valid = slot < end;
@@ -150,9 +152,13 @@ __device__ inline DenseRec dense_fetch(int64_t slot, int64_t end, const int4 *__
     if (r.valid) {
         r.st = rec_start[slot];
         r.off = *(const float4 *)(rec_off + 4 * slot);
-        r.c = *(const float2 *)(cv + 2 * slot);
     }
     return r;
+}
+
+__device__ inline void dense_fetch_strength(DenseRec &r, const float *__restrict__ cv)
+{
+    if (r.valid) r.c = *(const float2 *)(cv + 2 * (int64_t)(uint32_t)r.st.w);
 }
 
 // Window part sums: one wavefront per part (grid-stride), 32 records per
@@ -163,7 +169,7 @@ template <int MODE, int NXB>
 __global__ __launch_bounds__(64) void k_dense_windows(
     GridParams g, const uint2 *__restrict__ parts, const uint32_t *__restrict__ ctrl,
     const uint32_t *__restrict__ before, int64_t M, const int4 *__restrict__ rec_start,
-    const float *__restrict__ rec_off, const float *__restrict__ c_sorted,
+    const float *__restrict__ rec_off, const float *__restrict__ c,
     const uint32_t *__restrict__ cmax, float *__restrict__ win, int64_t win_cap)
 {
     using Op = Operand<MODE>;
@@ -186,7 +192,7 @@ __global__ __launch_bounds__(64) void k_dense_windows(
     const float beta_l2e = (float)(g.beta * 1.4426950408889634);
     const int64_t nparts = ctrl[kCtrlParts];
     const int64_t light_total = M - (int64_t)before[g.nbins];
-    const float *cv = c_sorted + 2 * v * M;
+    const float *cv = c + 2 * v * M;
     float *wv = win + v * win_cap * WF;
 
     for (int64_t part = blockIdx.x; part < nparts; part += gridDim.x) {
@@ -211,10 +217,14 @@ __global__ __launch_bounds__(64) void k_dense_windows(
         for (int xb = 0; xb < NXB; ++xb)
 #pragma unroll
             for (int m = 0; m < 4; ++m) acc[xb][m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        DenseRec nx = dense_fetch(s0 + j, s1, rec_start, rec_off, cv);
+        DenseRec nx = dense_fetch(s0 + j, s1, rec_start, rec_off);
+        DenseRec nx2 = dense_fetch(s0 + 32 + j, s1, rec_start, rec_off);
+        dense_fetch_strength(nx, cv);
         for (int64_t k0 = s0; k0 < s1; k0 += 32) {
             const DenseRec cur = nx;
-            nx = dense_fetch(k0 + 32 + j, s1, rec_start, rec_off, cv);
+            nx = nx2;
+            dense_fetch_strength(nx, cv);
+            nx2 = dense_fetch(k0 + 64 + j, s1, rec_start, rec_off);
             const int dx = cur.st.x - org[0], dy = cur.st.y - org[1], dz = cur.st.z - org[2];
 #pragma unroll
             for (int i = 0; i < EH; ++i) {
@@ -442,7 +452,7 @@ __global__ __launch_bounds__(256) void k_dense_add(GridParams g, SpreadParams sp
 
 template <int MODE, int NXB>
 void launch_dense(const GridParams &g, const SpreadParams &sp, const DenseBins &d, int64_t M,
-                  const int4 *rec_start, const float *rec_off, const float *c_sorted,
+                  const int4 *rec_start, const float *rec_off, const float *c,
                   const uint32_t *cmax, int64_t vectors, const uint32_t *touched, float *grid,
                   hipStream_t stream)
 {
@@ -451,7 +461,7 @@ void launch_dense(const GridParams &g, const SpreadParams &sp, const DenseBins &
     // part_cap windows per vector and a zero window
     const int64_t win_cap = d.part_cap + 1;
     k_dense_windows<MODE, NXB><<<dim3(pg, (unsigned)vectors), 64, 0, stream>>>(
-        g, d.parts, d.ctrl, d.before, M, rec_start, rec_off, c_sorted, cmax, d.win, win_cap);
+        g, d.parts, d.ctrl, d.before, M, rec_start, rec_off, c, cmax, d.win, win_cap);
     k_dense_reduce<NXB><<<dim3(std::min<unsigned>(pg, 8192), (unsigned)vectors), 256, 0,
                           stream>>>(d.parts, d.ctrl, d.before, d.win, win_cap);
     const unsigned ag = (unsigned)std::min<int64_t>(sp.block_count * sp.block[2], 16384);
@@ -462,22 +472,22 @@ void launch_dense(const GridParams &g, const SpreadParams &sp, const DenseBins &
 template <int NXB>
 bool launch_dense_mode(SpreadMethod method, const GridParams &g, const SpreadParams &sp,
                        const DenseBins &d, int64_t M, const int4 *rec_start,
-                       const float *rec_off, const float *c_sorted, const uint32_t *cmax,
+                       const float *rec_off, const float *c, const uint32_t *cmax,
                        int64_t vectors, const uint32_t *touched, float *grid, hipStream_t stream)
 {
     switch (method) {
     case SpreadMethod::MfmaF16:
-        launch_dense<(int)SpreadMethod::MfmaF16, NXB>(g, sp, d, M, rec_start, rec_off, c_sorted,
+        launch_dense<(int)SpreadMethod::MfmaF16, NXB>(g, sp, d, M, rec_start, rec_off, c,
                                                       cmax, vectors, touched, grid, stream);
         return true;
     case SpreadMethod::MfmaF16x3:
         launch_dense<(int)SpreadMethod::MfmaF16x3, NXB>(g, sp, d, M, rec_start, rec_off,
-                                                        c_sorted, cmax, vectors, touched, grid,
+                                                        c, cmax, vectors, touched, grid,
                                                         stream);
         return true;
     case SpreadMethod::MfmaBf16x3:
         launch_dense<(int)SpreadMethod::MfmaBf16x3, NXB>(g, sp, d, M, rec_start, rec_off,
-                                                         c_sorted, cmax, vectors, touched, grid,
+                                                         c, cmax, vectors, touched, grid,
                                                          stream);
         return true;
     default:
@@ -565,7 +575,7 @@ bool DenseBins::classify(const GridParams &g, const uint32_t *offsets, hipStream
 
 bool DenseBins::spread(SpreadMethod method, const GridParams &g, const SpreadParams &sp,
                        int64_t M, const int4 *rec_start, const float *rec_off,
-                       const float *c_sorted, const uint32_t *cmax, int64_t vectors,
+                       const float *c, const uint32_t *cmax, int64_t vectors,
                        const uint32_t *touched, float *grid, hipStream_t stream)
 {
     const int nxb = x_blocks(g);
@@ -581,9 +591,9 @@ bool DenseBins::spread(SpreadMethod method, const GridParams &g, const SpreadPar
         win_bytes = need;
     }
     const bool ok = nxb == 1 ? launch_dense_mode<1>(method, g, sp, *this, M, rec_start, rec_off,
-                                                    c_sorted, cmax, vectors, touched, grid, stream)
+                                                    c, cmax, vectors, touched, grid, stream)
                              : launch_dense_mode<2>(method, g, sp, *this, M, rec_start, rec_off,
-                                                    c_sorted, cmax, vectors, touched, grid, stream);
+                                                    c, cmax, vectors, touched, grid, stream);
     if (!ok) {
         set_error("dense bins need a matrix-core spread method");
         return false;
