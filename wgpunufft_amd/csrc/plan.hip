@@ -352,6 +352,17 @@ FftPlan *Plan::fft_for(int64_t vectors)
     return raw;
 }
 
+// The rocFFT plans (and work buffers) an FFT of `vectors` vectors uses,
+// created right after the grid so that later scratch (the dense-bin windows)
+// does not change where they land.
+bool Plan::ensure_fft(int64_t vectors)
+{
+    const int64_t per_exec = std::max<int64_t>(1, ((int64_t)1 << 31) / fine_total);
+    for (int64_t v0 = 0; v0 < vectors; v0 += per_exec)
+        if (!fft_for(std::min<int64_t>(per_exec, vectors - v0))) return false;
+    return true;
+}
+
 bool Plan::fft(int64_t vectors, void *data, hipStream_t stream)
 {
     // Batches of at most 2^31 complex values per rocFFT execution: larger
@@ -436,7 +447,7 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
 {
     if (fresh_marks) reset_marks();
     const int64_t M = point_count;
-    if (!ensure_grid(active)) return false;
+    if (!ensure_grid(active) || (!spread_only && !ensure_fft(active))) return false;
     ModeParams mp = mode_params();
     if (type == 1) {
         if (M == 0) return fill_zero<T>(out, 2 * mode_total * active, stream);

@@ -134,6 +134,7 @@ struct Plan {
     bool ensure_items(int64_t n);
     bool tiled_interp() const;
     FftPlan *fft_for(int64_t vectors);
+    bool ensure_fft(int64_t vectors);
     bool fft(int64_t vectors, void *data, hipStream_t stream);
     void mark(int stage, bool end, hipStream_t stream);
     void reset_marks();

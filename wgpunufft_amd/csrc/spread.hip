@@ -102,7 +102,8 @@ __device__ inline BlockReach<D> block_reach(int64_t block, const GridParams &g,
 
 // Records in reach of every block: one wavefront per block, lanes over the
 // block's bin segments.  With dense bins (dense_before: dense records before
-// each bin) touched[b] also tells whether a dense bin reaches the block.
+// each bin) the blocks a dense bin reaches are listed in touched[1..] (count
+// in touched[0], cleared by the caller; the list order is arbitrary).
 template <int D>
 __global__ __launch_bounds__(256) void k_classify(GridParams g, SpreadParams sp,
                                                   const uint32_t *__restrict__ offsets,
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(256) void k_classify(GridParams g, SpreadParams sp,
     }
     if (lane == 0) {
         totals[b] = (uint32_t)(total > 0xffffffffull ? 0xffffffffull : total);
-        if (touched) touched[b] = dense ? 1u : 0u;
+        if (touched && dense) touched[1 + atomicAdd(&touched[0], 1u)] = (uint32_t)b;
     }
 }
 
@@ -1137,6 +1138,7 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
     if (!ensure(sp.block_count, part_capacity * per_part * chunk, vectors)) return false;
 
     const int64_t nb = sp.block_count;
+    if (dense) k_fill_u32s<<<1, 64, 0, stream>>>(touched, 1);
     k_classify<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(
         g, sp, offsets, totals, dense ? dense->before : nullptr, dense ? touched : nullptr);
     k_fill_ctrl<<<1, 64, 0, stream>>>(ctrl);
@@ -1160,12 +1162,17 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
         bool ok = false;
         if (mfma) {
             if constexpr ((D == 2 || D == 3) && sizeof(T) == 4) {
-                ok = chunk_mfma<D>(*this, method, g, sp, offsets, rec_start, rec_off, cs, M, vc,
-                                   vmax + v0, gr, dense == nullptr, stream);
+                // dense windows first, so the grid the tile kernels write is
+                // still in the caches for the window adds and the FFT
+                ok = !dense || dense->spread(method, g, sp, M, rec_start, rec_off,
+                                             dense->strengths + 2 * v0 * M, vmax + v0, vc,
+                                             touched, gr, false, stream);
+                ok = ok && chunk_mfma<D>(*this, method, g, sp, offsets, rec_start, rec_off, cs,
+                                         M, vc, vmax + v0, gr, dense == nullptr, stream);
                 if (ok && dense)
                     ok = dense->spread(method, g, sp, M, rec_start, rec_off,
                                        dense->strengths + 2 * v0 * M, vmax + v0, vc, touched,
-                                       gr, stream);
+                                       gr, true, stream);
                 if (ok && dense && kernel_end) (void)hipEventRecord(kernel_end, stream);
             }
         } else if (tiles) {

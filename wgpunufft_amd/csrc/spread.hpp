@@ -8,7 +8,7 @@ namespace mnufft {
 // Dense bins of the 3D matrix-core spread (spread_dense.hip).  classify()
 // runs on sorted points: a bin of at least `threshold` records (the threshold
 // doubled, up to seven times, until the bins' window parts fit part_cap) is
-// dense.  Its records are then kept after the light records (prepare() with
+// dense, provided the dense bins then hold at least 1/8 of the records.  Its records are then kept after the light records (prepare() with
 // `before`), the tile kernels see the light records only (offsets `light`),
 // and spread() sums each dense bin's window of (side + w - 1) cells per axis
 // in parts of 1024 records, adds the parts in order and adds the windows to
@@ -22,12 +22,15 @@ struct DenseBins {
     static int x_blocks(const GridParams &g);
     static int64_t window_floats(const GridParams &g) { return 1024 * (int64_t)x_blocks(g); }
     bool classify(const GridParams &g, const uint32_t *offsets, hipStream_t stream);
-    // The dense records' strengths are read from the caller's vectors
-    // (strengths, point order) through the original index of each record.
+    // add == false: the windows of the dense bins (their parts summed, then
+    // added in part order), before the tile kernels; add == true: the
+    // windows added to the cells of the touched tiles, after them.  The
+    // dense records' strengths are read from the caller's vectors (point
+    // order) through the original index of each record.
     bool spread(SpreadMethod method, const GridParams &g, const SpreadParams &sp, int64_t M,
                 const int4 *rec_start, const float *rec_off, const float *c,
                 const uint32_t *cmax, int64_t vectors, const uint32_t *touched, float *grid,
-                hipStream_t stream);
+                bool add, hipStream_t stream);
 
     uint32_t threshold = 32;   // records of the smallest dense bin (0: no dense bins)
     const float *strengths = nullptr;  // the execution's strength vectors, point order
@@ -69,7 +72,7 @@ struct Spreader {
     uint32_t *item_off = nullptr;    // first work item per block
     uint32_t *heavy_base = nullptr;  // first partial slot per heavy block
     uint32_t *heavy_list = nullptr;  // heavy blocks
-    uint32_t *touched = nullptr;     // blocks reached by a dense bin
+    uint32_t *touched = nullptr;     // count, then the blocks a dense bin reaches
     uint32_t *ctrl = nullptr;        // items, heavy parts, heavy blocks, part size
     uint32_t *vmax = nullptr;        // largest strength magnitude per vector (float bits)
     void *partial = nullptr;

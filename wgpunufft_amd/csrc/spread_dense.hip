@@ -41,25 +41,29 @@ constexpr uint32_t kDensePart = 1024;  // records per window part
 constexpr int kDenseLevels = 8;        // thresholds t, 2t, ..., 128t
 constexpr int kDP = 36;                // LDS pitch of the weight rows (32 records + 4)
 // ctrl words: [0, 8) window parts needed at each threshold level, [8] the
-// threshold in use, [9] window parts handed out.
+// threshold in use, [9] window parts handed out, [16, 24) records of the
+// dense bins at each level.
 constexpr int kCtrlThreshold = 8;
 constexpr int kCtrlParts = 9;
+constexpr int kCtrlRecords = 16;  // [16, 24): records of the dense bins at each level
+constexpr int kDenseShare = 8;    // dense bins hold at least 1/8 of the records
 
 inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 __global__ void k_dense_clear(uint32_t *ctrl)
 {
-    if (threadIdx.x < 16) ctrl[threadIdx.x] = 0u;
+    if (threadIdx.x < 32) ctrl[threadIdx.x] = 0u;
 }
 
-// Window parts the dense bins would need at each threshold level.
+// Window parts (ctrl[i]) and records (ctrl[kCtrlRecords + i]) of the bins
+// that would be dense at each threshold level.
 __global__ __launch_bounds__(256) void k_dense_levels(const uint32_t *__restrict__ offsets,
                                                       int64_t nbins, uint32_t t0,
                                                       uint32_t *__restrict__ ctrl)
 {
-    uint32_t acc[kDenseLevels];
+    uint32_t acc[kDenseLevels], rec[kDenseLevels];
 #pragma unroll
-    for (int i = 0; i < kDenseLevels; ++i) acc[i] = 0;
+    for (int i = 0; i < kDenseLevels; ++i) acc[i] = rec[i] = 0;
     for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nbins;
          b += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t n = offsets[b + 1] - offsets[b];
@@ -67,23 +71,38 @@ __global__ __launch_bounds__(256) void k_dense_levels(const uint32_t *__restrict
             const uint32_t np = (n + kDensePart - 1) / kDensePart;
 #pragma unroll
             for (int i = 0; i < kDenseLevels; ++i)
-                if ((uint64_t)n >= ((uint64_t)t0 << i)) acc[i] += np;
+                if ((uint64_t)n >= ((uint64_t)t0 << i)) {
+                    acc[i] += np;
+                    rec[i] += n;
+                }
         }
     }
 #pragma unroll
     for (int i = 0; i < kDenseLevels; ++i) {
-        uint32_t x = acc[i];
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if ((threadIdx.x & 63) == 0 && x) atomicAdd(&ctrl[i], x);
+        uint32_t x = acc[i], y = rec[i];
+        for (int o = 32; o > 0; o >>= 1) {
+            x += __shfl_xor(x, o, 64);
+            y += __shfl_xor(y, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0 && x) {
+            atomicAdd(&ctrl[i], x);
+            atomicAdd(&ctrl[kCtrlRecords + i], y);
+        }
     }
 }
 
 // Smallest threshold level whose window parts fit the capacity (all-or-
-// nothing per level, so the choice depends on the bin counts only).
-__device__ inline uint32_t dense_threshold(const uint32_t *ctrl, uint32_t t0, uint32_t cap)
+// nothing per level, so the choice depends on the bin counts only); none
+// when the dense bins at that level hold less than 1/kDenseShare of the
+// records (the window sums would not pay for their adds).
+__device__ inline uint32_t dense_threshold(const uint32_t *ctrl, uint32_t t0, uint32_t cap,
+                                           int64_t M)
 {
     for (int i = 0; i < kDenseLevels; ++i)
-        if (ctrl[i] <= cap) return (uint32_t)min((uint64_t)t0 << i, (uint64_t)0xffffffffu);
+        if (ctrl[i] <= cap) {
+            if ((int64_t)ctrl[kCtrlRecords + i] * kDenseShare < M) return 0xffffffffu;
+            return (uint32_t)min((uint64_t)t0 << i, (uint64_t)0xffffffffu);
+        }
     return 0xffffffffu;
 }
 
@@ -93,7 +112,7 @@ __global__ __launch_bounds__(256) void k_dense_mark(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ ctrl,
                                                     uint32_t *__restrict__ cnt)
 {
-    const uint32_t t = dense_threshold(ctrl, t0, cap);
+    const uint32_t t = dense_threshold(ctrl, t0, cap, offsets[nbins]);
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b > nbins) return;
     uint32_t c = 0;
@@ -114,7 +133,7 @@ __global__ __launch_bounds__(256) void k_dense_finish(const uint32_t *__restrict
                                                       uint32_t *__restrict__ info,
                                                       uint2 *__restrict__ parts)
 {
-    const uint32_t t = dense_threshold(ctrl, t0, cap);
+    const uint32_t t = dense_threshold(ctrl, t0, cap, offsets[nbins]);
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b == 0) ctrl[kCtrlThreshold] = t;
     if (b > nbins) return;
@@ -344,12 +363,19 @@ __device__ inline AxisCand axis_cand(int64_t c, const AxisParams &a, int ext)
 }
 
 // Every cell of a tile reached by a dense window adds the windows of its
-// candidate bins (z, then y, then x candidates, in a fixed order).  One
-// workgroup per (touched tile, z cell), one thread per (x, y) cell (tiles of
-// at most 16 x 16 cells per z); the window tags of the slice's candidate bins
-// are staged in LDS first.
+// candidate bins, z, then y, then x candidates, each from the cell's own bin
+// down.  One workgroup per (listed tile, z cell), one thread per (x, y)
+// cell (tiles of at most 16 x 16 cells per z).  The window tags of the
+// slice's candidate bins are staged in LDS, with an ordered list of the
+// dense ones.  A short list is walked by every cell (a window that does not
+// reach the cell is skipped); otherwise every cell reads all its candidates,
+// a light one from the zero window.  Both visit a cell's dense candidates
+// in the same order and add exact zeros for the rest, so they give the same
+// bits.
 constexpr int kAddBx = 12, kAddBy = 24, kAddBz = 8;  // staged candidate bins per axis
-constexpr int kAddCx = 4, kAddCy = 8;  // x and y candidates of a cell at most
+constexpr int kAddStaged = kAddBx * kAddBy * kAddBz;
+constexpr int kAddCy = 8;      // y candidates of a cell at most
+constexpr int kAddList = 128;  // dense candidates walked by every cell at most
 
 template <int NXB>
 __global__ __launch_bounds__(256) void k_dense_add(GridParams g, SpreadParams sp,
@@ -360,7 +386,10 @@ __global__ __launch_bounds__(256) void k_dense_add(GridParams g, SpreadParams sp
 {
     constexpr int NX = 8 * NXB;
     constexpr int WF = 64 * 2 * NX;
-    __shared__ uint32_t tags[kAddBz * kAddBy * kAddBx + kAddCx];
+    constexpr int CX = NXB == 1 ? 4 : 3;  // x candidates of a cell at most
+    __shared__ uint32_t tags[kAddStaged + CX];
+    __shared__ uint32_t list[kAddStaged];  // (kz << 16 | ky << 8 | kx) of the dense candidates
+    __shared__ uint32_t wsum[4];
     const int64_t v = blockIdx.y;
     const int w = g.width;
     int ext[3], sh[3];
@@ -371,28 +400,35 @@ __global__ __launch_bounds__(256) void k_dense_add(GridParams g, SpreadParams sp
     }
     const float *wv = win + v * win_cap * WF;
     float *gv = grid + 2 * v * g.fine_total;
-    const int64_t units = sp.block_count * sp.block[2];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t units = (int64_t)touched[0] * sp.block[2];
     for (int64_t unit = blockIdx.x; unit < units; unit += gridDim.x) {
-        const int64_t blk = unit / sp.block[2];
-        if (!touched[blk]) continue;
+        const int64_t blk = touched[1 + unit / sp.block[2]];
         const int64_t bx = blk % sp.nblocks[0];
         const int64_t by = (blk / sp.nblocks[0]) % sp.nblocks[1];
         const int64_t bz = blk / (sp.nblocks[0] * sp.nblocks[1]);
         const int64_t x0 = bx * sp.block[0], y0 = by * sp.block[1];
         const int64_t cz = bz * sp.block[2] + unit % sp.block[2];
         if (cz >= g.ax[2].n) continue;
-        // candidate bins of the slice: per axis from the last cell's bin down
-        // to the lowest bin whose window reaches the first cell
-        // (the last cell of the slice inside the grid)
+        // candidate bins of the slice: per axis from the bin of the slice's
+        // last cell inside the grid down to the lowest bin whose window
+        // reaches the first cell
         const int64_t hx = (std::min<int64_t>(x0 + sp.block[0], g.ax[0].n) - 1) >> sh[0];
         const int64_t hy = (std::min<int64_t>(y0 + sp.block[1], g.ax[1].n) - 1) >> sh[1];
         const int64_t hz = cz >> sh[2];
         const int nbx = (int)(hx - ((x0 - ext[0] + 1) >> sh[0]) + 1);
         const int nby = (int)(hy - ((y0 - ext[1] + 1) >> sh[1]) + 1);
         const int nbz = (int)(hz - ((cz - ext[2] + 1) >> sh[2]) + 1);
-        // staged index: ((hz - iz) * kAddBy + (hy - iy)) * kAddBx + (hx - ix)
+        const int ns = nbz * nby * nbx;
+        // staged t = kx + nbx (ky + nby kz) is bin (hx - kx, hy - ky, hz - kz);
+        // each thread stages a run of consecutive t, then the dense ones are
+        // listed in t order (a block-wide scan of the runs' counts)
+        const int run = (ns + 255) / 256;
         __syncthreads();
-        for (int t = threadIdx.x; t < nbz * nby * nbx; t += blockDim.x) {
+        uint32_t mine = 0;
+        for (int r = 0; r < run; ++r) {
+            const int t = threadIdx.x * run + r;
+            if (t >= ns) break;
             const int kx = t % nbx, ky = (t / nbx) % nby, kz = t / (nbx * nby);
             int64_t ix = hx - kx, iy = hy - ky, iz = hz - kz;
             if (ix < 0) ix += g.ax[0].bins;
@@ -400,47 +436,98 @@ __global__ __launch_bounds__(256) void k_dense_add(GridParams g, SpreadParams sp
             if (iz < 0) iz += g.ax[2].bins;
             const uint32_t tag = info[ix + g.ax[0].bins * (iy + g.ax[1].bins * iz)];
             tags[(kz * kAddBy + ky) * kAddBx + kx] = tag ? tag - 1 : (uint32_t)(win_cap - 1);
+            mine += tag ? 1u : 0u;
+        }
+        uint32_t x = mine;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        uint32_t pos = x - mine, total = 0;
+        for (int k = 0; k < 4; ++k) {
+            pos += k < wid ? wsum[k] : 0u;
+            total += wsum[k];
+        }
+        for (int r = 0; r < run; ++r) {
+            const int t = threadIdx.x * run + r;
+            if (t >= ns) break;
+            const int kx = t % nbx, ky = (t / nbx) % nby, kz = t / (nbx * nby);
+            if (tags[(kz * kAddBy + ky) * kAddBx + kx] != (uint32_t)(win_cap - 1))
+                list[pos++] = (uint32_t)(kz << 16 | ky << 8 | kx);
         }
         __syncthreads();
+        if (total == 0) continue;
         const int64_t cx = x0 + (threadIdx.x % sp.block[0]);
         const int64_t cy = y0 + (threadIdx.x / sp.block[0]);
         if (threadIdx.x >= sp.block[0] * sp.block[1] || cx >= g.ax[0].n || cy >= g.ax[1].n)
             continue;
-        const AxisCand qx = axis_cand(cx, g.ax[0], ext[0]);
-        const AxisCand qy = axis_cand(cy, g.ax[1], ext[1]);
-        const AxisCand qz = axis_cand(cz, g.ax[2], ext[2]);
-        const int sx = (int)(hx - qx.bin0), sy = (int)(hy - qy.bin0);
-        // up to kAddCx x candidates per row, their loads issued together (a
-        // light bin or a missing candidate reads the zero window)
-        int ox[kAddCx];
-#pragma unroll
-        for (int kx = 0; kx < kAddCx; ++kx) ox[kx] = (qx.off0 + (kx << sh[0])) * 2;
-        // per z candidate, the loads of all (y, x) candidates in flight
-        // together, then added in order
         float sre = 0.0f, sim = 0.0f;
-        for (int kz = 0; kz < qz.count; ++kz) {
-            const int oz = qz.off0 + (kz << sh[2]);
-            float2 t[kAddCy][kAddCx];
+        if (total <= (uint32_t)kAddList) {
+            // the cell's offset in the window of list entry (kx, ky, kz)
+            const int bx0 = (int)(cx - (hx << sh[0])), by0 = (int)(cy - (hy << sh[1])),
+                      bz0 = (int)(cz - (hz << sh[2]));
+            for (uint32_t e0 = 0; e0 < total; e0 += 8) {
+                float2 t[8];
 #pragma unroll
-            for (int ky = 0; ky < kAddCy; ++ky) {
-                const bool yin = ky < qy.count;
-                const int oy = qy.off0 + (ky << sh[1]);
-                const uint32_t *trow = tags + (kz * kAddBy + sy + (yin ? ky : 0)) * kAddBx + sx;
-                const float *wrow = wv + (yin ? (oz * 8 + oy) * NX * 2 : 0);
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t e = e0 + k < total ? e0 + k : total - 1;
+                    const uint32_t c = list[e];
+                    const int kx = c & 0xff, ky = (c >> 8) & 0xff, kz = c >> 16;
+                    const int ox = bx0 + (kx << sh[0]), oy = by0 + (ky << sh[1]),
+                              oz = bz0 + (kz << sh[2]);
+                    const bool in = e0 + k < total && (unsigned)ox < (unsigned)ext[0] &&
+                                    (unsigned)oy < (unsigned)ext[1] &&
+                                    (unsigned)oz < (unsigned)ext[2];
+                    t[k] = make_float2(0.0f, 0.0f);
+                    if (in)
+                        t[k] = *(const float2 *)(wv + (int64_t)tags[(kz * kAddBy + ky) * kAddBx +
+                                                                   kx] * WF +
+                                                 ((oz * 8 + oy) * NX + ox) * 2);
+                }
 #pragma unroll
-                for (int kx = 0; kx < kAddCx; ++kx) {
-                    const bool in = yin && kx < qx.count;
-                    const int64_t wi = in ? trow[kx] : win_cap - 1;
-                    t[ky][kx] = *(const float2 *)(wrow + wi * WF + (in ? ox[kx] : 0));
+                for (int k = 0; k < 8; ++k) {
+                    sre += t[k].x;
+                    sim += t[k].y;
                 }
             }
+        } else {
+            const AxisCand qx = axis_cand(cx, g.ax[0], ext[0]);
+            const AxisCand qy = axis_cand(cy, g.ax[1], ext[1]);
+            const AxisCand qz = axis_cand(cz, g.ax[2], ext[2]);
+            const int sx = (int)(hx - qx.bin0), sy = (int)(hy - qy.bin0);
+            int ox[CX];
 #pragma unroll
-            for (int ky = 0; ky < kAddCy; ++ky)
+            for (int kx = 0; kx < CX; ++kx) ox[kx] = (qx.off0 + (kx << sh[0])) * 2;
+            // per z candidate, the loads of all (y, x) candidates in flight
+            // together, then added in order
+            for (int kz = 0; kz < qz.count; ++kz) {
+                const int oz = qz.off0 + (kz << sh[2]);
+                float2 t[kAddCy][CX];
 #pragma unroll
-                for (int kx = 0; kx < kAddCx; ++kx) {
-                    sre += t[ky][kx].x;
-                    sim += t[ky][kx].y;
+                for (int ky = 0; ky < kAddCy; ++ky) {
+                    const bool yin = ky < qy.count;
+                    const int oy = qy.off0 + (ky << sh[1]);
+                    const uint32_t *trow =
+                        tags + (kz * kAddBy + sy + (yin ? ky : 0)) * kAddBx + sx;
+                    const float *wrow = wv + (yin ? (oz * 8 + oy) * NX * 2 : 0);
+#pragma unroll
+                    for (int kx = 0; kx < CX; ++kx) {
+                        const bool in = yin && kx < qx.count;
+                        const int64_t wi = in ? trow[kx] : win_cap - 1;
+                        t[ky][kx] = *(const float2 *)(wrow + wi * WF + (in ? ox[kx] : 0));
+                    }
                 }
+#pragma unroll
+                for (int ky = 0; ky < kAddCy; ++ky)
+#pragma unroll
+                    for (int kx = 0; kx < CX; ++kx) {
+                        sre += t[ky][kx].x;
+                        sim += t[ky][kx].y;
+                    }
+            }
         }
         float2 *cell = (float2 *)(gv + 2 * (cx + g.ax[0].n * (cy + g.ax[1].n * cz)));
         float2 val = *cell;
@@ -454,17 +541,20 @@ template <int MODE, int NXB>
 void launch_dense(const GridParams &g, const SpreadParams &sp, const DenseBins &d, int64_t M,
                   const int4 *rec_start, const float *rec_off, const float *c,
                   const uint32_t *cmax, int64_t vectors, const uint32_t *touched, float *grid,
-                  hipStream_t stream)
+                  bool add, hipStream_t stream)
 {
     // one workgroup per window part while they fit the grid
     const unsigned pg = (unsigned)std::min<int64_t>(d.part_cap, 65535);
     // part_cap windows per vector and a zero window
     const int64_t win_cap = d.part_cap + 1;
-    k_dense_windows<MODE, NXB><<<dim3(pg, (unsigned)vectors), 64, 0, stream>>>(
-        g, d.parts, d.ctrl, d.before, M, rec_start, rec_off, c, cmax, d.win, win_cap);
-    k_dense_reduce<NXB><<<dim3(std::min<unsigned>(pg, 8192), (unsigned)vectors), 256, 0,
-                          stream>>>(d.parts, d.ctrl, d.before, d.win, win_cap);
-    const unsigned ag = (unsigned)std::min<int64_t>(sp.block_count * sp.block[2], 16384);
+    if (!add) {
+        k_dense_windows<MODE, NXB><<<dim3(pg, (unsigned)vectors), 64, 0, stream>>>(
+            g, d.parts, d.ctrl, d.before, M, rec_start, rec_off, c, cmax, d.win, win_cap);
+        k_dense_reduce<NXB><<<dim3(std::min<unsigned>(pg, 8192), (unsigned)vectors), 256, 0,
+                              stream>>>(d.parts, d.ctrl, d.before, d.win, win_cap);
+        return;
+    }
+    const unsigned ag = (unsigned)std::min<int64_t>(sp.block_count * sp.block[2], 8192);
     k_dense_add<NXB><<<dim3(ag, (unsigned)vectors), 256, 0, stream>>>(g, sp, touched, d.info,
                                                                      d.win, win_cap, grid);
 }
@@ -473,22 +563,24 @@ template <int NXB>
 bool launch_dense_mode(SpreadMethod method, const GridParams &g, const SpreadParams &sp,
                        const DenseBins &d, int64_t M, const int4 *rec_start,
                        const float *rec_off, const float *c, const uint32_t *cmax,
-                       int64_t vectors, const uint32_t *touched, float *grid, hipStream_t stream)
+                       int64_t vectors, const uint32_t *touched, float *grid, bool add,
+                       hipStream_t stream)
 {
     switch (method) {
     case SpreadMethod::MfmaF16:
         launch_dense<(int)SpreadMethod::MfmaF16, NXB>(g, sp, d, M, rec_start, rec_off, c,
-                                                      cmax, vectors, touched, grid, stream);
+                                                      cmax, vectors, touched, grid, add,
+                                                      stream);
         return true;
     case SpreadMethod::MfmaF16x3:
         launch_dense<(int)SpreadMethod::MfmaF16x3, NXB>(g, sp, d, M, rec_start, rec_off,
                                                         c, cmax, vectors, touched, grid,
-                                                        stream);
+                                                        add, stream);
         return true;
     case SpreadMethod::MfmaBf16x3:
         launch_dense<(int)SpreadMethod::MfmaBf16x3, NXB>(g, sp, d, M, rec_start, rec_off,
                                                          c, cmax, vectors, touched, grid,
-                                                         stream);
+                                                         add, stream);
         return true;
     default:
         return false;
@@ -528,10 +620,12 @@ int DenseBins::x_blocks(const GridParams &g)
     }
     const int ex = (1 << g.ax[0].shift) + w - 1;
     const int ey = (1 << g.ax[1].shift) + w - 1;
-    if ((ex + (1 << g.ax[0].shift) - 1) >> g.ax[0].shift > kAddCx ||
+    const int nxb = ex <= 8 ? 1 : (ex <= 16 ? 2 : 0);
+    // x and y candidates of a cell within the add kernel's unrolled loops
+    if ((ex + (1 << g.ax[0].shift) - 1) >> g.ax[0].shift > (nxb == 1 ? 4 : 3) ||
         (ey + (1 << g.ax[1].shift) - 1) >> g.ax[1].shift > kAddCy)
         return 0;
-    return ex <= 8 ? 1 : (ex <= 16 ? 2 : 0);
+    return nxb;
 }
 
 bool DenseBins::classify(const GridParams &g, const uint32_t *offsets, hipStream_t stream)
@@ -551,7 +645,7 @@ bool DenseBins::classify(const GridParams &g, const uint32_t *offsets, hipStream
             return false;
         bins_cap = nbins + 1;
     }
-    if (!ctrl && !hip_ok(hipMalloc(&ctrl, sizeof(uint32_t) * 16), "dense bin control"))
+    if (!ctrl && !hip_ok(hipMalloc(&ctrl, sizeof(uint32_t) * 32), "dense bin control"))
         return false;
     if (part_cap > parts_alloc) {
         if (parts) (void)hipFree(parts);
@@ -576,7 +670,7 @@ bool DenseBins::classify(const GridParams &g, const uint32_t *offsets, hipStream
 bool DenseBins::spread(SpreadMethod method, const GridParams &g, const SpreadParams &sp,
                        int64_t M, const int4 *rec_start, const float *rec_off,
                        const float *c, const uint32_t *cmax, int64_t vectors,
-                       const uint32_t *touched, float *grid, hipStream_t stream)
+                       const uint32_t *touched, float *grid, bool add, hipStream_t stream)
 {
     const int nxb = x_blocks(g);
     if (nxb == 0) {
@@ -591,9 +685,9 @@ bool DenseBins::spread(SpreadMethod method, const GridParams &g, const SpreadPar
         win_bytes = need;
     }
     const bool ok = nxb == 1 ? launch_dense_mode<1>(method, g, sp, *this, M, rec_start, rec_off,
-                                                    c, cmax, vectors, touched, grid, stream)
+                                                    c, cmax, vectors, touched, grid, add, stream)
                              : launch_dense_mode<2>(method, g, sp, *this, M, rec_start, rec_off,
-                                                    c, cmax, vectors, touched, grid, stream);
+                                                    c, cmax, vectors, touched, grid, add, stream);
     if (!ok) {
         set_error("dense bins need a matrix-core spread method");
         return false;
