@@ -29,6 +29,7 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;  // 4096
 constexpr int kSortThreads = 256;
+constexpr int64_t kSelfScanTiles = 64;  // radix tiles up to which a scatter scans for itself
 constexpr int kRunThreads = 1024;  // LDS bitonic runs: one thread per pair
 constexpr int kSortRun = 2048;
 
@@ -300,7 +301,7 @@ template <typename K>
 __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
     int64_t M, const K *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, int shift,
     const uint32_t *__restrict__ hist, int64_t ntiles, K *__restrict__ keys_out,
-    uint32_t *__restrict__ vals_out)
+    uint32_t *__restrict__ vals_out, int self_scan)
 {
     __shared__ uint32_t wcnt[kRadixWaves][kDigits];
     __shared__ uint32_t dstart[kDigits], gbase[kDigits];
@@ -336,7 +337,26 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
         }
         uint32_t total;
         dstart[d] = block_exclusive_scan(s, &total);
-        gbase[d] = hist[(int64_t)d * ntiles + blockIdx.x];
+        if (self_scan) {
+            // few tiles: the run's start from the unscanned table, every
+            // digit's total over all tiles plus this digit's earlier tiles
+            const uint32_t *hd = hist + (int64_t)d * ntiles;
+            uint32_t tot = 0, mine = 0;
+            for (int64_t t0 = 0; t0 < ntiles; t0 += 8) {
+                uint32_t h[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) h[q] = t0 + q < ntiles ? hd[t0 + q] : 0u;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    tot += h[q];
+                    mine += t0 + q < (int64_t)blockIdx.x ? h[q] : 0u;
+                }
+            }
+            uint32_t all;
+            gbase[d] = block_exclusive_scan(tot, &all) + mine;
+        } else {
+            gbase[d] = hist[(int64_t)d * ntiles + blockIdx.x];
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -1039,10 +1059,13 @@ void BinSorter::radix_passes(int64_t M, const T *coords, const GridParams &g, in
         else
             k_radix_hist<K><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(M, Kb[p & 1], shift,
                                                                             hist_, ntiles);
-        scan_u32(hist_, nhist, hist_sums_, hist_, stream);
+        // up to kSelfScanTiles tiles the scatter finds its runs' starts in the
+        // unscanned table itself
+        const bool self_scan = ntiles <= kSelfScanTiles;
+        if (!self_scan) scan_u32(hist_, nhist, hist_sums_, hist_, stream);
         k_radix_scatter<K><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
             M, Kb[p & 1], p == 0 ? nullptr : V[p & 1], shift, hist_, ntiles, Kb[(p + 1) & 1],
-            V[(p + 1) & 1]);
+            V[(p + 1) & 1], self_scan ? 1 : 0);
     }
     sorted_keys_ = Kb[passes & 1];
 }
