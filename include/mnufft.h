@@ -102,6 +102,22 @@ int mnufft_encode(mnufft_plan plan, int64_t active_batch, int64_t M, const void 
                   const void *d_in, void *d_out, void *stream);
 
 /*
+ * Type-2 encodes of `count` independent plans as one batch (the batching
+ * form of the reference's plan surface, src/plan.rs:606 type2_gpu per plan;
+ * no single reference entry point does this): plans[i] interpolates its
+ * modes d_in[i] at the M[i] points d_points[i] into d_out[i], as
+ * mnufft_encode(plans[i], 1, M[i], d_points[i], d_in[i], d_out[i]) would,
+ * bit for bit.  The points of all plans are binned by one sort and
+ * interpolated by one launch; each plan predeconvolves and transforms its
+ * own grid.  F32 type-2 plans of one dimension, 1D or 2D, each at most once;
+ * one vector per plan.  The plans keep no points afterwards (set them again
+ * before a plain execute); plans[0] keeps the batch scratch.
+ */
+int mnufft_execute_type2_many(int64_t count, const mnufft_plan *plans, const int64_t *M,
+                              const void *const *d_points, const void *const *d_in,
+                              void *const *d_out, void *stream);
+
+/*
  * Type 3: NufftType3Plan::encode_gpu_batch (src/type3.rs:473-539).  Reads M
  * sources and their strengths (M x active_batch), N targets, writes the N x
  * active_batch outputs.

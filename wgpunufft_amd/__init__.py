@@ -16,6 +16,7 @@ from ._lib import NufftError, build, check, lib
 
 __all__ = [
     "NufftConfig", "NufftPlan", "NufftType3Plan", "NufftError", "F32", "F64", "DF64",
+    "encode_type2_many",
     "build", "lib", "reference_type1_f64", "reference_type2_f64", "reference_type3_f64",
     "STAGES",
 ]
@@ -295,6 +296,39 @@ class NufftType3Plan(_PlanBase):
         check(lib().mnufft_execute_type3(self._handle, active, M, _ptr(sources), _ptr(strengths),
                                          N, _ptr(targets), _ptr(out), _stream_ptr()))
         return out
+
+
+def encode_type2_many(plans, points, values, outs=None):
+    """Type-2 encodes of many independent plans as one batch: plans[i]
+    interpolates values[i] (its modes) at points[i]; one bin sort over all
+    the points and one interpolation launch, each plan's FFT on its own
+    grid.  F32 type-2 plans of one dimension (1D or 2D), one vector each;
+    outputs equal each plan's own encode_gpu bit for bit.  The plans keep no
+    points afterwards."""
+    n = len(plans)
+    if not (len(points) == len(values) == n) or n == 0:
+        raise ValueError("plans, points and values must have the same nonzero length")
+    if outs is None:
+        outs = [None] * n
+    res = []
+    for p, x, v, o in zip(plans, points, values, outs):
+        if p.kind != 2:
+            raise ValueError("batched encodes take type-2 plans")
+        p._check_points(x)
+        modes = p.config.mode_count()
+        if v.dtype != p.complex_dtype or not v.is_contiguous() or v.numel() < modes:
+            raise ValueError("values must be a contiguous tensor of the plan's modes")
+        res.append(p._out(1, o, x.shape[0]))
+    vp = ctypes.c_void_p
+    hs = (vp * n)(*[p._handle for p in plans])
+    ms = (ctypes.c_int64 * n)(*[int(x.shape[0]) for x in points])
+    xs = (vp * n)(*[_ptr(x) for x in points])
+    vs = (vp * n)(*[_ptr(v) for v in values])
+    os_ = (vp * n)(*[_ptr(o) for o in res])
+    check(lib().mnufft_execute_type2_many(n, hs, ms, xs, vs, os_, _stream_ptr()))
+    for p in plans:
+        p._points = None
+    return res
 
 
 def _direct(kind, dim, n_modes, isign, mode_order, x, values, nout, sel, s):

@@ -147,6 +147,45 @@ def config4(args, W, S, torch):
                        f"M log-uniform [1e4,2e6] (total {total_pts})"},
           {"launch": "the 64 encodes captured in one HIP graph, replayed",
            "bitwise_equal_to_eager": bool(same)})
+    del graph
+    # the same 64 encodes as one batch: one bin sort over all 22.9M points,
+    # per-plan predeconvolution + FFT, one interpolation launch
+    plans = [p[0] for p in problems]
+    xs = [p[1] for p in problems]
+    fs = [p[2] for p in problems]
+    outs = [p[3] for p in problems]
+
+    def run_batch():
+        W.encode_type2_many(plans, xs, fs, outs)
+
+    t = _timed(run_batch, args.steps, args.warmup)
+    same = all(torch.equal(e.view(torch.float32), o.view(torch.float32))
+               for e, o in zip(eager, outs))
+    _line(args, "NU points/s, 64 independent 2D type-2 problems tol 1e-5 (batch)",
+          total_pts / t, "points/s", t * 1e3, "f32",
+          {"workload": f"config 4a: 64 2D type-2 plans, modes log-uniform [64,512], "
+                       f"M log-uniform [1e4,2e6] (total {total_pts})"},
+          {"launch": "mnufft_execute_type2_many: one bin sort and one interpolation "
+                     "launch for all 64 problems, per-plan predeconvolution and FFT",
+           "bitwise_equal_to_per_plan": bool(same)})
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        run_batch()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        run_batch()
+    t = _timed(graph.replay, args.steps, args.warmup)
+    same = all(torch.equal(e.view(torch.float32), o.view(torch.float32))
+               for e, o in zip(eager, outs))
+    _line(args, "NU points/s, 64 independent 2D type-2 problems tol 1e-5 (batch, graph)",
+          total_pts / t, "points/s", t * 1e3, "f32",
+          {"workload": f"config 4a: 64 2D type-2 plans, modes log-uniform [64,512], "
+                       f"M log-uniform [1e4,2e6] (total {total_pts})"},
+          {"launch": "the batch captured in one HIP graph, replayed",
+           "bitwise_equal_to_per_plan": bool(same)})
     del graph, eager
     # (b) 32 vectors over a radial 3D point set
     x = S.radial_spokes()

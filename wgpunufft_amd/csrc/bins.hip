@@ -259,6 +259,40 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_keys(int64_t M,
     radix_store_hist(wcnt, hist, ntiles);
 }
 
+// Pass 0 of a multi-problem sort: the key of point j (of the concatenated
+// point set) is its problem's first bin plus its bin in the problem's grid.
+template <int D, typename T>
+__global__ __launch_bounds__(kRadixThreads) void k_radix_keys_many(
+    int64_t M, const BatchProblem *__restrict__ probs, int nprob, uint32_t *__restrict__ keys,
+    uint32_t *__restrict__ hist, int64_t ntiles)
+{
+    __shared__ uint32_t wcnt[kRadixWaves][kDigits];
+    radix_clear(wcnt);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
+    for (int i = 0; i < kRadixItems; ++i) {
+        const int64_t j = radix_pos(t0, w, i, lane);
+        if (j >= M) continue;
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (probs[mid].m_off <= j)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BatchProblem &pb = probs[lo];
+        const T *c = (const T *)pb.coords + (j - pb.m_off) * D;
+        T x[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) x[a] = c[a];
+        const uint32_t key = (uint32_t)(pb.bin_base + sort_key_of<D, T>(x, pb.g, kKeyBin, 0));
+        keys[j] = key;
+        atomicAdd(&wcnt[w][key & (kDigits - 1)], 1u);
+    }
+    radix_store_hist(wcnt, hist, ntiles);
+}
+
 // The 32-bit word of a key that holds the digit at `shift`, and the digit.
 template <typename K>
 __device__ inline uint32_t key_digit(K k, int shift)
@@ -1044,6 +1078,15 @@ template <int D, typename T, typename K>
 void BinSorter::radix_passes(int64_t M, const T *coords, const GridParams &g, int mode,
                              int sub_bits, int bits, hipStream_t stream)
 {
+    radix_loop<K>(M, bits, stream, [&](int64_t ntiles, K *keys) {
+        k_radix_keys<D, T, K><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
+            M, coords, g, mode, sub_bits, keys, hist_, ntiles);
+    });
+}
+
+template <typename K, typename F>
+void BinSorter::radix_loop(int64_t M, int bits, hipStream_t stream, F keys0)
+{
     const int passes = bits > 8 ? (bits + 7) / 8 : 1;
     const int64_t ntiles = radix_tiles(M);
     const int64_t nhist = kDigits * ntiles;
@@ -1054,8 +1097,7 @@ void BinSorter::radix_passes(int64_t M, const T *coords, const GridParams &g, in
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         if (p == 0)
-            k_radix_keys<D, T, K><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
-                M, coords, g, mode, sub_bits, Kb[0], hist_, ntiles);
+            keys0(ntiles, Kb[0]);
         else
             k_radix_hist<K><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(M, Kb[p & 1], shift,
                                                                             hist_, ntiles);
@@ -1145,6 +1187,27 @@ bool BinSorter::sort(int64_t M, const T *coords, const GridParams &g, BinOrder o
 }
 
 template <int D, typename T>
+bool BinSorter::sort_many(int64_t M, const BatchProblem *probs, int nprob, uint64_t nbins,
+                          hipStream_t stream)
+{
+    nbins_ = 0;
+    M_ = M;
+    if (M == 0) return true;
+    if (nbins > 0xffffffffull) {
+        set_error("batched bin sort: more than 2^32 bins over the problems");
+        return false;
+    }
+    if (!ensure(1, M, false)) return false;
+    int bits = 0;
+    while (bits < 32 && ((uint64_t)1 << bits) < nbins) ++bits;
+    radix_loop<uint32_t>(M, bits, stream, [&](int64_t ntiles, uint32_t *keys) {
+        k_radix_keys_many<D, T><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
+            M, probs, nprob, keys, hist_, ntiles);
+    });
+    return hip_ok(hipGetLastError(), "batched bin sort launch");
+}
+
+template <int D, typename T>
 bool BinSorter::prepare(int64_t M, const T *coords, const GridParams &g, BinOrder order,
                         int4 *rec_start, T *rec_off, uint32_t *out_sorted, hipStream_t stream,
                         const uint32_t *dense_before)
@@ -1180,6 +1243,10 @@ bool gather_strengths(int64_t M, int64_t vectors, const uint32_t *perm, const T 
     template bool BinSorter::prepare<D, T>(int64_t, const T *, const GridParams &, BinOrder,       \
                                            int4 *, T *, uint32_t *, hipStream_t,                 \
                                            const uint32_t *);
+template bool BinSorter::sort_many<1, float>(int64_t, const BatchProblem *, int, uint64_t,
+                                             hipStream_t);
+template bool BinSorter::sort_many<2, float>(int64_t, const BatchProblem *, int, uint64_t,
+                                             hipStream_t);
 MNUFFT_BINS_INSTANTIATE(1, float)
 MNUFFT_BINS_INSTANTIATE(2, float)
 MNUFFT_BINS_INSTANTIATE(3, float)

@@ -32,6 +32,13 @@ public:
                  int4 *rec_start, T *rec_off, uint32_t *out_sorted, hipStream_t stream,
                  const uint32_t *dense_before = nullptr);
 
+    // One sort over the concatenated points of nprob problems (device
+    // table): sorted() then holds indices of the concatenated set, grouped
+    // by problem and, inside a problem, by bin (nbins: all problems' bins).
+    template <int D, typename T>
+    bool sort_many(int64_t M, const BatchProblem *probs, int nprob, uint64_t nbins,
+                   hipStream_t stream);
+
     const uint32_t *sorted() const { return sorted_; }
     const uint32_t *offsets() const { return offsets_; }
     int64_t nbins() const { return nbins_; }
@@ -41,6 +48,10 @@ private:
     template <int D, typename T, typename K>
     void radix_passes(int64_t M, const T *coords, const GridParams &g, int mode, int sub_bits,
                       int bits, hipStream_t stream);
+    // LSD passes over `bits` key bits; keys0(ntiles, keys) launches pass 0
+    // (writes the keys and the first digit's per-tile histogram).
+    template <typename K, typename F>
+    void radix_loop(int64_t M, int bits, hipStream_t stream, F keys0);
 
     uint32_t *ctrl_ = nullptr;     // large bins, largest large bin, mid gaps, long gaps
     uint32_t *offsets_ = nullptr;

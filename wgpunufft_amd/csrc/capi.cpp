@@ -209,7 +209,9 @@ int mnufft_execute(mnufft_plan plan, int64_t active_batch, const void *d_in, voi
                    void *stream)
 {
     return guarded([&] {
-        if (!plan || !d_out || (!d_in && plan->plan->point_count > 0)) {
+        // a type-2 execution at zero points has no output to write
+        if (!plan || (!d_out && !(plan->plan->type == 2 && plan->plan->point_count == 0)) ||
+            (!d_in && plan->plan->point_count > 0)) {
             set_error("invalid plan or null buffer");
             return MNUFFT_ERROR_INVALID_ARGUMENT;
         }
@@ -238,6 +240,40 @@ int mnufft_encode(mnufft_plan plan, int64_t active_batch, int64_t M, const void 
     int rc = mnufft_set_points(plan, M, d_points, stream);
     if (rc) return rc;
     return mnufft_execute(plan, active_batch, d_in, d_out, stream);
+}
+
+int mnufft_execute_type2_many(int64_t count, const mnufft_plan *plans, const int64_t *M,
+                              const void *const *d_points, const void *const *d_in,
+                              void *const *d_out, void *stream)
+{
+    return guarded([&] {
+        if (count < 1 || !plans || !M || !d_points || !d_in || !d_out) {
+            set_error("invalid batch arguments");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        std::vector<Plan *> ps((size_t)count);
+        for (int64_t i = 0; i < count; ++i) {
+            Plan *p = plans[i] ? plans[i]->plan.get() : nullptr;
+            if (!p || p->type != 2 || p->precision != Precision::F32 || (p->dim != 1 && p->dim != 2) ||
+                p->dim != plans[0]->plan->dim) {
+                set_error("a batch takes F32 type-2 plans of one dimension, 1D or 2D");
+                return MNUFFT_ERROR_INVALID_ARGUMENT;
+            }
+            if (M[i] < 0 || !d_in[i] || (M[i] > 0 && (!d_points[i] || !d_out[i]))) {
+                set_error("invalid point count or null buffer in batch");
+                return MNUFFT_ERROR_INVALID_ARGUMENT;
+            }
+            for (int64_t k = 0; k < i; ++k)
+                if (ps[(size_t)k] == p) {
+                    set_error("a plan appears twice in a batch");
+                    return MNUFFT_ERROR_INVALID_ARGUMENT;
+                }
+            ps[(size_t)i] = p;
+        }
+        if (!execute_type2_many(count, ps.data(), M, d_points, d_in, d_out, (hipStream_t)stream))
+            return MNUFFT_ERROR_GPU;
+        return MNUFFT_SUCCESS;
+    });
 }
 
 int mnufft_execute_type3(mnufft_plan plan, int64_t active_batch, int64_t M, const void *d_sources,

@@ -42,22 +42,20 @@ __device__ inline int64_t wrap_idx(int64_t i, int64_t n)
     return i;
 }
 
+// The G lanes of a point group: sorted slot s0 (valid below end) holds
+// point sorted[s0] - base of `coords`, written to out[2 (vo + point)].
 template <int D, typename T, int G>
-__global__ __launch_bounds__(256) void k_interp(int64_t M, int64_t vectors,
-                                                const uint32_t *__restrict__ sorted,
-                                                const T *__restrict__ coords, GridParams g,
-                                                const T *__restrict__ grid, T *__restrict__ out)
+__device__ inline void interp_group(int64_t s0, int64_t end, uint32_t base,
+                                    const uint32_t *__restrict__ sorted,
+                                    const T *__restrict__ coords, const GridParams &g,
+                                    const T *__restrict__ gv, T *__restrict__ out, int64_t vo)
 {
-    constexpr int P = 256 / G;
     const int j = threadIdx.x % G;
-    // blockIdx.y = vector, so each launch dimension stays within 2^32 lanes
-    const int64_t v = blockIdx.y;
-    const int64_t s0 = (int64_t)blockIdx.x * P + threadIdx.x / G;
-    const bool has = s0 < M;
+    const bool has = s0 < end;
     const int64_t s = has ? s0 : 0;
     const int w = g.width;
     const T beta = (T)g.beta;
-    uint32_t idx = has ? sorted[s] : 0u;
+    uint32_t idx = has ? sorted[s] - base : 0u;
     T x[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) x[a] = has ? coords[(int64_t)idx * D + a] : (T)0;
@@ -75,7 +73,6 @@ __global__ __launch_bounds__(256) void k_interp(int64_t M, int64_t vectors,
         wt[a] = j < w ? es_w<T>((T)j + (T)off, w, beta) : (T)0;
     }
     T are = (T)0, aim = (T)0;
-    const T *gv = grid + 2 * v * g.fine_total;
     if (has && j < w) {
         int64_t xc = wrap_idx(st[0] + j, g.ax[0].n);
         if (D == 1) {
@@ -122,9 +119,47 @@ __global__ __launch_bounds__(256) void k_interp(int64_t M, int64_t vectors,
         aim += __shfl_xor(aim, o, G);
     }
     if (has && j == 0) {
-        T *dst = out + 2 * (v * M + (int64_t)idx);
+        T *dst = out + 2 * (vo + (int64_t)idx);
         dst[0] = are;
         dst[1] = aim;
+    }
+}
+
+template <int D, typename T, int G>
+__global__ __launch_bounds__(256) void k_interp(int64_t M, int64_t vectors,
+                                                const uint32_t *__restrict__ sorted,
+                                                const T *__restrict__ coords, GridParams g,
+                                                const T *__restrict__ grid, T *__restrict__ out)
+{
+    constexpr int P = 256 / G;
+    // blockIdx.y = vector, so each launch dimension stays within 2^32 lanes
+    const int64_t v = blockIdx.y;
+    interp_group<D, T, G>((int64_t)blockIdx.x * P + threadIdx.x / G, M, 0u, sorted, coords, g,
+                          grid + 2 * v * g.fine_total, out, v * M);
+}
+
+// Many problems in one launch: the work items of problem p (P points each,
+// in the order of the batched sort) start at item_base; every workgroup
+// takes items with a stride.
+template <int D, typename T, int G>
+__global__ __launch_bounds__(256) void k_interp_many(const uint32_t *__restrict__ sorted,
+                                                     const BatchProblem *__restrict__ probs,
+                                                     int nprob, int64_t items)
+{
+    constexpr int P = 256 / G;
+    for (int64_t item = blockIdx.x; item < items; item += gridDim.x) {
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (probs[mid].item_base <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BatchProblem &pb = probs[lo];
+        const int64_t s0 = pb.m_off + (item - pb.item_base) * P + threadIdx.x / G;
+        interp_group<D, T, G>(s0, pb.m_off + pb.M, (uint32_t)pb.m_off, sorted,
+                              (const T *)pb.coords, pb.g, (const T *)pb.grid, (T *)pb.out, 0);
     }
 }
 
@@ -423,6 +458,25 @@ bool interpolate(int64_t M, int64_t vectors, const uint32_t *sorted, const T *co
     if (g.width <= 8) return launch<D, T, 8>(M, vectors, sorted, coords, g, grid, out, stream);
     return launch<D, T, 16>(M, vectors, sorted, coords, g, grid, out, stream);
 }
+
+template <int D, typename T>
+bool interpolate_many(const uint32_t *sorted, const BatchProblem *probs, int nprob,
+                      int64_t items, int width, hipStream_t stream)
+{
+    if (items == 0) return true;
+    const unsigned grid = (unsigned)std::min<int64_t>(items, 65535);
+    if (width <= 4)
+        k_interp_many<D, T, 4><<<grid, 256, 0, stream>>>(sorted, probs, nprob, items);
+    else if (width <= 8)
+        k_interp_many<D, T, 8><<<grid, 256, 0, stream>>>(sorted, probs, nprob, items);
+    else
+        k_interp_many<D, T, 16><<<grid, 256, 0, stream>>>(sorted, probs, nprob, items);
+    return hip_ok(hipGetLastError(), "batched interpolation launch");
+}
+template bool interpolate_many<1, float>(const uint32_t *, const BatchProblem *, int, int64_t,
+                                         int, hipStream_t);
+template bool interpolate_many<2, float>(const uint32_t *, const BatchProblem *, int, int64_t,
+                                         int, hipStream_t);
 
 template <typename T>
 bool interpolate_tiled3(int64_t M, int64_t vectors, const uint32_t *sorted, const uint32_t *offsets,

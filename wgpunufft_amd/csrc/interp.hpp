@@ -11,6 +11,15 @@ template <int D, typename T>
 bool interpolate(int64_t M, int64_t vectors, const uint32_t *sorted, const T *coords,
                  const GridParams &g, const T *grid, T *out, hipStream_t stream);
 
+// One launch for many problems (1D/2D): problem p's points, at slots
+// [m_off, m_off + M) of `sorted` (indices of the concatenated point set),
+// are interpolated from its grid into its output, in work items of
+// 256 / next_pow2(w) points from item_base on (items over all problems).
+// The same per-point sums as interpolate().
+template <int D, typename T>
+bool interpolate_many(const uint32_t *sorted, const BatchProblem *probs, int nprob,
+                      int64_t items, int width, hipStream_t stream);
+
 // 3D, width <= 8: points binned by 8^3 tiles of support starts (the bin
 // geometry must have sides 8), grid tiles staged in LDS.  item_off needs
 // nbins + 1 words of scratch.

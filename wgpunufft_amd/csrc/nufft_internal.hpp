@@ -115,6 +115,19 @@ struct GridParams {
     int64_t fine_total;
 };
 
+// One problem of a multi-plan type-2 execution: its points take indices
+// [m_off, m_off + M) of the concatenated point set and bins [bin_base,
+// bin_base + g.nbins) of the combined bin key; its interpolation work items
+// start at item_base.
+struct BatchProblem {
+    const void *coords;
+    void *out;
+    const void *grid;
+    int64_t m_off, M, item_base;
+    uint64_t bin_base;
+    GridParams g;
+};
+
 // Spread blocks of the output-stationary spreader: `run` cells per lane
 // along axis 0, 64 lanes per block.
 struct SpreadParams {

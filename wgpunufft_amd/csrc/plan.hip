@@ -125,7 +125,8 @@ Plan::~Plan()
 {
     for (void *p : {(void *)d_inv_phihat, grid, (void *)rec_start, rec_off, (void *)rec_perm,
                     c_sorted, t3_xr,
-                    t3_cp, t3_sr, t3_factor, t3_interp, (void *)item_off})
+                    t3_cp, t3_sr, t3_factor, t3_interp, (void *)item_off,
+                    (void *)batch_table})
         if (p) (void)hipFree(p);
     for (auto &sl : slots)
         for (auto &e : sl.ev)
@@ -725,6 +726,122 @@ bool Plan::stage_times(double *ms, int n, int64_t *calls)
     if (calls) *calls = filled;
     if (history) slots_filled = 0;
     return true;
+}
+
+// ----------------------------------------------------------------------
+// Many type-2 plans as one batch.
+// ----------------------------------------------------------------------
+namespace {
+
+// The problem table is written by kernels whose arguments carry up to
+// kTableChunk problems (kernel arguments are copied at launch, so the
+// sequence can be captured in a graph).
+constexpr int kTableChunk = 12;
+struct TableChunk {
+    BatchProblem p[kTableChunk];
+    int n;
+};
+
+__global__ void k_batch_table(TableChunk chunk, BatchProblem *dst)
+{
+    if ((int)threadIdx.x < chunk.n) dst[threadIdx.x] = chunk.p[threadIdx.x];
+}
+
+template <int D>
+bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
+                          const void *const *points, const void *const *inputs,
+                          void *const *outputs, hipStream_t stream)
+{
+    Plan &head = *plans[0];
+    int maxw = 0;
+    for (int64_t i = 0; i < count; ++i) maxw = std::max(maxw, plans[i]->kernel.width);
+    const int G = maxw <= 4 ? 4 : (maxw <= 8 ? 8 : 16);
+    const int64_t per_item = 256 / G;
+    std::vector<BatchProblem> table((size_t)count);
+    int64_t m_off = 0, items = 0;
+    uint64_t bins = 0;
+    for (int64_t i = 0; i < count; ++i) {
+        Plan &p = *plans[i];
+        p.points_set = false;
+        p.points = nullptr;
+        p.point_count = 0;
+        if (!p.ensure_grid(1) || !p.ensure_fft(1)) return false;
+        ModeParams mp = p.mode_params();
+        if (!predeconvolve<D, float>(mp, p.d_inv_phihat, (const float *)inputs[i],
+                                     (float *)p.grid, 1, stream) ||
+            !p.fft(1, p.grid, stream))
+            return false;
+        BatchProblem &b = table[(size_t)i];
+        b.coords = points[i];
+        b.out = outputs[i];
+        b.grid = p.grid;
+        b.m_off = m_off;
+        b.M = M[i];
+        b.item_base = items;
+        b.bin_base = bins;
+        b.g = p.type2_geometry();
+        m_off += M[i];
+        items += (M[i] + per_item - 1) / per_item;
+        bins += (uint64_t)b.g.nbins;
+    }
+    if (m_off == 0) return true;
+    if (m_off > 0xffffffffll) {
+        set_error("batched type 2: more than 2^32 points over the plans");
+        return false;
+    }
+    if (count > head.batch_table_cap) {
+        if (head.batch_table) (void)hipFree(head.batch_table);
+        head.batch_table = nullptr;
+        head.batch_table_cap = 0;
+        if (!hip_ok(hipMalloc(&head.batch_table, sizeof(BatchProblem) * count),
+                    "batched type-2 table"))
+            return false;
+        head.batch_table_cap = count;
+    }
+    for (int64_t i0 = 0; i0 < count; i0 += kTableChunk) {
+        TableChunk chunk;
+        chunk.n = (int)std::min<int64_t>(kTableChunk, count - i0);
+        for (int k = 0; k < chunk.n; ++k) chunk.p[k] = table[(size_t)(i0 + k)];
+        k_batch_table<<<1, 64, 0, stream>>>(chunk, head.batch_table + i0);
+    }
+    if (!head.batch_sorter) head.batch_sorter = std::make_unique<BinSorter>();
+    if (!head.batch_sorter->sort_many<D, float>(m_off, head.batch_table, (int)count, bins,
+                                                 stream))
+        return false;
+    return interpolate_many<D, float>(head.batch_sorter->sorted(), head.batch_table, (int)count,
+                                      items, maxw, stream);
+}
+
+}  // namespace
+
+bool execute_type2_many(int64_t count, Plan *const *plans, const int64_t *M,
+                        const void *const *points, const void *const *inputs,
+                        void *const *outputs, hipStream_t stream)
+{
+    if (count < 1 || !plans || !M || !points || !inputs || !outputs) {
+        set_error("batched type 2: invalid arguments");
+        return false;
+    }
+    const int dim = plans[0]->dim;
+    for (int64_t i = 0; i < count; ++i) {
+        const Plan *p = plans[i];
+        if (!p || p->type != 2 || p->dim != dim || p->precision != Precision::F32 ||
+            (dim != 1 && dim != 2)) {
+            set_error("batched type 2 takes F32 type-2 plans of one dimension, 1D or 2D");
+            return false;
+        }
+        if (M[i] < 0 || (M[i] > 0 && (!points[i] || !outputs[i])) || !inputs[i]) {
+            set_error("batched type 2: invalid point count or null buffer");
+            return false;
+        }
+        for (int64_t k = 0; k < i; ++k)
+            if (plans[k] == p) {
+                set_error("batched type 2: a plan appears twice");
+                return false;
+            }
+    }
+    return dim == 1 ? execute_type2_many_t<1>(count, plans, M, points, inputs, outputs, stream)
+                    : execute_type2_many_t<2>(count, plans, M, points, inputs, outputs, stream);
 }
 
 }  // namespace mnufft

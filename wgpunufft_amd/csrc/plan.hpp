@@ -141,6 +141,11 @@ struct Plan {
 
     bool set_points(int64_t M, const void *pts, hipStream_t stream);
     bool execute(int64_t active, const void *in, void *out, hipStream_t stream);
+    // Scratch of a multi-plan type-2 execution led by this plan: the
+    // combined bin sort and the device table of the problems.
+    std::unique_ptr<BinSorter> batch_sorter;
+    BatchProblem *batch_table = nullptr;
+    int64_t batch_table_cap = 0;
     bool bin_sort(int64_t M, const void *pts, uint32_t *sorted, uint32_t *offsets,
                   hipStream_t stream);
     bool execute_type3(int64_t active, int64_t M, const void *x, const void *c, int64_t N,
@@ -158,5 +163,15 @@ struct Plan {
     bool execute_type3_t(int64_t active, int64_t M, const T *x, const T *c, int64_t N,
                          const T *s, T *out, hipStream_t stream);
 };
+
+// Type-2 encodes of `count` plans (1D or 2D, F32, one vector each) as one
+// batch: plans[i] interpolates inputs[i] (modes) at points[i] (M[i] points)
+// into outputs[i].  One bin sort over all the points and one interpolation
+// launch; each plan predeconvolves and transforms its own grid.  Outputs
+// equal the plans' own encodes bit for bit.  plans[0] keeps the batch
+// scratch; the plans keep no points afterwards.
+bool execute_type2_many(int64_t count, Plan *const *plans, const int64_t *M,
+                        const void *const *points, const void *const *inputs,
+                        void *const *outputs, hipStream_t stream);
 
 }  // namespace mnufft
