@@ -1193,17 +1193,24 @@ bool BinSorter::sort_many(int64_t M, const BatchProblem *probs, int nprob, uint6
     nbins_ = 0;
     M_ = M;
     if (M == 0) return true;
-    if (nbins > 0xffffffffull) {
+    if (nbins >= 0xffffffffull) {
         set_error("batched bin sort: more than 2^32 bins over the problems");
         return false;
     }
-    if (!ensure(1, M, false)) return false;
+    if (!ensure((int64_t)nbins, M, false)) return false;
+    nbins_ = (int64_t)nbins;
+    k_fill_u32<<<1, 64, 0, stream>>>(ctrl_, 4, 0u);
     int bits = 0;
     while (bits < 32 && ((uint64_t)1 << bits) < nbins) ++bits;
     radix_loop<uint32_t>(M, bits, stream, [&](int64_t ntiles, uint32_t *keys) {
         k_radix_keys_many<D, T><<<(unsigned)ntiles, kRadixThreads, 0, stream>>>(
             M, probs, nprob, keys, hist_, ntiles);
     });
+    // offsets over the combined bin list
+    k_bin_offsets<uint32_t><<<grid_for(M + 1, 256), 256, 0, stream>>>(
+        M, (const uint32_t *)sorted_keys_, 0, (int64_t)nbins, offsets_, gaps_,
+        gaps_ + gaps_mid_cap_, ctrl_, nullptr);
+    k_fill_gaps<<<1024, 256, 0, stream>>>(gaps_, gaps_ + gaps_mid_cap_, ctrl_, offsets_);
     return hip_ok(hipGetLastError(), "batched bin sort launch");
 }
 

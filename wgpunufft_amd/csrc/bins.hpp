@@ -34,7 +34,8 @@ public:
 
     // One sort over the concatenated points of nprob problems (device
     // table): sorted() then holds indices of the concatenated set, grouped
-    // by problem and, inside a problem, by bin (nbins: all problems' bins).
+    // by problem and, inside a problem, by bin, and offsets() the nbins + 1
+    // offsets of the combined bin list (nbins: all problems' bins).
     template <int D, typename T>
     bool sort_many(int64_t M, const BatchProblem *probs, int nprob, uint64_t nbins,
                    hipStream_t stream);

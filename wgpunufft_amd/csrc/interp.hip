@@ -144,10 +144,11 @@ __global__ __launch_bounds__(256) void k_interp(int64_t M, int64_t vectors,
 template <int D, typename T, int G>
 __global__ __launch_bounds__(256) void k_interp_many(const uint32_t *__restrict__ sorted,
                                                      const BatchProblem *__restrict__ probs,
-                                                     int nprob, int64_t items)
+                                                     int nprob, int64_t item_begin,
+                                                     int64_t items)
 {
     constexpr int P = 256 / G;
-    for (int64_t item = blockIdx.x; item < items; item += gridDim.x) {
+    for (int64_t item = item_begin + blockIdx.x; item < items; item += gridDim.x) {
         int lo = 0, hi = nprob - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -411,6 +412,210 @@ __global__ __launch_bounds__(THREADS) void k_interp_tiled3(
     }
 }
 
+// ---------------------------------------------------------------------
+// LDS-tiled 2D interpolation: type-2 bins are tiles of TX x TY cells of
+// support starts; a work item is one tile and up to P of its points.  The
+// workgroup stages the tile and its (w-1)-cell halo (periodic) in LDS, then
+// every lane evaluates whole points from LDS (w rows of w cells, F32 rows
+// read as aligned pairs of cells).  With MANY, bins and items run over the
+// problems of a batch (bins [bin_lo, bin_hi) of the combined bin list, one
+// kernel width), every tile taking its problem's grid, points and output.
+// Each output is one lane's fixed-order sum.
+// ---------------------------------------------------------------------
+template <typename T, int W, int TX, int TY, int THREADS, int P, bool MANY>
+__global__ __launch_bounds__(THREADS) void k_interp_tiled2(
+    int64_t M, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
+    const uint32_t *__restrict__ item_off, int64_t bin_lo, int64_t bin_hi,
+    const T *__restrict__ coords0, GridParams g0, const T *__restrict__ grid0,
+    T *__restrict__ out0, const BatchProblem *__restrict__ probs, int nprob)
+{
+    constexpr bool kPairs = sizeof(T) == 4;
+    constexpr int NP = (W + 2) / 2;
+    constexpr int EX = TX + W - 1, EY = TY + W - 1;
+    constexpr int EXR = kPairs && TX + 2 * NP - 2 > EX ? TX + 2 * NP - 2 : EX;
+    constexpr int RX = kPairs ? (EXR + 1) / 2 * 2 : EX;  // LDS row stride (cells)
+    using T2 = typename Complex2<T>::type;
+    __shared__ __align__(16) T2 tile[RX * EY];
+    static_assert(RX <= 64, "a tile row must fit one wavefront");
+    constexpr int RPW = 64 / RX;
+    const int lane = threadIdx.x & 63;
+    const int rsub = lane / RX, lx = lane % RX;
+    const int r_first = (int)(threadIdx.x >> 6) * RPW + rsub;
+    constexpr int R_STEP = (THREADS / 64) * RPW;
+    const int64_t v = blockIdx.y;
+    const int64_t item_lo = item_off[bin_lo], item_hi = item_off[bin_hi];
+    for (int64_t item = item_lo + blockIdx.x; item < item_hi; item += gridDim.x) {
+        int64_t lo = bin_lo, hi = bin_hi - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if ((int64_t)item_off[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int64_t B = lo, k = item - item_off[lo];
+        GridParams g = g0;
+        const T *coords = coords0;
+        const T *gv = grid0;
+        T *out = out0;
+        int64_t bin = B, vo = 0;
+        uint32_t base = 0;
+        if constexpr (MANY) {
+            int plo = 0, phi = nprob - 1;
+            while (plo < phi) {
+                const int mid = (plo + phi + 1) >> 1;
+                if ((int64_t)probs[mid].bin_base <= B)
+                    plo = mid;
+                else
+                    phi = mid - 1;
+            }
+            const BatchProblem &pb = probs[plo];
+            g = pb.g;
+            coords = (const T *)pb.coords;
+            gv = (const T *)pb.grid;
+            out = (T *)pb.out;
+            bin = B - (int64_t)pb.bin_base;
+            base = (uint32_t)pb.m_off;
+        } else {
+            gv = grid0 + 2 * v * g0.fine_total;
+            vo = v * M;
+        }
+        const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n;
+        const int64_t x0 = (bin % g.ax[0].bins) * TX, y0 = (bin / g.ax[0].bins) * TY;
+        if (rsub < RPW) {
+            int64_t gx = x0 + lx;
+            gx = gx >= n0 ? gx - n0 : gx;
+            for (int r = r_first; r < EY; r += R_STEP) {
+                int64_t gy = y0 + r;
+                gy = gy >= n1 ? gy - n1 : gy;
+                tile[r * RX + lx] = *(const T2 *)(gv + 2 * (gx + n0 * gy));
+            }
+        }
+        __syncthreads();
+        const int64_t first = (int64_t)offsets[B] + k * P;
+        int64_t last = first + P;
+        if (last > (int64_t)offsets[B + 1]) last = offsets[B + 1];
+        for (int64_t s = first + threadIdx.x; s < last; s += THREADS) {
+            const uint32_t idx = sorted[s] - base;
+            T x[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) x[a] = coords[(int64_t)idx * 2 + a];
+            const bool reach = FoldTraits<T>::in_reach(x[0]) && FoldTraits<T>::in_reach(x[1]);
+            int l[2];
+            T wt[2][W];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                int64_t start;
+                double off;
+                support_start<T>(x[a], reach, g, a, &start, &off);
+                const int64_t ws = wrapped_start(start, g.ax[a].n);
+                l[a] = (int)(ws - (a == 0 ? x0 : y0));
+                if constexpr (sizeof(T) == 4) {
+                    constexpr float two_over_w = 2.0f / (float)W;
+                    const float u0 = (float)off * two_over_w;
+                    const float bl = (float)g.beta * 1.44269504088896341f;
+#pragma unroll
+                    for (int j = 0; j < W; ++j) {
+                        const float sv = u0 + (float)j * two_over_w;
+                        const float q = __builtin_fmaf(-sv, sv, 1.0f);
+                        const float e = __builtin_amdgcn_exp2f(
+                            __builtin_fmaf(__builtin_amdgcn_sqrtf(q), bl, -bl));
+                        wt[a][j] = q > 0.0f ? e : 0.0f;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < W; ++j) wt[a][j] = es_w<T>((T)j + (T)off, W, (T)g.beta);
+                }
+            }
+            T sre = (T)0, sim = (T)0;
+            if constexpr (kPairs) {
+                const int sh = l[0] & 1;
+                T wx[2 * NP];
+#pragma unroll
+                for (int q = 0; q < 2 * NP; ++q) {
+                    const T a0 = q < W ? wt[0][q] : (T)0;
+                    const T a1 = q >= 1 && q - 1 < W ? wt[0][q - 1] : (T)0;
+                    wx[q] = sh ? a1 : a0;
+                }
+#pragma unroll
+                for (int dy = 0; dy < W; ++dy) {
+                    const float4 *rp = (const float4 *)(tile + (l[1] + dy) * RX + l[0] - sh);
+                    T rre = (T)0, rim = (T)0;
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) {
+                        const float4 c2 = rp[q];
+                        rre += wx[2 * q] * c2.x;
+                        rim += wx[2 * q] * c2.y;
+                        rre += wx[2 * q + 1] * c2.z;
+                        rim += wx[2 * q + 1] * c2.w;
+                    }
+                    sre += wt[1][dy] * rre;
+                    sim += wt[1][dy] * rim;
+                }
+            } else {
+#pragma unroll
+                for (int dy = 0; dy < W; ++dy) {
+                    const int rowb = (l[1] + dy) * RX + l[0];
+                    T rre = (T)0, rim = (T)0;
+#pragma unroll
+                    for (int dx = 0; dx < W; ++dx) {
+                        const T2 cv = tile[rowb + dx];
+                        rre += wt[0][dx] * cv.x;
+                        rim += wt[0][dx] * cv.y;
+                    }
+                    sre += wt[1][dy] * rre;
+                    sim += wt[1][dy] * rim;
+                }
+            }
+            T *dst = out + 2 * (vo + (int64_t)idx);
+            dst[0] = sre;
+            dst[1] = sim;
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int kTile2Threads = 256, kTile2Points = 512;
+
+template <typename T, int W, bool MANY>
+void launch_tiled2_kernel(int64_t M, int64_t vectors, const uint32_t *sorted,
+                          const uint32_t *offsets, const uint32_t *item_off, int64_t bin_lo,
+                          int64_t bin_hi, unsigned grid_x, const T *coords, const GridParams &g,
+                          const T *grid, T *out, const BatchProblem *probs, int nprob,
+                          hipStream_t stream)
+{
+    k_interp_tiled2<T, W, 8, 8, kTile2Threads, kTile2Points, MANY>
+        <<<dim3(grid_x, (unsigned)vectors), kTile2Threads, 0, stream>>>(
+            M, sorted, offsets, item_off, bin_lo, bin_hi, coords, g, grid, out, probs, nprob);
+}
+
+template <typename T, bool MANY>
+bool launch_tiled2_w(int w, int64_t M, int64_t vectors, const uint32_t *sorted,
+                     const uint32_t *offsets, const uint32_t *item_off, int64_t bin_lo,
+                     int64_t bin_hi, unsigned grid_x, const T *coords, const GridParams &g,
+                     const T *grid, T *out, const BatchProblem *probs, int nprob,
+                     hipStream_t stream)
+{
+#define MNUFFT_TILED2(Wd)                                                                         \
+    case Wd:                                                                                      \
+        launch_tiled2_kernel<T, Wd, MANY>(M, vectors, sorted, offsets, item_off, bin_lo, bin_hi, \
+                                          grid_x, coords, g, grid, out, probs, nprob, stream);   \
+        return true;
+    switch (w) {
+        MNUFFT_TILED2(2)
+        MNUFFT_TILED2(3)
+        MNUFFT_TILED2(4)
+        MNUFFT_TILED2(5)
+        MNUFFT_TILED2(6)
+        MNUFFT_TILED2(7)
+        MNUFFT_TILED2(8)
+    default:
+        set_error("tiled interpolation needs a kernel width <= 8");
+        return false;
+    }
+#undef MNUFFT_TILED2
+}
+
 template <typename T, int W, int TX, int TY, int TZ, int THREADS, int P>
 bool launch_tiled(int64_t M, int64_t vectors, const uint32_t *sorted, const uint32_t *offsets,
                   uint32_t *item_off, const T *coords, const GridParams &g, const T *grid, T *out,
@@ -461,22 +666,58 @@ bool interpolate(int64_t M, int64_t vectors, const uint32_t *sorted, const T *co
 
 template <int D, typename T>
 bool interpolate_many(const uint32_t *sorted, const BatchProblem *probs, int nprob,
-                      int64_t items, int width, hipStream_t stream)
+                      int64_t item_begin, int64_t items, int width, hipStream_t stream)
 {
-    if (items == 0) return true;
-    const unsigned grid = (unsigned)std::min<int64_t>(items, 65535);
+    if (items <= item_begin) return true;
+    const unsigned grid = (unsigned)std::min<int64_t>(items - item_begin, 65535);
     if (width <= 4)
-        k_interp_many<D, T, 4><<<grid, 256, 0, stream>>>(sorted, probs, nprob, items);
+        k_interp_many<D, T, 4><<<grid, 256, 0, stream>>>(sorted, probs, nprob, item_begin, items);
     else if (width <= 8)
-        k_interp_many<D, T, 8><<<grid, 256, 0, stream>>>(sorted, probs, nprob, items);
+        k_interp_many<D, T, 8><<<grid, 256, 0, stream>>>(sorted, probs, nprob, item_begin, items);
     else
-        k_interp_many<D, T, 16><<<grid, 256, 0, stream>>>(sorted, probs, nprob, items);
+        k_interp_many<D, T, 16><<<grid, 256, 0, stream>>>(sorted, probs, nprob, item_begin,
+                                                          items);
     return hip_ok(hipGetLastError(), "batched interpolation launch");
 }
 template bool interpolate_many<1, float>(const uint32_t *, const BatchProblem *, int, int64_t,
-                                         int, hipStream_t);
+                                         int64_t, int, hipStream_t);
 template bool interpolate_many<2, float>(const uint32_t *, const BatchProblem *, int, int64_t,
-                                         int, hipStream_t);
+                                         int64_t, int, hipStream_t);
+
+template <typename T>
+bool interpolate_tiled2(int64_t M, int64_t vectors, const uint32_t *sorted,
+                        const uint32_t *offsets, uint32_t *item_off, const T *coords,
+                        const GridParams &g, const T *grid, T *out, hipStream_t stream)
+{
+    if (M == 0) return true;
+    k_tile_items<<<1, 1024, 0, stream>>>(g.nbins, (uint32_t)kTile2Points, offsets, item_off);
+    const unsigned gx = (unsigned)std::min<int64_t>(g.nbins + M / kTile2Points + 64, 16384);
+    if (!launch_tiled2_w<T, false>(g.width, M, vectors, sorted, offsets, item_off, 0, g.nbins, gx,
+                                   coords, g, grid, out, nullptr, 0, stream))
+        return false;
+    return hip_ok(hipGetLastError(), "tiled 2D interpolation launch");
+}
+template bool interpolate_tiled2<float>(int64_t, int64_t, const uint32_t *, const uint32_t *,
+                                        uint32_t *, const float *, const GridParams &,
+                                        const float *, float *, hipStream_t);
+template bool interpolate_tiled2<double>(int64_t, int64_t, const uint32_t *, const uint32_t *,
+                                         uint32_t *, const double *, const GridParams &,
+                                         const double *, double *, hipStream_t);
+
+bool interpolate_tiled2_many(const uint32_t *sorted, const uint32_t *offsets,
+                             uint32_t *item_off, int64_t nbins, const BatchProblem *probs,
+                             int nprob, const int *widths, const int64_t *bin_lo,
+                             const int64_t *bin_hi, int nclass, hipStream_t stream)
+{
+    if (nclass == 0) return true;
+    k_tile_items<<<1, 1024, 0, stream>>>(nbins, (uint32_t)kTile2Points, offsets, item_off);
+    for (int c = 0; c < nclass; ++c)
+        if (!launch_tiled2_w<float, true>(widths[c], 0, 1, sorted, offsets, item_off, bin_lo[c],
+                                          bin_hi[c], 8192, nullptr, GridParams{}, nullptr,
+                                          nullptr, probs, nprob, stream))
+            return false;
+    return hip_ok(hipGetLastError(), "batched tiled interpolation launch");
+}
 
 template <typename T>
 bool interpolate_tiled3(int64_t M, int64_t vectors, const uint32_t *sorted, const uint32_t *offsets,

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -241,6 +242,14 @@ bool Plan::tiled_interp() const
     if (dim != 3 || kernel.width > 8) return false;
     for (int a = 0; a < 3; ++a)
         if (fine[a] < 8 + kernel.width - 1) return false;
+    return true;
+}
+
+bool Plan::tiled2_interp() const
+{
+    if (dim != 2 || kernel.width > 8) return false;
+    for (int a = 0; a < 2; ++a)
+        if (fine[a] < 8 + kernel.width - 1 || fine[a] < 8) return false;
     return true;
 }
 
@@ -495,6 +504,11 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         if (D == 3 && tiled_interp()) {
             if (!ensure_items(gp.nbins + 1)) return false;
             if (!interpolate_tiled3<T>(M, active, sorter.sorted(), sorter.offsets(), item_off,
+                                       (const T *)points, gp, (const T *)grid, out, stream))
+                return false;
+        } else if (D == 2 && tiled2_interp()) {
+            if (!ensure_items(gp.nbins + 1)) return false;
+            if (!interpolate_tiled2<T>(M, active, sorter.sorted(), sorter.offsets(), item_off,
                                        (const T *)points, gp, (const T *)grid, out, stream))
                 return false;
         } else if (!interpolate<D, T>(M, active, sorter.sorted(), (const T *)points, gp,
@@ -753,14 +767,35 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
                           void *const *outputs, hipStream_t stream)
 {
     Plan &head = *plans[0];
+    // problems whose own execute uses the 2D tiled interpolation first,
+    // grouped by kernel width (one launch per width), then the others (the
+    // gather interpolation, one launch): every problem gets the kernel its
+    // own encode uses
+    std::vector<int64_t> order((size_t)count);
+    for (int64_t i = 0; i < count; ++i) order[(size_t)i] = i;
+    auto tiled_w = [&](int64_t i) {
+        return D == 2 && plans[i]->tiled2_interp() ? plans[i]->kernel.width : 1000;
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return tiled_w(a) < tiled_w(b); });
     int maxw = 0;
-    for (int64_t i = 0; i < count; ++i) maxw = std::max(maxw, plans[i]->kernel.width);
+    int64_t first_gather = count;
+    for (int64_t r = 0; r < count; ++r) {
+        const int64_t i = order[(size_t)r];
+        if (tiled_w(i) == 1000) {
+            first_gather = std::min(first_gather, r);
+            maxw = std::max(maxw, plans[i]->kernel.width);
+        }
+    }
     const int G = maxw <= 4 ? 4 : (maxw <= 8 ? 8 : 16);
     const int64_t per_item = 256 / G;
     std::vector<BatchProblem> table((size_t)count);
-    int64_t m_off = 0, items = 0;
+    std::vector<int> widths;
+    std::vector<int64_t> bin_lo, bin_hi;
+    int64_t m_off = 0, items = 0, gather_items = 0;
     uint64_t bins = 0;
-    for (int64_t i = 0; i < count; ++i) {
+    for (int64_t r = 0; r < count; ++r) {
+        const int64_t i = order[(size_t)r];
         Plan &p = *plans[i];
         p.points_set = false;
         p.points = nullptr;
@@ -771,7 +806,7 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
                                      (float *)p.grid, 1, stream) ||
             !p.fft(1, p.grid, stream))
             return false;
-        BatchProblem &b = table[(size_t)i];
+        BatchProblem &b = table[(size_t)r];
         b.coords = points[i];
         b.out = outputs[i];
         b.grid = p.grid;
@@ -780,10 +815,21 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
         b.item_base = items;
         b.bin_base = bins;
         b.g = p.type2_geometry();
+        if (r < first_gather) {
+            const int w = p.kernel.width;
+            if (widths.empty() || widths.back() != w) {
+                widths.push_back(w);
+                bin_lo.push_back((int64_t)bins);
+                bin_hi.push_back((int64_t)bins);
+            }
+            bin_hi.back() = (int64_t)(bins + (uint64_t)b.g.nbins);
+        }
+        if (r == first_gather) gather_items = items;
         m_off += M[i];
         items += (M[i] + per_item - 1) / per_item;
         bins += (uint64_t)b.g.nbins;
     }
+    if (first_gather == count) gather_items = items;
     if (m_off == 0) return true;
     if (m_off > 0xffffffffll) {
         set_error("batched type 2: more than 2^32 points over the plans");
@@ -805,11 +851,17 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
         k_batch_table<<<1, 64, 0, stream>>>(chunk, head.batch_table + i0);
     }
     if (!head.batch_sorter) head.batch_sorter = std::make_unique<BinSorter>();
-    if (!head.batch_sorter->sort_many<D, float>(m_off, head.batch_table, (int)count, bins,
-                                                 stream))
-        return false;
-    return interpolate_many<D, float>(head.batch_sorter->sorted(), head.batch_table, (int)count,
-                                      items, maxw, stream);
+    BinSorter &bs = *head.batch_sorter;
+    if (!bs.sort_many<D, float>(m_off, head.batch_table, (int)count, bins, stream)) return false;
+    if (!widths.empty()) {
+        if (!head.ensure_items((int64_t)bins + 1)) return false;
+        if (!interpolate_tiled2_many(bs.sorted(), bs.offsets(), head.item_off, (int64_t)bins,
+                                     head.batch_table, (int)count, widths.data(), bin_lo.data(),
+                                     bin_hi.data(), (int)widths.size(), stream))
+            return false;
+    }
+    return interpolate_many<D, float>(bs.sorted(), head.batch_table, (int)count, gather_items,
+                                      items, std::max(maxw, 2), stream);
 }
 
 }  // namespace

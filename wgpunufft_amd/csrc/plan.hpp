@@ -133,6 +133,7 @@ struct Plan {
     bool ensure_strengths(int64_t M, int64_t vectors);
     bool ensure_items(int64_t n);
     bool tiled_interp() const;
+    bool tiled2_interp() const;
     FftPlan *fft_for(int64_t vectors);
     bool ensure_fft(int64_t vectors);
     bool fft(int64_t vectors, void *data, hipStream_t stream);
