@@ -414,20 +414,21 @@ __global__ __launch_bounds__(THREADS) void k_interp_tiled3(
 
 // ---------------------------------------------------------------------
 // LDS-tiled 2D interpolation: type-2 bins are tiles of TX x TY cells of
-// support starts; a work item is one tile and up to P of its points.  The
+// support starts; workgroups take bins with a stride (a 2D tile holds a few
+// hundred points at most densities, so a bin is one unit of work).  The
 // workgroup stages the tile and its (w-1)-cell halo (periodic) in LDS, then
 // every lane evaluates whole points from LDS (w rows of w cells, F32 rows
-// read as aligned pairs of cells).  With MANY, bins and items run over the
-// problems of a batch (bins [bin_lo, bin_hi) of the combined bin list, one
-// kernel width), every tile taking its problem's grid, points and output.
-// Each output is one lane's fixed-order sum.
+// read as aligned pairs of cells).  With MANY, bins run over the problems of
+// a batch (bins [bin_lo, bin_hi) of the combined bin list, one kernel
+// width), every tile taking its problem's grid, points and output.  Each
+// output is one lane's fixed-order sum.
 // ---------------------------------------------------------------------
-template <typename T, int W, int TX, int TY, int THREADS, int P, bool MANY>
+template <typename T, int W, int TX, int TY, int THREADS, bool MANY>
 __global__ __launch_bounds__(THREADS) void k_interp_tiled2(
     int64_t M, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
-    const uint32_t *__restrict__ item_off, int64_t bin_lo, int64_t bin_hi,
-    const T *__restrict__ coords0, GridParams g0, const T *__restrict__ grid0,
-    T *__restrict__ out0, const BatchProblem *__restrict__ probs, int nprob)
+    int64_t bin_lo, int64_t bin_hi, const T *__restrict__ coords0, GridParams g0,
+    const T *__restrict__ grid0, T *__restrict__ out0, const BatchProblem *__restrict__ probs,
+    int nprob)
 {
     constexpr bool kPairs = sizeof(T) == 4;
     constexpr int NP = (W + 2) / 2;
@@ -443,17 +444,9 @@ __global__ __launch_bounds__(THREADS) void k_interp_tiled2(
     const int r_first = (int)(threadIdx.x >> 6) * RPW + rsub;
     constexpr int R_STEP = (THREADS / 64) * RPW;
     const int64_t v = blockIdx.y;
-    const int64_t item_lo = item_off[bin_lo], item_hi = item_off[bin_hi];
-    for (int64_t item = item_lo + blockIdx.x; item < item_hi; item += gridDim.x) {
-        int64_t lo = bin_lo, hi = bin_hi - 1;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi + 1) >> 1;
-            if ((int64_t)item_off[mid] <= item)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const int64_t B = lo, k = item - item_off[lo];
+    for (int64_t B = bin_lo + blockIdx.x; B < bin_hi; B += gridDim.x) {
+        const int64_t first = offsets[B], last = offsets[B + 1];
+        if (first == last) continue;
         GridParams g = g0;
         const T *coords = coords0;
         const T *gv = grid0;
@@ -492,9 +485,6 @@ __global__ __launch_bounds__(THREADS) void k_interp_tiled2(
             }
         }
         __syncthreads();
-        const int64_t first = (int64_t)offsets[B] + k * P;
-        int64_t last = first + P;
-        if (last > (int64_t)offsets[B + 1]) last = offsets[B + 1];
         for (int64_t s = first + threadIdx.x; s < last; s += THREADS) {
             const uint32_t idx = sorted[s] - base;
             T x[2];
@@ -575,31 +565,30 @@ __global__ __launch_bounds__(THREADS) void k_interp_tiled2(
     }
 }
 
-constexpr int kTile2Threads = 256, kTile2Points = 512;
+constexpr int kTile2Threads = 128;
 
 template <typename T, int W, bool MANY>
 void launch_tiled2_kernel(int64_t M, int64_t vectors, const uint32_t *sorted,
-                          const uint32_t *offsets, const uint32_t *item_off, int64_t bin_lo,
-                          int64_t bin_hi, unsigned grid_x, const T *coords, const GridParams &g,
-                          const T *grid, T *out, const BatchProblem *probs, int nprob,
-                          hipStream_t stream)
+                          const uint32_t *offsets, int64_t bin_lo, int64_t bin_hi,
+                          const T *coords, const GridParams &g, const T *grid, T *out,
+                          const BatchProblem *probs, int nprob, hipStream_t stream)
 {
-    k_interp_tiled2<T, W, 8, 8, kTile2Threads, kTile2Points, MANY>
-        <<<dim3(grid_x, (unsigned)vectors), kTile2Threads, 0, stream>>>(
-            M, sorted, offsets, item_off, bin_lo, bin_hi, coords, g, grid, out, probs, nprob);
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(bin_hi - bin_lo, 32768));
+    k_interp_tiled2<T, W, 8, 8, kTile2Threads, MANY>
+        <<<dim3(gx, (unsigned)vectors), kTile2Threads, 0, stream>>>(
+            M, sorted, offsets, bin_lo, bin_hi, coords, g, grid, out, probs, nprob);
 }
 
 template <typename T, bool MANY>
 bool launch_tiled2_w(int w, int64_t M, int64_t vectors, const uint32_t *sorted,
-                     const uint32_t *offsets, const uint32_t *item_off, int64_t bin_lo,
-                     int64_t bin_hi, unsigned grid_x, const T *coords, const GridParams &g,
-                     const T *grid, T *out, const BatchProblem *probs, int nprob,
-                     hipStream_t stream)
+                     const uint32_t *offsets, int64_t bin_lo, int64_t bin_hi, const T *coords,
+                     const GridParams &g, const T *grid, T *out, const BatchProblem *probs,
+                     int nprob, hipStream_t stream)
 {
 #define MNUFFT_TILED2(Wd)                                                                         \
     case Wd:                                                                                      \
-        launch_tiled2_kernel<T, Wd, MANY>(M, vectors, sorted, offsets, item_off, bin_lo, bin_hi, \
-                                          grid_x, coords, g, grid, out, probs, nprob, stream);   \
+        launch_tiled2_kernel<T, Wd, MANY>(M, vectors, sorted, offsets, bin_lo, bin_hi, coords,   \
+                                          g, grid, out, probs, nprob, stream);                   \
         return true;
     switch (w) {
         MNUFFT_TILED2(2)
@@ -686,35 +675,31 @@ template bool interpolate_many<2, float>(const uint32_t *, const BatchProblem *,
 
 template <typename T>
 bool interpolate_tiled2(int64_t M, int64_t vectors, const uint32_t *sorted,
-                        const uint32_t *offsets, uint32_t *item_off, const T *coords,
-                        const GridParams &g, const T *grid, T *out, hipStream_t stream)
+                        const uint32_t *offsets, const T *coords, const GridParams &g,
+                        const T *grid, T *out, hipStream_t stream)
 {
     if (M == 0) return true;
-    k_tile_items<<<1, 1024, 0, stream>>>(g.nbins, (uint32_t)kTile2Points, offsets, item_off);
-    const unsigned gx = (unsigned)std::min<int64_t>(g.nbins + M / kTile2Points + 64, 16384);
-    if (!launch_tiled2_w<T, false>(g.width, M, vectors, sorted, offsets, item_off, 0, g.nbins, gx,
-                                   coords, g, grid, out, nullptr, 0, stream))
+    if (!launch_tiled2_w<T, false>(g.width, M, vectors, sorted, offsets, 0, g.nbins, coords, g,
+                                   grid, out, nullptr, 0, stream))
         return false;
     return hip_ok(hipGetLastError(), "tiled 2D interpolation launch");
 }
 template bool interpolate_tiled2<float>(int64_t, int64_t, const uint32_t *, const uint32_t *,
-                                        uint32_t *, const float *, const GridParams &,
-                                        const float *, float *, hipStream_t);
+                                        const float *, const GridParams &, const float *,
+                                        float *, hipStream_t);
 template bool interpolate_tiled2<double>(int64_t, int64_t, const uint32_t *, const uint32_t *,
-                                         uint32_t *, const double *, const GridParams &,
-                                         const double *, double *, hipStream_t);
+                                         const double *, const GridParams &, const double *,
+                                         double *, hipStream_t);
 
 bool interpolate_tiled2_many(const uint32_t *sorted, const uint32_t *offsets,
-                             uint32_t *item_off, int64_t nbins, const BatchProblem *probs,
-                             int nprob, const int *widths, const int64_t *bin_lo,
-                             const int64_t *bin_hi, int nclass, hipStream_t stream)
+                             const BatchProblem *probs, int nprob, const int *widths,
+                             const int64_t *bin_lo, const int64_t *bin_hi, int nclass,
+                             hipStream_t stream)
 {
-    if (nclass == 0) return true;
-    k_tile_items<<<1, 1024, 0, stream>>>(nbins, (uint32_t)kTile2Points, offsets, item_off);
     for (int c = 0; c < nclass; ++c)
-        if (!launch_tiled2_w<float, true>(widths[c], 0, 1, sorted, offsets, item_off, bin_lo[c],
-                                          bin_hi[c], 8192, nullptr, GridParams{}, nullptr,
-                                          nullptr, probs, nprob, stream))
+        if (!launch_tiled2_w<float, true>(widths[c], 0, 1, sorted, offsets, bin_lo[c], bin_hi[c],
+                                          nullptr, GridParams{}, nullptr, nullptr, probs, nprob,
+                                          stream))
             return false;
     return hip_ok(hipGetLastError(), "batched tiled interpolation launch");
 }

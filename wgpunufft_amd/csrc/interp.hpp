@@ -21,19 +21,19 @@ bool interpolate_many(const uint32_t *sorted, const BatchProblem *probs, int npr
                       int64_t item_begin, int64_t items, int width, hipStream_t stream);
 
 // 2D, width <= 8, bins of 8 x 8 cells: tiles staged in LDS (like the 3D
-// tiled interpolation).  item_off needs nbins + 1 words of scratch.
+// tiled interpolation), one workgroup per bin.
 template <typename T>
 bool interpolate_tiled2(int64_t M, int64_t vectors, const uint32_t *sorted,
-                        const uint32_t *offsets, uint32_t *item_off, const T *coords,
-                        const GridParams &g, const T *grid, T *out, hipStream_t stream);
+                        const uint32_t *offsets, const T *coords, const GridParams &g,
+                        const T *grid, T *out, hipStream_t stream);
 
 // The same for the problems of a batch (F32): offsets over the combined bin
-// list (nbins), one launch per kernel width c over bins [bin_lo[c],
-// bin_hi[c]) (the problems of one width take consecutive bins).
+// list, one launch per kernel width c over bins [bin_lo[c], bin_hi[c]) (the
+// problems of one width take consecutive bins).
 bool interpolate_tiled2_many(const uint32_t *sorted, const uint32_t *offsets,
-                             uint32_t *item_off, int64_t nbins, const BatchProblem *probs,
-                             int nprob, const int *widths, const int64_t *bin_lo,
-                             const int64_t *bin_hi, int nclass, hipStream_t stream);
+                             const BatchProblem *probs, int nprob, const int *widths,
+                             const int64_t *bin_lo, const int64_t *bin_hi, int nclass,
+                             hipStream_t stream);
 
 // 3D, width <= 8: points binned by 8^3 tiles of support starts (the bin
 // geometry must have sides 8), grid tiles staged in LDS.  item_off needs

@@ -507,8 +507,7 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
                                        (const T *)points, gp, (const T *)grid, out, stream))
                 return false;
         } else if (D == 2 && tiled2_interp()) {
-            if (!ensure_items(gp.nbins + 1)) return false;
-            if (!interpolate_tiled2<T>(M, active, sorter.sorted(), sorter.offsets(), item_off,
+            if (!interpolate_tiled2<T>(M, active, sorter.sorted(), sorter.offsets(),
                                        (const T *)points, gp, (const T *)grid, out, stream))
                 return false;
         } else if (!interpolate<D, T>(M, active, sorter.sorted(), (const T *)points, gp,
@@ -853,13 +852,11 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
     if (!head.batch_sorter) head.batch_sorter = std::make_unique<BinSorter>();
     BinSorter &bs = *head.batch_sorter;
     if (!bs.sort_many<D, float>(m_off, head.batch_table, (int)count, bins, stream)) return false;
-    if (!widths.empty()) {
-        if (!head.ensure_items((int64_t)bins + 1)) return false;
-        if (!interpolate_tiled2_many(bs.sorted(), bs.offsets(), head.item_off, (int64_t)bins,
-                                     head.batch_table, (int)count, widths.data(), bin_lo.data(),
-                                     bin_hi.data(), (int)widths.size(), stream))
-            return false;
-    }
+    if (!widths.empty() &&
+        !interpolate_tiled2_many(bs.sorted(), bs.offsets(), head.batch_table, (int)count,
+                                 widths.data(), bin_lo.data(), bin_hi.data(), (int)widths.size(),
+                                 stream))
+        return false;
     return interpolate_many<D, float>(bs.sorted(), head.batch_table, (int)count, gather_items,
                                       items, std::max(maxw, 2), stream);
 }
