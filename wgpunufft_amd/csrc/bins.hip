@@ -270,10 +270,12 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_keys_many(
     radix_clear(wcnt);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
-    for (int i = 0; i < kRadixItems; ++i) {
-        const int64_t j = radix_pos(t0, w, i, lane);
-        if (j >= M) continue;
-        int lo = 0, hi = nprob - 1;
+    // the problem of the lane's first point by a binary search, later
+    // points (increasing j) by stepping forward
+    int lo = 0;
+    {
+        const int64_t j = radix_pos(t0, w, 0, lane);
+        int hi = nprob - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (probs[mid].m_off <= j)
@@ -281,6 +283,11 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_keys_many(
             else
                 hi = mid - 1;
         }
+    }
+    for (int i = 0; i < kRadixItems; ++i) {
+        const int64_t j = radix_pos(t0, w, i, lane);
+        if (j >= M) continue;
+        while (lo + 1 < nprob && probs[lo + 1].m_off <= j) ++lo;
         const BatchProblem &pb = probs[lo];
         const T *c = (const T *)pb.coords + (j - pb.m_off) * D;
         T x[D];
