@@ -141,3 +141,29 @@ def test_dense_bins_wide_windows():
     assert e_d <= 32 * 1e-6
     assert d <= 2e-6
     assert not torch.equal(_bits(fd), _bits(ft))
+
+
+def test_dense_bins_vector_chunks():
+    """18 vectors: the dense windows of a vector take 64 MB, so the vectors
+    are spread in two chunks; every vector equals its own execution bit for
+    bit."""
+    torch, W, O, S = _setup()
+    M, B, n_modes = 20000, 18, [16, 16, 16]
+    N = int(np.prod(n_modes))
+    x = _points(S, M, S.config_seed(2, 1500))
+    c = S.complex_normal(S.config_seed(2, 1501), M * B)
+    p = (W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
+         .set_spread_method("mfma_f16x3").set_point_order("coordinates"))
+    xt, ct = torch.from_numpy(x).cuda(), torch.from_numpy(c).cuda()
+    fb = p.encode_gpu(xt, ct)
+    torch.cuda.synchronize()
+    for v in (0, 15, 16, 17):
+        fv = p.encode_gpu(xt, ct[v * M:(v + 1) * M].contiguous(), active_batch=1)
+        torch.cuda.synchronize()
+        assert torch.equal(_bits(fv), _bits(fb[v * N:(v + 1) * N])), f"vector {v}"
+    tiles = (W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
+             .set_spread_method("mfma_f16x3").set_point_order("coordinates").set_dense_bins(0))
+    ft = tiles.encode_gpu(xt, ct)
+    torch.cuda.synchronize()
+    assert rel_l2(fb.cpu().numpy(), ft.cpu().numpy()) <= 2e-6
+    assert not torch.equal(_bits(fb), _bits(ft))
