@@ -437,7 +437,8 @@ bool Plan::set_points_t(int64_t M, const T *pts, hipStream_t stream)
         mark(kStageSort, true, stream);
         if (!ensure_records(M)) return false;
         mark(kStagePrepare, false, stream);
-        if (binned_dense && !dense.classify(gp, sorter.offsets(), stream)) return false;
+        if (binned_dense && !dense.classify(gp, spread_params(), sorter.offsets(), stream))
+            return false;
         if (!sorter.prepare<D, T>(M, pts, gp, point_order, rec_start, (T *)rec_off, rec_perm,
                                   stream, binned_dense ? dense.before : nullptr))
             return false;

@@ -56,27 +56,6 @@ struct BlockReach {
         *first = offsets[row + lo];
         *last = offsets[row + lo + len];
     }
-    // segment t -> bins [first, last)
-    __device__ void segment_bins(int64_t t, const GridParams &g, int64_t *first,
-                                 int64_t *last) const
-    {
-        int64_t nxi = ax[0].len1 > 0 ? 2 : 1;
-        int64_t ix = t % nxi;
-        int64_t rest = t / nxi;
-        int64_t row = 0, stride = g.ax[0].bins;
-#pragma unroll
-        for (int a = 1; a < D; ++a) {
-            int64_t cnt = ax[a].count();
-            int64_t k = rest % cnt;
-            rest /= cnt;
-            row += ax[a].bin(k) * stride;
-            stride *= g.ax[a].bins;
-        }
-        int64_t lo = ix == 0 ? ax[0].lo0 : ax[0].lo1;
-        int64_t len = ix == 0 ? ax[0].len0 : ax[0].len1;
-        *first = row + lo;
-        *last = row + lo + len;
-    }
 };
 
 template <int D>
@@ -101,37 +80,25 @@ __device__ inline BlockReach<D> block_reach(int64_t block, const GridParams &g,
 }
 
 // Records in reach of every block: one wavefront per block, lanes over the
-// block's bin segments.  With dense bins (dense_before: dense records before
-// each bin) the blocks a dense bin reaches are listed in touched[1..] (count
-// in touched[0], cleared by the caller; the list order is arbitrary).
+// block's bin segments.
 template <int D>
 __global__ __launch_bounds__(256) void k_classify(GridParams g, SpreadParams sp,
                                                   const uint32_t *__restrict__ offsets,
-                                                  uint32_t *__restrict__ totals,
-                                                  const uint32_t *__restrict__ dense_before,
-                                                  uint32_t *__restrict__ touched)
+                                                  uint32_t *__restrict__ totals)
 {
     const int lane = threadIdx.x & 63;
     const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (b >= sp.block_count) return;
     BlockReach<D> r = block_reach<D>(b, g, sp);
     uint64_t total = 0;
-    uint32_t dense = 0;
     for (int64_t t = lane; t < r.nseg; t += 64) {
         int64_t f, l;
-        r.segment_bins(t, g, &f, &l);
-        total += (uint64_t)(offsets[l] - offsets[f]);
-        if (dense_before) dense |= dense_before[l] - dense_before[f];
+        r.segment(t, g, offsets, &f, &l);
+        total += (uint64_t)(l - f);
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        total += __shfl_xor(total, o, 64);
-        dense |= __shfl_xor(dense, o, 64);
-    }
-    if (lane == 0) {
-        totals[b] = (uint32_t)(total > 0xffffffffull ? 0xffffffffull : total);
-        if (touched && dense) touched[1 + atomicAdd(&touched[0], 1u)] = (uint32_t)b;
-    }
+    for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o, 64);
+    if (lane == 0) totals[b] = (uint32_t)(total > 0xffffffffull ? 0xffffffffull : total);
 }
 
 // One workgroup plans the work items: a light block is one item, a heavy
@@ -1050,9 +1017,9 @@ Spreader::~Spreader() { release(); }
 void Spreader::release()
 {
     for (void *p : {(void *)totals, (void *)parts, (void *)item_off, (void *)heavy_base,
-                    (void *)heavy_list, (void *)touched, (void *)ctrl, (void *)vmax, partial})
+                    (void *)heavy_list, (void *)ctrl, (void *)vmax, partial})
         if (p) (void)hipFree(p);
-    totals = parts = item_off = heavy_base = heavy_list = touched = ctrl = vmax = nullptr;
+    totals = parts = item_off = heavy_base = heavy_list = ctrl = vmax = nullptr;
     partial = nullptr;
     blocks_cap = 0;
     partial_bytes = 0;
@@ -1068,7 +1035,7 @@ bool Spreader::ensure(int64_t nblocks, int64_t partial_need, int64_t vectors)
         vmax_cap = vectors;
     }
     if (nblocks > blocks_cap) {
-        for (uint32_t **p : {&totals, &parts, &item_off, &heavy_base, &heavy_list, &touched}) {
+        for (uint32_t **p : {&totals, &parts, &item_off, &heavy_base, &heavy_list}) {
             if (*p) (void)hipFree(*p);
             *p = nullptr;
             if (!hip_ok(hipMalloc(p, sizeof(uint32_t) * (nblocks + 1)), "spread block arrays"))
@@ -1138,9 +1105,7 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
     if (!ensure(sp.block_count, part_capacity * per_part * chunk, vectors)) return false;
 
     const int64_t nb = sp.block_count;
-    if (dense) k_fill_u32s<<<1, 64, 0, stream>>>(touched, 1);
-    k_classify<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(
-        g, sp, offsets, totals, dense ? dense->before : nullptr, dense ? touched : nullptr);
+    k_classify<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(g, sp, offsets, totals);
     k_fill_ctrl<<<1, 64, 0, stream>>>(ctrl);
     k_plan<<<1, 1024, 0, stream>>>(nb, totals, light_limit, min_part, part_capacity, parts,
                                    item_off, heavy_base, heavy_list, ctrl);
@@ -1165,14 +1130,14 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
                 // dense windows first, so the grid the tile kernels write is
                 // still in the caches for the window adds and the FFT
                 ok = !dense || dense->spread(method, g, sp, M, rec_start, rec_off,
-                                             dense->strengths + 2 * v0 * M, vmax + v0, vc,
-                                             touched, gr, false, stream);
+                                             dense->strengths + 2 * v0 * M, vmax + v0, vc, gr,
+                                             false, stream);
                 ok = ok && chunk_mfma<D>(*this, method, g, sp, offsets, rec_start, rec_off, cs,
                                          M, vc, vmax + v0, gr, dense == nullptr, stream);
                 if (ok && dense)
                     ok = dense->spread(method, g, sp, M, rec_start, rec_off,
-                                       dense->strengths + 2 * v0 * M, vmax + v0, vc, touched,
-                                       gr, true, stream);
+                                       dense->strengths + 2 * v0 * M, vmax + v0, vc, gr, true,
+                                       stream);
                 if (ok && dense && kernel_end) (void)hipEventRecord(kernel_end, stream);
             }
         } else if (tiles) {

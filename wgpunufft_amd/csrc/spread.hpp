@@ -21,16 +21,18 @@ struct DenseBins {
     // within 16 x 8 x 8 cells).
     static int x_blocks(const GridParams &g);
     static int64_t window_floats(const GridParams &g) { return 1024 * (int64_t)x_blocks(g); }
-    bool classify(const GridParams &g, const uint32_t *offsets, hipStream_t stream);
+    // (sp: the spread's tiles, whose list the dense windows reach is kept)
+    bool classify(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
+                  hipStream_t stream);
     // add == false: the windows of the dense bins (their parts summed, then
     // added in part order), before the tile kernels; add == true: the
-    // windows added to the cells of the touched tiles, after them.  The
+    // windows added to the cells of the tiles they reach, after them.  The
     // dense records' strengths are read from the caller's vectors (point
     // order) through the original index of each record.
     bool spread(SpreadMethod method, const GridParams &g, const SpreadParams &sp, int64_t M,
                 const int4 *rec_start, const float *rec_off, const float *c,
-                const uint32_t *cmax, int64_t vectors, const uint32_t *touched, float *grid,
-                bool add, hipStream_t stream);
+                const uint32_t *cmax, int64_t vectors, float *grid, bool add,
+                hipStream_t stream);
 
     uint32_t threshold = 32;   // records of the smallest dense bin (0: no dense bins)
     const float *strengths = nullptr;  // the execution's strength vectors, point order
@@ -42,7 +44,9 @@ struct DenseBins {
     uint32_t *ctrl = nullptr;    // parts per threshold level, threshold, parts
     uint32_t *sums = nullptr;    // scan scratch
     float *win = nullptr;        // window parts, part_cap per vector
-    int64_t bins_cap = 0, parts_alloc = 0, win_bytes = 0;
+    uint32_t *tile_flag = nullptr;  // per tile: a dense window reaches it
+    uint32_t *touched = nullptr;    // count, then those tiles (any order)
+    int64_t bins_cap = 0, parts_alloc = 0, win_bytes = 0, tiles_cap = 0;
 };
 
 // Device buffers of the spreader's work plan, reused across executions.
@@ -72,7 +76,6 @@ struct Spreader {
     uint32_t *item_off = nullptr;    // first work item per block
     uint32_t *heavy_base = nullptr;  // first partial slot per heavy block
     uint32_t *heavy_list = nullptr;  // heavy blocks
-    uint32_t *touched = nullptr;     // count, then the blocks a dense bin reaches
     uint32_t *ctrl = nullptr;        // items, heavy parts, heavy blocks, part size
     uint32_t *vmax = nullptr;        // largest strength magnitude per vector (float bits)
     void *partial = nullptr;

@@ -125,13 +125,32 @@ __global__ __launch_bounds__(256) void k_dense_mark(const uint32_t *__restrict__
 
 // Light-record offsets, and the window parts of every dense bin: info[b] =
 // 1 + its first part (0 for a light bin), parts[first + p] = (b, p).
+// Tiles of the spread (blocks of sp) that cells [lo, lo + ext) of one axis
+// (periodic) touch: at most 3 (a wrap through a partial last tile).
+__device__ inline int axis_tiles(int64_t lo, int ext, int64_t n, int64_t block, int64_t *t)
+{
+    int k = 0;
+    for (int e = 0; e < ext; ++e) {
+        int64_t c = lo + e;
+        if (c >= n) c -= n;
+        const int64_t q = c / block;
+        bool seen = false;
+        for (int i = 0; i < k; ++i) seen = seen || t[i] == q;
+        if (!seen && k < 3) t[k++] = q;
+    }
+    return k;
+}
+
 __global__ __launch_bounds__(256) void k_dense_finish(const uint32_t *__restrict__ offsets,
                                                       const uint32_t *__restrict__ before,
                                                       int64_t nbins, uint32_t t0, uint32_t cap,
+                                                      GridParams g, SpreadParams sp,
                                                       uint32_t *__restrict__ ctrl,
                                                       uint32_t *__restrict__ light,
                                                       uint32_t *__restrict__ info,
-                                                      uint2 *__restrict__ parts)
+                                                      uint2 *__restrict__ parts,
+                                                      uint32_t *__restrict__ tile_flag,
+                                                      uint32_t *__restrict__ touched)
 {
     const uint32_t t = dense_threshold(ctrl, t0, cap, offsets[nbins]);
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -146,8 +165,33 @@ __global__ __launch_bounds__(256) void k_dense_finish(const uint32_t *__restrict
         const uint32_t first = atomicAdd(&ctrl[kCtrlParts], np);
         for (uint32_t p = 0; p < np; ++p) parts[first + p] = make_uint2((uint32_t)b, p);
         tag = first + 1;
+        // list every tile the bin's window touches, once
+        int64_t tt[3][3];
+        int nt[3];
+        int64_t rest = b;
+        for (int a = 0; a < 3; ++a) {
+            const int64_t bi = rest % g.ax[a].bins;
+            rest /= g.ax[a].bins;
+            nt[a] = axis_tiles(bi << g.ax[a].shift, (1 << g.ax[a].shift) + g.width - 1,
+                               g.ax[a].n, sp.block[a], tt[a]);
+        }
+        for (int k2 = 0; k2 < nt[2]; ++k2)
+            for (int k1 = 0; k1 < nt[1]; ++k1)
+                for (int k0 = 0; k0 < nt[0]; ++k0) {
+                    const int64_t tile =
+                        tt[0][k0] + sp.nblocks[0] * (tt[1][k1] + sp.nblocks[1] * tt[2][k2]);
+                    if (atomicExch(&tile_flag[tile], 1u) == 0u)
+                        touched[1 + atomicAdd(&touched[0], 1u)] = (uint32_t)tile;
+                }
     }
     info[b] = tag;
+}
+
+__global__ void k_dense_clear_tiles(uint32_t *tile_flag, int64_t n, uint32_t *touched)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) tile_flag[i] = 0u;
+    if (i == 0) touched[0] = 0u;
 }
 
 // One lane's record of a k-block, its loads in flight until used: the
@@ -594,9 +638,10 @@ DenseBins::~DenseBins() { release(); }
 void DenseBins::release()
 {
     for (void *p : {(void *)before, (void *)light, (void *)info, (void *)parts, (void *)ctrl,
-                    (void *)sums, (void *)win})
+                    (void *)sums, (void *)win, (void *)tile_flag, (void *)touched})
         if (p) (void)hipFree(p);
-    before = light = info = ctrl = sums = nullptr;
+    before = light = info = ctrl = sums = tile_flag = touched = nullptr;
+    tiles_cap = 0;
     parts = nullptr;
     win = nullptr;
     bins_cap = parts_alloc = 0;
@@ -628,9 +673,19 @@ int DenseBins::x_blocks(const GridParams &g)
     return nxb;
 }
 
-bool DenseBins::classify(const GridParams &g, const uint32_t *offsets, hipStream_t stream)
+bool DenseBins::classify(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
+                         hipStream_t stream)
 {
     const int64_t nbins = g.nbins;
+    if (sp.block_count + 1 > tiles_cap) {
+        for (uint32_t **p : {&tile_flag, &touched}) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+            if (!hip_ok(hipMalloc(p, sizeof(uint32_t) * (sp.block_count + 1)), "dense tiles"))
+                return false;
+        }
+        tiles_cap = sp.block_count + 1;
+    }
     if (nbins + 1 > bins_cap) {
         for (uint32_t **p : {&before, &light, &info}) {
             if (*p) (void)hipFree(*p);
@@ -656,21 +711,23 @@ bool DenseBins::classify(const GridParams &g, const uint32_t *offsets, hipStream
     const uint32_t t0 = threshold;
     const uint32_t cap = (uint32_t)part_cap;
     k_dense_clear<<<1, 64, 0, stream>>>(ctrl);
+    k_dense_clear_tiles<<<grid_for(sp.block_count, 256), 256, 0, stream>>>(
+        tile_flag, sp.block_count, touched);
     k_dense_levels<<<std::min<unsigned>(grid_for(nbins, 256), 2048), 256, 0, stream>>>(
         offsets, nbins, t0, ctrl);
     // cnt lives in `info` until the scan has read it
     k_dense_mark<<<grid_for(nbins + 1, 256), 256, 0, stream>>>(offsets, nbins, t0, cap, ctrl,
                                                                info);
     scan_u32(info, nbins + 1, sums, before, stream);
-    k_dense_finish<<<grid_for(nbins + 1, 256), 256, 0, stream>>>(offsets, before, nbins, t0, cap,
-                                                                 ctrl, light, info, parts);
+    k_dense_finish<<<grid_for(nbins + 1, 256), 256, 0, stream>>>(
+        offsets, before, nbins, t0, cap, g, sp, ctrl, light, info, parts, tile_flag, touched);
     return hip_ok(hipGetLastError(), "dense bin launch");
 }
 
 bool DenseBins::spread(SpreadMethod method, const GridParams &g, const SpreadParams &sp,
                        int64_t M, const int4 *rec_start, const float *rec_off,
-                       const float *c, const uint32_t *cmax, int64_t vectors,
-                       const uint32_t *touched, float *grid, bool add, hipStream_t stream)
+                       const float *c, const uint32_t *cmax, int64_t vectors, float *grid,
+                       bool add, hipStream_t stream)
 {
     const int nxb = x_blocks(g);
     if (nxb == 0) {
