@@ -163,9 +163,9 @@ def main():
     spread_flops = 4.0 * w ** 3 * M
     interp_bytes = M * (4 * 3 + 8) + 8 * nf
 
-    def spread_roof(t_ms, name, peak, unit_note):
+    def spread_roof(t_ms, name, peak, unit_note, bound="mfma"):
         tf = spread_flops / (t_ms * 1e-3) / 1e12
-        return {"kernel": name, "bound": "mfma", "achieved": round(tf, 3), "peak": peak,
+        return {"kernel": name, "bound": bound, "achieved": round(tf, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(tf / peak, 5), "traffic": None,
                 "avg_ms": round(t_ms, 4),
                 "algorithmic": f"4*w^3*M = {spread_flops:.4g} flop (w={w}, M={M}); {unit_note}"}
@@ -181,7 +181,7 @@ def main():
     roof_plain = spread_roof(t_plain,
                              "k_sweep3 (type-1 3D row-sweep spread)" if sweep
                              else "k_spread (type-1 output-stationary spread)",
-                             FP32_PEAK_TFLOPS, "peak: FP32")
+                             FP32_PEAK_TFLOPS, "peak: FP32 vector", bound="fp32-vector")
     interp_gbs = interp_bytes / (t_interp * 1e-3) / 1e9
     roof_interp = {"kernel": "k_interp_tiled3 (type-2 LDS-tiled interpolation)" if tiled
                    else "k_interp (type-2 interpolation)", "bound": "hbm",
