@@ -226,6 +226,22 @@ int mnufft_set_spread_tuning(mnufft_plan plan, int64_t light_limit, int64_t min_
 int mnufft_set_dense_bins(mnufft_plan plan, int64_t min_records);
 
 /*
+ * Autotuner of a type-1 plan (no counterpart in the reference): times
+ * `repeats` encodes of the caller's points and strengths (d_out receives
+ * them) for every candidate -- the plain spread and the matrix-core methods
+ * whose operand error stays below the plan's tolerance (FP16 hi/lo split;
+ * bfloat16 split at tol >= 1e-4; single FP16 at tol >= 1e-3), in 3D each
+ * with dense bins on (32) and off -- and keeps the fastest on the plan
+ * (dense bins only when at least 3% faster than the same method without).
+ * choice (optional, 4 values): spread method, dense-bin threshold, time per
+ * encode in ns, candidates tried.  Synchronizes the stream; the plan's
+ * output stays bitwise repeatable for the chosen configuration.
+ */
+int mnufft_autotune(mnufft_plan plan, int64_t active_batch, int64_t M, const void *d_points,
+                    const void *d_in, void *d_out, int32_t repeats, int64_t *choice,
+                    void *stream);
+
+/*
  * FP64 direct sums on the GPU: reference_type1_f64 / reference_type2_f64
  * (src/direct.rs:9-118) and reference_type3_f64 (src/type3.rs:609-673).
  * kind 1: out[o] = sum_j in[j] exp(i sign k_o.x_j) over the linear mode

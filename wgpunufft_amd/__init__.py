@@ -155,6 +155,28 @@ class _PlanBase:
         check(lib().mnufft_set_dense_bins(self._handle, int(min_records)))
         return self
 
+    def autotune(self, points, values, out=None, repeats=3, active_batch=None):
+        """Type 1: times `repeats` encodes of these points and strengths for
+        the plain spread and the matrix-core methods within the plan's
+        tolerance (3D: dense bins on and off) and keeps the fastest; returns
+        the choice.  `out` receives the encodes."""
+        if self.kind != 1:
+            raise ValueError("autotune takes a type-1 plan")
+        self._check_points(points)
+        M = points.shape[0]
+        active = active_batch or max(1, values.numel() // max(M, 1))
+        if values.dtype != self.complex_dtype or not values.is_contiguous() or \
+                values.numel() < active * M:
+            raise ValueError("values must hold the active batch of strengths")
+        out = self._out(active, out, self.config.mode_count())
+        choice = (ctypes.c_int64 * 4)()
+        check(lib().mnufft_autotune(self._handle, active, M, _ptr(points), _ptr(values),
+                                    _ptr(out), int(repeats), choice, _stream_ptr()))
+        self._points = points
+        names = {v: k for k, v in SPREAD_METHODS.items()}
+        return {"spread_method": names.get(choice[0], choice[0]), "dense_bins": choice[1],
+                "ms_per_encode": choice[2] / 1e6, "candidates": choice[3]}
+
     def timing_history(self, on=True):
         """Sum stage times over every call until the next stage_times()."""
         check(lib().mnufft_timing_history(self._handle, 1 if on else 0))

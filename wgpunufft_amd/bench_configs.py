@@ -212,6 +212,35 @@ def config4(args, W, S, torch):
                                           "type1_rel_l2_last_vector": errs[1],
                                           "sample": "500 modes"}})
         del plan
+    # autotuned (spread method and dense bins timed on these points), then
+    # the execute of the kept points captured in a HIP graph and replayed
+    plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, 1e-6).with_batch(B))
+    choice = plan.autotune(xt, ct, out=out, repeats=1)
+    plan.set_points_gpu(xt)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            plan.execute_gpu(ct, out=out)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    eager = out.clone()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        plan.execute_gpu(ct, out=out)
+    t = _timed(graph.replay, args.steps, args.warmup)
+    same = bool(torch.equal(eager.view(torch.float32), out.view(torch.float32)))
+    errs = [_err_t1(W, S, torch, x, c[v * M:(v + 1) * M], n_modes, out[v * N:(v + 1) * N],
+                    sample=500, seed=S.config_seed(4, 9)) for v in (0, B - 1)]
+    _line(args, "NU points/s, 3D type-1 256^3 modes, 32 vectors, radial (autotuned, graph)",
+          M * B / t, "points/s", t * 1e3, "f32",
+          {"workload": "config 4b: 3D type-1 256^3 modes tol 1e-6, 8192 spokes x 512, B=32",
+           "spreader": choice["spread_method"], "dense_bins": choice["dense_bins"]},
+          {"launch": "execute of the kept points captured in one HIP graph, replayed",
+           "autotune": choice, "bitwise_equal_to_eager": same,
+           "error": {"type1_rel_l2_first_vector": errs[0],
+                     "type1_rel_l2_last_vector": errs[1], "sample": "500 modes"}})
+    del plan, graph
 
 
 def config5(args, W, S, torch):

@@ -399,6 +399,93 @@ int mnufft_set_spread_method(mnufft_plan plan, int32_t method)
     return MNUFFT_SUCCESS;
 }
 
+int mnufft_autotune(mnufft_plan plan, int64_t active_batch, int64_t M, const void *d_points,
+                    const void *d_in, void *d_out, int32_t repeats, int64_t *choice, void *stream)
+{
+    return guarded([&] {
+        if (!plan || plan->plan->type != 1 || M < 0 || (M > 0 && (!d_points || !d_in)) ||
+            !d_out || repeats < 1 || repeats > 100) {
+            set_error("autotune takes a type-1 plan, its points, strengths, an output and "
+                      "1..100 repeats");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        Plan &p = *plan->plan;
+        if (active_batch < 1 || active_batch > p.batch) {
+            set_error("active batch outside 1..batch");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        // candidates: the plain spread, and the matrix-core methods whose
+        // operand error stays below the plan's tolerance, each 3D one with
+        // dense bins on and off
+        struct Cand {
+            mnufft::SpreadMethod method;
+            uint32_t dense;
+        };
+        std::vector<Cand> cands{{mnufft::SpreadMethod::Plain, 0u}};
+        const bool mfma_ok = p.precision == Precision::F32 &&
+                             ((p.dim == 3 && p.sweep) || (p.dim == 2 && p.tiles2));
+        if (mfma_ok) {
+            std::vector<mnufft::SpreadMethod> ms{mnufft::SpreadMethod::MfmaF16x3};
+            if (p.kernel.eps >= 1e-4) ms.push_back(mnufft::SpreadMethod::MfmaBf16x3);
+            if (p.kernel.eps >= 1e-3) ms.push_back(mnufft::SpreadMethod::MfmaF16);
+            for (mnufft::SpreadMethod m : ms) {
+                cands.push_back({m, 0u});
+                if (p.dim == 3) cands.push_back({m, 32u});
+            }
+        }
+        const mnufft::SpreadMethod keep_method = p.spread_method;
+        const uint32_t keep_dense = p.dense.threshold;
+        hipStream_t st = (hipStream_t)stream;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (!mnufft::hip_ok(hipEventCreate(&e0), "autotune event") ||
+            !mnufft::hip_ok(hipEventCreate(&e1), "autotune event"))
+            return MNUFFT_ERROR_GPU;
+        double best = -1.0;
+        size_t best_i = 0;
+        bool ok = true;
+        std::vector<float> times;
+        for (size_t i = 0; i < cands.size() && ok; ++i) {
+            p.spread_method = cands[i].method;
+            p.dense.threshold = cands[i].dense;
+            // one untimed encode (sizes the scratch), then `repeats` timed
+            ok = p.set_points(M, d_points, st) && p.execute(active_batch, d_in, d_out, st);
+            ok = ok && mnufft::hip_ok(hipEventRecord(e0, st), "autotune event");
+            for (int r = 0; r < repeats && ok; ++r)
+                ok = p.set_points(M, d_points, st) && p.execute(active_batch, d_in, d_out, st);
+            ok = ok && mnufft::hip_ok(hipEventRecord(e1, st), "autotune event") &&
+                 mnufft::hip_ok(hipEventSynchronize(e1), "autotune wait");
+            float ms = 0.0f;
+            ok = ok && mnufft::hip_ok(hipEventElapsedTime(&ms, e0, e1), "autotune timing");
+            // dense bins are kept only when they gain at least 3% over the
+            // same method without them (they cost set-up work and scratch)
+            const bool dense_gain =
+                cands[i].dense == 0u || i == 0 || cands[i - 1].method != cands[i].method ||
+                ms < 0.97 * times.back();
+            times.push_back(ms);
+            if (ok && dense_gain && (best < 0.0 || ms < best)) {
+                best = ms;
+                best_i = i;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (!ok) {
+            p.spread_method = keep_method;
+            p.dense.threshold = keep_dense;
+            return MNUFFT_ERROR_GPU;
+        }
+        p.spread_method = cands[best_i].method;
+        p.dense.threshold = cands[best_i].dense;
+        if (choice) {
+            choice[0] = (int64_t)cands[best_i].method;
+            choice[1] = (int64_t)cands[best_i].dense;
+            choice[2] = (int64_t)(best / repeats * 1e6);  // ns per encode
+            choice[3] = (int64_t)cands.size();
+        }
+        return MNUFFT_SUCCESS;
+    });
+}
+
 int mnufft_set_point_order(mnufft_plan plan, int32_t order)
 {
     if (!plan) {
