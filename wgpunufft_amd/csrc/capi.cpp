@@ -476,6 +476,9 @@ int mnufft_autotune(mnufft_plan plan, int64_t active_batch, int64_t M, const voi
         }
         p.spread_method = cands[best_i].method;
         p.dense.threshold = cands[best_i].dense;
+        // without dense bins their scratch is given back (it cannot be
+        // needed before the threshold changes again)
+        if (cands[best_i].dense == 0u) p.dense.release();
         if (choice) {
             choice[0] = (int64_t)cands[best_i].method;
             choice[1] = (int64_t)cands[best_i].dense;
