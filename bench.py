@@ -189,6 +189,30 @@ def main():
                    "frac": round(interp_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                    "avg_ms": round(t_interp, 4),
                    "algorithmic": f"M*(4d+8)+8*N_f = {interp_bytes:.4g} B"}
+    def sort_roof(t_ms, name, nbins):
+        # a bin sort reads every coordinate once and writes the permutation
+        # and the bin offsets once; the radix passes move more than that
+        b = M * (4 * 3 + 4) + 4 * (nbins + 1)
+        gbs = b / (t_ms * 1e-3) / 1e9
+        return {"kernel": name, "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_ms": round(t_ms, 4),
+                "algorithmic": f"M*(4d+4)+4*(nbins+1) = {b:.4g} B (nbins={nbins})"}
+
+    def key_bits(nbins):
+        return max(1, int(nbins - 1).bit_length())
+
+    info2 = p2.info()
+    # type 1 in coordinate order: bin key, then 5 bits of in-bin position per
+    # axis; type 2: bin key, then the start cell inside the bin
+    bits1 = key_bits(info1["nbins"]) + 3 * min(5, (64 - key_bits(info1["nbins"])) // 3)
+    bits2 = key_bits(info2["nbins"]) + sum(int(s).bit_length() - 1 for s in info2["bin_side"])
+    roof_sort1 = sort_roof(stages_mfma["type1"]["sort"],
+                           f"type-1 bin sort (stage: radix keys, {-(-bits1 // 8)} passes of "
+                           "64-bit keys, bin offsets, large-group merges)", info1["nbins"])
+    roof_sort2 = sort_roof(stages_mfma["type2"]["sort"],
+                           f"type-2 bin sort (stage: radix keys, {-(-bits2 // 8)} passes of "
+                           "32-bit keys, bin offsets)", info2["nbins"])
     default_workload = (M, N, args.eps) == (10_000_000, 128, 1e-6)
     for roof in (roof_mfma, roof_interp):
         key = roof["kernel"].split(" ")[0]
@@ -224,7 +248,7 @@ def main():
                    "interp": "LDS-tiled" if tiled else "per-point gather"},
         "bitwise": bitwise,
         "roofline": dominant,
-        "roofline_other": [other, roof_plain],
+        "roofline_other": [other, roof_plain, roof_sort1, roof_sort2],
         "stages_ms": stages_mfma,
         "type1_points_per_s": pts(stages_mfma, "type1"),
         "type2_points_per_s": pts(stages_mfma, "type2"),
