@@ -1,6 +1,8 @@
 """CPU pinning of the oracle against the reference's own known answers and
 against mathematics that fixes each function independently."""
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -8,13 +10,18 @@ import pytest
 from conftest import rel_l2
 from oracle import oracle as O
 
+# known answers the reference's tests assert, kept as data
+with open(os.path.join(os.path.dirname(__file__), "golden", "reference_known_answers.json")) as _f:
+    GOLDEN = json.load(_f)
+
 
 def test_es_parameter_table():
     # tests/cpu_foundations.rs:482-501
-    for eps, w, beta in [(1e-1, 2, 4.40), (1e-2, 3, 6.78), (1e-3, 4, 9.52), (1e-4, 5, 11.50),
-                         (1e-5, 6, 13.80), (1e-6, 7, 16.10)]:
+    table = GOLDEN["es_parameters"]
+    assert len(table["rows"]) == 6
+    for eps, w, beta in table["rows"]:
         ww, bb = O.es_params(eps)
-        assert ww == w and abs(bb - beta) < 1e-13
+        assert ww == w and abs(bb - beta) < table["beta_abs_tol"]
         assert O.es_eval(w, bb, 0.0) == 1.0
         assert O.es_eval(w, bb, w / 2) == 0.0 and O.es_eval(w, bb, -w / 2) == 0.0
         assert abs(O.es_eval(w, bb, 0.31) - O.es_eval(w, bb, -0.31)) < 1e-15
@@ -31,7 +38,10 @@ def test_custom_sigma_formulas():
 
 
 def test_fine_grid_known_lengths():
-    # tests/cpu_foundations.rs:503-544, tests/gpu_nufft_3d.rs:472
+    # tests/cpu_foundations.rs:503-560, tests/gpu_nufft_3d.rs:472
+    for row in GOLDEN["fine_grid_lengths"]["rows"]:
+        w, _ = O.es_params(row["eps"], row["sigma"])
+        assert O.fine_size(row["modes"], row["sigma"], w) == row["length"], row
     w, _ = O.es_params(1e-6)
     assert O.fine_size(17, 2.0, w) == 36
     assert O.fine_size(1, 2.0, w) == 14
@@ -91,6 +101,10 @@ def test_es_ft_matches_phihat_at_grid_frequencies():
 def test_layout_reference_cases():
     # src/gpu_nd_spread.rs:2173-2275
     w, _ = O.es_params(1e-6)
+    for row in GOLDEN["spread_layouts"]["rows"]:
+        L = O.layout(row["fine"], w, row["precision"])
+        assert (L["run"], L["workgroup_size"], L["block"]) == (
+            row["run"], row["workgroup_size"], row["block"]), row
     L = O.layout([512, 512], w, 0)
     assert (L["run"], L["workgroup_size"], L["block"]) == (4, 32, [8, 16])
     assert O.layout([2000, 2000], w, 0)["geometries"][0] == [1, 1]
