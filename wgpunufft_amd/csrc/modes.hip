@@ -20,6 +20,9 @@
 // sequence free of memset nodes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "modes.hpp"
 
 namespace mnufft {
@@ -108,6 +111,72 @@ __global__ __launch_bounds__(256) void k_predeconv(ModeParams mp,
         }
         grid[2 * t] = re;
         grid[2 * t + 1] = im;
+    }
+}
+
+// Row form of the predeconvolution: workgroup (b, c) writes cells
+// [256 c, 256 c + 256) of the fine rows b, b + gridDim.x, ... (a row: the
+// cells of one vector with every axis above 0 fixed).  A lane forms its
+// axis-0 mode index and factor once; each row's part takes 32-bit divisions
+// only, and each cell is one vector store.  The amplitude is multiplied in
+// the axis order of k_predeconv, so both kernels give the same bits.
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_predeconv_rows(ModeParams mp,
+                                                        const double *__restrict__ inv_phihat,
+                                                        const T *__restrict__ modes,
+                                                        T *__restrict__ grid,
+                                                        uint32_t rows_per_vector, uint32_t rows)
+{
+    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+    const uint32_t n0 = (uint32_t)mp.fine[0], N0 = (uint32_t)mp.n_modes[0];
+    const uint32_t x = blockIdx.y * 256u + threadIdx.x;
+    if (x >= n0) return;
+    // the lane's axis-0 part, the same for every row
+    bool in = true;
+    int64_t m = 0;
+    if (x <= N0 - 1 - N0 / 2)
+        m = x;
+    else if (x >= n0 - N0 / 2)
+        m = (int64_t)x - n0;
+    else
+        in = false;
+    const int64_t k0 = mp.centered ? m + N0 / 2 : (m >= 0 ? m : m + N0);
+    const double ph0 = inv_phihat[mp.phihat_off[0] + (m < 0 ? -m : m)];
+    for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const uint32_t v = r / rows_per_vector;
+        const uint32_t row = r - v * rows_per_vector;
+        uint32_t rest = row;
+        int64_t mode_row = 0, stride = N0;
+        bool has = in;
+        double amp = 1.0;
+        amp *= ph0;
+#pragma unroll
+        for (int a = 1; a < D; ++a) {
+            const uint32_t N = (uint32_t)mp.n_modes[a], n = (uint32_t)mp.fine[a];
+            const uint32_t fa = rest % n;
+            rest /= n;
+            int64_t ma = 0;
+            if (fa <= N - 1 - N / 2)
+                ma = fa;
+            else if (fa >= n - N / 2)
+                ma = (int64_t)fa - n;
+            else
+                has = false;
+            const int64_t ka = mp.centered ? ma + N / 2 : (ma >= 0 ? ma : ma + N);
+            mode_row += ka * stride;
+            stride *= N;
+            amp *= inv_phihat[mp.phihat_off[a] + (ma < 0 ? -ma : ma)];
+        }
+        V2 val;
+        val.x = (T)0;
+        val.y = (T)0;
+        if (has) {
+            const T *src = modes + 2 * ((int64_t)v * mp.mode_total + mode_row + k0);
+            const T ta = (T)amp;
+            val.x = src[0] * ta;
+            val.y = src[1] * ta;
+        }
+        *(V2 *)(grid + 2 * ((int64_t)v * mp.fine_total + (int64_t)row * n0 + x)) = val;
     }
 }
 
@@ -279,7 +348,22 @@ bool predeconvolve(const ModeParams &mp, const double *inv_phihat, const T *mode
                    int64_t vectors, hipStream_t stream)
 {
     int64_t n = mp.fine_total * vectors;
-    k_predeconv<D, T><<<grid_for(n, 256), 256, 0, stream>>>(mp, inv_phihat, modes, grid, vectors);
+    if (n == 0) return true;
+    const int64_t rows_per_vector = mp.fine_total / mp.fine[0];
+    const int64_t rows = rows_per_vector * vectors;
+    const int64_t chunks = (mp.fine[0] + 255) / 256;
+    bool fits32 = rows <= 0x7fffffff && chunks <= 65535;
+    for (int a = 0; a < D; ++a) fits32 = fits32 && mp.fine[a] <= 0x7fffffff;
+    if (fits32) {
+        // a few thousand workgroups, each taking rows with a stride
+        const int64_t per = std::max<int64_t>(1, 8192 / chunks);
+        const unsigned gx = (unsigned)std::min<int64_t>(rows, per);
+        k_predeconv_rows<D, T><<<dim3(gx, (unsigned)chunks), 256, 0, stream>>>(
+            mp, inv_phihat, modes, grid, (uint32_t)rows_per_vector, (uint32_t)rows);
+    } else {
+        k_predeconv<D, T><<<grid_for(n, 256), 256, 0, stream>>>(mp, inv_phihat, modes, grid,
+                                                                 vectors);
+    }
     return hip_ok(hipGetLastError(), "predeconvolution launch");
 }
 
