@@ -13,6 +13,7 @@
 // recomputed, and the lane sums combine in a fixed butterfly.
 #include <hip/hip_runtime.h>
 
+#include "bins.hpp"
 #include "fold.hpp"
 #include "interp.hpp"
 #include "spread_device.hpp"
@@ -188,58 +189,21 @@ struct Complex2<double> {
     using type = double2;
 };
 
-// item_off[b] = first item of bin b (exclusive scan of ceil(count / P)),
-// item_off[nbins] = items.  One workgroup; every thread takes kItemsPerThread
-// consecutive bins per round.
-constexpr int kItemsPerThread = 8;
-
-__global__ __launch_bounds__(1024) void k_tile_items(int64_t nbins, uint32_t points_per_item,
+// Work items per bin, ceil(count / P), into item_off[b] (item_off[nbins] =
+// 0); the exclusive scan (scan_u32, in place) then makes item_off[b] the
+// first item of bin b and item_off[nbins] the number of items.
+__global__ __launch_bounds__(256) void k_item_counts(int64_t nbins, uint32_t points_per_item,
                                                      const uint32_t *__restrict__ offsets,
                                                      uint32_t *__restrict__ item_off)
 {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    constexpr int64_t kRound = 1024 * kItemsPerThread;
-    for (int64_t base = 0; base < nbins; base += kRound) {
-        const int64_t b0 = base + (int64_t)threadIdx.x * kItemsPerThread;
-        uint32_t t[kItemsPerThread];
-        uint32_t mine = 0;
-#pragma unroll
-        for (int i = 0; i < kItemsPerThread; ++i) {
-            const int64_t b = b0 + i;
-            t[i] = 0;
-            if (b < nbins) {
-                const uint32_t n = offsets[b + 1] - offsets[b];
-                t[i] = (n + points_per_item - 1) / points_per_item;
-            }
-            mine += t[i];
-        }
-        uint32_t x = mine;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) wsum[wid] = x;
-        __syncthreads();
-        uint32_t pre = 0, all = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-            if (w < wid) pre += wsum[w];
-            all += wsum[w];
-        }
-        uint32_t run = carry + pre + x - mine;
-#pragma unroll
-        for (int i = 0; i < kItemsPerThread; ++i) {
-            if (b0 + i < nbins) item_off[b0 + i] = run;
-            run += t[i];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) carry += all;
-        __syncthreads();
+    const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (b > nbins) return;
+    uint32_t t = 0;
+    if (b < nbins) {
+        const uint32_t n = offsets[b + 1] - offsets[b];
+        t = (n + points_per_item - 1) / points_per_item;
     }
-    if (threadIdx.x == 0) item_off[nbins] = carry;
+    item_off[b] = t;
 }
 
 // Tile and part of work item `item`.
@@ -610,7 +574,11 @@ bool launch_tiled(int64_t M, int64_t vectors, const uint32_t *sorted, const uint
                   uint32_t *item_off, const T *coords, const GridParams &g, const T *grid, T *out,
                   hipStream_t stream)
 {
-    k_tile_items<<<1, 1024, 0, stream>>>(g.nbins, (uint32_t)P, offsets, item_off);
+    // item_off holds nbins + 1 offsets, then the scan's tile sums
+    const int64_t n = g.nbins + 1;
+    k_item_counts<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(g.nbins, (uint32_t)P, offsets,
+                                                                  item_off);
+    scan_u32(item_off, n, item_off + n, item_off, stream);
     int64_t items = std::min<int64_t>(g.nbins + M / P + 64, 16384);
     dim3 grid3((unsigned)items, (unsigned)vectors);
     k_interp_tiled3<T, W, TX, TY, TZ, THREADS, P><<<grid3, THREADS, 0, stream>>>(

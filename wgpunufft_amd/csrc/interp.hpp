@@ -37,7 +37,7 @@ bool interpolate_tiled2_many(const uint32_t *sorted, const uint32_t *offsets,
 
 // 3D, width <= 8: points binned by 8^3 tiles of support starts (the bin
 // geometry must have sides 8), grid tiles staged in LDS.  item_off needs
-// nbins + 1 words of scratch.
+// nbins + 1 + scan_tiles(nbins + 1) words of scratch.
 template <typename T>
 bool interpolate_tiled3(int64_t M, int64_t vectors, const uint32_t *sorted, const uint32_t *offsets,
                         uint32_t *item_off, const T *coords, const GridParams &g, const T *grid,

@@ -503,7 +503,8 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         mark(kStageFft, true, stream);
         mark(kStageInterp, false, stream);
         if (D == 3 && tiled_interp()) {
-            if (!ensure_items(gp.nbins + 1)) return false;
+            // bin item offsets and the scratch of their scan
+            if (!ensure_items(gp.nbins + 1 + scan_tiles(gp.nbins + 1) + 1)) return false;
             if (!interpolate_tiled3<T>(M, active, sorter.sorted(), sorter.offsets(), item_off,
                                        (const T *)points, gp, (const T *)grid, out, stream))
                 return false;
