@@ -68,6 +68,54 @@ __global__ __launch_bounds__(256) void k_deconv(ModeParams mp, const double *__r
     }
 }
 
+// Row form of the deconvolution: workgroup (b, c) writes modes
+// [256 c, 256 c + 256) of the mode rows b, b + gridDim.x, ... (a row: the
+// modes of one vector with every axis above 0 fixed).  A lane forms its
+// axis-0 fine index and factor once; each row's part takes 32-bit divisions
+// only, and each mode is one vector store.  The amplitude is multiplied in
+// the axis order of k_deconv, so both kernels give the same bits.
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_deconv_rows(ModeParams mp,
+                                                     const double *__restrict__ inv_phihat,
+                                                     const T *__restrict__ grid,
+                                                     T *__restrict__ modes,
+                                                     uint32_t rows_per_vector, uint32_t rows)
+{
+    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+    const uint32_t N0 = (uint32_t)mp.n_modes[0], n0 = (uint32_t)mp.fine[0];
+    const uint32_t k = blockIdx.y * 256u + threadIdx.x;
+    if (k >= N0) return;
+    const int64_t m0 = mp.centered ? (int64_t)k - N0 / 2
+                                   : (k < (N0 + 1) / 2 ? (int64_t)k : (int64_t)k - N0);
+    const int64_t f0 = m0 < 0 ? m0 + n0 : m0;
+    const double ph0 = inv_phihat[mp.phihat_off[0] + (m0 < 0 ? -m0 : m0)];
+    for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const uint32_t v = r / rows_per_vector;
+        const uint32_t row = r - v * rows_per_vector;
+        uint32_t rest = row;
+        int64_t fine_row = 0, stride = n0;
+        double amp = 1.0;
+        amp *= ph0;
+#pragma unroll
+        for (int a = 1; a < D; ++a) {
+            const uint32_t N = (uint32_t)mp.n_modes[a], n = (uint32_t)mp.fine[a];
+            const uint32_t ka = rest % N;
+            rest /= N;
+            const int64_t m = mp.centered ? (int64_t)ka - N / 2
+                                          : (ka < (N + 1) / 2 ? (int64_t)ka : (int64_t)ka - N);
+            fine_row += (m < 0 ? m + n : m) * stride;
+            stride *= n;
+            amp *= inv_phihat[mp.phihat_off[a] + (m < 0 ? -m : m)];
+        }
+        const T *src = grid + 2 * ((int64_t)v * mp.fine_total + fine_row + f0);
+        const T ta = (T)amp;
+        V2 val;
+        val.x = src[0] * ta;
+        val.y = src[1] * ta;
+        *(V2 *)(modes + 2 * ((int64_t)v * mp.mode_total + (int64_t)row * N0 + k)) = val;
+    }
+}
+
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_predeconv(ModeParams mp,
                                                    const double *__restrict__ inv_phihat,
@@ -339,7 +387,22 @@ bool deconvolve(const ModeParams &mp, const double *inv_phihat, const T *grid, T
                 int64_t vectors, hipStream_t stream)
 {
     int64_t n = mp.mode_total * vectors;
-    k_deconv<D, T><<<grid_for(n, 256), 256, 0, stream>>>(mp, inv_phihat, grid, modes, vectors);
+    if (n == 0) return true;
+    const int64_t rows_per_vector = mp.mode_total / mp.n_modes[0];
+    const int64_t rows = rows_per_vector * vectors;
+    const int64_t chunks = (mp.n_modes[0] + 255) / 256;
+    bool fits32 = rows <= 0x7fffffff && chunks <= 65535;
+    for (int a = 0; a < D; ++a) fits32 = fits32 && mp.fine[a] <= 0x7fffffff;
+    if (fits32) {
+        // a few thousand workgroups, each taking rows with a stride
+        const int64_t per = std::max<int64_t>(1, 8192 / chunks);
+        const unsigned gx = (unsigned)std::min<int64_t>(rows, per);
+        k_deconv_rows<D, T><<<dim3(gx, (unsigned)chunks), 256, 0, stream>>>(
+            mp, inv_phihat, grid, modes, (uint32_t)rows_per_vector, (uint32_t)rows);
+    } else {
+        k_deconv<D, T><<<grid_for(n, 256), 256, 0, stream>>>(mp, inv_phihat, grid, modes,
+                                                              vectors);
+    }
     return hip_ok(hipGetLastError(), "deconvolution launch");
 }
 
