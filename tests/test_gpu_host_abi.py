@@ -191,3 +191,39 @@ def test_single_precision_plan_reports_f32():
         assert lib.wgpu_nufftf_plan_precision(plan) == F32
     finally:
         lib.wgpu_nufftf_destroy(plan)
+
+
+def test_errors_are_reported_with_messages():
+    """capi/tests/c_api.rs:433-506 on the GPU backend: codes and the key
+    phrase of each message."""
+    lib = _lib()
+    o = _opts(lib, 0)
+    plan = ctypes.c_void_p()
+    I64, D = ctypes.c_int64, ctypes.c_double
+    msg = lambda: lib.wgpu_nufft_last_error().decode()
+    assert lib.wgpu_nufft_makeplan(4, 1, (I64 * 1)(8), 1, I64(1), D(1e-6), ctypes.byref(plan),
+                                   ctypes.byref(o)) == 1
+    assert "type" in msg() and not plan.value
+    assert lib.wgpu_nufft_makeplan(1, 1, (I64 * 1)(0), 1, I64(1), D(1e-6), ctypes.byref(plan),
+                                   ctypes.byref(o)) == 1
+    assert lib.wgpu_nufft_makeplan(1, 1, (I64 * 1)(8), 1, I64(1), D(1e-20), ctypes.byref(plan),
+                                   ctypes.byref(o)) == 2
+    _check(lib, lib.wgpu_nufft_makeplan(1, 1, (I64 * 1)(8), 1, I64(1), D(1e-6),
+                                        ctypes.byref(plan), ctypes.byref(o)))
+    try:
+        f = np.zeros(16)
+        c = np.zeros(2)
+        assert lib.wgpu_nufft_execute(plan, _p(c, False), _p(f, False)) == 1
+        assert "set the points" in msg(), msg()
+        far = np.array([10.0])
+        assert lib.wgpu_nufft_setpts(plan, I64(1), _p(far, False), None, None, I64(0), None,
+                                     None, None) == 1
+        assert "3*pi" in msg(), msg()
+        assert lib.wgpu_nufft_setpts(plan, I64(1), None, None, None, I64(0), None, None,
+                                     None) == 1
+        # the single-precision functions refuse a double plan
+        assert lib.wgpu_nufftf_execute(plan, None, None) == 1
+        assert "array type" in msg(), msg()
+    finally:
+        lib.wgpu_nufft_destroy(plan)
+    lib.wgpu_nufft_destroy(None)

@@ -691,6 +691,17 @@ int hp_makeplan(bool float_api, int32_t type, int32_t dim, const int64_t *n_mode
         set_error("ntrans must be at least 1");
         return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
     }
+    if (type != 3) {
+        if (!n_modes) {
+            set_error("n_modes is null");
+            return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        for (int a = 0; a < dim; ++a)
+            if (n_modes[a] < 1) {
+                set_error("mode count along axis " + std::to_string(a) + " must be positive");
+                return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+            }
+    }
     std::string why;
     if (!gpu_available(&why)) {
         set_error(why);
@@ -739,7 +750,7 @@ bool pack_points(const HostPlan &hp, int64_t M, const S *const *axes, const S *n
             if (!std::isfinite(v) || (check_range && std::fabs(v) > kMaxCoordinate)) {
                 set_error(std::string(what) + " " + std::to_string(j) + " axis " +
                           std::to_string(a) + " is " + std::to_string(v) +
-                          (check_range ? ", outside [-3 pi, 3 pi]" : ", not finite"));
+                          (check_range ? ", outside [-3*pi, 3*pi]" : ", not finite"));
                 return false;
             }
             if (hp.precision == Precision::F32) {
@@ -758,6 +769,11 @@ int hp_setpts(HostPlan *hp, int64_t M, const S *const *axes, const S *nd, int64_
 {
     if (!hp) {
         set_error("plan is null");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if ((sizeof(S) == 4) != hp->float_api) {
+        set_error(std::string("the plan takes ") + (hp->float_api ? "float" : "double") +
+                  " arrays: this call has the other array type");
         return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
     }
     if (M < 0 || (M > 0 && !nd && !axes[0]) || (hp->dim > 1 && M > 0 && !nd && !axes[1]) ||
@@ -832,8 +848,13 @@ int hp_execute(HostPlan *hp, S *c, S *f)
         set_error("plan is null");
         return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
     }
+    if ((sizeof(S) == 4) != hp->float_api) {
+        set_error(std::string("the plan takes ") + (hp->float_api ? "float" : "double") +
+                  " arrays: this call has the other array type");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
     if (!hp->points_set) {
-        set_error("setpts must be called before execute");
+        set_error("set the points (setpts) before execute");
         return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
     }
     int64_t modes = 1;
