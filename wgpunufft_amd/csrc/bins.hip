@@ -18,6 +18,8 @@
 // all workgroups (LDS bitonic runs, merge-path rounds).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bins.hpp"
 #include "fold.hpp"
 
@@ -973,13 +975,11 @@ __global__ __launch_bounds__(256) void k_gather_strengths(int64_t M, int64_t vec
 {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M - (tail ? (int64_t)*tail : 0)) return;
-    int64_t idx = perm[s];
-    for (int64_t v = 0; v < vectors; ++v) {
-        const T *src = c + 2 * (v * M + idx);
-        T *dst = c_sorted + 2 * (v * M + s);
-        dst[0] = src[0];
-        dst[1] = src[1];
-    }
+    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+    const int64_t idx = perm[s];
+    // one vector load and store per complex value
+    for (int64_t v = 0; v < vectors; ++v)
+        *(V2 *)(c_sorted + 2 * (v * M + s)) = *(const V2 *)(c + 2 * (v * M + idx));
 }
 
 inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
