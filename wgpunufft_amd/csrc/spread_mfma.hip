@@ -210,6 +210,7 @@ __global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ c, int
     const float4 *c4 = (const float4 *)cv;
     const bool aligned = ((uintptr_t)cv & 15) == 0;
     if (aligned) {
+#pragma unroll 4
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
              i += (int64_t)gridDim.x * blockDim.x) {
             const float4 q = c4[i];
