@@ -142,6 +142,21 @@ int mnufft_bin_sort(mnufft_plan plan, int64_t M, const void *d_points, uint32_t 
 int mnufft_bin_count(mnufft_plan plan, int64_t M, int64_t *nbins);
 
 /*
+ * The binning the executes of a type-1 plan use, as left by its last
+ * set_points (or by an execute that re-binned the kept points): d_sorted
+ * receives the M point indices in record-slot order (the order the spread
+ * sums them in), d_offsets (may be NULL) the nbins + 1 bin offsets, bin_side
+ * (may be NULL, 3 values) the bin side per axis.  For the index point order
+ * this is the reference's stable bin permutation of that geometry (the
+ * contract of mnufft_bin_sort).  Reads nothing but the plan's own scratch;
+ * MNUFFT_ERROR_INVALID_ARGUMENT when no points are binned or dense bins
+ * hold records apart.  Call mnufft_bin_count-sized buffers: *nbins receives
+ * the bin count first.
+ */
+int mnufft_points_permutation(mnufft_plan plan, uint32_t *d_sorted, uint32_t *d_offsets,
+                              int64_t *nbins, int64_t *bin_side, void *stream);
+
+/*
  * Plan facts, for tests and benchmarks: info[0] kernel width, [1..3] fine
  * lengths, [4..6] bin sides of the geometry last chosen (type 1) or used
  * (type 2), [7] its bin count, [8] geometry index, [9..11] spread block edge,
@@ -269,6 +284,16 @@ int mnufft_direct_f64(int32_t kind, int32_t dim, const int64_t *n_modes, int32_t
  */
 int mnufft_kernel_params(double eps, double sigma, int32_t *width, double *beta);
 int64_t mnufft_fine_grid_size(int64_t n_modes, double sigma, int32_t width);
+/*
+ *  - mnufft_tolerance_check: the plan-time tolerance checks of a plan of
+ *    `dim` (1..8) axes -- the ES kernel of eps at sigma and, for
+ *    MNUFFT_PRECISION_F32, the estimate of the F32 rounding error against
+ *    the tolerance (validate_f32_rounding, src/config.rs:252-284).  0 when a
+ *    plan of that shape could meet eps, else MNUFFT_ERROR_PLAN (also for a
+ *    shape no fine grid fits).  mnufft_makeplan runs the same checks.
+ */
+int mnufft_tolerance_check(int32_t dim, const int64_t *n_modes, double eps, double sigma,
+                           int32_t precision);
 int mnufft_layout_query(int32_t dim, const int64_t *fine, int32_t width, int32_t precision,
                         int64_t point_count, int64_t *out);
 

@@ -89,3 +89,73 @@ def test_product_host_math_equals_oracle():
                     d = len(shape)
                     got_geoms = [[out[16 + 3 * g + a] for a in range(d)] for g in range(out[3])]
                     assert got_geoms == ref["geometries"]
+
+
+def _tolerance_check(n_modes, eps, sigma, precision):
+    import wgpunufft_amd as W
+    nm = (ctypes.c_int64 * len(n_modes))(*n_modes)
+    return W.lib().mnufft_tolerance_check(len(n_modes), nm, eps, sigma, precision)
+
+
+def test_f32_plans_reject_tolerances_their_rounding_misses_by_far():
+    """Known answers of the reference's own test
+    (src/config.rs:324-341, f32_plans_reject_tolerances_their_rounding_misses_by_far):
+    at sigma 1.25 and eps 1e-6 F32 is rejected in 2D, 3D and 4D while F64 and
+    Df64 are accepted; looser tolerances and sigma 2 stay reachable."""
+    F64, DF64, F32 = 1, 2, 3
+    for n_modes in ([256, 256], [64] * 3, [16] * 4):
+        assert _tolerance_check(n_modes, 1e-6, 1.25, F32) == 2
+        for prec in (F64, DF64):
+            assert _tolerance_check(n_modes, 1e-6, 1.25, prec) == 0
+    assert _tolerance_check([16] * 4, 1e-4, 1.25, F32) == 0
+    assert _tolerance_check([6, 5, 4, 4], 1e-3, 1.25, F32) == 0
+    for n_modes in ([1 << 20], [1024, 1024], [128] * 3, [16] * 6):
+        assert _tolerance_check(n_modes, 1e-9, 2.0, F32) == 0
+
+
+def test_tolerance_check_estimate_is_the_rms_gain():
+    """The F32 estimate is eps_f32/2 times the per-axis RMS of
+    phi_hat(0)/phi_hat(k) over the modes, rejected above 10 eps and 1e-5:
+    an independent evaluation of that product (oracle phi_hat) brackets the
+    rejection threshold -- the check accepts just above the estimate/10 and
+    rejects just below it."""
+    checked = 0
+    for n_modes, sigma in (([256, 256], 1.25), ([512, 128], 1.25), ([1000, 1000], 1.25),
+                           ([64] * 3, 1.25)):
+        w, beta = O.es_params(1e-6, sigma)
+        est = float(np.finfo(np.float32).eps) / 2.0
+        for n in n_modes:
+            coeffs = np.asarray(O.phihat(w, beta, O.fine_size(n, sigma, w)))
+            k = np.abs(np.arange(n) - n // 2)
+            est *= np.sqrt(np.mean((coeffs[0] / np.abs(coeffs[k])) ** 2))
+        if est <= 1e-5:
+            continue
+        # eps = est/10 * (1 +- 1e-3): the ES kernel is the same at 1e-6 only
+        # when eps keeps the width, so compare through a fixed-width window
+        lo, hi = est / 10.0 * 0.999, est / 10.0 * 1.001
+        if O.es_params(lo, sigma)[0] != w or O.es_params(hi, sigma)[0] != w:
+            continue
+        assert _tolerance_check(n_modes, hi, sigma, 3) == 0
+        assert _tolerance_check(n_modes, lo, sigma, 3) == 2
+        checked += 1
+    assert checked >= 3
+
+
+def test_oversize_point_counts_are_plan_errors_before_any_allocation():
+    """Point counts beyond the 32-bit index space are rejected as plan
+    errors (code 2) by the one-call host ABI before a plan or a device
+    buffer exists (no arrays of that size are passed: the check comes
+    first)."""
+    import wgpunufft_amd as W
+    L = W.lib()
+    x = (ctypes.c_double * 4)()
+    c = (ctypes.c_double * 8)()
+    f = (ctypes.c_double * 64)()
+    big = ctypes.c_int64(1 << 32)
+    assert L.wgpu_nufft1d1(big, x, c, 1, ctypes.c_double(1e-6), ctypes.c_int64(8), f, None) == 2
+    assert b"32-bit" in L.wgpu_nufft_last_error()
+    xf = (ctypes.c_float * 4)()
+    cf = (ctypes.c_float * 8)()
+    ff = (ctypes.c_float * 64)()
+    assert L.wgpu_nufftf1d3(ctypes.c_int64(4), xf, cf, 1, ctypes.c_double(1e-6), big, xf, ff,
+                            None) == 2

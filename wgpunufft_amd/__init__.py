@@ -154,6 +154,7 @@ class _PlanBase:
         them off.  Takes effect at the next set_points or execute."""
         check(lib().mnufft_set_dense_bins(self._handle, int(min_records)))
         return self
+        return self
 
     def autotune(self, points, values, out=None, repeats=3, active_batch=None):
         """Type 1: times `repeats` encodes of these points and strengths for
@@ -290,6 +291,26 @@ class NufftPlan(_PlanBase):
         check(lib().mnufft_bin_sort(self._handle, M, _ptr(points), _ptr(sorted_idx),
                                     _ptr(offsets), ctypes.byref(n), _stream_ptr()))
         return sorted_idx[:M], offsets
+
+
+    def points_permutation(self):
+        """The binning the plan's executes use after its last set_points:
+        (point indices in record-slot order, bin offsets, bin side per axis).
+        For the index point order this is the reference's stable bin
+        permutation of the plan's geometry."""
+        torch = _torch()
+        if self._points is None:
+            raise NufftError(1, "points must be set first")
+        M = self._points.shape[0]
+        nb = ctypes.c_int64()
+        side = (ctypes.c_int64 * 3)()
+        check(lib().mnufft_points_permutation(self._handle, None, None, ctypes.byref(nb), side,
+                                              _stream_ptr()))
+        sorted_idx = torch.empty(max(M, 1), dtype=torch.int32, device="cuda")
+        offsets = torch.empty(nb.value + 1, dtype=torch.int32, device="cuda")
+        check(lib().mnufft_points_permutation(self._handle, _ptr(sorted_idx), _ptr(offsets),
+                                              ctypes.byref(nb), side, _stream_ptr()))
+        return sorted_idx[:M], offsets, list(side)[: self.config.dimensions]
 
 
 class NufftType3Plan(_PlanBase):

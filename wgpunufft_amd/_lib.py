@@ -48,6 +48,7 @@ SIGNATURES = [
                                             ctypes.c_int64, _vp, _vp, _vp]),
     ("mnufft_bin_sort", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _c_i64p, _vp]),
     ("mnufft_bin_count", ctypes.c_int, [_vp, ctypes.c_int64, _c_i64p]),
+    ("mnufft_points_permutation", ctypes.c_int, [_vp, _vp, _vp, _c_i64p, _c_i64p, _vp]),
     ("mnufft_plan_info", ctypes.c_int, [_vp, _c_i64p, ctypes.c_int32]),
     ("mnufft_plan_beta", ctypes.c_double, [_vp]),
     ("mnufft_stage_times", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
@@ -63,6 +64,8 @@ SIGNATURES = [
                                             ctypes.POINTER(ctypes.c_int32),
                                             ctypes.POINTER(ctypes.c_double)]),
     ("mnufft_fine_grid_size", ctypes.c_int64, [ctypes.c_int64, ctypes.c_double, ctypes.c_int32]),
+    ("mnufft_tolerance_check", ctypes.c_int, [ctypes.c_int32, _c_i64p, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int32]),
     ("mnufft_layout_query", ctypes.c_int, [ctypes.c_int32, _c_i64p, ctypes.c_int32,
                                            ctypes.c_int32, ctypes.c_int64, _c_i64p]),
     ("mnufft_direct_f64", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _c_i64p, ctypes.c_int32,

@@ -102,15 +102,11 @@ def config3(args, W, S, torch):
 def config4(args, W, S, torch):
     dev = torch.device("cuda:0")
     # (a) 64 independent 2D type-2 problems
-    rng_m = S.uniform01(S.config_seed(4, 1), 64 * 3).reshape(64, 3)
     problems = []
     total_pts = 0
-    for i in range(64):
-        n0 = int(round(np.exp(np.log(64) + rng_m[i, 0] * (np.log(512) - np.log(64)))))
-        n1 = int(round(np.exp(np.log(64) + rng_m[i, 1] * (np.log(512) - np.log(64)))))
-        M = int(round(np.exp(np.log(1e4) + rng_m[i, 2] * (np.log(2e6) - np.log(1e4)))))
-        x = torch.from_numpy(S.uniform_coords(S.config_seed(4, 100 + i), M, 2)).to(dev)
-        f = torch.from_numpy(S.complex_normal(S.config_seed(4, 200 + i), n0 * n1)).to(dev)
+    for n0, n1, M, sx, sf in S.batch_problem_specs():
+        x = torch.from_numpy(S.uniform_coords(sx, M, 2)).to(dev)
+        f = torch.from_numpy(S.complex_normal(sf, n0 * n1)).to(dev)
         plan = W.NufftPlan.type2_gpu(W.NufftConfig([n0, n1], 1e-5))
         out = torch.empty(M, dtype=torch.complex64, device=dev)
         problems.append((plan, x, f, out))

@@ -196,10 +196,7 @@ int mnufft_set_points(mnufft_plan plan, int64_t M, const void *d_points, void *s
             set_error("type-3 plans take their points in mnufft_execute_type3");
             return MNUFFT_ERROR_INVALID_ARGUMENT;
         }
-        if (M > (int64_t)UINT32_MAX - 1) {
-            set_error("point count exceeds the 32-bit point index space");
-            return MNUFFT_ERROR_PLAN;
-        }
+        if (!mnufft::point_count_ok(M, "point")) return MNUFFT_ERROR_PLAN;
         if (!plan->plan->set_points(M, d_points, (hipStream_t)stream)) return MNUFFT_ERROR_GPU;
         return MNUFFT_SUCCESS;
     });
@@ -290,6 +287,10 @@ int mnufft_execute_type3(mnufft_plan plan, int64_t active_batch, int64_t M, cons
             set_error("active batch outside 1..batch");
             return MNUFFT_ERROR_INVALID_ARGUMENT;
         }
+        if (M > mnufft::kMaxPoints || N > mnufft::kMaxPoints) {
+            (void)(mnufft::point_count_ok(M, "source") && mnufft::point_count_ok(N, "target"));
+            return MNUFFT_ERROR_PLAN;
+        }
         if (M < 0 || N < 0 || (M > 0 && (!d_sources || !d_strengths)) ||
             (N > 0 && (!d_targets || !d_out))) {
             set_error("invalid counts or null buffers");
@@ -322,8 +323,43 @@ int mnufft_bin_sort(mnufft_plan plan, int64_t M, const void *d_points, uint32_t 
             set_error("bin sorting needs a type-1 plan and device buffers");
             return MNUFFT_ERROR_INVALID_ARGUMENT;
         }
+        if (!mnufft::point_count_ok(M, "point")) return MNUFFT_ERROR_PLAN;
         if (nbins) *nbins = plan->plan->type1_geometry(M, nullptr).nbins;
         if (!plan->plan->bin_sort(M, d_points, d_sorted, d_offsets, (hipStream_t)stream))
+            return MNUFFT_ERROR_GPU;
+        return MNUFFT_SUCCESS;
+    });
+}
+
+int mnufft_points_permutation(mnufft_plan plan, uint32_t *d_sorted, uint32_t *d_offsets,
+                              int64_t *nbins, int64_t *bin_side, void *stream)
+{
+    return guarded([&] {
+        if (!plan || plan->plan->type != 1) {
+            set_error("the point permutation is defined for type-1 plans");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        Plan &p = *plan->plan;
+        if (!p.points_set || p.binned_dense != 0u) {
+            set_error(p.points_set ? "dense bins keep their records apart from the bin order"
+                                   : "no points are binned");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (nbins) *nbins = p.gp.nbins;
+        if (bin_side)
+            for (int a = 0; a < 3; ++a)
+                bin_side[a] = a < p.dim ? ((int64_t)1 << p.gp.ax[a].shift) : 1;
+        hipStream_t st = (hipStream_t)stream;
+        if (p.point_count > 0 && d_sorted &&
+            !mnufft::hip_ok(hipMemcpyAsync(d_sorted, p.rec_perm, sizeof(uint32_t) * p.point_count,
+                                           hipMemcpyDeviceToDevice, st),
+                            "permutation copy"))
+            return MNUFFT_ERROR_GPU;
+        if (d_offsets &&
+            !mnufft::hip_ok(hipMemcpyAsync(d_offsets, p.sorter.offsets(),
+                                           sizeof(uint32_t) * (p.gp.nbins + 1),
+                                           hipMemcpyDeviceToDevice, st),
+                            "offsets copy"))
             return MNUFFT_ERROR_GPU;
         return MNUFFT_SUCCESS;
     });
@@ -414,6 +450,7 @@ int mnufft_autotune(mnufft_plan plan, int64_t active_batch, int64_t M, const voi
             set_error("active batch outside 1..batch");
             return MNUFFT_ERROR_INVALID_ARGUMENT;
         }
+        if (!mnufft::point_count_ok(M, "point")) return MNUFFT_ERROR_PLAN;
         // candidates: the plain spread, and the matrix-core methods whose
         // operand error stays below the plan's tolerance, each 3D one with
         // dense bins on and off
@@ -580,6 +617,40 @@ int mnufft_kernel_params(double eps, double sigma, int32_t *width, double *beta)
     return MNUFFT_SUCCESS;
 }
 
+int mnufft_tolerance_check(int32_t dim, const int64_t *n_modes, double eps, double sigma,
+                           int32_t precision)
+{
+    return guarded([&] {
+        if (dim < 1 || dim > 8 || !n_modes) {
+            set_error("tolerance check takes 1 to 8 axes and their mode counts");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        const double sg = sigma == 0.0 ? 2.0 : sigma;
+        mnufft::EsKernel k;
+        if (!mnufft::es_kernel_for_tolerance(eps, sg, &k)) return MNUFFT_ERROR_PLAN;
+        std::vector<int64_t> nm((size_t)dim);
+        std::vector<std::vector<double>> coeffs;
+        for (int a = 0; a < dim; ++a) {
+            if (n_modes[a] < 1) {
+                set_error("mode count along axis " + std::to_string(a) + " must be positive");
+                return MNUFFT_ERROR_INVALID_ARGUMENT;
+            }
+            nm[(size_t)a] = n_modes[a];
+            const int64_t n = mnufft::select_fine_grid_size(n_modes[a], sg, k.width);
+            if (n < 0) {
+                set_error("no supported fine-grid length for axis " + std::to_string(a));
+                return MNUFFT_ERROR_PLAN;
+            }
+            coeffs.push_back(mnufft::centered_fourier_coefficients(k, n));
+        }
+        const Precision pr = precision == MNUFFT_PRECISION_F32    ? Precision::F32
+                             : precision == MNUFFT_PRECISION_DF64 ? Precision::Df64
+                                                                  : Precision::F64;
+        return mnufft::validate_f32_rounding(nm, eps, sg, pr, coeffs) ? MNUFFT_SUCCESS
+                                                                     : MNUFFT_ERROR_PLAN;
+    });
+}
+
 int64_t mnufft_fine_grid_size(int64_t n_modes, double sigma, int32_t width)
 {
     return mnufft::select_fine_grid_size(n_modes, sigma == 0.0 ? 2.0 : sigma, width);
@@ -738,23 +809,33 @@ int hp_makeplan(bool float_api, int32_t type, int32_t dim, const int64_t *n_mode
 
 // Interleaves per-axis host arrays (or takes point-major ones) into the
 // plan's arithmetic type, validating the coordinate range of types 1 and 2.
+// An F32 plan sees coordinates rounded to float, so they are rounded before
+// the checks, and lo/hi (type-3 bounds, per axis) come from the rounded
+// values, as the reference does (capi/src/plan.rs:486-499).
 template <typename S>
 bool pack_points(const HostPlan &hp, int64_t M, const S *const *axes, const S *nd,
-                 std::vector<unsigned char> *out, bool check_range, const char *what)
+                 std::vector<unsigned char> *out, bool check_range, const char *what,
+                 double *lo = nullptr, double *hi = nullptr)
 {
     int d = hp.dim;
     out->resize((size_t)M * d * hp.real());
+    for (int a = 0; a < d && lo && hi; ++a) lo[a] = hi[a] = 0.0;
     for (int64_t j = 0; j < M; ++j)
         for (int a = 0; a < d; ++a) {
             double v = nd ? (double)nd[j * d + a] : (double)axes[a][j];
+            if (hp.precision == Precision::F32) v = (double)(float)v;
             if (!std::isfinite(v) || (check_range && std::fabs(v) > kMaxCoordinate)) {
                 set_error(std::string(what) + " " + std::to_string(j) + " axis " +
                           std::to_string(a) + " is " + std::to_string(v) +
                           (check_range ? ", outside [-3*pi, 3*pi]" : ", not finite"));
                 return false;
             }
+            if (lo && hi) {
+                lo[a] = j == 0 ? v : std::min(lo[a], v);
+                hi[a] = j == 0 ? v : std::max(hi[a], v);
+            }
             if (hp.precision == Precision::F32) {
-                float f = nd ? (float)nd[j * d + a] : (float)axes[a][j];
+                const float f = (float)v;
                 std::memcpy(out->data() + ((size_t)j * d + a) * 4, &f, 4);
             } else {
                 std::memcpy(out->data() + ((size_t)j * d + a) * 8, &v, 8);
@@ -776,13 +857,18 @@ int hp_setpts(HostPlan *hp, int64_t M, const S *const *axes, const S *nd, int64_
                   " arrays: this call has the other array type");
         return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
     }
+    if (M > mnufft::kMaxPoints || N > mnufft::kMaxPoints) {
+        (void)(mnufft::point_count_ok(M, "point") && mnufft::point_count_ok(N, "target"));
+        return WGPU_NUFFT_ERROR_PLAN;
+    }
     if (M < 0 || (M > 0 && !nd && !axes[0]) || (hp->dim > 1 && M > 0 && !nd && !axes[1]) ||
         (hp->dim > 2 && M > 0 && !nd && !axes[2])) {
         set_error("point count or coordinate arrays invalid");
         return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
     }
     std::vector<unsigned char> pts;
-    if (!pack_points<S>(*hp, M, axes, nd, &pts, hp->type != 3, "point"))
+    double slo[3], shi[3];
+    if (!pack_points<S>(*hp, M, axes, nd, &pts, hp->type != 3, "point", slo, shi))
         return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
     if (!grow(&hp->d_points, &hp->points_bytes, pts.size())) return WGPU_NUFFT_ERROR_GPU;
     if (!pts.empty() &&
@@ -797,7 +883,8 @@ int hp_setpts(HostPlan *hp, int64_t M, const S *const *axes, const S *nd, int64_
             return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
         }
         std::vector<unsigned char> tg;
-        if (!pack_points<S>(*hp, N, taxes, tnd, &tg, false, "target"))
+        double tlo[3], thi[3];
+        if (!pack_points<S>(*hp, N, taxes, tnd, &tg, false, "target", tlo, thi))
             return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
         if (!grow(&hp->d_targets, &hp->targets_bytes, tg.size())) return WGPU_NUFFT_ERROR_GPU;
         if (!tg.empty() &&
@@ -805,24 +892,14 @@ int hp_setpts(HostPlan *hp, int64_t M, const S *const *axes, const S *nd, int64_
                             "target upload"))
             return WGPU_NUFFT_ERROR_GPU;
         hp->N = N;
-        // bounds from the actual points, as the plan's caller guarantee
+        // bounds from the points the plan computes with (rounded for F32),
+        // as the plan's caller guarantee
         double sb[6], tb[6];
         for (int a = 0; a < hp->dim; ++a) {
-            double lo = 0, hi = 0, tlo = 0, thi = 0;
-            for (int64_t j = 0; j < M; ++j) {
-                double v = nd ? (double)nd[j * hp->dim + a] : (double)axes[a][j];
-                lo = j == 0 ? v : std::min(lo, v);
-                hi = j == 0 ? v : std::max(hi, v);
-            }
-            for (int64_t k = 0; k < N; ++k) {
-                double v = tnd ? (double)tnd[k * hp->dim + a] : (double)taxes[a][k];
-                tlo = k == 0 ? v : std::min(tlo, v);
-                thi = k == 0 ? v : std::max(thi, v);
-            }
-            sb[2 * a] = lo;
-            sb[2 * a + 1] = hi;
-            tb[2 * a] = tlo;
-            tb[2 * a + 1] = thi;
+            sb[2 * a] = M > 0 ? slo[a] : 0.0;
+            sb[2 * a + 1] = M > 0 ? shi[a] : 0.0;
+            tb[2 * a] = N > 0 ? tlo[a] : 0.0;
+            tb[2 * a + 1] = N > 0 ? thi[a] : 0.0;
         }
         auto p = std::make_unique<Plan>();
         p->type = 3;
@@ -940,6 +1017,11 @@ int one_call(bool float_api, int type, int dim, int64_t M, const S *x, const S *
              S *c, int isign, double eps, const int64_t *nm, int64_t N, const S *s, const S *t,
              const S *u, S *f, const wgpu_nufft_opts *opts)
 {
+    // counts are checked before any plan or device buffer exists
+    if (M > mnufft::kMaxPoints || N > mnufft::kMaxPoints) {
+        (void)(mnufft::point_count_ok(M, "point") && mnufft::point_count_ok(N, "target"));
+        return WGPU_NUFFT_ERROR_PLAN;
+    }
     HostPlan *hp = nullptr;
     int rc = hp_makeplan(float_api, type, dim, nm, isign, 1, eps, &hp, opts);
     if (rc) return rc;

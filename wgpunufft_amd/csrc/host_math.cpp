@@ -23,6 +23,14 @@ bool hip_ok(hipError_t e, const char *what)
     return false;
 }
 
+bool point_count_ok(int64_t count, const char *what)
+{
+    if (count >= 0 && count <= kMaxPoints) return true;
+    set_error(std::string(what) + " count " + std::to_string(count) +
+              " is outside the 32-bit point index space (0.." + std::to_string(kMaxPoints) + ")");
+    return false;
+}
+
 // EsKernel::evaluate (reference src/kernel.rs:81-89).
 double EsKernel::evaluate(double distance) const
 {

@@ -19,6 +19,9 @@ namespace mnufft {
 
 constexpr int kMaxDims = 3;
 constexpr int kMaxWidth = 16;
+// Point indices, bin offsets, record slots and permutations are 32-bit:
+// a point set (or type-3 target set) holds at most this many points.
+constexpr int64_t kMaxPoints = (int64_t)UINT32_MAX - 1;
 constexpr int kMaxGeometries = 16;
 // Bins above this many points are ordered by the large-bin sort; smaller
 // ones are ranked inside the prepare pass (src/gpu_bin_sort.rs SMALL_BIN).
@@ -145,6 +148,8 @@ struct SpreadParams {
 void set_error(const std::string &message);
 const char *last_error();
 bool hip_ok(hipError_t e, const char *what);
+// False (with the last error set) for a count outside 0..kMaxPoints.
+bool point_count_ok(int64_t count, const char *what);
 
 // Stage names of the per-execution timing (src/gpu_profile.rs:12-42).
 enum Stage : int {

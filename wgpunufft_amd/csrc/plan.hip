@@ -423,6 +423,7 @@ void Plan::reset_marks() { (void)begin_call(); }
 template <int D, typename T>
 bool Plan::set_points_t(int64_t M, const T *pts, hipStream_t stream)
 {
+    if (!point_count_ok(M, "point")) return false;
     reset_marks();
     points = pts;
     point_count = M;
@@ -525,6 +526,7 @@ template <int D, typename T>
 bool Plan::bin_sort_t(int64_t M, const T *pts, uint32_t *sorted, uint32_t *offsets,
                       hipStream_t stream)
 {
+    if (!point_count_ok(M, "point")) return false;
     GridParams g = type1_geometry(M, nullptr);
     reset_marks();
     mark(kStageSort, false, stream);
@@ -642,6 +644,7 @@ template <int D, typename T>
 bool Plan::execute_type3_t(int64_t active, int64_t M, const T *x, const T *c, int64_t N,
                            const T *s, T *out, hipStream_t stream)
 {
+    if (!point_count_ok(M, "source") || !point_count_ok(N, "target")) return false;
     reset_marks();
     if (N == 0) return true;
     size_t rb = real_bytes();

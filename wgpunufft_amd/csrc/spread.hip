@@ -30,55 +30,6 @@ namespace {
 constexpr int kLanes = 64;
 constexpr int kKw = 16;  // weights per axis staged per record
 
-template <int D>
-struct BlockReach {
-    AxisReach ax[3];
-    int64_t b0[3];
-    int64_t nseg;
-    // segment t -> record slots [first, last)
-    __device__ void segment(int64_t t, const GridParams &g, const uint32_t *offsets,
-                            int64_t *first, int64_t *last) const
-    {
-        int64_t nxi = ax[0].len1 > 0 ? 2 : 1;
-        int64_t ix = t % nxi;
-        int64_t rest = t / nxi;
-        int64_t row = 0, stride = g.ax[0].bins;
-#pragma unroll
-        for (int a = 1; a < D; ++a) {
-            int64_t cnt = ax[a].count();
-            int64_t k = rest % cnt;
-            rest /= cnt;
-            row += ax[a].bin(k) * stride;
-            stride *= g.ax[a].bins;
-        }
-        int64_t lo = ix == 0 ? ax[0].lo0 : ax[0].lo1;
-        int64_t len = ix == 0 ? ax[0].len0 : ax[0].len1;
-        *first = offsets[row + lo];
-        *last = offsets[row + lo + len];
-    }
-};
-
-template <int D>
-__device__ inline BlockReach<D> block_reach(int64_t block, const GridParams &g,
-                                            const SpreadParams &sp)
-{
-    BlockReach<D> r;
-    int64_t rest = block;
-    r.nseg = 1;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-        int64_t bi = rest % sp.nblocks[a];
-        rest /= sp.nblocks[a];
-        r.b0[a] = bi * sp.block[a];
-        r.ax[a] = axis_reach(r.b0[a], sp.block[a], g.width, g.ax[a]);
-        if (a == 0)
-            r.nseg *= r.ax[0].len1 > 0 ? 2 : 1;
-        else
-            r.nseg *= r.ax[a].count();
-    }
-    return r;
-}
-
 // Records in reach of every block: one wavefront per block, lanes over the
 // block's bin segments.
 template <int D>

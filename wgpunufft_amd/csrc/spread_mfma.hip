@@ -50,15 +50,6 @@ struct MfmaRec {
     float cre, cim;  // scaled strength
 };
 
-// Every reach row of a tile (y start, then z start), staged once per work
-// item: its one or two slot ranges along x (the second when the reach wraps)
-// and the exclusive scan of the record counts: (16 + w - 1)^2 rows in 3D,
-// 32 + w - 1 in 2D.
-template <int N>
-struct TileRows {
-    uint32_t f0[N], n0[N], f1[N], excl[N + 1];
-};
-
 template <int D, int TY, int TZ, int W, int N>
 __device__ inline uint32_t stage_tile_rows(TileRows<N> &rows, int lane, int64_t y0, int64_t z0,
                                            const AxisReach &rx, const GridParams &g,
@@ -68,7 +59,6 @@ __device__ inline uint32_t stage_tile_rows(TileRows<N> &rows, int lane, int64_t 
     constexpr int NROWS = D == 3 ? NRY * (TZ + W - 1) : NRY;
     constexpr int NB = (NROWS + 63) / 64;
     static_assert(NROWS <= N, "reach rows beyond the staging arrays");
-    const int64_t n1 = g.ax[1].n, n2 = g.ax[2].n;
     // all loads first, then the scans
     uint32_t f0[NB], c0[NB], f1[NB], c1[NB];
 #pragma unroll
@@ -76,17 +66,7 @@ __device__ inline uint32_t stage_tile_rows(TileRows<N> &rows, int lane, int64_t 
         const int r = 64 * i + lane;
         f0[i] = c0[i] = f1[i] = c1[i] = 0;
         if (r < NROWS) {
-            int64_t sy = y0 - (W - 1) + r % NRY;
-            if (sy < 0) sy += n1;
-            if (sy >= n1) sy -= n1;
-            int64_t line = sy;
-            if constexpr (D == 3) {
-                int64_t sz = z0 - (W - 1) + r / NRY;
-                if (sz < 0) sz += n2;
-                if (sz >= n2) sz -= n2;
-                line += n1 * sz;
-            }
-            const int64_t row = g.ax[0].bins * line;
+            const int64_t row = tile_row_base<D, NRY, W>(r, y0, z0, g);
             f0[i] = offsets[row + rx.lo0];
             c0[i] = offsets[row + rx.lo0 + rx.len0];
             if (rx.len1 > 0) {
@@ -120,23 +100,6 @@ __device__ inline uint32_t stage_tile_rows(TileRows<N> &rows, int lane, int64_t 
     return carry;
 }
 
-// Slot of record k of the tile; *q = its reach row.
-template <int NROWS>
-__device__ inline uint32_t tile_slot(const TileRows<NROWS> &rows, uint32_t k, int *q)
-{
-    int lo = 0, hi = NROWS - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (rows.excl[mid] <= k)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    *q = lo;
-    const uint32_t j = k - rows.excl[lo];
-    return j < rows.n0[lo] ? rows.f0[lo] + j : rows.f1[lo] + (j - rows.n0[lo]);
-}
-
 // One lane's record of a chunk, with its global loads in flight until used.
 struct Fetched {
     int q, sx0;
@@ -164,39 +127,6 @@ __device__ inline Fetched fetch_record(const TileRows<NROWS> &rows, int64_t k, i
         f.c = *(const float2 *)(cv + 2 * (int64_t)slot);
     }
     return f;
-}
-
-// Work item -> (tile, part, record range) of the tile plan.
-struct TileItem {
-    int64_t b, p, rlo, rhi;
-    uint32_t np;
-};
-
-__device__ inline TileItem tile_item(int64_t item, int64_t block_count,
-                                     const uint32_t *__restrict__ item_off,
-                                     const uint32_t *__restrict__ parts,
-                                     const uint32_t *__restrict__ totals, uint32_t part_len_cap)
-{
-    int64_t lo = 0, hi = block_count - 1;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi + 1) >> 1;
-        if ((int64_t)item_off[mid] <= item)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    TileItem t;
-    t.b = lo;
-    t.p = item - item_off[lo];
-    t.np = parts[lo];
-    const int64_t total = totals[lo];
-    t.rlo = 0;
-    t.rhi = total;
-    if (t.np > 1) {
-        t.rlo = t.p * (int64_t)part_len_cap;
-        t.rhi = t.rlo + part_len_cap < total ? t.rlo + part_len_cap : total;
-    }
-    return t;
 }
 
 // Largest |re|, |im| of each vector's strengths (blockIdx.y), as float bits
@@ -408,8 +338,7 @@ k_spread_mfma3(
                     if (gx < n0 && gy < n1 && gz < n2)
                         grid[2 * (v * g.fine_total + gx + n0 * (gy + n1 * gz)) + bpart] = val;
                 } else {
-                    partial[2 * (((pslot * gridDim.y + v) * TZ + zz) * (kTile * kTile) +
-                                 yy * kTile + bxc) +
+                    partial[partial_index<kTile, kTile, TZ>(pslot, gridDim.y, v, bxc, yy, zz) +
                             bpart] = val;
                 }
             }
@@ -558,7 +487,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     if (gx < n0 && gy < n1)
                         grid[2 * (v * g.fine_total + gx + n0 * gy) + bpart] = val;
                 } else {
-                    partial[2 * ((pslot * gridDim.y + v) * (kTile2 * kTile2) + yy * kTile2 + xx) +
+                    partial[partial_index<kTile2, kTile2, 1>(pslot, gridDim.y, v, xx, yy, 0) +
                             bpart] = val;
                 }
             }

@@ -94,3 +94,17 @@ def radial_spokes(spokes=8192, samples=512, dtype=np.float32):
     t = -np.pi + 2 * np.pi * np.arange(samples, dtype=np.float64) / samples
     pts = (t[None, :, None] * d[:, None, :]).reshape(-1, 3)
     return pts.astype(dtype)
+
+
+def batch_problem_specs(count=64):
+    """The 64 independent 2D type-2 problems of the batching workload:
+    (n0, n1, M, coordinate seed, mode seed) per problem, modes per axis
+    log-uniform in [64, 512], M log-uniform in [1e4, 2e6]."""
+    r = uniform01(config_seed(4, 1), count * 3).reshape(count, 3)
+    out = []
+    for i in range(count):
+        n0 = int(round(np.exp(np.log(64) + r[i, 0] * (np.log(512) - np.log(64)))))
+        n1 = int(round(np.exp(np.log(64) + r[i, 1] * (np.log(512) - np.log(64)))))
+        M = int(round(np.exp(np.log(1e4) + r[i, 2] * (np.log(2e6) - np.log(1e4)))))
+        out.append((n0, n1, M, config_seed(4, 100 + i), config_seed(4, 200 + i)))
+    return out
