@@ -19,12 +19,14 @@ def _setup():
     return torch, W, O, S
 
 
-@pytest.mark.parametrize("n_modes,eps,allowed", [
-    ([32, 28, 24], 1e-6, {"plain", "mfma_f16x3"}),
-    ([32, 28, 24], 1e-3, {"plain", "mfma_f16x3", "mfma_bf16x3", "mfma_f16"}),
-    ([64, 48], 1e-6, {"plain", "mfma_f16x3"}),
+# 2D plans offer the plain spread only: the 2D matrix-core tiles are switched
+# off in the engine (an open GPU memory fault at 1024^2 modes, 5e7 points).
+@pytest.mark.parametrize("n_modes,eps,allowed,min_cands", [
+    ([32, 28, 24], 1e-6, {"plain", "mfma_f16x3"}, 2),
+    ([32, 28, 24], 1e-3, {"plain", "mfma_f16x3", "mfma_bf16x3", "mfma_f16"}, 2),
+    ([64, 48], 1e-6, {"plain"}, 1),
 ])
-def test_autotune_picks_an_allowed_spread_and_meets_tolerance(n_modes, eps, allowed):
+def test_autotune_picks_an_allowed_spread_and_meets_tolerance(n_modes, eps, allowed, min_cands):
     torch, W, O, S = _setup()
     d = len(n_modes)
     M = 150000
@@ -36,7 +38,7 @@ def test_autotune_picks_an_allowed_spread_and_meets_tolerance(n_modes, eps, allo
     choice = plan.autotune(xt, ct, repeats=2)
     print(f"AUTOTUNE {n_modes} eps={eps}: {choice}", flush=True)
     assert choice["spread_method"] in allowed
-    assert choice["candidates"] >= 2
+    assert choice["candidates"] >= min_cands
     f1 = plan.encode_gpu(xt, ct).clone()
     f2 = plan.execute_gpu(ct).clone()
     torch.cuda.synchronize()

@@ -39,6 +39,11 @@ def _pts(S, kind, M, seed, d=3):
     return x
 
 
+# The 2D matrix-core tiles are switched off in the engine: the 1024^2-mode,
+# 5e7-point plan ended in a GPU memory fault whose cause is not found.
+OFF_2D = pytest.mark.skip(reason="2D matrix-core spread switched off: open GPU memory fault "
+                                 "at 1024^2 modes, 5e7 points")
+
 CASES = [
     # (method, n_modes, isign, modeord, kind, M)
     ("mfma_f16x3", [16, 14, 12], 1, 0, "uniform", 4000),
@@ -46,10 +51,10 @@ CASES = [
     ("mfma_bf16x3", [16, 14, 12], 1, 0, "clustered", 4000),
     ("mfma_f16", [16, 14, 12], -1, 0, "uniform", 4000),
     ("mfma_f16", [24, 20, 18], 1, 1, "clustered", 5000),
-    ("mfma_f16x3", [40, 36], 1, 0, "uniform", 6000),
-    ("mfma_f16x3", [70, 44], -1, 1, "clustered", 8000),
-    ("mfma_bf16x3", [48, 40], 1, 0, "uniform", 6000),
-    ("mfma_f16", [64, 50], -1, 0, "clustered", 8000),
+    pytest.param("mfma_f16x3", [40, 36], 1, 0, "uniform", 6000, marks=OFF_2D),
+    pytest.param("mfma_f16x3", [70, 44], -1, 1, "clustered", 8000, marks=OFF_2D),
+    pytest.param("mfma_bf16x3", [48, 40], 1, 0, "uniform", 6000, marks=OFF_2D),
+    pytest.param("mfma_f16", [64, 50], -1, 0, "clustered", 8000, marks=OFF_2D),
 ]
 
 
@@ -75,6 +80,7 @@ def test_mfma_type1_matches_direct_oracle(method, n_modes, isign, modeord, kind,
     assert d <= bound
 
 
+@OFF_2D
 def test_mfma_2d_bitwise_batch_shuffle_and_rebinning():
     """2D tiles: bitwise repeat, shuffle invariance and batch == loop; a
     method switch after set_points re-bins the kept points."""
@@ -172,6 +178,9 @@ def test_set_spread_method_rejects_unsupported_plans():
             .set_spread_method("mfma_f16x3")
     with pytest.raises(Exception):
         W.NufftPlan.type1_gpu(W.NufftConfig([12, 10], 1e-6)).set_spread_method("mfma_f16x3")
+    # 2D matrix-core tiles are switched off at every size
+    with pytest.raises(Exception):
+        W.NufftPlan.type1_gpu(W.NufftConfig([64, 50], 1e-6)).set_spread_method("mfma_f16x3")
     with pytest.raises(Exception):
         W.NufftPlan.type1_gpu(W.NufftConfig([200], 1e-6)).set_spread_method("mfma_f16x3")
     with pytest.raises(Exception):

@@ -424,11 +424,18 @@ int mnufft_set_spread_method(mnufft_plan plan, int32_t method)
         set_error("unknown spread method");
         return MNUFFT_ERROR_INVALID_ARGUMENT;
     }
+    if (method != MNUFFT_SPREAD_PLAIN && q.type == 1 && q.dim == 2) {
+        // The 2D matrix-core tiles (32 x 32 cells) ended in a GPU memory
+        // fault on a 1024^2-mode, 5e7-point plan whose cause is not found;
+        // they stay switched off until it is.
+        set_error("the 2D matrix-core spread is switched off (an open GPU memory fault at "
+                  "5e7 points); 2D plans spread in plain FP32");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
     if (method != MNUFFT_SPREAD_PLAIN &&
-        (q.type != 1 || q.precision != Precision::F32 || !((q.dim == 3 && q.sweep) ||
-                                                           (q.dim == 2 && q.tiles2)))) {
-        set_error("matrix-core spreading needs an FP32 type-1 plan, 3D with at least 16 + width "
-                  "fine cells per axis or 2D with at least 32 + width, and a width <= 8");
+        (q.type != 1 || q.precision != Precision::F32 || !(q.dim == 3 && q.sweep))) {
+        set_error("matrix-core spreading needs an FP32 3D type-1 plan with at least 16 + width "
+                  "fine cells per axis and a width <= 8");
         return MNUFFT_ERROR_INVALID_ARGUMENT;
     }
     q.spread_method = (mnufft::SpreadMethod)method;
@@ -459,8 +466,8 @@ int mnufft_autotune(mnufft_plan plan, int64_t active_batch, int64_t M, const voi
             uint32_t dense;
         };
         std::vector<Cand> cands{{mnufft::SpreadMethod::Plain, 0u}};
-        const bool mfma_ok = p.precision == Precision::F32 &&
-                             ((p.dim == 3 && p.sweep) || (p.dim == 2 && p.tiles2));
+        // (the 2D matrix-core tiles are switched off: an open GPU fault)
+        const bool mfma_ok = p.precision == Precision::F32 && p.dim == 3 && p.sweep;
         if (mfma_ok) {
             std::vector<mnufft::SpreadMethod> ms{mnufft::SpreadMethod::MfmaF16x3};
             if (p.kernel.eps >= 1e-4) ms.push_back(mnufft::SpreadMethod::MfmaBf16x3);

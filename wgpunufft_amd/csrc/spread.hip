@@ -1017,8 +1017,9 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
         set_error("at most 65535 vectors per execution");
         return false;
     }
-    if (mfma && !((D == 2 || D == 3) && sizeof(T) == 4 && tiles)) {
-        set_error("matrix-core spread needs an FP32 2D or 3D plan on row tiles");
+    if (mfma && !(D == 3 && sizeof(T) == 4 && tiles)) {
+        // the 2D matrix-core tiles are switched off: an open GPU memory fault
+        set_error("matrix-core spread needs an FP32 3D plan on row tiles");
         return false;
     }
     if (!mfma && tiles && D != 3) {
