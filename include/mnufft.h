@@ -137,6 +137,25 @@ int mnufft_execute_type3(mnufft_plan plan, int64_t active_batch, int64_t M, cons
 int mnufft_bin_sort(mnufft_plan plan, int64_t M, const void *d_points, uint32_t *d_sorted,
                     uint32_t *d_offsets, int64_t *nbins, void *stream);
 
+/*
+ * mnufft_bin_sort without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same permutation and offsets as mnufft_bin_sort, bit for bit, but every
+ * intermediate (keys, values, histograms, counters, gap lists) lives in one
+ * static device array of the library (32 MiB per GPU the library runs on,
+ * held until the process ends); d_offsets is required (nbins + 1 values).
+ * At most MNUFFT_NOALLOC_MAX_POINTS points and MNUFFT_NOALLOC_MAX_BINS bins
+ * (MNUFFT_ERROR_INVALID_ARGUMENT beyond).  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (two streams, two threads) race;
+ * calls in stream order are safe.
+ */
+#define MNUFFT_NOALLOC_MAX_POINTS 2097152
+#define MNUFFT_NOALLOC_MAX_BINS 16777216
+int mnufft_bin_sort_noalloc(mnufft_plan plan, int64_t M, const void *d_points,
+                            uint32_t *d_sorted, uint32_t *d_offsets, int64_t *nbins,
+                            void *stream);
+
 /* Bin count of the geometry a type-1 plan chooses for M points (sizes the
  * offsets buffer of mnufft_bin_sort). */
 int mnufft_bin_count(mnufft_plan plan, int64_t M, int64_t *nbins);

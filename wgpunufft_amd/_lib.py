@@ -47,6 +47,8 @@ SIGNATURES = [
     ("mnufft_execute_type3", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp,
                                             ctypes.c_int64, _vp, _vp, _vp]),
     ("mnufft_bin_sort", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _c_i64p, _vp]),
+    ("mnufft_bin_sort_noalloc", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _c_i64p,
+                                               _vp]),
     ("mnufft_bin_count", ctypes.c_int, [_vp, ctypes.c_int64, _c_i64p]),
     ("mnufft_points_permutation", ctypes.c_int, [_vp, _vp, _vp, _c_i64p, _c_i64p, _vp]),
     ("mnufft_plan_info", ctypes.c_int, [_vp, _c_i64p, ctypes.c_int32]),

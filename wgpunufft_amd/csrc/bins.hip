@@ -18,6 +18,8 @@
 // all workgroups (LDS bitonic runs, merge-path rounds).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "bins.hpp"
@@ -993,6 +995,16 @@ BinSorter::~BinSorter() { release(); }
 
 void BinSorter::release()
 {
+    if (external_) {
+        // bound to a caller's region: nothing here to free
+        ctrl_ = offsets_ = keys_ = keys_b_ = vals_ = sorted_ = hist_ = hist_sums_ = nullptr;
+        large_list_ = tile_off_ = nullptr;
+        gaps_ = nullptr;
+        scratch_a_ = scratch_b_ = nullptr;
+        bins_cap_ = points_cap_ = sort_scratch_cap_ = 0;
+        external_ = false;
+        return;
+    }
     for (uint32_t *p : {ctrl_, offsets_, keys_, keys_b_, vals_, sorted_, hist_, hist_sums_,
                         large_list_, tile_off_})
         if (p) (void)hipFree(p);
@@ -1018,8 +1030,84 @@ int64_t radix_tiles(int64_t M) { return (M + kRadixTile - 1) / kRadixTile; }
 
 }  // namespace
 
+namespace {
+
+// Region sizes of an external binding, in 256-byte units so every buffer
+// starts aligned.
+struct ExternalLayout {
+    size_t ctrl, keys, keys_b, vals, hist, hist_sums, gaps, total;
+    int64_t n_mid, n_long;
+};
+
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+ExternalLayout external_layout(int64_t nbins, int64_t M)
+{
+    ExternalLayout l{};
+    const int64_t m = M > 0 ? M : 1;
+    const int64_t hist = kDigits * radix_tiles(m);
+    l.n_mid = (nbins + 1) / (kGapInline + 1) + 2;
+    l.n_long = (nbins + 1) / (kGapMid + 1) + 2;
+    l.ctrl = round256(4 * sizeof(uint32_t));
+    l.keys = l.keys_b = l.vals = round256(sizeof(uint32_t) * (size_t)m);
+    l.hist = round256(sizeof(uint32_t) * (size_t)hist);
+    l.hist_sums = round256(sizeof(uint32_t) * (size_t)((hist + kScanTile - 1) / kScanTile + 1));
+    l.gaps = round256(sizeof(uint4) * (size_t)(l.n_mid + l.n_long));
+    l.total = l.ctrl + l.keys + l.keys_b + l.vals + l.hist + l.hist_sums + l.gaps;
+    return l;
+}
+
+}  // namespace
+
+size_t BinSorter::external_bytes(int64_t nbins, int64_t M)
+{
+    return external_layout(nbins, M).total;
+}
+
+bool BinSorter::bind_external(void *base, size_t bytes, int64_t nbins, int64_t M,
+                              uint32_t *sorted_out, uint32_t *offsets_out)
+{
+    release();
+    const ExternalLayout l = external_layout(nbins, M);
+    if (!base || !sorted_out || !offsets_out || l.total > bytes) {
+        set_error("the static sort scratch holds " + std::to_string(bytes) + " bytes; " +
+                  std::to_string(M) + " points in " + std::to_string(nbins) + " bins need " +
+                  std::to_string(l.total));
+        return false;
+    }
+    char *p = (char *)base;
+    auto take = [&p](size_t n) {
+        char *q = p;
+        p += n;
+        return q;
+    };
+    ctrl_ = (uint32_t *)take(l.ctrl);
+    keys_ = (uint32_t *)take(l.keys);
+    keys_b_ = (uint32_t *)take(l.keys_b);
+    vals_ = (uint32_t *)take(l.vals);
+    hist_ = (uint32_t *)take(l.hist);
+    hist_sums_ = (uint32_t *)take(l.hist_sums);
+    gaps_ = (uint4 *)take(l.gaps);
+    gaps_mid_cap_ = l.n_mid;
+    sorted_ = sorted_out;
+    offsets_ = offsets_out;
+    bins_cap_ = nbins;
+    points_cap_ = M > 0 ? M : 1;
+    sort_scratch_cap_ = 0;
+    external_ = true;
+    return true;
+}
+
 bool BinSorter::ensure(int64_t nbins, int64_t M, bool need_sort_scratch)
 {
+    if (external_) {
+        // a bound region is never grown: the sort must fit what was bound
+        if (nbins > bins_cap_ || M > points_cap_ || need_sort_scratch) {
+            set_error("sort outside the bound static scratch (index order, bound sizes only)");
+            return false;
+        }
+        return true;
+    }
     if (nbins > bins_cap_) {
         if (!alloc_u32(&offsets_, nbins + 1, "bin offsets")) return false;
         // gap lists: at most one gap of more than kGapInline (kGapMid) bins
@@ -1249,6 +1337,45 @@ bool gather_strengths(int64_t M, int64_t vectors, const uint32_t *perm, const T 
     k_gather_strengths<T><<<grid_for(M, 256), 256, 0, stream>>>(M, vectors, perm, c, c_sorted,
                                                                 tail);
     return hip_ok(hipGetLastError(), "strength gather launch");
+}
+
+// ---------------------------------------------------------------------
+// Static sort scratch of the entry point that allocates nothing: a
+// file-scope device array, one copy per GPU the library runs on (held until
+// the process ends).  Host code never passes `scratch` itself to a kernel:
+// its name is a host-side stand-in, and the device address comes from
+// hipGetSymbolAddress, once per device.
+// ---------------------------------------------------------------------
+namespace {
+constexpr size_t kStaticSortBytes = (size_t)32 << 20;
+static __device__ __align__(256) uint32_t scratch[kStaticSortBytes / sizeof(uint32_t)];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+size_t static_sort_bytes() { return kStaticSortBytes; }
+
+void *static_sort_scratch()
+{
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "current device")) return nullptr;
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device ordinal beyond the static scratch table");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        if (!hip_ok(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "static sort scratch lookup"))
+            return nullptr;
+        if (!ptr) {
+            set_error("static sort scratch lookup returned no address");
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
 }
 
 #define MNUFFT_BINS_INSTANTIATE(D, T)                                                             \

@@ -40,6 +40,17 @@ public:
     bool sort_many(int64_t M, const BatchProblem *probs, int nprob, uint64_t nbins,
                    hipStream_t stream);
 
+    // Index-order sorts in a caller-owned device region instead of buffers
+    // of their own: the keys, values, histograms, counters and gap lists are
+    // carved from [base, base + bytes) (nothing is allocated or freed), the
+    // final permutation goes to sorted_out (M values) and the offsets to
+    // offsets_out (nbins + 1 values).  False, with the last error set, when
+    // the region is too small for M points and nbins bins.
+    bool bind_external(void *base, size_t bytes, int64_t nbins, int64_t M, uint32_t *sorted_out,
+                       uint32_t *offsets_out);
+    // Bytes bind_external needs for M points and nbins bins.
+    static size_t external_bytes(int64_t nbins, int64_t M);
+
     const uint32_t *sorted() const { return sorted_; }
     const uint32_t *offsets() const { return offsets_; }
     int64_t nbins() const { return nbins_; }
@@ -70,8 +81,15 @@ private:
     void *scratch_a_ = nullptr;
     void *scratch_b_ = nullptr;
     int64_t bins_cap_ = 0, points_cap_ = 0, sort_scratch_cap_ = 0;
+    bool external_ = false;  // buffers bound by bind_external (never freed here)
     int64_t nbins_ = 0, M_ = 0;
 };
+
+// Device address, on the current device, of the library's static sort
+// scratch (a file-scope device array of static_sort_bytes() bytes), looked
+// up once per device; nullptr (last error set) when the lookup fails.
+void *static_sort_scratch();
+size_t static_sort_bytes();
 
 // Exclusive scan of n u32 values (wrapping) into out (may alias in); sums
 // holds scan_tiles(n) values of scratch.

@@ -331,6 +331,32 @@ int mnufft_bin_sort(mnufft_plan plan, int64_t M, const void *d_points, uint32_t 
     });
 }
 
+int mnufft_bin_sort_noalloc(mnufft_plan plan, int64_t M, const void *d_points,
+                            uint32_t *d_sorted, uint32_t *d_offsets, int64_t *nbins, void *stream)
+{
+    return guarded([&] {
+        if (!plan || plan->plan->type != 1 || !d_offsets || (M > 0 && (!d_points || !d_sorted))) {
+            set_error("bin sorting without allocation needs a type-1 plan, device points, a "
+                      "permutation buffer and an offsets buffer");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (!mnufft::point_count_ok(M, "point")) return MNUFFT_ERROR_PLAN;
+        const int64_t nb = plan->plan->type1_geometry(M, nullptr).nbins;
+        if (nbins) *nbins = nb;
+        if (M > MNUFFT_NOALLOC_MAX_POINTS || nb > MNUFFT_NOALLOC_MAX_BINS ||
+            mnufft::BinSorter::external_bytes(nb, M) > mnufft::static_sort_bytes()) {
+            set_error("bin sorting without allocation takes at most " +
+                      std::to_string(MNUFFT_NOALLOC_MAX_POINTS) + " points and " +
+                      std::to_string(MNUFFT_NOALLOC_MAX_BINS) + " bins (" + std::to_string(M) +
+                      " points in " + std::to_string(nb) + " bins asked)");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        if (!plan->plan->bin_sort_static(M, d_points, d_sorted, d_offsets, (hipStream_t)stream))
+            return MNUFFT_ERROR_GPU;
+        return MNUFFT_SUCCESS;
+    });
+}
+
 int mnufft_points_permutation(mnufft_plan plan, uint32_t *d_sorted, uint32_t *d_offsets,
                               int64_t *nbins, int64_t *bin_side, void *stream)
 {

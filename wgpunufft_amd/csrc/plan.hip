@@ -543,6 +543,21 @@ bool Plan::bin_sort_t(int64_t M, const T *pts, uint32_t *sorted, uint32_t *offse
     return true;
 }
 
+template <int D, typename T>
+bool Plan::bin_sort_static_t(int64_t M, const T *pts, uint32_t *sorted, uint32_t *offsets,
+                             hipStream_t stream)
+{
+    if (!point_count_ok(M, "point")) return false;
+    const GridParams g = type1_geometry(M, nullptr);
+    void *base = static_sort_scratch();
+    if (!base) return false;
+    // a sorter bound to the static region: its buffers are views, never
+    // allocated or freed
+    BinSorter s;
+    if (!s.bind_external(base, static_sort_bytes(), g.nbins, M, sorted, offsets)) return false;
+    return s.sort<D, T>(M, pts, g, BinOrder::Index, stream);
+}
+
 // ----------------------------------------------------------------------
 // Type 3: rescale + pre-phase, raw spread onto the outer grid, target
 // rescale + correction, inner type 2 on the outer grid, final multiply
@@ -714,6 +729,14 @@ bool Plan::bin_sort(int64_t M, const void *pts, uint32_t *sorted, uint32_t *offs
                     hipStream_t stream)
 {
 #define BS(D, T) bin_sort_t<D, T>(M, (const T *)pts, sorted, offsets, stream)
+    MNUFFT_DISPATCH(*this, BS, BS);
+#undef BS
+}
+
+bool Plan::bin_sort_static(int64_t M, const void *pts, uint32_t *sorted, uint32_t *offsets,
+                           hipStream_t stream)
+{
+#define BS(D, T) bin_sort_static_t<D, T>(M, (const T *)pts, sorted, offsets, stream)
     MNUFFT_DISPATCH(*this, BS, BS);
 #undef BS
 }

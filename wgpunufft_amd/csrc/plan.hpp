@@ -149,6 +149,11 @@ struct Plan {
     int64_t batch_table_cap = 0;
     bool bin_sort(int64_t M, const void *pts, uint32_t *sorted, uint32_t *offsets,
                   hipStream_t stream);
+    // The same permutation and offsets as bin_sort, with every intermediate
+    // in the library's static sort scratch: enqueues kernels only (no
+    // allocation, no timing marks).
+    bool bin_sort_static(int64_t M, const void *pts, uint32_t *sorted, uint32_t *offsets,
+                         hipStream_t stream);
     bool execute_type3(int64_t active, int64_t M, const void *x, const void *c, int64_t N,
                        const void *s, void *out, hipStream_t stream);
     bool stage_times(double *ms, int n, int64_t *calls);
@@ -160,6 +165,9 @@ struct Plan {
     template <int D, typename T>
     bool bin_sort_t(int64_t M, const T *pts, uint32_t *sorted, uint32_t *offsets,
                     hipStream_t stream);
+    template <int D, typename T>
+    bool bin_sort_static_t(int64_t M, const T *pts, uint32_t *sorted, uint32_t *offsets,
+                           hipStream_t stream);
     template <int D, typename T>
     bool execute_type3_t(int64_t active, int64_t M, const T *x, const T *c, int64_t N,
                          const T *s, T *out, hipStream_t stream);
