@@ -54,8 +54,10 @@ def parse():
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
-    ap.add_argument("--cpu-points", type=int, default=3_000_000,
-                    help="points of the CPU baseline sample")
+    ap.add_argument("--cpu-points", type=int, default=0,
+                    help="points of the CPU baseline sample (0: the whole workload)")
+    ap.add_argument("--cpu-reps", type=int, default=5,
+                    help="executes of each transform the CPU baseline takes the median of")
     ap.add_argument("--spread", default="mfma_f16x3",
                     choices=["mfma_f16x3", "mfma_bf16x3", "mfma_f16"],
                     help="matrix-core spread of the timed type-1 (the plain FP32 spread is "
@@ -323,25 +325,39 @@ def accuracy(W, S, np, torch, dev, x, c, fm, n_modes, f_out, c_out):
 
 def cpu_baseline(args, np, S, n_modes):
     """CPU port of the reference's CpuNufftPlan (oracle/cpu_nufft.c), timed
-    on this host on the first cpu_points points of the same workload."""
+    on this host's cores on the same workload (its first cpu_points points
+    when a sample is asked for): medians of cpu_reps executes of type 1
+    (split into its point sort, prepare_points, and the rest,
+    execute_prepared) and of type 2 (src/cpu.rs; benches/nufft_cpu_bench.rs
+    of the reference reports the same three figures)."""
     try:
         from oracle import cpu_nufft
     except Exception as e:  # the port is optional for the GPU numbers
         return {"value": None, "unit": "points/s", "error": str(e)}
-    m = args.cpu_points
+    m = args.cpu_points or args.points
     full = S.config2_points(args.points)
     half = args.points // 2
     # keep the workload's uniform / clustered mix
-    x = np.concatenate([full[: m // 2], full[half: half + m - m // 2]])
+    x = full if m == args.points else np.concatenate([full[: m // 2],
+                                                      full[half: half + m - m // 2]])
     c = S.complex_normal(S.config_seed(2, 3), m)
     fm = S.complex_normal(S.config_seed(2, 4), int(np.prod(n_modes)))
     threads = cpu_nufft.threads()
-    t1 = cpu_nufft.time_type1(x, c, n_modes, args.eps, threads)
-    t2 = cpu_nufft.time_type2(x, fm, n_modes, args.eps, threads)
-    return {"value": m / (t1 + t2), "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": f"first {m // 2} uniform + {m - m // 2} clustered points of the workload, "
-                      f"same 128^3 grid; median of 3 executes each of type 1 and type 2",
-            "type1_s": t1, "type2_s": t2}
+    st1 = cpu_nufft.time_type1_stages(x, c, n_modes, args.eps, threads, reps=args.cpu_reps)
+    t2 = cpu_nufft.time_type2(x, fm, n_modes, args.eps, threads, reps=args.cpu_reps)
+    t1 = st1["execute_s"]
+    sample = ("the whole workload" if m == args.points else
+              f"first {m // 2} uniform + {m - m // 2} clustered points of the workload")
+    out = {"value": m / (t1 + t2), "unit": "points/s", "cores": threads, "kind": "port",
+           "sample": f"{sample}, same 128^3 grid; median of {args.cpu_reps} executes each of "
+                     f"type 1 and type 2",
+           "threads_note": "threads = the job's CPU share on this host (OMP_NUM_THREADS), "
+                           "at most every online CPU",
+           "type1_s": t1, "type1_prepare_s": st1["prepare_s"],
+           "type1_execute_prepared_s": st1["execute_prepared_s"], "type2_s": t2,
+           "type1_points_per_s": m / t1, "type2_points_per_s": m / t2}
+    out.update(cpu_nufft.host_facts())
+    return out
 
 
 if __name__ == "__main__":

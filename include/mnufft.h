@@ -127,6 +127,19 @@ int mnufft_execute_type3(mnufft_plan plan, int64_t active_batch, int64_t M, cons
                          void *stream);
 
 /*
+ * Target boxes of a type-3 plan.  A target interval whose inner fine grid
+ * would exceed 2^30 cells (or 16 GiB) is cut into boxes along its longest
+ * axes; every box is a type-3 problem of its own with the box centre as
+ * target centre (the same outer and inner grids for every box), so the
+ * interval [-200, 200]^3 over sources in [-5, 5]^3 runs on one GPU.  No
+ * counterpart in the reference, which plans the whole interval at once
+ * (src/type3.rs:708-797).  min_boxes (may be NULL, dim values 1..64)
+ * re-plans with at least that many boxes per axis; boxes (may be NULL, dim
+ * values) receives the boxes per axis in use.
+ */
+int mnufft_type3_boxes(mnufft_plan plan, const int32_t *min_boxes, int32_t *boxes);
+
+/*
  * The reference's stable bin permutation of M points for a type-1 plan:
  * the count / scan / scatter / large-bin sort sequence of
  * NdPointBins::encode with NdBinOrder::Stable (src/gpu_nd_bins.rs:336-471,

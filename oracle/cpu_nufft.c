@@ -367,6 +367,20 @@ static long mode_fine_index(long k, long N, long n, long *m_out)
     return m < 0 ? m + n : m;
 }
 
+/* Wall time of the last type-1 call's point sort (prepare_points of the
+ * reference's CpuNufftPlan, src/cpu.rs); the rest of that call is its
+ * execute_prepared. */
+static double g_last_prepare_s = 0.0;
+
+double cpu_last_prepare_seconds(void) { return g_last_prepare_s; }
+
+static double now_seconds(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 /* One type-1 execute: sort + spread + FFT + deconvolution. */
 int cpu_type1(int d, const long *nm, double eps, int isign, long M, const float *x,
               const float *c, float *out, int threads)
@@ -374,7 +388,9 @@ int cpu_type1(int d, const long *nm, double eps, int isign, long M, const float 
     cplan p;
     if (cplan_init(&p, d, nm, eps, isign, threads)) return -1;
     sorted_points S;
+    const double t0 = now_seconds();
     sort_points(&p, M, x, &S);
+    g_last_prepare_s = now_seconds() - t0;
     cf *grid = malloc(sizeof(cf) * p.cells);
     pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
     spread_job J = {&p, x, c, &S, grid, 0, &mu};

@@ -17,7 +17,33 @@ def _lib():
                        ctypes.c_int]
         fn.restype = ctypes.c_int
     L.cpu_threads.restype = ctypes.c_int
+    L.cpu_last_prepare_seconds.restype = ctypes.c_double
     return L
+
+
+def host_facts():
+    """CPU model, sockets, cores, threads per core and online CPUs of this
+    host (from /proc/cpuinfo and sysconf)."""
+    import os
+    model, phys, cores, siblings = None, set(), None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys.add(v)
+                elif k == "cpu cores" and cores is None:
+                    cores = int(v)
+                elif k == "siblings" and siblings is None:
+                    siblings = int(v)
+    except OSError:
+        pass
+    tpc = siblings // cores if cores and siblings else None
+    return {"cpu_model": model, "sockets": len(phys) or None, "cores_per_socket": cores,
+            "threads_per_core": tpc, "online_cpus": os.cpu_count()}
 
 
 def threads():
@@ -63,6 +89,22 @@ def _median_time(fn, reps):
 
 def time_type1(x, c, n_modes, eps, nthreads, reps=3):
     return _median_time(lambda: type1(x, c, n_modes, eps, 1, nthreads), reps)
+
+
+def time_type1_stages(x, c, n_modes, eps, nthreads, reps=5):
+    """Medians over `reps` type-1 executes of the whole execute (sort
+    included), its point sort (prepare_points) and the rest
+    (execute_prepared), in seconds."""
+    L = _lib()
+    ex, pre = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        type1(x, c, n_modes, eps, 1, nthreads)
+        ex.append(time.perf_counter() - t0)
+        pre.append(L.cpu_last_prepare_seconds())
+    ep = [e - p for e, p in zip(ex, pre)]
+    return {"execute_s": float(np.median(ex)), "prepare_s": float(np.median(pre)),
+            "execute_prepared_s": float(np.median(ep))}
 
 
 def time_type2(x, f, n_modes, eps, nthreads, reps=3):

@@ -227,3 +227,42 @@ def test_errors_are_reported_with_messages():
     finally:
         lib.wgpu_nufft_destroy(plan)
     lib.wgpu_nufft_destroy(None)
+
+
+def test_one_call_transforms_reuse_kept_plans():
+    """Repeated one-call transforms reuse the plans they keep (the
+    reference keeps four, capi/src/cache.rs:10-51, and its GPU test repeats
+    the one-call suite, capi/tests/c_api.rs:698-702): new points on a kept
+    plan give the direct sums, more than four specs evict the oldest,
+    wgpu_nufft_shutdown drops them and later calls plan afresh."""
+    import time
+    lib = _lib()
+    lib.wgpu_nufft_shutdown()
+    o = _opts(lib, 0)
+    I64, D = ctypes.c_int64, ctypes.c_double
+    rng = Rng(0xCAC4E)
+
+    def call(m, nm):
+        x, y, z = (rng.values(m, np.pi) for _ in range(3))
+        c = rng.values(2 * m, 1.0)
+        f = np.zeros(2 * int(np.prod(nm)))
+        _check(lib, lib.wgpu_nufft3d1(I64(m), _p(x, False), _p(y, False), _p(z, False),
+                                      _p(c, False), 1, D(1e-9), I64(nm[0]), I64(nm[1]),
+                                      I64(nm[2]), _p(f, False), ctypes.byref(o)))
+        e = _err(f, _direct1([x, y, z], c, 1, nm))
+        assert e < 1e-8, f"{nm} m={m}: {e:.3e}"
+
+    t0 = time.perf_counter()
+    call(50, [6, 5, 4])
+    first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for m in (40, 70, 55):  # the kept plan, new point sets of other sizes
+        call(m, [6, 5, 4])
+    again = (time.perf_counter() - t0) / 3
+    print(f"one-call 3d1: first {first * 1e3:.1f} ms, kept plan {again * 1e3:.1f} ms", flush=True)
+    for k in range(5):  # five specs: the first one is evicted on the way
+        call(30, [4 + k, 5, 4])
+    call(45, [6, 5, 4])
+    lib.wgpu_nufft_shutdown()
+    call(35, [6, 5, 4])
+    lib.wgpu_nufft_shutdown()

@@ -300,6 +300,72 @@ def test_type3_matches_direct_oracle():
         assert err <= 100 * eps
 
 
+@pytest.mark.parametrize("dim,prec,eps,boxes", [
+    (3, "f64", 1e-9, [2, 2, 2]), (3, "f64", 1e-6, [3, 1, 2]), (3, "f32", 1e-5, [2, 3, 1]),
+    (2, "f32", 1e-5, [3, 2]), (1, "f64", 1e-10, [4])])
+def test_type3_target_boxes_match_oracle(dim, prec, eps, boxes):
+    """A type-3 plan whose target interval is cut into boxes (each box its
+    own type-3 problem around the box centre) gives the direct sums to the
+    plan's tolerance, like the uncut plan, for targets in every box and on
+    box edges."""
+    torch, W, O, S = _setup()
+    M, N = 3000, 2000
+    f64 = prec == "f64"
+    dt = np.float64 if f64 else np.float32
+    x = S.uniform_coords(S.config_seed(5, 70 + dim), M, dim, lo=-5.0, hi=5.0, dtype=dt)
+    s = S.uniform_coords(S.config_seed(5, 71 + dim), N, dim, lo=-30.0, hi=30.0, dtype=dt)
+    s[:3] = np.array([-30.0, 0.0, 29.999])[:, None]  # ends of the interval and a box edge
+    c = S.complex_normal(S.config_seed(5, 72 + dim), M,
+                         dtype=np.complex128 if f64 else np.complex64)
+    ref = O.direct_type3(x.astype(np.float64), c.astype(np.complex128), s.astype(np.float64))
+    xt, ct, st = (torch.from_numpy(a).cuda() for a in (x, c, s))
+    outs = {}
+    for name, kb in (("whole", None), ("boxes", boxes)):
+        plan = W.NufftType3Plan(dim, [(-5, 5)] * dim, [(-30, 30)] * dim, eps=eps,
+                                precision=W.F64 if f64 else W.F32)
+        if kb is not None:
+            assert plan.boxes(kb) == kb
+        else:
+            assert plan.boxes() == [1] * dim
+        outs[name] = plan.encode_gpu(xt, ct, st).cpu().numpy()
+        err = rel_l2(outs[name], ref)
+        print(f"T3 {name} d={dim} {prec} eps={eps} boxes={kb}: err={err:.3e}", flush=True)
+        assert err <= 100 * eps
+
+
+@pytest.mark.parametrize("dim,M,tgt", [(1, 300_000, 30.0), (2, 120_000, 12.0), (2, 700, 5.0)])
+def test_type3_small_outer_grid_f32(dim, M, tgt):
+    """F32 1D / 2D type-3 plans whose outer grid is small (at most 65,536
+    cells) spread their sources by point groups in input order with the
+    group partial grids added in group order (src/gpu_dense_spread.rs):
+    direct-sum tolerance, bitwise repeat, and a batch of two equal to the
+    vectors one by one."""
+    torch, W, O, S = _setup()
+    eps, N, B = 1e-5, 400, 2
+    x = S.uniform_coords(S.config_seed(5, 40 + dim), M, dim, lo=-5.0, hi=5.0)
+    s = S.uniform_coords(S.config_seed(5, 50 + dim), N, dim, lo=-tgt, hi=tgt)
+    c = S.complex_normal(S.config_seed(5, 60 + dim), M * B)
+    plan = W.NufftType3Plan(dim, [(-5, 5)] * dim, [(-tgt, tgt)] * dim, eps=eps,
+                            precision=W.F32, batch=B)
+    outer = plan.info()["outer"][:dim]
+    assert int(np.prod(outer)) <= 65536, outer
+    xt, st = torch.from_numpy(x).cuda(), torch.from_numpy(s).cuda()
+    ct = torch.from_numpy(c).cuda()
+    out = plan.encode_gpu(xt, ct, st).clone()
+    again = plan.encode_gpu(xt, ct, st)
+    torch.cuda.synchronize()
+    assert torch.equal(out.view(torch.float32), again.view(torch.float32))
+    for v in range(B):
+        one = plan.encode_gpu(xt, ct[v * M:(v + 1) * M].contiguous(), st, active_batch=1)
+        torch.cuda.synchronize()
+        assert torch.equal(one.view(torch.float32), out[v * N:(v + 1) * N].view(torch.float32))
+        ref = O.direct_type3(x.astype(np.float64), c[v * M:(v + 1) * M].astype(np.complex128),
+                             s.astype(np.float64))
+        err = rel_l2(out[v * N:(v + 1) * N].cpu().numpy(), ref)
+        print(f"T3 small grid d={dim} M={M} outer={outer} vector {v}: err={err:.3e}", flush=True)
+        assert err <= 100 * eps
+
+
 def test_gpu_direct_sums_match_oracle():
     torch, W, O, S = _setup()
     M = 500

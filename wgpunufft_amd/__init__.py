@@ -328,6 +328,17 @@ class NufftType3Plan(_PlanBase):
                                           ctypes.byref(h)))
         self._handle = h
 
+    def boxes(self, min_boxes=None):
+        """Target boxes per axis in use; min_boxes (one value per axis)
+        re-plans with at least that many.  Target intervals whose inner grid
+        would not fit are cut into boxes automatically."""
+        out = (ctypes.c_int32 * 3)()
+        arg = None
+        if min_boxes is not None:
+            arg = (ctypes.c_int32 * 3)(*(list(min_boxes) + [1, 1, 1])[:3])
+        check(lib().mnufft_type3_boxes(self._handle, arg, out))
+        return list(out[:self.dim])
+
     def encode_gpu(self, sources, strengths, targets, out=None, active_batch=None):
         torch = _torch()
         M = sources.shape[0]

@@ -50,6 +50,8 @@ SIGNATURES = [
     ("mnufft_bin_sort_noalloc", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _c_i64p,
                                                _vp]),
     ("mnufft_bin_count", ctypes.c_int, [_vp, ctypes.c_int64, _c_i64p]),
+    ("mnufft_type3_boxes", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32),
+                                          ctypes.POINTER(ctypes.c_int32)]),
     ("mnufft_points_permutation", ctypes.c_int, [_vp, _vp, _vp, _c_i64p, _c_i64p, _vp]),
     ("mnufft_plan_info", ctypes.c_int, [_vp, _c_i64p, ctypes.c_int32]),
     ("mnufft_plan_beta", ctypes.c_double, [_vp]),

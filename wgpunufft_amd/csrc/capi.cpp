@@ -175,6 +175,38 @@ int mnufft_makeplan_type3(int32_t dim, int32_t isign, int64_t batch, double eps,
     });
 }
 
+int mnufft_type3_boxes(mnufft_plan plan, const int32_t *min_boxes, int32_t *boxes)
+{
+    return guarded([&] {
+        if (!plan || plan->plan->type != 3) {
+            set_error("target boxes are defined for type-3 plans");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        Plan &p = *plan->plan;
+        if (min_boxes) {
+            int k[3] = {1, 1, 1};
+            for (int a = 0; a < p.dim; ++a) {
+                if (min_boxes[a] < 1 || min_boxes[a] > mnufft::kType3MaxBoxes) {
+                    set_error("boxes per axis must be 1.." +
+                              std::to_string(mnufft::kType3MaxBoxes));
+                    return MNUFFT_ERROR_INVALID_ARGUMENT;
+                }
+                k[a] = min_boxes[a];
+            }
+            (void)hipDeviceSynchronize();  // the plan's kernels may still run
+            double sb[6], tb[6];
+            for (int a = 0; a < 2 * p.dim; ++a) {
+                sb[a] = p.t3_bounds[a];
+                tb[a] = p.t3_bounds[2 * mnufft::kMaxDims + a];
+            }
+            if (!p.init_type3(sb, tb, k)) return MNUFFT_ERROR_PLAN;
+        }
+        if (boxes)
+            for (int a = 0; a < p.dim; ++a) boxes[a] = p.t3_boxes.k[a];
+        return MNUFFT_SUCCESS;
+    });
+}
+
 void mnufft_destroy(mnufft_plan plan)
 {
     if (!plan) return;
@@ -1045,6 +1077,83 @@ int hp_execute(HostPlan *hp, S *c, S *f)
 std::mutex g_name_mutex;
 std::string g_gpu_name;
 
+// Plans the one-call transforms keep for reuse (capi/src/cache.rs:10-51):
+// up to kOneCallPlans, most recently used last, keyed by everything plan
+// creation reads (API precision, type, rank, modes, sign, tolerance, the
+// options' bytes; no options = the defaults).  A loop of one-call
+// transforms of the same sizes then pays plan creation (rocFFT plans,
+// amplitude tables, scratch) once.  Each kept plan runs one call at a time;
+// wgpu_nufft_shutdown drops them.
+constexpr size_t kOneCallPlans = 4;
+
+struct OneCallKey {
+    bool float_api;
+    int type, dim, isign;
+    int64_t nm[3];
+    double eps;
+    unsigned char opts[sizeof(wgpu_nufft_opts)];
+    bool operator==(const OneCallKey &o) const
+    {
+        return float_api == o.float_api && type == o.type && dim == o.dim && isign == o.isign &&
+               std::memcmp(nm, o.nm, sizeof(nm)) == 0 &&
+               std::memcmp(&eps, &o.eps, sizeof(eps)) == 0 &&
+               std::memcmp(opts, o.opts, sizeof(opts)) == 0;
+    }
+};
+
+struct KeptPlan {
+    std::mutex mu;
+    std::unique_ptr<HostPlan> hp;
+};
+
+std::mutex g_kept_mu;
+// never destroyed: kept plans hold device memory, which must not be freed
+// from static destructors after the HIP runtime has shut down (the process's
+// exit reclaims it)
+auto &g_kept = *new std::vector<std::pair<OneCallKey, std::shared_ptr<KeptPlan>>>();
+
+int kept_plan_for(const OneCallKey &key, const int64_t *nm, const wgpu_nufft_opts *opts,
+                  std::shared_ptr<KeptPlan> *out)
+{
+    {
+        std::lock_guard<std::mutex> lock(g_kept_mu);
+        for (size_t i = 0; i < g_kept.size(); ++i) {
+            if (g_kept[i].first == key) {
+                auto entry = g_kept[i];
+                g_kept.erase(g_kept.begin() + (std::ptrdiff_t)i);
+                g_kept.push_back(entry);
+                *out = entry.second;
+                return WGPU_NUFFT_SUCCESS;
+            }
+        }
+    }
+    // created outside the lock, so other threads' calls are not held up
+    HostPlan *hp = nullptr;
+    int rc = hp_makeplan(key.float_api, key.type, key.dim, nm, key.isign, 1, key.eps, &hp, opts);
+    if (rc) return rc;
+    auto kept = std::make_shared<KeptPlan>();
+    kept->hp.reset(hp);
+    std::lock_guard<std::mutex> lock(g_kept_mu);
+    if (g_kept.size() == kOneCallPlans) g_kept.erase(g_kept.begin());
+    g_kept.emplace_back(key, kept);
+    *out = kept;
+    return WGPU_NUFFT_SUCCESS;
+}
+
+void drop_kept_plans()
+{
+    std::vector<std::pair<OneCallKey, std::shared_ptr<KeptPlan>>> old;
+    {
+        std::lock_guard<std::mutex> lock(g_kept_mu);
+        old.swap(g_kept);
+    }
+    // each plan is destroyed once no call holds it
+    for (auto &e : old) {
+        std::lock_guard<std::mutex> lock(e.second->mu);
+        e.second->hp.reset();
+    }
+}
+
 template <typename S>
 int one_call(bool float_api, int type, int dim, int64_t M, const S *x, const S *y, const S *z,
              S *c, int isign, double eps, const int64_t *nm, int64_t N, const S *s, const S *t,
@@ -1055,10 +1164,24 @@ int one_call(bool float_api, int type, int dim, int64_t M, const S *x, const S *
         (void)(mnufft::point_count_ok(M, "point") && mnufft::point_count_ok(N, "target"));
         return WGPU_NUFFT_ERROR_PLAN;
     }
-    HostPlan *hp = nullptr;
-    int rc = hp_makeplan(float_api, type, dim, nm, isign, 1, eps, &hp, opts);
+    OneCallKey key{};
+    key.float_api = float_api;
+    key.type = type;
+    key.dim = dim;
+    key.isign = isign;
+    key.eps = eps;
+    for (int a = 0; a < 3; ++a) key.nm[a] = nm && a < dim && dim >= 1 && dim <= 3 ? nm[a] : 0;
+    std::memset(key.opts, 0, sizeof(key.opts));
+    if (opts) std::memcpy(key.opts, opts, sizeof(key.opts));
+    std::shared_ptr<KeptPlan> kept;
+    int rc = kept_plan_for(key, nm, opts, &kept);
     if (rc) return rc;
-    std::unique_ptr<HostPlan> owner(hp);
+    std::lock_guard<std::mutex> lock(kept->mu);
+    if (!kept->hp) {
+        set_error("the kept plan was dropped by wgpu_nufft_shutdown");
+        return WGPU_NUFFT_ERROR_INTERNAL;
+    }
+    HostPlan *hp = kept->hp.get();
     const S *axes[3] = {x, y, z};
     const S *taxes[3] = {s, t, u};
     rc = hp_setpts<S>(hp, M, axes, nullptr, N, taxes, nullptr);
@@ -1127,7 +1250,13 @@ const char *wgpu_nufft_gpu_name(void)
     return g_gpu_name.c_str();
 }
 
-void wgpu_nufft_shutdown(void) {}
+void wgpu_nufft_shutdown(void)
+{
+    try {
+        drop_kept_plans();
+    } catch (...) {
+    }
+}
 
 int wgpu_nufft_makeplan(int32_t type, int32_t dim, const int64_t *n_modes, int32_t isign,
                         int64_t ntrans, double eps, wgpu_nufft_plan *plan,

@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void k_type3_source(Type3Params tp, int64_t M,
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             double xa = (double)x[j * D + a];
-            if (t < M) xr[j * D + a] = (T)((xa - tp.src_center[a]) * tp.inv_gamma[a]);
+            if (xr && t < M) xr[j * D + a] = (T)((xa - tp.src_center[a]) * tp.inv_gamma[a]);
             phase += xa * tp.tgt_center[a];
         }
         double sn, cs;
@@ -295,17 +295,72 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_type3_final(int64_t N, int64_t vectors,
                                                      const T *__restrict__ interp,
                                                      const double *__restrict__ factor,
-                                                     T *__restrict__ out)
+                                                     T *__restrict__ out,
+                                                     const uint32_t *__restrict__ perm,
+                                                     int64_t n_out)
 {
     const int64_t n_total = N * vectors;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_total;
          t += (int64_t)gridDim.x * blockDim.x) {
-        int64_t k = t % N;
+        const int64_t v = t / N, k = t - v * N;
         double fr = factor[2 * k], fi = factor[2 * k + 1];
         double re = (double)interp[2 * t], im = (double)interp[2 * t + 1];
-        out[2 * t] = (T)(re * fr - im * fi);
-        out[2 * t + 1] = (T)(re * fi + im * fr);
+        const int64_t o = perm ? v * n_out + perm[k] : t;
+        out[2 * o] = (T)(re * fr - im * fi);
+        out[2 * o + 1] = (T)(re * fi + im * fr);
     }
+}
+
+// Box of a target coordinate vector (clamped to the boxes).
+template <int D, typename T>
+__device__ inline uint32_t target_box(const Type3Boxes &bx, const T *sk)
+{
+    int64_t b = 0, stride = 1;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        double q = floor(((double)sk[a] - bx.lo[a]) / bx.width[a]);
+        q = q < 0.0 ? 0.0 : (q > (double)(bx.k[a] - 1) ? (double)(bx.k[a] - 1) : q);
+        if (!(q == q)) q = 0.0;  // NaN coordinates land in box 0
+        b += (int64_t)q * stride;
+        stride *= bx.k[a];
+    }
+    return (uint32_t)b;
+}
+
+__global__ void k_clear_u32(uint32_t *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_type3_box_count(Type3Boxes bx, int64_t N,
+                                                         const T *__restrict__ s,
+                                                         uint32_t *__restrict__ box,
+                                                         uint32_t *__restrict__ counts)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const uint32_t b = target_box<D, T>(bx, s + k * D);
+    box[k] = b;
+    atomicAdd(counts + b, 1u);
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_type3_box_scatter(int64_t N, const T *__restrict__ s,
+                                                           const uint32_t *__restrict__ box,
+                                                           const uint32_t *__restrict__ offsets,
+                                                           uint32_t *__restrict__ cursor,
+                                                           uint32_t *__restrict__ perm,
+                                                           T *__restrict__ s_sorted)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const uint32_t b = box[k];
+    const int64_t pos = (int64_t)offsets[b] + atomicAdd(cursor + b, 1u);
+    perm[pos] = (uint32_t)k;
+#pragma unroll
+    for (int a = 0; a < D; ++a) s_sorted[pos * D + a] = s[k * D + a];
 }
 
 // ---------------------------------------------------------------------
@@ -459,12 +514,34 @@ bool type3_target(const Type3Params &tp, int64_t N, const T *s, T *sr, double *f
 
 template <typename T>
 bool type3_final(int64_t N, int64_t vectors, const T *interp, const double *factor, T *out,
-                 hipStream_t stream)
+                 hipStream_t stream, const uint32_t *perm, int64_t n_out)
 {
     if (N == 0) return true;
     k_type3_final<T><<<grid_for(N * vectors, 256), 256, 0, stream>>>(N, vectors, interp, factor,
-                                                                     out);
+                                                                     out, perm, n_out);
     return hip_ok(hipGetLastError(), "type-3 final launch");
+}
+
+template <int D, typename T>
+bool type3_box_count(const Type3Boxes &bx, int64_t N, const T *s, uint32_t *box,
+                     uint32_t *counts, hipStream_t stream)
+{
+    k_clear_u32<<<grid_for(bx.count, 256), 256, 0, stream>>>(counts, bx.count);
+    if (N > 0)
+        k_type3_box_count<D, T><<<grid_for(N, 256), 256, 0, stream>>>(bx, N, s, box, counts);
+    return hip_ok(hipGetLastError(), "type-3 box count launch");
+}
+
+template <int D, typename T>
+bool type3_box_scatter(const Type3Boxes &bx, int64_t N, const T *s, const uint32_t *box,
+                       const uint32_t *offsets, uint32_t *cursor, uint32_t *perm, T *s_sorted,
+                       hipStream_t stream)
+{
+    k_clear_u32<<<grid_for(bx.count, 256), 256, 0, stream>>>(cursor, bx.count);
+    if (N > 0)
+        k_type3_box_scatter<D, T><<<grid_for(N, 256), 256, 0, stream>>>(N, s, box, offsets, cursor,
+                                                                       perm, s_sorted);
+    return hip_ok(hipGetLastError(), "type-3 box scatter launch");
 }
 
 bool direct_sum(int kind, int d, const DirectParams &dp, const double *x, const double *in,
@@ -493,7 +570,12 @@ bool direct_sum(int kind, int d, const DirectParams &dp, const double *x, const 
     template bool type3_source<D, T>(const Type3Params &, int64_t, int64_t, const T *, const T *,  \
                                       T *, T *, hipStream_t);                                     \
     template bool type3_target<D, T>(const Type3Params &, int64_t, const T *, T *, double *,       \
-                                      hipStream_t);
+                                      hipStream_t);                                               \
+    template bool type3_box_count<D, T>(const Type3Boxes &, int64_t, const T *, uint32_t *,        \
+                                         uint32_t *, hipStream_t);                                \
+    template bool type3_box_scatter<D, T>(const Type3Boxes &, int64_t, const T *,                  \
+                                           const uint32_t *, const uint32_t *, uint32_t *,        \
+                                           uint32_t *, T *, hipStream_t);
 MNUFFT_MODES_INSTANTIATE(1, float)
 MNUFFT_MODES_INSTANTIATE(2, float)
 MNUFFT_MODES_INSTANTIATE(3, float)
@@ -504,8 +586,8 @@ template bool fill_zero<float>(float *, int64_t, hipStream_t);
 template bool fill_zero<double>(double *, int64_t, hipStream_t);
 template bool fill_zero<uint32_t>(uint32_t *, int64_t, hipStream_t);
 template bool type3_final<float>(int64_t, int64_t, const float *, const double *, float *,
-                                 hipStream_t);
+                                 hipStream_t, const uint32_t *, int64_t);
 template bool type3_final<double>(int64_t, int64_t, const double *, const double *, double *,
-                                  hipStream_t);
+                                  hipStream_t, const uint32_t *, int64_t);
 
 }  // namespace mnufft

@@ -27,6 +27,15 @@ struct Type3Params {
     double quad_w[kMaxQuadrature];  // matching weights times 2 (w/2) phi(node)
 };
 
+// Target boxes of a partitioned type-3 plan: axis a is cut into k[a] boxes
+// of width[a] from lo[a]; box index = b0 + k0 (b1 + k1 b2).
+struct Type3Boxes {
+    int k[kMaxDims];
+    double lo[kMaxDims];
+    double width[kMaxDims];
+    int64_t count;
+};
+
 struct DirectParams {
     int64_t n_modes[kMaxDims];
     int64_t mode_total;
@@ -50,9 +59,22 @@ bool type3_source(const Type3Params &tp, int64_t M, int64_t vectors, const T *x,
 template <int D, typename T>
 bool type3_target(const Type3Params &tp, int64_t N, const T *s, T *sr, double *factor,
                   hipStream_t stream);
+// out[v * n_out + (perm ? perm[k] : k)] for k < N (n_out = N without perm).
 template <typename T>
 bool type3_final(int64_t N, int64_t vectors, const T *interp, const double *factor, T *out,
-                 hipStream_t stream);
+                 hipStream_t stream, const uint32_t *perm = nullptr, int64_t n_out = 0);
+// Target boxes: box[k] of every target and counts[box] (counts cleared
+// here first), then, with offsets[box] = the exclusive scan of the counts,
+// perm[] = the targets grouped by box (any order inside a box) and
+// s_sorted[] = their coordinates in that order; cursor (count values) is
+// scratch.
+template <int D, typename T>
+bool type3_box_count(const Type3Boxes &bx, int64_t N, const T *s, uint32_t *box,
+                     uint32_t *counts, hipStream_t stream);
+template <int D, typename T>
+bool type3_box_scatter(const Type3Boxes &bx, int64_t N, const T *s, const uint32_t *box,
+                       const uint32_t *offsets, uint32_t *cursor, uint32_t *perm, T *s_sorted,
+                       hipStream_t stream);
 // kind 1: out[o] = sum_j in[j] e^{i sign k_o.x_j} over the linear mode
 // indices sel (or all); kind 2: out[o] = sum_k in[k] e^{i sign k.x_o} for
 // the points sel; kind 3: out[o] = sum_j in[j] e^{i sign s_o.x_j}.

@@ -23,6 +23,10 @@ constexpr int kMaxWidth = 16;
 // a point set (or type-3 target set) holds at most this many points.
 constexpr int64_t kMaxPoints = (int64_t)UINT32_MAX - 1;
 constexpr int kMaxGeometries = 16;
+// Inner fine grids of a type-3 plan stay within 32-bit cell indices per
+// vector; larger target intervals are cut into boxes.
+constexpr int64_t kType3InnerCells = (int64_t)1 << 30;
+constexpr int kType3MaxBoxes = 64;  // per axis
 // Bins above this many points are ordered by the large-bin sort; smaller
 // ones are ranked inside the prepare pass (src/gpu_bin_sort.rs SMALL_BIN).
 constexpr uint32_t kSmallBin = 64;

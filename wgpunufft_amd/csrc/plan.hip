@@ -125,9 +125,8 @@ FftPlan::~FftPlan()
 Plan::~Plan()
 {
     for (void *p : {(void *)d_inv_phihat, grid, (void *)rec_start, rec_off, (void *)rec_perm,
-                    c_sorted, t3_xr,
-                    t3_cp, t3_sr, t3_factor, t3_interp, (void *)item_off,
-                    (void *)batch_table})
+                    c_sorted, t3_xr, t3_cp, t3_sr, t3_factor, t3_interp, t3_box, t3_perm,
+                    t3_s_sorted, t3_counts, (void *)item_off, (void *)batch_table})
         if (p) (void)hipFree(p);
     for (auto &sl : slots)
         for (auto &e : sl.ev)
@@ -563,32 +562,45 @@ bool Plan::bin_sort_static_t(int64_t M, const T *pts, uint32_t *sorted, uint32_t
 // rescale + correction, inner type 2 on the outer grid, final multiply
 // (src/gpu_type3.rs:267-430).
 // ----------------------------------------------------------------------
-bool Plan::init_type3(const double *src_bounds, const double *tgt_bounds)
+bool Plan::init_type3(const double *src_bounds, const double *tgt_bounds, const int *min_boxes)
 {
     if (!es_kernel_for_tolerance(eps, sigma, &kernel)) return false;
     std::memset(&t3, 0, sizeof(t3));
     t3.sign = isign >= 0 ? 1.0 : -1.0;
-    int64_t outer_n[kMaxDims] = {1, 1, 1};
+    for (int a = 0; a < 2 * dim; ++a) {
+        t3_bounds[a] = src_bounds[a];
+        t3_bounds[2 * kMaxDims + a] = tgt_bounds[a];
+    }
+    // source half-width and centre, target half-width and centre per axis
+    // (src/type3.rs:708-797)
+    double X[kMaxDims] = {}, C[kMaxDims] = {}, S[kMaxDims] = {}, Dc[kMaxDims] = {};
+    double tlo[kMaxDims] = {}, thi[kMaxDims] = {};
     for (int a = 0; a < dim; ++a) {
         double slo = src_bounds[2 * a], shi = src_bounds[2 * a + 1];
-        double tlo = tgt_bounds[2 * a], thi = tgt_bounds[2 * a + 1];
-        if (!(std::isfinite(slo) && std::isfinite(shi) && std::isfinite(tlo) &&
-              std::isfinite(thi)) ||
-            slo > shi || tlo > thi) {
+        tlo[a] = tgt_bounds[2 * a];
+        thi[a] = tgt_bounds[2 * a + 1];
+        if (!(std::isfinite(slo) && std::isfinite(shi) && std::isfinite(tlo[a]) &&
+              std::isfinite(thi[a])) ||
+            slo > shi || tlo[a] > thi[a]) {
             set_error("type-3 bounds must be finite with lower <= upper");
             return false;
         }
-        double X = (shi - slo) * 0.5, C = (shi + slo) * 0.5;
-        if (std::fabs(C) < 0.1 * X) {
-            X += std::fabs(C);
-            C = 0.0;
+        X[a] = (shi - slo) * 0.5;
+        C[a] = (shi + slo) * 0.5;
+        if (std::fabs(C[a]) < 0.1 * X[a]) {
+            X[a] += std::fabs(C[a]);
+            C[a] = 0.0;
         }
-        double S = (thi - tlo) * 0.5, Dc = (thi + tlo) * 0.5;
-        if (std::fabs(Dc) < 0.1 * S) {
-            S += std::fabs(Dc);
-            Dc = 0.0;
+        S[a] = (thi[a] - tlo[a]) * 0.5;
+        Dc[a] = (thi[a] + tlo[a]) * 0.5;
+        if (std::fabs(Dc[a]) < 0.1 * S[a]) {
+            S[a] += std::fabs(Dc[a]);
+            Dc[a] = 0.0;
         }
-        double x = X, s = S;
+    }
+    // outer grid length, gamma and h of an axis for a target half-width
+    auto axis_grid = [&](int a, double half, int64_t *nf_out, double *gamma_out, double *h_out) {
+        double x = X[a], s = half;
         if (x == 0.0) {
             if (s == 0.0) {
                 x = 1.0;
@@ -602,13 +614,60 @@ bool Plan::init_type3(const double *src_bounds, const double *tgt_bounds)
         double requested = 2.0 * sigma * s * x / M_PI + (double)(kernel.width + 1);
         int64_t minimum = std::max<int64_t>((int64_t)requested, 2 * (int64_t)kernel.width);
         int64_t nf = next_even_235(minimum);
-        double h = 2.0 * M_PI / (double)nf;
-        double gamma = (double)nf / (2.0 * sigma * s);
-        outer_n[a] = nf;
-        t3.src_center[a] = C;
-        t3.inv_gamma[a] = 1.0 / gamma;
-        t3.tgt_center[a] = Dc;
-        t3.tgt_scale[a] = h * gamma;
+        *nf_out = nf;
+        *h_out = 2.0 * M_PI / (double)nf;
+        *gamma_out = (double)nf / (2.0 * sigma * s);
+    };
+    // Target boxes: the whole interval per axis unless the inner fine grid
+    // would exceed kType3InnerCells cells or the byte budget; then the axis
+    // with the longest inner grid is cut into one more box, until it fits.
+    int k[kMaxDims] = {1, 1, 1};
+    for (int a = 0; a < dim; ++a)
+        if (min_boxes && min_boxes[a] > 1) k[a] = std::min(min_boxes[a], kType3MaxBoxes);
+    int64_t outer_n[kMaxDims] = {1, 1, 1};
+    double gam[kMaxDims] = {1, 1, 1}, hh[kMaxDims] = {1, 1, 1};
+    for (;;) {
+        int64_t fine_len[kMaxDims] = {1, 1, 1};
+        double cells = 1.0;
+        for (int a = 0; a < dim; ++a) {
+            const double half = k[a] == 1 ? S[a] : (thi[a] - tlo[a]) / (2.0 * k[a]);
+            axis_grid(a, half, &outer_n[a], &gam[a], &hh[a]);
+            fine_len[a] = select_fine_grid_size(outer_n[a], sigma, kernel.width);
+            if (fine_len[a] < 0) {
+                set_error("no supported inner fine-grid length for type-3 axis " +
+                          std::to_string(a));
+                return false;
+            }
+            cells *= (double)fine_len[a];
+        }
+        const double bytes = cells * 2.0 * (double)real_bytes() * (double)batch;
+        if (cells <= (double)kType3InnerCells && bytes <= (double)t3_inner_budget) break;
+        int split = -1;
+        for (int a = 0; a < dim; ++a)
+            if (thi[a] > tlo[a] && k[a] < kType3MaxBoxes &&
+                (split < 0 || fine_len[a] > fine_len[split]))
+                split = a;
+        if (split < 0) {
+            set_error("type-3 inner grid too large even with the target interval cut into boxes");
+            return false;
+        }
+        ++k[split];
+    }
+    t3_boxes = Type3Boxes{};
+    t3_boxes.count = 1;
+    for (int a = 0; a < kMaxDims; ++a) {
+        t3_boxes.k[a] = a < dim ? k[a] : 1;
+        t3_boxes.lo[a] = a < dim ? tlo[a] : 0.0;
+        t3_boxes.width[a] =
+            a < dim && thi[a] > tlo[a] ? (thi[a] - tlo[a]) / (double)k[a] : 1.0;
+        t3_boxes.count *= t3_boxes.k[a];
+        t3_center_full[a] = a < dim ? Dc[a] : 0.0;
+    }
+    for (int a = 0; a < dim; ++a) {
+        t3.src_center[a] = C[a];
+        t3.inv_gamma[a] = 1.0 / gam[a];
+        t3.tgt_center[a] = k[a] == 1 ? Dc[a] : tlo[a] + 0.5 * t3_boxes.width[a];
+        t3.tgt_scale[a] = hh[a] * gam[a];
     }
     int pn = kernel.width + 2;
     std::vector<double> nodes(2 * pn), weights(2 * pn);
@@ -652,7 +711,31 @@ bool Plan::init_type3(const double *src_bounds, const double *tgt_bounds)
     for (int a = 0; a < dim; ++a) inner->n_modes[a] = outer_n[a];
     if (!inner->init_common()) return false;
     for (int a = 0; a < dim; ++a) type3_outer[a] = outer_n[a];
+    // (src/gpu_type3.rs:98-110: F32 grids the dense spreader supports)
+    type3_small = precision == Precision::F32 && small_spread_supported(dim, outer_n, kernel.width);
     return true;
+}
+
+template <int D, typename T>
+bool Plan::spread_type3_sources(int64_t active, int64_t M, const T *cp, bool set_points,
+                                hipStream_t stream)
+{
+    if constexpr (D <= 2 && sizeof(T) == 4) {
+        if (type3_small && M > 0 && outer->spread_method == SpreadMethod::Plain) {
+            // many sources on a small outer grid: point groups in their
+            // original order, partial grids added in group order
+            if (!outer->ensure_grid(active)) return false;
+            const int64_t side[kMaxDims] = {1, 1, 1};
+            return small.spread(outer->grid_params(side), M, (const float *)t3_xr,
+                                (const float *)cp, active, (float *)outer->grid, stream);
+        }
+    }
+    if (set_points && !outer->set_points_t<D, T>(M, (const T *)t3_xr, stream)) return false;
+    if (M == 0) {
+        if (!outer->ensure_grid(active)) return false;
+        return fill_zero<T>((T *)outer->grid, 2 * outer->fine_total * active, stream);
+    }
+    return outer->execute_t<D, T>(active, cp, nullptr, stream, false);
 }
 
 template <int D, typename T>
@@ -679,20 +762,81 @@ bool Plan::execute_type3_t(int64_t active, int64_t M, const T *x, const T *c, in
         !grow(&t3_interp, &t3_interp_bytes, N * active * 2 * rb, "type-3 interpolated"))
         return false;
     mark(kStageType3, false, stream);
-    if (!type3_source<D, T>(t3, M, active, x, c, (T *)t3_xr, (T *)t3_cp, stream)) return false;
-    if (!outer->set_points_t<D, T>(M, (const T *)t3_xr, stream)) return false;
-    if (M == 0) {
-        if (!outer->ensure_grid(active)) return false;
-        if (!fill_zero<T>((T *)outer->grid, 2 * outer->fine_total * active, stream)) return false;
-    } else if (!outer->execute_t<D, T>(active, (const T *)t3_cp, nullptr, stream, false)) {
-        return false;
+    if (t3_boxes.count == 1) {
+        // one box: rescale + pre-phase, raw spread onto the outer grid,
+        // target rescale + correction, inner type 2, final multiply
+        // (src/gpu_type3.rs:267-430)
+        if (!type3_source<D, T>(t3, M, active, x, c, (T *)t3_xr, (T *)t3_cp, stream)) return false;
+        if (!spread_type3_sources<D, T>(active, M, (const T *)t3_cp, true, stream)) return false;
+        if (!type3_target<D, T>(t3, N, s, (T *)t3_sr, (double *)t3_factor, stream)) return false;
+        if (!inner->set_points_t<D, T>(N, (const T *)t3_sr, stream)) return false;
+        if (!inner->execute_t<D, T>(active, (const T *)outer->grid, (T *)t3_interp, stream, false))
+            return false;
+        if (!type3_final<T>(N, active, (const T *)t3_interp, (const double *)t3_factor, out,
+                            stream))
+            return false;
+        mark(kStageType3, true, stream);
+        return true;
     }
-    if (!type3_target<D, T>(t3, N, s, (T *)t3_sr, (double *)t3_factor, stream)) return false;
-    if (!inner->set_points_t<D, T>(N, (const T *)t3_sr, stream)) return false;
-    if (!inner->execute_t<D, T>(active, (const T *)outer->grid, (T *)t3_interp, stream, false))
+    // Target boxes: the targets are grouped by box (counted on the device;
+    // the counts are read back once to size each box's inner problem), and
+    // every box runs the pipeline above with its centre as target centre.
+    // The sources' rescaled positions and their binning on the outer grid
+    // are shared by every box; only the pre-phase of the strengths differs.
+    const int64_t nbox = t3_boxes.count;
+    if (!grow(&t3_box, &t3_box_bytes, N * sizeof(uint32_t), "type-3 target boxes") ||
+        !grow(&t3_perm, &t3_perm_bytes, N * sizeof(uint32_t), "type-3 target order") ||
+        !grow(&t3_s_sorted, &t3_s_sorted_bytes, N * D * rb, "type-3 grouped targets") ||
+        !grow(&t3_counts, &t3_counts_bytes, 3 * nbox * sizeof(uint32_t), "type-3 box counts"))
         return false;
-    if (!type3_final<T>(N, active, (const T *)t3_interp, (const double *)t3_factor, out, stream))
+    uint32_t *counts = (uint32_t *)t3_counts, *offsets = counts + nbox, *cursor = counts + 2 * nbox;
+    if (!type3_box_count<D, T>(t3_boxes, N, s, (uint32_t *)t3_box, counts, stream)) return false;
+    t3_host_counts.assign(2 * nbox, 0u);
+    if (!hip_ok(hipMemcpyAsync(t3_host_counts.data(), counts, nbox * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, stream),
+                "type-3 box counts read") ||
+        !hip_ok(hipStreamSynchronize(stream), "type-3 box counts wait"))
         return false;
+    uint32_t run = 0;
+    for (int64_t b = 0; b < nbox; ++b) {
+        t3_host_counts[nbox + b] = run;
+        run += t3_host_counts[b];
+    }
+    if (!hip_ok(hipMemcpy(offsets, t3_host_counts.data() + nbox, nbox * sizeof(uint32_t),
+                          hipMemcpyHostToDevice),
+                "type-3 box offsets"))
+        return false;
+    if (!type3_box_scatter<D, T>(t3_boxes, N, s, (const uint32_t *)t3_box, offsets, cursor,
+                                 (uint32_t *)t3_perm, (T *)t3_s_sorted, stream))
+        return false;
+    Type3Params tp = t3;
+    bool first = true;
+    for (int64_t b = 0; b < nbox; ++b) {
+        const int64_t nb = t3_host_counts[b], off = t3_host_counts[nbox + b];
+        if (nb == 0) continue;
+        int64_t rest = b;
+        for (int a = 0; a < D; ++a) {
+            const int64_t i = rest % t3_boxes.k[a];
+            rest /= t3_boxes.k[a];
+            tp.tgt_center[a] = t3_boxes.k[a] == 1
+                                   ? t3_center_full[a]
+                                   : t3_boxes.lo[a] + ((double)i + 0.5) * t3_boxes.width[a];
+        }
+        if (!type3_source<D, T>(tp, M, active, x, c, first ? (T *)t3_xr : nullptr, (T *)t3_cp,
+                                stream))
+            return false;
+        if (!spread_type3_sources<D, T>(active, M, (const T *)t3_cp, first, stream)) return false;
+        first = false;
+        const T *sb = (const T *)t3_s_sorted + off * D;
+        if (!type3_target<D, T>(tp, nb, sb, (T *)t3_sr, (double *)t3_factor, stream))
+            return false;
+        if (!inner->set_points_t<D, T>(nb, (const T *)t3_sr, stream)) return false;
+        if (!inner->execute_t<D, T>(active, (const T *)outer->grid, (T *)t3_interp, stream, false))
+            return false;
+        if (!type3_final<T>(nb, active, (const T *)t3_interp, (const double *)t3_factor, out,
+                            stream, (const uint32_t *)t3_perm + off, N))
+            return false;
+    }
     mark(kStageType3, true, stream);
     return true;
 }

@@ -102,6 +102,21 @@ struct Plan {
     Type3Params t3{};
     int64_t type3_outer[kMaxDims] = {1, 1, 1};
     std::unique_ptr<Plan> outer, inner;
+    // F32 1D / 2D type-3 plans whose outer grid is small spread their
+    // sources with the small-grid spreader (no point sort)
+    bool type3_small = false;
+    SmallSpreader small;
+    // Target boxes (one box per axis unless the inner grid of the whole
+    // target interval would exceed kType3InnerCells cells or
+    // t3_inner_budget bytes): every box is a type-3 problem of its own with
+    // the box centre as target centre, sharing the outer and inner plans.
+    Type3Boxes t3_boxes{};
+    double t3_center_full[kMaxDims] = {};  // target centre of an unsplit axis
+    double t3_bounds[4 * kMaxDims] = {};   // source then target bounds
+    int64_t t3_inner_budget = (int64_t)16 << 30;
+    void *t3_box = nullptr, *t3_perm = nullptr, *t3_s_sorted = nullptr, *t3_counts = nullptr;
+    size_t t3_box_bytes = 0, t3_perm_bytes = 0, t3_s_sorted_bytes = 0, t3_counts_bytes = 0;
+    std::vector<uint32_t> t3_host_counts;
     void *t3_xr = nullptr, *t3_cp = nullptr, *t3_sr = nullptr, *t3_factor = nullptr,
          *t3_interp = nullptr;
     size_t t3_xr_bytes = 0, t3_cp_bytes = 0, t3_sr_bytes = 0, t3_factor_bytes = 0,
@@ -121,7 +136,9 @@ struct Plan {
     bool begin_call();
 
     bool init_common();
-    bool init_type3(const double *src_bounds, const double *tgt_bounds);
+    // min_boxes (may be null): at least this many target boxes per axis.
+    bool init_type3(const double *src_bounds, const double *tgt_bounds,
+                    const int *min_boxes = nullptr);
     GridParams grid_params(const int64_t *side) const;
     GridParams type1_geometry(int64_t M, int *chosen) const;
     GridParams spread_geometry(int64_t M, int *chosen) const;
@@ -168,6 +185,11 @@ struct Plan {
     template <int D, typename T>
     bool bin_sort_static_t(int64_t M, const T *pts, uint32_t *sorted, uint32_t *offsets,
                            hipStream_t stream);
+    // Spreads the type-3 sources (rescaled positions in t3_xr, pre-phased
+    // strengths cp) onto the outer grid; set_points: bin them first.
+    template <int D, typename T>
+    bool spread_type3_sources(int64_t active, int64_t M, const T *cp, bool set_points,
+                              hipStream_t stream);
     template <int D, typename T>
     bool execute_type3_t(int64_t active, int64_t M, const T *x, const T *c, int64_t N,
                          const T *s, T *out, hipStream_t stream);

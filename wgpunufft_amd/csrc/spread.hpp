@@ -112,6 +112,32 @@ bool launch_mfma_tiles(SpreadMethod method, int dim, const GridParams &g, const 
 bool launch_absmax(const float *c, int64_t n, int64_t vectors, uint32_t *out,
                    hipStream_t stream);
 
+// Spread of many points onto a small F32 grid (the outer grid of a 1D or 2D
+// type-3 plan): points cut into equal groups in their original order, each
+// group's partial grid written by output-stationary lanes, the partial grids
+// added in group order (spread_small.hip; reference
+// src/gpu_dense_spread.rs:1-48).  Bitwise repeatable, no point sort.
+constexpr int64_t kSmallSpreadMaxCells = 1 << 16;
+constexpr int64_t kSmallGroupPoints = 256;
+constexpr int64_t kSmallMaxGroups = 1024;
+
+// 1D or 2D, every axis at least 2 * width long, at most
+// kSmallSpreadMaxCells cells (src/gpu_dense_spread.rs:65-77).
+bool small_spread_supported(int dim, const int64_t *fine, int width);
+int64_t small_spread_groups(int64_t M);
+
+struct SmallSpreader {
+    ~SmallSpreader();
+    // grid (vectors * fine_total complex values) = the spread of the M
+    // points (point-major, F32) with strengths c (vectors * M complex).
+    bool spread(const GridParams &g, int64_t M, const float *pts, const float *c,
+                int64_t vectors, float *grid, hipStream_t stream);
+    void *partial = nullptr;
+    int64_t partial_bytes = 0;
+    // partial grids of one chunk of vectors at most (one vector at least)
+    int64_t budget_bytes = (int64_t)1 << 30;
+};
+
 // Output-stationary blocks for a fine grid: 64 lanes, `run` cells per lane
 // along axis 0 (run + width - 1 <= n0 keeps every cell's support offset
 // unique within a run).
