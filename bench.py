@@ -180,8 +180,9 @@ def main():
                                     "k_reduce_tiles, k_dense_windows + k_dense_reduce + "
                                     "k_dense_add for the dense bins; one event span)",
                             FP16_MFMA_PEAK_TFLOPS, "peak: dense FP16 matrix cores")
+    tile = "x".join(str(v) for v in info1["block"])
     roof_plain = spread_roof(t_plain,
-                             "k_sweep3 (type-1 3D row-sweep spread)" if sweep
+                             f"k_spread_rows (type-1 plain FP32 row tiles, {tile} cells)" if sweep
                              else "k_spread (type-1 output-stationary spread)",
                              FP32_PEAK_TFLOPS, "peak: FP32 vector", bound="fp32-vector")
     interp_gbs = interp_bytes / (t_interp * 1e-3) / 1e9
@@ -211,7 +212,8 @@ def main():
     bits2 = key_bits(info2["nbins"]) + sum(int(s).bit_length() - 1 for s in info2["bin_side"])
     roof_sort1 = sort_roof(stages_mfma["type1"]["sort"],
                            f"type-1 bin sort (stage: radix keys, {-(-bits1 // 8)} passes of "
-                           "64-bit keys, bin offsets, large-group merges)", info1["nbins"])
+                           "64-bit keys, bin offsets, large-group merges)",
+                           info1["nbins"])
     roof_sort2 = sort_roof(stages_mfma["type2"]["sort"],
                            f"type-2 bin sort (stage: radix keys, {-(-bits2 // 8)} passes of "
                            "32-bit keys, bin offsets)", info2["nbins"])
@@ -255,7 +257,8 @@ def main():
         "type1_points_per_s": pts(stages_mfma, "type1"),
         "type2_points_per_s": pts(stages_mfma, "type2"),
         "plain": {"value": M / (ms_plain * 1e-3), "ms_per_step": round(ms_plain, 4),
-                  "spreader": "plain FP32 row sweep" if sweep else "plain FP32 output-stationary",
+                  "spreader": f"plain FP32 row tiles ({tile})" if sweep
+                              else "plain FP32 output-stationary",
                   "stages_ms": stages_plain,
                   "type1_points_per_s": pts(stages_plain, "type1")},
         "index_order": {"value": M / (ms_index * 1e-3), "ms_per_step": round(ms_index, 4),

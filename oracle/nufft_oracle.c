@@ -572,9 +572,15 @@ static int in_reach_f32(float x)
 }
 
 /* Support start, wrapped start, bin key and prepared offset of every point
- * of an F32 point set (point-major coordinates).  Also counts the points
- * whose f64 start differs from the exact one ("ambiguous": within rounding
- * of an integer boundary). */
+ * of an F32 point set (point-major coordinates).  Also counts the
+ * "ambiguous" points: those whose f64 start differs from the exact one, and
+ * those whose exact pos - w/2 lies within DF64_WINDOW * n of an integer,
+ * where the reference's double-float fold (a few df64 operations, each
+ * within about 2^-48 of the exact result relative to n) could round the
+ * start either way.  A point set with no ambiguous point has the same
+ * starts under the f64 fold, the exact fold and any fold within that
+ * window of it. */
+#define DF64_WINDOW 0x1p-44
 long or_bin_keys_f32(int d, const long *fine, int width, const long *side, long M,
                      const float *coords, uint32_t *keys, int32_t *starts, double *offsets)
 {
@@ -601,7 +607,11 @@ long or_bin_keys_f32(int d, const long *fine, int width, const long *side, long 
                 __float128 t = pe - (__float128)hw;
                 long se = (long)t; /* truncation, then up to the ceiling */
                 if ((__float128)se < t) se += 1;
-                if (se != start) ++ambiguous;
+                /* distance of t to the nearest integer (se or se - 1) */
+                __float128 up = (__float128)se - t, down = t - (__float128)(se - 1);
+                __float128 dist = up < down ? up : down;
+                __float128 window = (__float128)DF64_WINDOW * (__float128)fine[a];
+                if (se != start || dist <= window) ++ambiguous;
             }
             long ws = start < 0 ? start + fine[a] : start;
             if (ws < 0) ws = 0;
@@ -651,8 +661,8 @@ static long mode_of(int modeord, long index, long length)
 
 typedef struct {
     int kind; /* 1, 2 or 3 */
-    int d;
-    long nmodes[3];
+    int d;    /* 1..8 */
+    long nmodes[8];
     int modeord;
     double sign;
     long M;
@@ -672,7 +682,7 @@ static void *direct_worker(void *arg)
         double re = 0.0, im = 0.0;
         if (J->kind == 1) {
             long lin = J->sel ? J->sel[o] : o;
-            long k[3] = {0, 0, 0};
+            long k[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             long rem = lin;
             for (int a = 0; a < J->d; ++a) {
                 k[a] = mode_of(J->modeord, rem % J->nmodes[a], J->nmodes[a]);
@@ -723,7 +733,8 @@ static void *direct_worker(void *arg)
     return NULL;
 }
 
-/* Type-1 / type-2 / type-3 direct sums for one transform.  `sel` (may be
+/* Type-1 / type-2 / type-3 direct sums for one transform in 1 to 8
+ * dimensions (src/direct.rs:9-131; src/config.rs:9: eight at most).  `sel` (may be
  * NULL) selects the outputs: linear mode indices (type 1), point indices
  * (type 2) or target indices (type 3).  isign >= 0 is the positive sign
  * (src/config.rs:25-31). */
@@ -731,6 +742,7 @@ int or_direct(int kind, int d, const long *nmodes, int modeord, int isign, long 
               const double *x, const double *in, long nout, const long *sel, const double *s,
               double *out, int nthreads)
 {
+    if (d < 1 || d > 8) return -1;
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     pthread_t th[256];
@@ -739,7 +751,7 @@ int or_direct(int kind, int d, const long *nmodes, int modeord, int isign, long 
         direct_job *J = &jobs[t];
         J->kind = kind;
         J->d = d;
-        for (int a = 0; a < 3; ++a) J->nmodes[a] = (nmodes && a < d) ? nmodes[a] : 1;
+        for (int a = 0; a < 8; ++a) J->nmodes[a] = (nmodes && a < d) ? nmodes[a] : 1;
         J->modeord = modeord;
         J->sign = isign >= 0 ? 1.0 : -1.0;
         J->M = M;

@@ -16,8 +16,11 @@ What pins each oracle function (the checks live in ``tests/test_oracle.py``):
   "parity unpinned" (wgpu-fft is not vendored).
 * ``layout``: the reference layout tests (src/gpu_nd_spread.rs:2123-2275).
 * ``bin_keys_f32``: exact binary128 fold, with an "ambiguous" counter for
-  points whose f64 fold lands within rounding of an integer boundary; the
-  reference's df64 fold is un-vendored, so at such ties parity is unpinned.
+  points whose f64 start differs from the exact one or whose exact position
+  lies within 2^-44 * n of a support-start boundary, the window in which the
+  reference's double-float fold (un-vendored) could round either way; the
+  counter is zero on every seeded point set the tests use, and at such ties
+  parity is unpinned.
 * ``bin_sort``: the permutation contract (stable counting sort, ties by
   index), checked against numpy's stable argsort.
 * ``direct``: the analytic cases of tests/cpu_foundations.rs:327-351 and a

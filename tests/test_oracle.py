@@ -124,6 +124,19 @@ def test_layout_reference_cases():
     assert O.layout([256, 256], w, 0, 100_000)["geometries"][0] == [1, 1]
 
 
+def test_ambiguity_counter_covers_the_double_float_window():
+    """A coordinate whose folded position minus w/2 is exactly an integer
+    (x = 0 on a 256-cell axis with an even width: 128 - 4 = 124) is counted
+    as ambiguous although the f64 and exact folds agree: a double-float fold
+    could land on either side of the boundary.  An odd width puts the same
+    point half a cell from any boundary."""
+    x = np.zeros((1, 1), dtype=np.float32)
+    _, starts, _, amb = O.bin_keys_f32([256], 8, [1], x)
+    assert starts[0, 0] == 124 and amb == 1
+    _, _, _, amb = O.bin_keys_f32([256], 7, [1], x)
+    assert amb == 0
+
+
 def test_bin_keys_and_stable_sort_definition():
     rng = np.random.default_rng(7)
     x = rng.uniform(-np.pi, np.pi, size=(5000, 3)).astype(np.float32)

@@ -44,6 +44,42 @@ def _err_t1(W, S, torch, x, c, n_modes, out, sample=1000, seed=1):
     return float(np.linalg.norm(g - r) / np.linalg.norm(r))
 
 
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+FP16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def _spread_roof(st, w, d, M, B, method, f64=False):
+    """Roofline of a type-1 plan's spread kernel (its HIP-event span from
+    the last call's stage times): 4 w^d M B useful flop against the FP32
+    (FP64) vector peak, or the dense FP16 matrix-core peak for the
+    matrix-core spread."""
+    t = st.get("spread_kernel")
+    if not t:
+        return None
+    flops = 4.0 * w ** d * M * B
+    peak = FP16_MFMA_PEAK_TFLOPS if method.startswith("mfma") else (
+        FP64_PEAK_TFLOPS if f64 else FP32_PEAK_TFLOPS)
+    tf = flops / (t * 1e-3) / 1e12
+    return {"kernel": f"type-1 spread ({method})", "bound": "mfma" if method.startswith("mfma")
+            else ("fp64-vector" if f64 else "fp32-vector"), "achieved": round(tf, 3),
+            "peak": peak, "unit": "TFLOP/s", "frac": round(tf / peak, 5), "traffic": None,
+            "avg_ms": round(t, 4), "algorithmic": f"4*w^d*M*B = {flops:.4g} flop"}
+
+
+def _interp_roof(t_ms, M, d, B, nf, dsize=4):
+    """Roofline of a type-2 interpolation: M (d + 2B) words + 2 N_f B words
+    read/written once, against HBM."""
+    if not t_ms:
+        return None
+    b = M * (dsize * d + 2 * dsize * B) + 2 * dsize * nf * B
+    gbs = b / (t_ms * 1e-3) / 1e9
+    return {"kernel": "type-2 interpolation", "bound": "hbm", "achieved": round(gbs, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": None, "avg_ms": round(t_ms, 4), "algorithmic": f"{b:.4g} B"}
+
+
 def _line(args, metric, value, unit, ms, dtype, config, extra):
     d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
          "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
@@ -65,9 +101,11 @@ def config1(args, W, S, torch):
     plan.encode_gpu(xt, ct, out=out)
     st = plan.stage_times()
     err = _err_t1(W, S, torch, x, c, n_modes, out)
+    w = plan.info()["width"]
     _line(args, "NU points/s, 2D type-1 F32 128^2 modes tol 1e-6", M / t, "points/s", t * 1e3,
           "f32", {"workload": "config 1: 2D type-1, 128x128 modes, M=1e5 uniform"},
-          {"stages_ms": st, "error": {"type1_rel_l2": err, "sample": "1000 modes"}})
+          {"stages_ms": st, "roofline": _spread_roof(st, w, 2, M, 1, "plain"),
+           "error": {"type1_rel_l2": err, "sample": "1000 modes"}})
 
 
 def config3(args, W, S, torch):
@@ -88,7 +126,9 @@ def config3(args, W, S, torch):
         plan.encode_gpu(xt, ct, out=out)
         st = plan.stage_times()
         err = _err_t1(W, S, torch, x, c, n_modes, out, sample=2000, seed=S.config_seed(3, 7))
-        extra = {"stages_ms": st, "error": {"type1_rel_l2": err, "sample": "2000 modes"}}
+        extra = {"stages_ms": st, "roofline": _spread_roof(st, plan.info()["width"], 2, M, 1,
+                                                            method),
+                 "error": {"type1_rel_l2": err, "sample": "2000 modes"}}
         if method == "plain":
             plain_ms[eps] = st["spread_kernel"]
         elif eps in plain_ms and st["spread_kernel"]:
@@ -205,9 +245,10 @@ def config4(args, W, S, torch):
               M * B / t, "points/s", t * 1e3, "f32",
               {"workload": "config 4b: 3D type-1 256^3 modes tol 1e-6, 8192 spokes x 512, B=32",
                "spreader": method},
-              {"stages_ms": st, "error": {"type1_rel_l2_first_vector": errs[0],
-                                          "type1_rel_l2_last_vector": errs[1],
-                                          "sample": "500 modes"}})
+              {"stages_ms": st,
+               "roofline": _spread_roof(st, plan.info()["width"], 3, M, B, method),
+               "error": {"type1_rel_l2_first_vector": errs[0],
+                         "type1_rel_l2_last_vector": errs[1], "sample": "500 modes"}})
         del plan
     # autotuned (spread method and dense bins timed on these points), then
     # the execute of the kept points captured in a HIP graph and replayed

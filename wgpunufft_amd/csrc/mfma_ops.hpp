@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "nufft_internal.hpp"
+#include "tile_stage.hpp"
 
 namespace mnufft {
 namespace mfma {
@@ -88,16 +89,6 @@ __device__ inline f32x8 load8(const float *p)
     const float4 a = *(const float4 *)p;
     const float4 b = *(const float4 *)(p + 4);
     return f32x8{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-}
-
-// Wavefront-wide LDS ordering for one-wavefront workgroups: LDS
-// instructions of a wavefront execute in order, so a compiler barrier is
-// enough (no s_barrier, no wait for outstanding global loads, which stay in
-// flight across it).
-__device__ inline void wave_sync()
-{
-    __builtin_amdgcn_wave_barrier();
-    __asm__ volatile("" ::: "memory");
 }
 
 // Power-of-two scale that brings the largest strength magnitude below one.

@@ -79,21 +79,29 @@ SpreadParams make_spread_params(int d, const int64_t *fine, int width)
     return sp;
 }
 
-bool make_sweep_params(int d, const int64_t *fine, int width, SpreadParams *sp)
+SpreadParams make_tile_params(int d, const int64_t *fine, const int64_t *block)
 {
-    constexpr int64_t tile = 16;
-    if (d != 3 || width > 8) return false;
-    if (fine[0] < tile + width + 1 || fine[1] < tile + width || fine[2] < tile + width) return false;
     SpreadParams q{};
     q.run = 0;
     q.groups0 = 0;
     q.block_count = 1;
     for (int a = 0; a < kMaxDims; ++a) {
-        q.block[a] = tile;
-        q.nblocks[a] = (fine[a] + tile - 1) / tile;
+        q.block[a] = a < d ? block[a] : 1;
+        q.nblocks[a] = a < d ? (fine[a] + q.block[a] - 1) / q.block[a] : 1;
         q.block_count *= q.nblocks[a];
     }
-    *sp = q;
+    return q;
+}
+
+bool make_sweep_params(int d, const int64_t *fine, int width, int64_t tx, SpreadParams *sp)
+{
+    if (d != 3 || width > 8) return false;
+    // every axis at least 16 + width cells (the matrix-core tiles), axis 0
+    // at least tx + width + 1 (the plain row tiles' x reach within the grid)
+    if (fine[0] < 16 + width + 1 || fine[1] < 16 + width || fine[2] < 16 + width) return false;
+    if (fine[0] < tx + width + 1) return false;
+    const int64_t block[kMaxDims] = {tx, 8, 8};
+    *sp = make_tile_params(d, fine, block);
     return true;
 }
 
@@ -172,14 +180,13 @@ bool Plan::init_common()
     layout = build_spread_layout(dim, fine, kernel.width,
                                  precision == Precision::F32 ? Precision::F32 : Precision::F64);
     sp = make_spread_params(dim, fine, kernel.width);
-    sweep = make_sweep_params(dim, fine, kernel.width, &sp);
+    // plain row tiles of 8 x 8 x 8 cells
+    sweep = make_sweep_params(dim, fine, kernel.width, 8, &sp);
     tiles2 = make_tile2_params(dim, fine, kernel.width, &tile2_sp);
     if (sweep) {
-        // matrix-core tiles: the row sweep's 16 x 16 cells, 8 deep along z
-        tile3_sp = sp;
-        tile3_sp.block[2] = 8;
-        tile3_sp.nblocks[2] = (fine[2] + 7) / 8;
-        tile3_sp.block_count = tile3_sp.nblocks[0] * tile3_sp.nblocks[1] * tile3_sp.nblocks[2];
+        // matrix-core tiles: 16 x 16 cells, 8 deep along z
+        const int64_t block[kMaxDims] = {16, 16, 8};
+        tile3_sp = make_tile_params(3, fine, block);
     }
     return true;
 }

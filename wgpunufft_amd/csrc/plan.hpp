@@ -43,10 +43,10 @@ struct Plan {
     double *d_inv_phihat = nullptr;
     SpreadLayout layout;
     SpreadParams sp{};
-    bool sweep = false;  // 3D row-sweep spreader (else output-stationary blocks)
+    bool sweep = false;  // 3D grid qualifies for the row tiles (plain and matrix-core)
     SpreadParams tile2_sp{};
     bool tiles2 = false;  // 2D grid qualifies for the matrix-core 32 x 32 tiles
-    SpreadParams tile3_sp{};  // 3D matrix-core tiles (16 x 16 x 8; row-sweep grids)
+    SpreadParams tile3_sp{};  // 3D matrix-core tiles (16 x 16 x 8; row-tile grids)
     SpreadMethod spread_method = SpreadMethod::Plain;
     // Type-1 order of the points inside a bin: ascending original index (the
     // reference's), or coordinate bits then index (shuffle invariant).
@@ -65,7 +65,8 @@ struct Plan {
             return 0;
         return dense.threshold;
     }
-    // Type-1 spreads on row tiles: the 3D row sweep, or a 2D matrix-core method.
+    // Type-1 spreads on row tiles: the 3D plain or matrix-core tiles, or a 2D
+    // matrix-core method.
     bool rows_geometry() const
     {
         return sweep || (tiles2 && spread_method != SpreadMethod::Plain);

@@ -513,329 +513,7 @@ __global__ __launch_bounds__(256) void k_reduce(GridParams g, SpreadParams sp,
     }
 }
 
-// ---------------------------------------------------------------------
-// Row-sweep spreader (3D, w <= 8).
-//
-// With bins of one cell along axes 1 and 2, every record of a bin row
-// (sy, sz) reaches the same 7x7 (w x w) rows of cells.  One wavefront owns a
-// tile of TX x TY x TZ cells and walks the rows of its reach in order, z
-// outer and y inner.  Lane (ly, dz) owns support offset dz along axis 2 and,
-// along axis 1, the cell y = ly (mod w) of the current window [sy, sy + w):
-// as sy advances by one, the lane whose window cell leaves (dy = 0) is
-// complete for this sz and adds its TX accumulators into an LDS ring of w
-// cell planes, then takes the entering cell.  Every record therefore costs
-// w FMAs per lane on registers with static indices (a switch on the record's
-// x offset), with no redundant cells along x and none idle along y and z
-// except at the tile border.  A plane of the ring is complete once the sweep
-// has passed its z, and is written out exactly once.  Sums stay in a fixed
-// order: rows in order, records in their (canonical) row order, ring adds in
-// row order.
-// ---------------------------------------------------------------------
-constexpr int kSweepRows = 64;
-constexpr int kSweepTile = 16;  // tile edge of the row sweep along every axis
-
-template <typename T, int W>
-struct SweepRec {
-    int e;    // x offset of the support start from the tile's reach start
-    int row;  // row of the tile's reach the record belongs to
-    T cre, cim;
-    T kx[W];
-    T ky[W];
-    T kz[W];
-};
-
-template <int E, int W, int TX, typename T>
-__device__ __forceinline__ void sweep_accumulate(T (&ar)[TX], T (&ai)[TX], const T *kx, T vre,
-                                                 T vim)
-{
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-        constexpr int base = E - (W - 1);
-        const int x = base + j;
-        if (x >= 0 && x < TX) {
-            T k = kx[j];
-            ar[x] += k * vre;
-            ai[x] += k * vim;
-        }
-    }
-}
-
-template <int W, int TX, typename T>
-__device__ __forceinline__ void sweep_dispatch(int e, T (&ar)[TX], T (&ai)[TX], const T *kx, T vre,
-                                               T vim)
-{
-#define MNUFFT_SWEEP_CASE(E)                                                                      \
-    case E:                                                                                       \
-        if constexpr (E < TX + W - 1) sweep_accumulate<E, W, TX, T>(ar, ai, kx, vre, vim);        \
-        break;
-    switch (e) {
-        MNUFFT_SWEEP_CASE(0) MNUFFT_SWEEP_CASE(1) MNUFFT_SWEEP_CASE(2) MNUFFT_SWEEP_CASE(3)
-        MNUFFT_SWEEP_CASE(4) MNUFFT_SWEEP_CASE(5) MNUFFT_SWEEP_CASE(6) MNUFFT_SWEEP_CASE(7)
-        MNUFFT_SWEEP_CASE(8) MNUFFT_SWEEP_CASE(9) MNUFFT_SWEEP_CASE(10) MNUFFT_SWEEP_CASE(11)
-        MNUFFT_SWEEP_CASE(12) MNUFFT_SWEEP_CASE(13) MNUFFT_SWEEP_CASE(14) MNUFFT_SWEEP_CASE(15)
-        MNUFFT_SWEEP_CASE(16) MNUFFT_SWEEP_CASE(17) MNUFFT_SWEEP_CASE(18) MNUFFT_SWEEP_CASE(19)
-        MNUFFT_SWEEP_CASE(20) MNUFFT_SWEEP_CASE(21) MNUFFT_SWEEP_CASE(22) MNUFFT_SWEEP_CASE(23)
-        MNUFFT_SWEEP_CASE(24) MNUFFT_SWEEP_CASE(25) MNUFFT_SWEEP_CASE(26) MNUFFT_SWEEP_CASE(27)
-        MNUFFT_SWEEP_CASE(28) MNUFFT_SWEEP_CASE(29) MNUFFT_SWEEP_CASE(30) MNUFFT_SWEEP_CASE(31)
-    default: break;
-    }
-#undef MNUFFT_SWEEP_CASE
-}
-
-template <typename T, int W, int TX, int TY, int TZ>
-__global__ __launch_bounds__(kLanes) void k_sweep3(GridParams g, SpreadParams sp,
-                                                   const uint32_t *__restrict__ offsets,
-                                                   const int4 *__restrict__ rec_start,
-                                                   const T *__restrict__ rec_off,
-                                                   const T *__restrict__ c_sorted, int64_t M,
-                                                   const uint32_t *__restrict__ totals,
-                                                   const uint32_t *__restrict__ parts,
-                                                   const uint32_t *__restrict__ item_off,
-                                                   const uint32_t *__restrict__ heavy_base,
-                                                   const uint32_t *__restrict__ ctrl,
-                                                   T *__restrict__ grid, T *__restrict__ partial)
-{
-    static_assert(TX + W - 1 <= 32, "x offsets beyond the dispatch table");
-    constexpr int NRY = TY + W - 1;
-    constexpr int NRZ = TZ + W - 1;
-    constexpr int NROWS = NRY * NRZ;
-    __shared__ T ring_re[W][TY][TX];
-    __shared__ T ring_im[W][TY][TX];
-    __shared__ SweepRec<T, W> st[kLanes];
-    __shared__ int64_t row_f0[kSweepRows], row_n0[kSweepRows], row_f1[kSweepRows],
-        row_excl[kSweepRows];
-
-    const int lane = threadIdx.x;
-    const int ly = lane % W;
-    const int dz = lane / W;
-    const bool active = lane < W * W;
-    const int64_t v = blockIdx.y;
-    const T beta = (T)g.beta;
-    const int64_t n_items = ctrl[0];
-    const uint32_t part_len_cap = ctrl[3];
-    const T *cv = c_sorted + 2 * v * M;
-    const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n, n2 = g.ax[2].n;
-    const int64_t bins0 = g.ax[0].bins;
-
-    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-        int64_t lo = 0, hi = sp.block_count - 1;
-        while (lo < hi) {
-            int64_t mid = (lo + hi + 1) >> 1;
-            if ((int64_t)item_off[mid] <= item)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const int64_t b = lo;
-        const int64_t p = item - item_off[b];
-        const uint32_t np = parts[b];
-        const int64_t total = totals[b];
-        int64_t rlo = 0, rhi = total;
-        if (np > 1) {
-            rlo = p * (int64_t)part_len_cap;
-            rhi = rlo + part_len_cap < total ? rlo + part_len_cap : total;
-        }
-        const int64_t bx = b % sp.nblocks[0];
-        const int64_t by = (b / sp.nblocks[0]) % sp.nblocks[1];
-        const int64_t bz = b / (sp.nblocks[0] * sp.nblocks[1]);
-        const int64_t x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
-        const AxisReach rx = axis_reach(x0, TX, W, g.ax[0]);
-        int64_t xs = x0 - (W - 1);
-        if (xs < 0) xs += n0;
-
-        for (int t = lane; t < W * TY * TX; t += kLanes) {
-            (&ring_re[0][0][0])[t] = (T)0;
-            (&ring_im[0][0][0])[t] = (T)0;
-        }
-        T ar[TX], ai[TX];
-#pragma unroll
-        for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
-        __syncthreads();
-
-        // Rows are walked in order; `cur` is the row being accumulated and
-        // `last_rec` the row of the last record added (both wave-uniform).
-        int cur = 0;
-        int last_rec = -(1 << 20);
-        int dy = 0;
-        auto start_row = [&](int r) {
-            const int ry = r % NRY;
-            if (ry == 0) {
-#pragma unroll
-                for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
-            }
-            const int64_t sy_rel = y0 - (W - 1) + ry;
-            int d = (int)((ly - sy_rel) % W);
-            dy = d < 0 ? d + W : d;
-        };
-        auto finish_row = [&](int r) {
-            const int ry = r % NRY, rz = r / NRY;
-            // the lane whose window cell leaves adds it to the ring (only
-            // when a record landed in the window)
-            if (r - last_rec < W && active && dy == 0) {
-                const int yy = ry - (W - 1);
-                const int zz = rz - (W - 1) + dz;
-                if (yy >= 0 && yy < TY && zz >= 0 && zz < TZ) {
-                    const int slot_z = zz % W;
-#pragma unroll
-                    for (int i = 0; i < TX; ++i) {
-                        ring_re[slot_z][yy][i] += ar[i];
-                        ring_im[slot_z][yy][i] += ai[i];
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
-            }
-            if (ry == NRY - 1) {
-                // the plane of this sz is complete
-                __syncthreads();
-                const int zz = rz - (W - 1);
-                if (zz >= 0 && zz < TZ) {
-                    const int slot_z = zz % W;
-                    for (int t = lane; t < TY * TX; t += kLanes) {
-                        const int yy = t / TX, xx = t % TX;
-                        T re = ring_re[slot_z][yy][xx], im = ring_im[slot_z][yy][xx];
-                        ring_re[slot_z][yy][xx] = (T)0;
-                        ring_im[slot_z][yy][xx] = (T)0;
-                        if (np == 1) {
-                            const int64_t gx = x0 + xx, gy = y0 + yy, gz = z0 + zz;
-                            if (gx < n0 && gy < n1 && gz < n2) {
-                                T *o = grid + 2 * (v * g.fine_total + gx + n0 * (gy + n1 * gz));
-                                o[0] = re;
-                                o[1] = im;
-                            }
-                        } else {
-                            const int64_t pslot = (int64_t)heavy_base[b] + p;
-                            T *o = partial +
-                                   2 * (((pslot * gridDim.y + v) * TZ + zz) * (TY * TX) + t);
-                            o[0] = re;
-                            o[1] = im;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        };
-        start_row(0);
-
-        int64_t cursor = 0;
-        for (int rb = 0; rb < NROWS && cursor < rhi; rb += kSweepRows) {
-            int64_t batch_total;
-            {
-                int r = rb + lane;
-                int64_t f0 = 0, c0 = 0, f1 = 0, c1 = 0;
-                if (r < NROWS) {
-                    int ry = r % NRY, rz = r / NRY;
-                    int64_t sy = y0 - (W - 1) + ry;
-                    int64_t sz = z0 - (W - 1) + rz;
-                    if (sy < 0) sy += n1;
-                    if (sy >= n1) sy -= n1;
-                    if (sz < 0) sz += n2;
-                    if (sz >= n2) sz -= n2;
-                    int64_t row = bins0 * (sy + n1 * sz);
-                    f0 = offsets[row + rx.lo0];
-                    c0 = (int64_t)offsets[row + rx.lo0 + rx.len0] - f0;
-                    if (rx.len1 > 0) {
-                        f1 = offsets[row + rx.lo1];
-                        c1 = (int64_t)offsets[row + rx.lo1 + rx.len1] - f1;
-                    }
-                }
-                int64_t cnt = c0 + c1;
-                int64_t xsum = cnt;
-#pragma unroll
-                for (int o = 1; o < kLanes; o <<= 1) {
-                    int64_t y = __shfl_up(xsum, o, kLanes);
-                    if (lane >= o) xsum += y;
-                }
-                batch_total = __shfl(xsum, kLanes - 1, kLanes);
-                row_f0[lane] = f0;
-                row_n0[lane] = c0;
-                row_f1[lane] = f1;
-                row_excl[lane] = xsum - cnt;
-                __syncthreads();
-            }
-            const int64_t klo = rlo > cursor ? rlo - cursor : 0;
-            const int64_t khi = rhi - cursor < batch_total ? rhi - cursor : batch_total;
-            for (int64_t k0 = klo; k0 < khi; k0 += kLanes) {
-                const int64_t k = k0 + lane;
-                bool take = false;
-                int64_t slot = 0;
-                int e = 0, q = 0;
-                T rox = (T)0, roy = (T)0, roz = (T)0, rcre = (T)0, rcim = (T)0;
-                if (k < khi) {
-                    int s_lo = 0, s_hi = kSweepRows - 1;
-                    while (s_lo < s_hi) {
-                        int mid = (s_lo + s_hi + 1) >> 1;
-                        if (row_excl[mid] <= k)
-                            s_lo = mid;
-                        else
-                            s_hi = mid - 1;
-                    }
-                    q = s_lo;
-                    const int64_t j = k - row_excl[q];
-                    slot = j < row_n0[q] ? row_f0[q] + j : row_f1[q] + (j - row_n0[q]);
-                    // every load of the record in one round trip
-                    const int sx0 = rec_start[slot].x;
-                    rox = rec_off[4 * slot + 0];
-                    roy = rec_off[4 * slot + 1];
-                    roz = rec_off[4 * slot + 2];
-                    rcre = cv[2 * slot];
-                    rcim = cv[2 * slot + 1];
-                    int64_t ee = (int64_t)sx0 - xs;
-                    if (ee < 0) ee += n0;
-                    e = (int)ee;
-                    take = ee < TX + W - 1;
-                }
-                const uint64_t mask = __ballot(take);
-                const int pos = __popcll(mask & ((1ull << lane) - 1ull));
-                const int nst = __popcll(mask);
-                if (take) {
-                    SweepRec<T, W> &S = st[pos];
-                    S.e = e;
-                    S.row = rb + q;
-                    S.cre = rcre;
-                    S.cim = rcim;
-                    const T ox = rox, oy = roy, oz = roz;
-#pragma unroll
-                    for (int jj = 0; jj < W; ++jj) {
-                        S.kx[jj] = es_weight<T>((T)jj + ox, W, beta);
-                        S.ky[jj] = es_weight<T>((T)jj + oy, W, beta);
-                        S.kz[jj] = es_weight<T>((T)jj + oz, W, beta);
-                    }
-                }
-                __syncthreads();
-                for (int s2 = 0; s2 < nst; ++s2) {
-                    const SweepRec<T, W> &S = st[s2];
-                    const int rr = __builtin_amdgcn_readfirstlane(S.row);
-                    while (cur < rr) {
-                        finish_row(cur);
-                        ++cur;
-                        start_row(cur);
-                    }
-                    last_rec = rr;
-                    if (active) {
-                        T kx[W];
-#pragma unroll
-                        for (int jj = 0; jj < W; ++jj) kx[jj] = S.kx[jj];
-                        const T wyz = S.ky[dy] * S.kz[dz];
-                        const T vre = S.cre * wyz, vim = S.cim * wyz;
-                        const int eu = __builtin_amdgcn_readfirstlane(S.e);
-                        sweep_dispatch<W, TX, T>(eu, ar, ai, kx, vre, vim);
-                    }
-                }
-                __syncthreads();
-            }
-            cursor += batch_total;
-            __syncthreads();
-        }
-        while (cur < NROWS) {
-            finish_row(cur);
-            ++cur;
-            if (cur < NROWS) start_row(cur);
-        }
-    }
-}
-
-// Heavy tiles of the row sweep and the matrix-core tiles: partial sums as in
+// Heavy tiles of the row tiles and the matrix-core tiles: partial sums as in
 // k_reduce.
 template <typename T, int TX, int TY, int TZ>
 __global__ __launch_bounds__(256) void k_reduce_tiles(GridParams g, SpreadParams sp,
@@ -917,19 +595,22 @@ bool chunk_spread(Spreader &s, const GridParams &g, const SpreadParams &sp,
     return hip_ok(hipGetLastError(), "spread launch");
 }
 
-template <typename T, int W>
-bool chunk_sweep(Spreader &s, const GridParams &g, const SpreadParams &sp,
-                 const uint32_t *offsets, const int4 *rec_start, const T *rec_off,
-                 const T *c_sorted, int64_t M, int64_t vectors, T *grid, hipStream_t stream)
+template <typename T>
+bool chunk_rows(Spreader &s, const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
+                const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
+                int64_t vectors, T *grid, hipStream_t stream)
 {
-    constexpr int TX = kSweepTile, TY = kSweepTile, TZ = kSweepTile;
-    k_sweep3<T, W, TX, TY, TZ><<<items_grid(sp.block_count, vectors), kLanes, 0, stream>>>(
-        g, sp, offsets, rec_start, rec_off, c_sorted, M, s.totals, s.parts, s.item_off,
-        s.heavy_base, s.ctrl, grid, (T *)s.partial);
+    const dim3 ig = items_grid(sp.block_count, vectors);
+    TileWork w{s.totals, s.parts, s.item_off, s.heavy_base, s.ctrl, nullptr, ig.x};
+    if (!launch_rows_tiles<T>(g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, w, grid,
+                              (T *)s.partial, stream))
+        return false;
     if (s.kernel_end) (void)hipEventRecord(s.kernel_end, stream);
-    k_reduce_tiles<T, TX, TY, TZ><<<dim3(2048, (unsigned)vectors), 256, 0, stream>>>(
-        g, sp, s.parts, s.heavy_base, s.heavy_list, s.ctrl, (const T *)s.partial, grid);
-    return hip_ok(hipGetLastError(), "sweep launch");
+    const dim3 rg(2048, (unsigned)vectors);
+    k_reduce_tiles<T, 8, 8, 8><<<rg, 256, 0, stream>>>(g, sp, s.parts, s.heavy_base,
+                                                       s.heavy_list, s.ctrl, (const T *)s.partial,
+                                                       grid);
+    return hip_ok(hipGetLastError(), "row-tile spread launch");
 }
 
 template <int D>
@@ -938,7 +619,7 @@ bool chunk_mfma(Spreader &s, SpreadMethod method, const GridParams &g, const Spr
                 const float *c_sorted, int64_t M, int64_t vectors, const uint32_t *vmax,
                 float *grid, bool mark_end, hipStream_t stream)
 {
-    constexpr int TE = D == 3 ? kSweepTile : 32;  // tile edge along x and y
+    constexpr int TE = D == 3 ? 16 : 32;  // tile edge along x and y
     const dim3 ig = items_grid(sp.block_count, vectors);
     TileWork w{s.totals, s.parts, s.item_off, s.heavy_base, s.ctrl, vmax, ig.x};
     if (!launch_mfma_tiles(method, D, g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, w,
@@ -1027,7 +708,7 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
         return false;
     }
     if (!mfma && tiles && (D != 3 || g.width < 2 || g.width > 8)) {
-        set_error("row sweep needs a 3D plan of kernel width <= 8");
+        set_error("plain row tiles need a 3D plan of kernel width <= 8");
         return false;
     }
     if (!mfma && !tiles && sp.run != 2 && sp.run != 4 && sp.run != 8) {
@@ -1093,18 +774,9 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
                 if (ok && dense && kernel_end) (void)hipEventRecord(kernel_end, stream);
             }
         } else if (tiles) {
-            if constexpr (D == 3) {
-                switch (g.width) {
-                case 2: ok = chunk_sweep<T, 2>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
-                case 3: ok = chunk_sweep<T, 3>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
-                case 4: ok = chunk_sweep<T, 4>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
-                case 5: ok = chunk_sweep<T, 5>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
-                case 6: ok = chunk_sweep<T, 6>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
-                case 7: ok = chunk_sweep<T, 7>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
-                case 8: ok = chunk_sweep<T, 8>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
-                default: break;
-                }
-            }
+            if constexpr (D == 3)
+                ok = chunk_rows<T>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr,
+                                   stream);
         } else {
             switch (sp.run) {
             case 2: ok = chunk_spread<D, T, 2>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;

@@ -58,7 +58,7 @@ struct Spreader {
     // Spreads the prepared records (in bin order) of `vectors` strength
     // vectors onto `grid` (vectors * fine_total complex values).  Every cell
     // is written exactly once.
-    // A matrix-core method needs FP32 and row-sweep tiles (run == 0).
+    // A matrix-core method needs FP32 and row tiles (run == 0).
     // With `dense`, offsets are the light-record offsets and the dense bins'
     // windows are added after the tile kernels.
     template <int D, typename T>
@@ -101,7 +101,7 @@ struct TileWork {
 };
 
 // Tile kernel of the matrix-core spread (spread_mfma.hip); partial sums of
-// heavy parts go to `partial` in the layout of the row-sweep tiles.
+// heavy parts go to `partial` in the layout k_reduce_tiles reads.
 bool launch_mfma_tiles(SpreadMethod method, int dim, const GridParams &g, const SpreadParams &sp,
                        const uint32_t *offsets, const int4 *rec_start, const float *rec_off,
                        const float *c_sorted, int64_t M, int64_t vectors, const TileWork &w,
@@ -143,9 +143,21 @@ struct SmallSpreader {
 // unique within a run).
 SpreadParams make_spread_params(int d, const int64_t *fine, int width);
 
-// Row-sweep tiles (3D, width <= 8, every axis at least tile + width long):
-// run = 0 marks them.  Returns false when the grid does not qualify.
-bool make_sweep_params(int d, const int64_t *fine, int width, SpreadParams *sp);
+// Tiles of the given cells per axis (run = 0 marks tiles).
+SpreadParams make_tile_params(int d, const int64_t *fine, const int64_t *block);
+
+// Plain row tiles of tx x 8 x 8 cells (tx = 8; 3D, width <= 8, every axis
+// at least 16 + width long, axis 0 at least 16 + width + 1, the matrix-core
+// tiles' bound).  Returns false when the grid does not qualify.
+bool make_sweep_params(int d, const int64_t *fine, int width, int64_t tx, SpreadParams *sp);
+
+// Tile kernel of the plain row-tile spread (spread_rows.hip): 8 x 8 x 8
+// tiles, partial sums of heavy parts in the layout k_reduce_tiles reads.
+template <typename T>
+bool launch_rows_tiles(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
+                       const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
+                       int64_t vectors, const TileWork &w, T *grid, T *partial,
+                       hipStream_t stream);
 
 // 32 x 32 tiles of the 2D matrix-core spread (width <= 8, every axis at least
 // 32 + width long), run = 0.  Returns false when the grid does not qualify.

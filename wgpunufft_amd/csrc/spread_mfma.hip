@@ -34,6 +34,7 @@
 #include "mfma_ops.hpp"
 #include "spread.hpp"
 #include "spread_device.hpp"
+#include "tile_stage.hpp"
 
 namespace mnufft {
 
@@ -49,85 +50,6 @@ struct MfmaRec {
     float ox, oy, oz;
     float cre, cim;  // scaled strength
 };
-
-template <int D, int TY, int TZ, int W, int N>
-__device__ inline uint32_t stage_tile_rows(TileRows<N> &rows, int lane, int64_t y0, int64_t z0,
-                                           const AxisReach &rx, const GridParams &g,
-                                           const uint32_t *__restrict__ offsets)
-{
-    constexpr int NRY = TY + W - 1;
-    constexpr int NROWS = D == 3 ? NRY * (TZ + W - 1) : NRY;
-    constexpr int NB = (NROWS + 63) / 64;
-    static_assert(NROWS <= N, "reach rows beyond the staging arrays");
-    // all loads first, then the scans
-    uint32_t f0[NB], c0[NB], f1[NB], c1[NB];
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-        const int r = 64 * i + lane;
-        f0[i] = c0[i] = f1[i] = c1[i] = 0;
-        if (r < NROWS) {
-            const int64_t row = tile_row_base<D, NRY, W>(r, y0, z0, g);
-            f0[i] = offsets[row + rx.lo0];
-            c0[i] = offsets[row + rx.lo0 + rx.len0];
-            if (rx.len1 > 0) {
-                f1[i] = offsets[row + rx.lo1];
-                c1[i] = offsets[row + rx.lo1 + rx.len1];
-            }
-        }
-    }
-    uint32_t carry = 0;
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-        const int r = 64 * i + lane;
-        const uint32_t n0 = c0[i] - f0[i], n1c = c1[i] - f1[i];
-        const uint32_t cnt = n0 + n1c;
-        uint32_t x = cnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (r < NROWS) {
-            rows.f0[r] = f0[i];
-            rows.n0[r] = n0;
-            rows.f1[r] = f1[i];
-            rows.excl[r] = carry + x - cnt;
-        }
-        carry += __shfl(x, 63, 64);
-    }
-    if (lane == 0) rows.excl[NROWS] = carry;
-    wave_sync();
-    return carry;
-}
-
-// One lane's record of a chunk, with its global loads in flight until used.
-struct Fetched {
-    int q, sx0;
-    float4 off;
-    float2 c;
-    bool valid;
-};
-
-template <int NROWS>
-__device__ inline Fetched fetch_record(const TileRows<NROWS> &rows, int64_t k, int64_t khi,
-                                       const int4 *__restrict__ rec_start,
-                                       const float *__restrict__ rec_off,
-                                       const float *__restrict__ cv)
-{
-    Fetched f;
-    f.valid = k < khi;
-    f.q = 0;
-    f.sx0 = 0;
-    f.off = float4{0.0f, 0.0f, 0.0f, 0.0f};
-    f.c = float2{0.0f, 0.0f};
-    if (f.valid) {
-        const uint32_t slot = tile_slot<NROWS>(rows, (uint32_t)k, &f.q);
-        f.sx0 = rec_start[slot].x;
-        f.off = *(const float4 *)(rec_off + 4 * (int64_t)slot);
-        f.c = *(const float2 *)(cv + 2 * (int64_t)slot);
-    }
-    return f;
-}
 
 // Largest |re|, |im| of each vector's strengths (blockIdx.y), as float bits
 // (NaN above Inf).
@@ -236,10 +158,10 @@ k_spread_mfma3(
         // records in chunks of 64, the next chunk's loads in flight while
         // the current one is contracted
         const int64_t klo = rlo, khi = rhi;
-        Fetched nx = fetch_record<NROWS>(rows, klo + lane, khi, rec_start, rec_off, cv);
+        Fetched<float> nx = fetch_record<float, NROWS>(rows, klo + lane, khi, rec_start, rec_off, cv);
         for (int64_t k0 = klo; k0 < khi; k0 += 64) {
-            const Fetched cur = nx;
-            nx = fetch_record<NROWS>(rows, k0 + 64 + lane, khi, rec_start, rec_off, cv);
+            const Fetched<float> cur = nx;
+            nx = fetch_record<float, NROWS>(rows, k0 + 64 + lane, khi, rec_start, rec_off, cv);
             bool take = false;
             MfmaRec R{};
             if (cur.valid) {
@@ -402,10 +324,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         // records in chunks of 64, the next chunk's loads in flight while
         // the current one is contracted
         const int64_t klo = ti.rlo, khi = ti.rhi;
-        Fetched nx = fetch_record<NROWS>(rows, klo + lane, khi, rec_start, rec_off, cv);
+        Fetched<float> nx = fetch_record<float, NROWS>(rows, klo + lane, khi, rec_start, rec_off, cv);
         for (int64_t k0 = klo; k0 < khi; k0 += 64) {
-            const Fetched cur = nx;
-            nx = fetch_record<NROWS>(rows, k0 + 64 + lane, khi, rec_start, rec_off, cv);
+            const Fetched<float> cur = nx;
+            nx = fetch_record<float, NROWS>(rows, k0 + 64 + lane, khi, rec_start, rec_off, cv);
             bool take = false;
             MfmaRec R{};
             if (cur.valid) {

@@ -1,0 +1,213 @@
+// Plain (FP32 / FP64 vector) type-1 spreading of 3D row tiles.
+//
+// Same contract as the reference's output-stationary spreader
+// (src/gpu_nd_spread.rs:1-36, 1583-1667): every cell of an 8 x 8 x 8 tile
+// (x, y, z) is owned by one lane of one wavefront, which adds the records
+// whose support reaches the tile in a fixed order (reach row by reach row,
+// slot order inside a row) and writes the cell once; tiles with too many
+// records are split into parts whose partial sums are added in part order.
+// No atomics, so results are bitwise repeatable.
+//
+// Mapping: bins are one cell along y and z, so the records of one reach row
+// share their y and z support.  Lane l owns the row of cells (y = l % 8,
+// z = l / 8) across the tile's TX x cells, held in registers.  Every staged
+// record carries its weights laid out over the tile's own cells (zero off
+// its support), so a record costs each lane two weight reads (its y and z
+// cell), one product, the record's TX x weights (broadcast vector reads) and
+// TX complex FMAs on registers with static indices: no branch on the
+// record's position and no dynamically indexed registers.  Records are
+// staged 64 at a time (their global loads issued one chunk ahead),
+// compacted to those whose x support touches the tile, with their weights
+// computed once per tile visit.
+#include <hip/hip_runtime.h>
+
+#include "spread.hpp"
+#include "spread_device.hpp"
+#include "tile_stage.hpp"
+
+namespace mnufft {
+
+namespace {
+
+constexpr int kRowsTX = 8, kRowsTY = 8, kRowsTZ = 8;
+
+// A staged record: its weights over the tile's cells of each axis (zero
+// off the support) and its strength.
+template <typename T, int TX>
+struct RowRec {
+    T kx[TX];
+    T ky[kRowsTY];
+    T kz[kRowsTZ];
+    T cre, cim;
+};
+
+// Weights of a record over the n cells of a tile axis from `rel` (the
+// support start relative to the tile origin) and its offset.
+template <typename T, int W, int N>
+__device__ inline void tile_weights(T *dst, int rel, T off, T beta)
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const int t = i - rel;
+        dst[i] = (t >= 0 && t < W) ? es_weight<T>((T)t + off, W, beta) : (T)0;
+    }
+}
+
+template <typename T, int W, int TX>
+__global__ __launch_bounds__(64) void k_spread_rows(
+    GridParams g, SpreadParams sp, const uint32_t *__restrict__ offsets,
+    const int4 *__restrict__ rec_start, const T *__restrict__ rec_off,
+    const T *__restrict__ c_sorted, int64_t M, const uint32_t *__restrict__ totals,
+    const uint32_t *__restrict__ parts, const uint32_t *__restrict__ item_off,
+    const uint32_t *__restrict__ heavy_base, const uint32_t *__restrict__ ctrl,
+    T *__restrict__ grid, T *__restrict__ partial)
+{
+    static_assert(W <= 8, "widths beyond the row tiles");
+    constexpr int TY = kRowsTY, TZ = kRowsTZ;
+    constexpr int NRY = TY + W - 1;
+    constexpr int NROWS = NRY * (TZ + W - 1);
+    __shared__ RowRec<T, TX> st[64];
+    __shared__ TileRows<NROWS> rows;
+
+    const int lane = threadIdx.x;
+    const int ly = lane & 7, lz = lane >> 3;  // the lane's row of the tile
+    const int64_t v = blockIdx.y;
+    const T beta = (T)g.beta;
+    const int64_t n_items = ctrl[0];
+    const uint32_t part_len_cap = ctrl[3];
+    const T *cv = c_sorted + 2 * v * M;
+    const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n, n2 = g.ax[2].n;
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const TileItem ti = tile_item(item, sp.block_count, item_off, parts, totals, part_len_cap);
+        const int64_t b = ti.b;
+        const int64_t x0 = (b % sp.nblocks[0]) * TX;
+        const int64_t y0 = ((b / sp.nblocks[0]) % sp.nblocks[1]) * TY;
+        const int64_t z0 = (b / (sp.nblocks[0] * sp.nblocks[1])) * TZ;
+        const AxisReach rx = axis_reach(x0, TX, W, g.ax[0]);
+        int64_t xs = x0 - (W - 1);
+        if (xs < 0) xs += n0;
+
+        stage_tile_rows<3, TY, TZ, W, NROWS>(rows, lane, y0, z0, rx, g, offsets);
+        T ar[TX], ai[TX];
+#pragma unroll
+        for (int i = 0; i < TX; ++i) ar[i] = ai[i] = (T)0;
+        // records in chunks of 64, the next chunk's loads in flight while
+        // the current one is added
+        const int64_t klo = ti.rlo, khi = ti.rhi;
+        Fetched<T> nx = fetch_record<T, NROWS>(rows, klo + lane, khi, rec_start, rec_off, cv);
+        for (int64_t k0 = klo; k0 < khi; k0 += 64) {
+            const Fetched<T> cur = nx;
+            nx = fetch_record<T, NROWS>(rows, k0 + 64 + lane, khi, rec_start, rec_off, cv);
+            bool take = false;
+            int ee = 0;
+            if (cur.valid) {
+                int64_t e64 = (int64_t)cur.sx0 - xs;
+                if (e64 < 0) e64 += n0;
+                take = e64 < TX + W - 1;
+                ee = (int)e64;
+            }
+            const uint64_t mask = __ballot(take);
+            const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+            const int nst = __popcll(mask);
+            if (take) {
+                RowRec<T, TX> &R = st[pos];
+                tile_weights<T, W, TX>(R.kx, ee - (W - 1), (T)cur.off.x, beta);
+                tile_weights<T, W, TY>(R.ky, cur.q % NRY - (W - 1), (T)cur.off.y, beta);
+                tile_weights<T, W, TZ>(R.kz, cur.q / NRY - (W - 1), (T)cur.off.z, beta);
+                R.cre = cur.c.x;
+                R.cim = cur.c.y;
+            }
+            wave_sync();
+#pragma unroll 2
+            for (int s = 0; s < nst; ++s) {
+                const RowRec<T, TX> &S = st[s];
+                const T wyz = S.ky[ly] * S.kz[lz];
+                const T vre = S.cre * wyz, vim = S.cim * wyz;
+#pragma unroll
+                for (int i = 0; i < TX; ++i) {
+                    const T kx = S.kx[i];
+                    ar[i] += vre * kx;
+                    ai[i] += vim * kx;
+                }
+            }
+            wave_sync();
+        }
+
+        const int64_t gy = y0 + ly, gz = z0 + lz;
+        if (ti.np == 1) {
+            if (gy < n1 && gz < n2) {
+                T *row = grid + 2 * (v * g.fine_total + x0 + n0 * (gy + n1 * gz));
+#pragma unroll
+                for (int i = 0; i < TX; ++i) {
+                    if (x0 + i < n0) {
+                        row[2 * i] = ar[i];
+                        row[2 * i + 1] = ai[i];
+                    }
+                }
+            }
+        } else {
+            const int64_t pslot = (int64_t)heavy_base[b] + ti.p;
+#pragma unroll
+            for (int i = 0; i < TX; ++i) {
+                T *o = partial + partial_index<TX, TY, TZ>(pslot, gridDim.y, v, i, ly, lz);
+                o[0] = ar[i];
+                o[1] = ai[i];
+            }
+        }
+    }
+}
+
+template <typename T, int W>
+bool launch_rows_width(int tx, dim3 grid3, const GridParams &g, const SpreadParams &sp,
+                       const uint32_t *offsets, const int4 *rec_start, const T *rec_off,
+                       const T *c_sorted, int64_t M, const TileWork &w, T *grid, T *partial,
+                       hipStream_t stream)
+{
+    if (tx != kRowsTX) return false;
+    k_spread_rows<T, W, kRowsTX><<<grid3, 64, 0, stream>>>(
+        g, sp, offsets, rec_start, rec_off, c_sorted, M, w.totals, w.parts, w.item_off,
+        w.heavy_base, w.ctrl, grid, partial);
+    return true;
+}
+
+}  // namespace
+
+template <typename T>
+bool launch_rows_tiles(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
+                       const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
+                       int64_t vectors, const TileWork &w, T *grid, T *partial, hipStream_t stream)
+{
+    const dim3 grid3(w.items_grid, (unsigned)vectors);
+    const int tx = (int)sp.block[0];
+    if (sp.block[1] != kRowsTY || sp.block[2] != kRowsTZ) {
+        set_error("plain row tiles are 8 x 8 x 8 cells");
+        return false;
+    }
+    bool ok = false;
+    switch (g.width) {
+    case 2: ok = launch_rows_width<T, 2>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 3: ok = launch_rows_width<T, 3>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 4: ok = launch_rows_width<T, 4>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 5: ok = launch_rows_width<T, 5>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 6: ok = launch_rows_width<T, 6>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 7: ok = launch_rows_width<T, 7>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    case 8: ok = launch_rows_width<T, 8>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+    default: break;
+    }
+    if (!ok) {
+        set_error("plain row tiles need a 3D plan, a width of 2..8 and 8 x 8 x 8 tiles");
+        return false;
+    }
+    return hip_ok(hipGetLastError(), "row-tile spread launch");
+}
+
+template bool launch_rows_tiles<float>(const GridParams &, const SpreadParams &, const uint32_t *,
+                                       const int4 *, const float *, const float *, int64_t,
+                                       int64_t, const TileWork &, float *, float *, hipStream_t);
+template bool launch_rows_tiles<double>(const GridParams &, const SpreadParams &,
+                                        const uint32_t *, const int4 *, const double *,
+                                        const double *, int64_t, int64_t, const TileWork &,
+                                        double *, double *, hipStream_t);
+
+}  // namespace mnufft
