@@ -260,6 +260,25 @@ int mnufft_set_spread_tuning(mnufft_plan plan, int64_t light_limit, int64_t min_
                              int64_t part_budget_bytes);
 
 /*
+ * Bins and tiles of the type-1 spread (no counterpart in the reference):
+ * rows_side0, the side along axis 0 of the bins of the row geometry (the 3D
+ * plain and matrix-core tiles; 0 = the smallest power of two from 2 keeping
+ * the bin count within 2^24, else a power of two up to 64, never finer than
+ * the automatic side); plain_rows, the plain 3D spread on 8 x 8 x 8 row
+ * tiles (1, the default) or on the output-stationary blocks of the
+ * reference's layout (0).  Applies from the next set_points (an execute
+ * re-bins kept points); mnufft_autotune sets both.
+ */
+int mnufft_set_bin_tuning(mnufft_plan plan, int64_t rows_side0, int32_t plain_rows);
+
+/*
+ * The plan's spread tuning: vals[0] spread method, [1] dense-bin threshold,
+ * [2] rows_side0, [3] plain_rows, [4] light limit, [5] smallest part.
+ * Writes min(n, 6) values.
+ */
+int mnufft_plan_tuning(mnufft_plan plan, int64_t *vals, int32_t n);
+
+/*
  * Dense bins of the 3D matrix-core type-1 spread (no counterpart in the
  * reference, whose heavy blocks cover clusters): a bin holding at least
  * min_records points (default 32; doubled, up to seven times, while the
@@ -279,7 +298,12 @@ int mnufft_set_dense_bins(mnufft_plan plan, int64_t min_records);
  * whose operand error stays below the plan's tolerance (FP16 hi/lo split;
  * bfloat16 split at tol >= 1e-4; single FP16 at tol >= 1e-3), in 3D each
  * with dense bins on (32) and off -- and keeps the fastest on the plan
- * (dense bins only when at least 3% faster than the same method without).
+ * (dense bins only when at least 3% faster than the same method without);
+ * then, for the method kept, its bins and tiles one knob at a time, each
+ * kept when at least 3% faster: the plain 3D spread on the reference's
+ * output-stationary blocks instead of the row tiles, bins of 4 or 8 cells
+ * along axis 0 of the row geometry, and a light limit of 1024 or 16384
+ * records for the heavy-tile split (mnufft_plan_tuning reads the result).
  * choice (optional, 4 values): spread method, dense-bin threshold, time per
  * encode in ns, candidates tried.  Synchronizes the stream; the plan's
  * output stays bitwise repeatable for the chosen configuration.

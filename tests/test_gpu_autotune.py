@@ -59,3 +59,35 @@ def test_autotune_rejects_type2():
     c = torch.zeros(100, dtype=torch.complex64, device="cuda")
     with pytest.raises(ValueError):
         p.autotune(x, c)
+
+
+@pytest.mark.parametrize("method,side0,plain_rows", [
+    ("plain", 0, False), ("plain", 4, True), ("mfma_f16x3", 8, True), ("mfma_f16x3", 4, True)])
+def test_bin_tuning_knobs_meet_tolerance(method, side0, plain_rows):
+    """Every bins/tiles knob the autotuner may set gives the direct sums to
+    the reference's 3D tolerance and stays bitwise repeatable; the plain
+    output-stationary blocks and the row tiles agree to FP32 rounding."""
+    torch, W, O, S = _setup()
+    n_modes, eps, M = [32, 28, 24], 1e-6, 120000
+    x = S.clustered_coords(S.config_seed(2, 1700), M, (0.3, -1.1, 2.0), 0.05)
+    x[: M // 2] = S.uniform_coords(S.config_seed(2, 1701), M // 2, 3)
+    c = S.complex_normal(S.config_seed(2, 1702), M)
+    xt, ct = torch.from_numpy(x).cuda(), torch.from_numpy(c).cuda()
+    plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, eps)).set_spread_method(method)
+    plan.set_bin_tuning(side0, plain_rows)
+    t = plan.tuning()
+    assert t["rows_side0"] == side0 and t["plain_rows"] == plain_rows
+    f1 = plan.encode_gpu(xt, ct).clone()
+    f2 = plan.execute_gpu(ct).clone()
+    torch.cuda.synchronize()
+    assert torch.equal(f1.view(torch.int32), f2.view(torch.int32))
+    ref_plan = W.NufftPlan.type1_gpu(W.NufftConfig(n_modes, eps))
+    base = ref_plan.encode_gpu(xt, ct).cpu().numpy()
+    got = f1.cpu().numpy()
+    assert rel_l2(got, base) <= (2e-6 if method != "plain" else 1e-6)
+    N = int(np.prod(n_modes))
+    sel = np.arange(0, N, 17)
+    ref = O.direct_type1(x.astype(np.float64), c.astype(np.complex128), n_modes, modes=sel)
+    e = rel_l2(got[sel], ref)
+    print(f"BIN TUNING {method} side0={side0} rows={plain_rows}: err={e:.3e}", flush=True)
+    assert e <= 32 * eps

@@ -154,13 +154,32 @@ class _PlanBase:
         them off.  Takes effect at the next set_points or execute."""
         check(lib().mnufft_set_dense_bins(self._handle, int(min_records)))
         return self
+
+    def set_bin_tuning(self, rows_side0=0, plain_rows=True):
+        """Bins and tiles of the type-1 spread: the axis-0 side of the row
+        geometry's bins (0: automatic, else a power of two up to 64) and the
+        plain 3D spread on 8x8x8 row tiles (True) or on the reference's
+        output-stationary blocks (False).  Takes effect at the next
+        set_points or execute."""
+        check(lib().mnufft_set_bin_tuning(self._handle, int(rows_side0), 1 if plain_rows else 0))
         return self
+
+    def tuning(self):
+        """The spread tuning in use: method, dense-bin threshold, axis-0 bin
+        side (0 automatic), plain row tiles, light limit, smallest part."""
+        v = (ctypes.c_int64 * 6)()
+        check(lib().mnufft_plan_tuning(self._handle, v, 6))
+        names = {k: n for n, k in SPREAD_METHODS.items()}
+        return {"spread_method": names.get(v[0], v[0]), "dense_bins": v[1], "rows_side0": v[2],
+                "plain_rows": bool(v[3]), "light_limit": v[4], "min_part": v[5]}
 
     def autotune(self, points, values, out=None, repeats=3, active_batch=None):
         """Type 1: times `repeats` encodes of these points and strengths for
         the plain spread and the matrix-core methods within the plan's
-        tolerance (3D: dense bins on and off) and keeps the fastest; returns
-        the choice.  `out` receives the encodes."""
+        tolerance (3D: dense bins on and off) and keeps the fastest, then
+        tries the bins and tiles of the method kept (plain output-stationary
+        blocks, coarser axis-0 bins, the heavy-tile split); returns the
+        choice.  `out` receives the encodes."""
         if self.kind != 1:
             raise ValueError("autotune takes a type-1 plan")
         self._check_points(points)
@@ -175,8 +194,11 @@ class _PlanBase:
                                     _ptr(out), int(repeats), choice, _stream_ptr()))
         self._points = points
         names = {v: k for k, v in SPREAD_METHODS.items()}
+        t = self.tuning()
         return {"spread_method": names.get(choice[0], choice[0]), "dense_bins": choice[1],
-                "ms_per_encode": choice[2] / 1e6, "candidates": choice[3]}
+                "ms_per_encode": choice[2] / 1e6, "candidates": choice[3],
+                "rows_side0": t["rows_side0"], "plain_rows": t["plain_rows"],
+                "light_limit": t["light_limit"]}
 
     def timing_history(self, on=True):
         """Sum stage times over every call until the next stage_times()."""

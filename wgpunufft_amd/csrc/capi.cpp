@@ -578,6 +578,58 @@ int mnufft_autotune(mnufft_plan plan, int64_t active_batch, int64_t M, const voi
         }
         p.spread_method = cands[best_i].method;
         p.dense.threshold = cands[best_i].dense;
+        int64_t tried = (int64_t)cands.size();
+        // Then, for the method kept, the bins and tiles: the plain 3D spread
+        // on the output-stationary blocks of the reference layout instead of
+        // the row tiles; coarser bins of the row geometry along axis 0 (2^k
+        // cells); the heavy-tile split (light limit).  Each is kept when it
+        // gains at least 3% over the best so far.
+        struct Knob {
+            bool plain_rows;
+            int64_t side0;
+            uint32_t light;
+        };
+        const Knob base{p.plain_rows, p.rows_side0, p.spreader.light_limit};
+        std::vector<Knob> knobs;
+        if (p.sweep && p.spread_method == mnufft::SpreadMethod::Plain)
+            knobs.push_back({!base.plain_rows, base.side0, base.light});
+        if (p.sweep)
+            for (int64_t s0 : {4, 8}) knobs.push_back({base.plain_rows, s0, base.light});
+        for (uint32_t lim : {1024u, 16384u}) knobs.push_back({base.plain_rows, base.side0, lim});
+        Knob keep = base;
+        if (!mnufft::hip_ok(hipEventCreate(&e0), "autotune event") ||
+            !mnufft::hip_ok(hipEventCreate(&e1), "autotune event"))
+            return MNUFFT_ERROR_GPU;
+        for (const Knob &k : knobs) {
+            // each knob on top of the best so far
+            Knob cand = keep;
+            if (k.plain_rows != base.plain_rows) cand.plain_rows = k.plain_rows;
+            if (k.side0 != base.side0) cand.side0 = k.side0;
+            if (k.light != base.light) cand.light = k.light;
+            p.plain_rows = cand.plain_rows;
+            p.rows_side0 = cand.side0;
+            p.spreader.light_limit = cand.light;
+            ok = p.set_points(M, d_points, st) && p.execute(active_batch, d_in, d_out, st);
+            ok = ok && mnufft::hip_ok(hipEventRecord(e0, st), "autotune event");
+            for (int r = 0; r < repeats && ok; ++r)
+                ok = p.set_points(M, d_points, st) && p.execute(active_batch, d_in, d_out, st);
+            ok = ok && mnufft::hip_ok(hipEventRecord(e1, st), "autotune event") &&
+                 mnufft::hip_ok(hipEventSynchronize(e1), "autotune wait");
+            float ms = 0.0f;
+            ok = ok && mnufft::hip_ok(hipEventElapsedTime(&ms, e0, e1), "autotune timing");
+            if (!ok) break;
+            ++tried;
+            if (ms < 0.97 * best) {
+                best = ms;
+                keep = cand;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        p.plain_rows = keep.plain_rows;
+        p.rows_side0 = keep.side0;
+        p.spreader.light_limit = keep.light;
+        if (!ok) return MNUFFT_ERROR_GPU;
         // without dense bins their scratch is given back (it cannot be
         // needed before the threshold changes again)
         if (cands[best_i].dense == 0u) p.dense.release();
@@ -585,7 +637,7 @@ int mnufft_autotune(mnufft_plan plan, int64_t active_batch, int64_t M, const voi
             choice[0] = (int64_t)cands[best_i].method;
             choice[1] = (int64_t)cands[best_i].dense;
             choice[2] = (int64_t)(best / repeats * 1e6);  // ns per encode
-            choice[3] = (int64_t)cands.size();
+            choice[3] = tried;
         }
         return MNUFFT_SUCCESS;
     });
@@ -625,6 +677,38 @@ int mnufft_set_spread_tuning(mnufft_plan plan, int64_t light_limit, int64_t min_
     if (light_limit) q.spreader.light_limit = (uint32_t)light_limit;
     if (min_part) q.spreader.min_part = (uint32_t)min_part;
     if (part_budget_bytes) q.spreader.budget_bytes = part_budget_bytes;
+    return MNUFFT_SUCCESS;
+}
+
+int mnufft_set_bin_tuning(mnufft_plan plan, int64_t rows_side0, int32_t plain_rows)
+{
+    if (!plan) {
+        set_error("null plan");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (rows_side0 < 0 || rows_side0 > 64 || (rows_side0 & (rows_side0 - 1)) != 0) {
+        set_error("the axis-0 bin side is 0 (automatic) or a power of two up to 64");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    Plan &p = *plan->plan;
+    Plan &q = p.type == 3 && p.outer ? *p.outer : p;
+    q.rows_side0 = rows_side0;
+    q.plain_rows = plain_rows != 0;
+    return MNUFFT_SUCCESS;
+}
+
+int mnufft_plan_tuning(mnufft_plan plan, int64_t *vals, int32_t n)
+{
+    if (!plan || !vals) {
+        set_error("null argument");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    const Plan &p = *plan->plan;
+    const Plan &q = p.type == 3 && p.outer ? *p.outer : p;
+    const int64_t v[6] = {(int64_t)q.spread_method, (int64_t)q.dense.threshold, q.rows_side0,
+                          q.plain_rows ? 1 : 0, (int64_t)q.spreader.light_limit,
+                          (int64_t)q.spreader.min_part};
+    for (int i = 0; i < n && i < 6; ++i) vals[i] = v[i];
     return MNUFFT_SUCCESS;
 }
 

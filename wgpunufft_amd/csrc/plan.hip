@@ -180,6 +180,7 @@ bool Plan::init_common()
     layout = build_spread_layout(dim, fine, kernel.width,
                                  precision == Precision::F32 ? Precision::F32 : Precision::F64);
     sp = make_spread_params(dim, fine, kernel.width);
+    sp_generic = sp;
     // plain row tiles of 8 x 8 x 8 cells
     sweep = make_sweep_params(dim, fine, kernel.width, 8, &sp);
     tiles2 = make_tile2_params(dim, fine, kernel.width, &tile2_sp);
@@ -232,6 +233,8 @@ GridParams Plan::spread_geometry(int64_t M, int *chosen) const
     for (int a = 1; a < dim; ++a) lines *= fine[a];
     while (side[0] < fine[0] && ((fine[0] + side[0] - 1) / side[0]) * lines > ((int64_t)1 << 24))
         side[0] *= 2;
+    // a tuned side (a power of two): coarser only, never beyond the axis
+    if (rows_side0 > side[0]) side[0] = std::min<int64_t>(rows_side0, pow2_ceil(fine[0]));
     if (chosen) *chosen = -1;
     return grid_params(side);
 }
@@ -437,6 +440,7 @@ bool Plan::set_points_t(int64_t M, const T *pts, hipStream_t stream)
     if (type == 1) {
         gp = spread_geometry(M, &geometry_index);
         gp_rows = rows_geometry();
+        binned_side0 = rows_side0;
         binned_order = point_order;
         binned_dense = sizeof(T) == 4 ? dense_threshold(gp) : 0u;
         mark(kStageSort, false, stream);
@@ -472,6 +476,7 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         // a spread method that needs other bins, or another in-bin order,
         // re-bins the kept points
         if ((gp_rows != rows_geometry() || binned_order != point_order ||
+             binned_side0 != rows_side0 ||
              binned_dense != (sizeof(T) == 4 ? dense_threshold(gp) : 0u)) &&
             !set_points_t<D, T>(M, (const T *)points, stream))
             return false;

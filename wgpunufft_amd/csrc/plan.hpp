@@ -42,11 +42,19 @@ struct Plan {
     int64_t phihat_off[kMaxDims] = {0, 0, 0};
     double *d_inv_phihat = nullptr;
     SpreadLayout layout;
-    SpreadParams sp{};
+    SpreadParams sp{};         // plain spread: the row tiles where `sweep`, else sp_generic
+    SpreadParams sp_generic{};  // output-stationary blocks of the reference layout
     bool sweep = false;  // 3D grid qualifies for the row tiles (plain and matrix-core)
+    // Tuning the autotuner sets (and mnufft_set_bin_tuning): the plain 3D
+    // spread on row tiles (else the output-stationary blocks), and the side
+    // of the row geometry's bins along axis 0 (0: the smallest power of two
+    // from 2 keeping the bin count within 2^24).
+    bool plain_rows = true;
+    int64_t rows_side0 = 0;
+    int64_t binned_side0 = 0;  // axis-0 bin side of the current binning
     SpreadParams tile2_sp{};
     bool tiles2 = false;  // 2D grid qualifies for the matrix-core 32 x 32 tiles
-    SpreadParams tile3_sp{};  // 3D matrix-core tiles (16 x 16 x 8; row-tile grids)
+    SpreadParams tile3_sp{};  // 3D matrix-core tiles (16 x 16 x 8; row-sweep grids)
     SpreadMethod spread_method = SpreadMethod::Plain;
     // Type-1 order of the points inside a bin: ascending original index (the
     // reference's), or coordinate bits then index (shuffle invariant).
@@ -65,15 +73,16 @@ struct Plan {
             return 0;
         return dense.threshold;
     }
-    // Type-1 spreads on row tiles: the 3D plain or matrix-core tiles, or a 2D
-    // matrix-core method.
+    // Type-1 spreads on row tiles: the 3D plain row tiles or matrix-core
+    // tiles, or a 2D matrix-core method.
     bool rows_geometry() const
     {
-        return sweep || (tiles2 && spread_method != SpreadMethod::Plain);
+        if (spread_method == SpreadMethod::Plain) return sweep && plain_rows;
+        return sweep || tiles2;
     }
     const SpreadParams &spread_params() const
     {
-        if (spread_method == SpreadMethod::Plain) return sp;
+        if (spread_method == SpreadMethod::Plain) return sweep && plain_rows ? sp : sp_generic;
         return dim == 2 ? tile2_sp : tile3_sp;
     }
 
