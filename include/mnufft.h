@@ -58,18 +58,20 @@ typedef struct mnufft_plan_s *mnufft_plan;
 /*
  * A type-1 or type-2 plan for `batch` vectors over one point set.
  * Replaces NufftConfig (src/config.rs:64-230) with NufftPlan::type1_gpu /
- * NufftPlan::type2_gpu (src/plan.rs:457-463, 606-651): n_modes per axis
- * (axis 0 first), isign >= 0 positive, eps the tolerance, precision one of
- * MNUFFT_PRECISION_*, mode_order 0 centered / 1 FFT, sigma the upsampling
- * factor (0: 2).
+ * NufftPlan::type2_gpu (src/plan.rs:457-463, 606-651): dim 1..8 axes
+ * (MAX_GPU_NUFFT_DIMENSIONS, src/config.rs:7-12; ranks 4-8 take the
+ * rank-generic spread, interpolation and an FFT composed of rocFFT
+ * transforms over axis groups), n_modes per axis (axis 0 first), isign >= 0
+ * positive, eps the tolerance, precision one of MNUFFT_PRECISION_*,
+ * mode_order 0 centered / 1 FFT, sigma the upsampling factor (0: 2).
  */
 int mnufft_makeplan(int32_t type, int32_t dim, const int64_t *n_modes, int32_t isign,
                     int64_t batch, double eps, int32_t precision, int32_t mode_order, double sigma,
                     mnufft_plan *plan);
 
 /*
- * A type-3 plan: NufftType3Config + NufftType3Plan::new (src/type3.rs:90-344).
- * source_bounds / target_bounds hold (lower, upper) per axis; every point
+ * A type-3 plan: NufftType3Config + NufftType3Plan::new (src/type3.rs:90-344),
+ * dim 1..8.  source_bounds / target_bounds hold (lower, upper) per axis; every point
  * and target must lie inside them (a caller guarantee, src/type3.rs:346-353).
  */
 int mnufft_makeplan_type3(int32_t dim, int32_t isign, int64_t batch, double eps, int32_t precision,
@@ -178,7 +180,7 @@ int mnufft_bin_count(mnufft_plan plan, int64_t M, int64_t *nbins);
  * set_points (or by an execute that re-binned the kept points): d_sorted
  * receives the M point indices in record-slot order (the order the spread
  * sums them in), d_offsets (may be NULL) the nbins + 1 bin offsets, bin_side
- * (may be NULL, 3 values) the bin side per axis.  For the index point order
+ * (may be NULL, max(3, dim) values) the bin side per axis.  For the index point order
  * this is the reference's stable bin permutation of that geometry (the
  * contract of mnufft_bin_sort).  Reads nothing but the plan's own scratch;
  * MNUFFT_ERROR_INVALID_ARGUMENT when no points are binned or dense bins
@@ -192,7 +194,8 @@ int mnufft_points_permutation(mnufft_plan plan, uint32_t *d_sorted, uint32_t *d_
  * Plan facts, for tests and benchmarks: info[0] kernel width, [1..3] fine
  * lengths, [4..6] bin sides of the geometry last chosen (type 1) or used
  * (type 2), [7] its bin count, [8] geometry index, [9..11] spread block edge,
- * [12] spread run, [13..15] type-3 outer grid lengths.  Writes min(n, 16).
+ * [12] spread run, [13..15] type-3 outer grid lengths (axes 0-2 only for
+ * plans of more axes).  Writes min(n, 16).
  */
 int mnufft_plan_info(mnufft_plan plan, int64_t *info, int32_t n);
 
@@ -333,10 +336,10 @@ int mnufft_direct_f64(int32_t kind, int32_t dim, const int64_t *n_modes, int32_t
  *    (select_fine_grid_size, src/kernel.rs:278-333), -1 if none;
  *  - mnufft_layout_query: the reference spread layout and the bin geometry
  *    it chooses for point_count points (NdSpreadLayout::for_grid and
- *    choose_geometry, src/gpu_nd_spread.rs:194-459).  out (64 entries):
- *    [0] run, [1] workgroup size, [2] row batch, [3] geometry count,
- *    [4] chosen geometry, [5..7] block, [8..10] blocks, [16 + 3g + a] the
- *    bin side of geometry g along axis a.
+ *    choose_geometry, src/gpu_nd_spread.rs:194-459), dim 1..8.  out (128
+ *    entries): [0] run, [1] workgroup size, [2] row batch, [3] geometry
+ *    count, [4] chosen geometry, [8 + a] block, [16 + a] blocks,
+ *    [32 + 8g + a] the bin side of geometry g along axis a.
  */
 int mnufft_kernel_params(double eps, double sigma, int32_t *width, double *beta);
 int64_t mnufft_fine_grid_size(int64_t n_modes, double sigma, int32_t width);

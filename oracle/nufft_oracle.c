@@ -18,7 +18,7 @@
 #include <string.h>
 
 
-#define OR_MAX_DIMS 3
+#define OR_MAX_DIMS 8
 #define OR_MAX_GEOMETRIES 16
 
 /* ------------------------------------------------------------------ */
@@ -340,7 +340,8 @@ int or_layout_build(int d, const long *fine, int width, int prec, or_layout *L)
     long mr = max_run_of(d, prec);
     long el0 = EDGE_LIMIT(0);
     long run = pow2_at_most(mr < el0 ? mr : el0);
-    long block[OR_MAX_DIMS] = {1, 1, 1};
+    long block[OR_MAX_DIMS];
+    for (int a = 0; a < OR_MAX_DIMS; ++a) block[a] = 1;
     long inv = 1;
     for (;;) {
         int best = -1;
@@ -370,7 +371,8 @@ int or_layout_build(int d, const long *fine, int width, int prec, or_layout *L)
     L->workgroup_size = wg;
     for (int a = 0; a < d; ++a) L->block[a] = block[a];
 
-    long side[OR_MAX_DIMS] = {1, 1, 1};
+    long side[OR_MAX_DIMS];
+    for (int a = 0; a < OR_MAX_DIMS; ++a) side[a] = 1;
     if (!(d == 2 && prec == 0)) side[0] = 2;
     for (;;) {
         double rows = 1.0;
@@ -500,22 +502,24 @@ int or_choose_geometry(const or_layout *L, long point_count)
 
 /* Flattened accessors for ctypes. */
 int or_layout_query(int d, const long *fine, int width, int prec, long point_count,
-                    long *out /* >= 64 longs */)
+                    long *out /* >= 128 longs */)
 {
     or_layout L;
+    if (d < 1 || d > OR_MAX_DIMS) return -1;
     if (or_layout_build(d, fine, width, prec, &L)) return -1;
     int g = or_choose_geometry(&L, point_count);
+    for (int i = 0; i < 128; ++i) out[i] = 0;
     out[0] = L.run;
     out[1] = L.workgroup_size;
     out[2] = L.row_batch;
     out[3] = L.ngeom;
     out[4] = g;
-    for (int a = 0; a < 3; ++a) {
-        out[5 + a] = a < d ? L.block[a] : 1;
-        out[8 + a] = a < d ? L.blocks[a] : 1;
+    for (int a = 0; a < OR_MAX_DIMS; ++a) {
+        out[8 + a] = a < d ? L.block[a] : 1;
+        out[16 + a] = a < d ? L.blocks[a] : 1;
     }
-    for (int i = 0; i < L.ngeom && i < 16; ++i)
-        for (int a = 0; a < 3; ++a) out[16 + 3 * i + a] = a < d ? L.geom[i].side[a] : 1;
+    for (int i = 0; i < L.ngeom && i < 12; ++i)
+        for (int a = 0; a < OR_MAX_DIMS; ++a) out[32 + 8 * i + a] = a < d ? L.geom[i].side[a] : 1;
     return 0;
 }
 
@@ -584,8 +588,8 @@ static int in_reach_f32(float x)
 long or_bin_keys_f32(int d, const long *fine, int width, const long *side, long M,
                      const float *coords, uint32_t *keys, int32_t *starts, double *offsets)
 {
-    float hi[3], lo[3];
-    long bins[3];
+    float hi[OR_MAX_DIMS], lo[OR_MAX_DIMS];
+    long bins[OR_MAX_DIMS];
     for (int a = 0; a < d; ++a) {
         scale_split(fine[a], &hi[a], &lo[a]);
         bins[a] = (fine[a] + side[a] - 1) / side[a];

@@ -134,7 +134,7 @@ def layout(fine, width, precision=0, point_count=0):
     1 F64, 2 Df64.  Returns a dict with run, workgroup_size, row_batch,
     block, blocks, geometries (bin sides) and the geometry chosen for
     `point_count` points."""
-    out = (ctypes.c_long * 64)()
+    out = (ctypes.c_long * 128)()
     d = len(fine)
     if lib().or_layout_query(d, _longs(list(fine)), width, precision, point_count, out) != 0:
         raise ValueError("layout failed")
@@ -144,9 +144,9 @@ def layout(fine, width, precision=0, point_count=0):
         "workgroup_size": out[1],
         "row_batch": out[2],
         "chosen": out[4],
-        "block": [out[5 + a] for a in range(d)],
-        "blocks": [out[8 + a] for a in range(d)],
-        "geometries": [[out[16 + 3 * i + a] for a in range(d)] for i in range(ng)],
+        "block": [out[8 + a] for a in range(d)],
+        "blocks": [out[16 + a] for a in range(d)],
+        "geometries": [[out[32 + 8 * i + a] for a in range(d)] for i in range(ng)],
     }
 
 

@@ -73,21 +73,24 @@ def test_product_host_math_equals_oracle():
                 assert L.mnufft_fine_grid_size(modes, sigma, w.value) == O.fine_size(modes, sigma, ow)
     rng = np.random.default_rng(1)
     shapes = [[256, 256, 256], [512, 512, 512], [200, 200, 200], [36, 30, 28], [256, 256],
-              [2048, 2048], [14, 14, 14], [4096], [130, 62]]
+              [2048, 2048], [14, 14, 14], [4096], [130, 62], [64, 64, 64, 64], [32, 28, 24, 20],
+              [16, 16, 16, 16, 16], [20, 18, 16, 14, 12, 12], [12] * 7, [10] * 8]
     for shape in shapes:
         for eps in [1e-3, 1e-6, 1e-10]:
             w, _ = O.es_params(eps)
             for prec in [0, 1]:
                 for M in [0, 1000, 100_000, 10_000_000, int(rng.integers(1, 10**8))]:
                     ref = O.layout(shape, w, prec, M)
-                    out = (ctypes.c_int64 * 64)()
-                    fine = (ctypes.c_int64 * 3)(*shape)
+                    out = (ctypes.c_int64 * 128)()
+                    fine = (ctypes.c_int64 * len(shape))(*shape)
                     assert L.mnufft_layout_query(len(shape), fine, w, 3 if prec == 0 else 1, M,
                                                  out) == 0
                     assert out[0] == ref["run"] and out[1] == ref["workgroup_size"]
                     assert out[2] == ref["row_batch"] and out[4] == ref["chosen"]
                     d = len(shape)
-                    got_geoms = [[out[16 + 3 * g + a] for a in range(d)] for g in range(out[3])]
+                    assert [out[8 + a] for a in range(d)] == ref["block"]
+                    assert [out[16 + a] for a in range(d)] == ref["blocks"]
+                    got_geoms = [[out[32 + 8 * g + a] for a in range(d)] for g in range(out[3])]
                     assert got_geoms == ref["geometries"]
 
 

@@ -124,6 +124,31 @@ def test_layout_reference_cases():
     assert O.layout([256, 256], w, 0, 100_000)["geometries"][0] == [1, 1]
 
 
+def test_layout_reference_cases_ranks_4_to_8():
+    # src/gpu_nd_spread.rs:2122-2171, 2210-2220, 2244-2283
+    G = GOLDEN["spread_layouts_nd"]
+    for row in G["layouts"]:
+        w, _ = O.es_params(row["eps"])
+        L = O.layout(row["fine"], w, row["precision"])
+        if "run" in row:
+            assert (L["run"], L["workgroup_size"]) == (row["run"], row["workgroup_size"]), row
+        if "block" in row:
+            assert L["block"] == row["block"], row
+        if "finest_side" in row:
+            assert L["geometries"][0] == row["finest_side"], row
+        if "max_side" in row:
+            assert all(s <= row["max_side"] for g in L["geometries"] for s in g), row
+    for row in G["finest_geometry_for_many_points"]:
+        w, _ = O.es_params(row["eps"])
+        assert O.layout(row["fine"], w, row["precision"], row["points"])["chosen"] == row["chosen"]
+    for shape, eps in G["unique_support_starts"]["shapes"]:
+        w, _ = O.es_params(eps)
+        for prec in G["unique_support_starts"]["precisions"]:
+            L = O.layout(shape, w, prec)
+            assert all(b + w - 1 <= n for b, n in zip(L["block"], shape)), (shape, prec)
+            assert L["workgroup_size"] <= 256
+
+
 def test_ambiguity_counter_covers_the_double_float_window():
     """A coordinate whose folded position minus w/2 is exactly an integer
     (x = 0 on a 256-cell axis with an even width: 128 - 4 = 124) is counted
@@ -211,6 +236,28 @@ def test_direct_sums_against_numpy_closed_form(modeord):
     s = rng.normal(size=(7, 3)) * 3
     t3 = O.direct_type3(x, c, s, isign=1)
     assert rel_l2(t3, np.exp(1j * s @ x.T) @ c) < 1e-13
+
+
+@pytest.mark.parametrize("n_modes", [[3, 2, 2, 3], [2, 3, 2, 2, 2], [2] * 8])
+def test_direct_sums_ranks_4_to_8(n_modes):
+    """Ranks 4-8 (src/direct.rs:498-607; axis 0 fastest as
+    tests/cpu_foundations.rs:402-413 states for 2D): the closed form per
+    mode, and type 2 as the adjoint of type 1."""
+    rng = np.random.default_rng(len(n_modes))
+    d = len(n_modes)
+    x = rng.uniform(-3 * np.pi, 3 * np.pi, size=(23, d))
+    c = rng.normal(size=23) + 1j * rng.normal(size=23)
+    for modeord in (0, 1):
+        ref = _numpy_direct_type1(x, c, n_modes, 1, modeord)
+        got = O.direct_type1(x, c, n_modes, isign=1, modeord=modeord)
+        assert rel_l2(got, ref) < 1e-13
+        f = rng.normal(size=ref.size) + 1j * rng.normal(size=ref.size)
+        # type 2 of the other sign is the adjoint: <f, A c> = <A^H f, c>
+        t2 = O.direct_type2(x, f, n_modes, isign=-1, modeord=modeord)
+        assert abs(np.vdot(f, got) - np.vdot(t2, c)) < 1e-10 * abs(np.vdot(f, got))
+        # and the sign flips under conjugation
+        assert rel_l2(np.conj(t2), O.direct_type2(x, np.conj(f), n_modes, isign=1,
+                                                  modeord=modeord)) < 1e-13
 
 
 def test_type3_axis_metadata():

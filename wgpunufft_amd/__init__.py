@@ -325,7 +325,7 @@ class NufftPlan(_PlanBase):
             raise NufftError(1, "points must be set first")
         M = self._points.shape[0]
         nb = ctypes.c_int64()
-        side = (ctypes.c_int64 * 3)()
+        side = (ctypes.c_int64 * 8)()
         check(lib().mnufft_points_permutation(self._handle, None, None, ctypes.byref(nb), side,
                                               _stream_ptr()))
         sorted_idx = torch.empty(max(M, 1), dtype=torch.int32, device="cuda")
@@ -354,10 +354,10 @@ class NufftType3Plan(_PlanBase):
         """Target boxes per axis in use; min_boxes (one value per axis)
         re-plans with at least that many.  Target intervals whose inner grid
         would not fit are cut into boxes automatically."""
-        out = (ctypes.c_int32 * 3)()
+        out = (ctypes.c_int32 * 8)()
         arg = None
         if min_boxes is not None:
-            arg = (ctypes.c_int32 * 3)(*(list(min_boxes) + [1, 1, 1])[:3])
+            arg = (ctypes.c_int32 * 8)(*(list(min_boxes) + [1] * 8)[:8])
         check(lib().mnufft_type3_boxes(self._handle, arg, out))
         return list(out[:self.dim])
 
@@ -410,7 +410,7 @@ def encode_type2_many(plans, points, values, outs=None):
 def _direct(kind, dim, n_modes, isign, mode_order, x, values, nout, sel, s):
     torch = _torch()
     out = torch.empty(nout, dtype=torch.complex128, device="cuda")
-    nm = (ctypes.c_int64 * 3)(*(list(n_modes or []) + [1, 1, 1])[:3])
+    nm = (ctypes.c_int64 * 8)(*(list(n_modes or []) + [1] * 8)[:8])
     M = x.shape[0] if x is not None else 0
     check(lib().mnufft_direct_f64(kind, dim, nm, isign, mode_order, M, _ptr(x), _ptr(values),
                                   nout, _ptr(sel), _ptr(s), _ptr(out), _stream_ptr()))

@@ -842,6 +842,35 @@ __device__ inline int64_t bin_of_starts(const int *st, const GridParams &g)
     return b;
 }
 
+// Record s of rank D (layout of rec_quads<D>): starts and offsets of every
+// axis; ranks 1-3 keep the point index in the fourth start word.
+template <int D, typename T>
+__device__ inline void store_record(int4 *__restrict__ rec_start, T *__restrict__ rec_off,
+                                    int64_t s, const int *st, const T *off, uint32_t idx)
+{
+    if constexpr (D <= 3) {
+        rec_start[s] = make_int4(st[0], st[1], st[2], (int)idx);
+        if constexpr (sizeof(T) == 4) {
+            *(float4 *)(rec_off + s * 4) = make_float4(off[0], off[1], off[2], off[3]);
+        } else {
+            *(double2 *)(rec_off + s * 4) = make_double2(off[0], off[1]);
+            *(double2 *)(rec_off + s * 4 + 2) = make_double2(off[2], off[3]);
+        }
+    } else {
+        rec_start[2 * s] = make_int4(st[0], st[1], st[2], st[3]);
+        rec_start[2 * s + 1] = make_int4(st[4], st[5], st[6], st[7]);
+        T *o = rec_off + s * 8;
+        if constexpr (sizeof(T) == 4) {
+            *(float4 *)o = make_float4(off[0], off[1], off[2], off[3]);
+            *(float4 *)(o + 4) = make_float4(off[4], off[5], off[6], off[7]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                *(double2 *)(o + 2 * q) = make_double2(off[2 * q], off[2 * q + 1]);
+        }
+    }
+}
+
 // Grouped and index orders: slot s keeps the point the radix sort put there.
 // The coordinates are gathered once (all loads in flight before any store)
 // and each record is written with two 16-byte stores.
@@ -864,8 +893,8 @@ __global__ __launch_bounds__(256) void k_prepare_grouped(int64_t M,
     bool reach = true;
 #pragma unroll
     for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
-    int st[3] = {0, 0, 0};
-    T off[4] = {(T)0, (T)0, (T)0, (T)0};
+    int st[kMaxDims] = {};
+    T off[kMaxDims] = {};
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         int64_t start;
@@ -878,13 +907,7 @@ __global__ __launch_bounds__(256) void k_prepare_grouped(int64_t M,
         const int64_t sorted_slot = s;
         s = dense_slot(sorted_slot, bin_of_starts<D>(st, g), M, g.nbins, offsets, dense_before);
     }
-    rec_start[s] = make_int4(st[0], st[1], st[2], (int)idx);
-    if constexpr (sizeof(T) == 4) {
-        *(float4 *)(rec_off + s * 4) = make_float4(off[0], off[1], off[2], off[3]);
-    } else {
-        *(double2 *)(rec_off + s * 4) = make_double2(off[0], off[1]);
-        *(double2 *)(rec_off + s * 4 + 2) = make_double2(off[2], off[3]);
-    }
+    store_record<D, T>(rec_start, rec_off, s, st, off, idx);
     if (out_sorted) out_sorted[s] = idx;
 }
 
@@ -916,7 +939,7 @@ __global__ __launch_bounds__(256) void k_prepare(int64_t M, const uint32_t *__re
     while (hi < M && hi - s <= (int64_t)kSmallBin && skeys[hi] == key) ++hi;
     const bool large = hi - lo > (int64_t)kSmallBin;
     uint32_t idx;
-    T x[3];
+    T x[D];
     int64_t dst = s;
     if (large) {
         const SortElem e = large_elems[s];
@@ -933,7 +956,7 @@ __global__ __launch_bounds__(256) void k_prepare(int64_t M, const uint32_t *__re
             for (int64_t m = lo; m < hi; ++m) {
                 if (m == s) continue;
                 const uint32_t im = sorted[m];
-                T y[3];
+                T y[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) y[a] = coords[(int64_t)im * D + a];
                 rank += elem_less(elem_of<D, T>(y, im), me) ? 1 : 0;
@@ -944,8 +967,8 @@ __global__ __launch_bounds__(256) void k_prepare(int64_t M, const uint32_t *__re
     bool reach = true;
 #pragma unroll
     for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
-    int st[3] = {0, 0, 0};
-    T off[4] = {(T)0, (T)0, (T)0, (T)0};
+    int st[kMaxDims] = {};
+    T off[kMaxDims] = {};
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         int64_t start;
@@ -956,13 +979,7 @@ __global__ __launch_bounds__(256) void k_prepare(int64_t M, const uint32_t *__re
     }
     if (dense_before)
         dst = dense_slot(dst, bin_of_starts<D>(st, g), M, g.nbins, offsets, dense_before);
-    rec_start[dst] = make_int4(st[0], st[1], st[2], (int)idx);
-    if constexpr (sizeof(T) == 4) {
-        *(float4 *)(rec_off + dst * 4) = make_float4(off[0], off[1], off[2], off[3]);
-    } else {
-        *(double2 *)(rec_off + dst * 4) = make_double2(off[0], off[1]);
-        *(double2 *)(rec_off + dst * 4 + 2) = make_double2(off[2], off[3]);
-    }
+    store_record<D, T>(rec_start, rec_off, dst, st, off, idx);
     if (out_sorted) out_sorted[dst] = idx;
 }
 
@@ -1394,6 +1411,16 @@ MNUFFT_BINS_INSTANTIATE(3, float)
 MNUFFT_BINS_INSTANTIATE(1, double)
 MNUFFT_BINS_INSTANTIATE(2, double)
 MNUFFT_BINS_INSTANTIATE(3, double)
+MNUFFT_BINS_INSTANTIATE(4, float)
+MNUFFT_BINS_INSTANTIATE(5, float)
+MNUFFT_BINS_INSTANTIATE(6, float)
+MNUFFT_BINS_INSTANTIATE(7, float)
+MNUFFT_BINS_INSTANTIATE(8, float)
+MNUFFT_BINS_INSTANTIATE(4, double)
+MNUFFT_BINS_INSTANTIATE(5, double)
+MNUFFT_BINS_INSTANTIATE(6, double)
+MNUFFT_BINS_INSTANTIATE(7, double)
+MNUFFT_BINS_INSTANTIATE(8, double)
 template bool gather_strengths<float>(int64_t, int64_t, const uint32_t *, const float *,
                                       float *, hipStream_t, const uint32_t *);
 template bool gather_strengths<double>(int64_t, int64_t, const uint32_t *, const double *,

@@ -63,8 +63,8 @@ int make_plan(int type, int dim, const int64_t *n_modes, int isign, int64_t batc
         set_error("plan type must be 1 or 2 here (type 3 has its own constructor)");
         return MNUFFT_ERROR_INVALID_ARGUMENT;
     }
-    if (dim < 1 || dim > 3) {
-        set_error("GPU plans take 1 to 3 dimensions, got " + std::to_string(dim));
+    if (dim < 1 || dim > mnufft::kMaxDims) {
+        set_error("GPU plans take 1 to 8 dimensions, got " + std::to_string(dim));
         return MNUFFT_ERROR_PLAN;
     }
     if (!n_modes) {
@@ -148,8 +148,8 @@ int mnufft_makeplan_type3(int32_t dim, int32_t isign, int64_t batch, double eps,
             return MNUFFT_ERROR_INVALID_ARGUMENT;
         }
         *plan = nullptr;
-        if (dim < 1 || dim > 3) {
-            set_error("GPU plans take 1 to 3 dimensions");
+        if (dim < 1 || dim > mnufft::kMaxDims) {
+            set_error("GPU plans take 1 to 8 dimensions, got " + std::to_string(dim));
             return MNUFFT_ERROR_PLAN;
         }
         if (batch < 1) {
@@ -184,7 +184,7 @@ int mnufft_type3_boxes(mnufft_plan plan, const int32_t *min_boxes, int32_t *boxe
         }
         Plan &p = *plan->plan;
         if (min_boxes) {
-            int k[3] = {1, 1, 1};
+            int k[mnufft::kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
             for (int a = 0; a < p.dim; ++a) {
                 if (min_boxes[a] < 1 || min_boxes[a] > mnufft::kType3MaxBoxes) {
                     set_error("boxes per axis must be 1.." +
@@ -194,7 +194,7 @@ int mnufft_type3_boxes(mnufft_plan plan, const int32_t *min_boxes, int32_t *boxe
                 k[a] = min_boxes[a];
             }
             (void)hipDeviceSynchronize();  // the plan's kernels may still run
-            double sb[6], tb[6];
+            double sb[2 * mnufft::kMaxDims], tb[2 * mnufft::kMaxDims];
             for (int a = 0; a < 2 * p.dim; ++a) {
                 sb[a] = p.t3_bounds[a];
                 tb[a] = p.t3_bounds[2 * mnufft::kMaxDims + a];
@@ -405,7 +405,7 @@ int mnufft_points_permutation(mnufft_plan plan, uint32_t *d_sorted, uint32_t *d_
         }
         if (nbins) *nbins = p.gp.nbins;
         if (bin_side)
-            for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < std::max(3, p.dim); ++a)
                 bin_side[a] = a < p.dim ? ((int64_t)1 << p.gp.ax[a].shift) : 1;
         hipStream_t st = (hipStream_t)stream;
         if (p.point_count > 0 && d_sorted &&
@@ -730,13 +730,14 @@ int mnufft_direct_f64(int32_t kind, int32_t dim, const int64_t *n_modes, int32_t
                       void *stream)
 {
     return guarded([&] {
-        if (kind < 1 || kind > 3 || dim < 1 || dim > 3 || !d_out || (kind != 3 && !n_modes)) {
+        if (kind < 1 || kind > 3 || dim < 1 || dim > mnufft::kMaxDims || !d_out ||
+            (kind != 3 && !n_modes)) {
             set_error("invalid direct-sum arguments");
             return MNUFFT_ERROR_INVALID_ARGUMENT;
         }
         mnufft::DirectParams dp{};
         dp.mode_total = 1;
-        for (int a = 0; a < 3; ++a) {
+        for (int a = 0; a < mnufft::kMaxDims; ++a) {
             dp.n_modes[a] = (kind != 3 && a < dim) ? n_modes[a] : 1;
             dp.mode_total *= dp.n_modes[a];
         }
@@ -809,24 +810,25 @@ int mnufft_layout_query(int32_t dim, const int64_t *fine, int32_t width, int32_t
                         int64_t point_count, int64_t *out)
 {
     return guarded([&] {
-        if (dim < 1 || dim > 3 || !fine || !out) {
+        if (dim < 1 || dim > mnufft::kMaxDims || !fine || !out) {
             set_error("invalid layout query");
             return MNUFFT_ERROR_INVALID_ARGUMENT;
         }
         mnufft::SpreadLayout L = mnufft::build_spread_layout(
             dim, fine, width, precision == MNUFFT_PRECISION_F32 ? Precision::F32 : Precision::F64);
-        for (int i = 0; i < 64; ++i) out[i] = 0;
+        for (int i = 0; i < 128; ++i) out[i] = 0;
         out[0] = L.run;
         out[1] = L.workgroup_size;
         out[2] = L.row_batch;
         out[3] = (int64_t)L.geometries.size();
         out[4] = L.choose_geometry(point_count);
-        for (int a = 0; a < 3; ++a) {
-            out[5 + a] = a < dim ? L.block[a] : 1;
-            out[8 + a] = a < dim ? L.blocks[a] : 1;
+        for (int a = 0; a < mnufft::kMaxDims; ++a) {
+            out[8 + a] = a < dim ? L.block[a] : 1;
+            out[16 + a] = a < dim ? L.blocks[a] : 1;
         }
-        for (size_t g = 0; g < L.geometries.size() && g < 16; ++g)
-            for (int a = 0; a < 3; ++a) out[16 + 3 * g + a] = a < dim ? L.geometries[g].side[a] : 1;
+        for (size_t g = 0; g < L.geometries.size() && g < 12; ++g)
+            for (int a = 0; a < mnufft::kMaxDims; ++a)
+                out[32 + 8 * g + a] = a < dim ? L.geometries[g].side[a] : 1;
         return MNUFFT_SUCCESS;
     });
 }
@@ -841,7 +843,7 @@ namespace {
 struct HostPlan {
     int type = 1;
     int dim = 1;
-    int64_t n_modes[3] = {1, 1, 1};
+    int64_t n_modes[mnufft::kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     int isign = 1;
     int64_t ntrans = 1;
     double eps = 1e-6;
@@ -903,8 +905,8 @@ int hp_makeplan(bool float_api, int32_t type, int32_t dim, const int64_t *n_mode
         set_error("type must be 1, 2 or 3");
         return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
     }
-    if (dim < 1 || dim > 3) {
-        set_error("dim must be 1, 2 or 3");
+    if (dim < 1 || dim > mnufft::kMaxDims) {
+        set_error("dim must be between 1 and 8, not " + std::to_string(dim));
         return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
     }
     if (ntrans < 1) {
@@ -1010,13 +1012,18 @@ int hp_setpts(HostPlan *hp, int64_t M, const S *const *axes, const S *nd, int64_
         (void)(mnufft::point_count_ok(M, "point") && mnufft::point_count_ok(N, "target"));
         return WGPU_NUFFT_ERROR_PLAN;
     }
+    if (!nd && hp->dim > 3) {
+        // (capi/src/lib.rs:190-194)
+        set_error("plans of more than three dimensions take their points through setpts_nd");
+        return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
+    }
     if (M < 0 || (M > 0 && !nd && !axes[0]) || (hp->dim > 1 && M > 0 && !nd && !axes[1]) ||
         (hp->dim > 2 && M > 0 && !nd && !axes[2])) {
         set_error("point count or coordinate arrays invalid");
         return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
     }
     std::vector<unsigned char> pts;
-    double slo[3], shi[3];
+    double slo[mnufft::kMaxDims], shi[mnufft::kMaxDims];
     if (!pack_points<S>(*hp, M, axes, nd, &pts, hp->type != 3, "point", slo, shi))
         return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
     if (!grow(&hp->d_points, &hp->points_bytes, pts.size())) return WGPU_NUFFT_ERROR_GPU;
@@ -1032,7 +1039,7 @@ int hp_setpts(HostPlan *hp, int64_t M, const S *const *axes, const S *nd, int64_
             return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
         }
         std::vector<unsigned char> tg;
-        double tlo[3], thi[3];
+        double tlo[mnufft::kMaxDims], thi[mnufft::kMaxDims];
         if (!pack_points<S>(*hp, N, taxes, tnd, &tg, false, "target", tlo, thi))
             return WGPU_NUFFT_ERROR_INVALID_ARGUMENT;
         if (!grow(&hp->d_targets, &hp->targets_bytes, tg.size())) return WGPU_NUFFT_ERROR_GPU;
@@ -1043,7 +1050,7 @@ int hp_setpts(HostPlan *hp, int64_t M, const S *const *axes, const S *nd, int64_
         hp->N = N;
         // bounds from the points the plan computes with (rounded for F32),
         // as the plan's caller guarantee
-        double sb[6], tb[6];
+        double sb[2 * mnufft::kMaxDims], tb[2 * mnufft::kMaxDims];
         for (int a = 0; a < hp->dim; ++a) {
             sb[2 * a] = M > 0 ? slo[a] : 0.0;
             sb[2 * a + 1] = M > 0 ? shi[a] : 0.0;

@@ -255,7 +255,8 @@ int64_t max_run(int d, Precision p)
 {
     if (d == 1 || (d == 2 && p == Precision::Df64)) return 2;
     if (d == 2) return 4;
-    return 8;
+    if (d == 3 || p != Precision::F32) return 8;
+    return 16;
 }
 
 }  // namespace
@@ -269,7 +270,7 @@ SpreadLayout build_spread_layout(int d, const int64_t *fine, int width, Precisio
     int64_t target = std::min<int64_t>(target_workgroup_size(d, precision), 256);
     auto edge_limit = [&](int a) { return std::max<int64_t>(fine[a] + 1 - width, 1); };
     int64_t run = pow2_at_most(std::min(max_run(d, precision), edge_limit(0)));
-    int64_t block[kMaxDims] = {1, 1, 1};
+    int64_t block[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     int64_t inv = 1;
     for (;;) {
         int best = -1;
@@ -324,7 +325,7 @@ SpreadLayout build_spread_layout(int d, const int64_t *fine, int width, Precisio
         return g;
     };
 
-    int64_t side[kMaxDims] = {1, 1, 1};
+    int64_t side[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     if (!(d == 2 && precision == Precision::F32)) side[0] = 2;
     for (;;) {
         double rows = 1.0;
@@ -368,7 +369,7 @@ SpreadLayout build_spread_layout(int d, const int64_t *fine, int width, Precisio
     L.geometries.push_back(finest);
     for (int others = 1; others <= 3; ++others) {
         for (int first = 0; first <= others; ++first) {
-            int64_t s[kMaxDims] = {1, 1, 1};
+            int64_t s[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
             for (int a = 0; a < d; ++a)
                 s[a] = std::min<int64_t>(side[a] << (a == 0 ? first : others), next_pow2(fine[a]));
             bool dup = false;

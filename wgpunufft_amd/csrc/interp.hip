@@ -43,6 +43,32 @@ __device__ inline int64_t wrap_idx(int64_t i, int64_t n)
     return i;
 }
 
+// Ranks 4-8: the weighted sum over the support of axes 1..A of one lane's
+// axis-0 cell (grid offset `base` so far), axis A outermost; each level sums
+// its w inner sums in order, weighted by the axis-A weight of lane d of the
+// point's group.
+template <int A, typename T, int G>
+__device__ inline void nest_sum(const T *__restrict__ gv, int64_t base, const int64_t *st,
+                                const T *wt, const int64_t *stride, const GridParams &g, int w,
+                                T &re, T &im)
+{
+    re = (T)0;
+    im = (T)0;
+    for (int d = 0; d < w; ++d) {
+        const T wa = __shfl(wt[A], d, G);
+        const int64_t off = base + stride[A] * wrap_idx(st[A] + d, g.ax[A].n);
+        T pr, pi;
+        if constexpr (A == 1) {
+            pr = gv[2 * off];
+            pi = gv[2 * off + 1];
+        } else {
+            nest_sum<A - 1, T, G>(gv, off, st, wt, stride, g, w, pr, pi);
+        }
+        re += wa * pr;
+        im += wa * pi;
+    }
+}
+
 // The G lanes of a point group: sorted slot s0 (valid below end) holds
 // point sorted[s0] - base of `coords`, written to out[2 (vo + point)].
 template <int D, typename T, int G>
@@ -63,8 +89,8 @@ __device__ inline void interp_group(int64_t s0, int64_t end, uint32_t base,
     bool reach = has;
 #pragma unroll
     for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
-    int64_t st[3] = {0, 0, 0};
-    T wt[3] = {(T)0, (T)0, (T)0};
+    int64_t st[D];
+    T wt[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         int64_t start = 0;
@@ -88,6 +114,12 @@ __device__ inline void interp_group(int64_t s0, int64_t end, uint32_t base,
                 are += wy * p[0];
                 aim += wy * p[1];
             }
+        } else if constexpr (D >= 4) {
+            int64_t stride[D];
+            stride[0] = 1;
+#pragma unroll
+            for (int a = 1; a < D; ++a) stride[a] = stride[a - 1] * g.ax[a - 1].n;
+            nest_sum<D - 1, T, G>(gv, xc, st, wt, stride, g, w, are, aim);
         } else {
             int64_t n0 = g.ax[0].n, n01 = n0 * g.ax[1].n;
             int64_t ys[kMaxWidth];
@@ -705,5 +737,15 @@ MNUFFT_INTERP_INSTANTIATE(3, float)
 MNUFFT_INTERP_INSTANTIATE(1, double)
 MNUFFT_INTERP_INSTANTIATE(2, double)
 MNUFFT_INTERP_INSTANTIATE(3, double)
+MNUFFT_INTERP_INSTANTIATE(4, float)
+MNUFFT_INTERP_INSTANTIATE(5, float)
+MNUFFT_INTERP_INSTANTIATE(6, float)
+MNUFFT_INTERP_INSTANTIATE(7, float)
+MNUFFT_INTERP_INSTANTIATE(8, float)
+MNUFFT_INTERP_INSTANTIATE(4, double)
+MNUFFT_INTERP_INSTANTIATE(5, double)
+MNUFFT_INTERP_INSTANTIATE(6, double)
+MNUFFT_INTERP_INSTANTIATE(7, double)
+MNUFFT_INTERP_INSTANTIATE(8, double)
 
 }  // namespace mnufft

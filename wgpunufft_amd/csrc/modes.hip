@@ -364,6 +364,36 @@ __global__ __launch_bounds__(256) void k_type3_box_scatter(int64_t N, const T *_
 }
 
 // ---------------------------------------------------------------------
+// Complex matrix transpose through LDS: 32 x 32 tiles, read along rows and
+// written along the columns of the output, each workgroup taking tiles with
+// a grid stride; blockIdx.y = vector.
+// ---------------------------------------------------------------------
+template <typename V2>
+__global__ __launch_bounds__(256) void k_transpose(const V2 *__restrict__ in, V2 *__restrict__ out,
+                                                   int64_t rows, int64_t cols, int64_t tiles_c,
+                                                   int64_t tiles)
+{
+    __shared__ V2 t[32][33];
+    const int64_t off = (int64_t)blockIdx.y * rows * cols;
+    const V2 *src = in + off;
+    V2 *dst = out + off;
+    const int x = threadIdx.x & 31, y = threadIdx.x >> 5;
+    for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int64_t r0 = (tile / tiles_c) * 32, c0 = (tile % tiles_c) * 32;
+        for (int j = y; j < 32; j += 8) {
+            const int64_t r = r0 + j, c = c0 + x;
+            if (r < rows && c < cols) t[j][x] = src[r * cols + c];
+        }
+        __syncthreads();
+        for (int j = y; j < 32; j += 8) {
+            const int64_t c = c0 + j, r = r0 + x;
+            if (r < rows && c < cols) dst[c * rows + r] = t[x][j];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------
 // FP64 direct sums (reference src/direct.rs:9-131, src/type3.rs:609-673):
 // one lane per output, the inputs streamed through LDS in tiles.
 // ---------------------------------------------------------------------
@@ -375,14 +405,14 @@ __global__ __launch_bounds__(256) void k_direct(int kind, DirectParams dp,
                                                 const double *__restrict__ s,
                                                 double *__restrict__ out)
 {
-    __shared__ double tx[256 * 3];
+    __shared__ double tx[256 * D];
     __shared__ double tc[256 * 2];
     int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool has = o < dp.nout;
     int64_t target = has ? (sel ? sel[o] : o) : 0;
     // what this lane holds fixed: a mode (type 1), a point (type 2) or a
     // target frequency (type 3)
-    double fixed[3] = {0.0, 0.0, 0.0};
+    double fixed[D];
     if (kind == 1) {
         int64_t rest = target;
         for (int a = 0; a < D; ++a) {
@@ -407,11 +437,11 @@ __global__ __launch_bounds__(256) void k_direct(int kind, DirectParams dp,
                     int64_t N = dp.n_modes[a];
                     int64_t ka = rest % N;
                     rest /= N;
-                    tx[threadIdx.x * 3 + a] =
+                    tx[threadIdx.x * D + a] =
                         (double)(dp.centered ? ka - N / 2 : (ka < (N + 1) / 2 ? ka : ka - N));
                 }
             } else {
-                for (int a = 0; a < D; ++a) tx[threadIdx.x * 3 + a] = x[i * D + a];
+                for (int a = 0; a < D; ++a) tx[threadIdx.x * D + a] = x[i * D + a];
             }
             tc[2 * threadIdx.x] = in[2 * i];
             tc[2 * threadIdx.x + 1] = in[2 * i + 1];
@@ -420,7 +450,7 @@ __global__ __launch_bounds__(256) void k_direct(int kind, DirectParams dp,
         int64_t m = count - base < 256 ? count - base : 256;
         for (int q = 0; q < m; ++q) {
             double ph = 0.0;
-            for (int a = 0; a < D; ++a) ph += fixed[a] * tx[q * 3 + a];
+            for (int a = 0; a < D; ++a) ph += fixed[a] * tx[q * D + a];
             double sn, cs;
             sincos(dp.sign * ph, &sn, &cs);
             double cr = tc[2 * q], ci = tc[2 * q + 1];
@@ -544,6 +574,28 @@ bool type3_box_scatter(const Type3Boxes &bx, int64_t N, const T *s, const uint32
     return hip_ok(hipGetLastError(), "type-3 box scatter launch");
 }
 
+template <typename T>
+bool transpose_complex(const T *in, T *out, int64_t rows, int64_t cols, int64_t vectors,
+                       hipStream_t stream)
+{
+    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+    if (rows <= 0 || cols <= 0 || vectors <= 0) return true;
+    if (vectors > 65535) {
+        set_error("transpose: at most 65535 vectors");
+        return false;
+    }
+    const int64_t tiles_c = (cols + 31) / 32;
+    const int64_t tiles = ((rows + 31) / 32) * tiles_c;
+    const unsigned gx = (unsigned)std::min<int64_t>(tiles, 16384);
+    k_transpose<V2><<<dim3(gx, (unsigned)vectors), 256, 0, stream>>>(
+        (const V2 *)in, (V2 *)out, rows, cols, tiles_c, tiles);
+    return hip_ok(hipGetLastError(), "transpose launch");
+}
+template bool transpose_complex<float>(const float *, float *, int64_t, int64_t, int64_t,
+                                       hipStream_t);
+template bool transpose_complex<double>(const double *, double *, int64_t, int64_t, int64_t,
+                                        hipStream_t);
+
 bool direct_sum(int kind, int d, const DirectParams &dp, const double *x, const double *in,
                 const int64_t *sel, const double *s, double *out, hipStream_t stream)
 {
@@ -557,7 +609,12 @@ bool direct_sum(int kind, int d, const DirectParams &dp, const double *x, const 
     case 1: k_direct<1><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
     case 2: k_direct<2><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
     case 3: k_direct<3><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
-    default: set_error("direct sums take 1 to 3 dimensions"); return false;
+    case 4: k_direct<4><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
+    case 5: k_direct<5><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
+    case 6: k_direct<6><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
+    case 7: k_direct<7><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
+    case 8: k_direct<8><<<g, 256, 0, stream>>>(kind, dp, x, in, sel, s, out); break;
+    default: set_error("direct sums take 1 to 8 dimensions"); return false;
     }
     return hip_ok(hipGetLastError(), "direct-sum launch");
 }
@@ -582,6 +639,16 @@ MNUFFT_MODES_INSTANTIATE(3, float)
 MNUFFT_MODES_INSTANTIATE(1, double)
 MNUFFT_MODES_INSTANTIATE(2, double)
 MNUFFT_MODES_INSTANTIATE(3, double)
+MNUFFT_MODES_INSTANTIATE(4, float)
+MNUFFT_MODES_INSTANTIATE(5, float)
+MNUFFT_MODES_INSTANTIATE(6, float)
+MNUFFT_MODES_INSTANTIATE(7, float)
+MNUFFT_MODES_INSTANTIATE(8, float)
+MNUFFT_MODES_INSTANTIATE(4, double)
+MNUFFT_MODES_INSTANTIATE(5, double)
+MNUFFT_MODES_INSTANTIATE(6, double)
+MNUFFT_MODES_INSTANTIATE(7, double)
+MNUFFT_MODES_INSTANTIATE(8, double)
 template bool fill_zero<float>(float *, int64_t, hipStream_t);
 template bool fill_zero<double>(double *, int64_t, hipStream_t);
 template bool fill_zero<uint32_t>(uint32_t *, int64_t, hipStream_t);

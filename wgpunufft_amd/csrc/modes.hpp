@@ -75,6 +75,12 @@ template <int D, typename T>
 bool type3_box_scatter(const Type3Boxes &bx, int64_t N, const T *s, const uint32_t *box,
                        const uint32_t *offsets, uint32_t *cursor, uint32_t *perm, T *s_sorted,
                        hipStream_t stream);
+// Per vector v < vectors: out[v][c][r] = in[v][r][c] for a rows x cols
+// matrix of complex values (the rank 4-8 FFT moves the axes it has
+// transformed behind the others with it).
+template <typename T>
+bool transpose_complex(const T *in, T *out, int64_t rows, int64_t cols, int64_t vectors,
+                       hipStream_t stream);
 // kind 1: out[o] = sum_j in[j] e^{i sign k_o.x_j} over the linear mode
 // indices sel (or all); kind 2: out[o] = sum_k in[k] e^{i sign k.x_o} for
 // the points sel; kind 3: out[o] = sum_j in[j] e^{i sign s_o.x_j}.

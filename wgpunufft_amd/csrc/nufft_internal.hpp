@@ -17,8 +17,16 @@
 
 namespace mnufft {
 
-constexpr int kMaxDims = 3;
+constexpr int kMaxDims = 8;
 constexpr int kMaxWidth = 16;
+// Prepared type-1 records of rank D: ranks 1-3 one int4 per record (the
+// wrapped support starts along x, y, z and the original point index) and
+// four offsets; ranks 4-8 two int4 (eight starts) and eight offsets.
+template <int D>
+constexpr int rec_quads()
+{
+    return D <= 3 ? 1 : 2;
+}
 // Point indices, bin offsets, record slots and permutations are 32-bit:
 // a point set (or type-3 target set) holds at most this many points.
 constexpr int64_t kMaxPoints = (int64_t)UINT32_MAX - 1;
@@ -89,10 +97,10 @@ struct BinGeometry {
 struct SpreadLayout {
     int dims = 0;
     int width = 0;
-    int64_t fine[kMaxDims] = {1, 1, 1};
+    int64_t fine[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     int64_t run = 1, groups0 = 1, workgroup_size = 1, row_batch = 1;
-    int64_t block[kMaxDims] = {1, 1, 1};
-    int64_t blocks[kMaxDims] = {1, 1, 1};
+    int64_t block[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
+    int64_t blocks[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     std::vector<BinGeometry> geometries;
     int choose_geometry(int64_t point_count) const;
 };

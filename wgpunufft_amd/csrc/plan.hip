@@ -50,20 +50,41 @@ int log2i(int64_t v)
 
 }  // namespace
 
-SpreadParams make_spread_params(int d, const int64_t *fine, int width)
+SpreadParams make_spread_params(int d, const int64_t *fine, int width, bool f64)
 {
     SpreadParams sp{};
-    // cells per lane along axis 0: 8 in 3D, 4 in 2D, 4 in 1D; never more
-    // than n0 + 1 - w, so a run sees every support offset at most once
-    int run = d == 3 ? 8 : 4;
+    // cells per lane along axis 0: 8 in 3D, 4 in 2D, 4 in 1D, from 4D on 16
+    // (F32) or 8 (F64) as the reference's max_run (src/gpu_nd_spread.rs:
+    // 575-582); never more than n0 + 1 - w, so a run sees every support
+    // offset at most once
+    int run = d == 3 ? 8 : (d >= 4 ? (f64 ? 8 : 16) : 4);
     while (run > 2 && run > fine[0] + 1 - width) run /= 2;
     int64_t lanes_other = 1;
-    int64_t block[kMaxDims] = {1, 1, 1};
+    int64_t block[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     if (d == 3) {
         block[1] = 8;
         block[2] = 8;
     } else if (d == 2) {
         block[1] = 16;
+    } else if (d >= 4) {
+        // one cell per lane along the other axes: their block edges doubled
+        // in turn up to 64 lanes, the smallest edge first (ties: the longer
+        // axis), each at most 16 cells and n + 1 - w, as the reference's
+        // layout does for 64-invocation workgroups (194-260)
+        int64_t inv = 1;
+        for (;;) {
+            int best = -1;
+            for (int a = 1; a < d; ++a) {
+                const int64_t lim = std::min<int64_t>(std::max<int64_t>(fine[a] + 1 - width, 1), 16);
+                if (block[a] * 2 <= lim && inv * 2 <= 64 &&
+                    (best < 0 || block[a] < block[best] ||
+                     (block[a] == block[best] && fine[a] > fine[best])))
+                    best = a;
+            }
+            if (best < 0) break;
+            block[best] *= 2;
+            inv *= 2;
+        }
     }
     for (int a = 1; a < d; ++a) lanes_other *= block[a];
     int groups0 = (int)(64 / lanes_other);
@@ -100,7 +121,7 @@ bool make_sweep_params(int d, const int64_t *fine, int width, int64_t tx, Spread
     // at least tx + width + 1 (the plain row tiles' x reach within the grid)
     if (fine[0] < 16 + width + 1 || fine[1] < 16 + width || fine[2] < 16 + width) return false;
     if (fine[0] < tx + width + 1) return false;
-    const int64_t block[kMaxDims] = {tx, 8, 8};
+    const int64_t block[kMaxDims] = {tx, 8, 8, 1, 1, 1, 1, 1};
     *sp = make_tile_params(d, fine, block);
     return true;
 }
@@ -134,7 +155,7 @@ Plan::~Plan()
 {
     for (void *p : {(void *)d_inv_phihat, grid, (void *)rec_start, rec_off, (void *)rec_perm,
                     c_sorted, t3_xr, t3_cp, t3_sr, t3_factor, t3_interp, t3_box, t3_perm,
-                    t3_s_sorted, t3_counts, (void *)item_off, (void *)batch_table})
+                    t3_s_sorted, t3_counts, (void *)item_off, (void *)batch_table, fft_scratch})
         if (p) (void)hipFree(p);
     for (auto &sl : slots)
         for (auto &e : sl.ev)
@@ -179,14 +200,14 @@ bool Plan::init_common()
         return false;
     layout = build_spread_layout(dim, fine, kernel.width,
                                  precision == Precision::F32 ? Precision::F32 : Precision::F64);
-    sp = make_spread_params(dim, fine, kernel.width);
+    sp = make_spread_params(dim, fine, kernel.width, precision != Precision::F32);
     sp_generic = sp;
     // plain row tiles of 8 x 8 x 8 cells
     sweep = make_sweep_params(dim, fine, kernel.width, 8, &sp);
     tiles2 = make_tile2_params(dim, fine, kernel.width, &tile2_sp);
     if (sweep) {
         // matrix-core tiles: 16 x 16 cells, 8 deep along z
-        const int64_t block[kMaxDims] = {16, 16, 8};
+        const int64_t block[kMaxDims] = {16, 16, 8, 1, 1, 1, 1, 1};
         tile3_sp = make_tile_params(3, fine, block);
     }
     return true;
@@ -228,7 +249,7 @@ GridParams Plan::spread_geometry(int64_t M, int *chosen) const
     if (!rows_geometry()) return type1_geometry(M, chosen);
     // one-cell bins along the other axes; along axis 0 the smallest power of
     // two from 2 that keeps the bin count within 2^24
-    int64_t side[kMaxDims] = {2, 1, 1};
+    int64_t side[kMaxDims] = {2, 1, 1, 1, 1, 1, 1, 1};
     int64_t lines = 1;
     for (int a = 1; a < dim; ++a) lines *= fine[a];
     while (side[0] < fine[0] && ((fine[0] + side[0] - 1) / side[0]) * lines > ((int64_t)1 << 24))
@@ -264,9 +285,9 @@ bool Plan::tiled2_interp() const
 
 GridParams Plan::type2_geometry() const
 {
-    int64_t side[kMaxDims] = {1, 1, 1};
+    int64_t side[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     for (int a = 0; a < dim; ++a) {
-        int64_t s = dim == 3 ? (tiled_interp() ? 8 : 4) : dim == 2 ? 8 : 32;
+        int64_t s = dim >= 4 ? 4 : dim == 3 ? (tiled_interp() ? 8 : 4) : dim == 2 ? 8 : 32;
         side[a] = std::min<int64_t>(s, pow2_ceil(fine[a]));
     }
     return grid_params(side);
@@ -310,8 +331,11 @@ bool Plan::ensure_records(int64_t M)
     rec_off = nullptr;
     rec_perm = nullptr;
     int64_t cap = M + M / 4 + 64;
-    if (!hip_ok(hipMalloc(&rec_start, sizeof(int4) * cap), "prepared starts")) return false;
-    if (!hip_ok(hipMalloc(&rec_off, 4 * real_bytes() * cap), "prepared offsets")) return false;
+    const size_t quads = dim <= 3 ? 1 : 2;  // rec_quads of the plan's rank
+    if (!hip_ok(hipMalloc(&rec_start, sizeof(int4) * quads * cap), "prepared starts"))
+        return false;
+    if (!hip_ok(hipMalloc(&rec_off, 4 * quads * real_bytes() * cap), "prepared offsets"))
+        return false;
     if (!hip_ok(hipMalloc(&rec_perm, sizeof(uint32_t) * cap), "record permutation")) return false;
     rec_cap = cap;
     return true;
@@ -338,21 +362,24 @@ bool Plan::ensure_strengths(int64_t M, int64_t vectors)
     return true;
 }
 
-FftPlan *Plan::fft_for(int64_t vectors)
+namespace {
+
+// An in-place batched C2C plan over `rank` contiguous axes (fastest first)
+// with its execution info and work buffer.
+std::unique_ptr<FftPlan> create_fft(int rank, const int64_t *len, int64_t batch, int isign,
+                                    Precision precision)
 {
-    auto it = ffts.find(vectors);
-    if (it != ffts.end()) return it->second.get();
     if (!rocfft_ready()) return nullptr;
     auto fp = std::make_unique<FftPlan>();
-    size_t lengths[kMaxDims];
-    for (int a = 0; a < dim; ++a) lengths[a] = (size_t)fine[a];  // fastest first
+    size_t lengths[3];
+    for (int a = 0; a < rank; ++a) lengths[a] = (size_t)len[a];
     rocfft_transform_type tt = isign >= 0 ? rocfft_transform_type_complex_inverse
                                           : rocfft_transform_type_complex_forward;
     rocfft_precision pr =
         precision == Precision::F32 ? rocfft_precision_single : rocfft_precision_double;
     rocfft_plan p = nullptr;
-    if (rocfft_plan_create(&p, rocfft_placement_inplace, tt, pr, (size_t)dim, lengths,
-                           (size_t)vectors, nullptr) != rocfft_status_success) {
+    if (rocfft_plan_create(&p, rocfft_placement_inplace, tt, pr, (size_t)rank, lengths,
+                           (size_t)batch, nullptr) != rocfft_status_success) {
         set_error("rocFFT plan creation failed");
         return nullptr;
     }
@@ -366,9 +393,106 @@ FftPlan *Plan::fft_for(int64_t vectors)
         if (!hip_ok(hipMalloc(&fp->work, ws), "rocFFT work buffer")) return nullptr;
         rocfft_execution_info_set_work_buffer(info, fp->work, ws);
     }
+    return fp;
+}
+
+bool run_fft(FftPlan *fp, void *data, hipStream_t stream)
+{
+    rocfft_execution_info info = (rocfft_execution_info)fp->info;
+    rocfft_execution_info_set_stream(info, stream);
+    void *buf[1] = {data};
+    if (rocfft_execute((rocfft_plan)fp->plan, buf, nullptr, info) != rocfft_status_success) {
+        set_error("rocFFT execution failed");
+        return false;
+    }
+    return true;
+}
+
+// Axis groups of the rank 4-8 FFT, fastest axes first: an even number of
+// groups, so the transposes (one per group) leave the result in the grid.
+int fft_axis_groups(int d, int *len)
+{
+    static const int groups[5][4] = {{3, 1}, {3, 2}, {3, 3}, {2, 2, 2, 1}, {2, 2, 2, 2}};
+    const int n = d <= 6 ? 2 : 4;
+    for (int g = 0; g < n; ++g) len[g] = groups[d - 4][g];
+    return n;
+}
+
+}  // namespace
+
+FftPlan *Plan::fft_for(int64_t vectors)
+{
+    auto it = ffts.find(vectors);
+    if (it != ffts.end()) return it->second.get();
+    auto fp = create_fft(dim, fine, vectors, isign, precision);
+    if (!fp) return nullptr;
     FftPlan *raw = fp.get();
     ffts[vectors] = std::move(fp);
     return raw;
+}
+
+FftPlan *Plan::fft_group_plan(int rank, const int64_t *lengths, int64_t batch)
+{
+    std::vector<int64_t> key{rank};
+    for (int a = 0; a < rank; ++a) key.push_back(lengths[a]);
+    key.push_back(batch);
+    auto it = fft_groups.find(key);
+    if (it != fft_groups.end()) return it->second.get();
+    auto fp = create_fft(rank, lengths, batch, isign, precision);
+    if (!fp) return nullptr;
+    FftPlan *raw = fp.get();
+    fft_groups[key] = std::move(fp);
+    return raw;
+}
+
+bool Plan::fft_nd(int64_t vectors, void *data, hipStream_t stream, bool run)
+{
+    const size_t need = (size_t)vectors * (size_t)fine_total * 2 * real_bytes();
+    if (need > fft_scratch_bytes) {
+        if (fft_scratch) (void)hipFree(fft_scratch);
+        fft_scratch = nullptr;
+        fft_scratch_bytes = 0;
+        if (!hip_ok(hipMalloc(&fft_scratch, need), "FFT transpose buffer")) return false;
+        fft_scratch_bytes = need;
+    }
+    int len[4];
+    const int ng = fft_axis_groups(dim, len);
+    int order[kMaxDims];
+    for (int a = 0; a < dim; ++a) order[a] = a;
+    void *cur = data, *other = fft_scratch;
+    const size_t elem = 2 * real_bytes();
+    for (int gi = 0; gi < ng; ++gi) {
+        const int r = len[gi];
+        int64_t lengths[3], lead = 1;
+        for (int a = 0; a < r; ++a) {
+            lengths[a] = fine[order[a]];
+            lead *= lengths[a];
+        }
+        const int64_t rest = fine_total / lead;
+        // the lead axes of every (rest index, vector), at most 2^31 complex
+        // values per rocFFT execution
+        const int64_t total = rest * vectors;
+        const int64_t per = std::max<int64_t>(1, ((int64_t)1 << 31) / lead);
+        for (int64_t b0 = 0; b0 < total; b0 += per) {
+            FftPlan *fp = fft_group_plan(r, lengths, std::min<int64_t>(per, total - b0));
+            if (!fp) return false;
+            if (run && !run_fft(fp, (char *)cur + (size_t)b0 * (size_t)lead * elem, stream))
+                return false;
+        }
+        if (run) {
+            const bool ok = precision == Precision::F32
+                                ? transpose_complex<float>((const float *)cur, (float *)other,
+                                                           rest, lead, vectors, stream)
+                                : transpose_complex<double>((const double *)cur, (double *)other,
+                                                            rest, lead, vectors, stream);
+            if (!ok) return false;
+        }
+        std::swap(cur, other);
+        int rotated[kMaxDims];
+        for (int a = 0; a < dim; ++a) rotated[a] = order[(a + r) % dim];
+        for (int a = 0; a < dim; ++a) order[a] = rotated[a];
+    }
+    return true;
 }
 
 // The rocFFT plans (and work buffers) an FFT of `vectors` vectors uses,
@@ -376,6 +500,7 @@ FftPlan *Plan::fft_for(int64_t vectors)
 // does not change where they land.
 bool Plan::ensure_fft(int64_t vectors)
 {
+    if (dim >= 4) return fft_nd(vectors, nullptr, nullptr, false);
     const int64_t per_exec = std::max<int64_t>(1, ((int64_t)1 << 31) / fine_total);
     for (int64_t v0 = 0; v0 < vectors; v0 += per_exec)
         if (!fft_for(std::min<int64_t>(per_exec, vectors - v0))) return false;
@@ -384,6 +509,7 @@ bool Plan::ensure_fft(int64_t vectors)
 
 bool Plan::fft(int64_t vectors, void *data, hipStream_t stream)
 {
+    if (dim >= 4) return fft_nd(vectors, data, stream, true);
     // Batches of at most 2^31 complex values per rocFFT execution: larger
     // batched executions were seen to corrupt their last transform.
     const int64_t per_exec = std::max<int64_t>(1, ((int64_t)1 << 31) / fine_total);
@@ -391,14 +517,7 @@ bool Plan::fft(int64_t vectors, void *data, hipStream_t stream)
     for (int64_t v0 = 0; v0 < vectors; v0 += per_exec) {
         const int64_t vc = std::min<int64_t>(per_exec, vectors - v0);
         FftPlan *fp = fft_for(vc);
-        if (!fp) return false;
-        rocfft_execution_info info = (rocfft_execution_info)fp->info;
-        rocfft_execution_info_set_stream(info, stream);
-        void *buf[1] = {(char *)data + (size_t)v0 * vbytes};
-        if (rocfft_execute((rocfft_plan)fp->plan, buf, nullptr, info) != rocfft_status_success) {
-            set_error("rocFFT execution failed");
-            return false;
-        }
+        if (!fp || !run_fft(fp, (char *)data + (size_t)v0 * vbytes, stream)) return false;
     }
     return true;
 }
@@ -633,13 +752,13 @@ bool Plan::init_type3(const double *src_bounds, const double *tgt_bounds, const 
     // Target boxes: the whole interval per axis unless the inner fine grid
     // would exceed kType3InnerCells cells or the byte budget; then the axis
     // with the longest inner grid is cut into one more box, until it fits.
-    int k[kMaxDims] = {1, 1, 1};
+    int k[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     for (int a = 0; a < dim; ++a)
         if (min_boxes && min_boxes[a] > 1) k[a] = std::min(min_boxes[a], kType3MaxBoxes);
-    int64_t outer_n[kMaxDims] = {1, 1, 1};
-    double gam[kMaxDims] = {1, 1, 1}, hh[kMaxDims] = {1, 1, 1};
+    int64_t outer_n[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
+    double gam[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1}, hh[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     for (;;) {
-        int64_t fine_len[kMaxDims] = {1, 1, 1};
+        int64_t fine_len[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
         double cells = 1.0;
         for (int a = 0; a < dim; ++a) {
             const double half = k[a] == 1 ? S[a] : (thi[a] - tlo[a]) / (2.0 * k[a]);
@@ -737,7 +856,7 @@ bool Plan::spread_type3_sources(int64_t active, int64_t M, const T *cp, bool set
             // many sources on a small outer grid: point groups in their
             // original order, partial grids added in group order
             if (!outer->ensure_grid(active)) return false;
-            const int64_t side[kMaxDims] = {1, 1, 1};
+            const int64_t side[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
             return small.spread(outer->grid_params(side), M, (const float *)t3_xr,
                                 (const float *)cp, active, (float *)outer->grid, stream);
         }
@@ -863,7 +982,12 @@ bool Plan::execute_type3_t(int64_t active, int64_t M, const T *x, const T *c, in
         case 1: return f32 ? CALL_F(1, float) : CALL_D(1, double);                                \
         case 2: return f32 ? CALL_F(2, float) : CALL_D(2, double);                                \
         case 3: return f32 ? CALL_F(3, float) : CALL_D(3, double);                                \
-        default: set_error("GPU plans take 1 to 3 dimensions"); return false;                     \
+        case 4: return f32 ? CALL_F(4, float) : CALL_D(4, double);                                \
+        case 5: return f32 ? CALL_F(5, float) : CALL_D(5, double);                                \
+        case 6: return f32 ? CALL_F(6, float) : CALL_D(6, double);                                \
+        case 7: return f32 ? CALL_F(7, float) : CALL_D(7, double);                                \
+        case 8: return f32 ? CALL_F(8, float) : CALL_D(8, double);                                \
+        default: set_error("GPU plans take 1 to 8 dimensions"); return false;                     \
         }                                                                                         \
     } while (0)
 
@@ -932,12 +1056,13 @@ namespace {
 
 // The problem table is written by kernels whose arguments carry up to
 // kTableChunk problems (kernel arguments are copied at launch, so the
-// sequence can be captured in a graph).
-constexpr int kTableChunk = 12;
+// sequence can be captured in a graph), within 4 KiB of arguments.
+constexpr int kTableChunk = (int)((4096 - 64) / sizeof(BatchProblem));
 struct TableChunk {
     BatchProblem p[kTableChunk];
     int n;
 };
+static_assert(kTableChunk >= 1 && sizeof(TableChunk) <= 4096, "table chunk exceeds 4 KiB");
 
 __global__ void k_batch_table(TableChunk chunk, BatchProblem *dst)
 {

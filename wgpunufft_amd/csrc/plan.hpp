@@ -24,7 +24,7 @@ struct Plan {
     // configuration (NufftConfig, src/config.rs:64-74)
     int type = 1;
     int dim = 1;
-    int64_t n_modes[kMaxDims] = {1, 1, 1};
+    int64_t n_modes[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     int isign = 1;
     int64_t batch = 1;
     double eps = 1e-6;
@@ -37,9 +37,9 @@ struct Plan {
 
     // planned quantities
     EsKernel kernel;
-    int64_t fine[kMaxDims] = {1, 1, 1};
+    int64_t fine[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     int64_t fine_total = 1, mode_total = 1;
-    int64_t phihat_off[kMaxDims] = {0, 0, 0};
+    int64_t phihat_off[kMaxDims] = {};
     double *d_inv_phihat = nullptr;
     SpreadLayout layout;
     SpreadParams sp{};         // plain spread: the row tiles where `sweep`, else sp_generic
@@ -107,10 +107,17 @@ struct Plan {
     uint32_t *item_off = nullptr;  // work items of the tiled interpolation
     int64_t item_cap = 0;
     std::map<int64_t, std::unique_ptr<FftPlan>> ffts;
+    // Ranks 4-8 (rocFFT plans take at most three axes): batched FFTs over
+    // groups of the fastest axes, each followed by a transpose that moves
+    // the transformed axes behind the others; plans keyed by {group rank,
+    // lengths, batch}, and the transpose's second buffer.
+    std::map<std::vector<int64_t>, std::unique_ptr<FftPlan>> fft_groups;
+    void *fft_scratch = nullptr;
+    size_t fft_scratch_bytes = 0;
 
     // type 3
     Type3Params t3{};
-    int64_t type3_outer[kMaxDims] = {1, 1, 1};
+    int64_t type3_outer[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
     std::unique_ptr<Plan> outer, inner;
     // F32 1D / 2D type-3 plans whose outer grid is small spread their
     // sources with the small-grid spreader (no point sort)
@@ -164,6 +171,9 @@ struct Plan {
     FftPlan *fft_for(int64_t vectors);
     bool ensure_fft(int64_t vectors);
     bool fft(int64_t vectors, void *data, hipStream_t stream);
+    FftPlan *fft_group_plan(int rank, const int64_t *lengths, int64_t batch);
+    // run = false: create the plans and the scratch only
+    bool fft_nd(int64_t vectors, void *data, hipStream_t stream, bool run);
     void mark(int stage, bool end, hipStream_t stream);
     void reset_marks();
 

@@ -175,14 +175,33 @@ __global__ __launch_bounds__(1024) void k_plan(int64_t nblocks, const uint32_t *
     }
 }
 
-template <typename T, int R>
+// A staged record: wrapped starts, strength, the axis-0 weights padded for
+// runs that begin before or after the support, the weights of the other axes.
+template <typename T, int R, int D>
 struct Staged {
-    int s[3];
+    int s[D];
     T cre, cim;
     T kxp[2 * R + kKw];
-    T ky[kKw];
-    T kz[kKw];
+    T kr[D > 1 ? D - 1 : 1][kKw];
 };
+
+// Wrapped starts of record `slot` (layout of rec_quads<D>).
+template <int D>
+__device__ inline void load_starts(const int4 *__restrict__ rec_start, int64_t slot, int *ss)
+{
+    const int4 a = rec_start[rec_quads<D>() * slot];
+    ss[0] = a.x;
+    if constexpr (D > 1) ss[1] = a.y;
+    if constexpr (D > 2) ss[2] = a.z;
+    if constexpr (D > 3) {
+        ss[3] = a.w;
+        const int4 b = rec_start[2 * slot + 1];
+        if constexpr (D > 4) ss[4] = b.x;
+        if constexpr (D > 5) ss[5] = b.y;
+        if constexpr (D > 6) ss[6] = b.z;
+        if constexpr (D > 7) ss[7] = b.w;
+    }
+}
 
 // One work item: records [rlo, rhi) of block b's reach, for vector v.
 // Writes the block's cells (light) or a partial slot (heavy part).
@@ -200,7 +219,8 @@ __global__ __launch_bounds__(kLanes) void k_spread(GridParams g, SpreadParams sp
                                                    T *__restrict__ grid, T *__restrict__ partial,
                                                    int64_t block_cells)
 {
-    __shared__ Staged<T, R> st[kLanes];
+    __shared__ Staged<T, R, D> st[kLanes];
+    constexpr int RS = 4 * rec_quads<D>();  // offsets per record
     __shared__ int64_t seg_first[kLanes];
     __shared__ int64_t seg_excl[kLanes + 1];
 
@@ -215,7 +235,7 @@ __global__ __launch_bounds__(kLanes) void k_spread(GridParams g, SpreadParams sp
     // lane -> cells: run of R cells along axis 0 and one cell per other axis
     int lane_x = lane % sp.groups0;
     int rest_lane = lane / sp.groups0;
-    int lane_c[3] = {0, 0, 0};
+    int lane_c[D];
     lane_c[0] = lane_x * R;
 #pragma unroll
     for (int a = 1; a < D; ++a) {
@@ -244,7 +264,7 @@ __global__ __launch_bounds__(kLanes) void k_spread(GridParams g, SpreadParams sp
             rhi = rlo + len < total ? rlo + len : total;
         }
         BlockReach<D> r = block_reach<D>(b, g, sp);
-        int64_t cell[3];
+        int64_t cell[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) cell[a] = r.b0[a] + lane_c[a];
 
@@ -286,8 +306,8 @@ __global__ __launch_bounds__(kLanes) void k_spread(GridParams g, SpreadParams sp
                             s_hi = mid - 1;
                     }
                     slot = seg_first[s_lo] + (k - seg_excl[s_lo]);
-                    int4 s4 = rec_start[slot];
-                    int ss[3] = {s4.x, s4.y, s4.z};
+                    int ss[D];
+                    load_starts<D>(rec_start, slot, ss);
                     touch = true;
 #pragma unroll
                     for (int a = 0; a < D; ++a) {
@@ -302,44 +322,37 @@ __global__ __launch_bounds__(kLanes) void k_spread(GridParams g, SpreadParams sp
                 int pos = __popcll(mask & ((1ull << lane) - 1ull));
                 int nstaged = __popcll(mask);
                 if (touch) {
-                    Staged<T, R> &S = st[pos];
-                    int4 s4 = rec_start[slot];
-                    S.s[0] = s4.x;
-                    S.s[1] = s4.y;
-                    S.s[2] = s4.z;
+                    Staged<T, R, D> &S = st[pos];
+                    int ss[D];
+                    load_starts<D>(rec_start, slot, ss);
+#pragma unroll
+                    for (int a = 0; a < D; ++a) S.s[a] = ss[a];
                     S.cre = cv[2 * slot];
                     S.cim = cv[2 * slot + 1];
-                    T ox = rec_off[4 * slot + 0];
+                    T ox = rec_off[RS * slot + 0];
 #pragma unroll
                     for (int j = 0; j < 2 * R + kKw; ++j) {
                         int tt = j - (R - 1);
                         S.kxp[j] = (tt >= 0 && tt < w) ? es_weight<T>((T)tt + ox, w, beta) : (T)0;
                     }
-                    if (D > 1) {
-                        T oy = rec_off[4 * slot + 1];
-                        for (int j = 0; j < w; ++j) S.ky[j] = es_weight<T>((T)j + oy, w, beta);
-                    }
-                    if (D > 2) {
-                        T oz = rec_off[4 * slot + 2];
-                        for (int j = 0; j < w; ++j) S.kz[j] = es_weight<T>((T)j + oz, w, beta);
+#pragma unroll
+                    for (int a = 1; a < D; ++a) {
+                        T oa = rec_off[RS * slot + a];
+                        for (int j = 0; j < w; ++j) S.kr[a - 1][j] = es_weight<T>((T)j + oa, w, beta);
                     }
                 }
                 __syncthreads();
                 for (int q = 0; q < nstaged; ++q) {
-                    const Staged<T, R> &S = st[q];
+                    const Staged<T, R, D> &S = st[q];
+                    // product of the other axes' weights, axis 1 first
                     T wyz = (T)1;
                     bool on = true;
-                    if (D > 1) {
-                        int64_t dy = cell[1] - S.s[1];
-                        if (dy < 0) dy += g.ax[1].n;
-                        on = dy < w;
-                        wyz = on ? S.ky[dy] : (T)0;
-                    }
-                    if (D > 2) {
-                        int64_t dz = cell[2] - S.s[2];
-                        if (dz < 0) dz += g.ax[2].n;
-                        on = on && dz < w;
-                        wyz = on ? wyz * S.kz[dz] : (T)0;
+#pragma unroll
+                    for (int a = 1; a < D; ++a) {
+                        int64_t da = cell[a] - S.s[a];
+                        if (da < 0) da += g.ax[a].n;
+                        on = on && da < w;
+                        if (on) wyz = wyz * S.kr[a - 1][da];
                     }
                     if (!on) continue;
                     int64_t nx = g.ax[0].n;
@@ -482,7 +495,7 @@ __global__ __launch_bounds__(256) void k_reduce(GridParams g, SpreadParams sp,
             int lane_x = lane % sp.groups0;
             int rl = lane / sp.groups0;
             int64_t rest = b;
-            int64_t cell[3] = {0, 0, 0};
+            int64_t cell[D];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 int64_t bi = rest % sp.nblocks[a];
@@ -711,7 +724,8 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
         set_error("plain row tiles need a 3D plan of kernel width <= 8");
         return false;
     }
-    if (!mfma && !tiles && sp.run != 2 && sp.run != 4 && sp.run != 8) {
+    if (!mfma && !tiles && sp.run != 2 && sp.run != 4 && sp.run != 8 &&
+        !(D >= 4 && sp.run == 16)) {
         set_error("unsupported spread run");
         return false;
     }
@@ -782,6 +796,10 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
             case 2: ok = chunk_spread<D, T, 2>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
             case 4: ok = chunk_spread<D, T, 4>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
             case 8: ok = chunk_spread<D, T, 8>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream); break;
+            case 16:
+                if constexpr (D >= 4)
+                    ok = chunk_spread<D, T, 16>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr, stream);
+                break;
             default: break;
             }
         }
@@ -801,5 +819,15 @@ MNUFFT_SPREAD_INSTANTIATE(3, float)
 MNUFFT_SPREAD_INSTANTIATE(1, double)
 MNUFFT_SPREAD_INSTANTIATE(2, double)
 MNUFFT_SPREAD_INSTANTIATE(3, double)
+MNUFFT_SPREAD_INSTANTIATE(4, float)
+MNUFFT_SPREAD_INSTANTIATE(5, float)
+MNUFFT_SPREAD_INSTANTIATE(6, float)
+MNUFFT_SPREAD_INSTANTIATE(7, float)
+MNUFFT_SPREAD_INSTANTIATE(8, float)
+MNUFFT_SPREAD_INSTANTIATE(4, double)
+MNUFFT_SPREAD_INSTANTIATE(5, double)
+MNUFFT_SPREAD_INSTANTIATE(6, double)
+MNUFFT_SPREAD_INSTANTIATE(7, double)
+MNUFFT_SPREAD_INSTANTIATE(8, double)
 
 }  // namespace mnufft

@@ -141,7 +141,7 @@ struct SmallSpreader {
 // Output-stationary blocks for a fine grid: 64 lanes, `run` cells per lane
 // along axis 0 (run + width - 1 <= n0 keeps every cell's support offset
 // unique within a run).
-SpreadParams make_spread_params(int d, const int64_t *fine, int width);
+SpreadParams make_spread_params(int d, const int64_t *fine, int width, bool f64);
 
 // Tiles of the given cells per axis (run = 0 marks tiles).
 SpreadParams make_tile_params(int d, const int64_t *fine, const int64_t *block);

@@ -50,8 +50,8 @@ __host__ __device__ inline AxisReach axis_reach(int64_t b0, int64_t B, int w, co
 // x interval (two when it wraps) per bin row of the other axes' reach.
 template <int D>
 struct BlockReach {
-    AxisReach ax[3];
-    int64_t b0[3];
+    AxisReach ax[D];
+    int64_t b0[D];
     int64_t nseg;
     // segment t -> record slots [first, last)
     __host__ __device__ void segment(int64_t t, const GridParams &g, const uint32_t *offsets,
