@@ -110,12 +110,33 @@ int mnufft_encode(mnufft_plan plan, int64_t active_batch, int64_t M, const void 
  * modes d_in[i] at the M[i] points d_points[i] into d_out[i], as
  * mnufft_encode(plans[i], 1, M[i], d_points[i], d_in[i], d_out[i]) would,
  * bit for bit.  The points of all plans are binned by one sort and
- * interpolated by one launch; each plan predeconvolves and transforms its
- * own grid.  F32 type-2 plans of one dimension, 1D or 2D, each at most once;
- * one vector per plan.  The plans keep no points afterwards (set them again
- * before a plain execute); plans[0] keeps the batch scratch.
+ * interpolated by one launch; one launch predeconvolves every plan's grid,
+ * and each plan transforms its own.  F32 type-2 plans of one dimension, 1D
+ * or 2D, each at most once; one vector per plan.  The plans keep no points
+ * afterwards (set them again before a plain execute); plans[0] keeps the
+ * batch scratch.
  */
 int mnufft_execute_type2_many(int64_t count, const mnufft_plan *plans, const int64_t *M,
+                              const void *const *d_points, const void *const *d_in,
+                              void *const *d_out, void *stream);
+
+/*
+ * Type-1 encodes of `count` independent plans as one batch (the reference
+ * runs one type1_gpu plan per problem, src/plan.rs:457-463, 808-872; no
+ * single reference entry point does this): plans[i] spreads its strengths
+ * d_in[i] at the M[i] points d_points[i] and writes its modes d_out[i], as
+ * mnufft_encode(plans[i], 1, M[i], d_points[i], d_in[i], d_out[i]) would,
+ * bit for bit.  One bin sort over all the points, one prepare and strength
+ * gather, one spread launch and one heavy-part reduction over every plan's
+ * blocks (per run length; each plan's blocks split by its own work plan),
+ * each plan's FFT, one deconvolution launch.  Type-1 plans of one rank
+ * (1D to 3D) and precision, each at most once, one vector per plan, that
+ * spread on the output-stationary blocks in index point order (the plain
+ * spread; 3D plans with mnufft_set_bin_tuning(plan, 0, 0)); else
+ * MNUFFT_ERROR_INVALID_ARGUMENT.  The plans keep no points afterwards;
+ * plans[0] keeps the batch scratch.
+ */
+int mnufft_execute_type1_many(int64_t count, const mnufft_plan *plans, const int64_t *M,
                               const void *const *d_points, const void *const *d_in,
                               void *const *d_out, void *stream);
 

@@ -407,6 +407,40 @@ def encode_type2_many(plans, points, values, outs=None):
     return res
 
 
+def encode_type1_many(plans, points, values, outs=None):
+    """Type-1 encodes of many independent plans as one batch: plans[i]
+    spreads values[i] (its strengths) at points[i] into its modes; one bin
+    sort over all the points, one spread launch over every plan's blocks,
+    each plan's FFT, one deconvolution launch.  Type-1 plans of one rank
+    (1D to 3D) and precision, one vector each, on the plain spread in index
+    order (3D plans: set_bin_tuning(plain_rows=False)); outputs equal each
+    plan's own encode_gpu bit for bit.  The plans keep no points
+    afterwards."""
+    n = len(plans)
+    if not (len(points) == len(values) == n) or n == 0:
+        raise ValueError("plans, points and values must have the same nonzero length")
+    if outs is None:
+        outs = [None] * n
+    res = []
+    for p, x, v, o in zip(plans, points, values, outs):
+        if p.kind != 1:
+            raise ValueError("batched type-1 encodes take type-1 plans")
+        p._check_points(x)
+        if v.dtype != p.complex_dtype or not v.is_contiguous() or v.numel() < x.shape[0]:
+            raise ValueError("values must be a contiguous tensor of the plan's strengths")
+        res.append(p._out(1, o, p.config.mode_count()))
+    vp = ctypes.c_void_p
+    hs = (vp * n)(*[p._handle for p in plans])
+    ms = (ctypes.c_int64 * n)(*[int(x.shape[0]) for x in points])
+    xs = (vp * n)(*[_ptr(x) for x in points])
+    vs = (vp * n)(*[_ptr(v) for v in values])
+    os_ = (vp * n)(*[_ptr(o) for o in res])
+    check(lib().mnufft_execute_type1_many(n, hs, ms, xs, vs, os_, _stream_ptr()))
+    for p in plans:
+        p._points = None
+    return res
+
+
 def _direct(kind, dim, n_modes, isign, mode_order, x, values, nout, sel, s):
     torch = _torch()
     out = torch.empty(nout, dtype=torch.complex128, device="cuda")

@@ -44,6 +44,10 @@ SIGNATURES = [
                                                  ctypes.POINTER(ctypes.c_int64),
                                                  ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                                  ctypes.POINTER(_vp), _vp]),
+    ("mnufft_execute_type1_many", ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_vp),
+                                                 ctypes.POINTER(ctypes.c_int64),
+                                                 ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                 ctypes.POINTER(_vp), _vp]),
     ("mnufft_execute_type3", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp,
                                             ctypes.c_int64, _vp, _vp, _vp]),
     ("mnufft_bin_sort", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _c_i64p, _vp]),

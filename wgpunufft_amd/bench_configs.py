@@ -202,8 +202,8 @@ def config4(args, W, S, torch):
           total_pts / t, "points/s", t * 1e3, "f32",
           {"workload": f"config 4a: 64 2D type-2 plans, modes log-uniform [64,512], "
                        f"M log-uniform [1e4,2e6] (total {total_pts})"},
-          {"launch": "mnufft_execute_type2_many: one bin sort and one interpolation "
-                     "launch for all 64 problems, per-plan predeconvolution and FFT",
+          {"launch": "mnufft_execute_type2_many: one predeconvolution launch, per-plan "
+                     "FFT, one bin sort and one interpolation launch for all 64 problems",
            "bitwise_equal_to_per_plan": bool(same)})
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
@@ -224,6 +224,59 @@ def config4(args, W, S, torch):
           {"launch": "the batch captured in one HIP graph, replayed",
            "bitwise_equal_to_per_plan": bool(same)})
     del graph, eager
+    # the same 64 problems as type-1 encodes (strengths at the points ->
+    # modes): one plan each, then one batch, then the batch in a graph
+    t1 = []
+    for i, ((n0, n1, M, _, _), (_, x, _, _)) in enumerate(zip(S.batch_problem_specs(),
+                                                            problems)):
+        plan = W.NufftPlan.type1_gpu(W.NufftConfig([n0, n1], 1e-5))
+        c = torch.from_numpy(S.complex_normal(S.config_seed(4, 300 + i), M)).to(dev)
+        t1.append((plan, x, c, torch.empty(n0 * n1, dtype=torch.complex64, device=dev)))
+
+    def run_all_t1():
+        for plan, x, c, out in t1:
+            plan.encode_gpu(x, c, out=out)
+
+    t = _timed(run_all_t1, args.steps, args.warmup)
+    eager = [o.clone() for _, _, _, o in t1]
+    wl = {"workload": f"config 4a points as type-1: 64 2D type-1 plans, modes log-uniform "
+                      f"[64,512], M log-uniform [1e4,2e6] (total {total_pts})"}
+    _line(args, "NU points/s, 64 independent 2D type-1 problems tol 1e-5", total_pts / t,
+          "points/s", t * 1e3, "f32", wl,
+          {"launch": "one plan per problem, back to back on one stream"})
+    plans1 = [p[0] for p in t1]
+    xs1 = [p[1] for p in t1]
+    cs1 = [p[2] for p in t1]
+    outs1 = [p[3] for p in t1]
+
+    def run_batch_t1():
+        W.encode_type1_many(plans1, xs1, cs1, outs1)
+
+    t = _timed(run_batch_t1, args.steps, args.warmup)
+    same = all(torch.equal(e.view(torch.float32), o.view(torch.float32))
+               for e, o in zip(eager, outs1))
+    _line(args, "NU points/s, 64 independent 2D type-1 problems tol 1e-5 (batch)",
+          total_pts / t, "points/s", t * 1e3, "f32", wl,
+          {"launch": "mnufft_execute_type1_many: one bin sort, prepare and spread launch "
+                     "for all 64 problems, per-plan FFT, one deconvolution launch",
+           "bitwise_equal_to_per_plan": bool(same)})
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        run_batch_t1()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        run_batch_t1()
+    t = _timed(graph.replay, args.steps, args.warmup)
+    same = all(torch.equal(e.view(torch.float32), o.view(torch.float32))
+               for e, o in zip(eager, outs1))
+    _line(args, "NU points/s, 64 independent 2D type-1 problems tol 1e-5 (batch, graph)",
+          total_pts / t, "points/s", t * 1e3, "f32", wl,
+          {"launch": "the type-1 batch captured in one HIP graph, replayed",
+           "bitwise_equal_to_per_plan": bool(same)})
+    del graph, eager, t1, plans1
     # (b) 32 vectors over a radial 3D point set
     x = S.radial_spokes()
     M = x.shape[0]

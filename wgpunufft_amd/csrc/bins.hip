@@ -1001,9 +1001,103 @@ __global__ __launch_bounds__(256) void k_gather_strengths(int64_t M, int64_t vec
         *(V2 *)(c_sorted + 2 * (v * M + s)) = *(const V2 *)(c + 2 * (v * M + idx));
 }
 
+// Problem of concatenated slot s: the last problem whose m_off <= s
+// (problems without points share the next one's m_off).
+__device__ inline int problem_of_slot(const BatchProblem *__restrict__ probs, int nprob,
+                                      int64_t s)
+{
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (probs[mid].m_off <= s)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_prepare_many(int64_t M,
+                                                      const uint32_t *__restrict__ sorted,
+                                                      const BatchProblem *__restrict__ probs,
+                                                      int nprob, int4 *__restrict__ rec_start,
+                                                      T *__restrict__ rec_off,
+                                                      uint32_t *__restrict__ rec_perm)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    const BatchProblem &pb = probs[problem_of_slot(probs, nprob, s)];
+    const uint32_t idx = (uint32_t)((int64_t)sorted[s] - pb.m_off);
+    const T *coords = (const T *)pb.coords;
+    T x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = coords[(int64_t)idx * D + a];
+    bool reach = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) reach = reach && FoldTraits<T>::in_reach(x[a]);
+    int st[kMaxDims] = {};
+    T off[kMaxDims] = {};
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        int64_t start;
+        double o;
+        support_start<T>(x[a], reach, pb.g, a, &start, &o);
+        st[a] = (int)wrapped_start(start, pb.g.ax[a].n);
+        off[a] = (T)o;
+    }
+    store_record<D, T>(rec_start, rec_off, s, st, off, idx);
+    rec_perm[s] = idx;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_strengths_many(
+    int64_t M, const uint32_t *__restrict__ perm, const BatchProblem *__restrict__ probs,
+    int nprob, T *__restrict__ c_sorted)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+    const BatchProblem &pb = probs[problem_of_slot(probs, nprob, s)];
+    *(V2 *)(c_sorted + 2 * s) = *((const V2 *)pb.in + perm[s]);
+}
+
 inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 }  // namespace
+
+template <int D, typename T>
+bool prepare_many(int64_t M, const uint32_t *sorted, const BatchProblem *probs, int nprob,
+                  int4 *rec_start, T *rec_off, uint32_t *rec_perm, hipStream_t stream)
+{
+    if (M == 0) return true;
+    k_prepare_many<D, T><<<grid_for(M, 256), 256, 0, stream>>>(M, sorted, probs, nprob,
+                                                               rec_start, rec_off, rec_perm);
+    return hip_ok(hipGetLastError(), "batched prepare launch");
+}
+
+template <typename T>
+bool gather_strengths_many(int64_t M, const uint32_t *rec_perm, const BatchProblem *probs,
+                           int nprob, T *c_sorted, hipStream_t stream)
+{
+    if (M == 0) return true;
+    k_gather_strengths_many<T><<<grid_for(M, 256), 256, 0, stream>>>(M, rec_perm, probs, nprob,
+                                                                     c_sorted);
+    return hip_ok(hipGetLastError(), "batched strength gather launch");
+}
+#define MNUFFT_PREPARE_MANY_INSTANTIATE(D, T)                                                     \
+    template bool prepare_many<D, T>(int64_t, const uint32_t *, const BatchProblem *, int,        \
+                                     int4 *, T *, uint32_t *, hipStream_t);
+MNUFFT_PREPARE_MANY_INSTANTIATE(1, float)
+MNUFFT_PREPARE_MANY_INSTANTIATE(2, float)
+MNUFFT_PREPARE_MANY_INSTANTIATE(3, float)
+MNUFFT_PREPARE_MANY_INSTANTIATE(1, double)
+MNUFFT_PREPARE_MANY_INSTANTIATE(2, double)
+MNUFFT_PREPARE_MANY_INSTANTIATE(3, double)
+template bool gather_strengths_many<float>(int64_t, const uint32_t *, const BatchProblem *, int,
+                                           float *, hipStream_t);
+template bool gather_strengths_many<double>(int64_t, const uint32_t *, const BatchProblem *, int,
+                                            double *, hipStream_t);
 
 // ---------------------------------------------------------------------
 // Host side.
@@ -1401,10 +1495,15 @@ void *static_sort_scratch()
     template bool BinSorter::prepare<D, T>(int64_t, const T *, const GridParams &, BinOrder,       \
                                            int4 *, T *, uint32_t *, hipStream_t,                 \
                                            const uint32_t *);
-template bool BinSorter::sort_many<1, float>(int64_t, const BatchProblem *, int, uint64_t,
+#define MNUFFT_SORT_MANY_INSTANTIATE(D, T)                                                        \
+    template bool BinSorter::sort_many<D, T>(int64_t, const BatchProblem *, int, uint64_t,        \
                                              hipStream_t);
-template bool BinSorter::sort_many<2, float>(int64_t, const BatchProblem *, int, uint64_t,
-                                             hipStream_t);
+MNUFFT_SORT_MANY_INSTANTIATE(1, float)
+MNUFFT_SORT_MANY_INSTANTIATE(2, float)
+MNUFFT_SORT_MANY_INSTANTIATE(3, float)
+MNUFFT_SORT_MANY_INSTANTIATE(1, double)
+MNUFFT_SORT_MANY_INSTANTIATE(2, double)
+MNUFFT_SORT_MANY_INSTANTIATE(3, double)
 MNUFFT_BINS_INSTANTIATE(1, float)
 MNUFFT_BINS_INSTANTIATE(2, float)
 MNUFFT_BINS_INSTANTIATE(3, float)

@@ -85,6 +85,20 @@ private:
     int64_t nbins_ = 0, M_ = 0;
 };
 
+// Records of a multi-plan type-1 batch, from the permutation of
+// BinSorter::sort_many (indices of the concatenated point set; a problem's
+// points take the slots [m_off, m_off + M)): rec_start / rec_off as
+// prepare() writes them with the problem's geometry, rec_perm[slot] = the
+// point's index inside its problem.
+template <int D, typename T>
+bool prepare_many(int64_t M, const uint32_t *sorted, const BatchProblem *probs, int nprob,
+                  int4 *rec_start, T *rec_off, uint32_t *rec_perm, hipStream_t stream);
+// c_sorted[s] = the problem's strength (probs[q].in) of the point at slot s
+// (one vector per problem).
+template <typename T>
+bool gather_strengths_many(int64_t M, const uint32_t *rec_perm, const BatchProblem *probs,
+                           int nprob, T *c_sorted, hipStream_t stream);
+
 // Device address, on the current device, of the library's static sort
 // scratch (a file-scope device array of static_sort_bytes() bytes), looked
 // up once per device; nullptr (last error set) when the lookup fails.

@@ -305,6 +305,30 @@ int mnufft_execute_type2_many(int64_t count, const mnufft_plan *plans, const int
     });
 }
 
+int mnufft_execute_type1_many(int64_t count, const mnufft_plan *plans, const int64_t *M,
+                              const void *const *d_points, const void *const *d_in,
+                              void *const *d_out, void *stream)
+{
+    return guarded([&] {
+        if (count < 1 || !plans || !M || !d_points || !d_in || !d_out) {
+            set_error("invalid batch arguments");
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        }
+        std::vector<Plan *> ps((size_t)count);
+        for (int64_t i = 0; i < count; ++i) {
+            ps[(size_t)i] = plans[i] ? plans[i]->plan.get() : nullptr;
+            if (ps[(size_t)i] && !mnufft::point_count_ok(M[i], "point"))
+                return MNUFFT_ERROR_PLAN;
+        }
+        if (!mnufft::type1_many_valid(count, ps.data(), M, d_points, d_in, d_out))
+            return MNUFFT_ERROR_INVALID_ARGUMENT;
+        if (!mnufft::execute_type1_many(count, ps.data(), M, d_points, d_in, d_out,
+                                        (hipStream_t)stream))
+            return MNUFFT_ERROR_GPU;
+        return MNUFFT_SUCCESS;
+    });
+}
+
 int mnufft_execute_type3(mnufft_plan plan, int64_t active_batch, int64_t M, const void *d_sources,
                          const void *d_strengths, int64_t N, const void *d_targets, void *d_out,
                          void *stream)

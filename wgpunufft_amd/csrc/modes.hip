@@ -38,6 +38,31 @@ inline unsigned grid_for(int64_t n, int threads)
     return (unsigned)(b < 1 ? 1 : b);
 }
 
+// Mode k of one vector: out[2k..] = grid[fold(k)] * amplitude (the
+// amplitude multiplied in axis order).
+template <int D, typename T>
+__device__ inline void deconv_mode(const ModeParams &mp, const double *__restrict__ inv_phihat,
+                                   const T *__restrict__ grid_v, int64_t k, T *__restrict__ out)
+{
+    int64_t rest = k, fine_lin = 0, stride = 1;
+    double amp = 1.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        int64_t N = mp.n_modes[a], n = mp.fine[a];
+        int64_t ka = rest % N;
+        rest /= N;
+        int64_t m = mp.centered ? ka - N / 2 : (ka < (N + 1) / 2 ? ka : ka - N);
+        int64_t f = m < 0 ? m + n : m;
+        fine_lin += f * stride;
+        stride *= n;
+        amp *= inv_phihat[mp.phihat_off[a] + (m < 0 ? -m : m)];
+    }
+    const T *src = grid_v + 2 * fine_lin;
+    T a = (T)amp;
+    out[0] = src[0] * a;
+    out[1] = src[1] * a;
+}
+
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_deconv(ModeParams mp, const double *__restrict__ inv_phihat,
                                                 const T *__restrict__ grid, T *__restrict__ modes,
@@ -48,23 +73,42 @@ __global__ __launch_bounds__(256) void k_deconv(ModeParams mp, const double *__r
          t += (int64_t)gridDim.x * blockDim.x) {
         int64_t v = t / mp.mode_total;
         int64_t k = t - v * mp.mode_total;
-        int64_t rest = k, fine_lin = 0, stride = 1;
-        double amp = 1.0;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            int64_t N = mp.n_modes[a], n = mp.fine[a];
-            int64_t ka = rest % N;
-            rest /= N;
-            int64_t m = mp.centered ? ka - N / 2 : (ka < (N + 1) / 2 ? ka : ka - N);
-            int64_t f = m < 0 ? m + n : m;
-            fine_lin += f * stride;
-            stride *= n;
-            amp *= inv_phihat[mp.phihat_off[a] + (m < 0 ? -m : m)];
+        deconv_mode<D, T>(mp, inv_phihat, grid + 2 * v * mp.fine_total, k, modes + 2 * t);
+    }
+}
+
+// Problem of element t of a combined list (the last problem whose base <= t).
+template <bool Modes>
+__device__ inline int problem_of(const ModeProblem *__restrict__ mprobs, int nprob, int64_t t)
+{
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((Modes ? mprobs[mid].mode_base : mprobs[mid].fine_base) <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_deconv_many(const BatchProblem *__restrict__ probs,
+                                                     const ModeProblem *__restrict__ mprobs,
+                                                     int nprob, int64_t total)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int q = problem_of<true>(mprobs, nprob, t);
+        const ModeProblem &mq = mprobs[q];
+        const int64_t k = t - mq.mode_base;
+        T *out = (T *)probs[q].out + 2 * k;
+        if (mq.zero) {
+            out[0] = (T)0;
+            out[1] = (T)0;
+        } else {
+            deconv_mode<D, T>(mq.mp, mq.inv_phihat, (const T *)probs[q].grid, k, out);
         }
-        const T *src = grid + 2 * (v * mp.fine_total + fine_lin);
-        T a = (T)amp;
-        modes[2 * t] = src[0] * a;
-        modes[2 * t + 1] = src[1] * a;
     }
 }
 
@@ -116,6 +160,46 @@ __global__ __launch_bounds__(256) void k_deconv_rows(ModeParams mp,
     }
 }
 
+// Fine cell f of one vector: the mode folded onto it times its amplitude
+// (amplitude in axis order), zero off the modes.
+template <int D, typename T>
+__device__ inline void predeconv_cell(const ModeParams &mp, const double *__restrict__ inv_phihat,
+                                      const T *__restrict__ modes_v, int64_t f, T *__restrict__ out)
+{
+    int64_t rest = f, mode_lin = 0, stride = 1;
+    bool has = true;
+    double amp = 1.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        int64_t N = mp.n_modes[a], n = mp.fine[a];
+        int64_t fa = rest % n;
+        rest /= n;
+        int64_t hi = N - 1 - N / 2;  // largest frequency
+        int64_t m;
+        if (fa <= hi)
+            m = fa;
+        else if (fa >= n - N / 2)
+            m = fa - n;
+        else {
+            has = false;
+            m = 0;
+        }
+        int64_t ka = mp.centered ? m + N / 2 : (m >= 0 ? m : m + N);
+        mode_lin += ka * stride;
+        stride *= N;
+        amp *= inv_phihat[mp.phihat_off[a] + (m < 0 ? -m : m)];
+    }
+    T re = (T)0, im = (T)0;
+    if (has) {
+        const T *src = modes_v + 2 * mode_lin;
+        T a = (T)amp;
+        re = src[0] * a;
+        im = src[1] * a;
+    }
+    out[0] = re;
+    out[1] = im;
+}
+
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_predeconv(ModeParams mp,
                                                    const double *__restrict__ inv_phihat,
@@ -127,38 +211,22 @@ __global__ __launch_bounds__(256) void k_predeconv(ModeParams mp,
          t += (int64_t)gridDim.x * blockDim.x) {
         int64_t v = t / mp.fine_total;
         int64_t f = t - v * mp.fine_total;
-        int64_t rest = f, mode_lin = 0, stride = 1;
-        bool has = true;
-        double amp = 1.0;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            int64_t N = mp.n_modes[a], n = mp.fine[a];
-            int64_t fa = rest % n;
-            rest /= n;
-            int64_t hi = N - 1 - N / 2;  // largest frequency
-            int64_t m;
-            if (fa <= hi)
-                m = fa;
-            else if (fa >= n - N / 2)
-                m = fa - n;
-            else {
-                has = false;
-                m = 0;
-            }
-            int64_t ka = mp.centered ? m + N / 2 : (m >= 0 ? m : m + N);
-            mode_lin += ka * stride;
-            stride *= N;
-            amp *= inv_phihat[mp.phihat_off[a] + (m < 0 ? -m : m)];
-        }
-        T re = (T)0, im = (T)0;
-        if (has) {
-            const T *src = modes + 2 * (v * mp.mode_total + mode_lin);
-            T a = (T)amp;
-            re = src[0] * a;
-            im = src[1] * a;
-        }
-        grid[2 * t] = re;
-        grid[2 * t + 1] = im;
+        predeconv_cell<D, T>(mp, inv_phihat, modes + 2 * v * mp.mode_total, f, grid + 2 * t);
+    }
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_predeconv_many(const BatchProblem *__restrict__ probs,
+                                                        const ModeProblem *__restrict__ mprobs,
+                                                        int nprob, int64_t total)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int q = problem_of<false>(mprobs, nprob, t);
+        const ModeProblem &mq = mprobs[q];
+        const int64_t f = t - mq.fine_base;
+        predeconv_cell<D, T>(mq.mp, mq.inv_phihat, (const T *)probs[q].in, f,
+                             (T *)probs[q].grid + 2 * f);
     }
 }
 
@@ -595,6 +663,37 @@ template bool transpose_complex<float>(const float *, float *, int64_t, int64_t,
                                        hipStream_t);
 template bool transpose_complex<double>(const double *, double *, int64_t, int64_t, int64_t,
                                         hipStream_t);
+
+template <int D, typename T>
+bool deconvolve_many(const BatchProblem *probs, const ModeProblem *mprobs, int nprob,
+                     int64_t total_modes, hipStream_t stream)
+{
+    if (total_modes == 0) return true;
+    k_deconv_many<D, T><<<grid_for(total_modes, 256), 256, 0, stream>>>(probs, mprobs, nprob,
+                                                                         total_modes);
+    return hip_ok(hipGetLastError(), "batched deconvolution launch");
+}
+
+template <int D, typename T>
+bool predeconvolve_many(const BatchProblem *probs, const ModeProblem *mprobs, int nprob,
+                        int64_t total_cells, hipStream_t stream)
+{
+    if (total_cells == 0) return true;
+    k_predeconv_many<D, T><<<grid_for(total_cells, 256), 256, 0, stream>>>(probs, mprobs, nprob,
+                                                                            total_cells);
+    return hip_ok(hipGetLastError(), "batched predeconvolution launch");
+}
+#define MNUFFT_MODES_MANY_INSTANTIATE(D, T)                                                       \
+    template bool deconvolve_many<D, T>(const BatchProblem *, const ModeProblem *, int, int64_t,  \
+                                        hipStream_t);                                             \
+    template bool predeconvolve_many<D, T>(const BatchProblem *, const ModeProblem *, int,        \
+                                           int64_t, hipStream_t);
+MNUFFT_MODES_MANY_INSTANTIATE(1, float)
+MNUFFT_MODES_MANY_INSTANTIATE(2, float)
+MNUFFT_MODES_MANY_INSTANTIATE(3, float)
+MNUFFT_MODES_MANY_INSTANTIATE(1, double)
+MNUFFT_MODES_MANY_INSTANTIATE(2, double)
+MNUFFT_MODES_MANY_INSTANTIATE(3, double)
 
 bool direct_sum(int kind, int d, const DirectParams &dp, const double *x, const double *in,
                 const int64_t *sel, const double *s, double *out, hipStream_t stream)

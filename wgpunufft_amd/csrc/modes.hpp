@@ -45,9 +45,29 @@ struct DirectParams {
     double sign;
 };
 
+// Mode stage of one problem of a multi-plan execution: its modes take
+// [mode_base, mode_base + mp.mode_total) and its fine cells [fine_base,
+// fine_base + mp.fine_total) of the combined lists; `zero` (type 1 without
+// points) writes zeros as the plan's own execute does.
+struct ModeProblem {
+    ModeParams mp;
+    const double *inv_phihat;
+    int64_t mode_base, fine_base;
+    int zero;
+};
+
 template <int D, typename T>
 bool deconvolve(const ModeParams &mp, const double *inv_phihat, const T *grid, T *modes,
                 int64_t vectors, hipStream_t stream);
+// Deconvolution (type 1: probs[q].grid -> probs[q].out) and
+// predeconvolution (type 2: probs[q].in -> probs[q].grid) of every problem
+// in one launch each, the same bits as the per-plan kernels.
+template <int D, typename T>
+bool deconvolve_many(const BatchProblem *probs, const ModeProblem *mprobs, int nprob,
+                     int64_t total_modes, hipStream_t stream);
+template <int D, typename T>
+bool predeconvolve_many(const BatchProblem *probs, const ModeProblem *mprobs, int nprob,
+                        int64_t total_cells, hipStream_t stream);
 template <int D, typename T>
 bool predeconvolve(const ModeParams &mp, const double *inv_phihat, const T *modes, T *grid,
                    int64_t vectors, hipStream_t stream);

@@ -130,14 +130,17 @@ struct GridParams {
     int64_t fine_total;
 };
 
-// One problem of a multi-plan type-2 execution: its points take indices
+// One problem of a multi-plan execution: its points take indices
 // [m_off, m_off + M) of the concatenated point set and bins [bin_base,
 // bin_base + g.nbins) of the combined bin key; its interpolation work items
-// start at item_base.
+// start at item_base.  Type 2 reads `grid` and writes `out` (values at the
+// points); type 1 reads `in` (strengths), spreads onto `grid` and writes
+// `out` (modes).
 struct BatchProblem {
     const void *coords;
     void *out;
     const void *grid;
+    const void *in;
     int64_t m_off, M, item_base;
     uint64_t bin_base;
     GridParams g;

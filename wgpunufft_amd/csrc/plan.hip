@@ -155,7 +155,8 @@ Plan::~Plan()
 {
     for (void *p : {(void *)d_inv_phihat, grid, (void *)rec_start, rec_off, (void *)rec_perm,
                     c_sorted, t3_xr, t3_cp, t3_sr, t3_factor, t3_interp, t3_box, t3_perm,
-                    t3_s_sorted, t3_counts, (void *)item_off, (void *)batch_table, fft_scratch})
+                    t3_s_sorted, t3_counts, (void *)item_off, (void *)batch_table, (void *)batch_modes,
+                    (void *)batch_spread, fft_scratch})
         if (p) (void)hipFree(p);
     for (auto &sl : slots)
         for (auto &e : sl.ev)
@@ -1054,19 +1055,42 @@ bool Plan::stage_times(double *ms, int n, int64_t *calls)
 // ----------------------------------------------------------------------
 namespace {
 
-// The problem table is written by kernels whose arguments carry up to
-// kTableChunk problems (kernel arguments are copied at launch, so the
+// Problem tables are written by kernels whose arguments carry up to
+// TableChunk<E>::N entries (kernel arguments are copied at launch, so the
 // sequence can be captured in a graph), within 4 KiB of arguments.
-constexpr int kTableChunk = (int)((4096 - 64) / sizeof(BatchProblem));
+template <typename E>
 struct TableChunk {
-    BatchProblem p[kTableChunk];
+    static constexpr int N = (int)std::min<size_t>(64, (4096 - 64) / sizeof(E));
+    E p[N];
     int n;
 };
-static_assert(kTableChunk >= 1 && sizeof(TableChunk) <= 4096, "table chunk exceeds 4 KiB");
 
-__global__ void k_batch_table(TableChunk chunk, BatchProblem *dst)
+template <typename E>
+__global__ void k_batch_table(TableChunk<E> chunk, E *dst)
 {
     if ((int)threadIdx.x < chunk.n) dst[threadIdx.x] = chunk.p[threadIdx.x];
+}
+
+// dst[0..n) = v on the device, *cap grown (allocation) when n exceeds it.
+template <typename E>
+bool upload_table(const std::vector<E> &v, E **dst, int64_t *cap, hipStream_t stream)
+{
+    static_assert(TableChunk<E>::N >= 1 && sizeof(TableChunk<E>) <= 4096, "table chunk");
+    const int64_t n = (int64_t)v.size();
+    if (n > *cap) {
+        if (*dst) (void)hipFree(*dst);
+        *dst = nullptr;
+        *cap = 0;
+        if (!hip_ok(hipMalloc(dst, sizeof(E) * n), "batch table")) return false;
+        *cap = n;
+    }
+    for (int64_t i0 = 0; i0 < n; i0 += TableChunk<E>::N) {
+        TableChunk<E> chunk;
+        chunk.n = (int)std::min<int64_t>(TableChunk<E>::N, n - i0);
+        for (int k = 0; k < chunk.n; ++k) chunk.p[k] = v[(size_t)(i0 + k)];
+        k_batch_table<E><<<1, 64, 0, stream>>>(chunk, *dst + i0);
+    }
+    return hip_ok(hipGetLastError(), "batch table launch");
 }
 
 template <int D>
@@ -1098,9 +1122,10 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
     const int G = maxw <= 4 ? 4 : (maxw <= 8 ? 8 : 16);
     const int64_t per_item = 256 / G;
     std::vector<BatchProblem> table((size_t)count);
+    std::vector<ModeProblem> mtable((size_t)count);
     std::vector<int> widths;
     std::vector<int64_t> bin_lo, bin_hi;
-    int64_t m_off = 0, items = 0, gather_items = 0;
+    int64_t m_off = 0, items = 0, gather_items = 0, cells = 0;
     uint64_t bins = 0;
     for (int64_t r = 0; r < count; ++r) {
         const int64_t i = order[(size_t)r];
@@ -1109,15 +1134,18 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
         p.points = nullptr;
         p.point_count = 0;
         if (!p.ensure_grid(1) || !p.ensure_fft(1)) return false;
-        ModeParams mp = p.mode_params();
-        if (!predeconvolve<D, float>(mp, p.d_inv_phihat, (const float *)inputs[i],
-                                     (float *)p.grid, 1, stream) ||
-            !p.fft(1, p.grid, stream))
-            return false;
+        ModeProblem &mq = mtable[(size_t)r];
+        mq.mp = p.mode_params();
+        mq.inv_phihat = p.d_inv_phihat;
+        mq.mode_base = 0;
+        mq.fine_base = cells;
+        mq.zero = 0;
+        cells += p.fine_total;
         BatchProblem &b = table[(size_t)r];
         b.coords = points[i];
         b.out = outputs[i];
         b.grid = p.grid;
+        b.in = inputs[i];
         b.m_off = m_off;
         b.M = M[i];
         b.item_base = items;
@@ -1138,26 +1166,20 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
         bins += (uint64_t)b.g.nbins;
     }
     if (first_gather == count) gather_items = items;
-    if (m_off == 0) return true;
     if (m_off > 0xffffffffll) {
         set_error("batched type 2: more than 2^32 points over the plans");
         return false;
     }
-    if (count > head.batch_table_cap) {
-        if (head.batch_table) (void)hipFree(head.batch_table);
-        head.batch_table = nullptr;
-        head.batch_table_cap = 0;
-        if (!hip_ok(hipMalloc(&head.batch_table, sizeof(BatchProblem) * count),
-                    "batched type-2 table"))
-            return false;
-        head.batch_table_cap = count;
-    }
-    for (int64_t i0 = 0; i0 < count; i0 += kTableChunk) {
-        TableChunk chunk;
-        chunk.n = (int)std::min<int64_t>(kTableChunk, count - i0);
-        for (int k = 0; k < chunk.n; ++k) chunk.p[k] = table[(size_t)(i0 + k)];
-        k_batch_table<<<1, 64, 0, stream>>>(chunk, head.batch_table + i0);
-    }
+    if (!upload_table(table, &head.batch_table, &head.batch_table_cap, stream) ||
+        !upload_table(mtable, &head.batch_modes, &head.batch_modes_cap, stream))
+        return false;
+    // every plan's grid from its modes in one launch, then its FFT
+    if (!predeconvolve_many<D, float>(head.batch_table, head.batch_modes, (int)count, cells,
+                                      stream))
+        return false;
+    for (int64_t r = 0; r < count; ++r)
+        if (!plans[order[(size_t)r]]->fft(1, plans[order[(size_t)r]]->grid, stream)) return false;
+    if (m_off == 0) return true;
     if (!head.batch_sorter) head.batch_sorter = std::make_unique<BinSorter>();
     BinSorter &bs = *head.batch_sorter;
     if (!bs.sort_many<D, float>(m_off, head.batch_table, (int)count, bins, stream)) return false;
@@ -1200,6 +1222,171 @@ bool execute_type2_many(int64_t count, Plan *const *plans, const int64_t *M,
     }
     return dim == 1 ? execute_type2_many_t<1>(count, plans, M, points, inputs, outputs, stream)
                     : execute_type2_many_t<2>(count, plans, M, points, inputs, outputs, stream);
+}
+
+namespace {
+
+template <int D, typename T>
+bool execute_type1_many_t(int64_t count, Plan *const *plans, const int64_t *M,
+                          const void *const *points, const void *const *inputs,
+                          void *const *outputs, hipStream_t stream)
+{
+    Plan &head = *plans[0];
+    // problems grouped by run length (one spread launch per run), in their
+    // order otherwise
+    std::vector<int64_t> order((size_t)count);
+    for (int64_t i = 0; i < count; ++i) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return plans[a]->spread_params().run < plans[b]->spread_params().run;
+    });
+    std::vector<BatchProblem> table((size_t)count);
+    std::vector<SpreadProblem> stable((size_t)count);
+    std::vector<ModeProblem> mtable((size_t)count);
+    std::vector<SpreadClass> classes;
+    int64_t m_off = 0, modes = 0;
+    uint64_t bins = 0;
+    for (int64_t r = 0; r < count; ++r) {
+        const int64_t i = order[(size_t)r];
+        Plan &p = *plans[i];
+        p.points_set = false;
+        p.points = nullptr;
+        p.point_count = 0;
+        if (!p.ensure_grid(1) || !p.ensure_fft(1)) return false;
+        const SpreadParams &sp = p.spread_params();
+        BatchProblem &b = table[(size_t)r];
+        b.coords = points[i];
+        b.out = outputs[i];
+        b.grid = p.grid;
+        b.in = inputs[i];
+        b.m_off = m_off;
+        b.M = M[i];
+        b.item_base = 0;
+        b.bin_base = bins;
+        b.g = p.type1_geometry(M[i], nullptr);
+        if (classes.empty() || classes.back().run != sp.run)
+            classes.push_back(SpreadClass{(int)r, (int)r, sp.run, 0, 0});
+        SpreadClass &k = classes.back();
+        SpreadProblem &q = stable[(size_t)r];
+        q.sp = sp;
+        q.block_base = k.blocks;
+        const int64_t per_part = (int64_t)64 * sp.run * 2 * (int64_t)sizeof(T);
+        q.part_capacity = std::max<int64_t>(p.spreader.budget_bytes / per_part, 64);
+        q.light_limit = p.spreader.light_limit;
+        q.min_part = p.spreader.min_part;
+        // partial slots the problem's heavy parts can take: at most its
+        // capacity, and at most a part per heavy block plus its records in
+        // reach (each record counted by at most `reach` blocks) over the
+        // smallest part, with a factor two of margin
+        double reach = 1.0;
+        for (int a = 0; a < D; ++a) {
+            const int64_t B = sp.block[a], n = p.fine[a];
+            const int64_t side = (int64_t)1 << b.g.ax[a].shift;
+            const int64_t L = std::min<int64_t>(B + p.kernel.width - 1, n);
+            const int64_t all = sp.nblocks[a];
+            reach *= (double)(L + side > n
+                                  ? all
+                                  : std::min<int64_t>(all, (side + p.kernel.width + 2 * B - 3) / B + 1));
+        }
+        const double bound = (double)sp.block_count +
+                             2.0 * reach * (double)M[i] / (double)std::max<uint32_t>(q.min_part, 1);
+        k.slots += std::min<int64_t>(q.part_capacity, (int64_t)std::min(bound, 9.0e15));
+        k.blocks += sp.block_count;
+        k.p1 = (int)r + 1;
+        ModeProblem &mq = mtable[(size_t)r];
+        mq.mp = p.mode_params();
+        mq.inv_phihat = p.d_inv_phihat;
+        mq.mode_base = modes;
+        mq.fine_base = 0;
+        mq.zero = M[i] == 0 ? 1 : 0;
+        m_off += M[i];
+        bins += (uint64_t)b.g.nbins;
+        modes += p.mode_total;
+    }
+    if (m_off > 0xffffffffll) {
+        set_error("batched type 1: more than 2^32 points over the plans");
+        return false;
+    }
+    if (!upload_table(table, &head.batch_table, &head.batch_table_cap, stream) ||
+        !upload_table(stable, &head.batch_spread, &head.batch_spread_cap, stream) ||
+        !upload_table(mtable, &head.batch_modes, &head.batch_modes_cap, stream))
+        return false;
+    if (m_off > 0) {
+        if (!head.batch_sorter) head.batch_sorter = std::make_unique<BinSorter>();
+        BinSorter &bs = *head.batch_sorter;
+        if (!bs.sort_many<D, T>(m_off, head.batch_table, (int)count, bins, stream) ||
+            !head.ensure_records(m_off) || !head.ensure_strengths(m_off, 1) ||
+            !prepare_many<D, T>(m_off, bs.sorted(), head.batch_table, (int)count,
+                                head.rec_start, (T *)head.rec_off, head.rec_perm, stream) ||
+            !gather_strengths_many<T>(m_off, head.rec_perm, head.batch_table, (int)count,
+                                      (T *)head.c_sorted, stream) ||
+            !head.spreader.spread_many<D, T>(head.batch_table, head.batch_spread, classes,
+                                             bs.offsets(), head.rec_start,
+                                             (const T *)head.rec_off,
+                                             (const T *)head.c_sorted, stream))
+            return false;
+        for (int64_t r = 0; r < count; ++r) {
+            Plan &p = *plans[order[(size_t)r]];
+            if (M[order[(size_t)r]] > 0 && !p.fft(1, p.grid, stream)) return false;
+        }
+    }
+    return deconvolve_many<D, T>(head.batch_table, head.batch_modes, (int)count, modes, stream);
+}
+
+}  // namespace
+
+bool type1_many_valid(int64_t count, Plan *const *plans, const int64_t *M,
+                      const void *const *points, const void *const *inputs,
+                      void *const *outputs)
+{
+    if (count < 1 || !plans || !M || !points || !inputs || !outputs) {
+        set_error("batched type 1: invalid arguments");
+        return false;
+    }
+    const int dim = plans[0] ? plans[0]->dim : 0;
+    const Precision prec = plans[0] ? plans[0]->precision : Precision::F32;
+    for (int64_t i = 0; i < count; ++i) {
+        const Plan *p = plans[i];
+        if (!p || p->type != 1 || p->dim != dim || p->precision != prec || dim < 1 || dim > 3) {
+            set_error("batched type 1 takes type-1 plans of one rank (1D to 3D) and precision");
+            return false;
+        }
+        if (p->spread_method != SpreadMethod::Plain || p->rows_geometry() ||
+            p->point_order != BinOrder::Index || p->spread_params().run == 0) {
+            set_error("batched type 1 takes plans that spread on the output-stationary blocks "
+                      "in index order (plain spread; 3D plans with the row tiles off)");
+            return false;
+        }
+        if (M[i] < 0 || (M[i] > 0 && !points[i]) || (M[i] > 0 && !inputs[i]) || !outputs[i]) {
+            set_error("batched type 1: invalid point count or null buffer");
+            return false;
+        }
+        for (int64_t k = 0; k < i; ++k)
+            if (plans[k] == p) {
+                set_error("batched type 1: a plan appears twice");
+                return false;
+            }
+    }
+    return true;
+}
+
+bool execute_type1_many(int64_t count, Plan *const *plans, const int64_t *M,
+                        const void *const *points, const void *const *inputs,
+                        void *const *outputs, hipStream_t stream)
+{
+    if (!type1_many_valid(count, plans, M, points, inputs, outputs)) return false;
+    const int dim = plans[0]->dim;
+    const bool f32 = plans[0]->precision == Precision::F32;
+    switch (dim) {
+    case 1:
+        return f32 ? execute_type1_many_t<1, float>(count, plans, M, points, inputs, outputs, stream)
+                   : execute_type1_many_t<1, double>(count, plans, M, points, inputs, outputs, stream);
+    case 2:
+        return f32 ? execute_type1_many_t<2, float>(count, plans, M, points, inputs, outputs, stream)
+                   : execute_type1_many_t<2, double>(count, plans, M, points, inputs, outputs, stream);
+    default:
+        return f32 ? execute_type1_many_t<3, float>(count, plans, M, points, inputs, outputs, stream)
+                   : execute_type1_many_t<3, double>(count, plans, M, points, inputs, outputs, stream);
+    }
 }
 
 }  // namespace mnufft

@@ -184,6 +184,9 @@ struct Plan {
     std::unique_ptr<BinSorter> batch_sorter;
     BatchProblem *batch_table = nullptr;
     int64_t batch_table_cap = 0;
+    ModeProblem *batch_modes = nullptr;     // mode stages of the problems
+    SpreadProblem *batch_spread = nullptr;  // spread blocks of the problems (type 1)
+    int64_t batch_modes_cap = 0, batch_spread_cap = 0;
     bool bin_sort(int64_t M, const void *pts, uint32_t *sorted, uint32_t *offsets,
                   hipStream_t stream);
     // The same permutation and offsets as bin_sort, with every intermediate
@@ -222,6 +225,24 @@ struct Plan {
 // equal the plans' own encodes bit for bit.  plans[0] keeps the batch
 // scratch; the plans keep no points afterwards.
 bool execute_type2_many(int64_t count, Plan *const *plans, const int64_t *M,
+                        const void *const *points, const void *const *inputs,
+                        void *const *outputs, hipStream_t stream);
+
+// Type-1 encodes of `count` plans (one rank and precision, 1D to 3D, one
+// vector each) as one batch: plans[i] spreads inputs[i] (strengths) at
+// points[i] (M[i] points) and writes outputs[i] (modes).  One bin sort over
+// all the points, one prepare, one strength gather, per run length one
+// spread launch and one heavy-part reduction over every problem's blocks
+// (each problem's work plan made with its own plan's limits), each plan's
+// FFT, one deconvolution launch.  The plans must spread on the
+// output-stationary blocks in index order (the plain spread; 3D plans with
+// the row tiles switched off); outputs then equal the plans' own encodes
+// bit for bit.  plans[0] keeps the batch scratch; the plans keep no points.
+// The argument checks of execute_type1_many (last error set when false).
+bool type1_many_valid(int64_t count, Plan *const *plans, const int64_t *M,
+                      const void *const *points, const void *const *inputs,
+                      void *const *outputs);
+bool execute_type1_many(int64_t count, Plan *const *plans, const int64_t *M,
                         const void *const *points, const void *const *inputs,
                         void *const *outputs, hipStream_t stream);
 

@@ -54,15 +54,16 @@ __global__ __launch_bounds__(256) void k_classify(GridParams g, SpreadParams sp,
 
 // One workgroup plans the work items: a light block is one item, a heavy
 // block as many parts as its records need.  ctrl[0] = items, ctrl[1] = heavy
-// parts, ctrl[2] = heavy blocks, ctrl[3] = records per part.
-__global__ __launch_bounds__(1024) void k_plan(int64_t nblocks, const uint32_t *__restrict__ totals,
-                                               uint32_t light_limit, uint32_t min_part,
-                                               int64_t part_capacity,
-                                               uint32_t *__restrict__ parts,
-                                               uint32_t *__restrict__ item_off,
-                                               uint32_t *__restrict__ heavy_base,
-                                               uint32_t *__restrict__ heavy_list,
-                                               uint32_t *__restrict__ ctrl)
+// parts, ctrl[2] = heavy blocks, ctrl[3] = records per part.  (One
+// instantiation per kernel: each keeps its LDS variables to itself.)
+template <int Kernel>
+__device__ inline void plan_blocks(int64_t nblocks, const uint32_t *__restrict__ totals,
+                                   uint32_t light_limit, uint32_t min_part,
+                                   int64_t part_capacity, uint32_t *__restrict__ parts,
+                                   uint32_t *__restrict__ item_off,
+                                   uint32_t *__restrict__ heavy_base,
+                                   uint32_t *__restrict__ heavy_list,
+                                   uint32_t *__restrict__ ctrl)
 {
     __shared__ uint64_t s_heavy_records;
     __shared__ uint32_t s_heavy_blocks;
@@ -175,6 +176,57 @@ __global__ __launch_bounds__(1024) void k_plan(int64_t nblocks, const uint32_t *
     }
 }
 
+__global__ __launch_bounds__(1024) void k_plan(int64_t nblocks, const uint32_t *__restrict__ totals,
+                                               uint32_t light_limit, uint32_t min_part,
+                                               int64_t part_capacity,
+                                               uint32_t *__restrict__ parts,
+                                               uint32_t *__restrict__ item_off,
+                                               uint32_t *__restrict__ heavy_base,
+                                               uint32_t *__restrict__ heavy_list,
+                                               uint32_t *__restrict__ ctrl)
+{
+    plan_blocks<0>(nblocks, totals, light_limit, min_part, part_capacity, parts, item_off,
+                   heavy_base, heavy_list, ctrl);
+}
+
+// Many problems: workgroup q plans problem q's blocks alone, with its own
+// plan's limits, exactly as that plan's own spread does (its heavy list
+// holds its own block indices).  ctrl: 8 words per problem, cleared first.
+__global__ __launch_bounds__(1024) void k_plan_many(const SpreadProblem *__restrict__ sprobs,
+                                                    uint32_t *__restrict__ totals,
+                                                    uint32_t *__restrict__ parts,
+                                                    uint32_t *__restrict__ item_off,
+                                                    uint32_t *__restrict__ heavy_base,
+                                                    uint32_t *__restrict__ heavy_list,
+                                                    uint32_t *__restrict__ ctrl)
+{
+    const SpreadProblem &q = sprobs[blockIdx.x];
+    const int64_t bb = q.block_base;
+    plan_blocks<1>(q.sp.block_count, totals + bb, q.light_limit, q.min_part, q.part_capacity,
+                   parts + bb, item_off + bb, heavy_base + bb, heavy_list + bb,
+                   ctrl + 8 * (int64_t)blockIdx.x);
+}
+
+// Bases of the problems' work in the combined lists: base[4q] first item,
+// [4q + 1] first partial slot, [4q + 2] first heavy block; base[4n..4n+1]
+// the totals of items and heavy blocks.
+__global__ void k_plan_bases(int nprob, const uint32_t *__restrict__ ctrl,
+                             int64_t *__restrict__ base)
+{
+    if (threadIdx.x != 0) return;
+    int64_t items = 0, slots = 0, heavy = 0;
+    for (int q = 0; q < nprob; ++q) {
+        base[4 * q] = items;
+        base[4 * q + 1] = slots;
+        base[4 * q + 2] = heavy;
+        items += ctrl[8 * q];
+        slots += ctrl[8 * q + 1];
+        heavy += ctrl[8 * q + 2];
+    }
+    base[4 * nprob] = items;
+    base[4 * nprob + 1] = heavy;
+}
+
 // A staged record: wrapped starts, strength, the axis-0 weights padded for
 // runs that begin before or after the support, the weights of the other axes.
 template <typename T, int R, int D>
@@ -203,8 +255,226 @@ __device__ inline void load_starts(const int4 *__restrict__ rec_start, int64_t s
     }
 }
 
-// One work item: records [rlo, rhi) of block b's reach, for vector v.
-// Writes the block's cells (light) or a partial slot (heavy part).
+// LDS of one spreading workgroup.
+template <int D, typename T, int R>
+struct SpreadShared {
+    Staged<T, R, D> st[kLanes];
+    int64_t seg_first[kLanes];
+    int64_t seg_excl[kLanes + 1];
+};
+
+// Records [rlo, rhi) of the reach of block b (grid g, blocks sp), strengths
+// cv in record-slot order: the lane's cells summed in record order, written
+// to the block's cells of grid_v (np == 1) or to the lane's cells of the
+// partial slot part_v (a heavy part).
+template <int D, typename T, int R>
+__device__ __forceinline__ void spread_block(SpreadShared<D, T, R> &sh, const GridParams &g,
+                                    const SpreadParams &sp, int64_t b, int64_t rlo, int64_t rhi,
+                                    uint32_t np, const uint32_t *__restrict__ offsets,
+                                    const int4 *__restrict__ rec_start,
+                                    const T *__restrict__ rec_off, const T *__restrict__ cv,
+                                    T *__restrict__ grid_v, T *__restrict__ part_v)
+{
+    constexpr int RS = 4 * rec_quads<D>();  // offsets per record
+    Staged<T, R, D> *st = sh.st;
+    int64_t *seg_first = sh.seg_first;
+    int64_t *seg_excl = sh.seg_excl;
+    const int lane = threadIdx.x;
+    const int w = g.width;
+    const T beta = (T)g.beta;
+
+    // lane -> cells: run of R cells along axis 0 and one cell per other axis
+    int lane_x = lane % sp.groups0;
+    int rest_lane = lane / sp.groups0;
+    int lane_c[D];
+    lane_c[0] = lane_x * R;
+#pragma unroll
+    for (int a = 1; a < D; ++a) {
+        lane_c[a] = rest_lane % (int)sp.block[a];
+        rest_lane /= (int)sp.block[a];
+    }
+
+    BlockReach<D> r = block_reach<D>(b, g, sp);
+    int64_t cell[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) cell[a] = r.b0[a] + lane_c[a];
+
+    T acc_re[R], acc_im[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) acc_re[i] = acc_im[i] = (T)0;
+
+    int64_t cursor = 0;
+    for (int64_t sb = 0; sb < r.nseg && cursor < rhi; sb += kLanes) {
+        int64_t t = sb + lane;
+        int64_t f = 0, l = 0;
+        if (t < r.nseg) r.segment(t, g, offsets, &f, &l);
+        int64_t cnt = l - f;
+        // inclusive wave scan of segment lengths
+        int64_t x = cnt;
+#pragma unroll
+        for (int o = 1; o < kLanes; o <<= 1) {
+            int64_t y = __shfl_up(x, o, kLanes);
+            if (lane >= o) x += y;
+        }
+        int64_t batch_total = __shfl(x, kLanes - 1, kLanes);
+        seg_first[lane] = f;
+        seg_excl[lane] = x - cnt;
+        if (lane == 0) seg_excl[kLanes] = batch_total;
+        __syncthreads();
+        int64_t klo = rlo > cursor ? rlo - cursor : 0;
+        int64_t khi = rhi - cursor < batch_total ? rhi - cursor : batch_total;
+        for (int64_t k0 = klo; k0 < khi; k0 += kLanes) {
+            int64_t k = k0 + lane;
+            bool touch = false;
+            int64_t slot = 0;
+            if (k < khi) {
+                int s_lo = 0, s_hi = kLanes - 1;
+                while (s_lo < s_hi) {
+                    int mid = (s_lo + s_hi + 1) >> 1;
+                    if (seg_excl[mid] <= k)
+                        s_lo = mid;
+                    else
+                        s_hi = mid - 1;
+                }
+                slot = seg_first[s_lo] + (k - seg_excl[s_lo]);
+                int ss[D];
+                load_starts<D>(rec_start, slot, ss);
+                touch = true;
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    int64_t n = g.ax[a].n;
+                    int64_t e = r.b0[a] - ss[a];
+                    if (e < 0) e += n;
+                    touch = touch && (e < w || e > n - sp.block[a]);
+                }
+            }
+            // compact the touching records into LDS
+            uint64_t mask = __ballot(touch);
+            int pos = __popcll(mask & ((1ull << lane) - 1ull));
+            int nstaged = __popcll(mask);
+            if (touch) {
+                Staged<T, R, D> &S = st[pos];
+                int ss[D];
+                load_starts<D>(rec_start, slot, ss);
+#pragma unroll
+                for (int a = 0; a < D; ++a) S.s[a] = ss[a];
+                S.cre = cv[2 * slot];
+                S.cim = cv[2 * slot + 1];
+                T ox = rec_off[RS * slot + 0];
+#pragma unroll
+                for (int j = 0; j < 2 * R + kKw; ++j) {
+                    int tt = j - (R - 1);
+                    S.kxp[j] = (tt >= 0 && tt < w) ? es_weight<T>((T)tt + ox, w, beta) : (T)0;
+                }
+#pragma unroll
+                for (int a = 1; a < D; ++a) {
+                    T oa = rec_off[RS * slot + a];
+                    for (int j = 0; j < w; ++j) S.kr[a - 1][j] = es_weight<T>((T)j + oa, w, beta);
+                }
+            }
+            __syncthreads();
+            for (int q = 0; q < nstaged; ++q) {
+                const Staged<T, R, D> &S = st[q];
+                // product of the other axes' weights, axis 1 first
+                T wyz = (T)1;
+                bool on = true;
+#pragma unroll
+                for (int a = 1; a < D; ++a) {
+                    int64_t da = cell[a] - S.s[a];
+                    if (da < 0) da += g.ax[a].n;
+                    on = on && da < w;
+                    if (on) wyz = wyz * S.kr[a - 1][da];
+                }
+                if (!on) continue;
+                int64_t nx = g.ax[0].n;
+                int64_t e = cell[0] - S.s[0];
+                if (e < 0) e += nx;
+                int base;
+                if (e < w)
+                    base = (int)e + R - 1;
+                else if (e > nx - R)
+                    base = (int)(e - nx) + R - 1;
+                else
+                    continue;
+                T vre = S.cre * wyz, vim = S.cim * wyz;
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    T wx = S.kxp[base + i];
+                    acc_re[i] += wx * vre;
+                    acc_im[i] += wx * vim;
+                }
+            }
+            __syncthreads();
+        }
+        cursor += batch_total;
+        __syncthreads();
+    }
+
+    // write the lane's cells
+    if (np == 1) {
+        bool inside = true;
+        int64_t lin = 0, stride = 1;
+#pragma unroll
+        for (int a = 1; a < D; ++a) {
+            inside = inside && cell[a] < g.ax[a].n;
+            stride *= g.ax[a - 1].n;
+            lin += cell[a] * stride;
+        }
+        if (inside) {
+            T *out = grid_v + 2 * lin;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                int64_t xc = cell[0] + i;
+                if (xc < g.ax[0].n) {
+                    out[2 * xc] = acc_re[i];
+                    out[2 * xc + 1] = acc_im[i];
+                }
+            }
+        }
+    } else {
+        T *out = part_v + 2 * ((int64_t)lane * R);
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            out[2 * i] = acc_re[i];
+            out[2 * i + 1] = acc_im[i];
+        }
+    }
+    __syncthreads();
+}
+
+// Record range of work item `item` of the plan: block b, part p of np.
+__device__ inline void item_records(int64_t item, int64_t nblocks,
+                                    const uint32_t *__restrict__ item_off,
+                                    const uint32_t *__restrict__ parts,
+                                    const uint32_t *__restrict__ totals, uint32_t part_len_cap,
+                                    int64_t *b, int64_t *p, uint32_t *np, int64_t *rlo,
+                                    int64_t *rhi)
+{
+    // block of this item: last b with item_off[b] <= item
+    int64_t lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if ((int64_t)item_off[mid] <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    *b = lo;
+    *p = item - item_off[lo];
+    *np = parts[lo];
+    const int64_t total = totals[lo];
+    *rlo = 0;
+    *rhi = total;
+    if (*np > 1) {
+        const int64_t len = part_len_cap;
+        *rlo = *p * len;
+        *rhi = *rlo + len < total ? *rlo + len : total;
+    }
+}
+
+// One work item per loop turn: records [rlo, rhi) of block b's reach, for
+// vector v = blockIdx.y; writes the block's cells (light) or a partial slot
+// (heavy part).
 template <int D, typename T, int R>
 __global__ __launch_bounds__(kLanes) void k_spread(GridParams g, SpreadParams sp,
                                                    const uint32_t *__restrict__ offsets,
@@ -219,197 +489,20 @@ __global__ __launch_bounds__(kLanes) void k_spread(GridParams g, SpreadParams sp
                                                    T *__restrict__ grid, T *__restrict__ partial,
                                                    int64_t block_cells)
 {
-    __shared__ Staged<T, R, D> st[kLanes];
-    constexpr int RS = 4 * rec_quads<D>();  // offsets per record
-    __shared__ int64_t seg_first[kLanes];
-    __shared__ int64_t seg_excl[kLanes + 1];
-
-    const int lane = threadIdx.x;
+    __shared__ SpreadShared<D, T, R> sh;
     const int64_t v = blockIdx.y;
-    const int w = g.width;
-    const T beta = (T)g.beta;
     const int64_t n_items = ctrl[0];
     const uint32_t part_len_cap = ctrl[3];
     const T *cv = c_sorted + 2 * v * M;
-
-    // lane -> cells: run of R cells along axis 0 and one cell per other axis
-    int lane_x = lane % sp.groups0;
-    int rest_lane = lane / sp.groups0;
-    int lane_c[D];
-    lane_c[0] = lane_x * R;
-#pragma unroll
-    for (int a = 1; a < D; ++a) {
-        lane_c[a] = rest_lane % (int)sp.block[a];
-        rest_lane /= (int)sp.block[a];
-    }
-
     for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-        // block of this item: last b with item_off[b] <= item
-        int64_t lo = 0, hi = sp.block_count - 1;
-        while (lo < hi) {
-            int64_t mid = (lo + hi + 1) >> 1;
-            if ((int64_t)item_off[mid] <= item)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const int64_t b = lo;
-        const int64_t p = item - item_off[b];
-        const uint32_t np = parts[b];
-        const int64_t total = totals[b];
-        int64_t rlo = 0, rhi = total;
-        if (np > 1) {
-            int64_t len = part_len_cap;
-            rlo = p * len;
-            rhi = rlo + len < total ? rlo + len : total;
-        }
-        BlockReach<D> r = block_reach<D>(b, g, sp);
-        int64_t cell[D];
-#pragma unroll
-        for (int a = 0; a < D; ++a) cell[a] = r.b0[a] + lane_c[a];
-
-        T acc_re[R], acc_im[R];
-#pragma unroll
-        for (int i = 0; i < R; ++i) acc_re[i] = acc_im[i] = (T)0;
-
-        int64_t cursor = 0;
-        for (int64_t sb = 0; sb < r.nseg && cursor < rhi; sb += kLanes) {
-            int64_t t = sb + lane;
-            int64_t f = 0, l = 0;
-            if (t < r.nseg) r.segment(t, g, offsets, &f, &l);
-            int64_t cnt = l - f;
-            // inclusive wave scan of segment lengths
-            int64_t x = cnt;
-#pragma unroll
-            for (int o = 1; o < kLanes; o <<= 1) {
-                int64_t y = __shfl_up(x, o, kLanes);
-                if (lane >= o) x += y;
-            }
-            int64_t batch_total = __shfl(x, kLanes - 1, kLanes);
-            seg_first[lane] = f;
-            seg_excl[lane] = x - cnt;
-            if (lane == 0) seg_excl[kLanes] = batch_total;
-            __syncthreads();
-            int64_t klo = rlo > cursor ? rlo - cursor : 0;
-            int64_t khi = rhi - cursor < batch_total ? rhi - cursor : batch_total;
-            for (int64_t k0 = klo; k0 < khi; k0 += kLanes) {
-                int64_t k = k0 + lane;
-                bool touch = false;
-                int64_t slot = 0;
-                if (k < khi) {
-                    int s_lo = 0, s_hi = kLanes - 1;
-                    while (s_lo < s_hi) {
-                        int mid = (s_lo + s_hi + 1) >> 1;
-                        if (seg_excl[mid] <= k)
-                            s_lo = mid;
-                        else
-                            s_hi = mid - 1;
-                    }
-                    slot = seg_first[s_lo] + (k - seg_excl[s_lo]);
-                    int ss[D];
-                    load_starts<D>(rec_start, slot, ss);
-                    touch = true;
-#pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        int64_t n = g.ax[a].n;
-                        int64_t e = r.b0[a] - ss[a];
-                        if (e < 0) e += n;
-                        touch = touch && (e < w || e > n - sp.block[a]);
-                    }
-                }
-                // compact the touching records into LDS
-                uint64_t mask = __ballot(touch);
-                int pos = __popcll(mask & ((1ull << lane) - 1ull));
-                int nstaged = __popcll(mask);
-                if (touch) {
-                    Staged<T, R, D> &S = st[pos];
-                    int ss[D];
-                    load_starts<D>(rec_start, slot, ss);
-#pragma unroll
-                    for (int a = 0; a < D; ++a) S.s[a] = ss[a];
-                    S.cre = cv[2 * slot];
-                    S.cim = cv[2 * slot + 1];
-                    T ox = rec_off[RS * slot + 0];
-#pragma unroll
-                    for (int j = 0; j < 2 * R + kKw; ++j) {
-                        int tt = j - (R - 1);
-                        S.kxp[j] = (tt >= 0 && tt < w) ? es_weight<T>((T)tt + ox, w, beta) : (T)0;
-                    }
-#pragma unroll
-                    for (int a = 1; a < D; ++a) {
-                        T oa = rec_off[RS * slot + a];
-                        for (int j = 0; j < w; ++j) S.kr[a - 1][j] = es_weight<T>((T)j + oa, w, beta);
-                    }
-                }
-                __syncthreads();
-                for (int q = 0; q < nstaged; ++q) {
-                    const Staged<T, R, D> &S = st[q];
-                    // product of the other axes' weights, axis 1 first
-                    T wyz = (T)1;
-                    bool on = true;
-#pragma unroll
-                    for (int a = 1; a < D; ++a) {
-                        int64_t da = cell[a] - S.s[a];
-                        if (da < 0) da += g.ax[a].n;
-                        on = on && da < w;
-                        if (on) wyz = wyz * S.kr[a - 1][da];
-                    }
-                    if (!on) continue;
-                    int64_t nx = g.ax[0].n;
-                    int64_t e = cell[0] - S.s[0];
-                    if (e < 0) e += nx;
-                    int base;
-                    if (e < w)
-                        base = (int)e + R - 1;
-                    else if (e > nx - R)
-                        base = (int)(e - nx) + R - 1;
-                    else
-                        continue;
-                    T vre = S.cre * wyz, vim = S.cim * wyz;
-#pragma unroll
-                    for (int i = 0; i < R; ++i) {
-                        T wx = S.kxp[base + i];
-                        acc_re[i] += wx * vre;
-                        acc_im[i] += wx * vim;
-                    }
-                }
-                __syncthreads();
-            }
-            cursor += batch_total;
-            __syncthreads();
-        }
-
-        // write the lane's cells
-        if (np == 1) {
-            bool inside = true;
-            int64_t lin = 0, stride = 1;
-#pragma unroll
-            for (int a = 1; a < D; ++a) {
-                inside = inside && cell[a] < g.ax[a].n;
-                stride *= g.ax[a - 1].n;
-                lin += cell[a] * stride;
-            }
-            if (inside) {
-                T *out = grid + 2 * (v * g.fine_total + lin);
-#pragma unroll
-                for (int i = 0; i < R; ++i) {
-                    int64_t xc = cell[0] + i;
-                    if (xc < g.ax[0].n) {
-                        out[2 * xc] = acc_re[i];
-                        out[2 * xc + 1] = acc_im[i];
-                    }
-                }
-            }
-        } else {
-            int64_t slot = (int64_t)heavy_base[b] + p;
-            T *out = partial + 2 * ((slot * gridDim.y + v) * block_cells + (int64_t)lane * R);
-#pragma unroll
-            for (int i = 0; i < R; ++i) {
-                out[2 * i] = acc_re[i];
-                out[2 * i + 1] = acc_im[i];
-            }
-        }
-        __syncthreads();
+        int64_t b, p, rlo, rhi;
+        uint32_t np;
+        item_records(item, sp.block_count, item_off, parts, totals, part_len_cap, &b, &p, &np,
+                     &rlo, &rhi);
+        const int64_t slot = (int64_t)heavy_base[b] + p;
+        spread_block<D, T, R>(sh, g, sp, b, rlo, rhi, np, offsets, rec_start, rec_off, cv,
+                              grid + 2 * v * g.fine_total,
+                              partial + 2 * (slot * gridDim.y + v) * block_cells);
     }
 }
 
@@ -458,6 +551,43 @@ __device__ inline void combine_lanes(const PartSums<T> &s, int u, T &re, T &im)
     im = i[0];
 }
 
+// Cell c (= lane * R + i) of block b: its sum written to grid_v.
+template <int D, typename T, int R>
+__device__ inline void write_block_cell(const GridParams &g, const SpreadParams &sp, int64_t b,
+                                        int64_t c, T re, T im, T *__restrict__ grid_v)
+{
+    int lane = (int)(c / R), i = (int)(c % R);
+    int lane_x = lane % sp.groups0;
+    int rl = lane / sp.groups0;
+    int64_t rest = b;
+    int64_t cell[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        int64_t bi = rest % sp.nblocks[a];
+        rest /= sp.nblocks[a];
+        cell[a] = bi * sp.block[a];
+    }
+    cell[0] += lane_x * R + i;
+#pragma unroll
+    for (int a = 1; a < D; ++a) {
+        cell[a] += rl % (int)sp.block[a];
+        rl /= (int)sp.block[a];
+    }
+    bool inside = true;
+    int64_t lin = 0, stride = 1;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        inside = inside && cell[a] < g.ax[a].n;
+        lin += cell[a] * stride;
+        stride *= g.ax[a].n;
+    }
+    if (inside) {
+        T *out = grid_v + 2 * lin;
+        out[0] = re;
+        out[1] = im;
+    }
+}
+
 template <int D, typename T, int R>
 __global__ __launch_bounds__(256) void k_reduce(GridParams g, SpreadParams sp,
                                                 const uint32_t *__restrict__ parts,
@@ -490,37 +620,133 @@ __global__ __launch_bounds__(256) void k_reduce(GridParams g, SpreadParams sp,
         __syncthreads();
         if (pl == 0 && t < work) {
             combine_lanes(ps, u, sre, sim);
-            // cell coordinates
-            int lane = (int)(c / R), i = (int)(c % R);
-            int lane_x = lane % sp.groups0;
-            int rl = lane / sp.groups0;
-            int64_t rest = b;
-            int64_t cell[D];
+            write_block_cell<D, T, R>(g, sp, b, c, sre, sim, grid + 2 * v * g.fine_total);
+        }
+        __syncthreads();
+    }
+}
+
+// Problems of a multi-plan type-1 spread: the problem of class block gb
+// (last problem whose block_base <= gb).
+__device__ inline int problem_of_block(const SpreadProblem *__restrict__ sprobs, int nprob,
+                                       int64_t gb)
+{
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sprobs[mid].block_base <= gb)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// k_classify over the blocks of every problem of a class.
+template <int D>
+__global__ __launch_bounds__(256) void k_classify_many(const BatchProblem *__restrict__ probs,
+                                                       const SpreadProblem *__restrict__ sprobs,
+                                                       int nprob, int64_t nblocks,
+                                                       const uint32_t *__restrict__ offsets,
+                                                       uint32_t *__restrict__ totals)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t gb = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (gb >= nblocks) return;
+    const int q = problem_of_block(sprobs, nprob, gb);
+    const BatchProblem &pb = probs[q];
+    const SpreadProblem &spb = sprobs[q];
+    BlockReach<D> r = block_reach<D>(gb - spb.block_base, pb.g, spb.sp);
+    const uint32_t *off = offsets + pb.bin_base;
+    uint64_t total = 0;
+    for (int64_t t = lane; t < r.nseg; t += 64) {
+        int64_t f, l;
+        r.segment(t, pb.g, off, &f, &l);
+        total += (uint64_t)(l - f);
+    }
 #pragma unroll
-            for (int a = 0; a < D; ++a) {
-                int64_t bi = rest % sp.nblocks[a];
-                rest /= sp.nblocks[a];
-                cell[a] = bi * sp.block[a];
-            }
-            cell[0] += lane_x * R + i;
-#pragma unroll
-            for (int a = 1; a < D; ++a) {
-                cell[a] += rl % (int)sp.block[a];
-                rl /= (int)sp.block[a];
-            }
-            bool inside = true;
-            int64_t lin = 0, stride = 1;
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                inside = inside && cell[a] < g.ax[a].n;
-                lin += cell[a] * stride;
-                stride *= g.ax[a].n;
-            }
-            if (inside) {
-                T *out = grid + 2 * (v * g.fine_total + lin);
-                out[0] = sre;
-                out[1] = sim;
-            }
+    for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o, 64);
+    if (lane == 0) totals[gb] = (uint32_t)(total > 0xffffffffull ? 0xffffffffull : total);
+}
+
+// Problem of combined element t: the last problem whose base[4q + k] <= t.
+__device__ inline int problem_by_base(const int64_t *__restrict__ base, int nprob, int k,
+                                      int64_t t)
+{
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[4 * mid + k] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// k_spread over the work items of every problem of a class (one vector):
+// item -> problem (item bases) -> the problem's block and part, summed as
+// the problem's own spread sums it.
+template <int D, typename T, int R>
+__global__ __launch_bounds__(kLanes) void k_spread_many(
+    const BatchProblem *__restrict__ probs, const SpreadProblem *__restrict__ sprobs, int nprob,
+    const uint32_t *__restrict__ offsets, const int4 *__restrict__ rec_start,
+    const T *__restrict__ rec_off, const T *__restrict__ c_sorted,
+    const uint32_t *__restrict__ totals, const uint32_t *__restrict__ parts,
+    const uint32_t *__restrict__ item_off, const uint32_t *__restrict__ heavy_base,
+    const uint32_t *__restrict__ pctrl, const int64_t *__restrict__ pbase,
+    T *__restrict__ partial, int64_t block_cells)
+{
+    __shared__ SpreadShared<D, T, R> sh;
+    const int64_t n_items = pbase[4 * nprob];
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const int q = problem_by_base(pbase, nprob, 0, item);
+        const BatchProblem &pb = probs[q];
+        const SpreadProblem &spb = sprobs[q];
+        const int64_t bb = spb.block_base;
+        int64_t b, p, rlo, rhi;
+        uint32_t np;
+        item_records(item - pbase[4 * q], spb.sp.block_count, item_off + bb, parts + bb,
+                     totals + bb, pctrl[8 * q + 3], &b, &p, &np, &rlo, &rhi);
+        const int64_t slot = pbase[4 * q + 1] + (int64_t)heavy_base[bb + b] + p;
+        spread_block<D, T, R>(sh, pb.g, spb.sp, b, rlo, rhi, np, offsets + pb.bin_base, rec_start,
+                              rec_off, c_sorted, (T *)pb.grid, partial + 2 * slot * block_cells);
+    }
+}
+
+// k_reduce over the heavy blocks of every problem of a class (one vector).
+template <int D, typename T, int R>
+__global__ __launch_bounds__(256) void k_reduce_many(
+    const BatchProblem *__restrict__ probs, const SpreadProblem *__restrict__ sprobs, int nprob,
+    const uint32_t *__restrict__ parts, const uint32_t *__restrict__ heavy_base,
+    const uint32_t *__restrict__ heavy_list, const int64_t *__restrict__ pbase,
+    const T *__restrict__ partial, int64_t block_cells)
+{
+    __shared__ PartSums<T> ps;
+    const int64_t work = pbase[4 * nprob + 1] * block_cells;
+    const int u = threadIdx.x % kPartLanes, pl = threadIdx.x / kPartLanes;
+    for (int64_t t0 = (int64_t)blockIdx.x * kPartLanes; t0 < work;
+         t0 += (int64_t)gridDim.x * kPartLanes) {
+        const int64_t t = t0 + u;
+        int64_t c = 0, b = 0;
+        int q = 0;
+        T sre = (T)0, sim = (T)0;
+        if (t < work) {
+            const int64_t hb = t / block_cells;
+            c = t - hb * block_cells;
+            q = problem_by_base(pbase, nprob, 2, hb);
+            const int64_t bb = sprobs[q].block_base;
+            b = heavy_list[bb + (hb - pbase[4 * q + 2])];
+            part_lane_sum(partial, pbase[4 * q + 1] + (int64_t)heavy_base[bb + b],
+                          (int64_t)parts[bb + b], 1, 0, block_cells, c, pl, sre, sim);
+        }
+        ps.re[pl][u] = sre;
+        ps.im[pl][u] = sim;
+        __syncthreads();
+        if (pl == 0 && t < work) {
+            combine_lanes(ps, u, sre, sim);
+            write_block_cell<D, T, R>(probs[q].g, sprobs[q].sp, b, c, sre, sim,
+                                      (T *)probs[q].grid);
         }
         __syncthreads();
     }
@@ -662,9 +888,12 @@ Spreader::~Spreader() { release(); }
 void Spreader::release()
 {
     for (void *p : {(void *)totals, (void *)parts, (void *)item_off, (void *)heavy_base,
-                    (void *)heavy_list, (void *)ctrl, (void *)vmax, partial})
+                    (void *)heavy_list, (void *)ctrl, (void *)vmax, (void *)pctrl, (void *)pbase,
+                    partial})
         if (p) (void)hipFree(p);
-    totals = parts = item_off = heavy_base = heavy_list = ctrl = vmax = nullptr;
+    totals = parts = item_off = heavy_base = heavy_list = ctrl = vmax = pctrl = nullptr;
+    pbase = nullptr;
+    problems_cap = 0;
     partial = nullptr;
     blocks_cap = 0;
     partial_bytes = 0;
@@ -808,11 +1037,93 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
     return true;
 }
 
+template <int D, typename T>
+bool Spreader::spread_many(const BatchProblem *probs, const SpreadProblem *sprobs,
+                           const std::vector<SpreadClass> &classes, const uint32_t *offsets,
+                           const int4 *rec_start, const T *rec_off, const T *c_sorted,
+                           hipStream_t stream)
+{
+    // scratch for the largest class first: nothing is freed between launches
+    int64_t most_blocks = 1, most_partial = 0, most_problems = 1;
+    for (const SpreadClass &k : classes) {
+        const int R = k.run;
+        if (!(R == 2 || R == 4 || R == 8 || (D >= 4 && R == 16))) {
+            set_error("batched spread: unsupported run");
+            return false;
+        }
+        const int64_t per_part = (int64_t)kLanes * R * 2 * (int64_t)sizeof(T);
+        most_blocks = std::max(most_blocks, k.blocks);
+        most_partial = std::max(most_partial, std::max<int64_t>(k.slots, 1) * per_part);
+        most_problems = std::max<int64_t>(most_problems, k.p1 - k.p0);
+    }
+    if (!ensure(most_blocks, most_partial, 1)) return false;
+    if (most_problems > problems_cap) {
+        if (pctrl) (void)hipFree(pctrl);
+        if (pbase) (void)hipFree(pbase);
+        pctrl = nullptr;
+        pbase = nullptr;
+        problems_cap = 0;
+        if (!hip_ok(hipMalloc(&pctrl, sizeof(uint32_t) * 8 * most_problems),
+                    "batched spread counters") ||
+            !hip_ok(hipMalloc(&pbase, sizeof(int64_t) * (4 * most_problems + 4)),
+                    "batched spread bases"))
+            return false;
+        problems_cap = most_problems;
+    }
+    for (const SpreadClass &k : classes) {
+        const int R = k.run;
+        const int64_t nb = k.blocks;
+        const int n = k.p1 - k.p0;
+        if (nb == 0 || n == 0) continue;
+        const int64_t cells = (int64_t)kLanes * R;
+        const BatchProblem *pb = probs + k.p0;
+        const SpreadProblem *sb = sprobs + k.p0;
+        k_classify_many<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(pb, sb, n, nb, offsets,
+                                                                      totals);
+        k_fill_u32s<<<grid_for(8 * n, 256), 256, 0, stream>>>(pctrl, 8 * n);
+        k_plan_many<<<(unsigned)n, 1024, 0, stream>>>(sb, totals, parts, item_off, heavy_base,
+                                                      heavy_list, pctrl);
+        k_plan_bases<<<1, 64, 0, stream>>>(n, pctrl, pbase);
+        const dim3 ig = items_grid(nb, 1);
+        const dim3 rg(2048, 1);
+#define MNUFFT_SPREAD_MANY(RR)                                                                    \
+    do {                                                                                          \
+        k_spread_many<D, T, RR><<<ig, kLanes, 0, stream>>>(pb, sb, n, offsets, rec_start,         \
+                                                           rec_off, c_sorted, totals, parts,      \
+                                                           item_off, heavy_base, pctrl, pbase,    \
+                                                           (T *)partial, cells);                  \
+        k_reduce_many<D, T, RR><<<rg, 256, 0, stream>>>(pb, sb, n, parts, heavy_base, heavy_list, \
+                                                        pbase, (const T *)partial, cells);        \
+    } while (0)
+        switch (R) {
+        case 2: MNUFFT_SPREAD_MANY(2); break;
+        case 4: MNUFFT_SPREAD_MANY(4); break;
+        case 8: MNUFFT_SPREAD_MANY(8); break;
+        default:
+            if constexpr (D >= 4) MNUFFT_SPREAD_MANY(16);
+            break;
+        }
+#undef MNUFFT_SPREAD_MANY
+        if (!hip_ok(hipGetLastError(), "batched spread launch")) return false;
+    }
+    return true;
+}
+
 #define MNUFFT_SPREAD_INSTANTIATE(D, T)                                                           \
     template bool Spreader::spread<D, T>(const GridParams &, const SpreadParams &,                 \
                                          const uint32_t *, const int4 *, const T *, const T *,     \
                                          int64_t, int64_t, T *, hipStream_t, SpreadMethod,         \
                                          DenseBins *);
+#define MNUFFT_SPREAD_MANY_INSTANTIATE(D, T)                                                      \
+    template bool Spreader::spread_many<D, T>(const BatchProblem *, const SpreadProblem *,        \
+                                              const std::vector<SpreadClass> &, const uint32_t *, \
+                                              const int4 *, const T *, const T *, hipStream_t);
+MNUFFT_SPREAD_MANY_INSTANTIATE(1, float)
+MNUFFT_SPREAD_MANY_INSTANTIATE(2, float)
+MNUFFT_SPREAD_MANY_INSTANTIATE(3, float)
+MNUFFT_SPREAD_MANY_INSTANTIATE(1, double)
+MNUFFT_SPREAD_MANY_INSTANTIATE(2, double)
+MNUFFT_SPREAD_MANY_INSTANTIATE(3, double)
 MNUFFT_SPREAD_INSTANTIATE(1, float)
 MNUFFT_SPREAD_INSTANTIATE(2, float)
 MNUFFT_SPREAD_INSTANTIATE(3, float)

@@ -1,6 +1,8 @@
 // Deterministic output-stationary type-1 spreader.
 #pragma once
 
+#include <vector>
+
 #include "nufft_internal.hpp"
 
 namespace mnufft {
@@ -49,6 +51,27 @@ struct DenseBins {
     int64_t bins_cap = 0, parts_alloc = 0, win_bytes = 0, tiles_cap = 0;
 };
 
+// Output-stationary blocks of one problem of a multi-plan type-1 spread:
+// its blocks take indices [block_base, block_base + sp.block_count) of the
+// combined block list of its class (problems of one run length).
+// Its work plan uses its own plan's limits (light limit, smallest part,
+// partial slots), so its heavy blocks split exactly as in its own spread.
+struct SpreadProblem {
+    SpreadParams sp;
+    int64_t block_base;
+    int64_t part_capacity;
+    uint32_t light_limit, min_part;
+};
+
+// A class of consecutive problems [p0, p1) of one run length: its blocks
+// and a bound on the partial slots its heavy parts take.
+struct SpreadClass {
+    int p0, p1;
+    int run;
+    int64_t blocks;
+    int64_t slots;
+};
+
 // Device buffers of the spreader's work plan, reused across executions.
 struct Spreader {
     ~Spreader();
@@ -67,6 +90,18 @@ struct Spreader {
                 int64_t vectors, T *grid, hipStream_t stream,
                 SpreadMethod method = SpreadMethod::Plain, DenseBins *dense = nullptr);
 
+    // Many problems (one vector each; table `probs` / `sprobs` on the
+    // device, classes on the host): each class's blocks get one work plan
+    // over all its problems, one spread launch and one heavy-part
+    // reduction; every problem's grid is written cell by cell as its own
+    // spread writes it.  offsets: the combined bin offsets of the batched
+    // sort; records and strengths in combined slot order.
+    template <int D, typename T>
+    bool spread_many(const BatchProblem *probs, const SpreadProblem *sprobs,
+                     const std::vector<SpreadClass> &classes, const uint32_t *offsets,
+                     const int4 *rec_start, const T *rec_off, const T *c_sorted,
+                     hipStream_t stream);
+
     // Optional events recorded right before the first and after the last
     // spread kernel launch.
     hipEvent_t kernel_start = nullptr, kernel_end = nullptr;
@@ -78,6 +113,9 @@ struct Spreader {
     uint32_t *heavy_list = nullptr;  // heavy blocks
     uint32_t *ctrl = nullptr;        // items, heavy parts, heavy blocks, part size
     uint32_t *vmax = nullptr;        // largest strength magnitude per vector (float bits)
+    uint32_t *pctrl = nullptr;       // many problems: work-plan counters, 8 per problem
+    int64_t *pbase = nullptr;        // many problems: item, slot, heavy-block bases
+    int64_t problems_cap = 0;
     void *partial = nullptr;
     int64_t blocks_cap = 0;
     int64_t partial_bytes = 0;
