@@ -250,6 +250,40 @@ def test_config4a_batch_matches_oracle_and_per_plan_encodes():
     _say(f"config4a: 64 problems equal their own encodes bitwise; worst rel l2 {worst:.3e}")
 
 
+def test_config4a_points_as_type1_batch_matches_oracle_and_per_plan_encodes():
+    """The same 64 point sets as type-1 problems (strengths -> modes) in one
+    batch: each output equals its plan's own encode bit for bit and the
+    FP64 direct sums on sampled modes (2D tolerance 20 eps)."""
+    torch, W, O, S = _setup()
+    _say("CASE config4a points as 64 2D type-1 problems, one batch")
+    plans, xs, cs, hx, hc, dims = [], [], [], [], [], []
+    for i, (n0, n1, M, sx, _) in enumerate(S.batch_problem_specs()):
+        x = S.uniform_coords(sx, M, 2)
+        c = S.complex_normal(S.config_seed(4, 300 + i), M)
+        plans.append(W.NufftPlan.type1_gpu(W.NufftConfig([n0, n1], 1e-5)))
+        xs.append(torch.from_numpy(x).cuda())
+        cs.append(torch.from_numpy(c).cuda())
+        hx.append(x)
+        hc.append(c)
+        dims.append([n0, n1])
+    outs = W.encode_type1_many(plans, xs, cs)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for i, (p, x, c, o) in enumerate(zip(plans, xs, cs, outs)):
+        own = p.encode_gpu(x, c)
+        torch.cuda.synchronize()
+        assert torch.equal(_bits(own), _bits(o)), f"problem {i}"
+        if i % 8 == 0:
+            nm = dims[i][0] * dims[i][1]
+            sel = _sample(S, S.config_seed(4, 400 + i), 48, nm)
+            ref = O.direct_type1(hx[i].astype(np.float64), hc[i].astype(np.complex128), dims[i],
+                                 modes=sel)
+            err = rel_l2(o.cpu().numpy()[sel], ref)
+            worst = max(worst, err)
+            assert err <= 20 * 1e-5, f"problem {i}: {err}"
+    _say(f"config4a type 1: 64 problems equal their own encodes bitwise; worst rel l2 {worst:.3e}")
+
+
 # ---------------------------------------------------------------------------
 # Config 4b: 32 vectors over 8192 radial spokes x 512 samples, 256^3 modes.
 # ---------------------------------------------------------------------------
@@ -328,7 +362,8 @@ def test_config5_full_interval_sampled_oracle(prec, eps):
 # The GPU FP64 direct sums the benchmark's error figures use.
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("n_modes,order", [([23], 0), ([24], 1), ([8, 7, 6], 0),
-                                           ([8, 7, 6], 1), ([5, 6, 4], 0)])
+                                           ([8, 7, 6], 1), ([5, 6, 4], 0), ([5, 4, 3, 3], 1),
+                                           ([3, 2, 3, 2, 2, 3], 0), ([2] * 8, 1)])
 def test_gpu_direct_sums_match_oracle_1d_3d(n_modes, order):
     torch, W, O, S = _setup()
     d = len(n_modes)

@@ -53,11 +53,12 @@ int log2i(int64_t v)
 SpreadParams make_spread_params(int d, const int64_t *fine, int width, bool f64)
 {
     SpreadParams sp{};
-    // cells per lane along axis 0: 8 in 3D, 4 in 2D, 4 in 1D, from 4D on 16
-    // (F32) or 8 (F64) as the reference's max_run (src/gpu_nd_spread.rs:
-    // 575-582); never more than n0 + 1 - w, so a run sees every support
-    // offset at most once
-    int run = d == 3 ? 8 : (d >= 4 ? (f64 ? 8 : 16) : 4);
+    // cells per lane along axis 0: 8 in 3D, 4 in 1D, in 2D 2 for F32 (blocks
+    // of 8 x 16 cells, the reference's 2D F32 block; measured 1.1-1.6x
+    // faster than 16 x 16) and 4 for F64, from 4D on 16 (F32) or 8 (F64) as
+    // the reference's max_run (src/gpu_nd_spread.rs:575-582); never more
+    // than n0 + 1 - w, so a run sees every support offset at most once
+    int run = d == 3 ? 8 : (d >= 4 ? (f64 ? 8 : 16) : (d == 2 && !f64 ? 2 : 4));
     while (run > 2 && run > fine[0] + 1 - width) run /= 2;
     int64_t lanes_other = 1;
     int64_t block[kMaxDims] = {1, 1, 1, 1, 1, 1, 1, 1};
