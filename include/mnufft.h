@@ -112,7 +112,7 @@ int mnufft_encode(mnufft_plan plan, int64_t active_batch, int64_t M, const void 
  * bit for bit.  The points of all plans are binned by one sort and
  * interpolated by one launch; one launch predeconvolves every plan's grid,
  * and each plan transforms its own.  F32 type-2 plans of one dimension, 1D
- * or 2D, each at most once; one vector per plan.  The plans keep no points
+ * to 3D, each at most once; one vector per plan.  The plans keep no points
  * afterwards (set them again before a plain execute); plans[0] keeps the
  * batch scratch.
  */

@@ -1,6 +1,7 @@
 """Batched type-2 encodes of many independent plans (the batching form of
 the reference's plan surface: one type2_gpu plan per problem,
-src/plan.rs:606): one bin sort over all points and one interpolation launch.
+src/plan.rs:606): one predeconvolution launch, one bin sort over all points
+and one interpolation launch per kernel class (1D, 2D and 3D).
 Every output must equal the plan's own encode bit for bit (each output is a
 fixed-order sum over its support, independent of the point order).  It must
 match the FP64 direct sums to the 2D tolerance (tests/gpu_nufft_2d.rs:
@@ -93,11 +94,44 @@ def test_batch_1d_and_argument_errors():
         W.encode_type2_many([plans[0], p2d], [xs[0], x2], [fs[0], f2])
     with pytest.raises(W.NufftError):
         W.encode_type2_many([plans[0], plans[0]], [xs[0], xs[0]], [fs[0], fs[0]])
-    p3d = W.NufftPlan.type2_gpu(W.NufftConfig([12, 12, 12], 1e-6))
-    x3 = torch.from_numpy(S.uniform_coords(S.config_seed(4, 3192), 100, 3)).cuda()
-    f3 = torch.from_numpy(S.complex_normal(S.config_seed(4, 3193), 12 ** 3)).cuda()
+    # an F64 plan
+    p64 = W.NufftPlan.type2_gpu(W.NufftConfig([20, 20], 1e-9).with_precision(W.F64))
     with pytest.raises(W.NufftError):
-        W.encode_type2_many([p3d], [x3], [f3])
+        W.encode_type2_many([plans[0], p64], [xs[0], x2.double()],
+                            [fs[0], f2.to(torch.complex128)])
+
+
+# 3D problems: LDS-tiled ones (w <= 8, every axis at least 8 + w - 1 cells)
+# grouped by width, then the gather ones (small grids, w > 8)
+PROBLEMS_3D = [
+    ([32, 28, 24], 20000, 1e-6, 1, 0),
+    ([16, 20, 12], 3000, 1e-3, -1, 1),
+    ([24, 24, 24], 0, 1e-5, 1, 0),
+    ([6, 5, 4], 800, 1e-6, -1, 0),
+    ([20, 16, 18], 5000, 1e-7, 1, 1),
+    ([40, 36, 32], 60000, 1e-5, -1, 0),
+]
+
+
+def test_batch_3d_equals_each_plan_and_direct_sums():
+    torch, W, O, S = _setup()
+    plans, xs, fs = _make(W, S, torch, PROBLEMS_3D, 3600)
+    single = [p.encode_gpu(x, f).clone() for p, x, f in zip(plans, xs, fs)]
+    batch = W.encode_type2_many(plans, xs, fs)
+    again = W.encode_type2_many(plans, xs, fs)
+    torch.cuda.synchronize()
+    for i, (a, b, c) in enumerate(zip(single, batch, again)):
+        assert torch.equal(_bits(a), _bits(b)), f"problem {i} differs from its own encode"
+        assert torch.equal(_bits(b), _bits(c)), f"problem {i} not repeatable"
+    for i in (0, 1, 3, 4, 5):
+        n_modes, M, eps, isign, order = PROBLEMS_3D[i]
+        sel = np.arange(0, M, 11)
+        ref = O.direct_type2(xs[i].cpu().numpy().astype(np.float64),
+                             fs[i].cpu().numpy().astype(np.complex128), n_modes, isign=isign,
+                             modeord=order, points=sel)
+        e = rel_l2(batch[i].cpu().numpy()[sel], ref)
+        print(f"BATCH 3D problem {i} {n_modes} M={M} eps={eps}: err={e:.3e}", flush=True)
+        assert e <= 32 * max(eps, 1e-6)
 
 
 def test_batch_captured_in_a_graph():

@@ -378,7 +378,7 @@ def encode_type2_many(plans, points, values, outs=None):
     """Type-2 encodes of many independent plans as one batch: plans[i]
     interpolates values[i] (its modes) at points[i]; one bin sort over all
     the points and one interpolation launch, each plan's FFT on its own
-    grid.  F32 type-2 plans of one dimension (1D or 2D), one vector each;
+    grid.  F32 type-2 plans of one dimension (1D to 3D), one vector each;
     outputs equal each plan's own encode_gpu bit for bit.  The plans keep no
     points afterwards."""
     n = len(plans)

@@ -283,9 +283,9 @@ int mnufft_execute_type2_many(int64_t count, const mnufft_plan *plans, const int
         std::vector<Plan *> ps((size_t)count);
         for (int64_t i = 0; i < count; ++i) {
             Plan *p = plans[i] ? plans[i]->plan.get() : nullptr;
-            if (!p || p->type != 2 || p->precision != Precision::F32 || (p->dim != 1 && p->dim != 2) ||
+            if (!p || p->type != 2 || p->precision != Precision::F32 || p->dim < 1 || p->dim > 3 ||
                 p->dim != plans[0]->plan->dim) {
-                set_error("a batch takes F32 type-2 plans of one dimension, 1D or 2D");
+                set_error("a batch takes F32 type-2 plans of one dimension, 1D to 3D");
                 return MNUFFT_ERROR_INVALID_ARGUMENT;
             }
             if (M[i] < 0 || !d_in[i] || (M[i] > 0 && (!d_points[i] || !d_out[i]))) {

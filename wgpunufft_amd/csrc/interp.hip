@@ -263,28 +263,37 @@ __device__ inline TileItem3 tile_item3(int64_t item, const uint32_t *__restrict_
     return t;
 }
 
-template <typename T, int W, int TX, int TY, int TZ, int THREADS, int P>
-__global__ __launch_bounds__(THREADS) void k_interp_tiled3(
-    int64_t M, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
-    const uint32_t *__restrict__ item_off, const T *__restrict__ coords, GridParams g,
-    const T *__restrict__ grid, T *__restrict__ out)
+// LDS tile of the 3D tiled interpolation: TX x TY x TZ cells and their
+// halo.  F32: every row of a point's window is read as NP aligned pairs of
+// cells (16-byte LDS reads), starting at the even cell at or before the
+// window, with a zero weight on the cell outside it.
+template <typename T, int W, int TX, int TY, int TZ>
+struct Tile3 {
+    static constexpr bool kPairs = sizeof(T) == 4;
+    static constexpr int NP = (W + 2) / 2;
+    static constexpr int EX = TX + W - 1, EY = TY + W - 1, EZ = TZ + W - 1;
+    static constexpr int EXR = kPairs && TX + 2 * NP - 2 > EX ? TX + 2 * NP - 2 : EX;
+    static constexpr int RX = kPairs ? (EXR + 1) / 2 * 2 : EX;  // LDS row stride (cells)
+    static constexpr int CELLS = RX * EY * EZ;
+};
+
+// One work item: the tile at (x0, y0, z0) of grid gv staged in LDS, then the
+// points at sorted slots [first, last) (point index sorted[s] - base of
+// coords) interpolated from it into out_v.
+template <typename T, int W, int TX, int TY, int TZ, int THREADS>
+__device__ __forceinline__ void interp_tile3(typename Complex2<T>::type *tile, const GridParams &g,
+                                             const T *__restrict__ gv, int64_t x0, int64_t y0,
+                                             int64_t z0, int64_t first, int64_t last,
+                                             const uint32_t *__restrict__ sorted, uint32_t base,
+                                             const T *__restrict__ coords, T *__restrict__ out_v)
 {
-    // F32: every row of a point's window is read as NP aligned pairs of
-    // cells (16-byte LDS reads), starting at the even cell at or before the
-    // window, with a zero weight on the cell outside it
-    constexpr bool kPairs = sizeof(T) == 4;
-    constexpr int NP = (W + 2) / 2;
-    constexpr int EX = TX + W - 1, EY = TY + W - 1, EZ = TZ + W - 1;
-    constexpr int EXR = kPairs && TX + 2 * NP - 2 > EX ? TX + 2 * NP - 2 : EX;
-    constexpr int RX = kPairs ? (EXR + 1) / 2 * 2 : EX;  // LDS row stride (cells)
-    constexpr int CELLS = RX * EY * EZ;
-    using T2 = typename Complex2<T>::type;
-    __shared__ __align__(16) T2 tile[CELLS];
-    const int64_t n_items = item_off[g.nbins];
-    const int64_t v = blockIdx.y;
+    using Shape = Tile3<T, W, TX, TY, TZ>;
+    constexpr bool kPairs = Shape::kPairs;
+    constexpr int NP = Shape::NP;
+    constexpr int EY = Shape::EY, EZ = Shape::EZ;
+    constexpr int RX = Shape::RX;
     const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n, n2 = g.ax[2].n;
     const T beta = (T)g.beta;
-    const T *gv = grid + 2 * v * g.fine_total;
     // tile loads: every wavefront fills RPW rows of RX cells at a time, each
     // lane keeping its x cell
     constexpr int ROWS = EY * EZ;
@@ -294,117 +303,177 @@ __global__ __launch_bounds__(THREADS) void k_interp_tiled3(
     const int rsub = lane / RX, lx = lane % RX;
     const int r_first = (int)(threadIdx.x >> 6) * RPW + rsub;
     constexpr int R_STEP = (THREADS / 64) * RPW;
-    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-        const TileItem3 ti = tile_item3(item, item_off, g, TX, TY, TZ);
-        if (rsub < RPW) {
-            int64_t gx = ti.x0 + lx;
-            gx = gx >= n0 ? gx - n0 : gx;
-            for (int r = r_first; r < ROWS; r += R_STEP) {
-                const int ly = r % EY, lz = r / EY;
-                int64_t gy = ti.y0 + ly, gz = ti.z0 + lz;
-                gy = gy >= n1 ? gy - n1 : gy;
-                gz = gz >= n2 ? gz - n2 : gz;
-                tile[r * RX + lx] = *(const T2 *)(gv + 2 * (gx + n0 * (gy + n1 * gz)));
-            }
+    using T2 = typename Complex2<T>::type;
+    if (rsub < RPW) {
+        int64_t gx = x0 + lx;
+        gx = gx >= n0 ? gx - n0 : gx;
+        for (int r = r_first; r < ROWS; r += R_STEP) {
+            const int ly = r % EY, lz = r / EY;
+            int64_t gy = y0 + ly, gz = z0 + lz;
+            gy = gy >= n1 ? gy - n1 : gy;
+            gz = gz >= n2 ? gz - n2 : gz;
+            tile[r * RX + lx] = *(const T2 *)(gv + 2 * (gx + n0 * (gy + n1 * gz)));
         }
-        __syncthreads();
-        const int64_t first = (int64_t)offsets[ti.b] + ti.k * P;
-        int64_t last = first + P;
-        if (last > (int64_t)offsets[ti.b + 1]) last = offsets[ti.b + 1];
-        for (int64_t s = first + threadIdx.x; s < last; s += THREADS) {
-            const uint32_t idx = sorted[s];
-            T x[3];
+    }
+    __syncthreads();
+    for (int64_t s = first + threadIdx.x; s < last; s += THREADS) {
+        const uint32_t idx = sorted[s] - base;
+        T x[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) x[a] = coords[(int64_t)idx * 3 + a];
-            const bool reach = FoldTraits<T>::in_reach(x[0]) && FoldTraits<T>::in_reach(x[1]) &&
-                               FoldTraits<T>::in_reach(x[2]);
-            int l[3];
-            T wt[3][W];
+        for (int a = 0; a < 3; ++a) x[a] = coords[(int64_t)idx * 3 + a];
+        const bool reach = FoldTraits<T>::in_reach(x[0]) && FoldTraits<T>::in_reach(x[1]) &&
+                           FoldTraits<T>::in_reach(x[2]);
+        int l[3];
+        T wt[3][W];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                int64_t start;
-                double off;
-                support_start<T>(x[a], reach, g, a, &start, &off);
-                int64_t ws = wrapped_start(start, g.ax[a].n);
-                int64_t o = a == 0 ? ti.x0 : (a == 1 ? ti.y0 : ti.z0);
-                l[a] = (int)(ws - o);
-                if constexpr (sizeof(T) == 4) {
-                    // the kernel at distances j + off: scaled distance off * 2/w +
-                    // j * 2/w, exponent as one FMA (hardware sqrt / exp2)
-                    constexpr float two_over_w = 2.0f / (float)W;
-                    const float u0 = (float)off * two_over_w;
-                    const float bl = (float)beta * 1.44269504088896341f;
+        for (int a = 0; a < 3; ++a) {
+            int64_t start;
+            double off;
+            support_start<T>(x[a], reach, g, a, &start, &off);
+            int64_t ws = wrapped_start(start, g.ax[a].n);
+            int64_t o = a == 0 ? x0 : (a == 1 ? y0 : z0);
+            l[a] = (int)(ws - o);
+            if constexpr (sizeof(T) == 4) {
+                // the kernel at distances j + off: scaled distance off * 2/w +
+                // j * 2/w, exponent as one FMA (hardware sqrt / exp2)
+                constexpr float two_over_w = 2.0f / (float)W;
+                const float u0 = (float)off * two_over_w;
+                const float bl = (float)beta * 1.44269504088896341f;
 #pragma unroll
-                    for (int j = 0; j < W; ++j) {
-                        const float sv = u0 + (float)j * two_over_w;
-                        const float q = __builtin_fmaf(-sv, sv, 1.0f);
-                        const float e = __builtin_amdgcn_exp2f(
-                            __builtin_fmaf(__builtin_amdgcn_sqrtf(q), bl, -bl));
-                        wt[a][j] = q > 0.0f ? e : 0.0f;
-                    }
-                } else {
-#pragma unroll
-                    for (int j = 0; j < W; ++j) wt[a][j] = es_w<T>((T)j + (T)off, W, beta);
-                }
-            }
-            T sre = (T)0, sim = (T)0;
-            if constexpr (kPairs) {
-                const int sh = l[0] & 1;
-                T wx[2 * NP];
-#pragma unroll
-                for (int k = 0; k < 2 * NP; ++k) {
-                    const T a0 = k < W ? wt[0][k] : (T)0;
-                    const T a1 = k >= 1 && k - 1 < W ? wt[0][k - 1] : (T)0;
-                    wx[k] = sh ? a1 : a0;
-                }
-#pragma unroll
-                for (int dz = 0; dz < W; ++dz) {
-                    T pre = (T)0, pim = (T)0;
-#pragma unroll
-                    for (int dy = 0; dy < W; ++dy) {
-                        const int rowb = ((l[2] + dz) * EY + (l[1] + dy)) * RX + l[0] - sh;
-                        const float4 *rp = (const float4 *)(tile + rowb);
-                        T rre = (T)0, rim = (T)0;
-#pragma unroll
-                        for (int q = 0; q < NP; ++q) {
-                            const float4 c2 = rp[q];
-                            rre += wx[2 * q] * c2.x;
-                            rim += wx[2 * q] * c2.y;
-                            rre += wx[2 * q + 1] * c2.z;
-                            rim += wx[2 * q + 1] * c2.w;
-                        }
-                        pre += wt[1][dy] * rre;
-                        pim += wt[1][dy] * rim;
-                    }
-                    sre += wt[2][dz] * pre;
-                    sim += wt[2][dz] * pim;
+                for (int j = 0; j < W; ++j) {
+                    const float sv = u0 + (float)j * two_over_w;
+                    const float q = __builtin_fmaf(-sv, sv, 1.0f);
+                    const float e = __builtin_amdgcn_exp2f(
+                        __builtin_fmaf(__builtin_amdgcn_sqrtf(q), bl, -bl));
+                    wt[a][j] = q > 0.0f ? e : 0.0f;
                 }
             } else {
 #pragma unroll
-                for (int dz = 0; dz < W; ++dz) {
-                    T pre = (T)0, pim = (T)0;
-#pragma unroll
-                    for (int dy = 0; dy < W; ++dy) {
-                        const int rowb = ((l[2] + dz) * EY + (l[1] + dy)) * RX + l[0];
-                        T rre = (T)0, rim = (T)0;
-#pragma unroll
-                        for (int dx = 0; dx < W; ++dx) {
-                            const T2 cv = tile[rowb + dx];
-                            rre += wt[0][dx] * cv.x;
-                            rim += wt[0][dx] * cv.y;
-                        }
-                        pre += wt[1][dy] * rre;
-                        pim += wt[1][dy] * rim;
-                    }
-                    sre += wt[2][dz] * pre;
-                    sim += wt[2][dz] * pim;
-                }
+                for (int j = 0; j < W; ++j) wt[a][j] = es_w<T>((T)j + (T)off, W, beta);
             }
-            T *dst = out + 2 * (v * M + (int64_t)idx);
-            dst[0] = sre;
-            dst[1] = sim;
         }
-        __syncthreads();
+        T sre = (T)0, sim = (T)0;
+        if constexpr (kPairs) {
+            const int sh = l[0] & 1;
+            T wx[2 * NP];
+#pragma unroll
+            for (int k = 0; k < 2 * NP; ++k) {
+                const T a0 = k < W ? wt[0][k] : (T)0;
+                const T a1 = k >= 1 && k - 1 < W ? wt[0][k - 1] : (T)0;
+                wx[k] = sh ? a1 : a0;
+            }
+#pragma unroll
+            for (int dz = 0; dz < W; ++dz) {
+                T pre = (T)0, pim = (T)0;
+#pragma unroll
+                for (int dy = 0; dy < W; ++dy) {
+                    const int rowb = ((l[2] + dz) * EY + (l[1] + dy)) * RX + l[0] - sh;
+                    const float4 *rp = (const float4 *)(tile + rowb);
+                    T rre = (T)0, rim = (T)0;
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) {
+                        const float4 c2 = rp[q];
+                        rre += wx[2 * q] * c2.x;
+                        rim += wx[2 * q] * c2.y;
+                        rre += wx[2 * q + 1] * c2.z;
+                        rim += wx[2 * q + 1] * c2.w;
+                    }
+                    pre += wt[1][dy] * rre;
+                    pim += wt[1][dy] * rim;
+                }
+                sre += wt[2][dz] * pre;
+                sim += wt[2][dz] * pim;
+            }
+        } else {
+#pragma unroll
+            for (int dz = 0; dz < W; ++dz) {
+                T pre = (T)0, pim = (T)0;
+#pragma unroll
+                for (int dy = 0; dy < W; ++dy) {
+                    const int rowb = ((l[2] + dz) * EY + (l[1] + dy)) * RX + l[0];
+                    T rre = (T)0, rim = (T)0;
+#pragma unroll
+                    for (int dx = 0; dx < W; ++dx) {
+                        const T2 cv = tile[rowb + dx];
+                        rre += wt[0][dx] * cv.x;
+                        rim += wt[0][dx] * cv.y;
+                    }
+                    pre += wt[1][dy] * rre;
+                    pim += wt[1][dy] * rim;
+                }
+                sre += wt[2][dz] * pre;
+                sim += wt[2][dz] * pim;
+            }
+        }
+        T *dst = out_v + 2 * (int64_t)idx;
+        dst[0] = sre;
+        dst[1] = sim;
+    }
+    __syncthreads();
+}
+
+template <typename T, int W, int TX, int TY, int TZ, int THREADS, int P>
+__global__ __launch_bounds__(THREADS) void k_interp_tiled3(
+    int64_t M, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
+    const uint32_t *__restrict__ item_off, const T *__restrict__ coords, GridParams g,
+    const T *__restrict__ grid, T *__restrict__ out)
+{
+    using T2 = typename Complex2<T>::type;
+    __shared__ __align__(16) T2 tile[Tile3<T, W, TX, TY, TZ>::CELLS];
+    const int64_t n_items = item_off[g.nbins];
+    const int64_t v = blockIdx.y;
+    const T *gv = grid + 2 * v * g.fine_total;
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const TileItem3 ti = tile_item3(item, item_off, g, TX, TY, TZ);
+        const int64_t first = (int64_t)offsets[ti.b] + ti.k * P;
+        int64_t last = first + P;
+        if (last > (int64_t)offsets[ti.b + 1]) last = offsets[ti.b + 1];
+        interp_tile3<T, W, TX, TY, TZ, THREADS>(tile, g, gv, ti.x0, ti.y0, ti.z0, first, last,
+                                                sorted, 0u, coords, out + 2 * v * M);
+    }
+}
+
+// Many problems (one vector each): the work items of the combined bins
+// [bin_lo, bin_hi) of a batched sort (per bin ceil(points / P) items in
+// item_off); an item's bin -> its problem (bin bases) -> the problem's tile.
+template <typename T, int W, int TX, int TY, int TZ, int THREADS, int P>
+__global__ __launch_bounds__(THREADS) void k_interp_tiled3_many(
+    const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
+    const uint32_t *__restrict__ item_off, const BatchProblem *__restrict__ probs, int nprob,
+    int64_t bin_lo, int64_t bin_hi)
+{
+    using T2 = typename Complex2<T>::type;
+    __shared__ __align__(16) T2 tile[Tile3<T, W, TX, TY, TZ>::CELLS];
+    const int64_t i0 = item_off[bin_lo], i1 = item_off[bin_hi];
+    for (int64_t item = i0 + blockIdx.x; item < i1; item += gridDim.x) {
+        int64_t lo = bin_lo, hi = bin_hi - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if ((int64_t)item_off[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int64_t b = lo;
+        int qlo = 0, qhi = nprob - 1;
+        while (qlo < qhi) {
+            const int mid = (qlo + qhi + 1) >> 1;
+            if ((int64_t)probs[mid].bin_base <= b)
+                qlo = mid;
+            else
+                qhi = mid - 1;
+        }
+        const BatchProblem &pb = probs[qlo];
+        const int64_t lb = b - (int64_t)pb.bin_base;
+        const int64_t x0 = (lb % pb.g.ax[0].bins) * TX;
+        const int64_t y0 = ((lb / pb.g.ax[0].bins) % pb.g.ax[1].bins) * TY;
+        const int64_t z0 = lb / (pb.g.ax[0].bins * pb.g.ax[1].bins) * TZ;
+        const int64_t first = (int64_t)offsets[b] + (item - item_off[b]) * P;
+        int64_t last = first + P;
+        if (last > (int64_t)offsets[b + 1]) last = offsets[b + 1];
+        interp_tile3<T, W, TX, TY, TZ, THREADS>(tile, pb.g, (const T *)pb.grid, x0, y0, z0, first,
+                                                last, sorted, (uint32_t)pb.m_off,
+                                                (const T *)pb.coords, (T *)pb.out);
     }
 }
 
@@ -672,6 +741,8 @@ template bool interpolate_many<1, float>(const uint32_t *, const BatchProblem *,
                                          int64_t, int, hipStream_t);
 template bool interpolate_many<2, float>(const uint32_t *, const BatchProblem *, int, int64_t,
                                          int64_t, int, hipStream_t);
+template bool interpolate_many<3, float>(const uint32_t *, const BatchProblem *, int, int64_t,
+                                         int64_t, int, hipStream_t);
 
 template <typename T>
 bool interpolate_tiled2(int64_t M, int64_t vectors, const uint32_t *sorted,
@@ -727,6 +798,38 @@ template bool interpolate_tiled3<float>(int64_t, int64_t, const uint32_t *, cons
 template bool interpolate_tiled3<double>(int64_t, int64_t, const uint32_t *, const uint32_t *,
                                          uint32_t *, const double *, const GridParams &,
                                          const double *, double *, hipStream_t);
+
+bool interpolate_tiled3_many(const uint32_t *sorted, const uint32_t *offsets, uint32_t *item_off,
+                             int64_t nbins, const BatchProblem *probs, int nprob,
+                             const int *widths, const int64_t *bin_lo, const int64_t *bin_hi,
+                             const int64_t *points, int nclass, hipStream_t stream)
+{
+    constexpr int P = 1024;
+    const int64_t n = nbins + 1;
+    k_item_counts<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(nbins, (uint32_t)P, offsets,
+                                                                  item_off);
+    scan_u32(item_off, n, item_off + n, item_off, stream);
+    for (int c = 0; c < nclass; ++c) {
+        const int64_t items =
+            std::min<int64_t>(bin_hi[c] - bin_lo[c] + points[c] / P + 64, 16384);
+        const dim3 grid3((unsigned)std::max<int64_t>(items, 1), 1);
+#define MNUFFT_TILED3_MANY(WW)                                                                    \
+    k_interp_tiled3_many<float, WW, 8, 8, 8, 256, P><<<grid3, 256, 0, stream>>>(                \
+        sorted, offsets, item_off, probs, nprob, bin_lo[c], bin_hi[c])
+        switch (widths[c]) {
+        case 2: MNUFFT_TILED3_MANY(2); break;
+        case 3: MNUFFT_TILED3_MANY(3); break;
+        case 4: MNUFFT_TILED3_MANY(4); break;
+        case 5: MNUFFT_TILED3_MANY(5); break;
+        case 6: MNUFFT_TILED3_MANY(6); break;
+        case 7: MNUFFT_TILED3_MANY(7); break;
+        case 8: MNUFFT_TILED3_MANY(8); break;
+        default: set_error("batched tiled interpolation needs a kernel width <= 8"); return false;
+        }
+#undef MNUFFT_TILED3_MANY
+    }
+    return hip_ok(hipGetLastError(), "batched tiled 3D interpolation launch");
+}
 
 #define MNUFFT_INTERP_INSTANTIATE(D, T)                                                           \
     template bool interpolate<D, T>(int64_t, int64_t, const uint32_t *, const T *,                 \

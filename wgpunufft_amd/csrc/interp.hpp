@@ -43,4 +43,14 @@ bool interpolate_tiled3(int64_t M, int64_t vectors, const uint32_t *sorted, cons
                         uint32_t *item_off, const T *coords, const GridParams &g, const T *grid,
                         T *out, hipStream_t stream);
 
+// The same for the problems of a batch (F32, one vector each): offsets over
+// the combined bin list (every problem's bins of 8^3 cells), one launch per
+// kernel width c over bins [bin_lo[c], bin_hi[c]) (the problems of one width
+// take consecutive bins; points[c] bounds their points).  item_off needs
+// nbins + 1 + scan_tiles(nbins + 1) words of scratch.
+bool interpolate_tiled3_many(const uint32_t *sorted, const uint32_t *offsets, uint32_t *item_off,
+                             int64_t nbins, const BatchProblem *probs, int nprob,
+                             const int *widths, const int64_t *bin_lo, const int64_t *bin_hi,
+                             const int64_t *points, int nclass, hipStream_t stream);
+
 }  // namespace mnufft

@@ -1107,7 +1107,8 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
     std::vector<int64_t> order((size_t)count);
     for (int64_t i = 0; i < count; ++i) order[(size_t)i] = i;
     auto tiled_w = [&](int64_t i) {
-        return D == 2 && plans[i]->tiled2_interp() ? plans[i]->kernel.width : 1000;
+        const bool tiled = (D == 2 && plans[i]->tiled2_interp()) || (D == 3 && plans[i]->tiled_interp());
+        return tiled ? plans[i]->kernel.width : 1000;
     };
     std::stable_sort(order.begin(), order.end(),
                      [&](int64_t a, int64_t b) { return tiled_w(a) < tiled_w(b); });
@@ -1125,7 +1126,7 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
     std::vector<BatchProblem> table((size_t)count);
     std::vector<ModeProblem> mtable((size_t)count);
     std::vector<int> widths;
-    std::vector<int64_t> bin_lo, bin_hi;
+    std::vector<int64_t> bin_lo, bin_hi, class_points;
     int64_t m_off = 0, items = 0, gather_items = 0, cells = 0;
     uint64_t bins = 0;
     for (int64_t r = 0; r < count; ++r) {
@@ -1158,8 +1159,10 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
                 widths.push_back(w);
                 bin_lo.push_back((int64_t)bins);
                 bin_hi.push_back((int64_t)bins);
+                class_points.push_back(0);
             }
             bin_hi.back() = (int64_t)(bins + (uint64_t)b.g.nbins);
+            class_points.back() += M[i];
         }
         if (r == first_gather) gather_items = items;
         m_off += M[i];
@@ -1184,11 +1187,22 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
     if (!head.batch_sorter) head.batch_sorter = std::make_unique<BinSorter>();
     BinSorter &bs = *head.batch_sorter;
     if (!bs.sort_many<D, float>(m_off, head.batch_table, (int)count, bins, stream)) return false;
-    if (!widths.empty() &&
-        !interpolate_tiled2_many(bs.sorted(), bs.offsets(), head.batch_table, (int)count,
-                                 widths.data(), bin_lo.data(), bin_hi.data(), (int)widths.size(),
-                                 stream))
-        return false;
+    if (!widths.empty()) {
+        if constexpr (D == 3) {
+            // work-item offsets of the combined bins and their scan scratch
+            const int64_t nb = (int64_t)bins;
+            if (!head.ensure_items(nb + 1 + scan_tiles(nb + 1) + 1) ||
+                !interpolate_tiled3_many(bs.sorted(), bs.offsets(), head.item_off, nb,
+                                         head.batch_table, (int)count, widths.data(),
+                                         bin_lo.data(), bin_hi.data(), class_points.data(),
+                                         (int)widths.size(), stream))
+                return false;
+        } else if (!interpolate_tiled2_many(bs.sorted(), bs.offsets(), head.batch_table,
+                                            (int)count, widths.data(), bin_lo.data(),
+                                            bin_hi.data(), (int)widths.size(), stream)) {
+            return false;
+        }
+    }
     return interpolate_many<D, float>(bs.sorted(), head.batch_table, (int)count, gather_items,
                                       items, std::max(maxw, 2), stream);
 }
@@ -1206,9 +1220,9 @@ bool execute_type2_many(int64_t count, Plan *const *plans, const int64_t *M,
     const int dim = plans[0]->dim;
     for (int64_t i = 0; i < count; ++i) {
         const Plan *p = plans[i];
-        if (!p || p->type != 2 || p->dim != dim || p->precision != Precision::F32 ||
-            (dim != 1 && dim != 2)) {
-            set_error("batched type 2 takes F32 type-2 plans of one dimension, 1D or 2D");
+        if (!p || p->type != 2 || p->dim != dim || p->precision != Precision::F32 || dim < 1 ||
+            dim > 3) {
+            set_error("batched type 2 takes F32 type-2 plans of one dimension, 1D to 3D");
             return false;
         }
         if (M[i] < 0 || (M[i] > 0 && (!points[i] || !outputs[i])) || !inputs[i]) {
@@ -1221,8 +1235,9 @@ bool execute_type2_many(int64_t count, Plan *const *plans, const int64_t *M,
                 return false;
             }
     }
-    return dim == 1 ? execute_type2_many_t<1>(count, plans, M, points, inputs, outputs, stream)
-                    : execute_type2_many_t<2>(count, plans, M, points, inputs, outputs, stream);
+    if (dim == 1) return execute_type2_many_t<1>(count, plans, M, points, inputs, outputs, stream);
+    if (dim == 2) return execute_type2_many_t<2>(count, plans, M, points, inputs, outputs, stream);
+    return execute_type2_many_t<3>(count, plans, M, points, inputs, outputs, stream);
 }
 
 namespace {
