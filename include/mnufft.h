@@ -296,6 +296,22 @@ int mnufft_set_spread_tuning(mnufft_plan plan, int64_t light_limit, int64_t min_
 int mnufft_set_bin_tuning(mnufft_plan plan, int64_t rows_side0, int32_t plain_rows);
 
 /*
+ * FFT of the fine grid (no counterpart in the reference, whose FFT is its
+ * wgpu-fft dependency): MNUFFT_FFT_LINES (default) runs 3D grids whose axes
+ * are at most 4096 (FP32) / 2048 (FP64) cells long as
+ * three passes of line transforms in LDS, pruned to the lines that carry
+ * modes; MNUFFT_FFT_ROCFFT keeps rocFFT's plan.  Other grids always use
+ * rocFFT.  A type-3 plan applies it to its inner type-2 plan.  Takes effect
+ * at the next execute.
+ */
+#define MNUFFT_FFT_LINES 0
+#define MNUFFT_FFT_ROCFFT 1
+int mnufft_set_fft_engine(mnufft_plan plan, int32_t engine);
+
+/* The engine the plan's next FFT runs on (MNUFFT_FFT_LINES or _ROCFFT). */
+int mnufft_fft_engine(mnufft_plan plan, int32_t *engine);
+
+/*
  * The plan's spread tuning: vals[0] spread method, [1] dense-bin threshold,
  * [2] rows_side0, [3] plain_rows, [4] light limit, [5] smallest part.
  * Writes min(n, 6) values.

@@ -164,6 +164,22 @@ class _PlanBase:
         check(lib().mnufft_set_bin_tuning(self._handle, int(rows_side0), 1 if plain_rows else 0))
         return self
 
+    def set_fft_engine(self, engine):
+        """FFT of the fine grid: "lines" (default; 3D grids of axes up to
+        4096 FP32 / 2048 FP64 cells as pruned line-transform passes) or
+        "rocfft".  Type-3 plans apply it to their inner plan."""
+        codes = {"lines": 0, "rocfft": 1}
+        if engine not in codes:
+            raise ValueError(f"unknown FFT engine {engine!r}")
+        check(lib().mnufft_set_fft_engine(self._handle, codes[engine]))
+        return self
+
+    def fft_engine(self):
+        """The engine the next FFT runs on: "lines" or "rocfft"."""
+        e = ctypes.c_int32(0)
+        check(lib().mnufft_fft_engine(self._handle, ctypes.byref(e)))
+        return "lines" if e.value == 0 else "rocfft"
+
     def tuning(self):
         """The spread tuning in use: method, dense-bin threshold, axis-0 bin
         side (0 automatic), plain row tiles, light limit, smallest part."""

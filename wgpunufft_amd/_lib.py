@@ -68,6 +68,8 @@ SIGNATURES = [
     ("mnufft_set_dense_bins", ctypes.c_int, [_vp, ctypes.c_int64]),
     ("mnufft_set_bin_tuning", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32]),
     ("mnufft_plan_tuning", ctypes.c_int, [_vp, _c_i64p, ctypes.c_int32]),
+    ("mnufft_set_fft_engine", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("mnufft_fft_engine", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     ("mnufft_autotune", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp,
                                        ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _vp]),
     ("mnufft_kernel_params", ctypes.c_int, [ctypes.c_double, ctypes.c_double,

@@ -721,6 +721,34 @@ int mnufft_set_bin_tuning(mnufft_plan plan, int64_t rows_side0, int32_t plain_ro
     return MNUFFT_SUCCESS;
 }
 
+int mnufft_set_fft_engine(mnufft_plan plan, int32_t engine)
+{
+    if (!plan) {
+        set_error("null plan");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    if (engine != MNUFFT_FFT_LINES && engine != MNUFFT_FFT_ROCFFT) {
+        set_error("the FFT engine is MNUFFT_FFT_LINES (0) or MNUFFT_FFT_ROCFFT (1)");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    Plan &p = *plan->plan;
+    Plan &q = p.type == 3 && p.inner ? *p.inner : p;
+    q.fft_engine = engine;
+    return MNUFFT_SUCCESS;
+}
+
+int mnufft_fft_engine(mnufft_plan plan, int32_t *engine)
+{
+    if (!plan || !engine) {
+        set_error("null argument");
+        return MNUFFT_ERROR_INVALID_ARGUMENT;
+    }
+    const Plan &p = *plan->plan;
+    const Plan &q = p.type == 3 && p.inner ? *p.inner : p;
+    *engine = q.use_line_fft() ? MNUFFT_FFT_LINES : MNUFFT_FFT_ROCFFT;
+    return MNUFFT_SUCCESS;
+}
+
 int mnufft_plan_tuning(mnufft_plan plan, int64_t *vals, int32_t n)
 {
     if (!plan || !vals) {

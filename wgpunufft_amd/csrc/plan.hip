@@ -115,9 +115,10 @@ SpreadParams make_tile_params(int d, const int64_t *fine, const int64_t *block)
     return q;
 }
 
-bool make_sweep_params(int d, const int64_t *fine, int width, int64_t tx, SpreadParams *sp)
+bool make_sweep_params(int d, const int64_t *fine, int width, int64_t tx, int max_width,
+                       SpreadParams *sp)
 {
-    if (d != 3 || width > 8) return false;
+    if (d != 3 || width > max_width) return false;
     // every axis at least 16 + width cells (the matrix-core tiles), axis 0
     // at least tx + width + 1 (the plain row tiles' x reach within the grid)
     if (fine[0] < 16 + width + 1 || fine[1] < 16 + width || fine[2] < 16 + width) return false;
@@ -204,10 +205,11 @@ bool Plan::init_common()
                                  precision == Precision::F32 ? Precision::F32 : Precision::F64);
     sp = make_spread_params(dim, fine, kernel.width, precision != Precision::F32);
     sp_generic = sp;
-    // plain row tiles of 8 x 8 x 8 cells
-    sweep = make_sweep_params(dim, fine, kernel.width, 8, &sp);
+    // plain row tiles of 8 x 8 x 8 cells (FP64 plans up to width 16)
+    sweep = make_sweep_params(dim, fine, kernel.width, 8,
+                              precision == Precision::F32 ? 8 : kMaxRowsWidth, &sp);
     tiles2 = make_tile2_params(dim, fine, kernel.width, &tile2_sp);
-    if (sweep) {
+    if (sweep && kernel.width <= 8) {
         // matrix-core tiles: 16 x 16 cells, 8 deep along z
         const int64_t block[kMaxDims] = {16, 16, 8, 1, 1, 1, 1, 1};
         tile3_sp = make_tile_params(3, fine, block);
@@ -500,8 +502,21 @@ bool Plan::fft_nd(int64_t vectors, void *data, hipStream_t stream, bool run)
 // The rocFFT plans (and work buffers) an FFT of `vectors` vectors uses,
 // created right after the grid so that later scratch (the dense-bin windows)
 // does not change where they land.
+bool Plan::use_line_fft() const
+{
+    return fft_engine == 0 && (type == 1 || type == 2) &&
+           LineFft::supported(dim, fine, precision != Precision::F32);
+}
+
 bool Plan::ensure_fft(int64_t vectors)
 {
+    if (use_line_fft()) {
+        if (line_fft) return true;
+        auto lf = std::make_unique<LineFft>();
+        if (!lf->init(dim, fine, n_modes, type, precision != Precision::F32, isign)) return false;
+        line_fft = std::move(lf);
+        return true;
+    }
     if (dim >= 4) return fft_nd(vectors, nullptr, nullptr, false);
     const int64_t per_exec = std::max<int64_t>(1, ((int64_t)1 << 31) / fine_total);
     for (int64_t v0 = 0; v0 < vectors; v0 += per_exec)
@@ -511,6 +526,10 @@ bool Plan::ensure_fft(int64_t vectors)
 
 bool Plan::fft(int64_t vectors, void *data, hipStream_t stream)
 {
+    if (use_line_fft()) {
+        if (!ensure_fft(vectors)) return false;
+        return line_fft->run(data, vectors, stream);
+    }
     if (dim >= 4) return fft_nd(vectors, data, stream, true);
     // Batches of at most 2^31 complex values per rocFFT execution: larger
     // batched executions were seen to corrupt their last transform.

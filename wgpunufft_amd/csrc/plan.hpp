@@ -6,6 +6,7 @@
 
 #include "bins.hpp"
 #include "modes.hpp"
+#include "fft_lines.hpp"
 #include "nufft_internal.hpp"
 #include "spread.hpp"
 
@@ -107,6 +108,11 @@ struct Plan {
     uint32_t *item_off = nullptr;  // work items of the tiled interpolation
     int64_t item_cap = 0;
     std::map<int64_t, std::unique_ptr<FftPlan>> ffts;
+    // 3D grids: the pruned line-transform FFT (fft_engine 0, where the grid
+    // qualifies) or rocFFT's 3D plan (fft_engine 1).
+    int fft_engine = 0;
+    std::unique_ptr<LineFft> line_fft;
+    bool use_line_fft() const;
     // Ranks 4-8 (rocFFT plans take at most three axes): batched FFTs over
     // groups of the fastest axes, each followed by a transpose that moves
     // the transformed axes behind the others; plans keyed by {group rank,

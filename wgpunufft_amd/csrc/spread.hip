@@ -949,8 +949,9 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
         set_error("row tiles of a 2D plan need a matrix-core spread method");
         return false;
     }
-    if (!mfma && tiles && (D != 3 || g.width < 2 || g.width > 8)) {
-        set_error("plain row tiles need a 3D plan of kernel width <= 8");
+    if (!mfma && tiles &&
+        (D != 3 || g.width < 2 || g.width > (sizeof(T) == 4 ? 8 : kMaxRowsWidth))) {
+        set_error("plain row tiles need a 3D plan of kernel width <= 8 (FP32) or <= 16 (FP64)");
         return false;
     }
     if (!mfma && !tiles && sp.run != 2 && sp.run != 4 && sp.run != 8 &&

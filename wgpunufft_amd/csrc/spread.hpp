@@ -184,10 +184,14 @@ SpreadParams make_spread_params(int d, const int64_t *fine, int width, bool f64)
 // Tiles of the given cells per axis (run = 0 marks tiles).
 SpreadParams make_tile_params(int d, const int64_t *fine, const int64_t *block);
 
-// Plain row tiles of tx x 8 x 8 cells (tx = 8; 3D, width <= 8, every axis
-// at least 16 + width long, axis 0 at least 16 + width + 1, the matrix-core
-// tiles' bound).  Returns false when the grid does not qualify.
-bool make_sweep_params(int d, const int64_t *fine, int width, int64_t tx, SpreadParams *sp);
+// Widest kernel of the plain row tiles (FP64 plans; FP32 plans stop at 8).
+constexpr int kMaxRowsWidth = 16;
+
+// Plain row tiles of tx x 8 x 8 cells (tx = 8; 3D, width <= max_width, every
+// axis at least 16 + width long, axis 0 at least 16 + width + 1, the
+// matrix-core tiles' bound).  Returns false when the grid does not qualify.
+bool make_sweep_params(int d, const int64_t *fine, int width, int64_t tx, int max_width,
+                       SpreadParams *sp);
 
 // Tile kernel of the plain row-tile spread (spread_rows.hip): 8 x 8 x 8
 // tiles, partial sums of heavy parts in the layout k_reduce_tiles reads.

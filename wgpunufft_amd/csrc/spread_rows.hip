@@ -1,4 +1,5 @@
-// Plain (FP32 / FP64 vector) type-1 spreading of 3D row tiles.
+// Plain (FP32 / FP64 vector) type-1 spreading of 3D row tiles (kernel
+// widths 2..8 in FP32, 2..16 in FP64).
 //
 // Same contract as the reference's output-stationary spreader
 // (src/gpu_nd_spread.rs:1-36, 1583-1667): every cell of an 8 x 8 x 8 tile
@@ -62,7 +63,7 @@ __global__ __launch_bounds__(64) void k_spread_rows(
     const uint32_t *__restrict__ heavy_base, const uint32_t *__restrict__ ctrl,
     T *__restrict__ grid, T *__restrict__ partial)
 {
-    static_assert(W <= 8, "widths beyond the row tiles");
+    static_assert(W <= kMaxRowsWidth, "widths beyond the row tiles");
     constexpr int TY = kRowsTY, TZ = kRowsTZ;
     constexpr int NRY = TY + W - 1;
     constexpr int NROWS = NRY * (TZ + W - 1);
@@ -193,10 +194,25 @@ bool launch_rows_tiles(const GridParams &g, const SpreadParams &sp, const uint32
     case 6: ok = launch_rows_width<T, 6>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
     case 7: ok = launch_rows_width<T, 7>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
     case 8: ok = launch_rows_width<T, 8>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
-    default: break;
+    default:
+        if constexpr (sizeof(T) == 8) {
+            switch (g.width) {
+            case 9: ok = launch_rows_width<T, 9>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+            case 10: ok = launch_rows_width<T, 10>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+            case 11: ok = launch_rows_width<T, 11>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+            case 12: ok = launch_rows_width<T, 12>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+            case 13: ok = launch_rows_width<T, 13>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+            case 14: ok = launch_rows_width<T, 14>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+            case 15: ok = launch_rows_width<T, 15>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+            case 16: ok = launch_rows_width<T, 16>(tx, grid3, g, sp, offsets, rec_start, rec_off, c_sorted, M, w, grid, partial, stream); break;
+            default: break;
+            }
+        }
+        break;
     }
     if (!ok) {
-        set_error("plain row tiles need a 3D plan, a width of 2..8 and 8 x 8 x 8 tiles");
+        set_error("plain row tiles need a 3D plan, a width of 2..8 (FP32) or 2..16 (FP64) and "
+                  "8 x 8 x 8 tiles");
         return false;
     }
     return hip_ok(hipGetLastError(), "row-tile spread launch");
