@@ -226,17 +226,25 @@ __device__ inline void dft(Cx<T> *u)
     if constexpr (R == 8) dft8<T, DIR>(u);
 }
 
-// One Stockham pass in place over the workgroup's lines in LDS.
-template <typename T, int DIR, int R, int LDS>
-__device__ inline void lds_pass(Cx<T> *X, const Cx<T> *tw, const LineFftAxis &P,
-                                const LineFftPass &ps)
+// One Stockham pass over the workgroup's lines.  kFromGrid: the pass reads
+// its butterflies' inputs from the grid (the first pass: p = 1, no
+// twiddles; cells outside the read window are zeros); kToGrid: it writes its
+// outputs to the grid (the last pass: p R = n; cells outside the write
+// window are dropped).  Otherwise LDS in place: every thread reads its
+// butterflies into registers, the workgroup waits, every thread writes.
+constexpr int kFromGrid = 1, kToGrid = 2;
+
+template <typename T, int DIR, int R, int LDS, int MODE>
+__device__ inline void line_pass(Cx<T> *X, const Cx<T> *tw, const LineFftAxis &P,
+                                 const LineFftPass &ps, Cx<T> *__restrict__ grid,
+                                 const int64_t *base)
 {
     constexpr int MAXB = (LDS / R + kThreads - 1) / kThreads;
     const int n = P.n, nb = n / R, total = P.lines * nb;
     const int ks = P.line_major ? 1 : P.lines, cs = P.line_major ? n : 1;
     const float inv_nb = 1.0f / (float)nb, inv_p = 1.0f / (float)ps.p;
     Cx<T> u[MAXB][R];
-    int dst[MAXB];
+    int dst[MAXB], line[MAXB];
 #pragma unroll
     for (int s = 0; s < MAXB; ++s) {
         const int b = (int)threadIdx.x + s * kThreads;
@@ -251,31 +259,77 @@ __device__ inline void lds_pass(Cx<T> *X, const Cx<T> *tw, const LineFftAxis &P,
             }
             int q, k;
             divmod_small(i, ps.p, inv_p, q, k);
+            line[s] = c;
+            if constexpr ((MODE & kFromGrid) != 0) {
+                const int64_t bc = base[c];
 #pragma unroll
-            for (int t = 0; t < R; ++t) u[s][t] = X[(i + t * nb) * ks + c * cs];
+                for (int t = 0; t < R; ++t) {
+                    const int e = i + t * nb;
+                    u[s][t] = Cx<T>{(T)0, (T)0};
+                    if (bc >= 0 && (e < P.in_hi || e >= n - P.in_lo))
+                        u[s][t] = grid[bc + (int64_t)e * P.estride];
+                }
+            } else {
 #pragma unroll
-            for (int t = 1; t < R; ++t) u[s][t] = cmul(u[s][t], tw[t * k * ps.tw_step]);
+                for (int t = 0; t < R; ++t) u[s][t] = X[(i + t * nb) * ks + c * cs];
+#pragma unroll
+                for (int t = 1; t < R; ++t) u[s][t] = cmul(u[s][t], tw[t * k * ps.tw_step]);
+            }
             dft<T, DIR, R>(u[s]);
-            dst[s] = (q * ps.p * R + k) * ks + c * cs;
+            dst[s] = q * ps.p * R + k;
         }
     }
-    __syncthreads();
+    if constexpr ((MODE & kToGrid) != 0) {
 #pragma unroll
-    for (int s = 0; s < MAXB; ++s) {
-        if (dst[s] >= 0) {
+        for (int s = 0; s < MAXB; ++s) {
+            if (dst[s] >= 0) {
+                const int64_t bc = base[line[s]];
 #pragma unroll
-            for (int t = 0; t < R; ++t) X[dst[s] + t * ps.p * ks] = u[s][t];
+                for (int t = 0; t < R; ++t) {
+                    const int e = dst[s] + t * ps.p;
+                    if (bc >= 0 && (e < P.out_hi || e >= n - P.out_lo))
+                        grid[bc + (int64_t)e * P.estride] = u[s][t];
+                }
+            }
         }
+    } else {
+        if constexpr ((MODE & kFromGrid) == 0) __syncthreads();
+#pragma unroll
+        for (int s = 0; s < MAXB; ++s) {
+            if (dst[s] >= 0) {
+#pragma unroll
+                for (int t = 0; t < R; ++t)
+                    X[(dst[s] + t * ps.p) * ks + line[s] * cs] = u[s][t];
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
+}
+
+template <typename T, int DIR, int LDS, int MODE>
+__device__ inline void line_pass_any(Cx<T> *X, const Cx<T> *tw, const LineFftAxis &P,
+                                     const LineFftPass &ps, Cx<T> *grid, const int64_t *base)
+{
+    switch (ps.radix) {
+    case 2: line_pass<T, DIR, 2, LDS, MODE>(X, tw, P, ps, grid, base); break;
+    case 3: line_pass<T, DIR, 3, LDS, MODE>(X, tw, P, ps, grid, base); break;
+    case 4: line_pass<T, DIR, 4, LDS, MODE>(X, tw, P, ps, grid, base); break;
+    case 5: line_pass<T, DIR, 5, LDS, MODE>(X, tw, P, ps, grid, base); break;
+    case 7: line_pass<T, DIR, 7, LDS, MODE>(X, tw, P, ps, grid, base); break;
+    case 8: line_pass<T, DIR, 8, LDS, MODE>(X, tw, P, ps, grid, base); break;
+    case 11: line_pass<T, DIR, 11, LDS, MODE>(X, tw, P, ps, grid, base); break;
+    default: line_pass<T, DIR, 13, LDS, MODE>(X, tw, P, ps, grid, base); break;
+    }
 }
 
 template <typename T, int DIR>
 __global__ __launch_bounds__(kThreads) void k_fft_lines(LineFftAxis P, T *__restrict__ data,
                                                         const Cx<T> *__restrict__ tw_global)
 {
+    // lines + twiddles: (lines + 1) n elements of dynamic LDS, at most LDS
     constexpr int LDS = LdsElems<T>::value;
-    __shared__ Cx<T> X[LDS];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    Cx<T> *X = (Cx<T> *)lds_raw;
     __shared__ int64_t base[kMaxGroupLines];
     Cx<T> *tw = X + P.lines * P.n;
     const int n = P.n;
@@ -295,60 +349,35 @@ __global__ __launch_bounds__(kThreads) void k_fft_lines(LineFftAxis P, T *__rest
         base[c] = b;
     }
     __syncthreads();
-    const int elems = P.lines * n;
-    // load (LDS index = e in both layouts), 8 loads in flight per thread;
-    // cells outside the read window are zeros
-    constexpr int U = 8;
-    for (int e0 = 0; e0 < elems; e0 += U * kThreads) {
-        Cx<T> v[U];
-#pragma unroll
-        for (int j = 0; j < U; ++j) {
-            const int e = e0 + j * kThreads + (int)threadIdx.x;
-            v[j] = Cx<T>{(T)0, (T)0};
-            if (e < elems) {
-                int c, k;
-                if (P.line_major) {
-                    divmod_small(e, n, P.inv_n, c, k);
-                } else {
-                    k = e >> P.log_lines;
-                    c = e & (P.lines - 1);
-                }
-                const int64_t b = base[c];
-                if (b >= 0 && (k < P.in_hi || k >= n - P.in_lo))
-                    v[j] = grid[b + (int64_t)k * P.estride];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < U; ++j) {
-            const int e = e0 + j * kThreads + (int)threadIdx.x;
-            if (e < elems) X[e] = v[j];
-        }
+    const int np = P.npass;
+    if (np == 1) {
+        line_pass_any<T, DIR, LDS, kFromGrid | kToGrid>(X, tw, P, P.pass[0], grid, base);
+        return;
     }
-    __syncthreads();
-    // passes grouped by radix (5s, 3s, 7s, 11s, 13s, 8s, then one 4 or 2), one loop each:
-    // separate loops keep each radix's registers to itself
-    int s = 0;
-    for (; s < P.npass && P.pass[s].radix == 5; ++s) lds_pass<T, DIR, 5, LDS>(X, tw, P, P.pass[s]);
-    for (; s < P.npass && P.pass[s].radix == 3; ++s) lds_pass<T, DIR, 3, LDS>(X, tw, P, P.pass[s]);
-    for (; s < P.npass && P.pass[s].radix == 7; ++s) lds_pass<T, DIR, 7, LDS>(X, tw, P, P.pass[s]);
-    for (; s < P.npass && P.pass[s].radix == 11; ++s)
-        lds_pass<T, DIR, 11, LDS>(X, tw, P, P.pass[s]);
-    for (; s < P.npass && P.pass[s].radix == 13; ++s)
-        lds_pass<T, DIR, 13, LDS>(X, tw, P, P.pass[s]);
-    for (; s < P.npass && P.pass[s].radix == 8; ++s) lds_pass<T, DIR, 8, LDS>(X, tw, P, P.pass[s]);
-    for (; s < P.npass && P.pass[s].radix == 4; ++s) lds_pass<T, DIR, 4, LDS>(X, tw, P, P.pass[s]);
-    for (; s < P.npass && P.pass[s].radix == 2; ++s) lds_pass<T, DIR, 2, LDS>(X, tw, P, P.pass[s]);
-    for (int e = threadIdx.x; e < elems; e += kThreads) {
-        int c, k;
-        if (P.line_major) {
-            divmod_small(e, n, P.inv_n, c, k);
-        } else {
-            k = e >> P.log_lines;
-            c = e & (P.lines - 1);
-        }
-        const int64_t b = base[c];
-        if (b >= 0 && (k < P.out_hi || k >= n - P.out_lo)) grid[b + (int64_t)k * P.estride] = X[e];
-    }
+    if (np == 0) return;
+    // first pass from the grid, the middle ones in LDS grouped by radix (5s,
+    // 3s, 7s, 11s, 13s, 8s, then a 4 or a 2; separate loops keep each
+    // radix's registers to itself), the last one to the grid
+    line_pass_any<T, DIR, LDS, kFromGrid>(X, tw, P, P.pass[0], grid, base);
+    const int last = np - 1;
+    int s = 1;
+    for (; s < last && P.pass[s].radix == 5; ++s)
+        line_pass<T, DIR, 5, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+    for (; s < last && P.pass[s].radix == 3; ++s)
+        line_pass<T, DIR, 3, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+    for (; s < last && P.pass[s].radix == 7; ++s)
+        line_pass<T, DIR, 7, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+    for (; s < last && P.pass[s].radix == 11; ++s)
+        line_pass<T, DIR, 11, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+    for (; s < last && P.pass[s].radix == 13; ++s)
+        line_pass<T, DIR, 13, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+    for (; s < last && P.pass[s].radix == 8; ++s)
+        line_pass<T, DIR, 8, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+    for (; s < last && P.pass[s].radix == 4; ++s)
+        line_pass<T, DIR, 4, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+    for (; s < last && P.pass[s].radix == 2; ++s)
+        line_pass<T, DIR, 2, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+    line_pass_any<T, DIR, LDS, kToGrid>(X, tw, P, P.pass[last], grid, base);
 }
 
 // Radix plan: 5s, 3s, 7s, 11s, 13s, 8s, then a 4 or a 2; false for other
@@ -394,18 +423,24 @@ bool radix_plan(int n, LineFftAxis *ax)
     return true;
 }
 
-// Lines per workgroup: the lines and the twiddle table fit the LDS array.
-bool group_lines(int n, bool line_major, bool f64, LineFftAxis *ax)
+// Lines per workgroup: the lines and the twiddle table fit the LDS array;
+// along the contiguous axis a line per thread at least, else few enough
+// lines for about 2048 workgroups (`count` lines in all), small groups
+// sharing a CU.
+bool group_lines(int n, bool line_major, bool f64, int64_t count, LineFftAxis *ax)
 {
     const int lds = f64 ? LdsElems<double>::value : LdsElems<float>::value;
     if (2 * n > lds) return false;
     ax->line_major = line_major ? 1 : 0;
     if (line_major) {
-        ax->lines = std::min((lds - n) / n, kMaxGroupLines);
+        const int64_t fill = std::max<int64_t>((kThreads + n - 1) / n, count / 2048);
+        int c = (int)std::min<int64_t>(kMaxGroupLines, fill);
+        while (c > 1 && (c + 1) * n > lds) --c;
+        ax->lines = c;
         ax->log_lines = 0;
     } else {
         int c = f64 ? LdsElems<double>::columns : LdsElems<float>::columns;
-        while (c > 1 && c * n + n > lds) c /= 2;
+        while (c > 1 && (c + 1) * n > lds) c /= 2;
         ax->lines = c;
         ax->log_lines = 0;
         while ((1 << ax->log_lines) < c) ++ax->log_lines;
@@ -435,7 +470,7 @@ bool LineFft::supported(int dim, const int64_t *fine, bool f64)
     for (int a = 0; a < 3; ++a) {
         LineFftAxis ax;
         if (fine[a] < 1 || fine[a] > 65536 || !radix_plan((int)fine[a], &ax) ||
-            !group_lines((int)fine[a], a == 0, f64, &ax))
+            !group_lines((int)fine[a], a == 0, f64, 1, &ax))
             return false;
     }
     return true;
@@ -464,7 +499,6 @@ bool LineFft::init(int dim, const int64_t *fine, const int64_t *n_modes, int typ
         LineFftAxis &x = ax_[a];
         x = LineFftAxis{};
         radix_plan((int)fine[a], &x);
-        group_lines((int)fine[a], a == 0, f64, &x);
         x.in_hi = x.out_hi = (int)fine[a];
         x.in_lo = x.out_lo = 0;
         x.vec_stride = fine_total_;
@@ -505,6 +539,7 @@ bool LineFft::init(int dim, const int64_t *fine, const int64_t *n_modes, int typ
         keep_v(ax_[2], n1, N1);
         window(ax_[2].out_hi, ax_[2].out_lo, N2);
     }
+    for (int a = 0; a < 3; ++a) group_lines((int)fine[a], a == 0, f64, ax_[a].nu * ax_[a].nv, &ax_[a]);
     // twiddles w_n^m = exp(dir 2 pi i m / n), from long double
     for (int a = 0; a < 3; ++a) {
         const int n = (int)fine[a];
@@ -544,20 +579,21 @@ bool LineFft::run(void *data, int64_t vectors, hipStream_t stream) const
             const int64_t groups = (x.nu * x.nv + x.lines - 1) / x.lines;
             if (groups == 0) continue;
             const dim3 g((unsigned)groups, vc);
+            const size_t lds = (size_t)(x.lines + 1) * (size_t)x.n * (f64_ ? 16 : 8);
             if (f64_) {
                 double *d = (double *)data + 2 * v0 * fine_total_;
                 const Cx<double> *t = (const Cx<double> *)tw_[a];
                 if (dir_ > 0)
-                    k_fft_lines<double, 1><<<g, kThreads, 0, stream>>>(x, d, t);
+                    k_fft_lines<double, 1><<<g, kThreads, lds, stream>>>(x, d, t);
                 else
-                    k_fft_lines<double, -1><<<g, kThreads, 0, stream>>>(x, d, t);
+                    k_fft_lines<double, -1><<<g, kThreads, lds, stream>>>(x, d, t);
             } else {
                 float *d = (float *)data + 2 * v0 * fine_total_;
                 const Cx<float> *t = (const Cx<float> *)tw_[a];
                 if (dir_ > 0)
-                    k_fft_lines<float, 1><<<g, kThreads, 0, stream>>>(x, d, t);
+                    k_fft_lines<float, 1><<<g, kThreads, lds, stream>>>(x, d, t);
                 else
-                    k_fft_lines<float, -1><<<g, kThreads, 0, stream>>>(x, d, t);
+                    k_fft_lines<float, -1><<<g, kThreads, lds, stream>>>(x, d, t);
             }
         }
     }

@@ -17,7 +17,8 @@
 // and a type-2 grid needs no zeros between the modes.  The kept window of
 // an axis of n cells holding N modes is [0, ceil(N/2)) and [n - floor(N/2), n)
 // (the fine index of mode m is m mod n).  Unnormalized; isign > 0 is
-// exp(+i k x).  Every axis at most 4096 (FP32) or 2048 (FP64) cells.
+// exp(+i k x).  Every axis at most 4096 (FP32) or 2048 (FP64) cells (one
+// line and its twiddles within 64 KB of LDS).
 #pragma once
 
 #include <hip/hip_runtime.h>

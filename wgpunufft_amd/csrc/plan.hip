@@ -648,7 +648,17 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
     } else {
         if (M == 0) return true;
         mark(kStageDeconv, false, stream);
-        if (!predeconvolve<D, T>(mp, d_inv_phihat, in, (T *)grid, active, stream)) return false;
+        // the line FFT reads the modes' cells only: no zeros between them
+        bool kept = false;
+        if constexpr (D == 3) {
+            if (use_line_fft()) {
+                if (!predeconvolve_kept<D, T>(mp, d_inv_phihat, in, (T *)grid, active, stream))
+                    return false;
+                kept = true;
+            }
+        }
+        if (!kept && !predeconvolve<D, T>(mp, d_inv_phihat, in, (T *)grid, active, stream))
+            return false;
         mark(kStageDeconv, true, stream);
         mark(kStageFft, false, stream);
         if (!fft(active, grid, stream)) return false;
