@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "fft_lines.hpp"
@@ -26,18 +27,25 @@ namespace {
 constexpr int kThreads = 512;
 constexpr int kMaxGroupLines = 256;
 
+// Dynamic LDS per workgroup: lines + twiddles, at most 80 KB (two
+// workgroups per CU) where the device allows it, else 64 KB; the static
+// register arrays are sized for the largest.  Strided axes take `columns`
+// adjacent columns (128-byte rows) when they fit.
 template <typename T>
 struct LdsElems;
 template <>
 struct LdsElems<float> {
-    static constexpr int value = 8192;  // 64 KB: two workgroups per CU
-    static constexpr int columns = 16;  // strided axes: 128-byte rows
+    static constexpr int value = 9984;  // 80 KB less the line bases
+    static constexpr int base = 8192;
+    static constexpr int columns = 16;
 };
 template <>
 struct LdsElems<double> {
-    static constexpr int value = 4096;
+    static constexpr int value = 4992;
+    static constexpr int base = 4096;
     static constexpr int columns = 8;
 };
+constexpr size_t kLdsWide = 80 * 1024 - kMaxGroupLines * sizeof(int64_t);
 
 template <typename T>
 struct Cx {
@@ -427,9 +435,34 @@ bool radix_plan(int n, LineFftAxis *ax)
 // along the contiguous axis a line per thread at least, else few enough
 // lines for about 2048 workgroups (`count` lines in all), small groups
 // sharing a CU.
-bool group_lines(int n, bool line_major, bool f64, int64_t count, LineFftAxis *ax)
+// Whether the line kernels may take the wide LDS budget (set once).
+bool wide_lds()
 {
-    const int lds = f64 ? LdsElems<double>::value : LdsElems<float>::value;
+    static std::once_flag once;
+    static bool ok = false;
+    std::call_once(once, [] {
+        ok = hipFuncSetAttribute((const void *)k_fft_lines<float, 1>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kLdsWide) == hipSuccess &&
+             hipFuncSetAttribute((const void *)k_fft_lines<float, -1>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kLdsWide) == hipSuccess &&
+             hipFuncSetAttribute((const void *)k_fft_lines<double, 1>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kLdsWide) == hipSuccess &&
+             hipFuncSetAttribute((const void *)k_fft_lines<double, -1>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kLdsWide) == hipSuccess;
+        (void)hipGetLastError();
+    });
+    return ok;
+}
+
+// `wide`: the 80 KB budget, else 64 KB (the support test uses 64 KB).
+bool group_lines(int n, bool line_major, bool f64, int64_t count, bool wide, LineFftAxis *ax)
+{
+    const int lds = f64 ? (wide ? LdsElems<double>::value : LdsElems<double>::base)
+                        : (wide ? LdsElems<float>::value : LdsElems<float>::base);
     if (2 * n > lds) return false;
     ax->line_major = line_major ? 1 : 0;
     if (line_major) {
@@ -470,10 +503,15 @@ bool LineFft::supported(int dim, const int64_t *fine, bool f64)
     for (int a = 0; a < 3; ++a) {
         LineFftAxis ax;
         if (fine[a] < 1 || fine[a] > 65536 || !radix_plan((int)fine[a], &ax) ||
-            !group_lines((int)fine[a], a == 0, f64, 1, &ax))
+            !group_lines((int)fine[a], a == 0, f64, 1, false, &ax))
             return false;
     }
     return true;
+}
+
+int64_t LineFft::row_length_limit(bool f64)
+{
+    return f64 ? LdsElems<double>::value / 5 : LdsElems<float>::value / 9;
 }
 
 LineFft::~LineFft()
@@ -539,7 +577,9 @@ bool LineFft::init(int dim, const int64_t *fine, const int64_t *n_modes, int typ
         keep_v(ax_[2], n1, N1);
         window(ax_[2].out_hi, ax_[2].out_lo, N2);
     }
-    for (int a = 0; a < 3; ++a) group_lines((int)fine[a], a == 0, f64, ax_[a].nu * ax_[a].nv, &ax_[a]);
+    const bool wide = wide_lds();
+    for (int a = 0; a < 3; ++a)
+        group_lines((int)fine[a], a == 0, f64, ax_[a].nu * ax_[a].nv, wide, &ax_[a]);
     // twiddles w_n^m = exp(dir 2 pi i m / n), from long double
     for (int a = 0; a < 3; ++a) {
         const int n = (int)fine[a];

@@ -56,6 +56,9 @@ public:
     ~LineFft();
     // False (no error set) when the grid does not qualify.
     static bool supported(int dim, const int64_t *fine, bool f64);
+    // Longest axis whose strided passes still move 64-byte rows (4 FP64 or
+    // 8 FP32 adjacent columns per workgroup): 998 FP64, 1109 FP32 cells.
+    static int64_t row_length_limit(bool f64);
     // type 1: output pruned to the modes; type 2: input pruned to them.
     bool init(int dim, const int64_t *fine, const int64_t *n_modes, int type, bool f64,
               int isign);

@@ -803,12 +803,23 @@ bool Plan::init_type3(const double *src_bounds, const double *tgt_bounds, const 
             cells *= (double)fine_len[a];
         }
         const double bytes = cells * 2.0 * (double)real_bytes() * (double)batch;
-        if (cells <= (double)kType3InnerCells && bytes <= (double)t3_inner_budget) break;
+        // 3D inner grids on the line FFT: axes short enough for 64-byte rows
+        // in the strided passes (a long axis costs more than extra boxes)
+        bool rows_ok = true;
+        if (dim == 3 && LineFft::supported(dim, fine_len, precision != Precision::F32))
+            for (int a = 0; a < dim; ++a)
+                rows_ok = rows_ok && fine_len[a] <= LineFft::row_length_limit(
+                                                        precision != Precision::F32);
+        const bool fits = cells <= (double)kType3InnerCells && bytes <= (double)t3_inner_budget;
+        if (fits && rows_ok) break;
+        // the longest splittable axis (once the grid fits: one beyond the row limit)
         int split = -1;
         for (int a = 0; a < dim; ++a)
             if (thi[a] > tlo[a] && k[a] < kType3MaxBoxes &&
+                (!fits || fine_len[a] > LineFft::row_length_limit(precision != Precision::F32)) &&
                 (split < 0 || fine_len[a] > fine_len[split]))
                 split = a;
+        if (split < 0 && fits) break;
         if (split < 0) {
             set_error("type-3 inner grid too large even with the target interval cut into boxes");
             return false;
