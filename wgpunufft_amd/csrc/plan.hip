@@ -998,7 +998,10 @@ bool Plan::execute_type3_t(int64_t active, int64_t M, const T *x, const T *c, in
         if (!type3_source<D, T>(tp, M, active, x, c, first ? (T *)t3_xr : nullptr, (T *)t3_cp,
                                 stream))
             return false;
+        // later boxes spread the same records: the outer work plan is kept
+        outer->spreader.reuse_plan = !first;
         if (!spread_type3_sources<D, T>(active, M, (const T *)t3_cp, first, stream)) return false;
+        outer->spreader.reuse_plan = false;
         first = false;
         const T *sb = (const T *)t3_s_sorted + off * D;
         if (!type3_target<D, T>(tp, nb, sb, (T *)t3_sr, (double *)t3_factor, stream))

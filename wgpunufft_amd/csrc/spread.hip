@@ -918,6 +918,7 @@ bool Spreader::ensure(int64_t nblocks, int64_t partial_need, int64_t vectors)
         if (!ctrl && !hip_ok(hipMalloc(&ctrl, sizeof(uint32_t) * 64), "spread control"))
             return false;
         blocks_cap = nblocks;
+        ++blocks_epoch;
     }
     if (partial_need > partial_bytes) {
         if (partial) (void)hipFree(partial);
@@ -982,10 +983,22 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
     if (!ensure(sp.block_count, part_capacity * per_part * chunk, vectors)) return false;
 
     const int64_t nb = sp.block_count;
-    k_classify<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(g, sp, offsets, totals);
-    k_fill_ctrl<<<1, 64, 0, stream>>>(ctrl);
-    k_plan<<<1, 1024, 0, stream>>>(nb, totals, light_limit, min_part, part_capacity, parts,
-                                   item_off, heavy_base, heavy_list, ctrl);
+    const bool reuse = reuse_plan && planned_epoch == blocks_epoch && planned_blocks == nb &&
+                       planned_capacity == part_capacity && planned_light == light_limit &&
+                       planned_min == min_part && planned_offsets == offsets;
+    reuse_plan = false;
+    if (!reuse) {
+        k_classify<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(g, sp, offsets, totals);
+        k_fill_ctrl<<<1, 64, 0, stream>>>(ctrl);
+        k_plan<<<1, 1024, 0, stream>>>(nb, totals, light_limit, min_part, part_capacity, parts,
+                                       item_off, heavy_base, heavy_list, ctrl);
+        planned_epoch = blocks_epoch;
+        planned_blocks = nb;
+        planned_capacity = part_capacity;
+        planned_light = light_limit;
+        planned_min = min_part;
+        planned_offsets = offsets;
+    }
     if constexpr (sizeof(T) == 4) {
         if (mfma) {
             k_fill_u32s<<<grid_for(vectors, 256), 256, 0, stream>>>(vmax, vectors);
@@ -1044,6 +1057,8 @@ bool Spreader::spread_many(const BatchProblem *probs, const SpreadProblem *sprob
                            const int4 *rec_start, const T *rec_off, const T *c_sorted,
                            hipStream_t stream)
 {
+    planned_epoch = ~0ull;  // the plan buffers take the batch's plans
+    reuse_plan = false;
     // scratch for the largest class first: nothing is freed between launches
     int64_t most_blocks = 1, most_partial = 0, most_problems = 1;
     for (const SpreadClass &k : classes) {

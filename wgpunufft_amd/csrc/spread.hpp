@@ -106,6 +106,15 @@ struct Spreader {
     // spread kernel launch.
     hipEvent_t kernel_start = nullptr, kernel_end = nullptr;
 
+    // One-shot request: the next spread keeps the work plan of the previous
+    // one (same records, offsets and parameters; only the strengths
+    // differ), when the plan buffers still hold it.
+    bool reuse_plan = false;
+    uint64_t blocks_epoch = 0, planned_epoch = ~0ull;
+    int64_t planned_blocks = -1, planned_capacity = -1;
+    uint32_t planned_light = 0, planned_min = 0;
+    const uint32_t *planned_offsets = nullptr;
+
     uint32_t *totals = nullptr;      // records in reach per block
     uint32_t *parts = nullptr;       // work items per block
     uint32_t *item_off = nullptr;    // first work item per block
