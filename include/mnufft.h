@@ -297,10 +297,10 @@ int mnufft_set_bin_tuning(mnufft_plan plan, int64_t rows_side0, int32_t plain_ro
 
 /*
  * FFT of the fine grid (no counterpart in the reference, whose FFT is its
- * wgpu-fft dependency): MNUFFT_FFT_LINES (default) runs 3D grids whose axes
- * are at most 4096 (FP32) / 2048 (FP64) cells long as
- * three passes of line transforms in LDS, pruned to the lines that carry
- * modes; MNUFFT_FFT_ROCFFT keeps rocFFT's plan.  Other grids always use
+ * wgpu-fft dependency): MNUFFT_FFT_LINES (default) runs 2D and 3D grids
+ * whose axes are at most 4096 (FP32) / 2048 (FP64) cells long as
+ * one pass of line transforms in LDS per axis, pruned to the lines that
+ * carry modes (batches of plans: one launch per pass over all of them); MNUFFT_FFT_ROCFFT keeps rocFFT's plan.  Other grids always use
  * rocFFT.  A type-3 plan applies it to its inner type-2 plan.  Takes effect
  * at the next execute.
  */

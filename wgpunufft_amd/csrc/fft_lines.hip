@@ -330,21 +330,19 @@ __device__ inline void line_pass_any(Cx<T> *X, const Cx<T> *tw, const LineFftAxi
     }
 }
 
+// The transform of workgroup `group`'s lines of one axis pass (P) of the
+// grid `grid` (one vector), X: (lines + 1) n elements of LDS.
 template <typename T, int DIR>
-__global__ __launch_bounds__(kThreads) void k_fft_lines(LineFftAxis P, T *__restrict__ data,
-                                                        const Cx<T> *__restrict__ tw_global)
+__device__ __forceinline__ void fft_lines_body(const LineFftAxis &P, Cx<T> *__restrict__ grid,
+                                               const Cx<T> *__restrict__ tw_global,
+                                               int64_t group, Cx<T> *X, int64_t *base)
 {
-    // lines + twiddles: (lines + 1) n elements of dynamic LDS, at most LDS
     constexpr int LDS = LdsElems<T>::value;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    Cx<T> *X = (Cx<T> *)lds_raw;
-    __shared__ int64_t base[kMaxGroupLines];
     Cx<T> *tw = X + P.lines * P.n;
     const int n = P.n;
     for (int m = threadIdx.x; m < n; m += kThreads) tw[m] = tw_global[m];
-    Cx<T> *grid = (Cx<T> *)data + (int64_t)blockIdx.y * P.vec_stride;
     const int64_t nlines = P.nu * P.nv;
-    const int64_t line0 = (int64_t)blockIdx.x * P.lines;
+    const int64_t line0 = group * P.lines;
     for (int c = threadIdx.x; c < P.lines; c += kThreads) {
         const int64_t L = line0 + c;
         int64_t b = -1;
@@ -386,6 +384,39 @@ __global__ __launch_bounds__(kThreads) void k_fft_lines(LineFftAxis P, T *__rest
     for (; s < last && P.pass[s].radix == 2; ++s)
         line_pass<T, DIR, 2, LDS, 0>(X, tw, P, P.pass[s], grid, base);
     line_pass_any<T, DIR, LDS, kToGrid>(X, tw, P, P.pass[last], grid, base);
+}
+
+// One plan's axis pass: workgroup (group, vector).
+template <typename T, int DIR>
+__global__ __launch_bounds__(kThreads) void k_fft_lines(LineFftAxis P, T *__restrict__ data,
+                                                        const Cx<T> *__restrict__ tw_global)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ int64_t base[kMaxGroupLines];
+    Cx<T> *grid = (Cx<T> *)data + (int64_t)blockIdx.y * P.vec_stride;
+    fft_lines_body<T, DIR>(P, grid, tw_global, blockIdx.x, (Cx<T> *)lds_raw, base);
+}
+
+// The same axis pass of many plans (one vector each) in one launch: job q
+// takes workgroups [group0[q], group0[q + 1]).
+template <typename T, int DIR>
+__global__ __launch_bounds__(kThreads) void k_fft_lines_many(const LineFftJob *__restrict__ jobs,
+                                                             int njobs)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ int64_t base[kMaxGroupLines];
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].group0 <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const LineFftJob &J = jobs[lo];
+    fft_lines_body<T, DIR>(J.ax, (Cx<T> *)J.grid, (const Cx<T> *)J.tw, b - J.group0,
+                           (Cx<T> *)lds_raw, base);
 }
 
 // Radix plan: 5s, 3s, 7s, 11s, 13s, 8s, then a 4 or a 2; false for other
@@ -452,6 +483,18 @@ bool wide_lds()
                                  (int)kLdsWide) == hipSuccess &&
              hipFuncSetAttribute((const void *)k_fft_lines<double, -1>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kLdsWide) == hipSuccess &&
+             hipFuncSetAttribute((const void *)k_fft_lines_many<float, 1>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kLdsWide) == hipSuccess &&
+             hipFuncSetAttribute((const void *)k_fft_lines_many<float, -1>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kLdsWide) == hipSuccess &&
+             hipFuncSetAttribute((const void *)k_fft_lines_many<double, 1>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kLdsWide) == hipSuccess &&
+             hipFuncSetAttribute((const void *)k_fft_lines_many<double, -1>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)kLdsWide) == hipSuccess;
         (void)hipGetLastError();
     });
@@ -499,8 +542,8 @@ void keep_v(LineFftAxis &a, int64_t n, int64_t N)
 
 bool LineFft::supported(int dim, const int64_t *fine, bool f64)
 {
-    if (dim != 3) return false;
-    for (int a = 0; a < 3; ++a) {
+    if (dim != 2 && dim != 3) return false;
+    for (int a = 0; a < dim; ++a) {
         LineFftAxis ax;
         if (fine[a] < 1 || fine[a] > 65536 || !radix_plan((int)fine[a], &ax) ||
             !group_lines((int)fine[a], a == 0, f64, 1, false, &ax))
@@ -524,16 +567,17 @@ bool LineFft::init(int dim, const int64_t *fine, const int64_t *n_modes, int typ
                    int isign)
 {
     if (!supported(dim, fine, f64)) {
-        set_error("line FFT needs a 3D grid of 2,3,5,7,11,13-smooth lengths within the LDS lines");
+        set_error("line FFT needs a 2D or 3D grid of 2,3,5,7,11,13-smooth lengths within the "
+                  "LDS lines");
         return false;
     }
-    naxes_ = 3;
+    naxes_ = dim;
     f64_ = f64;
     dir_ = isign >= 0 ? 1 : -1;
-    const int64_t n0 = fine[0], n1 = fine[1], n2 = fine[2];
+    const int64_t n0 = fine[0], n1 = fine[1], n2 = dim == 3 ? fine[2] : 1;
     fine_total_ = n0 * n1 * n2;
     const int64_t plane = n0 * n1;
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < dim; ++a) {
         LineFftAxis &x = ax_[a];
         x = LineFftAxis{};
         radix_plan((int)fine[a], &x);
@@ -541,7 +585,7 @@ bool LineFft::init(int dim, const int64_t *fine, const int64_t *n_modes, int typ
         x.in_lo = x.out_lo = 0;
         x.vec_stride = fine_total_;
     }
-    // x: lines (y, z); y: columns (x, z); z: columns (x, y)
+    // x: lines (y, z); y: columns (x, z); z: columns (x, y); a 2D grid has no z
     ax_[0].estride = 1;
     ax_[0].nu = ax_[0].u_hi = n1;
     ax_[0].su = n0;
@@ -552,36 +596,42 @@ bool LineFft::init(int dim, const int64_t *fine, const int64_t *n_modes, int typ
     ax_[1].su = 1;
     ax_[1].nv = ax_[1].v_hi = n2;
     ax_[1].sv = plane;
-    ax_[2].estride = plane;
-    ax_[2].nu = ax_[2].u_hi = n0;
-    ax_[2].su = 1;
-    ax_[2].nv = ax_[2].v_hi = n1;
-    ax_[2].sv = n0;
-    const int64_t N0 = n_modes[0], N1 = n_modes[1], N2 = n_modes[2];
+    if (dim == 3) {
+        ax_[2].estride = plane;
+        ax_[2].nu = ax_[2].u_hi = n0;
+        ax_[2].su = 1;
+        ax_[2].nv = ax_[2].v_hi = n1;
+        ax_[2].sv = n0;
+    }
+    const int64_t N0 = n_modes[0], N1 = n_modes[1], N2 = dim == 3 ? n_modes[2] : 1;
     auto window = [](int &hi, int &lo, int64_t N) {
         hi = (int)((N + 1) / 2);
         lo = (int)(N / 2);
     };
     if (type == 2) {
         keep_u(ax_[0], n1, N1);
-        keep_v(ax_[0], n2, N2);
         window(ax_[0].in_hi, ax_[0].in_lo, N0);
-        keep_v(ax_[1], n2, N2);
         window(ax_[1].in_hi, ax_[1].in_lo, N1);
-        window(ax_[2].in_hi, ax_[2].in_lo, N2);
+        if (dim == 3) {
+            keep_v(ax_[0], n2, N2);
+            keep_v(ax_[1], n2, N2);
+            window(ax_[2].in_hi, ax_[2].in_lo, N2);
+        }
     } else {
         window(ax_[0].out_hi, ax_[0].out_lo, N0);
         keep_u(ax_[1], n0, N0);
         window(ax_[1].out_hi, ax_[1].out_lo, N1);
-        keep_u(ax_[2], n0, N0);
-        keep_v(ax_[2], n1, N1);
-        window(ax_[2].out_hi, ax_[2].out_lo, N2);
+        if (dim == 3) {
+            keep_u(ax_[2], n0, N0);
+            keep_v(ax_[2], n1, N1);
+            window(ax_[2].out_hi, ax_[2].out_lo, N2);
+        }
     }
     const bool wide = wide_lds();
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < dim; ++a)
         group_lines((int)fine[a], a == 0, f64, ax_[a].nu * ax_[a].nv, wide, &ax_[a]);
     // twiddles w_n^m = exp(dir 2 pi i m / n), from long double
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < dim; ++a) {
         const int n = (int)fine[a];
         const size_t eb = f64 ? 2 * sizeof(double) : 2 * sizeof(float);
         std::vector<double> hd(f64 ? 2 * (size_t)n : 0);
@@ -607,6 +657,94 @@ bool LineFft::init(int dim, const int64_t *fine, const int64_t *n_modes, int typ
             return false;
     }
     return true;
+}
+
+namespace {
+
+struct JobChunk {
+    static constexpr int N = (int)((4096 - 64) / sizeof(LineFftJob));
+    LineFftJob j[N];
+    int n;
+};
+
+__global__ void k_line_jobs(JobChunk chunk, LineFftJob *dst)
+{
+    if ((int)threadIdx.x < chunk.n) dst[threadIdx.x] = chunk.j[threadIdx.x];
+}
+
+}  // namespace
+
+bool LineFft::run_many(const std::vector<const LineFft *> &ffts, const std::vector<void *> &grids,
+                       LineFftJob **jobs, int64_t *cap, hipStream_t stream)
+{
+    static_assert(JobChunk::N >= 1 && sizeof(JobChunk) <= 4096, "job chunk");
+    if (ffts.empty()) return true;
+    const int naxes = ffts[0]->naxes_;
+    const bool f64 = ffts[0]->f64_;
+    for (const LineFft *f : ffts)
+        if (f->naxes_ != naxes || f->f64_ != f64) {
+            set_error("batched line FFT: plans of one rank and precision");
+            return false;
+        }
+    // jobs grouped by (axis, sign), each group's workgroups numbered from 0
+    struct Launch {
+        int dir;
+        size_t first, count;
+        int64_t groups;
+        size_t lds;
+    };
+    std::vector<LineFftJob> all;
+    std::vector<Launch> launches;
+    for (int a = 0; a < naxes; ++a)
+        for (int dir : {1, -1}) {
+            Launch L{dir, all.size(), 0, 0, 0};
+            for (size_t i = 0; i < ffts.size(); ++i) {
+                const LineFft &f = *ffts[i];
+                if (f.dir_ != dir) continue;
+                const LineFftAxis &x = f.ax_[a];
+                const int64_t g = (x.nu * x.nv + x.lines - 1) / x.lines;
+                if (g == 0) continue;
+                all.push_back(LineFftJob{x, grids[i], f.tw_[a], L.groups});
+                L.groups += g;
+                L.lds = std::max(L.lds, (size_t)(x.lines + 1) * (size_t)x.n * (f64 ? 16 : 8));
+            }
+            L.count = all.size() - L.first;
+            if (L.count) launches.push_back(L);
+        }
+    const int64_t n = (int64_t)all.size();
+    if (n > *cap) {
+        if (*jobs) (void)hipFree(*jobs);
+        *jobs = nullptr;
+        *cap = 0;
+        if (!hip_ok(hipMalloc(jobs, sizeof(LineFftJob) * n), "line FFT job table")) return false;
+        *cap = n;
+    }
+    for (int64_t i0 = 0; i0 < n; i0 += JobChunk::N) {
+        JobChunk chunk;
+        chunk.n = (int)std::min<int64_t>(JobChunk::N, n - i0);
+        for (int k = 0; k < chunk.n; ++k) chunk.j[k] = all[(size_t)(i0 + k)];
+        k_line_jobs<<<1, 64, 0, stream>>>(chunk, *jobs + i0);
+    }
+    for (const Launch &L : launches) {
+        if (L.groups > 0x7fffffff) {
+            set_error("batched line FFT: too many workgroups");
+            return false;
+        }
+        const dim3 g((unsigned)L.groups);
+        const LineFftJob *J = *jobs + L.first;
+        if (f64) {
+            if (L.dir > 0)
+                k_fft_lines_many<double, 1><<<g, kThreads, L.lds, stream>>>(J, (int)L.count);
+            else
+                k_fft_lines_many<double, -1><<<g, kThreads, L.lds, stream>>>(J, (int)L.count);
+        } else {
+            if (L.dir > 0)
+                k_fft_lines_many<float, 1><<<g, kThreads, L.lds, stream>>>(J, (int)L.count);
+            else
+                k_fft_lines_many<float, -1><<<g, kThreads, L.lds, stream>>>(J, (int)L.count);
+        }
+    }
+    return hip_ok(hipGetLastError(), "batched line FFT launch");
 }
 
 bool LineFft::run(void *data, int64_t vectors, hipStream_t stream) const

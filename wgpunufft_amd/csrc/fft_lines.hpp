@@ -1,4 +1,4 @@
-// Fine-grid FFT of 3D plans as three passes of batched line transforms
+// Fine-grid FFT of 2D and 3D plans as passes of batched line transforms
 // (one per axis), each line staged in LDS and transformed there by
 // Stockham passes of radix 2, 3, 4, 5, 7, 8, 11 and 13 (the fine lengths'
 // prime factors), with the passes pruned to the
@@ -13,6 +13,7 @@
 //     the kept x of every z plane, writing kept y; z over the kept (x, y)
 //     columns, writing kept z.
 //
+// (2D: x over the kept rows then y over every column, or the reverse.)
 // That is 1/4 + 1/2 + 1 of the full grid's lines instead of 3 full passes,
 // and a type-2 grid needs no zeros between the modes.  The kept window of
 // an axis of n cells holding N modes is [0, ceil(N/2)) and [n - floor(N/2), n)
@@ -24,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "nufft_internal.hpp"
 
@@ -51,6 +53,15 @@ struct LineFftAxis {
     float inv_n = 1.0f;
 };
 
+// One plan's axis pass inside a batched launch: its workgroups start at
+// group0 of the launch.
+struct LineFftJob {
+    LineFftAxis ax;
+    void *grid;
+    const void *tw;
+    int64_t group0;
+};
+
 class LineFft {
 public:
     ~LineFft();
@@ -63,6 +74,13 @@ public:
     bool init(int dim, const int64_t *fine, const int64_t *n_modes, int type, bool f64,
               int isign);
     bool run(void *data, int64_t vectors, hipStream_t stream) const;
+    // One vector of each of many plans (one rank and precision): every axis
+    // pass of all of them in one launch per transform sign.  The job table
+    // (*jobs, *cap entries) grows as needed and is filled by kernel launches
+    // (no host copies), so the call can be captured in a graph.
+    static bool run_many(const std::vector<const LineFft *> &ffts,
+                         const std::vector<void *> &grids, LineFftJob **jobs, int64_t *cap,
+                         hipStream_t stream);
 
 private:
     int naxes_ = 0;

@@ -814,6 +814,10 @@ MNUFFT_MODES_INSTANTIATE(5, double)
 MNUFFT_MODES_INSTANTIATE(6, double)
 MNUFFT_MODES_INSTANTIATE(7, double)
 MNUFFT_MODES_INSTANTIATE(8, double)
+template bool predeconvolve_kept<2, float>(const ModeParams &, const double *, const float *,
+                                           float *, int64_t, hipStream_t);
+template bool predeconvolve_kept<2, double>(const ModeParams &, const double *, const double *,
+                                            double *, int64_t, hipStream_t);
 template bool predeconvolve_kept<3, float>(const ModeParams &, const double *, const float *,
                                            float *, int64_t, hipStream_t);
 template bool predeconvolve_kept<3, double>(const ModeParams &, const double *, const double *,

@@ -158,7 +158,7 @@ Plan::~Plan()
     for (void *p : {(void *)d_inv_phihat, grid, (void *)rec_start, rec_off, (void *)rec_perm,
                     c_sorted, t3_xr, t3_cp, t3_sr, t3_factor, t3_interp, t3_box, t3_perm,
                     t3_s_sorted, t3_counts, (void *)item_off, (void *)batch_table, (void *)batch_modes,
-                    (void *)batch_spread, fft_scratch})
+                    (void *)batch_spread, fft_scratch, (void *)line_jobs})
         if (p) (void)hipFree(p);
     for (auto &sl : slots)
         for (auto &e : sl.ev)
@@ -650,7 +650,7 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         mark(kStageDeconv, false, stream);
         // the line FFT reads the modes' cells only: no zeros between them
         bool kept = false;
-        if constexpr (D == 3) {
+        if constexpr (D == 2 || D == 3) {
             if (use_line_fft()) {
                 if (!predeconvolve_kept<D, T>(mp, d_inv_phihat, in, (T *)grid, active, stream))
                     return false;
@@ -1137,6 +1137,27 @@ bool upload_table(const std::vector<E> &v, E **dst, int64_t *cap, hipStream_t st
     return hip_ok(hipGetLastError(), "batch table launch");
 }
 
+// FFT of one vector of each plan in `ps`: on the line FFT one launch per
+// axis pass and transform sign over all of them, else plan by plan.
+bool fft_batch(Plan &head, const std::vector<Plan *> &ps, hipStream_t stream)
+{
+    bool lines = !ps.empty();
+    for (Plan *p : ps) lines = lines && p->use_line_fft();
+    if (!lines) {
+        for (Plan *p : ps)
+            if (!p->fft(1, p->grid, stream)) return false;
+        return true;
+    }
+    std::vector<const LineFft *> f;
+    std::vector<void *> g;
+    for (Plan *p : ps) {
+        if (!p->ensure_fft(1)) return false;
+        f.push_back(p->line_fft.get());
+        g.push_back(p->grid);
+    }
+    return LineFft::run_many(f, g, &head.line_jobs, &head.line_jobs_cap, stream);
+}
+
 template <int D>
 bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
                           const void *const *points, const void *const *inputs,
@@ -1224,8 +1245,11 @@ bool execute_type2_many_t(int64_t count, Plan *const *plans, const int64_t *M,
     if (!predeconvolve_many<D, float>(head.batch_table, head.batch_modes, (int)count, cells,
                                       stream))
         return false;
-    for (int64_t r = 0; r < count; ++r)
-        if (!plans[order[(size_t)r]]->fft(1, plans[order[(size_t)r]]->grid, stream)) return false;
+    {
+        std::vector<Plan *> ps;
+        for (int64_t r = 0; r < count; ++r) ps.push_back(plans[order[(size_t)r]]);
+        if (!fft_batch(head, ps, stream)) return false;
+    }
     if (m_off == 0) return true;
     if (!head.batch_sorter) head.batch_sorter = std::make_unique<BinSorter>();
     BinSorter &bs = *head.batch_sorter;
@@ -1383,10 +1407,10 @@ bool execute_type1_many_t(int64_t count, Plan *const *plans, const int64_t *M,
                                              (const T *)head.rec_off,
                                              (const T *)head.c_sorted, stream))
             return false;
-        for (int64_t r = 0; r < count; ++r) {
-            Plan &p = *plans[order[(size_t)r]];
-            if (M[order[(size_t)r]] > 0 && !p.fft(1, p.grid, stream)) return false;
-        }
+        std::vector<Plan *> ps;
+        for (int64_t r = 0; r < count; ++r)
+            if (M[order[(size_t)r]] > 0) ps.push_back(plans[order[(size_t)r]]);
+        if (!fft_batch(head, ps, stream)) return false;
     }
     return deconvolve_many<D, T>(head.batch_table, head.batch_modes, (int)count, modes, stream);
 }

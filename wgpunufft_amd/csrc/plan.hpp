@@ -193,6 +193,8 @@ struct Plan {
     ModeProblem *batch_modes = nullptr;     // mode stages of the problems
     SpreadProblem *batch_spread = nullptr;  // spread blocks of the problems (type 1)
     int64_t batch_modes_cap = 0, batch_spread_cap = 0;
+    LineFftJob *line_jobs = nullptr;  // batched line-FFT job table
+    int64_t line_jobs_cap = 0;
     bool bin_sort(int64_t M, const void *pts, uint32_t *sorted, uint32_t *offsets,
                   hipStream_t stream);
     // The same permutation and offsets as bin_sort, with every intermediate
