@@ -298,7 +298,7 @@ int mnufft_set_bin_tuning(mnufft_plan plan, int64_t rows_side0, int32_t plain_ro
 /*
  * FFT of the fine grid (no counterpart in the reference, whose FFT is its
  * wgpu-fft dependency): MNUFFT_FFT_LINES (default) runs 2D and 3D grids
- * whose axes are at most 4096 (FP32) / 2048 (FP64) cells long as
+ * whose axes are at most 4864 (FP32) / 2432 (FP64) cells long as
  * one pass of line transforms in LDS per axis, pruned to the lines that
  * carry modes (batches of plans: one launch per pass over all of them); MNUFFT_FFT_ROCFFT keeps rocFFT's plan.  Other grids always use
  * rocFFT.  A type-3 plan applies it to its inner type-2 plan.  Takes effect

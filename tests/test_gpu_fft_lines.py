@@ -103,7 +103,7 @@ def test_line_fft_type3_and_unsupported_grids():
     p2 = W.NufftPlan.type1_gpu(W.NufftConfig([32, 32], 1e-6))
     assert p2.fft_engine() == "lines"
     assert W.NufftPlan.type1_gpu(W.NufftConfig([100], 1e-6)).fft_engine() == "rocfft"
-    big = W.NufftPlan.type2_gpu(W.NufftConfig([2100, 8, 8], 1e-3))
+    big = W.NufftPlan.type2_gpu(W.NufftConfig([2500, 8, 8], 1e-3))
     assert big.fft_engine() == "rocfft"
     with pytest.raises(ValueError):
         p2.set_fft_engine("other")

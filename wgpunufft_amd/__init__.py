@@ -166,7 +166,7 @@ class _PlanBase:
 
     def set_fft_engine(self, engine):
         """FFT of the fine grid: "lines" (default; 2D and 3D grids of axes up
-        to 4096 FP32 / 2048 FP64 cells as pruned line-transform passes) or
+        to 4864 FP32 / 2432 FP64 cells as pruned line-transform passes) or
         "rocfft".  Type-3 plans apply it to their inner plan."""
         codes = {"lines": 0, "rocfft": 1}
         if engine not in codes:

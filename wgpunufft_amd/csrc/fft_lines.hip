@@ -25,7 +25,7 @@ namespace mnufft {
 namespace {
 
 constexpr int kThreads = 512;
-constexpr int kMaxGroupLines = 256;
+constexpr int kMaxGroupLines = 128;
 
 // Dynamic LDS per workgroup: lines + twiddles, at most 80 KB (two
 // workgroups per CU) where the device allows it, else 64 KB; the static
@@ -35,17 +35,19 @@ template <typename T>
 struct LdsElems;
 template <>
 struct LdsElems<float> {
-    static constexpr int value = 9984;  // 80 KB less the line bases
-    static constexpr int base = 8192;
+    static constexpr int value = 9728;  // 80 KB less the per-line arrays
+    static constexpr int base = 7680;   // 64 KB less the per-line arrays
     static constexpr int columns = 16;
 };
 template <>
 struct LdsElems<double> {
-    static constexpr int value = 4992;
-    static constexpr int base = 4096;
+    static constexpr int value = 4864;
+    static constexpr int base = 3840;
     static constexpr int columns = 8;
 };
-constexpr size_t kLdsWide = 80 * 1024 - kMaxGroupLines * sizeof(int64_t);
+// static LDS: line bases, mode rows and two factors per line
+constexpr size_t kLineLds = kMaxGroupLines * (2 * sizeof(int64_t) + 2 * sizeof(double));
+constexpr size_t kLdsWide = 80 * 1024 - kLineLds;
 
 template <typename T>
 struct Cx {
@@ -242,11 +244,31 @@ __device__ inline void dft(Cx<T> *u)
 // butterflies into registers, the workgroup waits, every thread writes.
 constexpr int kFromGrid = 1, kToGrid = 2;
 
-template <typename T, int DIR, int R, int LDS, int MODE>
+// Per-line data of a workgroup in LDS: grid base, mode row and the line's
+// deconvolution factors (type 2: the u and v axes' 1/phi_hat; type 1: their
+// product in axis order).
+struct LineData {
+    int64_t *base, *mrow;
+    double *f1, *f2;
+    void *modes;  // this vector's modes (IO != 0)
+};
+
+// Mode storage index of signed frequency m on an axis of N modes.
+__device__ inline int64_t mode_slot(int64_t m, int64_t N, int centered)
+{
+    return centered ? m + N / 2 : (m >= 0 ? m : m + N);
+}
+
+// IO (the launch's mode reads or writes): 0 grid only; 1 the first pass
+// reads the modes and predeconvolves them (type 2); 2 the last pass writes
+// the deconvolved modes (type 1).
+template <typename T, int DIR, int R, int LDS, int MODE, int IO>
 __device__ inline void line_pass(Cx<T> *X, const Cx<T> *tw, const LineFftAxis &P,
                                  const LineFftPass &ps, Cx<T> *__restrict__ grid,
-                                 const int64_t *base)
+                                 const LineData &ld)
 {
+    const int64_t *base = ld.base;
+    Cx<T> *modes = (Cx<T> *)ld.modes;
     constexpr int MAXB = (LDS / R + kThreads - 1) / kThreads;
     const int n = P.n, nb = n / R, total = P.lines * nb;
     const int ks = P.line_major ? 1 : P.lines, cs = P.line_major ? n : 1;
@@ -270,12 +292,33 @@ __device__ inline void line_pass(Cx<T> *X, const Cx<T> *tw, const LineFftAxis &P
             line[s] = c;
             if constexpr ((MODE & kFromGrid) != 0) {
                 const int64_t bc = base[c];
+                if constexpr (IO == 1) {
+                    // the predeconvolution: mode times (1/phi_hat) in axis order
+                    const int64_t mr = ld.mrow[c];
+                    const double g1 = ld.f1[c], g2 = ld.f2[c];
 #pragma unroll
-                for (int t = 0; t < R; ++t) {
-                    const int e = i + t * nb;
-                    u[s][t] = Cx<T>{(T)0, (T)0};
-                    if (bc >= 0 && (e < P.in_hi || e >= n - P.in_lo))
-                        u[s][t] = grid[bc + (int64_t)e * P.estride];
+                    for (int t = 0; t < R; ++t) {
+                        const int e = i + t * nb;
+                        u[s][t] = Cx<T>{(T)0, (T)0};
+                        if (bc >= 0 && (e < P.in_hi || e >= n - P.in_lo)) {
+                            const int64_t m = e < P.in_hi ? e : (int64_t)e - n;
+                            double amp = 1.0;
+                            amp *= P.inv_phihat[P.eph + (m < 0 ? -m : m)];
+                            amp *= g1;
+                            amp *= g2;
+                            const Cx<T> src = modes[mr + mode_slot(m, P.eN, P.centered) * P.ems];
+                            const T ta = (T)amp;
+                            u[s][t] = Cx<T>{src.re * ta, src.im * ta};
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        const int e = i + t * nb;
+                        u[s][t] = Cx<T>{(T)0, (T)0};
+                        if (bc >= 0 && (e < P.in_hi || e >= n - P.in_lo))
+                            u[s][t] = grid[bc + (int64_t)e * P.estride];
+                    }
                 }
             } else {
 #pragma unroll
@@ -292,11 +335,29 @@ __device__ inline void line_pass(Cx<T> *X, const Cx<T> *tw, const LineFftAxis &P
         for (int s = 0; s < MAXB; ++s) {
             if (dst[s] >= 0) {
                 const int64_t bc = base[line[s]];
+                if constexpr (IO == 2) {
+                    // the deconvolution: cell times (1/phi_hat) in axis order
+                    const int64_t mr = ld.mrow[line[s]];
+                    const double g = ld.f1[line[s]];
 #pragma unroll
-                for (int t = 0; t < R; ++t) {
-                    const int e = dst[s] + t * ps.p;
-                    if (bc >= 0 && (e < P.out_hi || e >= n - P.out_lo))
-                        grid[bc + (int64_t)e * P.estride] = u[s][t];
+                    for (int t = 0; t < R; ++t) {
+                        const int e = dst[s] + t * ps.p;
+                        if (bc >= 0 && (e < P.out_hi || e >= n - P.out_lo)) {
+                            const int64_t m = e < P.out_hi ? e : (int64_t)e - n;
+                            double amp = g;
+                            amp *= P.inv_phihat[P.eph + (m < 0 ? -m : m)];
+                            const T ta = (T)amp;
+                            modes[mr + mode_slot(m, P.eN, P.centered) * P.ems] =
+                                Cx<T>{u[s][t].re * ta, u[s][t].im * ta};
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        const int e = dst[s] + t * ps.p;
+                        if (bc >= 0 && (e < P.out_hi || e >= n - P.out_lo))
+                            grid[bc + (int64_t)e * P.estride] = u[s][t];
+                    }
                 }
             }
         }
@@ -314,29 +375,30 @@ __device__ inline void line_pass(Cx<T> *X, const Cx<T> *tw, const LineFftAxis &P
     }
 }
 
-template <typename T, int DIR, int LDS, int MODE>
+template <typename T, int DIR, int LDS, int MODE, int IO>
 __device__ inline void line_pass_any(Cx<T> *X, const Cx<T> *tw, const LineFftAxis &P,
-                                     const LineFftPass &ps, Cx<T> *grid, const int64_t *base)
+                                     const LineFftPass &ps, Cx<T> *grid, const LineData &base)
 {
     switch (ps.radix) {
-    case 2: line_pass<T, DIR, 2, LDS, MODE>(X, tw, P, ps, grid, base); break;
-    case 3: line_pass<T, DIR, 3, LDS, MODE>(X, tw, P, ps, grid, base); break;
-    case 4: line_pass<T, DIR, 4, LDS, MODE>(X, tw, P, ps, grid, base); break;
-    case 5: line_pass<T, DIR, 5, LDS, MODE>(X, tw, P, ps, grid, base); break;
-    case 7: line_pass<T, DIR, 7, LDS, MODE>(X, tw, P, ps, grid, base); break;
-    case 8: line_pass<T, DIR, 8, LDS, MODE>(X, tw, P, ps, grid, base); break;
-    case 11: line_pass<T, DIR, 11, LDS, MODE>(X, tw, P, ps, grid, base); break;
-    default: line_pass<T, DIR, 13, LDS, MODE>(X, tw, P, ps, grid, base); break;
+    case 2: line_pass<T, DIR, 2, LDS, MODE, IO>(X, tw, P, ps, grid, base); break;
+    case 3: line_pass<T, DIR, 3, LDS, MODE, IO>(X, tw, P, ps, grid, base); break;
+    case 4: line_pass<T, DIR, 4, LDS, MODE, IO>(X, tw, P, ps, grid, base); break;
+    case 5: line_pass<T, DIR, 5, LDS, MODE, IO>(X, tw, P, ps, grid, base); break;
+    case 7: line_pass<T, DIR, 7, LDS, MODE, IO>(X, tw, P, ps, grid, base); break;
+    case 8: line_pass<T, DIR, 8, LDS, MODE, IO>(X, tw, P, ps, grid, base); break;
+    case 11: line_pass<T, DIR, 11, LDS, MODE, IO>(X, tw, P, ps, grid, base); break;
+    default: line_pass<T, DIR, 13, LDS, MODE, IO>(X, tw, P, ps, grid, base); break;
     }
 }
 
 // The transform of workgroup `group`'s lines of one axis pass (P) of the
 // grid `grid` (one vector), X: (lines + 1) n elements of LDS.
-template <typename T, int DIR>
+template <typename T, int DIR, int IO>
 __device__ __forceinline__ void fft_lines_body(const LineFftAxis &P, Cx<T> *__restrict__ grid,
                                                const Cx<T> *__restrict__ tw_global,
-                                               int64_t group, Cx<T> *X, int64_t *base)
+                                               int64_t group, Cx<T> *X, const LineData &ld)
 {
+    int64_t *base = ld.base;
     constexpr int LDS = LdsElems<T>::value;
     Cx<T> *tw = X + P.lines * P.n;
     const int n = P.n;
@@ -351,50 +413,87 @@ __device__ __forceinline__ void fft_lines_body(const LineFftAxis &P, Cx<T> *__re
             const int64_t uu = u < P.u_hi ? u : u + P.u_gap;
             const int64_t vv = w < P.v_hi ? w : w + P.v_gap;
             b = uu * P.su + vv * P.sv;
+            if constexpr (IO != 0) {
+                // the line's mode row and factors (u, v: fine indices of the
+                // kept windows of their axes)
+                int64_t mr = 0;
+                double fu = 1.0, fv = 1.0;
+                if (P.has_u) {
+                    const int64_t m = uu < (P.uN + 1) / 2 ? uu : uu - P.un;
+                    mr += mode_slot(m, P.uN, P.centered) * P.ums;
+                    fu = P.inv_phihat[P.uph + (m < 0 ? -m : m)];
+                }
+                if (P.has_v) {
+                    const int64_t m = vv < (P.vN + 1) / 2 ? vv : vv - P.vn;
+                    mr += mode_slot(m, P.vN, P.centered) * P.vms;
+                    fv = P.inv_phihat[P.vph + (m < 0 ? -m : m)];
+                }
+                ld.mrow[c] = mr;
+                if constexpr (IO == 1) {
+                    ld.f1[c] = fu;
+                    ld.f2[c] = fv;
+                } else {
+                    double amp = 1.0;
+                    amp *= fu;
+                    if (P.has_v) amp *= fv;
+                    ld.f1[c] = amp;
+                }
+            }
         }
         base[c] = b;
     }
     __syncthreads();
     const int np = P.npass;
     if (np == 1) {
-        line_pass_any<T, DIR, LDS, kFromGrid | kToGrid>(X, tw, P, P.pass[0], grid, base);
+        line_pass_any<T, DIR, LDS, kFromGrid | kToGrid, IO>(X, tw, P, P.pass[0], grid, ld);
         return;
     }
     if (np == 0) return;
     // first pass from the grid, the middle ones in LDS grouped by radix (5s,
     // 3s, 7s, 11s, 13s, 8s, then a 4 or a 2; separate loops keep each
     // radix's registers to itself), the last one to the grid
-    line_pass_any<T, DIR, LDS, kFromGrid>(X, tw, P, P.pass[0], grid, base);
+    line_pass_any<T, DIR, LDS, kFromGrid, IO == 1 ? 1 : 0>(X, tw, P, P.pass[0], grid, ld);
     const int last = np - 1;
     int s = 1;
     for (; s < last && P.pass[s].radix == 5; ++s)
-        line_pass<T, DIR, 5, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+        line_pass<T, DIR, 5, LDS, 0, 0>(X, tw, P, P.pass[s], grid, ld);
     for (; s < last && P.pass[s].radix == 3; ++s)
-        line_pass<T, DIR, 3, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+        line_pass<T, DIR, 3, LDS, 0, 0>(X, tw, P, P.pass[s], grid, ld);
     for (; s < last && P.pass[s].radix == 7; ++s)
-        line_pass<T, DIR, 7, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+        line_pass<T, DIR, 7, LDS, 0, 0>(X, tw, P, P.pass[s], grid, ld);
     for (; s < last && P.pass[s].radix == 11; ++s)
-        line_pass<T, DIR, 11, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+        line_pass<T, DIR, 11, LDS, 0, 0>(X, tw, P, P.pass[s], grid, ld);
     for (; s < last && P.pass[s].radix == 13; ++s)
-        line_pass<T, DIR, 13, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+        line_pass<T, DIR, 13, LDS, 0, 0>(X, tw, P, P.pass[s], grid, ld);
     for (; s < last && P.pass[s].radix == 8; ++s)
-        line_pass<T, DIR, 8, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+        line_pass<T, DIR, 8, LDS, 0, 0>(X, tw, P, P.pass[s], grid, ld);
     for (; s < last && P.pass[s].radix == 4; ++s)
-        line_pass<T, DIR, 4, LDS, 0>(X, tw, P, P.pass[s], grid, base);
+        line_pass<T, DIR, 4, LDS, 0, 0>(X, tw, P, P.pass[s], grid, ld);
     for (; s < last && P.pass[s].radix == 2; ++s)
-        line_pass<T, DIR, 2, LDS, 0>(X, tw, P, P.pass[s], grid, base);
-    line_pass_any<T, DIR, LDS, kToGrid>(X, tw, P, P.pass[last], grid, base);
+        line_pass<T, DIR, 2, LDS, 0, 0>(X, tw, P, P.pass[s], grid, ld);
+    line_pass_any<T, DIR, LDS, kToGrid, IO == 2 ? 2 : 0>(X, tw, P, P.pass[last], grid, ld);
 }
 
-// One plan's axis pass: workgroup (group, vector).
-template <typename T, int DIR>
-__global__ __launch_bounds__(kThreads) void k_fft_lines(LineFftAxis P, T *__restrict__ data,
+// One plan's axis pass: workgroup (group, vector); IO / modes_all: this
+// launch's mode reads or writes (all vectors' modes).
+template <typename T, int DIR, int IO>
+__global__ __launch_bounds__(kThreads) void k_fft_lines(LineFftAxis P,
+                                                        void *modes_all, T *__restrict__ data,
                                                         const Cx<T> *__restrict__ tw_global)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ int64_t base[kMaxGroupLines];
     Cx<T> *grid = (Cx<T> *)data + (int64_t)blockIdx.y * P.vec_stride;
-    fft_lines_body<T, DIR>(P, grid, tw_global, blockIdx.x, (Cx<T> *)lds_raw, base);
+    if constexpr (IO == 0) {
+        __shared__ int64_t base[kMaxGroupLines];
+        fft_lines_body<T, DIR, 0>(P, grid, tw_global, blockIdx.x, (Cx<T> *)lds_raw,
+                                  LineData{base, nullptr, nullptr, nullptr, nullptr});
+    } else {
+        __shared__ int64_t base[kMaxGroupLines], mrow[kMaxGroupLines];
+        __shared__ double f1[kMaxGroupLines], f2[kMaxGroupLines];
+        Cx<T> *modes = (Cx<T> *)modes_all + (int64_t)blockIdx.y * P.mode_total;
+        fft_lines_body<T, DIR, IO>(P, grid, tw_global, blockIdx.x, (Cx<T> *)lds_raw,
+                                   LineData{base, mrow, f1, f2, modes});
+    }
 }
 
 // The same axis pass of many plans (one vector each) in one launch: job q
@@ -415,8 +514,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_lines_many(const LineFftJob *_
             hi = mid - 1;
     }
     const LineFftJob &J = jobs[lo];
-    fft_lines_body<T, DIR>(J.ax, (Cx<T> *)J.grid, (const Cx<T> *)J.tw, b - J.group0,
-                           (Cx<T> *)lds_raw, base);
+    fft_lines_body<T, DIR, 0>(J.ax, (Cx<T> *)J.grid, (const Cx<T> *)J.tw, b - J.group0,
+                              (Cx<T> *)lds_raw, LineData{base, nullptr, nullptr, nullptr, nullptr});
 }
 
 // Radix plan: 5s, 3s, 7s, 11s, 13s, 8s, then a 4 or a 2; false for other
@@ -472,30 +571,19 @@ bool wide_lds()
     static std::once_flag once;
     static bool ok = false;
     std::call_once(once, [] {
-        ok = hipFuncSetAttribute((const void *)k_fft_lines<float, 1>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kLdsWide) == hipSuccess &&
-             hipFuncSetAttribute((const void *)k_fft_lines<float, -1>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kLdsWide) == hipSuccess &&
-             hipFuncSetAttribute((const void *)k_fft_lines<double, 1>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kLdsWide) == hipSuccess &&
-             hipFuncSetAttribute((const void *)k_fft_lines<double, -1>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kLdsWide) == hipSuccess &&
-             hipFuncSetAttribute((const void *)k_fft_lines_many<float, 1>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kLdsWide) == hipSuccess &&
-             hipFuncSetAttribute((const void *)k_fft_lines_many<float, -1>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kLdsWide) == hipSuccess &&
-             hipFuncSetAttribute((const void *)k_fft_lines_many<double, 1>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kLdsWide) == hipSuccess &&
-             hipFuncSetAttribute((const void *)k_fft_lines_many<double, -1>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kLdsWide) == hipSuccess;
+        const void *kernels[] = {
+            (const void *)k_fft_lines<float, 1, 0>,      (const void *)k_fft_lines<float, 1, 1>,
+            (const void *)k_fft_lines<float, 1, 2>,      (const void *)k_fft_lines<float, -1, 0>,
+            (const void *)k_fft_lines<float, -1, 1>,     (const void *)k_fft_lines<float, -1, 2>,
+            (const void *)k_fft_lines<double, 1, 0>,     (const void *)k_fft_lines<double, 1, 1>,
+            (const void *)k_fft_lines<double, 1, 2>,     (const void *)k_fft_lines<double, -1, 0>,
+            (const void *)k_fft_lines<double, -1, 1>,    (const void *)k_fft_lines<double, -1, 2>,
+            (const void *)k_fft_lines_many<float, 1>,    (const void *)k_fft_lines_many<float, -1>,
+            (const void *)k_fft_lines_many<double, 1>,   (const void *)k_fft_lines_many<double, -1>};
+        ok = true;
+        for (const void *k : kernels)
+            ok = ok && hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)kLdsWide) == hipSuccess;
         (void)hipGetLastError();
     });
     return ok;
@@ -546,7 +634,7 @@ bool LineFft::supported(int dim, const int64_t *fine, bool f64)
     for (int a = 0; a < dim; ++a) {
         LineFftAxis ax;
         if (fine[a] < 1 || fine[a] > 65536 || !radix_plan((int)fine[a], &ax) ||
-            !group_lines((int)fine[a], a == 0, f64, 1, false, &ax))
+            !group_lines((int)fine[a], a == 0, f64, 1, wide_lds(), &ax))
             return false;
     }
     return true;
@@ -564,7 +652,8 @@ LineFft::~LineFft()
 }
 
 bool LineFft::init(int dim, const int64_t *fine, const int64_t *n_modes, int type, bool f64,
-                   int isign)
+                   int isign, const double *inv_phihat, const int64_t *phihat_off,
+                   bool centered)
 {
     if (!supported(dim, fine, f64)) {
         set_error("line FFT needs a 2D or 3D grid of 2,3,5,7,11,13-smooth lengths within the "
@@ -572,6 +661,7 @@ bool LineFft::init(int dim, const int64_t *fine, const int64_t *n_modes, int typ
         return false;
     }
     naxes_ = dim;
+    type_ = type;
     f64_ = f64;
     dir_ = isign >= 0 ? 1 : -1;
     const int64_t n0 = fine[0], n1 = fine[1], n2 = dim == 3 ? fine[2] : 1;
@@ -630,6 +720,35 @@ bool LineFft::init(int dim, const int64_t *fine, const int64_t *n_modes, int typ
     const bool wide = wide_lds();
     for (int a = 0; a < dim; ++a)
         group_lines((int)fine[a], a == 0, f64, ax_[a].nu * ax_[a].nv, wide, &ax_[a]);
+    // the pass that reads (type 2: x) or writes (type 1: the last axis) modes
+    LineFftAxis &mx = type == 2 ? ax_[0] : ax_[dim - 1];
+    mx.centered = centered ? 1 : 0;
+    mx.inv_phihat = inv_phihat;
+    int64_t mstride[3] = {1, 1, 1};
+    int64_t stride = 1;
+    for (int a = 0; a < dim; ++a) {
+        mstride[a] = stride;
+        stride *= n_modes[a];
+    }
+    mx.mode_total = stride;
+    // type 2: x lines over (y, z); type 1: last-axis lines over (x, y)
+    const int ua = type == 2 ? 1 : 0, va = dim == 3 ? (type == 2 ? 2 : 1) : -1;
+    const int ea = type == 2 ? 0 : dim - 1;
+    mx.has_u = 1;
+    mx.uN = n_modes[ua];
+    mx.un = fine[ua];
+    mx.ums = mstride[ua];
+    mx.uph = phihat_off[ua];
+    mx.has_v = va >= 0 ? 1 : 0;
+    if (va >= 0) {
+        mx.vN = n_modes[va];
+        mx.vn = fine[va];
+        mx.vms = mstride[va];
+        mx.vph = phihat_off[va];
+    }
+    mx.eN = n_modes[ea];
+    mx.ems = mstride[ea];
+    mx.eph = phihat_off[ea];
     // twiddles w_n^m = exp(dir 2 pi i m / n), from long double
     for (int a = 0; a < dim; ++a) {
         const int n = (int)fine[a];
@@ -747,31 +866,56 @@ bool LineFft::run_many(const std::vector<const LineFft *> &ffts, const std::vect
     return hip_ok(hipGetLastError(), "batched line FFT launch");
 }
 
-bool LineFft::run(void *data, int64_t vectors, hipStream_t stream) const
+namespace {
+
+template <typename T>
+void launch_axis(int dir, int io, dim3 g, size_t lds, hipStream_t stream, const LineFftAxis *hx,
+                 void *mv, T *d, const Cx<T> *t)
+{
+    if (dir > 0) {
+        if (io == 1)
+            k_fft_lines<T, 1, 1><<<g, kThreads, lds, stream>>>(*hx, mv, d, t);
+        else if (io == 2)
+            k_fft_lines<T, 1, 2><<<g, kThreads, lds, stream>>>(*hx, mv, d, t);
+        else
+            k_fft_lines<T, 1, 0><<<g, kThreads, lds, stream>>>(*hx, mv, d, t);
+    } else {
+        if (io == 1)
+            k_fft_lines<T, -1, 1><<<g, kThreads, lds, stream>>>(*hx, mv, d, t);
+        else if (io == 2)
+            k_fft_lines<T, -1, 2><<<g, kThreads, lds, stream>>>(*hx, mv, d, t);
+        else
+            k_fft_lines<T, -1, 0><<<g, kThreads, lds, stream>>>(*hx, mv, d, t);
+    }
+}
+
+}  // namespace
+
+bool LineFft::run(void *data, int64_t vectors, hipStream_t stream, void *modes) const
 {
     if (vectors <= 0) return true;
+    const size_t eb = f64_ ? 16 : 8;
     for (int64_t v0 = 0; v0 < vectors; v0 += 65535) {
         const unsigned vc = (unsigned)std::min<int64_t>(65535, vectors - v0);
         for (int a = 0; a < naxes_; ++a) {
             const LineFftAxis &x = ax_[a];
+            const bool io = modes && a == (type_ == 2 ? 0 : naxes_ - 1);
+            const int mode_io = io ? (type_ == 2 ? 1 : 2) : 0;
+            void *mv = io ? (void *)((char *)modes + (size_t)v0 * (size_t)x.mode_total * eb)
+                          : nullptr;
             const int64_t groups = (x.nu * x.nv + x.lines - 1) / x.lines;
             if (groups == 0) continue;
             const dim3 g((unsigned)groups, vc);
-            const size_t lds = (size_t)(x.lines + 1) * (size_t)x.n * (f64_ ? 16 : 8);
+            const size_t lds = (size_t)(x.lines + 1) * (size_t)x.n * eb;
+            const LineFftAxis *hx = &ax_[a];
             if (f64_) {
                 double *d = (double *)data + 2 * v0 * fine_total_;
-                const Cx<double> *t = (const Cx<double> *)tw_[a];
-                if (dir_ > 0)
-                    k_fft_lines<double, 1><<<g, kThreads, lds, stream>>>(x, d, t);
-                else
-                    k_fft_lines<double, -1><<<g, kThreads, lds, stream>>>(x, d, t);
+                launch_axis<double>(dir_, mode_io, g, lds, stream, hx, mv, d,
+                                    (const Cx<double> *)tw_[a]);
             } else {
                 float *d = (float *)data + 2 * v0 * fine_total_;
-                const Cx<float> *t = (const Cx<float> *)tw_[a];
-                if (dir_ > 0)
-                    k_fft_lines<float, 1><<<g, kThreads, lds, stream>>>(x, d, t);
-                else
-                    k_fft_lines<float, -1><<<g, kThreads, lds, stream>>>(x, d, t);
+                launch_axis<float>(dir_, mode_io, g, lds, stream, hx, mv, d,
+                                   (const Cx<float> *)tw_[a]);
             }
         }
     }

@@ -18,8 +18,8 @@
 // and a type-2 grid needs no zeros between the modes.  The kept window of
 // an axis of n cells holding N modes is [0, ceil(N/2)) and [n - floor(N/2), n)
 // (the fine index of mode m is m mod n).  Unnormalized; isign > 0 is
-// exp(+i k x).  Every axis at most 4096 (FP32) or 2048 (FP64) cells (one
-// line and its twiddles within 64 KB of LDS).
+// exp(+i k x).  Every axis at most 4864 (FP32) or 2432 (FP64) cells (one
+// line and its twiddles within the 76 KB LDS budget).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -51,6 +51,17 @@ struct LineFftAxis {
     int in_hi = 1, in_lo = 0, out_hi = 1, out_lo = 0;  // elements read / written
     int64_t vec_stride = 0;
     float inv_n = 1.0f;
+    // Modes in place of the grid (chosen per launch): the first pass reads
+    // each cell from the modes, scaled by the deconvolution factors (type 2),
+    // or the last pass writes each kept cell to the modes, scaled (type 1).
+    // Per role (u, v, e = the line's elements): modes N, fine cells n,
+    // mode-array stride and the axis's offset in the 1/phi_hat table.
+    int centered = 0, has_u = 0, has_v = 0;
+    int64_t uN = 1, un = 1, ums = 0, uph = 0;
+    int64_t vN = 1, vn = 1, vms = 0, vph = 0;
+    int64_t eN = 1, ems = 0, eph = 0;
+    int64_t mode_total = 0;
+    const double *inv_phihat = nullptr;
 };
 
 // One plan's axis pass inside a batched launch: its workgroups start at
@@ -68,12 +79,17 @@ public:
     // False (no error set) when the grid does not qualify.
     static bool supported(int dim, const int64_t *fine, bool f64);
     // Longest axis whose strided passes still move 64-byte rows (4 FP64 or
-    // 8 FP32 adjacent columns per workgroup): 998 FP64, 1109 FP32 cells.
+    // 8 FP32 adjacent columns per workgroup): 972 FP64, 1080 FP32 cells.
     static int64_t row_length_limit(bool f64);
     // type 1: output pruned to the modes; type 2: input pruned to them.
+    // inv_phihat / phihat_off / centered: the deconvolution factors 1/phi_hat
+    // per axis and the mode order, for the runs that read or write modes.
     bool init(int dim, const int64_t *fine, const int64_t *n_modes, int type, bool f64,
-              int isign);
-    bool run(void *data, int64_t vectors, hipStream_t stream) const;
+              int isign, const double *inv_phihat, const int64_t *phihat_off, bool centered);
+    // With `modes`: type 2 reads the modes (vectors x mode_total) and scales
+    // them as the predeconvolution does, type 1 writes the deconvolved modes;
+    // the results equal predeconvolution + FFT (+ deconvolution) bit for bit.
+    bool run(void *data, int64_t vectors, hipStream_t stream, void *modes = nullptr) const;
     // One vector of each of many plans (one rank and precision): every axis
     // pass of all of them in one launch per transform sign.  The job table
     // (*jobs, *cap entries) grows as needed and is filled by kernel launches
@@ -83,7 +99,7 @@ public:
                          hipStream_t stream);
 
 private:
-    int naxes_ = 0;
+    int naxes_ = 0, type_ = 1;
     bool f64_ = false;
     int dir_ = 1;
     int64_t fine_total_ = 1;

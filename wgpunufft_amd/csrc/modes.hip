@@ -296,53 +296,6 @@ __global__ __launch_bounds__(256) void k_predeconv_rows(ModeParams mp,
     }
 }
 
-// Predeconvolution of the modes' cells only (the other fine cells are left
-// as they are, for FFTs that never read them): workgroup (b, c) takes modes
-// [256 c, 256 c + 256) of the mode rows b, b + gridDim.x, ... (a row: the
-// modes of one vector with every axis above 0 fixed).  Values and factors as
-// in k_predeconv_rows, so every written cell has the same bits.
-template <int D, typename T>
-__global__ __launch_bounds__(256) void k_predeconv_kept(ModeParams mp,
-                                                        const double *__restrict__ inv_phihat,
-                                                        const T *__restrict__ modes,
-                                                        T *__restrict__ grid,
-                                                        uint32_t rows_per_vector, uint32_t rows)
-{
-    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
-    const uint32_t n0 = (uint32_t)mp.fine[0], N0 = (uint32_t)mp.n_modes[0];
-    const uint32_t k0 = blockIdx.y * 256u + threadIdx.x;
-    if (k0 >= N0) return;
-    const int64_t m0 = mp.centered ? (int64_t)k0 - N0 / 2
-                                   : (k0 < (N0 + 1) / 2 ? (int64_t)k0 : (int64_t)k0 - N0);
-    const int64_t f0 = m0 < 0 ? m0 + n0 : m0;
-    const double ph0 = inv_phihat[mp.phihat_off[0] + (m0 < 0 ? -m0 : m0)];
-    for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x) {
-        const uint32_t v = r / rows_per_vector;
-        const uint32_t row = r - v * rows_per_vector;
-        uint32_t rest = row;
-        int64_t fine_row = 0, stride = 1;
-        double amp = 1.0;
-        amp *= ph0;
-#pragma unroll
-        for (int a = 1; a < D; ++a) {
-            const uint32_t N = (uint32_t)mp.n_modes[a], n = (uint32_t)mp.fine[a];
-            const uint32_t ka = rest % N;
-            rest /= N;
-            const int64_t ma = mp.centered ? (int64_t)ka - N / 2
-                                           : (ka < (N + 1) / 2 ? (int64_t)ka : (int64_t)ka - N);
-            fine_row += (ma < 0 ? ma + n : ma) * stride;
-            stride *= n;
-            amp *= inv_phihat[mp.phihat_off[a] + (ma < 0 ? -ma : ma)];
-        }
-        const T *src = modes + 2 * ((int64_t)v * mp.mode_total + (int64_t)row * N0 + k0);
-        const T ta = (T)amp;
-        V2 val;
-        val.x = src[0] * ta;
-        val.y = src[1] * ta;
-        *(V2 *)(grid + 2 * ((int64_t)v * mp.fine_total + fine_row * n0 + f0)) = val;
-    }
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_fill(T *__restrict__ p, int64_t n)
 {
@@ -630,25 +583,6 @@ bool predeconvolve(const ModeParams &mp, const double *inv_phihat, const T *mode
     return hip_ok(hipGetLastError(), "predeconvolution launch");
 }
 
-template <int D, typename T>
-bool predeconvolve_kept(const ModeParams &mp, const double *inv_phihat, const T *modes, T *grid,
-                        int64_t vectors, hipStream_t stream)
-{
-    if (mp.mode_total * vectors == 0) return true;
-    const int64_t rows_per_vector = mp.mode_total / mp.n_modes[0];
-    const int64_t rows = rows_per_vector * vectors;
-    const int64_t chunks = (mp.n_modes[0] + 255) / 256;
-    if (rows > 0x7fffffff || chunks > 65535) {
-        set_error("predeconvolution of the modes' cells needs fewer than 2^31 mode rows");
-        return false;
-    }
-    const int64_t per = std::max<int64_t>(1, 8192 / chunks);
-    const unsigned gx = (unsigned)std::min<int64_t>(rows, per);
-    k_predeconv_kept<D, T><<<dim3(gx, (unsigned)chunks), 256, 0, stream>>>(
-        mp, inv_phihat, modes, grid, (uint32_t)rows_per_vector, (uint32_t)rows);
-    return hip_ok(hipGetLastError(), "predeconvolution launch");
-}
-
 template <typename T>
 bool fill_zero(T *p, int64_t n, hipStream_t stream)
 {
@@ -814,14 +748,6 @@ MNUFFT_MODES_INSTANTIATE(5, double)
 MNUFFT_MODES_INSTANTIATE(6, double)
 MNUFFT_MODES_INSTANTIATE(7, double)
 MNUFFT_MODES_INSTANTIATE(8, double)
-template bool predeconvolve_kept<2, float>(const ModeParams &, const double *, const float *,
-                                           float *, int64_t, hipStream_t);
-template bool predeconvolve_kept<2, double>(const ModeParams &, const double *, const double *,
-                                            double *, int64_t, hipStream_t);
-template bool predeconvolve_kept<3, float>(const ModeParams &, const double *, const float *,
-                                           float *, int64_t, hipStream_t);
-template bool predeconvolve_kept<3, double>(const ModeParams &, const double *, const double *,
-                                            double *, int64_t, hipStream_t);
 template bool fill_zero<float>(float *, int64_t, hipStream_t);
 template bool fill_zero<double>(double *, int64_t, hipStream_t);
 template bool fill_zero<uint32_t>(uint32_t *, int64_t, hipStream_t);

@@ -71,11 +71,6 @@ bool predeconvolve_many(const BatchProblem *probs, const ModeProblem *mprobs, in
 template <int D, typename T>
 bool predeconvolve(const ModeParams &mp, const double *inv_phihat, const T *modes, T *grid,
                    int64_t vectors, hipStream_t stream);
-// Predeconvolution into the modes' cells of the fine grid only (2D, 3D); the
-// other cells keep their contents (for an FFT that reads the modes' cells).
-template <int D, typename T>
-bool predeconvolve_kept(const ModeParams &mp, const double *inv_phihat, const T *modes, T *grid,
-                        int64_t vectors, hipStream_t stream);
 template <typename T>
 bool fill_zero(T *p, int64_t n, hipStream_t stream);
 template <int D, typename T>

@@ -513,7 +513,9 @@ bool Plan::ensure_fft(int64_t vectors)
     if (use_line_fft()) {
         if (line_fft) return true;
         auto lf = std::make_unique<LineFft>();
-        if (!lf->init(dim, fine, n_modes, type, precision != Precision::F32, isign)) return false;
+        if (!lf->init(dim, fine, n_modes, type, precision != Precision::F32, isign, d_inv_phihat,
+                      phihat_off, mode_order == ModeOrder::Centered))
+            return false;
         line_fft = std::move(lf);
         return true;
     }
@@ -639,6 +641,12 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         mark(kStageSpread, true, stream);
         if (spread_only) return true;
         mark(kStageFft, false, stream);
+        if (use_line_fft()) {
+            // the last line pass writes the deconvolved modes
+            if (!ensure_fft(active) || !line_fft->run(grid, active, stream, out)) return false;
+            mark(kStageFft, true, stream);
+            return true;
+        }
         if (!fft(active, grid, stream)) return false;
         mark(kStageFft, true, stream);
         mark(kStageDeconv, false, stream);
@@ -647,22 +655,21 @@ bool Plan::execute_t(int64_t active, const T *in, T *out, hipStream_t stream, bo
         mark(kStageDeconv, true, stream);
     } else {
         if (M == 0) return true;
-        mark(kStageDeconv, false, stream);
-        // the line FFT reads the modes' cells only: no zeros between them
-        bool kept = false;
-        if constexpr (D == 2 || D == 3) {
-            if (use_line_fft()) {
-                if (!predeconvolve_kept<D, T>(mp, d_inv_phihat, in, (T *)grid, active, stream))
-                    return false;
-                kept = true;
-            }
+        if (use_line_fft()) {
+            // the first line pass reads the modes and predeconvolves them
+            mark(kStageFft, false, stream);
+            if (!ensure_fft(active) || !line_fft->run(grid, active, stream, (void *)in))
+                return false;
+            mark(kStageFft, true, stream);
+        } else {
+            mark(kStageDeconv, false, stream);
+            if (!predeconvolve<D, T>(mp, d_inv_phihat, in, (T *)grid, active, stream))
+                return false;
+            mark(kStageDeconv, true, stream);
+            mark(kStageFft, false, stream);
+            if (!fft(active, grid, stream)) return false;
+            mark(kStageFft, true, stream);
         }
-        if (!kept && !predeconvolve<D, T>(mp, d_inv_phihat, in, (T *)grid, active, stream))
-            return false;
-        mark(kStageDeconv, true, stream);
-        mark(kStageFft, false, stream);
-        if (!fft(active, grid, stream)) return false;
-        mark(kStageFft, true, stream);
         mark(kStageInterp, false, stream);
         if (D == 3 && tiled_interp()) {
             // bin item offsets and the scratch of their scan
