@@ -2,7 +2,7 @@
 (the default engine) against rocFFT's plan on the same plan and points, and
 both against the FP64 direct sums to the reference's tolerances
 (tests/gpu_nufft_2d.rs:15, gpu_nufft_3d.rs:13: 20 and 32 eps).  The reference's FFT is its wgpu-fft
-dependency, so the engines are pinned by the direct sums
+dependency (Cargo.toml:34, not vendored), so the engines are pinned by the direct sums
 and by each other: type 1 reads only the kept modes of the transformed grid,
 type 2 reads only the modes' cells, and both agree with rocFFT to the FFT's
 rounding (FP32 4e-7, FP64 1e-14 relative).
