@@ -132,7 +132,7 @@ int mnufft_execute_type2_many(int64_t count, const mnufft_plan *plans, const int
  * each plan's FFT, one deconvolution launch.  Type-1 plans of one rank
  * (1D to 3D) and precision, each at most once, one vector per plan, that
  * spread on the output-stationary blocks in index point order (the plain
- * spread; 3D plans with mnufft_set_bin_tuning(plan, 0, 0)); else
+ * spread; 2D and 3D plans with mnufft_set_bin_tuning(plan, 0, 0)); else
  * MNUFFT_ERROR_INVALID_ARGUMENT.  The plans keep no points afterwards;
  * plans[0] keeps the batch scratch.
  */
@@ -285,12 +285,12 @@ int mnufft_set_spread_tuning(mnufft_plan plan, int64_t light_limit, int64_t min_
 
 /*
  * Bins and tiles of the type-1 spread (no counterpart in the reference):
- * rows_side0, the side along axis 0 of the bins of the row geometry (the 3D
+ * rows_side0, the side along axis 0 of the bins of the row geometry (the
  * plain and matrix-core tiles; 0 = the smallest power of two from 2 keeping
  * the bin count within 2^24, else a power of two up to 64, never finer than
- * the automatic side); plain_rows, the plain 3D spread on 8 x 8 x 8 row
- * tiles (1, the default) or on the output-stationary blocks of the
- * reference's layout (0).  Applies from the next set_points (an execute
+ * the automatic side); plain_rows, the plain spread on row tiles (3D: 8 x 8
+ * x 8 cells, 2D: 16 x 16; 1, the default) or on the output-stationary
+ * blocks of the reference's layout (0).  Applies from the next set_points (an execute
  * re-bins kept points); mnufft_autotune sets both.
  */
 int mnufft_set_bin_tuning(mnufft_plan plan, int64_t rows_side0, int32_t plain_rows);

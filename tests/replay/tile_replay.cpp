@@ -271,19 +271,21 @@ bool replay_width(int W, const GridParams &g, const SpreadParams &sp, const uint
 
 extern "C" {
 
-// dim 2: 32 x 32 tiles (k_spread_mfma2); dim 3: 16 x 16 x 8 tiles
-// (k_spread_mfma3).  offsets: nbins + 1 bin offsets of the row geometry
-// `side`.  report (16 values): violations, first violation code, its work
-// item, items, heavy tiles, heavy parts, part length, largest slot, largest
-// partial index, partial floats, records visited, largest tile total,
-// largest offsets index, nbins.  Returns -1 for an unsupported shape.
-int tile_replay(int dim, int width, const int64_t *fine, const int64_t *side,
-                const uint32_t *offsets, int64_t M, uint32_t light_limit, uint32_t min_part,
-                int64_t budget_bytes, int64_t vectors, int64_t *report)
+// dim 2: 32 x 32 tiles (k_spread_mfma2), or 16 x 16 (k_spread_rows2) with
+// edge2 16; dim 3: 16 x 16 x 8 tiles (k_spread_mfma3).  offsets: nbins + 1
+// bin offsets of the row geometry `side`.  report (16 values): violations,
+// first violation code, its work item, items, heavy tiles, heavy parts, part
+// length, largest slot, largest partial index, partial floats, records
+// visited, largest tile total, largest offsets index, nbins.  Returns -1 for
+// an unsupported shape.
+int tile_replay_edge(int dim, int edge2, int width, const int64_t *fine, const int64_t *side,
+                     const uint32_t *offsets, int64_t M, uint32_t light_limit, uint32_t min_part,
+                     int64_t budget_bytes, int64_t vectors, int64_t *report)
 {
+    if (dim == 2 && edge2 != 16 && edge2 != 32) return -1;
     GridParams g = make_grid(dim, width, fine, side);
     SpreadParams sp{};
-    const int64_t tile[3] = {dim == 2 ? 32 : 16, dim == 2 ? 32 : 16, dim == 2 ? 1 : 8};
+    const int64_t tile[3] = {dim == 2 ? edge2 : 16, dim == 2 ? edge2 : 16, dim == 2 ? 1 : 8};
     sp.block_count = 1;
     for (int a = 0; a < kMaxDims; ++a) {
         sp.block[a] = a < dim ? tile[a] : 1;
@@ -291,8 +293,11 @@ int tile_replay(int dim, int width, const int64_t *fine, const int64_t *side,
         sp.block_count *= sp.nblocks[a];
     }
     Report rep;
-    bool ok = dim == 2 ? replay_width<2, 32, 32, 1>(width, g, sp, offsets, M, light_limit,
-                                                     min_part, budget_bytes, vectors, rep)
+    bool ok = dim == 2 && edge2 == 32
+                  ? replay_width<2, 32, 32, 1>(width, g, sp, offsets, M, light_limit, min_part,
+                                               budget_bytes, vectors, rep)
+              : dim == 2 ? replay_width<2, 16, 16, 1>(width, g, sp, offsets, M, light_limit,
+                                                      min_part, budget_bytes, vectors, rep)
               : dim == 3 ? replay_width<3, 16, 16, 8>(width, g, sp, offsets, M, light_limit,
                                                       min_part, budget_bytes, vectors, rep)
                          : false;
@@ -303,6 +308,14 @@ int tile_replay(int dim, int width, const int64_t *fine, const int64_t *side,
                            rep.max_offset_index, g.nbins};
     for (int i = 0; i < 14; ++i) report[i] = v[i];
     return 0;
+}
+
+int tile_replay(int dim, int width, const int64_t *fine, const int64_t *side,
+                const uint32_t *offsets, int64_t M, uint32_t light_limit, uint32_t min_part,
+                int64_t budget_bytes, int64_t vectors, int64_t *report)
+{
+    return tile_replay_edge(dim, 32, width, fine, side, offsets, M, light_limit, min_part,
+                            budget_bytes, vectors, report);
 }
 
 }  // extern "C"

@@ -34,7 +34,9 @@ def _bits(t):
     return t.view(torch.float32).view(torch.int32)
 
 
-def _make(W, S, torch, problems, seed0, precision=None, plain_rows=True):
+def _make(W, S, torch, problems, seed0, precision=None, plain_rows=False):
+    """Plans for a batch: 2D and 3D plans spread on the output-stationary
+    blocks (the batch's own spread), unless plain_rows."""
     plans, xs, cs = [], [], []
     f64 = precision is not None and precision != W.F32
     for i, (n_modes, M, eps, isign, order, kind) in enumerate(problems):
@@ -42,7 +44,7 @@ def _make(W, S, torch, problems, seed0, precision=None, plain_rows=True):
         if precision is not None:
             cfg = cfg.with_precision(precision)
         p = W.NufftPlan.type1_gpu(cfg)
-        if len(n_modes) == 3 and not plain_rows:
+        if len(n_modes) >= 2 and not plain_rows:
             p.set_bin_tuning(0, plain_rows=False)
         plans.append(p)
         d = len(n_modes)

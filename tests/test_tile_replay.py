@@ -35,19 +35,23 @@ def replay_lib():
     subprocess.run(["make", "-s", "-C", REPLAY], check=True)
     L = ctypes.CDLL(LIB)
     L.tile_replay.restype = ctypes.c_int
-    L.tile_replay.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                              ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint32,
-                              ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]
+    L.tile_replay_edge.restype = ctypes.c_int
+    L.tile_replay_edge.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int64,
+                                   ctypes.c_int64, ctypes.c_void_p]
     return L
 
 
-def _run(L, dim, width, fine, side, offsets, M, vectors=1):
+def _run(L, dim, width, fine, side, offsets, M, vectors=1, edge2=32):
+    """Light limit 4096 records, parts of at least 1024, 256 MiB of partial
+    sums: the spreader's defaults (spread.hpp)."""
     f = np.asarray(fine, dtype=np.int64)
     s = np.asarray(side, dtype=np.int64)
     off = np.ascontiguousarray(offsets, dtype=np.uint32)
     rep = np.zeros(16, dtype=np.int64)
-    rc = L.tile_replay(dim, width, f.ctypes.data, s.ctypes.data, off.ctypes.data, M, 4096, 1024,
-                       256 << 20, vectors, rep.ctypes.data)
+    rc = L.tile_replay_edge(dim, edge2, width, f.ctypes.data, s.ctypes.data, off.ctypes.data, M,
+                            4096, 1024, 256 << 20, vectors, rep.ctypes.data)
     assert rc == 0
     return dict(zip(FIELDS, rep.tolist()))
 
@@ -90,6 +94,24 @@ def test_replay_2d_tiles_at_5e7_points(replay_lib):
                                                      S.uniform_coords(S.config_seed(3, 1), M, 2)),
               M)
     assert r4["violations"] == 0, r4
+
+
+def test_replay_2d_plain_row_tiles_at_5e7_points(replay_lib):
+    """The plain 2D row tiles (16 x 16 cells, k_spread_rows2 and
+    k_reduce_tiles<T,16,16,1>) on the same 5e7 uniform points and on a
+    half-clustered set: every index in bounds."""
+    M, fine, w = 50_000_000, [2048, 2048], 7
+    side = _row_side(fine)
+    x = S.uniform_coords(S.config_seed(3, 1), M, 2)
+    r = _run(replay_lib, 2, w, fine, side, _offsets(fine, w, side, x), M, edge2=16)
+    print("REPLAY 2D rows", r)
+    assert r["violations"] == 0, r
+    assert r["heavy_tiles"] == 128 * 128
+    assert r["max_partial"] < r["partial_floats"] and r["max_slot"] < M
+    x[: M // 2] = S.clustered_coords(S.config_seed(3, 2), M // 2, (0.3, -1.1), 0.05)
+    r = _run(replay_lib, 2, w, fine, side, _offsets(fine, w, side, x), M, edge2=16)
+    print("REPLAY 2D rows clustered", r)
+    assert r["violations"] == 0, r
 
 
 def test_replay_3d_tiles_at_1e7_clustered_points(replay_lib):

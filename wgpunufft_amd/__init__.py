@@ -158,8 +158,8 @@ class _PlanBase:
     def set_bin_tuning(self, rows_side0=0, plain_rows=True):
         """Bins and tiles of the type-1 spread: the axis-0 side of the row
         geometry's bins (0: automatic, else a power of two up to 64) and the
-        plain 3D spread on 8x8x8 row tiles (True) or on the reference's
-        output-stationary blocks (False).  Takes effect at the next
+        plain spread on row tiles (3D 8x8x8, 2D 16x16 cells; True) or on the
+        reference's output-stationary blocks (False).  Takes effect at the next
         set_points or execute."""
         check(lib().mnufft_set_bin_tuning(self._handle, int(rows_side0), 1 if plain_rows else 0))
         return self

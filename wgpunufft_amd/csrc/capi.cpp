@@ -603,9 +603,9 @@ int mnufft_autotune(mnufft_plan plan, int64_t active_batch, int64_t M, const voi
         p.spread_method = cands[best_i].method;
         p.dense.threshold = cands[best_i].dense;
         int64_t tried = (int64_t)cands.size();
-        // Then, for the method kept, the bins and tiles: the plain 3D spread
-        // on the output-stationary blocks of the reference layout instead of
-        // the row tiles; coarser bins of the row geometry along axis 0 (2^k
+        // Then, for the method kept, the bins and tiles: the plain 2D/3D
+        // spread on the output-stationary blocks of the reference layout
+        // instead of the row tiles; coarser bins of the row geometry along axis 0 (2^k
         // cells); the heavy-tile split (light limit).  Each is kept when it
         // gains at least 3% over the best so far.
         struct Knob {
@@ -615,9 +615,10 @@ int mnufft_autotune(mnufft_plan plan, int64_t active_batch, int64_t M, const voi
         };
         const Knob base{p.plain_rows, p.rows_side0, p.spreader.light_limit};
         std::vector<Knob> knobs;
-        if (p.sweep && p.spread_method == mnufft::SpreadMethod::Plain)
+        const bool rows = p.sweep || (p.rows2 && p.spread_method == mnufft::SpreadMethod::Plain);
+        if (rows && p.spread_method == mnufft::SpreadMethod::Plain)
             knobs.push_back({!base.plain_rows, base.side0, base.light});
-        if (p.sweep)
+        if (rows)
             for (int64_t s0 : {4, 8}) knobs.push_back({base.plain_rows, s0, base.light});
         for (uint32_t lim : {1024u, 16384u}) knobs.push_back({base.plain_rows, base.side0, lim});
         Knob keep = base;

@@ -209,6 +209,15 @@ bool Plan::init_common()
     sweep = make_sweep_params(dim, fine, kernel.width, 8,
                               precision == Precision::F32 ? 8 : kMaxRowsWidth, &sp);
     tiles2 = make_tile2_params(dim, fine, kernel.width, &tile2_sp);
+    // plain 2D row tiles of 16 x 16 cells: widths 2..8, every axis at
+    // least 16 + width cells (axis 0 one more)
+    rows2 = dim == 2 && kernel.width >= 2 && kernel.width <= 8 &&
+            fine[0] >= 16 + kernel.width + 1 &&
+            fine[1] >= 16 + kernel.width;
+    if (rows2) {
+        const int64_t block[kMaxDims] = {16, 16, 1, 1, 1, 1, 1, 1};
+        rows2_sp = make_tile_params(2, fine, block);
+    }
     if (sweep && kernel.width <= 8) {
         // matrix-core tiles: 16 x 16 cells, 8 deep along z
         const int64_t block[kMaxDims] = {16, 16, 8, 1, 1, 1, 1, 1};

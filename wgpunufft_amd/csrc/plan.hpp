@@ -46,7 +46,7 @@ struct Plan {
     SpreadParams sp{};         // plain spread: the row tiles where `sweep`, else sp_generic
     SpreadParams sp_generic{};  // output-stationary blocks of the reference layout
     bool sweep = false;  // 3D grid qualifies for the row tiles (plain and matrix-core)
-    // Tuning the autotuner sets (and mnufft_set_bin_tuning): the plain 3D
+    // Tuning the autotuner sets (and mnufft_set_bin_tuning): the plain 2D/3D
     // spread on row tiles (else the output-stationary blocks), and the side
     // of the row geometry's bins along axis 0 (0: the smallest power of two
     // from 2 keeping the bin count within 2^24).
@@ -55,6 +55,8 @@ struct Plan {
     int64_t binned_side0 = 0;  // axis-0 bin side of the current binning
     SpreadParams tile2_sp{};
     bool tiles2 = false;  // 2D grid qualifies for the matrix-core 32 x 32 tiles
+    SpreadParams rows2_sp{};
+    bool rows2 = false;   // 2D grid qualifies for the plain 16 x 16 row tiles
     SpreadParams tile3_sp{};  // 3D matrix-core tiles (16 x 16 x 8; row-sweep grids)
     SpreadMethod spread_method = SpreadMethod::Plain;
     // Type-1 order of the points inside a bin: ascending original index (the
@@ -78,12 +80,15 @@ struct Plan {
     // tiles, or a 2D matrix-core method.
     bool rows_geometry() const
     {
-        if (spread_method == SpreadMethod::Plain) return sweep && plain_rows;
+        if (spread_method == SpreadMethod::Plain) return (sweep || rows2) && plain_rows;
         return sweep || tiles2;
     }
     const SpreadParams &spread_params() const
     {
-        if (spread_method == SpreadMethod::Plain) return sweep && plain_rows ? sp : sp_generic;
+        if (spread_method == SpreadMethod::Plain) {
+            if (!plain_rows) return sp_generic;
+            return sweep ? sp : (rows2 ? rows2_sp : sp_generic);
+        }
         return dim == 2 ? tile2_sp : tile3_sp;
     }
 

@@ -846,9 +846,14 @@ bool chunk_rows(Spreader &s, const GridParams &g, const SpreadParams &sp, const 
         return false;
     if (s.kernel_end) (void)hipEventRecord(s.kernel_end, stream);
     const dim3 rg(2048, (unsigned)vectors);
-    k_reduce_tiles<T, 8, 8, 8><<<rg, 256, 0, stream>>>(g, sp, s.parts, s.heavy_base,
-                                                       s.heavy_list, s.ctrl, (const T *)s.partial,
-                                                       grid);
+    if (g.dims == 2)
+        k_reduce_tiles<T, 16, 16, 1><<<rg, 256, 0, stream>>>(g, sp, s.parts, s.heavy_base,
+                                                             s.heavy_list, s.ctrl,
+                                                             (const T *)s.partial, grid);
+    else
+        k_reduce_tiles<T, 8, 8, 8><<<rg, 256, 0, stream>>>(g, sp, s.parts, s.heavy_base,
+                                                           s.heavy_list, s.ctrl,
+                                                           (const T *)s.partial, grid);
     return hip_ok(hipGetLastError(), "row-tile spread launch");
 }
 
@@ -946,13 +951,13 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
         set_error("matrix-core spread needs an FP32 3D plan on row tiles");
         return false;
     }
-    if (!mfma && tiles && D != 3) {
-        set_error("row tiles of a 2D plan need a matrix-core spread method");
+    if (!mfma && tiles && D != 3 && D != 2) {
+        set_error("plain row tiles need a 2D or 3D plan");
         return false;
     }
     if (!mfma && tiles &&
-        (D != 3 || g.width < 2 || g.width > (sizeof(T) == 4 ? 8 : kMaxRowsWidth))) {
-        set_error("plain row tiles need a 3D plan of kernel width <= 8 (FP32) or <= 16 (FP64)");
+        (g.width < 2 || g.width > (D == 3 && sizeof(T) == 8 ? kMaxRowsWidth : 8))) {
+        set_error("plain row tiles need a kernel width of 2..8 (2..16 for 3D FP64)");
         return false;
     }
     if (!mfma && !tiles && sp.run != 2 && sp.run != 4 && sp.run != 8 &&
@@ -1031,7 +1036,7 @@ bool Spreader::spread(const GridParams &g, const SpreadParams &sp, const uint32_
                 if (ok && dense && kernel_end) (void)hipEventRecord(kernel_end, stream);
             }
         } else if (tiles) {
-            if constexpr (D == 3)
+            if constexpr (D == 2 || D == 3)
                 ok = chunk_rows<T>(*this, g, sp, offsets, rec_start, rec_off, cs, M, vc, gr,
                                    stream);
         } else {

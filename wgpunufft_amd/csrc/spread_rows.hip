@@ -1,5 +1,5 @@
 // Plain (FP32 / FP64 vector) type-1 spreading of 3D row tiles (kernel
-// widths 2..8 in FP32, 2..16 in FP64).
+// widths 2..8 in FP32, 2..16 in FP64) and of 2D row tiles (widths 2..8).
 //
 // Same contract as the reference's output-stationary spreader
 // (src/gpu_nd_spread.rs:1-36, 1583-1667): every cell of an 8 x 8 x 8 tile
@@ -159,6 +159,120 @@ __global__ __launch_bounds__(64) void k_spread_rows(
     }
 }
 
+// 2D row tiles of 16 x 16 cells: lane l owns the four x cells 4 (l / 16) ..
+// + 3 of row y = l % 16, in registers.  Bins are one cell along y, so the
+// records of a reach row share their y support; a staged record carries its
+// weights over the tile's 16 x and 16 y cells (zero off its support), so a
+// record costs each lane one weight read, one product, a four-weight read
+// and eight FMAs.  Same order, parts and partial-sum layout as the 3D tiles.
+constexpr int kRows2T = 16;
+
+template <typename T>
+struct RowRec2 {
+    T kx[kRows2T];
+    T ky[kRows2T];
+    T cre, cim;
+};
+
+template <typename T, int W>
+__global__ __launch_bounds__(64) void k_spread_rows2(
+    GridParams g, SpreadParams sp, const uint32_t *__restrict__ offsets,
+    const int4 *__restrict__ rec_start, const T *__restrict__ rec_off,
+    const T *__restrict__ c_sorted, int64_t M, const uint32_t *__restrict__ totals,
+    const uint32_t *__restrict__ parts, const uint32_t *__restrict__ item_off,
+    const uint32_t *__restrict__ heavy_base, const uint32_t *__restrict__ ctrl,
+    T *__restrict__ grid, T *__restrict__ partial)
+{
+    static_assert(W <= 8, "widths beyond the 2D row tiles");
+    constexpr int TX = kRows2T, TY = kRows2T;
+    constexpr int NROWS = TY + W - 1;
+    __shared__ RowRec2<T> st[64];
+    __shared__ TileRows<NROWS> rows;
+
+    const int lane = threadIdx.x;
+    const int ly = lane & 15, xc = 4 * (lane >> 4);  // the lane's row and first x cell
+    const int64_t v = blockIdx.y;
+    const T beta = (T)g.beta;
+    const int64_t n_items = ctrl[0];
+    const uint32_t part_len_cap = ctrl[3];
+    const T *cv = c_sorted + 2 * v * M;
+    const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n;
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const TileItem ti = tile_item(item, sp.block_count, item_off, parts, totals, part_len_cap);
+        const int64_t b = ti.b;
+        const int64_t x0 = (b % sp.nblocks[0]) * TX, y0 = (b / sp.nblocks[0]) * TY;
+        const AxisReach rx = axis_reach(x0, TX, W, g.ax[0]);
+        int64_t xs = x0 - (W - 1);
+        if (xs < 0) xs += n0;
+
+        stage_tile_rows<2, TY, 1, W, NROWS>(rows, lane, y0, 0, rx, g, offsets);
+        T ar[4], ai[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ar[i] = ai[i] = (T)0;
+        const int64_t klo = ti.rlo, khi = ti.rhi;
+        Fetched<T> nx = fetch_record<T, NROWS>(rows, klo + lane, khi, rec_start, rec_off, cv);
+        for (int64_t k0 = klo; k0 < khi; k0 += 64) {
+            const Fetched<T> cur = nx;
+            nx = fetch_record<T, NROWS>(rows, k0 + 64 + lane, khi, rec_start, rec_off, cv);
+            bool take = false;
+            int ee = 0;
+            if (cur.valid) {
+                int64_t e64 = (int64_t)cur.sx0 - xs;
+                if (e64 < 0) e64 += n0;
+                take = e64 < TX + W - 1;
+                ee = (int)e64;
+            }
+            const uint64_t mask = __ballot(take);
+            const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+            const int nst = __popcll(mask);
+            if (take) {
+                RowRec2<T> &R = st[pos];
+                tile_weights<T, W, TX>(R.kx, ee - (W - 1), (T)cur.off.x, beta);
+                tile_weights<T, W, TY>(R.ky, cur.q - (W - 1), (T)cur.off.y, beta);
+                R.cre = cur.c.x;
+                R.cim = cur.c.y;
+            }
+            wave_sync();
+#pragma unroll 2
+            for (int s = 0; s < nst; ++s) {
+                const RowRec2<T> &S = st[s];
+                const T wy = S.ky[ly];
+                const T vre = S.cre * wy, vim = S.cim * wy;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const T kx = S.kx[xc + i];
+                    ar[i] += vre * kx;
+                    ai[i] += vim * kx;
+                }
+            }
+            wave_sync();
+        }
+
+        const int64_t gy = y0 + ly;
+        if (ti.np == 1) {
+            if (gy < n1) {
+                T *row = grid + 2 * (v * g.fine_total + x0 + xc + n0 * gy);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (x0 + xc + i < n0) {
+                        row[2 * i] = ar[i];
+                        row[2 * i + 1] = ai[i];
+                    }
+                }
+            }
+        } else {
+            const int64_t pslot = (int64_t)heavy_base[b] + ti.p;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                T *o = partial + partial_index<TX, TY, 1>(pslot, gridDim.y, v, xc + i, ly, 0);
+                o[0] = ar[i];
+                o[1] = ai[i];
+            }
+        }
+    }
+}
+
 template <typename T, int W>
 bool launch_rows_width(int tx, dim3 grid3, const GridParams &g, const SpreadParams &sp,
                        const uint32_t *offsets, const int4 *rec_start, const T *rec_off,
@@ -175,10 +289,46 @@ bool launch_rows_width(int tx, dim3 grid3, const GridParams &g, const SpreadPara
 }  // namespace
 
 template <typename T>
+bool launch_rows2_tiles(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
+                        const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
+                        int64_t vectors, const TileWork &w, T *grid, T *partial,
+                        hipStream_t stream)
+{
+    const dim3 grid3(w.items_grid, (unsigned)vectors);
+    if (sp.block[0] != kRows2T || sp.block[1] != kRows2T) {
+        set_error("plain 2D row tiles are 16 x 16 cells");
+        return false;
+    }
+#define MNUFFT_ROWS2(W)                                                                             \
+    case W:                                                                                         \
+        k_spread_rows2<T, W><<<grid3, 64, 0, stream>>>(g, sp, offsets, rec_start, rec_off,          \
+                                                       c_sorted, M, w.totals, w.parts, w.item_off,  \
+                                                       w.heavy_base, w.ctrl, grid, partial);        \
+        break;
+    switch (g.width) {
+        MNUFFT_ROWS2(2)
+        MNUFFT_ROWS2(3)
+        MNUFFT_ROWS2(4)
+        MNUFFT_ROWS2(5)
+        MNUFFT_ROWS2(6)
+        MNUFFT_ROWS2(7)
+        MNUFFT_ROWS2(8)
+    default:
+        set_error("plain 2D row tiles need a width of 2..8");
+        return false;
+    }
+#undef MNUFFT_ROWS2
+    return hip_ok(hipGetLastError(), "2D row-tile spread launch");
+}
+
+template <typename T>
 bool launch_rows_tiles(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
                        const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,
                        int64_t vectors, const TileWork &w, T *grid, T *partial, hipStream_t stream)
 {
+    if (g.dims == 2)
+        return launch_rows2_tiles<T>(g, sp, offsets, rec_start, rec_off, c_sorted, M, vectors, w,
+                                     grid, partial, stream);
     const dim3 grid3(w.items_grid, (unsigned)vectors);
     const int tx = (int)sp.block[0];
     if (sp.block[1] != kRowsTY || sp.block[2] != kRowsTZ) {
