@@ -127,12 +127,13 @@ int mnufft_execute_type2_many(int64_t count, const mnufft_plan *plans, const int
  * d_in[i] at the M[i] points d_points[i] and writes its modes d_out[i], as
  * mnufft_encode(plans[i], 1, M[i], d_points[i], d_in[i], d_out[i]) would,
  * bit for bit.  One bin sort over all the points, one prepare and strength
- * gather, one spread launch and one heavy-part reduction over every plan's
- * blocks (per run length; each plan's blocks split by its own work plan),
- * each plan's FFT, one deconvolution launch.  Type-1 plans of one rank
- * (1D to 3D) and precision, each at most once, one vector per plan, that
- * spread on the output-stationary blocks in index point order (the plain
- * spread; 2D and 3D plans with mnufft_set_bin_tuning(plan, 0, 0)); else
+ * gather, one spread launch and one heavy-part reduction per class of plans
+ * (the output-stationary blocks of one run length, or the 2D row tiles of
+ * one kernel width; each plan's tiles split by its own work plan), each
+ * plan's FFT, one deconvolution launch.  Type-1 plans of one rank (1D to
+ * 3D) and precision, each at most once, one vector per plan, on the plain
+ * spread in index point order (3D plans with mnufft_set_bin_tuning(plan, 0,
+ * 0)); else
  * MNUFFT_ERROR_INVALID_ARGUMENT.  The plans keep no points afterwards;
  * plans[0] keeps the batch scratch.
  */

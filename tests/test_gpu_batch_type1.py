@@ -35,8 +35,9 @@ def _bits(t):
 
 
 def _make(W, S, torch, problems, seed0, precision=None, plain_rows=False):
-    """Plans for a batch: 2D and 3D plans spread on the output-stationary
-    blocks (the batch's own spread), unless plain_rows."""
+    """Plans for a batch: 3D plans spread on the output-stationary blocks
+    (the batch's own 3D spread) unless plain_rows; 1D and 2D plans keep
+    their defaults (2D: the 16 x 16 row tiles where the grid qualifies)."""
     plans, xs, cs = [], [], []
     f64 = precision is not None and precision != W.F32
     for i, (n_modes, M, eps, isign, order, kind) in enumerate(problems):
@@ -44,7 +45,7 @@ def _make(W, S, torch, problems, seed0, precision=None, plain_rows=False):
         if precision is not None:
             cfg = cfg.with_precision(precision)
         p = W.NufftPlan.type1_gpu(cfg)
-        if len(n_modes) >= 2 and not plain_rows:
+        if len(n_modes) == 3 and not plain_rows:
             p.set_bin_tuning(0, plain_rows=False)
         plans.append(p)
         d = len(n_modes)
@@ -100,6 +101,22 @@ def test_batch_type1_2d_equals_each_plan_and_direct_sums():
     torch, W, O, S = _setup()
     plans, xs, cs = _make(W, S, torch, PROBLEMS_2D, 5000)
     _check(torch, W, O, PROBLEMS_2D, plans, xs, cs, sample=(0, 3, 5, 7))
+
+
+def test_batch_type1_2d_row_tiles_and_blocks_in_one_batch():
+    """2D plans on the row tiles (three kernel widths: three tile classes)
+    and on the output-stationary blocks in one batch: each output equals its
+    plan's own encode bit for bit, and the direct sums."""
+    torch, W, O, S = _setup()
+    plans, xs, cs = _make(W, S, torch, PROBLEMS_2D, 5050)
+    tiles = 0
+    for i, p in enumerate(plans):
+        if i % 3 == 1:
+            p.set_bin_tuning(0, plain_rows=False)
+        elif p.tuning()["plain_rows"]:
+            tiles += 1
+    assert tiles >= 3
+    _check(torch, W, O, PROBLEMS_2D, plans, xs, cs, sample=(1, 3, 5))
 
 
 def test_batch_type1_1d_3d_and_f64():

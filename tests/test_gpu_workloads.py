@@ -260,8 +260,7 @@ def test_config4a_points_as_type1_batch_matches_oracle_and_per_plan_encodes():
     for i, (n0, n1, M, sx, _) in enumerate(S.batch_problem_specs()):
         x = S.uniform_coords(sx, M, 2)
         c = S.complex_normal(S.config_seed(4, 300 + i), M)
-        # batches spread on the output-stationary blocks
-        plans.append(W.NufftPlan.type1_gpu(W.NufftConfig([n0, n1], 1e-5)).set_bin_tuning(0, False))
+        plans.append(W.NufftPlan.type1_gpu(W.NufftConfig([n0, n1], 1e-5)))
         xs.append(torch.from_numpy(x).cuda())
         cs.append(torch.from_numpy(c).cuda())
         hx.append(x)

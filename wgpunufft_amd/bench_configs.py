@@ -229,8 +229,7 @@ def config4(args, W, S, torch):
     t1 = []
     for i, ((n0, n1, M, _, _), (_, x, _, _)) in enumerate(zip(S.batch_problem_specs(),
                                                             problems)):
-        # the output-stationary blocks, the spread the batch shares
-        plan = W.NufftPlan.type1_gpu(W.NufftConfig([n0, n1], 1e-5)).set_bin_tuning(0, False)
+        plan = W.NufftPlan.type1_gpu(W.NufftConfig([n0, n1], 1e-5))
         c = torch.from_numpy(S.complex_normal(S.config_seed(4, 300 + i), M)).to(dev)
         t1.append((plan, x, c, torch.empty(n0 * n1, dtype=torch.complex64, device=dev)))
 

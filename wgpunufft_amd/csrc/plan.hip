@@ -1331,13 +1331,16 @@ bool execute_type1_many_t(int64_t count, Plan *const *plans, const int64_t *M,
                           void *const *outputs, hipStream_t stream)
 {
     Plan &head = *plans[0];
-    // problems grouped by run length (one spread launch per run), in their
-    // order otherwise
+    // problems grouped by run length (one spread launch per run; the 2D row
+    // tiles, run 0, one launch per kernel width), in their order otherwise
+    auto class_key = [&](int64_t i) {
+        const int run = plans[i]->spread_params().run;
+        return run == 0 ? plans[i]->kernel.width : 64 * run;
+    };
     std::vector<int64_t> order((size_t)count);
     for (int64_t i = 0; i < count; ++i) order[(size_t)i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-        return plans[a]->spread_params().run < plans[b]->spread_params().run;
-    });
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return class_key(a) < class_key(b); });
     std::vector<BatchProblem> table((size_t)count);
     std::vector<SpreadProblem> stable((size_t)count);
     std::vector<ModeProblem> mtable((size_t)count);
@@ -1361,14 +1364,17 @@ bool execute_type1_many_t(int64_t count, Plan *const *plans, const int64_t *M,
         b.M = M[i];
         b.item_base = 0;
         b.bin_base = bins;
-        b.g = p.type1_geometry(M[i], nullptr);
-        if (classes.empty() || classes.back().run != sp.run)
-            classes.push_back(SpreadClass{(int)r, (int)r, sp.run, 0, 0});
+        b.g = p.spread_geometry(M[i], nullptr);
+        if (classes.empty() || classes.back().run != sp.run ||
+            (sp.run == 0 && classes.back().width != p.kernel.width))
+            classes.push_back(SpreadClass{(int)r, (int)r, sp.run, p.kernel.width, 0, 0});
         SpreadClass &k = classes.back();
         SpreadProblem &q = stable[(size_t)r];
         q.sp = sp;
         q.block_base = k.blocks;
-        const int64_t per_part = (int64_t)64 * sp.run * 2 * (int64_t)sizeof(T);
+        const int64_t cells =
+            sp.run == 0 ? sp.block[0] * sp.block[1] * sp.block[2] : (int64_t)64 * sp.run;
+        const int64_t per_part = cells * 2 * (int64_t)sizeof(T);
         q.part_capacity = std::max<int64_t>(p.spreader.budget_bytes / per_part, 64);
         q.light_limit = p.spreader.light_limit;
         q.min_part = p.spreader.min_part;
@@ -1449,10 +1455,12 @@ bool type1_many_valid(int64_t count, Plan *const *plans, const int64_t *M,
             set_error("batched type 1 takes type-1 plans of one rank (1D to 3D) and precision");
             return false;
         }
-        if (p->spread_method != SpreadMethod::Plain || p->rows_geometry() ||
-            p->point_order != BinOrder::Index || p->spread_params().run == 0) {
-            set_error("batched type 1 takes plans that spread on the output-stationary blocks "
-                      "in index order (plain spread; 3D plans with the row tiles off)");
+        if (p->spread_method != SpreadMethod::Plain || p->point_order != BinOrder::Index ||
+            (p->rows_geometry() && !(p->dim == 2 && p->rows2)) ||
+            (p->spread_params().run == 0 && p->dim != 2)) {
+            set_error("batched type 1 takes plans of the plain spread in index order: "
+                      "output-stationary blocks or 2D row tiles (3D plans with the row tiles "
+                      "off)");
             return false;
         }
         if (M[i] < 0 || (M[i] > 0 && !points[i]) || (M[i] > 0 && !inputs[i]) || !outputs[i]) {

@@ -669,21 +669,6 @@ __global__ __launch_bounds__(256) void k_classify_many(const BatchProblem *__res
     if (lane == 0) totals[gb] = (uint32_t)(total > 0xffffffffull ? 0xffffffffull : total);
 }
 
-// Problem of combined element t: the last problem whose base[4q + k] <= t.
-__device__ inline int problem_by_base(const int64_t *__restrict__ base, int nprob, int k,
-                                      int64_t t)
-{
-    int lo = 0, hi = nprob - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (base[4 * mid + k] <= t)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
 // k_spread over the work items of every problem of a class (one vector):
 // item -> problem (item bases) -> the problem's block and part, summed as
 // the problem's own spread sums it.
@@ -700,7 +685,7 @@ __global__ __launch_bounds__(kLanes) void k_spread_many(
     __shared__ SpreadShared<D, T, R> sh;
     const int64_t n_items = pbase[4 * nprob];
     for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-        const int q = problem_by_base(pbase, nprob, 0, item);
+        const int q = batch_problem_of(pbase, nprob, 0, item);
         const BatchProblem &pb = probs[q];
         const SpreadProblem &spb = sprobs[q];
         const int64_t bb = spb.block_base;
@@ -734,7 +719,7 @@ __global__ __launch_bounds__(256) void k_reduce_many(
         if (t < work) {
             const int64_t hb = t / block_cells;
             c = t - hb * block_cells;
-            q = problem_by_base(pbase, nprob, 2, hb);
+            q = batch_problem_of(pbase, nprob, 2, hb);
             const int64_t bb = sprobs[q].block_base;
             b = heavy_list[bb + (hb - pbase[4 * q + 2])];
             part_lane_sum(partial, pbase[4 * q + 1] + (int64_t)heavy_base[bb + b],
@@ -747,6 +732,53 @@ __global__ __launch_bounds__(256) void k_reduce_many(
             combine_lanes(ps, u, sre, sim);
             write_block_cell<D, T, R>(probs[q].g, sprobs[q].sp, b, c, sre, sim,
                                       (T *)probs[q].grid);
+        }
+        __syncthreads();
+    }
+}
+
+// k_reduce_many for the 2D row tiles (TX x TY cells) of a batch class.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(256) void k_reduce_tiles_many(
+    const BatchProblem *__restrict__ probs, const SpreadProblem *__restrict__ sprobs, int nprob,
+    const uint32_t *__restrict__ parts, const uint32_t *__restrict__ heavy_base,
+    const uint32_t *__restrict__ heavy_list, const int64_t *__restrict__ pbase,
+    const T *__restrict__ partial)
+{
+    constexpr int64_t cells = (int64_t)TX * TY;
+    __shared__ PartSums<T> ps;
+    const int64_t work = pbase[4 * nprob + 1] * cells;
+    const int u = threadIdx.x % kPartLanes, pl = threadIdx.x / kPartLanes;
+    for (int64_t t0 = (int64_t)blockIdx.x * kPartLanes; t0 < work;
+         t0 += (int64_t)gridDim.x * kPartLanes) {
+        const int64_t t = t0 + u;
+        int64_t c = 0, b = 0;
+        int q = 0;
+        T sre = (T)0, sim = (T)0;
+        if (t < work) {
+            const int64_t hb = t / cells;
+            c = t - hb * cells;
+            q = batch_problem_of(pbase, nprob, 2, hb);
+            const int64_t bb = sprobs[q].block_base;
+            b = heavy_list[bb + (hb - pbase[4 * q + 2])];
+            part_lane_sum(partial, pbase[4 * q + 1] + (int64_t)heavy_base[bb + b],
+                          (int64_t)parts[bb + b], 1, 0, cells, c, pl, sre, sim);
+        }
+        ps.re[pl][u] = sre;
+        ps.im[pl][u] = sim;
+        __syncthreads();
+        if (pl == 0 && t < work) {
+            combine_lanes(ps, u, sre, sim);
+            const GridParams &g = probs[q].g;
+            const SpreadParams &sp = sprobs[q].sp;
+            const int64_t gx = (b % sp.nblocks[0]) * TX + c % TX;
+            const int64_t gy = (b / sp.nblocks[0]) * TY + c / TX;
+            const int64_t n0 = g.ax[0].n;
+            if (gx < n0 && gy < g.ax[1].n) {
+                T *o = (T *)probs[q].grid + 2 * (gx + n0 * gy);
+                o[0] = sre;
+                o[1] = sim;
+            }
         }
         __syncthreads();
     }
@@ -1066,13 +1098,18 @@ bool Spreader::spread_many(const BatchProblem *probs, const SpreadProblem *sprob
     reuse_plan = false;
     // scratch for the largest class first: nothing is freed between launches
     int64_t most_blocks = 1, most_partial = 0, most_problems = 1;
+    constexpr int64_t kRows2Cells = 16 * 16;  // the 2D row tiles
     for (const SpreadClass &k : classes) {
         const int R = k.run;
-        if (!(R == 2 || R == 4 || R == 8 || (D >= 4 && R == 16))) {
-            set_error("batched spread: unsupported run");
+        const bool tiles = R == 0;
+        if (tiles ? !(D == 2 && k.width >= 2 && k.width <= 8)
+                  : !(R == 2 || R == 4 || R == 8 || (D >= 4 && R == 16))) {
+            set_error(tiles ? "batched spread: row tiles need 2D plans of width 2..8"
+                            : "batched spread: unsupported run");
             return false;
         }
-        const int64_t per_part = (int64_t)kLanes * R * 2 * (int64_t)sizeof(T);
+        const int64_t per_part =
+            (tiles ? kRows2Cells : (int64_t)kLanes * R) * 2 * (int64_t)sizeof(T);
         most_blocks = std::max(most_blocks, k.blocks);
         most_partial = std::max(most_partial, std::max<int64_t>(k.slots, 1) * per_part);
         most_problems = std::max<int64_t>(most_problems, k.p1 - k.p0);
@@ -1096,7 +1133,7 @@ bool Spreader::spread_many(const BatchProblem *probs, const SpreadProblem *sprob
         const int64_t nb = k.blocks;
         const int n = k.p1 - k.p0;
         if (nb == 0 || n == 0) continue;
-        const int64_t cells = (int64_t)kLanes * R;
+        const int64_t cells = R == 0 ? kRows2Cells : (int64_t)kLanes * R;
         const BatchProblem *pb = probs + k.p0;
         const SpreadProblem *sb = sprobs + k.p0;
         k_classify_many<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(pb, sb, n, nb, offsets,
@@ -1117,6 +1154,16 @@ bool Spreader::spread_many(const BatchProblem *probs, const SpreadProblem *sprob
                                                         pbase, (const T *)partial, cells);        \
     } while (0)
         switch (R) {
+        case 0:
+            if constexpr (D == 2) {
+                if (!launch_rows2_many<T>(k.width, pb, sb, n, ig.x, offsets, rec_start, rec_off,
+                                          c_sorted, totals, parts, item_off, heavy_base, pctrl,
+                                          pbase, (T *)partial, stream))
+                    return false;
+                k_reduce_tiles_many<T, 16, 16><<<rg, 256, 0, stream>>>(
+                    pb, sb, n, parts, heavy_base, heavy_list, pbase, (const T *)partial);
+            }
+            break;
         case 2: MNUFFT_SPREAD_MANY(2); break;
         case 4: MNUFFT_SPREAD_MANY(4); break;
         case 8: MNUFFT_SPREAD_MANY(8); break;

@@ -63,11 +63,13 @@ struct SpreadProblem {
     uint32_t light_limit, min_part;
 };
 
-// A class of consecutive problems [p0, p1) of one run length: its blocks
+// A class of consecutive problems [p0, p1) of one run length (or of the 2D
+// row tiles and one kernel width): its blocks
 // and a bound on the partial slots its heavy parts take.
 struct SpreadClass {
     int p0, p1;
-    int run;
+    int run;    // 0: the 2D row tiles, all of kernel width `width`
+    int width;
     int64_t blocks;
     int64_t slots;
 };
@@ -204,6 +206,17 @@ bool make_sweep_params(int d, const int64_t *fine, int width, int64_t tx, int ma
 
 // Tile kernel of the plain row-tile spread (spread_rows.hip): 8 x 8 x 8
 // tiles, partial sums of heavy parts in the layout k_reduce_tiles reads.
+// The 2D row tiles (16 x 16 cells) of the problems of one batch class, all
+// of kernel width `width` (one vector each); the work plan arrays and the
+// item / slot bases as the batched output-stationary spread takes them.
+template <typename T>
+bool launch_rows2_many(int width, const BatchProblem *probs, const SpreadProblem *sprobs,
+                       int nprob, unsigned items_grid, const uint32_t *offsets,
+                       const int4 *rec_start, const T *rec_off, const T *c_sorted,
+                       const uint32_t *totals, const uint32_t *parts, const uint32_t *item_off,
+                       const uint32_t *heavy_base, const uint32_t *pctrl, const int64_t *pbase,
+                       T *partial, hipStream_t stream);
+
 template <typename T>
 bool launch_rows_tiles(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,
                        const int4 *rec_start, const T *rec_off, const T *c_sorted, int64_t M,

@@ -174,6 +174,98 @@ struct RowRec2 {
     T cre, cim;
 };
 
+// Sums the records [klo, khi) of the reach of the tile at (x0, y0) into the
+// lane's four cells (ar, ai).  `offsets` are the bin offsets of the tile's
+// own grid; slots index the records and strengths `cv` directly.
+template <typename T, int W, int NROWS>
+__device__ __forceinline__ void rows2_accumulate(RowRec2<T> (&st)[64], TileRows<NROWS> &rows,
+                                                 int lane, const GridParams &g, int64_t x0,
+                                                 int64_t y0, int64_t klo, int64_t khi,
+                                                 const uint32_t *__restrict__ offsets,
+                                                 const int4 *__restrict__ rec_start,
+                                                 const T *__restrict__ rec_off,
+                                                 const T *__restrict__ cv, T (&ar)[4], T (&ai)[4])
+{
+    constexpr int TX = kRows2T, TY = kRows2T;
+    const int ly = lane & 15, xc = 4 * (lane >> 4);
+    const T beta = (T)g.beta;
+    const int64_t n0 = g.ax[0].n;
+    const AxisReach rx = axis_reach(x0, TX, W, g.ax[0]);
+    int64_t xs = x0 - (W - 1);
+    if (xs < 0) xs += n0;
+
+    stage_tile_rows<2, TY, 1, W, NROWS>(rows, lane, y0, 0, rx, g, offsets);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ar[i] = ai[i] = (T)0;
+    Fetched<T> nx = fetch_record<T, NROWS>(rows, klo + lane, khi, rec_start, rec_off, cv);
+    for (int64_t k0 = klo; k0 < khi; k0 += 64) {
+        const Fetched<T> cur = nx;
+        nx = fetch_record<T, NROWS>(rows, k0 + 64 + lane, khi, rec_start, rec_off, cv);
+        bool take = false;
+        int ee = 0;
+        if (cur.valid) {
+            int64_t e64 = (int64_t)cur.sx0 - xs;
+            if (e64 < 0) e64 += n0;
+            take = e64 < TX + W - 1;
+            ee = (int)e64;
+        }
+        const uint64_t mask = __ballot(take);
+        const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+        const int nst = __popcll(mask);
+        if (take) {
+            RowRec2<T> &R = st[pos];
+            tile_weights<T, W, TX>(R.kx, ee - (W - 1), (T)cur.off.x, beta);
+            tile_weights<T, W, TY>(R.ky, cur.q - (W - 1), (T)cur.off.y, beta);
+            R.cre = cur.c.x;
+            R.cim = cur.c.y;
+        }
+        wave_sync();
+#pragma unroll 2
+        for (int s = 0; s < nst; ++s) {
+            const RowRec2<T> &S = st[s];
+            const T wy = S.ky[ly];
+            const T vre = S.cre * wy, vim = S.cim * wy;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const T kx = S.kx[xc + i];
+                ar[i] += vre * kx;
+                ai[i] += vim * kx;
+            }
+        }
+        wave_sync();
+    }
+}
+
+// The lane's four cells: into the grid `gv` (one vector) when the tile is
+// one part, else into the part's 16 x 16 partial-sum slot `slot`.
+template <typename T>
+__device__ __forceinline__ void rows2_store(int lane, const GridParams &g, int64_t x0, int64_t y0,
+                                            bool whole, T *__restrict__ gv, T *__restrict__ slot,
+                                            const T (&ar)[4], const T (&ai)[4])
+{
+    const int ly = lane & 15, xc = 4 * (lane >> 4);
+    const int64_t n0 = g.ax[0].n, gy = y0 + ly;
+    if (whole) {
+        if (gy < g.ax[1].n) {
+            T *row = gv + 2 * (x0 + xc + n0 * gy);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (x0 + xc + i < n0) {
+                    row[2 * i] = ar[i];
+                    row[2 * i + 1] = ai[i];
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            T *o = slot + 2 * (ly * kRows2T + xc + i);
+            o[0] = ar[i];
+            o[1] = ai[i];
+        }
+    }
+}
+
 template <typename T, int W>
 __global__ __launch_bounds__(64) void k_spread_rows2(
     GridParams g, SpreadParams sp, const uint32_t *__restrict__ offsets,
@@ -184,92 +276,71 @@ __global__ __launch_bounds__(64) void k_spread_rows2(
     T *__restrict__ grid, T *__restrict__ partial)
 {
     static_assert(W <= 8, "widths beyond the 2D row tiles");
-    constexpr int TX = kRows2T, TY = kRows2T;
-    constexpr int NROWS = TY + W - 1;
+    constexpr int NROWS = kRows2T + W - 1;
     __shared__ RowRec2<T> st[64];
     __shared__ TileRows<NROWS> rows;
 
     const int lane = threadIdx.x;
-    const int ly = lane & 15, xc = 4 * (lane >> 4);  // the lane's row and first x cell
     const int64_t v = blockIdx.y;
-    const T beta = (T)g.beta;
     const int64_t n_items = ctrl[0];
     const uint32_t part_len_cap = ctrl[3];
     const T *cv = c_sorted + 2 * v * M;
-    const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n;
 
     for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         const TileItem ti = tile_item(item, sp.block_count, item_off, parts, totals, part_len_cap);
         const int64_t b = ti.b;
-        const int64_t x0 = (b % sp.nblocks[0]) * TX, y0 = (b / sp.nblocks[0]) * TY;
-        const AxisReach rx = axis_reach(x0, TX, W, g.ax[0]);
-        int64_t xs = x0 - (W - 1);
-        if (xs < 0) xs += n0;
-
-        stage_tile_rows<2, TY, 1, W, NROWS>(rows, lane, y0, 0, rx, g, offsets);
+        const int64_t x0 = (b % sp.nblocks[0]) * kRows2T, y0 = (b / sp.nblocks[0]) * kRows2T;
         T ar[4], ai[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) ar[i] = ai[i] = (T)0;
-        const int64_t klo = ti.rlo, khi = ti.rhi;
-        Fetched<T> nx = fetch_record<T, NROWS>(rows, klo + lane, khi, rec_start, rec_off, cv);
-        for (int64_t k0 = klo; k0 < khi; k0 += 64) {
-            const Fetched<T> cur = nx;
-            nx = fetch_record<T, NROWS>(rows, k0 + 64 + lane, khi, rec_start, rec_off, cv);
-            bool take = false;
-            int ee = 0;
-            if (cur.valid) {
-                int64_t e64 = (int64_t)cur.sx0 - xs;
-                if (e64 < 0) e64 += n0;
-                take = e64 < TX + W - 1;
-                ee = (int)e64;
-            }
-            const uint64_t mask = __ballot(take);
-            const int pos = __popcll(mask & ((1ull << lane) - 1ull));
-            const int nst = __popcll(mask);
-            if (take) {
-                RowRec2<T> &R = st[pos];
-                tile_weights<T, W, TX>(R.kx, ee - (W - 1), (T)cur.off.x, beta);
-                tile_weights<T, W, TY>(R.ky, cur.q - (W - 1), (T)cur.off.y, beta);
-                R.cre = cur.c.x;
-                R.cim = cur.c.y;
-            }
-            wave_sync();
-#pragma unroll 2
-            for (int s = 0; s < nst; ++s) {
-                const RowRec2<T> &S = st[s];
-                const T wy = S.ky[ly];
-                const T vre = S.cre * wy, vim = S.cim * wy;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const T kx = S.kx[xc + i];
-                    ar[i] += vre * kx;
-                    ai[i] += vim * kx;
-                }
-            }
-            wave_sync();
-        }
+        rows2_accumulate<T, W, NROWS>(st, rows, lane, g, x0, y0, ti.rlo, ti.rhi, offsets,
+                                      rec_start, rec_off, cv, ar, ai);
+        const bool whole = ti.np == 1;
+        T *slot = whole ? nullptr
+                        : partial + partial_index<kRows2T, kRows2T, 1>(
+                                        (int64_t)heavy_base[b] + ti.p, gridDim.y, v, 0, 0, 0);
+        rows2_store<T>(lane, g, x0, y0, whole, grid + 2 * v * g.fine_total, slot, ar, ai);
+    }
+}
 
-        const int64_t gy = y0 + ly;
-        if (ti.np == 1) {
-            if (gy < n1) {
-                T *row = grid + 2 * (v * g.fine_total + x0 + xc + n0 * gy);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    if (x0 + xc + i < n0) {
-                        row[2 * i] = ar[i];
-                        row[2 * i + 1] = ai[i];
-                    }
-                }
-            }
-        } else {
-            const int64_t pslot = (int64_t)heavy_base[b] + ti.p;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                T *o = partial + partial_index<TX, TY, 1>(pslot, gridDim.y, v, xc + i, ly, 0);
-                o[0] = ar[i];
-                o[1] = ai[i];
-            }
-        }
+// The 2D row tiles of every problem of a batch class (one vector each, one
+// kernel width): item -> problem (item bases) -> the problem's tile and
+// part, summed as the problem's own spread sums it.
+template <typename T, int W>
+__global__ __launch_bounds__(64) void k_spread_rows2_many(
+    const BatchProblem *__restrict__ probs, const SpreadProblem *__restrict__ sprobs, int nprob,
+    const uint32_t *__restrict__ offsets, const int4 *__restrict__ rec_start,
+    const T *__restrict__ rec_off, const T *__restrict__ c_sorted,
+    const uint32_t *__restrict__ totals, const uint32_t *__restrict__ parts,
+    const uint32_t *__restrict__ item_off, const uint32_t *__restrict__ heavy_base,
+    const uint32_t *__restrict__ pctrl, const int64_t *__restrict__ pbase,
+    T *__restrict__ partial)
+{
+    static_assert(W <= 8, "widths beyond the 2D row tiles");
+    constexpr int NROWS = kRows2T + W - 1;
+    __shared__ RowRec2<T> st[64];
+    __shared__ TileRows<NROWS> rows;
+
+    const int lane = threadIdx.x;
+    const int64_t n_items = pbase[4 * nprob];
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const int q = batch_problem_of(pbase, nprob, 0, item);
+        const BatchProblem &pb = probs[q];
+        const SpreadProblem &spb = sprobs[q];
+        const int64_t bb = spb.block_base;
+        const TileItem ti = tile_item(item - pbase[4 * q], spb.sp.block_count, item_off + bb,
+                                      parts + bb, totals + bb, pctrl[8 * q + 3]);
+        const int64_t b = ti.b;
+        const int64_t x0 = (b % spb.sp.nblocks[0]) * kRows2T;
+        const int64_t y0 = (b / spb.sp.nblocks[0]) * kRows2T;
+        T ar[4], ai[4];
+        rows2_accumulate<T, W, NROWS>(st, rows, lane, pb.g, x0, y0, ti.rlo, ti.rhi,
+                                      offsets + pb.bin_base, rec_start, rec_off, c_sorted, ar,
+                                      ai);
+        const bool whole = ti.np == 1;
+        T *slot = whole ? nullptr
+                        : partial + partial_index<kRows2T, kRows2T, 1>(
+                                        pbase[4 * q + 1] + (int64_t)heavy_base[bb + b] + ti.p, 1,
+                                        0, 0, 0, 0);
+        rows2_store<T>(lane, pb.g, x0, y0, whole, (T *)pb.grid, slot, ar, ai);
     }
 }
 
@@ -320,6 +391,47 @@ bool launch_rows2_tiles(const GridParams &g, const SpreadParams &sp, const uint3
 #undef MNUFFT_ROWS2
     return hip_ok(hipGetLastError(), "2D row-tile spread launch");
 }
+
+template <typename T>
+bool launch_rows2_many(int width, const BatchProblem *probs, const SpreadProblem *sprobs,
+                       int nprob, unsigned items_grid, const uint32_t *offsets,
+                       const int4 *rec_start, const T *rec_off, const T *c_sorted,
+                       const uint32_t *totals, const uint32_t *parts, const uint32_t *item_off,
+                       const uint32_t *heavy_base, const uint32_t *pctrl, const int64_t *pbase,
+                       T *partial, hipStream_t stream)
+{
+#define MNUFFT_ROWS2_MANY(W)                                                                        \
+    case W:                                                                                         \
+        k_spread_rows2_many<T, W><<<items_grid, 64, 0, stream>>>(                                   \
+            probs, sprobs, nprob, offsets, rec_start, rec_off, c_sorted, totals, parts, item_off,   \
+            heavy_base, pctrl, pbase, partial);                                                     \
+        break;
+    switch (width) {
+        MNUFFT_ROWS2_MANY(2)
+        MNUFFT_ROWS2_MANY(3)
+        MNUFFT_ROWS2_MANY(4)
+        MNUFFT_ROWS2_MANY(5)
+        MNUFFT_ROWS2_MANY(6)
+        MNUFFT_ROWS2_MANY(7)
+        MNUFFT_ROWS2_MANY(8)
+    default:
+        set_error("batched 2D row tiles need a width of 2..8");
+        return false;
+    }
+#undef MNUFFT_ROWS2_MANY
+    return hip_ok(hipGetLastError(), "batched 2D row-tile spread launch");
+}
+
+template bool launch_rows2_many<float>(int, const BatchProblem *, const SpreadProblem *, int,
+                                       unsigned, const uint32_t *, const int4 *, const float *,
+                                       const float *, const uint32_t *, const uint32_t *,
+                                       const uint32_t *, const uint32_t *, const uint32_t *,
+                                       const int64_t *, float *, hipStream_t);
+template bool launch_rows2_many<double>(int, const BatchProblem *, const SpreadProblem *, int,
+                                        unsigned, const uint32_t *, const int4 *, const double *,
+                                        const double *, const uint32_t *, const uint32_t *,
+                                        const uint32_t *, const uint32_t *, const uint32_t *,
+                                        const int64_t *, double *, hipStream_t);
 
 template <typename T>
 bool launch_rows_tiles(const GridParams &g, const SpreadParams &sp, const uint32_t *offsets,

@@ -177,6 +177,22 @@ __host__ __device__ inline TileItem tile_item(int64_t item, int64_t block_count,
     return t;
 }
 
+// Problem of a batch's combined element t (work item, partial slot, heavy
+// tile): the last problem q whose base[4q + k] <= t.
+__host__ __device__ inline int batch_problem_of(const int64_t *__restrict__ base, int nprob, int k,
+                                                int64_t t)
+{
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[4 * mid + k] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // Float index of a heavy part's partial sum of cell (x, y, z) of a tile of
 // TX x TY x TZ cells (real part; the imaginary part follows): slot `pslot`,
 // vector v of `vectors`.
