@@ -128,12 +128,11 @@ int mnufft_execute_type2_many(int64_t count, const mnufft_plan *plans, const int
  * mnufft_encode(plans[i], 1, M[i], d_points[i], d_in[i], d_out[i]) would,
  * bit for bit.  One bin sort over all the points, one prepare and strength
  * gather, one spread launch and one heavy-part reduction per class of plans
- * (the output-stationary blocks of one run length, or the 2D row tiles of
- * one kernel width; each plan's tiles split by its own work plan), each
- * plan's FFT, one deconvolution launch.  Type-1 plans of one rank (1D to
- * 3D) and precision, each at most once, one vector per plan, on the plain
- * spread in index point order (3D plans with mnufft_set_bin_tuning(plan, 0,
- * 0)); else
+ * (the output-stationary blocks of one run length, or the 2D / 3D row
+ * tiles of one kernel width; each plan's tiles split by its own work plan),
+ * each plan's FFT, one deconvolution launch.  Type-1 plans of one rank (1D
+ * to 3D) and precision, each at most once, one vector per plan, on the
+ * plain spread in index point order; else
  * MNUFFT_ERROR_INVALID_ARGUMENT.  The plans keep no points afterwards;
  * plans[0] keeps the batch scratch.
  */
@@ -215,8 +214,9 @@ int mnufft_points_permutation(mnufft_plan plan, uint32_t *d_sorted, uint32_t *d_
 /*
  * Plan facts, for tests and benchmarks: info[0] kernel width, [1..3] fine
  * lengths, [4..6] bin sides of the geometry last chosen (type 1) or used
- * (type 2), [7] its bin count, [8] geometry index, [9..11] spread block edge,
- * [12] spread run, [13..15] type-3 outer grid lengths (axes 0-2 only for
+ * (type 2), [7] its bin count, [8] geometry index, [9..11] the tile or
+ * block edge of the type-1 spread the plan's method and tuning select,
+ * [12] its run (0: tiles), [13..15] type-3 outer grid lengths (axes 0-2 only for
  * plans of more axes).  Writes min(n, 16).
  */
 int mnufft_plan_info(mnufft_plan plan, int64_t *info, int32_t n);

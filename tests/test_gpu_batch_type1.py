@@ -34,10 +34,9 @@ def _bits(t):
     return t.view(torch.float32).view(torch.int32)
 
 
-def _make(W, S, torch, problems, seed0, precision=None, plain_rows=False):
-    """Plans for a batch: 3D plans spread on the output-stationary blocks
-    (the batch's own 3D spread) unless plain_rows; 1D and 2D plans keep
-    their defaults (2D: the 16 x 16 row tiles where the grid qualifies)."""
+def _make(W, S, torch, problems, seed0, precision=None, plain_rows=True):
+    """Plans for a batch: their defaults (2D and 3D: the row tiles where the
+    grid qualifies), or the output-stationary blocks unless plain_rows."""
     plans, xs, cs = [], [], []
     f64 = precision is not None and precision != W.F32
     for i, (n_modes, M, eps, isign, order, kind) in enumerate(problems):
@@ -45,7 +44,7 @@ def _make(W, S, torch, problems, seed0, precision=None, plain_rows=False):
         if precision is not None:
             cfg = cfg.with_precision(precision)
         p = W.NufftPlan.type1_gpu(cfg)
-        if len(n_modes) == 3 and not plain_rows:
+        if len(n_modes) >= 2 and not plain_rows:
             p.set_bin_tuning(0, plain_rows=False)
         plans.append(p)
         d = len(n_modes)
@@ -109,13 +108,11 @@ def test_batch_type1_2d_row_tiles_and_blocks_in_one_batch():
     plan's own encode bit for bit, and the direct sums."""
     torch, W, O, S = _setup()
     plans, xs, cs = _make(W, S, torch, PROBLEMS_2D, 5050)
-    tiles = 0
     for i, p in enumerate(plans):
         if i % 3 == 1:
             p.set_bin_tuning(0, plain_rows=False)
-        elif p.tuning()["plain_rows"]:
-            tiles += 1
-    assert tiles >= 3
+    tiles = [i for i, p in enumerate(plans) if p.info()["run"] == 0]
+    assert len({plans[i].info()["width"] for i in tiles}) == 3 and len(tiles) < len(plans)
     _check(torch, W, O, PROBLEMS_2D, plans, xs, cs, sample=(1, 3, 5))
 
 
@@ -128,8 +125,19 @@ def test_batch_type1_1d_3d_and_f64():
     p3 = [([24, 20, 16], 30000, 1e-6, 1, 0, "clustered"), ([12, 12, 12], 2000, 1e-3, -1, 1,
                                                            "uniform"),
           ([32, 28, 24], 0, 1e-5, 1, 0, "uniform"), ([16, 16, 16], 8000, 1e-5, -1, 0, "uniform")]
-    plans, xs, cs = _make(W, S, torch, p3, 5200, plain_rows=False)
+    plans, xs, cs = _make(W, S, torch, p3, 5200)
+    assert sum(p.info()["run"] == 0 for p in plans) >= 2
     _check(torch, W, O, p3, plans, xs, cs, sample=(0, 3))
+    # the same problems on the output-stationary blocks, and mixed
+    plans, xs, cs = _make(W, S, torch, p3, 5200, plain_rows=False)
+    _check(torch, W, O, p3, plans, xs, cs, sample=(0,))
+    plans, xs, cs = _make(W, S, torch, p3, 5200)
+    plans[1].set_bin_tuning(0, plain_rows=False)
+    _check(torch, W, O, p3, plans, xs, cs, sample=(1,))
+    p3d = [([20, 18, 16], 20000, 1e-9, 1, 0, "clustered"), ([16, 16, 12], 3000, 1e-12, -1, 1,
+                                                           "uniform")]
+    plans, xs, cs = _make(W, S, torch, p3d, 5250, precision=W.F64)
+    _check(torch, W, O, p3d, plans, xs, cs, sample=(0, 1))
     p64 = [([40, 36], 4000, 1e-9, 1, 0, "uniform"), ([24, 20], 30000, 1e-12, -1, 1, "clustered")]
     plans, xs, cs = _make(W, S, torch, p64, 5300, precision=W.F64)
     _check(torch, W, O, p64, plans, xs, cs, sample=(0, 1))
@@ -139,9 +147,10 @@ def test_batch_type1_refuses_other_spreads():
     torch, W, O, S = _setup()
     x = torch.from_numpy(S.uniform_coords(S.config_seed(7, 5400), 100, 3)).cuda()
     c = torch.from_numpy(S.complex_normal(S.config_seed(7, 5401), 100)).cuda()
-    rows = W.NufftPlan.type1_gpu(W.NufftConfig([32, 32, 32], 1e-6))  # 3D row tiles
+    coord = W.NufftPlan.type1_gpu(W.NufftConfig([32, 32, 32], 1e-6))
+    coord.set_point_order("coordinates")
     with pytest.raises(W.NufftError):
-        W.encode_type1_many([rows], [x], [c])
+        W.encode_type1_many([coord], [x], [c])
     mfma = W.NufftPlan.type1_gpu(W.NufftConfig([32, 32, 32], 1e-6))
     mfma.set_spread_method("mfma_f16x3")
     with pytest.raises(W.NufftError):

@@ -427,10 +427,10 @@ def encode_type1_many(plans, points, values, outs=None):
     """Type-1 encodes of many independent plans as one batch: plans[i]
     spreads values[i] (its strengths) at points[i] into its modes; one bin
     sort over all the points, one spread launch per class of plans (run
-    length of the blocks, or 2D row tiles of one kernel width), each plan's
+    length of the blocks, or row tiles of one kernel width), each plan's
     FFT, one deconvolution launch.  Type-1 plans of one rank (1D to 3D) and
-    precision, one vector each, on the plain spread in index order (3D
-    plans: set_bin_tuning(plain_rows=False)); outputs equal each
+    precision, one vector each, on the plain spread in index order; outputs
+    equal each
     plan's own encode_gpu bit for bit.  The plans keep no points
     afterwards."""
     n = len(plans)

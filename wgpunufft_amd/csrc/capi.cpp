@@ -460,12 +460,12 @@ int mnufft_plan_info(mnufft_plan plan, int64_t *info, int32_t n)
     for (int a = 0; a < 3; ++a) {
         v[1 + a] = a < q.dim ? q.fine[a] : 1;
         v[4 + a] = a < q.dim ? ((int64_t)1 << q.gp.ax[a].shift) : 1;
-        v[9 + a] = a < q.dim ? q.sp.block[a] : 1;
+        v[9 + a] = a < q.dim ? q.spread_params().block[a] : 1;
         v[13 + a] = a < p.dim ? p.type3_outer[a] : 1;
     }
     v[7] = q.gp.nbins;
     v[8] = q.geometry_index;
-    v[12] = q.sp.run;
+    v[12] = q.spread_params().run;
     for (int i = 0; i < n && i < 16; ++i) info[i] = v[i];
     return MNUFFT_SUCCESS;
 }

@@ -1331,7 +1331,7 @@ bool execute_type1_many_t(int64_t count, Plan *const *plans, const int64_t *M,
                           void *const *outputs, hipStream_t stream)
 {
     Plan &head = *plans[0];
-    // problems grouped by run length (one spread launch per run; the 2D row
+    // problems grouped by run length (one spread launch per run; the row
     // tiles, run 0, one launch per kernel width), in their order otherwise
     auto class_key = [&](int64_t i) {
         const int run = plans[i]->spread_params().run;
@@ -1455,12 +1455,9 @@ bool type1_many_valid(int64_t count, Plan *const *plans, const int64_t *M,
             set_error("batched type 1 takes type-1 plans of one rank (1D to 3D) and precision");
             return false;
         }
-        if (p->spread_method != SpreadMethod::Plain || p->point_order != BinOrder::Index ||
-            (p->rows_geometry() && !(p->dim == 2 && p->rows2)) ||
-            (p->spread_params().run == 0 && p->dim != 2)) {
-            set_error("batched type 1 takes plans of the plain spread in index order: "
-                      "output-stationary blocks or 2D row tiles (3D plans with the row tiles "
-                      "off)");
+        if (p->spread_method != SpreadMethod::Plain || p->point_order != BinOrder::Index) {
+            set_error("batched type 1 takes plans of the plain spread (output-stationary "
+                      "blocks or row tiles) in index order");
             return false;
         }
         if (M[i] < 0 || (M[i] > 0 && !points[i]) || (M[i] > 0 && !inputs[i]) || !outputs[i]) {

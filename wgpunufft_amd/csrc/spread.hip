@@ -737,15 +737,15 @@ __global__ __launch_bounds__(256) void k_reduce_many(
     }
 }
 
-// k_reduce_many for the 2D row tiles (TX x TY cells) of a batch class.
-template <typename T, int TX, int TY>
+// k_reduce_many for the row tiles (TX x TY x TZ cells) of a batch class.
+template <typename T, int TX, int TY, int TZ>
 __global__ __launch_bounds__(256) void k_reduce_tiles_many(
     const BatchProblem *__restrict__ probs, const SpreadProblem *__restrict__ sprobs, int nprob,
     const uint32_t *__restrict__ parts, const uint32_t *__restrict__ heavy_base,
     const uint32_t *__restrict__ heavy_list, const int64_t *__restrict__ pbase,
     const T *__restrict__ partial)
 {
-    constexpr int64_t cells = (int64_t)TX * TY;
+    constexpr int64_t cells = (int64_t)TX * TY * TZ;
     __shared__ PartSums<T> ps;
     const int64_t work = pbase[4 * nprob + 1] * cells;
     const int u = threadIdx.x % kPartLanes, pl = threadIdx.x / kPartLanes;
@@ -771,11 +771,13 @@ __global__ __launch_bounds__(256) void k_reduce_tiles_many(
             combine_lanes(ps, u, sre, sim);
             const GridParams &g = probs[q].g;
             const SpreadParams &sp = sprobs[q].sp;
-            const int64_t gx = (b % sp.nblocks[0]) * TX + c % TX;
-            const int64_t gy = (b / sp.nblocks[0]) * TY + c / TX;
-            const int64_t n0 = g.ax[0].n;
-            if (gx < n0 && gy < g.ax[1].n) {
-                T *o = (T *)probs[q].grid + 2 * (gx + n0 * gy);
+            const int64_t nb0 = sp.nblocks[0], nb1 = sp.nblocks[1];
+            const int64_t gx = (b % nb0) * TX + c % TX;
+            const int64_t gy = ((b / nb0) % nb1) * TY + (c / TX) % TY;
+            const int64_t gz = (b / (nb0 * nb1)) * TZ + c / (TX * TY);
+            const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n;
+            if (gx < n0 && gy < n1 && gz < g.ax[2].n) {
+                T *o = (T *)probs[q].grid + 2 * (gx + n0 * (gy + n1 * gz));
                 o[0] = sre;
                 o[1] = sim;
             }
@@ -1098,18 +1100,21 @@ bool Spreader::spread_many(const BatchProblem *probs, const SpreadProblem *sprob
     reuse_plan = false;
     // scratch for the largest class first: nothing is freed between launches
     int64_t most_blocks = 1, most_partial = 0, most_problems = 1;
-    constexpr int64_t kRows2Cells = 16 * 16;  // the 2D row tiles
+    // the row tiles: 16 x 16 cells (2D), 8 x 8 x 8 (3D)
+    constexpr int64_t kTileCells = D == 2 ? 16 * 16 : 8 * 8 * 8;
+    constexpr int kTileWidthMax = D == 3 && sizeof(T) == 8 ? kMaxRowsWidth : 8;
     for (const SpreadClass &k : classes) {
         const int R = k.run;
         const bool tiles = R == 0;
-        if (tiles ? !(D == 2 && k.width >= 2 && k.width <= 8)
+        if (tiles ? !((D == 2 || D == 3) && k.width >= 2 && k.width <= kTileWidthMax)
                   : !(R == 2 || R == 4 || R == 8 || (D >= 4 && R == 16))) {
-            set_error(tiles ? "batched spread: row tiles need 2D plans of width 2..8"
+            set_error(tiles ? "batched spread: row tiles need 2D or 3D plans of width 2..8 "
+                              "(2..16 for 3D FP64)"
                             : "batched spread: unsupported run");
             return false;
         }
         const int64_t per_part =
-            (tiles ? kRows2Cells : (int64_t)kLanes * R) * 2 * (int64_t)sizeof(T);
+            (tiles ? kTileCells : (int64_t)kLanes * R) * 2 * (int64_t)sizeof(T);
         most_blocks = std::max(most_blocks, k.blocks);
         most_partial = std::max(most_partial, std::max<int64_t>(k.slots, 1) * per_part);
         most_problems = std::max<int64_t>(most_problems, k.p1 - k.p0);
@@ -1133,7 +1138,7 @@ bool Spreader::spread_many(const BatchProblem *probs, const SpreadProblem *sprob
         const int64_t nb = k.blocks;
         const int n = k.p1 - k.p0;
         if (nb == 0 || n == 0) continue;
-        const int64_t cells = R == 0 ? kRows2Cells : (int64_t)kLanes * R;
+        const int64_t cells = R == 0 ? kTileCells : (int64_t)kLanes * R;
         const BatchProblem *pb = probs + k.p0;
         const SpreadProblem *sb = sprobs + k.p0;
         k_classify_many<D><<<grid_for(nb * 64, 256), 256, 0, stream>>>(pb, sb, n, nb, offsets,
@@ -1160,7 +1165,14 @@ bool Spreader::spread_many(const BatchProblem *probs, const SpreadProblem *sprob
                                           c_sorted, totals, parts, item_off, heavy_base, pctrl,
                                           pbase, (T *)partial, stream))
                     return false;
-                k_reduce_tiles_many<T, 16, 16><<<rg, 256, 0, stream>>>(
+                k_reduce_tiles_many<T, 16, 16, 1><<<rg, 256, 0, stream>>>(
+                    pb, sb, n, parts, heavy_base, heavy_list, pbase, (const T *)partial);
+            } else if constexpr (D == 3) {
+                if (!launch_rows3_many<T>(k.width, pb, sb, n, ig.x, offsets, rec_start, rec_off,
+                                          c_sorted, totals, parts, item_off, heavy_base, pctrl,
+                                          pbase, (T *)partial, stream))
+                    return false;
+                k_reduce_tiles_many<T, 8, 8, 8><<<rg, 256, 0, stream>>>(
                     pb, sb, n, parts, heavy_base, heavy_list, pbase, (const T *)partial);
             }
             break;

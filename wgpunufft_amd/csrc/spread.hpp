@@ -63,12 +63,12 @@ struct SpreadProblem {
     uint32_t light_limit, min_part;
 };
 
-// A class of consecutive problems [p0, p1) of one run length (or of the 2D
-// row tiles and one kernel width): its blocks
+// A class of consecutive problems [p0, p1) of one run length (or of the row
+// tiles and one kernel width): its blocks
 // and a bound on the partial slots its heavy parts take.
 struct SpreadClass {
     int p0, p1;
-    int run;    // 0: the 2D row tiles, all of kernel width `width`
+    int run;    // 0: the 2D or 3D row tiles, all of kernel width `width`
     int width;
     int64_t blocks;
     int64_t slots;
@@ -211,6 +211,15 @@ bool make_sweep_params(int d, const int64_t *fine, int width, int64_t tx, int ma
 // item / slot bases as the batched output-stationary spread takes them.
 template <typename T>
 bool launch_rows2_many(int width, const BatchProblem *probs, const SpreadProblem *sprobs,
+                       int nprob, unsigned items_grid, const uint32_t *offsets,
+                       const int4 *rec_start, const T *rec_off, const T *c_sorted,
+                       const uint32_t *totals, const uint32_t *parts, const uint32_t *item_off,
+                       const uint32_t *heavy_base, const uint32_t *pctrl, const int64_t *pbase,
+                       T *partial, hipStream_t stream);
+
+// The same for the 3D row tiles (8 x 8 x 8 cells).
+template <typename T>
+bool launch_rows3_many(int width, const BatchProblem *probs, const SpreadProblem *sprobs,
                        int nprob, unsigned items_grid, const uint32_t *offsets,
                        const int4 *rec_start, const T *rec_off, const T *c_sorted,
                        const uint32_t *totals, const uint32_t *parts, const uint32_t *item_off,
