@@ -343,6 +343,9 @@ __device__ __forceinline__ void rows2_store(int lane, const GridParams &g, int64
     }
 }
 
+// The single-plan 2D kernel keeps its loop written out in full: built on
+// rows2_accumulate / rows2_store it ran 7% slower (3.95 vs 3.68 ms on 5e7
+// points over 2048^2 cells, width 7).
 template <typename T, int W>
 __global__ __launch_bounds__(64) void k_spread_rows2(
     GridParams g, SpreadParams sp, const uint32_t *__restrict__ offsets,
@@ -353,28 +356,92 @@ __global__ __launch_bounds__(64) void k_spread_rows2(
     T *__restrict__ grid, T *__restrict__ partial)
 {
     static_assert(W <= 8, "widths beyond the 2D row tiles");
-    constexpr int NROWS = kRows2T + W - 1;
+    constexpr int TX = kRows2T, TY = kRows2T;
+    constexpr int NROWS = TY + W - 1;
     __shared__ RowRec2<T> st[64];
     __shared__ TileRows<NROWS> rows;
 
     const int lane = threadIdx.x;
+    const int ly = lane & 15, xc = 4 * (lane >> 4);  // the lane's row and first x cell
     const int64_t v = blockIdx.y;
+    const T beta = (T)g.beta;
     const int64_t n_items = ctrl[0];
     const uint32_t part_len_cap = ctrl[3];
     const T *cv = c_sorted + 2 * v * M;
+    const int64_t n0 = g.ax[0].n, n1 = g.ax[1].n;
 
     for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         const TileItem ti = tile_item(item, sp.block_count, item_off, parts, totals, part_len_cap);
         const int64_t b = ti.b;
-        const int64_t x0 = (b % sp.nblocks[0]) * kRows2T, y0 = (b / sp.nblocks[0]) * kRows2T;
+        const int64_t x0 = (b % sp.nblocks[0]) * TX, y0 = (b / sp.nblocks[0]) * TY;
+        const AxisReach rx = axis_reach(x0, TX, W, g.ax[0]);
+        int64_t xs = x0 - (W - 1);
+        if (xs < 0) xs += n0;
+
+        stage_tile_rows<2, TY, 1, W, NROWS>(rows, lane, y0, 0, rx, g, offsets);
         T ar[4], ai[4];
-        rows2_accumulate<T, W, NROWS>(st, rows, lane, g, x0, y0, ti.rlo, ti.rhi, offsets,
-                                      rec_start, rec_off, cv, ar, ai);
-        const bool whole = ti.np == 1;
-        T *slot = whole ? nullptr
-                        : partial + partial_index<kRows2T, kRows2T, 1>(
-                                        (int64_t)heavy_base[b] + ti.p, gridDim.y, v, 0, 0, 0);
-        rows2_store<T>(lane, g, x0, y0, whole, grid + 2 * v * g.fine_total, slot, ar, ai);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ar[i] = ai[i] = (T)0;
+        const int64_t klo = ti.rlo, khi = ti.rhi;
+        Fetched<T> nx = fetch_record<T, NROWS>(rows, klo + lane, khi, rec_start, rec_off, cv);
+        for (int64_t k0 = klo; k0 < khi; k0 += 64) {
+            const Fetched<T> cur = nx;
+            nx = fetch_record<T, NROWS>(rows, k0 + 64 + lane, khi, rec_start, rec_off, cv);
+            bool take = false;
+            int ee = 0;
+            if (cur.valid) {
+                int64_t e64 = (int64_t)cur.sx0 - xs;
+                if (e64 < 0) e64 += n0;
+                take = e64 < TX + W - 1;
+                ee = (int)e64;
+            }
+            const uint64_t mask = __ballot(take);
+            const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+            const int nst = __popcll(mask);
+            if (take) {
+                RowRec2<T> &R = st[pos];
+                tile_weights<T, W, TX>(R.kx, ee - (W - 1), (T)cur.off.x, beta);
+                tile_weights<T, W, TY>(R.ky, cur.q - (W - 1), (T)cur.off.y, beta);
+                R.cre = cur.c.x;
+                R.cim = cur.c.y;
+            }
+            wave_sync();
+#pragma unroll 2
+            for (int s = 0; s < nst; ++s) {
+                const RowRec2<T> &S = st[s];
+                const T wy = S.ky[ly];
+                const T vre = S.cre * wy, vim = S.cim * wy;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const T kx = S.kx[xc + i];
+                    ar[i] += vre * kx;
+                    ai[i] += vim * kx;
+                }
+            }
+            wave_sync();
+        }
+
+        const int64_t gy = y0 + ly;
+        if (ti.np == 1) {
+            if (gy < n1) {
+                T *row = grid + 2 * (v * g.fine_total + x0 + xc + n0 * gy);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (x0 + xc + i < n0) {
+                        row[2 * i] = ar[i];
+                        row[2 * i + 1] = ai[i];
+                    }
+                }
+            }
+        } else {
+            const int64_t pslot = (int64_t)heavy_base[b] + ti.p;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                T *o = partial + partial_index<TX, TY, 1>(pslot, gridDim.y, v, xc + i, ly, 0);
+                o[0] = ar[i];
+                o[1] = ai[i];
+            }
+        }
     }
 }
 
